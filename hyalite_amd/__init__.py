@@ -1,0 +1,620 @@
+"""hyalite_amd — the MI355X (gfx950) database-scan backend of hyalite, driven from Python.
+
+A thin ctypes layer over the C ABI in ``include/hyalite_hip.h`` (``libhyalite_hip.so``, built from
+``hyalite_amd/csrc``). It mirrors hyalite's public API for the database-scan path so that parity
+tests read like the reference's own:
+
+    Scoring.new(...)                                  src/scoring.rs:65-100
+    Database.builder().sequences(..).scoring(..).mode(..).search_type(..)
+            .max_query_len(..).backend(..).build()    src/database.rs:657-877
+    Scratch(db)                                       src/database.rs:954-999
+    db.scan / db.scan_all / db.scan_scores            src/database.rs:83-515
+    BestHit, Mode, SearchType, ScoreWidth, Backend, BackendChoice, errors
+                                                      src/hit.rs, mode.rs, search.rs, width.rs,
+                                                      backend.rs, error.rs
+
+Every result comes from the HIP kernels; there is no CPU fallback. Importing works without a GPU,
+but building a database raises ``BackendUnavailable`` unless a gfx950 device is present, and any
+use raises ``HipLibraryMissing`` when the shared library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes
+import enum
+import os
+import subprocess
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import numpy as np
+
+__all__ = [
+    "Mode", "SearchType", "ScoreWidth", "Backend", "BackendChoice", "BestHit", "Scoring",
+    "Database", "DatabaseBuilder", "Scratch", "HyaliteError", "EmptyAlphabet", "MatrixShape",
+    "NegativeGapPenalty", "GapOpenLessThanExtend", "ScoreRangeTooWide", "SymbolOutOfRange",
+    "IncompleteBuilder", "EmptyDatabase", "BackendUnavailable", "InvalidBackendName",
+    "ScanContractViolation", "DeviceError", "HipLibraryMissing", "BACKEND_ENV_VAR", "build",
+    "library_path", "load_library",
+]
+
+BACKEND_ENV_VAR = "HYALITE_BACKEND"
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+_ROOT = os.path.dirname(_PKG)
+_LIB_NAME = "libhyalite_hip.so"
+_SOURCES = ["csrc/hy_host.cpp", "csrc/hy_kernels.hip"]
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def library_path() -> str:
+    return os.path.join(_PKG, _LIB_NAME)
+
+
+def build(verbose: bool = False) -> str:
+    """Compile libhyalite_hip.so for gfx950 in-tree with hipcc."""
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    if not os.path.exists(hipcc):
+        hipcc = "hipcc"
+    out = library_path()
+    tmp = out + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, *[os.path.join(_PKG, s) for s in _SOURCES], "-o", tmp]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+    if verbose:
+        print(" ".join(cmd))
+    os.replace(tmp, out)
+    return out
+
+
+# ---- errors (src/error.rs:16-96) ---------------------------------------------------------------
+
+class HyaliteError(Exception):
+    """Base of every construction-time error (the reference's `Error` enum)."""
+
+
+class EmptyAlphabet(HyaliteError):
+    pass
+
+
+class MatrixShape(HyaliteError):
+    pass
+
+
+class NegativeGapPenalty(HyaliteError):
+    pass
+
+
+class GapOpenLessThanExtend(HyaliteError):
+    pass
+
+
+class ScoreRangeTooWide(HyaliteError):
+    def __init__(self, bound: int):
+        super().__init__(f"reachable score magnitude bound ({bound}) exceeds the i32 range; "
+                         "reduce sequence lengths or penalty magnitudes")
+        self.bound = bound
+
+
+class SymbolOutOfRange(HyaliteError):
+    def __init__(self, symbol: int, alphabet_len: int):
+        super().__init__(f"encoded symbol {symbol} is out of range for alphabet_len "
+                         f"{alphabet_len} (must be in 0..{alphabet_len})")
+        self.symbol = symbol
+        self.alphabet_len = alphabet_len
+
+
+class IncompleteBuilder(HyaliteError):
+    def __init__(self, field: str):
+        super().__init__(f"database builder is missing required field: {field}")
+        self.field = field
+
+
+class EmptyDatabase(HyaliteError):
+    pass
+
+
+class BackendUnavailable(HyaliteError):
+    def __init__(self, backend: str = "hip"):
+        super().__init__(f"backend {backend} is not available on this build/device")
+        self.backend = backend
+
+
+class InvalidBackendName(HyaliteError):
+    def __init__(self, name: str):
+        super().__init__(f"unrecognised backend name {name!r}; expected one of: "
+                         "auto, scalar, sse4.1, avx2, neon, hip")
+        self.name = name
+
+
+class ScanContractViolation(Exception):
+    """A scan-time contract breach (over-long query, symbol out of range). The reference panics
+    here (src/database.rs:79-89)."""
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure or a failed device-side self-check."""
+
+
+class HipLibraryMissing(RuntimeError):
+    """libhyalite_hip.so has not been built; run hyalite_amd.build()."""
+
+
+# ---- plain types --------------------------------------------------------------------------------
+
+class Mode(enum.IntEnum):
+    """src/mode.rs:11-27."""
+    Sw = 0
+    Nw = 1
+    Hw = 2
+    Ov = 3
+    Shw = 4
+
+    def is_local(self) -> bool:
+        return self is Mode.Sw
+
+    def code(self) -> str:
+        return {0: "SW", 1: "NW", 2: "HW", 3: "OV", 4: "SHW"}[int(self)]
+
+    def __str__(self) -> str:
+        return self.code()
+
+
+class SearchType(enum.IntEnum):
+    """src/search.rs:13-27 (Alignment is outside the scan backend)."""
+    Score = 0
+    ScoreEnd = 1
+
+    def tracks_end(self) -> bool:
+        return self is SearchType.ScoreEnd
+
+
+class ScoreWidth(enum.IntEnum):
+    """src/width.rs:26-34."""
+    I8 = 8
+    I16 = 16
+    I32 = 32
+
+    def max_abs(self) -> int:
+        return {8: 127, 16: 32767, 32: 536_870_911}[int(self)]
+
+    def bytes(self) -> int:
+        return int(self) // 8
+
+    def __str__(self) -> str:
+        return f"i{int(self)}"
+
+
+class Backend(enum.Enum):
+    """The backend this library provides (a new variant of src/backend.rs:27-36)."""
+    Hip = "hip"
+
+    def name_str(self) -> str:
+        return self.value
+
+    def is_available(self) -> bool:
+        return bool(load_library().hy_backend_available())
+
+    def __str__(self) -> str:
+        return self.value
+
+
+@dataclass(frozen=True)
+class BackendChoice:
+    """src/backend.rs:126-155. ``BackendChoice.AUTO`` or ``BackendChoice.force(Backend.Hip)``."""
+    forced: Optional[Backend] = None
+
+    @staticmethod
+    def force(backend: Backend) -> "BackendChoice":
+        return BackendChoice(backend)
+
+    @staticmethod
+    def parse(name: str) -> "BackendChoice":
+        out = ctypes.c_int32(0)
+        rc = load_library().hy_backend_parse(name.encode(), ctypes.byref(out))
+        if rc != 0:
+            raise InvalidBackendName(name)
+        if out.value == 9:
+            return BackendChoice(Backend.Hip)
+        if out.value == 0:
+            return BackendChoice(None)
+        raise BackendUnavailable(name.strip().lower())
+
+
+BackendChoice.AUTO = BackendChoice(None)  # type: ignore[attr-defined]
+
+
+@dataclass(frozen=True)
+class BestHit:
+    """src/hit.rs:16-25. Ends are 0-based inclusive, None when nothing is aligned."""
+    score: int
+    db_index: int
+    query_end: Optional[int]
+    target_end: Optional[int]
+
+    def __str__(self) -> str:
+        s = f"score={self.score} db={self.db_index}"
+        if self.query_end is not None and self.target_end is not None:
+            s += f" query_end={self.query_end} target_end={self.target_end}"
+        return s
+
+
+class Scoring:
+    """Substitution matrix plus affine gaps, validated like src/scoring.rs:65-100."""
+
+    def __init__(self, alphabet_len: int, matrix: Sequence[int], gap_open: int, gap_ext: int):
+        if alphabet_len == 0:
+            raise EmptyAlphabet("alphabet length must be greater than zero")
+        if len(matrix) != alphabet_len * alphabet_len:
+            raise MatrixShape(f"substitution matrix must have {alphabet_len * alphabet_len} "
+                              f"entries (alphabet_len {alphabet_len} squared), but got "
+                              f"{len(matrix)}")
+        if gap_open < 0 or gap_ext < 0:
+            raise NegativeGapPenalty(f"gap penalties must be non-negative magnitudes, but got "
+                                     f"gap_open={gap_open}, gap_ext={gap_ext}")
+        if gap_open < gap_ext:
+            raise GapOpenLessThanExtend(f"gap_open ({gap_open}) must be >= gap_ext ({gap_ext}) "
+                                        "(Opal issue #28)")
+        self._al = int(alphabet_len)
+        self._matrix = np.ascontiguousarray(np.asarray(matrix, dtype=np.int32))
+        self._go = int(gap_open)
+        self._ge = int(gap_ext)
+
+    new = classmethod(lambda cls, al, matrix, go, ge: cls(al, matrix, go, ge))
+
+    def alphabet_len(self) -> int:
+        return self._al
+
+    def gap_open(self) -> int:
+        return self._go
+
+    def gap_ext(self) -> int:
+        return self._ge
+
+    def entry_bounds(self) -> tuple[int, int]:
+        return int(self._matrix.min()), int(self._matrix.max())
+
+    def score(self, q: int, t: int) -> int:
+        if not (0 <= q < self._al and 0 <= t < self._al):
+            raise IndexError(f"symbol index out of range: q={q}, t={t}, alphabet_len={self._al}")
+        return int(self._matrix[q * self._al + t])
+
+    def matrix(self) -> np.ndarray:
+        return self._matrix
+
+
+# ---- the library --------------------------------------------------------------------------------
+
+_lib = None
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_sizep = ctypes.POINTER(ctypes.c_size_t)
+
+
+def load_library():
+    """Load libhyalite_hip.so (raises HipLibraryMissing when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = library_path()
+    if not os.path.exists(path):
+        raise HipLibraryMissing(f"{path} not found: build it with hyalite_amd.build() "
+                                "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.hy_backend_available.restype = ctypes.c_int
+    lib.hy_backend_available.argtypes = []
+    lib.hy_backend_parse.restype = ctypes.c_int
+    lib.hy_backend_parse.argtypes = [ctypes.c_char_p, _i32p]
+    lib.hy_db_build.restype = ctypes.c_int
+    lib.hy_db_build.argtypes = [ctypes.POINTER(_u8p), _sizep, ctypes.c_size_t, ctypes.c_uint32,
+                                _i32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int32,
+                                ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t,
+                                ctypes.c_uint32, ctypes.POINTER(vp), _i64p]
+    lib.hy_db_retain.restype = ctypes.c_int
+    lib.hy_db_retain.argtypes = [vp]
+    lib.hy_db_free.restype = None
+    lib.hy_db_free.argtypes = [vp]
+    lib.hy_db_info.restype = ctypes.c_int
+    lib.hy_db_info.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), _u64p, _u64p, _u64p,
+                               ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+    lib.hy_db_partitions.restype = ctypes.c_int
+    lib.hy_db_partitions.argtypes = [vp, _u64p, _u64p, _u64p]
+    lib.hy_scratch_new.restype = ctypes.c_int
+    lib.hy_scratch_new.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.hy_scratch_free.restype = None
+    lib.hy_scratch_free.argtypes = [vp]
+    lib.hy_scan.restype = ctypes.c_int
+    lib.hy_scan.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _i32p, _u64p, _i64p, _i64p]
+    lib.hy_scan_all.restype = ctypes.c_int
+    lib.hy_scan_all.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _i32p, _i64p, _i64p]
+    lib.hy_scan_scores.restype = ctypes.c_int
+    lib.hy_scan_scores.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _i32p]
+    lib.hy_scan_batch.restype = ctypes.c_int
+    lib.hy_scan_batch.argtypes = [vp, vp, ctypes.POINTER(_u8p), _sizep, ctypes.c_size_t, _i32p,
+                                  _u64p, _i64p, _i64p]
+    lib.hy_strerror.restype = ctypes.c_char_p
+    lib.hy_strerror.argtypes = [ctypes.c_int]
+    _lib = lib
+    return lib
+
+
+_FIELDS = ["sequences", "scoring", "mode", "max_query_len"]
+
+
+def _raise_build_error(rc: int, detail: int, alphabet_len: int, env_name: Optional[str]):
+    lib = load_library()
+    msg = lib.hy_strerror(rc).decode()
+    if rc == 1:
+        raise EmptyAlphabet(msg)
+    if rc == 2:
+        raise MatrixShape(msg)
+    if rc == 3:
+        raise NegativeGapPenalty(msg)
+    if rc == 4:
+        raise GapOpenLessThanExtend(msg)
+    if rc == 5:
+        raise ScoreRangeTooWide(detail)
+    if rc == 6:
+        raise SymbolOutOfRange(detail, alphabet_len)
+    if rc == 7:
+        raise IncompleteBuilder(_FIELDS[detail])
+    if rc == 8:
+        raise EmptyDatabase(msg)
+    if rc == 9:
+        raise BackendUnavailable()
+    if rc == 10:
+        raise InvalidBackendName(env_name or "")
+    raise DeviceError(f"hy_db_build failed: {msg} (status {rc})")
+
+
+def _check_scan(rc: int):
+    if rc == 0:
+        return
+    msg = load_library().hy_strerror(rc).decode()
+    if rc in (100, 101):
+        raise ScanContractViolation(msg)
+    raise DeviceError(f"{msg} (status {rc})")
+
+
+def _as_u8(q) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(q, dtype=np.uint8))
+    return a if a.size else np.zeros(1, dtype=np.uint8)
+
+
+class DatabaseBuilder:
+    """src/database.rs:657-735. Required: sequences, scoring, mode, max_query_len."""
+
+    def __init__(self):
+        self._seqs = None
+        self._scoring = None
+        self._mode = None
+        self._st = SearchType.Score
+        self._mql = None
+        self._backend = None
+
+    def sequences(self, seqs) -> "DatabaseBuilder":
+        self._seqs = [np.ascontiguousarray(np.asarray(s, dtype=np.uint8)) for s in seqs]
+        return self
+
+    def scoring(self, scoring: Scoring) -> "DatabaseBuilder":
+        self._scoring = scoring
+        return self
+
+    def mode(self, mode: Mode) -> "DatabaseBuilder":
+        self._mode = Mode(mode)
+        return self
+
+    def search_type(self, st: SearchType) -> "DatabaseBuilder":
+        self._st = SearchType(st)
+        return self
+
+    def max_query_len(self, n: int) -> "DatabaseBuilder":
+        self._mql = int(n)
+        return self
+
+    def backend(self, choice: BackendChoice) -> "DatabaseBuilder":
+        self._backend = choice
+        return self
+
+    def build(self) -> "Database":
+        lib = load_library()
+        have = 0
+        if self._seqs is not None:
+            have |= 1
+        if self._scoring is not None:
+            have |= 2
+        if self._mode is not None:
+            have |= 4
+        if self._mql is not None:
+            have |= 8
+        seqs = self._seqs or []
+        n = len(seqs)
+        ptrs = (_u8p * max(n, 1))()
+        lens = (ctypes.c_size_t * max(n, 1))()
+        keep = []
+        for k, s in enumerate(seqs):
+            a = s if s.size else np.zeros(1, dtype=np.uint8)
+            keep.append(a)
+            ptrs[k] = a.ctypes.data_as(_u8p)
+            lens[k] = int(s.size)
+        sc = self._scoring
+        mat = sc.matrix() if sc is not None else np.zeros(1, dtype=np.int32)
+        backend = 0
+        if self._backend is not None and self._backend.forced is not None:
+            backend = 9
+        out = ctypes.c_void_p(None)
+        detail = ctypes.c_int64(0)
+        rc = lib.hy_db_build(ptrs, lens, n, have, mat.ctypes.data_as(_i32p),
+                             mat.size if sc is not None else 0,
+                             sc.alphabet_len() if sc is not None else 0,
+                             sc.gap_open() if sc is not None else 0,
+                             sc.gap_ext() if sc is not None else 0,
+                             int(self._mode) if self._mode is not None else 0, int(self._st),
+                             self._mql if self._mql is not None else 0, backend,
+                             ctypes.byref(out), ctypes.byref(detail))
+        if rc != 0:
+            _raise_build_error(rc, detail.value, sc.alphabet_len() if sc else 0,
+                               os.environ.get(BACKEND_ENV_VAR))
+        return Database(out, sc, self._mode, self._st, [int(s.size) for s in seqs])
+
+
+class Database:
+    """An immutable, device-resident database (src/database.rs:44-63). Share it freely across
+    threads; give each thread its own Scratch."""
+
+    def __init__(self, handle: ctypes.c_void_p, scoring: Scoring, mode: Mode, st: SearchType,
+                 lens: list[int]):
+        self._h = handle
+        self._scoring = scoring
+        self._mode = mode
+        self._st = st
+        self._lens = lens
+        lib = load_library()
+        w, n, mql, mtl = ctypes.c_uint32(0), ctypes.c_uint64(0), ctypes.c_uint64(0), \
+            ctypes.c_uint64(0)
+        lib.hy_db_info(handle, ctypes.byref(w), ctypes.byref(n), ctypes.byref(mql),
+                       ctypes.byref(mtl), None, None)
+        self._width = ScoreWidth(w.value)
+        self._n = n.value
+        self._mql = mql.value
+        self._mtl = mtl.value
+
+    @staticmethod
+    def builder() -> DatabaseBuilder:
+        return DatabaseBuilder()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.hy_db_free(h)
+            self._h = None
+
+    def clone(self) -> "Database":
+        load_library().hy_db_retain(self._h)
+        return Database(self._h, self._scoring, self._mode, self._st, self._lens)
+
+    # accessors (src/database.rs:581-656)
+    def sequence_count(self) -> int:
+        return self._n
+
+    def mode(self) -> Mode:
+        return self._mode
+
+    def search_type(self) -> SearchType:
+        return self._st
+
+    def scoring(self) -> Scoring:
+        return self._scoring
+
+    def score_width(self) -> ScoreWidth:
+        return self._width
+
+    def backend(self) -> Backend:
+        return Backend.Hip
+
+    def max_query_len(self) -> int:
+        return self._mql
+
+    def max_target_len(self) -> int:
+        return self._mtl
+
+    def partition_sizes(self) -> dict:
+        a, b, c = ctypes.c_uint64(0), ctypes.c_uint64(0), ctypes.c_uint64(0)
+        load_library().hy_db_partitions(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {ScoreWidth.I8: a.value, ScoreWidth.I16: b.value, ScoreWidth.I32: c.value}
+
+    # scans
+    def scan(self, scratch: "Scratch", query) -> BestHit:
+        q = np.asarray(query, dtype=np.uint8)
+        qa = _as_u8(q)
+        s, idx = ctypes.c_int32(0), ctypes.c_uint64(0)
+        qe, te = ctypes.c_int64(0), ctypes.c_int64(0)
+        _check_scan(load_library().hy_scan(self._h, scratch._h, qa.ctypes.data_as(_u8p), q.size,
+                                           ctypes.byref(s), ctypes.byref(idx), ctypes.byref(qe),
+                                           ctypes.byref(te)))
+        return BestHit(s.value, idx.value, None if qe.value < 0 else qe.value,
+                       None if te.value < 0 else te.value)
+
+    def scan_all_arrays(self, scratch: "Scratch", query):
+        """(scores int32[n], query_ends int64[n], target_ends int64[n]); -1 means None."""
+        q = np.asarray(query, dtype=np.uint8)
+        qa = _as_u8(q)
+        n = self._n
+        scores = np.empty(n, dtype=np.int32)
+        qe = np.empty(n, dtype=np.int64)
+        te = np.empty(n, dtype=np.int64)
+        _check_scan(load_library().hy_scan_all(self._h, scratch._h, qa.ctypes.data_as(_u8p),
+                                               q.size, scores.ctypes.data_as(_i32p),
+                                               qe.ctypes.data_as(_i64p), te.ctypes.data_as(_i64p)))
+        return scores, qe, te
+
+    def scan_all(self, scratch: "Scratch", query, out: Optional[list] = None) -> list:
+        s, qe, te = self.scan_all_arrays(scratch, query)
+        hits = [BestHit(int(a), k, None if b < 0 else int(b), None if c < 0 else int(c))
+                for k, (a, b, c) in enumerate(zip(s, qe, te))]
+        if out is not None:
+            out.clear()
+            out.extend(hits)
+            return out
+        return hits
+
+    def scan_scores_array(self, scratch: "Scratch", query) -> np.ndarray:
+        q = np.asarray(query, dtype=np.uint8)
+        qa = _as_u8(q)
+        scores = np.empty(self._n, dtype=np.int32)
+        _check_scan(load_library().hy_scan_scores(self._h, scratch._h, qa.ctypes.data_as(_u8p),
+                                                  q.size, scores.ctypes.data_as(_i32p)))
+        return scores
+
+    def scan_scores(self, scratch: "Scratch", query, out: Optional[list] = None) -> list:
+        s = [int(v) for v in self.scan_scores_array(scratch, query)]
+        if out is not None:
+            out.clear()
+            out.extend(s)
+            return out
+        return s
+
+    def scan_batch(self, scratch: "Scratch", queries) -> list:
+        """Batched form: one hit per query, each equal to scan() of that query."""
+        qs = [np.ascontiguousarray(np.asarray(q, dtype=np.uint8)) for q in queries]
+        nq = len(qs)
+        ptrs = (_u8p * max(nq, 1))()
+        lens = (ctypes.c_size_t * max(nq, 1))()
+        keep = []
+        for k, q in enumerate(qs):
+            a = q if q.size else np.zeros(1, dtype=np.uint8)
+            keep.append(a)
+            ptrs[k] = a.ctypes.data_as(_u8p)
+            lens[k] = q.size
+        s = np.empty(max(nq, 1), dtype=np.int32)
+        idx = np.empty(max(nq, 1), dtype=np.uint64)
+        qe = np.empty(max(nq, 1), dtype=np.int64)
+        te = np.empty(max(nq, 1), dtype=np.int64)
+        _check_scan(load_library().hy_scan_batch(self._h, scratch._h, ptrs, lens, nq,
+                                                 s.ctypes.data_as(_i32p), idx.ctypes.data_as(_u64p),
+                                                 qe.ctypes.data_as(_i64p),
+                                                 te.ctypes.data_as(_i64p)))
+        return [BestHit(int(s[k]), int(idx[k]), None if qe[k] < 0 else int(qe[k]),
+                        None if te[k] < 0 else int(te[k])) for k in range(nq)]
+
+
+class Scratch:
+    """Per-thread working memory: a HIP stream plus device buffers (src/database.rs:954-999)."""
+
+    def __init__(self, db: Database):
+        out = ctypes.c_void_p(None)
+        rc = load_library().hy_scratch_new(db._h, ctypes.byref(out))
+        if rc != 0:
+            raise DeviceError(load_library().hy_strerror(rc).decode())
+        self._h = out
+        self._db = db  # keeps the database alive while the scratch exists
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.hy_scratch_free(h)
+            self._h = None

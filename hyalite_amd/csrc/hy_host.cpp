@@ -1,0 +1,675 @@
+// Host side of the MI355X database-scan backend: the C ABI of include/hyalite_hip.h.
+//
+// Build (hy_db_build) mirrors DatabaseBuilder::build (src/database.rs:737-877): the same
+// validation order and error codes, the per-sequence width proof (src/database.rs:884-948,
+// src/width.rs:107-169), then a one-time upload of the database into HBM in the layouts the
+// kernels read. Scans mirror Database::scan / scan_all / scan_scores (src/database.rs:83-515);
+// they allocate nothing, run on the Scratch's own stream, and use no CPU fallback: every score
+// and end position comes from a kernel in hy_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cctype>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/hyalite_hip.h"
+#include "hy_internal.h"
+
+using hy::Group;
+using hy::Result;
+
+namespace {
+
+constexpr uint32_t kNoPadSymbol = 255;  // al_eff <= 255 leaves symbol value al_eff free for padding
+
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+template <typename T>
+hipError_t dev_upload(T** dst, const std::vector<T>& src) {
+    *dst = nullptr;
+    if (src.empty()) return hipSuccess;
+    hipError_t e = hipMalloc((void**)dst, src.size() * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// ScoreWidth::max_abs (src/width.rs:48-55) and required_width (src/width.rs:145-169).
+int width_bits_for(__int128 bound, int64_t* bound_out) {
+    const int64_t b = bound > (__int128)INT64_MAX ? INT64_MAX : (int64_t)bound;
+    if (bound_out) *bound_out = b;
+    if (b <= 127) return 8;
+    if (b <= 32767) return 16;
+    if (b <= 536870911) return 32;
+    return 0;
+}
+
+// magnitude_bound (src/width.rs:107-137), evaluated in 128-bit.
+__int128 magnitude_bound(uint32_t mode, int32_t min_e, int32_t max_e, int32_t go, int32_t ge,
+                         uint64_t mql, uint64_t mtl) {
+    const __int128 m = (__int128)mql, n = (__int128)mtl;
+    const __int128 min_ij = m < n ? m : n;
+    const __int128 max_pos = max_e > 0 ? (__int128)max_e : 0;
+    const __int128 max_neg = min_e < 0 ? -(__int128)min_e : 0;
+    const __int128 positive = min_ij * max_pos;
+    __int128 negative;
+    if (mode == HY_MODE_SW) {
+        negative = go;
+    } else if (mode == HY_MODE_OV) {
+        negative = min_ij * max_neg + go;
+    } else {
+        __int128 run = m + n - 1;
+        if (run < 0) run = 0;
+        negative = (m + n) * max_neg + go + run * ge;
+    }
+    return positive > negative ? positive : negative;
+}
+
+std::string lower_trim(const char* s) {
+    std::string v(s ? s : "");
+    size_t a = 0, b = v.size();
+    while (a < b && std::isspace((unsigned char)v[a])) ++a;
+    while (b > a && std::isspace((unsigned char)v[b - 1])) --b;
+    v = v.substr(a, b - a);
+    for (auto& c : v) c = (char)std::tolower((unsigned char)c);
+    return v;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Database and Scratch
+// ---------------------------------------------------------------------------------------------
+
+struct hy_db {
+    std::atomic<int> refs{1};
+    int device = 0;
+    // Validated scheme (src/scoring.rs:41-48) and builder fields.
+    uint32_t al = 0, al_eff = 0;
+    int32_t go = 0, ge = 0, min_e = 0, max_e = 0;
+    uint32_t mode = 0, search_type = 0;
+    uint64_t n = 0, mql = 0, mtl = 0, total_residues = 0;
+    int widest_bits = 8;
+    uint64_t count_by_width[3] = {0, 0, 0};
+    // Device copies.
+    int32_t* d_matrix = nullptr;   // al x al, entries clamped to +-kEntryMax32
+    uint8_t* d_flat = nullptr;     // residues in database order
+    uint64_t* d_off = nullptr;     // n+1 offsets
+    // Inter-sequence partition (one target per lane).
+    std::vector<Group> groups;
+    Group* d_groups = nullptr;
+    uint32_t* d_words = nullptr;
+    uint32_t* d_lens = nullptr;
+    uint32_t* d_gidx = nullptr;
+    uint64_t inter_spill = 0;      // int2 entries
+    // Systolic list (long targets, or everything when the alphabet leaves no padding symbol).
+    std::vector<uint32_t> sys_list;
+    uint32_t* d_sys_list = nullptr;
+    uint64_t sys_max_len = 0;
+
+    ~hy_db() {
+        (void)hipSetDevice(device);
+        dev_free(d_matrix); dev_free(d_flat); dev_free(d_off);
+        dev_free(d_groups); dev_free(d_words); dev_free(d_lens); dev_free(d_gidx);
+        dev_free(d_sys_list);
+    }
+};
+
+struct hy_scratch {
+    int device = 0;
+    const hy_db* db = nullptr;  // sized for this database (not owned)
+    hipStream_t stream = nullptr;
+    uint8_t* d_query = nullptr;
+    uint8_t* h_query = nullptr;          // pinned staging
+    int2* d_spill = nullptr;
+    uint64_t spill_entries = 0;
+    int2* d_sys_spill = nullptr;
+    uint64_t sys_spill_stride = 0;
+    int32_t* d_scores = nullptr;
+    int64_t* d_qend = nullptr;
+    int64_t* d_tend = nullptr;
+    unsigned long long* d_key = nullptr;
+    Result* d_result = nullptr;
+    Result* h_result = nullptr;          // pinned
+
+    ~hy_scratch() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        dev_free(d_query); dev_free(d_spill); dev_free(d_sys_spill);
+        dev_free(d_scores); dev_free(d_qend); dev_free(d_tend);
+        dev_free(d_key); dev_free(d_result);
+        if (h_query) (void)hipHostFree(h_query);
+        if (h_result) (void)hipHostFree(h_result);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Backend resolution (src/backend.rs)
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int hy_backend_available(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return 0;
+    for (int d = 0; d < count; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) != hipSuccess) continue;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) return 1;
+    }
+    return 0;
+}
+
+extern "C" int hy_backend_parse(const char* name, int32_t* out) {
+    if (!out) return HY_E_NULL_ARGUMENT;
+    const std::string v = lower_trim(name);
+    if (v == "auto") { *out = HY_BACKEND_AUTO; return HY_OK; }
+    if (v == "hip") { *out = HY_BACKEND_HIP; return HY_OK; }
+    if (v == "scalar" || v == "sse4.1" || v == "sse41" || v == "avx2" || v == "neon") {
+        *out = -1;  // a reference CPU backend: valid name, not served by this library
+        return HY_OK;
+    }
+    return HY_E_INVALID_BACKEND_NAME;
+}
+
+static int first_gfx950_device() {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return -1;
+    for (int d = 0; d < count; ++d) {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) != hipSuccess) continue;
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) == 0) return d;
+    }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Build
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_t n,
+                           uint32_t have, const int32_t* matrix, size_t matrix_len,
+                           uint32_t alphabet_len, int32_t gap_open, int32_t gap_ext, uint32_t mode,
+                           uint32_t search_type, size_t max_query_len, uint32_t backend,
+                           hy_db** out, int64_t* err_detail) {
+    if (!out) return HY_E_NULL_ARGUMENT;
+    *out = nullptr;
+    if (err_detail) *err_detail = 0;
+
+    // Scoring::new (src/scoring.rs:65-100) runs before the builder sees the scheme.
+    int32_t min_e = 0, max_e = 0;
+    if (have & HY_HAVE_SCORING) {
+        if (alphabet_len == 0) return HY_E_EMPTY_ALPHABET;
+        if ((uint64_t)alphabet_len * (uint64_t)alphabet_len != (uint64_t)matrix_len) {
+            if (err_detail) *err_detail = (int64_t)matrix_len;
+            return HY_E_MATRIX_SHAPE;
+        }
+        if (!matrix) return HY_E_NULL_ARGUMENT;
+        if (gap_open < 0 || gap_ext < 0) return HY_E_NEGATIVE_GAP_PENALTY;
+        if (gap_open < gap_ext) return HY_E_GAP_OPEN_LESS_THAN_EXTEND;
+        min_e = max_e = matrix[0];
+        for (size_t k = 1; k < matrix_len; ++k) {
+            min_e = std::min(min_e, matrix[k]);
+            max_e = std::max(max_e, matrix[k]);
+        }
+    }
+    // IncompleteBuilder in field order (src/database.rs:738-749).
+    const uint32_t order[4] = {HY_HAVE_SEQUENCES, HY_HAVE_SCORING, HY_HAVE_MODE,
+                               HY_HAVE_MAX_QUERY_LEN};
+    for (int k = 0; k < 4; ++k) {
+        if (!(have & order[k])) {
+            if (err_detail) *err_detail = k;
+            return HY_E_INCOMPLETE_BUILDER;
+        }
+    }
+    if (mode > HY_MODE_SHW || search_type > HY_SEARCH_SCORE_END) return HY_E_INVALID_ARGUMENT;
+    if (n == 0) return HY_E_EMPTY_DATABASE;
+    if (!seqs || !lens) return HY_E_NULL_ARGUMENT;
+    // SymbolOutOfRange: the first bad symbol in database order (src/database.rs:756-766).
+    uint64_t mtl = 0, total = 0;
+    for (size_t k = 0; k < n; ++k) {
+        if (lens[k] && !seqs[k]) return HY_E_NULL_ARGUMENT;
+        for (size_t x = 0; x < lens[k]; ++x) {
+            if ((uint32_t)seqs[k][x] >= alphabet_len) {
+                if (err_detail) *err_detail = seqs[k][x];
+                return HY_E_SYMBOL_OUT_OF_RANGE;
+            }
+        }
+        mtl = std::max<uint64_t>(mtl, lens[k]);
+        total += lens[k];
+    }
+    if (mtl > 0x7FFFFFF0ull || max_query_len > 0x7FFFFFF0ull || n > 0xFFFFFFF0ull)
+        return HY_E_INVALID_ARGUMENT;
+    // Width proof at the declared maxima (src/database.rs:771).
+    int64_t bound = 0;
+    const int widest = width_bits_for(
+        magnitude_bound(mode, min_e, max_e, gap_open, gap_ext, max_query_len, mtl), &bound);
+    if (widest == 0) {
+        if (err_detail) *err_detail = bound;
+        return HY_E_SCORE_RANGE_TOO_WIDE;
+    }
+    // Backend: explicit choice, else HYALITE_BACKEND, else Auto (src/database.rs:789-794).
+    if (backend != HY_BACKEND_AUTO && backend != HY_BACKEND_HIP) return HY_E_INVALID_ARGUMENT;
+    if (backend == HY_BACKEND_AUTO) {
+        const char* env = std::getenv("HYALITE_BACKEND");
+        if (env && !lower_trim(env).empty()) {
+            int32_t b = 0;
+            if (hy_backend_parse(env, &b) != HY_OK) return HY_E_INVALID_BACKEND_NAME;
+            if (b < 0) return HY_E_BACKEND_UNAVAILABLE;  // a CPU backend was requested
+        }
+    }
+    const int device = first_gfx950_device();
+    if (device < 0) return HY_E_BACKEND_UNAVAILABLE;
+    if (hipSetDevice(device) != hipSuccess) return HY_E_DEVICE;
+
+    hy_db* db = new (std::nothrow) hy_db();
+    if (!db) return HY_E_OUT_OF_MEMORY;
+    db->device = device;
+    db->al = alphabet_len;
+    db->al_eff = std::min<uint32_t>(alphabet_len, 256);
+    db->go = gap_open;
+    db->ge = gap_ext;
+    db->min_e = min_e;
+    db->max_e = max_e;
+    db->mode = mode;
+    db->search_type = search_type;
+    db->n = n;
+    db->mql = max_query_len;
+    db->mtl = mtl;
+    db->total_residues = total;
+    db->widest_bits = widest;
+
+    // Per-sequence widths (src/database.rs:896-903) — reported through hy_db_partitions. All
+    // partitions currently run in the 32-bit kernels, which are exact for every proven width.
+    for (size_t k = 0; k < n; ++k) {
+        const int w = width_bits_for(
+            magnitude_bound(mode, min_e, max_e, gap_open, gap_ext, max_query_len, lens[k]),
+            nullptr);
+        db->count_by_width[w == 8 ? 0 : (w == 16 ? 1 : 2)]++;
+    }
+
+    int rc = HY_OK;
+    auto fail = [&](int code) { delete db; return code; };
+
+    // Matrix, clamped for the 32-bit kernels.
+    std::vector<int32_t> mat(matrix, matrix + matrix_len);
+    for (auto& v : mat) v = std::max(-hy::kEntryMax32, std::min(hy::kEntryMax32, v));
+    if (dev_upload(&db->d_matrix, mat) != hipSuccess) return fail(HY_E_DEVICE);
+
+    // Database residues in original order.
+    {
+        std::vector<uint64_t> off(n + 1, 0);
+        for (size_t k = 0; k < n; ++k) off[k + 1] = off[k] + lens[k];
+        std::vector<uint8_t> flat(std::max<uint64_t>(total, 1), 0);
+        for (size_t k = 0; k < n; ++k)
+            if (lens[k]) std::memcpy(flat.data() + off[k], seqs[k], lens[k]);
+        if (dev_upload(&db->d_off, off) != hipSuccess) return fail(HY_E_DEVICE);
+        if (dev_upload(&db->d_flat, flat) != hipSuccess) return fail(HY_E_DEVICE);
+    }
+
+    // Split: long targets go to the systolic kernel, the rest to the inter-sequence kernel. A
+    // database whose alphabet uses every byte value has no padding symbol and runs systolic.
+    std::vector<uint32_t> order_idx(n);
+    std::iota(order_idx.begin(), order_idx.end(), 0u);
+    std::stable_sort(order_idx.begin(), order_idx.end(),
+                     [&](uint32_t a, uint32_t b) { return lens[a] > lens[b]; });
+    const double mean = (double)total / (double)n;
+    const uint64_t long_threshold = std::max<uint64_t>(1024, (uint64_t)(8.0 * mean));
+    const bool all_systolic = db->al_eff > kNoPadSymbol;
+    std::vector<uint32_t> inter_idx;
+    inter_idx.reserve(n);
+    for (uint32_t k : order_idx) {
+        if (all_systolic || lens[k] > long_threshold) {
+            db->sys_list.push_back(k);
+            db->sys_max_len = std::max<uint64_t>(db->sys_max_len, lens[k]);
+        } else {
+            inter_idx.push_back(k);
+        }
+    }
+
+    // Inter layout: groups of 64 consecutive (length-sorted) targets; residues interleaved so a
+    // wavefront's load of column word w reads 64 consecutive 32-bit words.
+    {
+        const uint32_t pad = db->al_eff;  // padding symbol
+        const size_t ns = inter_idx.size();
+        std::vector<uint32_t> slot_len(ns), slot_gidx(ns);
+        for (size_t s = 0; s < ns; ++s) {
+            slot_gidx[s] = inter_idx[s];
+            slot_len[s] = (uint32_t)lens[inter_idx[s]];
+        }
+        uint64_t words = 0, spill = 0;
+        for (size_t s = 0; s < ns; s += 64) {
+            Group g{};
+            g.first = (uint32_t)s;
+            g.count = (uint32_t)std::min<size_t>(64, ns - s);
+            g.ncols = slot_len[s];
+            g.word_off = words;
+            g.spill_off = spill;
+            words += (uint64_t)((g.ncols + 3) / 4) * 64;
+            spill += (uint64_t)(g.ncols + 1) * 64;
+            db->groups.push_back(g);
+        }
+        std::vector<uint32_t> w((size_t)words, 0);
+        for (const Group& g : db->groups) {
+            const uint32_t nw = (g.ncols + 3) / 4;
+            for (uint32_t lane = 0; lane < 64; ++lane) {
+                const bool live = lane < g.count;
+                const uint32_t k = live ? slot_gidx[g.first + lane] : 0;
+                const uint32_t L = live ? (uint32_t)lens[k] : 0;
+                for (uint32_t jw = 0; jw < nw; ++jw) {
+                    uint32_t word = 0;
+                    for (uint32_t b = 0; b < 4; ++b) {
+                        const uint32_t col = jw * 4 + b;
+                        const uint32_t sym = col < L ? seqs[k][col] : pad;
+                        word |= (sym & 0xffu) << (8 * b);
+                    }
+                    w[(size_t)g.word_off + (size_t)jw * 64 + lane] = word;
+                }
+            }
+        }
+        db->inter_spill = spill;
+        if (dev_upload(&db->d_groups, db->groups) != hipSuccess) return fail(HY_E_DEVICE);
+        if (dev_upload(&db->d_words, w) != hipSuccess) return fail(HY_E_DEVICE);
+        if (dev_upload(&db->d_lens, slot_len) != hipSuccess) return fail(HY_E_DEVICE);
+        if (dev_upload(&db->d_gidx, slot_gidx) != hipSuccess) return fail(HY_E_DEVICE);
+    }
+    if (dev_upload(&db->d_sys_list, db->sys_list) != hipSuccess) return fail(HY_E_DEVICE);
+    (void)rc;
+    *out = db;
+    return HY_OK;
+}
+
+extern "C" int hy_db_retain(hy_db* db) {
+    if (!db) return HY_E_NULL_ARGUMENT;
+    db->refs.fetch_add(1);
+    return HY_OK;
+}
+
+extern "C" void hy_db_free(hy_db* db) {
+    if (db && db->refs.fetch_sub(1) == 1) delete db;
+}
+
+extern "C" int hy_db_info(const hy_db* db, uint32_t* widest, uint64_t* n, uint64_t* mql,
+                          uint64_t* mtl, uint32_t* mode, uint32_t* st) {
+    if (!db) return HY_E_NULL_ARGUMENT;
+    if (widest) *widest = (uint32_t)db->widest_bits;
+    if (n) *n = db->n;
+    if (mql) *mql = db->mql;
+    if (mtl) *mtl = db->mtl;
+    if (mode) *mode = db->mode;
+    if (st) *st = db->search_type;
+    return HY_OK;
+}
+
+extern "C" int hy_db_partitions(const hy_db* db, uint64_t* n8, uint64_t* n16, uint64_t* n32) {
+    if (!db) return HY_E_NULL_ARGUMENT;
+    if (n8) *n8 = db->count_by_width[0];
+    if (n16) *n16 = db->count_by_width[1];
+    if (n32) *n32 = db->count_by_width[2];
+    return HY_OK;
+}
+
+extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
+    if (!db || !out) return HY_E_NULL_ARGUMENT;
+    *out = nullptr;
+    if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
+    hy_scratch* s = new (std::nothrow) hy_scratch();
+    if (!s) return HY_E_OUT_OF_MEMORY;
+    s->device = db->device;
+    s->db = db;
+    auto fail = [&](int code) { delete s; return code; };
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(HY_E_DEVICE);
+    const size_t qcap = std::max<uint64_t>(db->mql, 1);
+    if (hipMalloc((void**)&s->d_query, qcap) != hipSuccess) return fail(HY_E_OUT_OF_MEMORY);
+    if (hipHostMalloc((void**)&s->h_query, qcap) != hipSuccess) return fail(HY_E_OUT_OF_MEMORY);
+    s->spill_entries = std::max<uint64_t>(db->inter_spill, 1);
+    if (hipMalloc((void**)&s->d_spill, s->spill_entries * sizeof(int2)) != hipSuccess)
+        return fail(HY_E_OUT_OF_MEMORY);
+    // Systolic boundary rows are needed only when a query can exceed one pass (512 rows).
+    const bool multipass = db->mql > 512;
+    s->sys_spill_stride = db->sys_max_len + 1;
+    const uint64_t sys_entries =
+        std::max<uint64_t>(multipass ? (db->sys_list.size() * s->sys_spill_stride) : 1,
+                           multipass ? db->mtl + 1 : 1);
+    if (hipMalloc((void**)&s->d_sys_spill, sys_entries * sizeof(int2)) != hipSuccess)
+        return fail(HY_E_OUT_OF_MEMORY);
+    const uint64_t n = db->n;
+    if (hipMalloc((void**)&s->d_scores, n * sizeof(int32_t)) != hipSuccess ||
+        hipMalloc((void**)&s->d_qend, n * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc((void**)&s->d_tend, n * sizeof(int64_t)) != hipSuccess)
+        return fail(HY_E_OUT_OF_MEMORY);
+    if (hipMalloc((void**)&s->d_key, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void**)&s->d_result, sizeof(Result)) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_result, sizeof(Result)) != hipSuccess)
+        return fail(HY_E_OUT_OF_MEMORY);
+    *out = s;
+    return HY_OK;
+}
+
+extern "C" void hy_scratch_free(hy_scratch* s) { delete s; }
+
+// ---------------------------------------------------------------------------------------------
+// Scans
+// ---------------------------------------------------------------------------------------------
+
+namespace {
+
+// Contract checks the reference turns into panics (src/database.rs:79-89): an over-long query or
+// a symbol outside the alphabet.
+int check_query(const hy_db* db, const uint8_t* q, size_t qlen) {
+    if (qlen > db->mql) return HY_E_QUERY_TOO_LONG;
+    if (qlen && !q) return HY_E_NULL_ARGUMENT;
+    for (size_t k = 0; k < qlen; ++k)
+        if ((uint32_t)q[k] >= db->al) return HY_E_QUERY_SYMBOL;
+    return HY_OK;
+}
+
+int stage_query(const hy_db* db, hy_scratch* s, const uint8_t* q, size_t qlen) {
+    if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
+    if (qlen) {
+        std::memcpy(s->h_query, q, qlen);
+        if (hipMemcpyAsync(s->d_query, s->h_query, qlen, hipMemcpyHostToDevice, s->stream) !=
+            hipSuccess)
+            return HY_E_DEVICE;
+    }
+    return HY_OK;
+}
+
+hy::InterArgs inter_args(const hy_db* db, hy_scratch* s, int m) {
+    hy::InterArgs a{};
+    a.groups = db->d_groups;
+    a.words = db->d_words;
+    a.lens = db->d_lens;
+    a.gidx = db->d_gidx;
+    a.query = s->d_query;
+    a.m = m;
+    a.matrix = db->d_matrix;
+    a.al = (int)db->al;
+    a.al_eff = (int)db->al_eff;
+    a.go = db->go;
+    a.ge = db->ge;
+    a.spill = s->d_spill;
+    return a;
+}
+
+hy::SysArgs sys_args(const hy_db* db, hy_scratch* s, int m) {
+    hy::SysArgs a{};
+    a.flat = db->d_flat;
+    a.off = db->d_off;
+    a.list = db->d_sys_list;
+    a.query = s->d_query;
+    a.m = m;
+    a.matrix = db->d_matrix;
+    a.al = (int)db->al;
+    a.al_eff = (int)db->al_eff;
+    a.go = db->go;
+    a.ge = db->ge;
+    a.spill = s->d_sys_spill;
+    a.spill_stride = s->sys_spill_stride;
+    return a;
+}
+
+// Enqueue the per-target pass. ends: write (score, qend, tend) per target; otherwise write scores
+// (when `scores` is set) and/or fold the argmax key (when `key` is set).
+int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write_scores,
+                    unsigned long long* key) {
+    hy::InterArgs ia = inter_args(db, s, m);
+    ia.out_score = (ends || write_scores) ? s->d_scores : nullptr;
+    ia.out_qend = s->d_qend;
+    ia.out_tend = s->d_tend;
+    ia.key = ends ? nullptr : key;
+    if (hy::launch_inter32((int)db->mode, ends, ia, (uint32_t)db->groups.size(), s->stream) !=
+        hipSuccess)
+        return HY_E_DEVICE;
+    hy::SysArgs sa = sys_args(db, s, m);
+    sa.out_score = ia.out_score;
+    sa.out_qend = s->d_qend;
+    sa.out_tend = s->d_tend;
+    sa.key = ia.key;
+    if (hy::launch_sys32((int)db->mode, ends, false, sa, (uint32_t)db->sys_list.size(),
+                         s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    return HY_OK;
+}
+
+}  // namespace
+
+extern "C" int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
+                       int32_t* score, uint64_t* db_index, int64_t* query_end,
+                       int64_t* target_end) {
+    if (!db || !s || !score || !db_index || !query_end || !target_end) return HY_E_NULL_ARGUMENT;
+    if (s->db != db) return HY_E_INVALID_ARGUMENT;
+    int rc = check_query(db, query, qlen);
+    if (rc != HY_OK) return rc;
+    if ((rc = stage_query(db, s, query, qlen)) != HY_OK) return rc;
+    const int m = (int)qlen;
+    if (hy::launch_fill_u64(s->d_key, 1, 0ull, s->stream) != hipSuccess) return HY_E_DEVICE;
+    if ((rc = enqueue_targets(db, s, m, false, false, s->d_key)) != HY_OK) return rc;
+    const bool ends = db->search_type == HY_SEARCH_SCORE_END;
+    if (ends) {
+        // The winner's end positions: one systolic pass over the winning target only (the role
+        // of align_core on the winner in src/database.rs:183-197), chosen on the device.
+        hy::SysArgs sa = sys_args(db, s, m);
+        sa.winner_key = s->d_key;
+        sa.result = s->d_result;
+        if (hy::launch_sys32((int)db->mode, true, true, sa, 1, s->stream) != hipSuccess)
+            return HY_E_DEVICE;
+    } else {
+        if (hy::launch_key_result(s->d_key, s->d_result, s->stream) != hipSuccess)
+            return HY_E_DEVICE;
+    }
+    if (hipMemcpyAsync(s->h_result, s->d_result, sizeof(Result), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    const Result r = *s->h_result;
+    if (r.check != 0) return HY_E_INTERNAL;
+    *score = r.score;
+    *db_index = r.db_index;
+    *query_end = r.query_end;
+    *target_end = r.target_end;
+    return HY_OK;
+}
+
+extern "C" int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
+                           int32_t* scores, int64_t* query_ends, int64_t* target_ends) {
+    if (!db || !s || !scores) return HY_E_NULL_ARGUMENT;
+    if (s->db != db) return HY_E_INVALID_ARGUMENT;
+    int rc = check_query(db, query, qlen);
+    if (rc != HY_OK) return rc;
+    if ((rc = stage_query(db, s, query, qlen)) != HY_OK) return rc;
+    const bool ends = db->search_type == HY_SEARCH_SCORE_END;
+    if ((rc = enqueue_targets(db, s, (int)qlen, ends, true, nullptr)) != HY_OK) return rc;
+    const size_t n = (size_t)db->n;
+    if (hipMemcpyAsync(scores, s->d_scores, n * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    if (ends) {
+        if (query_ends && hipMemcpyAsync(query_ends, s->d_qend, n * sizeof(int64_t),
+                                         hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+            return HY_E_DEVICE;
+        if (target_ends && hipMemcpyAsync(target_ends, s->d_tend, n * sizeof(int64_t),
+                                          hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+            return HY_E_DEVICE;
+    }
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    if (!ends) {
+        for (size_t k = 0; k < n; ++k) {
+            if (query_ends) query_ends[k] = -1;
+            if (target_ends) target_ends[k] = -1;
+        }
+    }
+    return HY_OK;
+}
+
+extern "C" int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
+                              int32_t* scores) {
+    if (!db || !s || !scores) return HY_E_NULL_ARGUMENT;
+    if (s->db != db) return HY_E_INVALID_ARGUMENT;
+    int rc = check_query(db, query, qlen);
+    if (rc != HY_OK) return rc;
+    if ((rc = stage_query(db, s, query, qlen)) != HY_OK) return rc;
+    if ((rc = enqueue_targets(db, s, (int)qlen, false, true, nullptr)) != HY_OK) return rc;
+    if (hipMemcpyAsync(scores, s->d_scores, (size_t)db->n * sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    return HY_OK;
+}
+
+extern "C" int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
+                             const size_t* qlens, size_t nq, int32_t* scores,
+                             uint64_t* db_indices, int64_t* query_ends, int64_t* target_ends) {
+    if (!db || !s || (nq && (!queries || !qlens || !scores || !db_indices || !query_ends ||
+                             !target_ends)))
+        return HY_E_NULL_ARGUMENT;
+    for (size_t k = 0; k < nq; ++k) {
+        const int rc = check_query(db, queries[k], qlens[k]);
+        if (rc != HY_OK) return rc;
+    }
+    for (size_t k = 0; k < nq; ++k) {
+        const int rc = hy_scan(db, s, queries[k], qlens[k], &scores[k], &db_indices[k],
+                               &query_ends[k], &target_ends[k]);
+        if (rc != HY_OK) return rc;
+    }
+    return HY_OK;
+}
+
+extern "C" const char* hy_strerror(int code) {
+    switch (code) {
+    case HY_OK: return "ok";
+    case HY_E_EMPTY_ALPHABET: return "alphabet length must be greater than zero";
+    case HY_E_MATRIX_SHAPE: return "substitution matrix must have alphabet_len squared entries";
+    case HY_E_NEGATIVE_GAP_PENALTY: return "gap penalties must be non-negative magnitudes";
+    case HY_E_GAP_OPEN_LESS_THAN_EXTEND: return "gap_open must be >= gap_ext (Opal issue #28)";
+    case HY_E_SCORE_RANGE_TOO_WIDE:
+        return "reachable score magnitude bound exceeds the i32 range; reduce sequence lengths "
+               "or penalty magnitudes";
+    case HY_E_SYMBOL_OUT_OF_RANGE: return "encoded symbol is out of range for alphabet_len";
+    case HY_E_INCOMPLETE_BUILDER: return "database builder is missing a required field";
+    case HY_E_EMPTY_DATABASE: return "database must contain at least one sequence";
+    case HY_E_BACKEND_UNAVAILABLE: return "backend hip is not available on this build/device";
+    case HY_E_INVALID_BACKEND_NAME:
+        return "unrecognised backend name; expected one of: auto, scalar, sse4.1, avx2, neon, hip";
+    case HY_E_TRACEBACK_BUDGET: return "traceback budget exceeded";
+    case HY_E_QUERY_TOO_LONG: return "query length exceeds the declared max_query_len";
+    case HY_E_QUERY_SYMBOL: return "query symbol is out of range for alphabet_len";
+    case HY_E_NULL_ARGUMENT: return "a required pointer argument was null";
+    case HY_E_INVALID_ARGUMENT: return "invalid argument";
+    case HY_E_DEVICE: return "a HIP runtime call failed";
+    case HY_E_OUT_OF_MEMORY: return "out of memory";
+    case HY_E_INTERNAL: return "device self-check failed";
+    default: return "unknown status code";
+    }
+}

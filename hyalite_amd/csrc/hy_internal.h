@@ -1,0 +1,97 @@
+// Internal types shared by the host library (hy_host.cpp) and the kernels (hy_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hy {
+
+// Alignment modes (src/mode.rs:11-27); numbering matches the C ABI.
+enum : int { kSW = 0, kNW = 1, kHW = 2, kOV = 3, kSHW = 4 };
+
+// Sentinel of the 32-bit realisation (src/kernel.rs:39, src/inter.rs:80-98).
+constexpr int kNeg32 = INT32_MIN / 4;
+// Substitution value used for padded query rows and padded target columns in the 32-bit
+// kernels: low enough that a diagonal step through it never beats a real cell, high enough that
+// adding it to any real cell cannot wrap.
+constexpr int kPad32 = -(1 << 29);
+// Matrix entries are clamped to this range before they reach a 32-bit kernel. Within the width
+// proof no non-local cell can use an entry beyond it, and in local mode a clamped entry still
+// drives its diagonal below 0, so results are unchanged.
+constexpr int kEntryMax32 = (1 << 29) - 1;
+
+// Lanes per wavefront on CDNA4.
+constexpr int kWave = 64;
+
+// One wavefront's worth of targets for the inter-sequence kernels. Targets are sorted by length
+// (descending) before grouping, so the lanes of a group sweep similar column counts.
+struct Group {
+    uint32_t first;      // first slot (index into the slot-ordered arrays)
+    uint32_t count;      // live lanes, <= 64
+    uint32_t ncols;      // longest target in the group = columns swept
+    uint32_t pad;
+    uint64_t word_off;   // residue words: words[word_off + (col/4)*64 + lane], byte col%4
+    uint64_t spill_off;  // boundary rows: spill[spill_off + col*64 + lane], col in [0, ncols]
+};
+
+// Single-best result slot written by the device.
+struct Result {
+    int32_t score;
+    uint32_t db_index;
+    int64_t query_end;   // -1 = None
+    int64_t target_end;  // -1 = None
+    int32_t check;       // 0 when the device self-check passed
+    int32_t pad;
+};
+
+struct QueryRef {
+    const uint8_t* seq;  // device pointer to the encoded query
+    int m;               // query length
+};
+
+struct InterArgs {
+    const Group* groups;
+    const uint32_t* words;
+    const uint32_t* lens;    // per slot
+    const uint32_t* gidx;    // per slot -> database index
+    const uint8_t* query;
+    int m;
+    const int32_t* matrix;   // al x al, entries clamped to +-kEntryMax32
+    int al;                  // matrix row stride
+    int al_eff;              // symbols that can occur (min(al, 255)); symbol al_eff = padding
+    int go, ge;
+    int2* spill;
+    int32_t* out_score;      // database order (may be null)
+    int64_t* out_qend;       // database order (ends variant)
+    int64_t* out_tend;
+    unsigned long long* key; // fused argmax key (may be null)
+};
+
+struct SysArgs {
+    const uint8_t* flat;      // database residues in original order
+    const uint64_t* off;      // n+1 offsets
+    const uint32_t* list;     // database indices to align (list mode)
+    const unsigned long long* winner_key;  // winner mode: the index comes from this key
+    const uint8_t* query;
+    int m;
+    const int32_t* matrix;
+    int al, al_eff;
+    int go, ge;
+    int2* spill;              // per block: spill[blockIdx.x * spill_stride + col]
+    uint64_t spill_stride;
+    int32_t* out_score;       // list mode: database order
+    int64_t* out_qend;
+    int64_t* out_tend;
+    unsigned long long* key;  // list mode: fused argmax key (may be null)
+    Result* result;           // winner mode output
+};
+
+// Launchers (hy_kernels.hip). All launch on `stream` and never synchronise.
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
+                           hipStream_t stream);
+hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngroups,
+                          hipStream_t stream);
+hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                        hipStream_t stream);
+hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream);
+
+}  // namespace hy

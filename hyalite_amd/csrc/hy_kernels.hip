@@ -1,0 +1,452 @@
+// Database-scan kernels for CDNA4 (gfx950).
+//
+// Two families compute the affine-gap (Gotoh) recurrence of hyalite's scalar oracle
+// (src/kernel.rs:168-217) with its answer regions and tie-break (src/kernel.rs:113-129, 223-260):
+//
+//  * k_inter32 — inter-sequence (Rognes) kernel: one target per lane, a tile of Q query rows held
+//    in VGPRs, the query profile for the tile staged in LDS, the target columns streamed from a
+//    lane-interleaved HBM layout (one 32-bit word carries four columns of a lane). The tile's
+//    bottom row is spilled to a per-group boundary buffer and read back by the next tile. It
+//    replaces scan_batch / scan_batch_ends (src/inter.rs:683-997).
+//
+//  * k_sys32 — intra-sequence systolic kernel: one (query, target) pair per wavefront, each lane
+//    owning R consecutive query rows; lane l works on column s-l+1 at step s and receives the row
+//    above from lane l-1 through a DPP wave shift. It serves long targets and recovers the end
+//    positions of the single-best winner (the role align_core plays in src/database.rs:183-197).
+//
+// Arithmetic is the 32-bit realisation of DETERMINISM.md §1: sentinel i32::MIN/4, wrapping
+// add/sub, exact for every input the width proof accepts.
+#include <hip/hip_runtime.h>
+#include <climits>
+
+#include "hy_internal.h"
+
+namespace hy {
+namespace {
+
+__device__ __forceinline__ int wadd(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
+__device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
+
+// Flags::for_mode (src/kernel.rs:43-98).
+template <int MODE>
+struct MF {
+    static constexpr bool top = MODE == kHW || MODE == kOV || MODE == kSW;
+    static constexpr bool left = MODE == kOV || MODE == kSW || MODE == kSHW;
+    static constexpr bool lrow = MODE == kHW || MODE == kOV;
+    static constexpr bool lcol = MODE == kOV || MODE == kSHW;
+    static constexpr bool local = MODE == kSW;
+};
+
+// gap_penalty (src/kernel.rs:103-109).
+__device__ __forceinline__ int gap32(int go, int ge, int len) {
+    return len == 0 ? 0 : (int)((long long)go + (long long)(len - 1) * (long long)ge);
+}
+
+// H on the DP border: row 0 (top) or column 0 (left).
+template <int MODE>
+__device__ __forceinline__ int border(int i, int j, int go, int ge) {
+    if (i == 0) return (j == 0 || MF<MODE>::top) ? 0 : -gap32(go, ge, j);
+    return MF<MODE>::left ? 0 : -gap32(go, ge, i);
+}
+
+// Best::consider (src/kernel.rs:113-129): maximise score, then minimise (col, row).
+__device__ __forceinline__ void consider(int& bs, int& bj, int& bi, int s, int j, int i) {
+    const bool take = s > bs || (s == bs && (j < bj || (j == bj && i < bi)));
+    if (take) { bs = s; bj = j; bi = i; }
+}
+
+// The answer-region cells that lie on the DP border, reduced in closed form. Borders never
+// increase away from the origin (gaps are non-negative), so on each border segment of the region
+// the cell nearest the origin dominates or ties-and-wins.
+template <int MODE>
+__device__ __forceinline__ void border_best(int m, int n, int go, int ge, int& bs, int& bj,
+                                            int& bi) {
+    bs = INT_MIN; bj = INT_MAX; bi = INT_MAX;
+    if (MF<MODE>::local) { consider(bs, bj, bi, 0, 0, 0); return; }
+    if (m == 0 || n == 0) consider(bs, bj, bi, border<MODE>(m, n, go, ge), n, m);
+    if (MF<MODE>::lrow) consider(bs, bj, bi, border<MODE>(m, 0, go, ge), 0, m);
+    if (MF<MODE>::lcol) consider(bs, bj, bi, border<MODE>(0, n, go, ge), n, 0);
+}
+
+// 64-bit argmax key: higher score first, then the smaller database index.
+__device__ __forceinline__ unsigned long long make_key(int score, uint32_t gi) {
+    return ((unsigned long long)((uint32_t)score ^ 0x80000000u) << 32) |
+           (unsigned long long)(0xFFFFFFFFu - gi);
+}
+__device__ __forceinline__ int key_score(unsigned long long k) {
+    return (int)((uint32_t)(k >> 32) ^ 0x80000000u);
+}
+__device__ __forceinline__ uint32_t key_index(unsigned long long k) {
+    return 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFull);
+}
+
+__device__ __forceinline__ void wave_argmax(unsigned long long k, unsigned long long* dst) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(k, off);
+        k = o > k ? o : k;
+    }
+    if (threadIdx.x == 0 && k != 0ull) atomicMax(dst, k);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_key_result(const unsigned long long* key, Result* r) {
+    if (threadIdx.x == 0) {
+        const unsigned long long k = *key;
+        r->score = key_score(k);
+        r->db_index = key_index(k);
+        r->query_end = -1;
+        r->target_end = -1;
+        r->check = k == 0ull ? 1 : 0;
+        r->pad = 0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Inter-sequence kernel, 32-bit lanes.
+// ------------------------------------------------------------------------------------------------
+//
+// Per cell (i, j) with HG = H - gap_open kept per row (so H - go is computed once and feeds both
+// E of the next column and F of the next row) and the profile holding sub + go:
+//     diag = HG[i-1][j-1] + (sub + go)       = H[i-1][j-1] + sub
+//     E    = max(HG[i][j-1], E - ge)
+//     F    = max(HG[i-1][j], F - ge)
+//     H    = max(diag, E, F)   (local: max(H, 0))
+// The padding symbol (al_eff) and padded query rows use kPad32, so in local mode padded cells
+// never exceed a real cell that precedes them in (col, row) order; in the other modes the answer
+// cells are selected explicitly.
+template <int MODE, int Q, bool ENDS>
+__global__ __launch_bounds__(64) void k_inter32(InterArgs a) {
+    using F = MF<MODE>;
+    extern __shared__ int prof[];
+    constexpr int S = Q + 4;  // row stride: consecutive symbols start 4 banks apart
+    const int lane = threadIdx.x;
+    const Group g = a.groups[blockIdx.x];
+    const bool live = lane < (int)g.count;
+    const uint32_t slot = g.first + (live ? lane : 0);
+    const int n = live ? (int)a.lens[slot] : 0;
+    const int L = (int)g.ncols;
+    const int m = a.m, go = a.go, ge = a.ge;
+
+    int bs, bj, bi;
+    border_best<MODE>(m, n, go, ge, bs, bj, bi);
+
+    const int nchunks = (m + Q - 1) / Q;
+    int2* sp = a.spill + g.spill_off + lane;
+    const uint32_t* wp = a.words + g.word_off + lane;
+
+    for (int c = 0; c < nchunks; ++c) {
+        const int i0 = c * Q + 1;
+        const bool lastc = c == nchunks - 1;
+        const int rm = m - i0;  // register of row m in the last tile
+        __syncthreads();
+        for (int k = lane; k < (a.al_eff + 1) * Q; k += 64) {
+            const int sym = k / Q, r = k - sym * Q, i = i0 + r;
+            int v = kPad32;
+            if (sym < a.al_eff && i <= m) v = a.matrix[(int)a.query[i - 1] * a.al + sym] + go;
+            prof[sym * S + r] = v;
+        }
+        __syncthreads();
+
+        int HG[Q], E[Q];
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            HG[r] = wsub(border<MODE>(i0 + r, 0, go, ge), go);
+            E[r] = kNeg32;
+        }
+        int HGd = wsub(border<MODE>(i0 - 1, 0, go, ge), go);
+        int cbs = INT_MIN, cbj = 0, cbi = 0;  // tile-local best (local mode, ends)
+
+        for (int jb = 0; jb < L; jb += 4) {
+            const uint32_t w = wp[(size_t)(jb >> 2) * 64];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = jb + b + 1;
+                if (j > L) break;
+                int hgup, f;
+                if (c == 0) {
+                    hgup = wsub(border<MODE>(0, j, go, ge), go);
+                    f = kNeg32;
+                } else {
+                    const int2 t = sp[(size_t)j * 64];
+                    hgup = t.x;
+                    f = t.y;
+                }
+                const int sym = (int)((w >> (8 * b)) & 0xffu);
+                const int4* pr = reinterpret_cast<const int4*>(&prof[sym * S]);
+                int sg[Q];
+#pragma unroll
+                for (int r4 = 0; r4 < Q / 4; ++r4) {
+                    const int4 v = pr[r4];
+                    sg[4 * r4 + 0] = v.x; sg[4 * r4 + 1] = v.y;
+                    sg[4 * r4 + 2] = v.z; sg[4 * r4 + 3] = v.w;
+                }
+                int hgd = HGd;
+                HGd = hgup;
+                int cmax = INT_MIN;
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    const int diag = wadd(hgd, sg[r]);
+                    const int e = max(HG[r], wsub(E[r], ge));
+                    f = max(hgup, wsub(f, ge));
+                    int h = max(max(diag, e), f);
+                    if (F::local) h = max(h, 0);
+                    hgd = HG[r];
+                    E[r] = e;
+                    HG[r] = wsub(h, go);
+                    hgup = HG[r];
+                    if (F::local) {
+                        if (ENDS) {
+                            if (h > cbs) { cbs = h; cbj = j; cbi = i0 + r; }
+                        } else {
+                            cmax = max(cmax, h);
+                        }
+                    }
+                }
+                if (F::local && !ENDS) bs = max(bs, cmax);
+                if (c + 1 < nchunks) sp[(size_t)j * 64] = make_int2(HG[Q - 1], f);
+                if (!F::local && lastc && (F::lrow || j == n) && j <= n) {
+                    int v = HG[0];
+#pragma unroll
+                    for (int r = 1; r < Q; ++r)
+                        if (r == rm) v = HG[r];
+                    v = wadd(v, go);
+                    if (ENDS) consider(bs, bj, bi, v, j, m);
+                    else bs = max(bs, v);
+                }
+                if (F::lcol && j == n) {
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) {
+                        if (i0 + r <= m) {
+                            const int v = wadd(HG[r], go);
+                            if (ENDS) consider(bs, bj, bi, v, n, i0 + r);
+                            else bs = max(bs, v);
+                        }
+                    }
+                }
+            }
+        }
+        if (F::local && ENDS && cbs != INT_MIN) consider(bs, bj, bi, cbs, cbj, cbi);
+    }
+
+    const uint32_t gi = a.gidx[slot];
+    if (ENDS) {
+        if (live) {
+            a.out_score[gi] = bs;
+            a.out_qend[gi] = bi == 0 ? -1 : (int64_t)bi - 1;
+            a.out_tend[gi] = bj == 0 ? -1 : (int64_t)bj - 1;
+        }
+    } else {
+        if (live && a.out_score) a.out_score[gi] = bs;
+        if (a.key) wave_argmax(live ? make_key(bs, gi) : 0ull, a.key);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Intra-sequence systolic kernel, 32-bit lanes, one (query, target) pair per wavefront.
+// ------------------------------------------------------------------------------------------------
+template <int MODE, int R, bool ENDS, bool WINNER>
+__global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
+    using F = MF<MODE>;
+    extern __shared__ int smat[];
+    const int lane = threadIdx.x;
+    // Winner mode: an all-zero key means no hit was recorded; fall back to index 0 and let the
+    // self-check below flag it.
+    const unsigned long long wk = WINNER ? *a.winner_key : 0ull;
+    const uint32_t gi = WINNER ? (wk != 0ull ? key_index(wk) : 0u) : a.list[blockIdx.x];
+    const uint8_t* t = a.flat + a.off[gi];
+    const int n = (int)(a.off[gi + 1] - a.off[gi]);
+    const int m = a.m, go = a.go, ge = a.ge, ale = a.al_eff;
+    const bool lds_mat = ale * ale <= 16384;
+    const int rowstride = lds_mat ? ale : a.al;
+    const int* mat = lds_mat ? smat : a.matrix;
+    if (lds_mat) {
+        for (int k = lane; k < ale * ale; k += 64) smat[k] = a.matrix[(k / ale) * a.al + (k % ale)];
+    }
+    __syncthreads();
+
+    int bs = INT_MIN, bj = INT_MAX, bi = INT_MAX;
+    if (lane == 0) border_best<MODE>(m, n, go, ge, bs, bj, bi);
+
+    constexpr int P = 64 * R;
+    const int npass = (m + P - 1) / P;
+    int2* sp = a.spill + (WINNER ? 0 : (size_t)blockIdx.x * a.spill_stride);
+    for (int p = 0; p < npass; ++p) {
+        const int r0 = p * P + lane * R + 1;
+        int qb[R], HG[R], E[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r0 + r;
+            qb[r] = i <= m ? (int)a.query[i - 1] * rowstride : -1;
+            HG[r] = wsub(border<MODE>(i, 0, go, ge), go);
+            E[r] = kNeg32;
+        }
+        int HGd = wsub(border<MODE>(r0 - 1, 0, go, ge), go);
+        int topHG = 0, topF = kNeg32;
+        const int steps = n + 63;
+        for (int s = 0; s < steps; ++s) {
+            const int j = s - lane + 1;
+            if (lane == 0 && j <= n) {
+                if (p == 0) {
+                    topHG = wsub(border<MODE>(0, j, go, ge), go);
+                    topF = kNeg32;
+                } else {
+                    const int2 v = sp[j];
+                    topHG = v.x;
+                    topF = v.y;
+                }
+            }
+            int outHG = 0, outF = 0;
+            if (j >= 1 && j <= n) {
+                const int c = t[j - 1];
+                int f = topF, hgup = topHG, hgd = HGd;
+                HGd = topHG;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int sub = qb[r] >= 0 ? mat[qb[r] + c] : kPad32;
+                    const int diag = wadd(wadd(hgd, go), sub);
+                    const int e = max(HG[r], wsub(E[r], ge));
+                    f = max(hgup, wsub(f, ge));
+                    int h = max(max(diag, e), f);
+                    if (F::local) h = max(h, 0);
+                    hgd = HG[r];
+                    E[r] = e;
+                    HG[r] = wsub(h, go);
+                    hgup = HG[r];
+                    const int i = r0 + r;
+                    if (F::local) {
+                        if (i <= m) {
+                            if (ENDS) consider(bs, bj, bi, h, j, i);
+                            else bs = max(bs, h);
+                        }
+                    } else {
+                        if (F::lcol && j == n && i <= m) {
+                            if (ENDS) consider(bs, bj, bi, h, j, i);
+                            else bs = max(bs, h);
+                        }
+                        if (i == m && (F::lrow || j == n)) {
+                            if (ENDS) consider(bs, bj, bi, h, j, i);
+                            else bs = max(bs, h);
+                        }
+                    }
+                }
+                outHG = HG[R - 1];
+                outF = f;
+                if (lane == 63 && p + 1 < npass) sp[j] = make_int2(outHG, outF);
+            }
+            // Lane l receives lane l-1's bottom row (wave_shr:1); lane 0 keeps its own value.
+            topHG = __builtin_amdgcn_update_dpp(topHG, outHG, 0x138, 0xf, 0xf, false);
+            topF = __builtin_amdgcn_update_dpp(topF, outF, 0x138, 0xf, 0xf, false);
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+
+    if (ENDS) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const int os = __shfl_xor(bs, off), oj = __shfl_xor(bj, off), oi = __shfl_xor(bi, off);
+            consider(bs, bj, bi, os, oj, oi);
+        }
+    } else {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) bs = max(bs, __shfl_xor(bs, off));
+    }
+    if (WINNER) {
+        if (lane == 0) {
+            const int ks = key_score(wk);
+            Result* r = a.result;
+            r->score = bs;
+            r->db_index = gi;
+            r->query_end = ENDS ? (bi == 0 ? -1 : (int64_t)bi - 1) : -1;
+            r->target_end = ENDS ? (bj == 0 ? -1 : (int64_t)bj - 1) : -1;
+            r->check = (bs == ks && wk != 0ull) ? 0 : 1;
+            r->pad = 0;
+        }
+    } else {
+        if (lane == 0) {
+            if (a.out_score) a.out_score[gi] = bs;
+            if (ENDS) {
+                a.out_qend[gi] = bi == 0 ? -1 : (int64_t)bi - 1;
+                a.out_tend[gi] = bj == 0 ? -1 : (int64_t)bj - 1;
+            }
+            if (a.key) atomicMax(a.key, make_key(bs, gi));
+        }
+    }
+}
+
+template <int MODE>
+hipError_t launch_inter32_m(bool ends, const InterArgs& a, uint32_t ngroups, hipStream_t st) {
+    constexpr int Q = 32;
+    const size_t lds = (size_t)(a.al_eff + 1) * (Q + 4) * sizeof(int);
+    if (ends)
+        hipLaunchKernelGGL((k_inter32<MODE, Q, true>), dim3(ngroups), dim3(64), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_inter32<MODE, Q, false>), dim3(ngroups), dim3(64), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                          hipStream_t st) {
+    constexpr int R = 8;
+    const size_t lds = a.al_eff * a.al_eff <= 16384 ? (size_t)a.al_eff * a.al_eff * sizeof(int) : 0;
+    if (winner) {
+        if (ends) hipLaunchKernelGGL((k_sys32<MODE, R, true, true>), dim3(1), dim3(64), lds, st, a);
+        else hipLaunchKernelGGL((k_sys32<MODE, R, false, true>), dim3(1), dim3(64), lds, st, a);
+    } else {
+        if (ends)
+            hipLaunchKernelGGL((k_sys32<MODE, R, true, false>), dim3(nblocks), dim3(64), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_sys32<MODE, R, false, false>), dim3(nblocks), dim3(64), lds, st,
+                               a);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
+                           hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream) {
+    hipLaunchKernelGGL(k_key_result, dim3(1), dim3(64), 0, stream, key, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngroups,
+                          hipStream_t stream) {
+    if (ngroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_inter32_m<kSW>(ends, a, ngroups, stream);
+    case kNW: return launch_inter32_m<kNW>(ends, a, ngroups, stream);
+    case kHW: return launch_inter32_m<kHW>(ends, a, ngroups, stream);
+    case kOV: return launch_inter32_m<kOV>(ends, a, ngroups, stream);
+    case kSHW: return launch_inter32_m<kSHW>(ends, a, ngroups, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                        hipStream_t stream) {
+    if (!winner && nblocks == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_sys32_m<kSW>(ends, winner, a, nblocks, stream);
+    case kNW: return launch_sys32_m<kNW>(ends, winner, a, nblocks, stream);
+    case kHW: return launch_sys32_m<kHW>(ends, winner, a, nblocks, stream);
+    case kOV: return launch_sys32_m<kOV>(ends, winner, a, nblocks, stream);
+    case kSHW: return launch_sys32_m<kSHW>(ends, winner, a, nblocks, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace hy

@@ -1,0 +1,145 @@
+/*
+ * hyalite_hip — C ABI of the MI355X (gfx950) database-scan backend.
+ *
+ * This is the drop-in boundary a `Backend::Hip` arm of hyalite would bind through a thin FFI
+ * crate. It replaces, for the Hip backend only, the packed-database build and the
+ * inter-sequence scan of hyalite 0.3.0:
+ *
+ *   hy_db_build      <- DatabaseBuilder::build            src/database.rs:737-877
+ *                       (+ build_width_groups             src/database.rs:884-948,
+ *                          PackedDb::build                 src/inter.rs:431-508)
+ *   hy_scratch_new   <- Scratch::new                      src/database.rs:965-999
+ *   hy_scan          <- Database::scan / scan_dispatch    src/database.rs:83-208, src/inter.rs:1074-1124
+ *   hy_scan_all      <- Database::scan_all / fill_ends    src/database.rs:213-301, 347-464,
+ *                                                          src/inter.rs:1226-1262
+ *   hy_scan_scores   <- Database::scan_scores / fill_scores src/database.rs:305-329, 481-515,
+ *                                                          src/inter.rs:1161-1205
+ *   hy_scan_batch    <- a loop of Database::scan over many queries (batched form; no single
+ *                       reference counterpart — each hit equals Database::scan of that query)
+ *   hy_backend_available <- Backend::is_available         src/backend.rs:77-84
+ *   hy_strerror      <- impl Display for Error             src/error.rs:98-157
+ *
+ * Conventions
+ *   - Sequences and queries are pre-encoded alphabet indices (uint8_t), as in the reference.
+ *   - Ends are 0-based inclusive positions; -1 stands for `None` (hit.rs:16-25).
+ *   - Every function returns 0 on success or one of the HY_E_* codes below; no exception or
+ *     longjmp crosses the ABI. Construction errors use the numbering of the reference's `Error`
+ *     variants (error.rs:16-96); scan-time codes (>= 100) are contract violations that the
+ *     reference turns into panics (database.rs:74-81), so a Rust wrapper should panic on them.
+ *   - A hy_db is immutable after build and may be shared by many threads; each thread uses its
+ *     own hy_scratch (one HIP stream each). hy_db_retain/hy_db_free reference-count it, which is
+ *     what a Rust `Clone` of `Database` maps to.
+ */
+#ifndef HYALITE_HIP_H
+#define HYALITE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hy_db hy_db;
+typedef struct hy_scratch hy_scratch;
+
+/* Mode (src/mode.rs:11-27). */
+enum { HY_MODE_SW = 0, HY_MODE_NW = 1, HY_MODE_HW = 2, HY_MODE_OV = 3, HY_MODE_SHW = 4 };
+/* SearchType (src/search.rs:13-27); Alignment is outside this backend. */
+enum { HY_SEARCH_SCORE = 0, HY_SEARCH_SCORE_END = 1 };
+/* Backend choice: AUTO or FORCE(HIP). The CPU backends stay in the reference crate. */
+enum { HY_BACKEND_AUTO = 0, HY_BACKEND_HIP = 9 };
+
+/* Status codes. 1..11 mirror error.rs:16-96 in declaration order. */
+enum {
+    HY_OK = 0,
+    HY_E_EMPTY_ALPHABET = 1,
+    HY_E_MATRIX_SHAPE = 2,
+    HY_E_NEGATIVE_GAP_PENALTY = 3,
+    HY_E_GAP_OPEN_LESS_THAN_EXTEND = 4,
+    HY_E_SCORE_RANGE_TOO_WIDE = 5,   /* err_detail = the width-proof bound */
+    HY_E_SYMBOL_OUT_OF_RANGE = 6,    /* err_detail = the offending symbol */
+    HY_E_INCOMPLETE_BUILDER = 7,     /* err_detail = 0 sequences, 1 scoring, 2 mode, 3 max_query_len */
+    HY_E_EMPTY_DATABASE = 8,
+    HY_E_BACKEND_UNAVAILABLE = 9,
+    HY_E_INVALID_BACKEND_NAME = 10,
+    HY_E_TRACEBACK_BUDGET = 11,      /* never produced: Alignment search is not offered here */
+    /* Scan-time contract violations and runtime failures. */
+    HY_E_QUERY_TOO_LONG = 100,       /* err: query longer than max_query_len */
+    HY_E_QUERY_SYMBOL = 101,         /* query symbol >= alphabet_len */
+    HY_E_NULL_ARGUMENT = 102,
+    HY_E_INVALID_ARGUMENT = 103,
+    HY_E_DEVICE = 104,               /* a HIP runtime call failed */
+    HY_E_OUT_OF_MEMORY = 105,
+    HY_E_INTERNAL = 106              /* a device-side self-check failed */
+};
+
+/* Field flags for hy_db_build's `have` mask: which builder fields were set. A field left out
+ * yields HY_E_INCOMPLETE_BUILDER in the reference's check order. */
+enum {
+    HY_HAVE_SEQUENCES = 1u << 0,
+    HY_HAVE_SCORING = 1u << 1,
+    HY_HAVE_MODE = 1u << 2,
+    HY_HAVE_MAX_QUERY_LEN = 1u << 3,
+    HY_HAVE_ALL = 0xFu
+};
+
+/* Whether a gfx950 HIP device is usable (Backend::is_available for Hip). */
+int hy_backend_available(void);
+
+/* Parse a backend name as HYALITE_BACKEND would (src/backend.rs:143-155 plus "hip").
+ * Writes HY_BACKEND_AUTO or HY_BACKEND_HIP to *out; the reference CPU names
+ * (scalar, sse4.1, sse41, avx2, neon) parse but are not served here: *out = -1. */
+int hy_backend_parse(const char* name, int32_t* out);
+
+/* Build a device-resident database. Validation order, error codes and the per-sequence width
+ * proof follow DatabaseBuilder::build (database.rs:737-877). Sequences are copied; the caller's
+ * buffers may be freed afterwards. `backend` is HY_BACKEND_AUTO or HY_BACKEND_HIP; AUTO also
+ * consults HYALITE_BACKEND. err_detail (may be NULL) receives the payload documented above. */
+int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_t n, uint32_t have,
+                const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len, int32_t gap_open,
+                int32_t gap_ext, uint32_t mode, uint32_t search_type, size_t max_query_len,
+                uint32_t backend, hy_db** out, int64_t* err_detail);
+
+/* Reference-count a database (a Rust `Clone`); every retain needs one hy_db_free. */
+int hy_db_retain(hy_db* db);
+void hy_db_free(hy_db* db);
+
+/* Accessors (Database::sequence_count, score_width, max_query_len, max_target_len, mode,
+ * search_type; database.rs:581-656). widest_width_bits is 8, 16 or 32. */
+int hy_db_info(const hy_db* db, uint32_t* widest_width_bits, uint64_t* n, uint64_t* max_query_len,
+               uint64_t* max_target_len, uint32_t* mode, uint32_t* search_type);
+
+/* Per-width partition sizes (number of targets proven i8, i16, i32). */
+int hy_db_partitions(const hy_db* db, uint64_t* n_i8, uint64_t* n_i16, uint64_t* n_i32);
+
+/* Per-thread working memory: a HIP stream plus device scratch sized for `db`. */
+int hy_scratch_new(const hy_db* db, hy_scratch** out);
+void hy_scratch_free(hy_scratch* s);
+
+/* Single best hit (Database::scan). Highest score, smallest db_index on ties; ends follow the
+ * (target_end, query_end) tie-break and are -1 for SearchType::Score. */
+int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen, int32_t* score,
+            uint64_t* db_index, int64_t* query_end, int64_t* target_end);
+
+/* One hit per target in database order (Database::scan_all). Each output array has n entries;
+ * query_ends/target_ends may be NULL for SearchType::Score (they would be all -1). */
+int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen, int32_t* scores,
+                int64_t* query_ends, int64_t* target_ends);
+
+/* Per-target scores in database order (Database::scan_scores; SearchType-independent). */
+int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
+                   int32_t* scores);
+
+/* Batched form: nq queries in one submission; hit k equals hy_scan of query k. */
+int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
+                  const size_t* qlens, size_t nq, int32_t* scores, uint64_t* db_indices,
+                  int64_t* query_ends, int64_t* target_ends);
+
+/* Human-readable message for a status code (error.rs:98-157 wording where one exists). */
+const char* hy_strerror(int code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

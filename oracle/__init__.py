@@ -1,0 +1,238 @@
+"""CPU oracle for the database scan — TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings over ``liboracle.so`` (built from ``oracle.c`` by ``oracle/Makefile``), a plain-C
+restatement of hyalite's scalar Gotoh kernel (src/kernel.rs:39-277), width proof
+(src/width.rs:36-169), scoring validation (src/scoring.rs:65-100) and scalar database
+reductions (src/database.rs:115-144, 405-414, 511-513), plus the brute-force enumerators of
+tests/common/mod.rs:65-171.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py`` may import
+this package, and only as the checker. The product path (``hyalite_amd``) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+MODE_SW, MODE_NW, MODE_HW, MODE_OV, MODE_SHW = 0, 1, 2, 3, 4
+ALL_MODES = (MODE_SW, MODE_NW, MODE_HW, MODE_OV, MODE_SHW)
+
+OK = 0
+E_EMPTY_ALPHABET = 1
+E_MATRIX_SHAPE = 2
+E_NEGATIVE_GAP = 3
+E_GAP_OPEN_LT_EXT = 4
+E_SCORE_RANGE = 5
+
+_lib = None
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE, "-B"], check=True)
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    lib.hyo_gap_penalty.restype = ctypes.c_int32
+    lib.hyo_gap_penalty.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64]
+    lib.hyo_scoring_check.restype = ctypes.c_int
+    lib.hyo_scoring_check.argtypes = [ctypes.c_uint32, _i32p, ctypes.c_uint64, ctypes.c_int32,
+                                      ctypes.c_int32, _i32p, _i32p]
+    lib.hyo_required_width.restype = ctypes.c_int
+    lib.hyo_required_width.argtypes = [ctypes.c_uint32, ctypes.c_int32, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
+                                       ctypes.c_uint64, _i64p, _i32p]
+    lib.hyo_align_core.restype = ctypes.c_int
+    lib.hyo_align_core.argtypes = [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64, ctypes.c_uint32,
+                                   _i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _i32p,
+                                   _i64p, _i64p]
+    lib.hyo_brute_score.restype = ctypes.c_int64
+    lib.hyo_brute_score.argtypes = [ctypes.c_uint32, _u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64,
+                                    ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32]
+    lib.hyo_brute_end.restype = None
+    lib.hyo_brute_end.argtypes = [ctypes.c_uint32, _u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64,
+                                  ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32, _i32p,
+                                  _i64p, _i64p]
+    lib.hyo_scan_all.restype = ctypes.c_int
+    lib.hyo_scan_all.argtypes = [_u8p, _u64p, ctypes.c_uint64, _u8p, ctypes.c_uint64,
+                                 ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32,
+                                 ctypes.c_uint32, _i32p, _i64p, _i64p, ctypes.c_int]
+    lib.hyo_scan.restype = ctypes.c_int
+    lib.hyo_scan.argtypes = [_u8p, _u64p, ctypes.c_uint64, _u8p, ctypes.c_uint64,
+                             ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32,
+                             ctypes.c_uint32, ctypes.c_uint32, _i32p, _u64p, _i64p, _i64p,
+                             ctypes.c_int]
+    if hasattr(lib, "hyo_avx2_scan_scores"):
+        lib.hyo_avx2_scan_scores.restype = ctypes.c_int
+        lib.hyo_avx2_scan_scores.argtypes = [_u8p, _u64p, ctypes.c_uint64, _u8p, ctypes.c_uint64,
+                                             ctypes.c_uint32, _i32p, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_uint32, _i32p,
+                                             ctypes.c_int, ctypes.c_int]
+    _lib = lib
+    return lib
+
+
+def _u8(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.uint8))
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _ptr(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+def _opt(v: int):
+    return None if v < 0 else int(v)
+
+
+def gap_penalty(go: int, ge: int, length: int) -> int:
+    return int(_load().hyo_gap_penalty(go, ge, length))
+
+
+def scoring_check(al: int, matrix, go: int, ge: int):
+    """Returns (status, min_entry, max_entry)."""
+    m = _i32(matrix) if len(matrix) else np.zeros(1, dtype=np.int32)
+    lo, hi = ctypes.c_int32(0), ctypes.c_int32(0)
+    rc = _load().hyo_scoring_check(al, _ptr(m, _i32p), len(matrix), go, ge, ctypes.byref(lo),
+                                   ctypes.byref(hi))
+    return rc, lo.value, hi.value
+
+
+def required_width(mode: int, min_e: int, max_e: int, go: int, ge: int, mql: int, mtl: int):
+    """Returns (status, bound, width_bits) where width_bits is 8, 16, 32 (0 when too wide)."""
+    bound, bits = ctypes.c_int64(0), ctypes.c_int32(0)
+    rc = _load().hyo_required_width(mode, min_e, max_e, go, ge, mql, mtl, ctypes.byref(bound),
+                                    ctypes.byref(bits))
+    return rc, bound.value, bits.value
+
+
+def align_core(q, t, al: int, matrix, go: int, ge: int, mode: int):
+    """(score, query_end or None, target_end or None) for one pair."""
+    qa, ta, ma = _u8(q), _u8(t), _i32(matrix)
+    s, qe, te = ctypes.c_int32(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    rc = _load().hyo_align_core(_ptr(qa, _u8p), len(qa), _ptr(ta, _u8p), len(ta), al,
+                                _ptr(ma, _i32p), go, ge, mode, ctypes.byref(s), ctypes.byref(qe),
+                                ctypes.byref(te))
+    if rc != OK:
+        raise MemoryError("oracle allocation failed")
+    return s.value, _opt(qe.value), _opt(te.value)
+
+
+def brute_score(mode: int, q, t, al: int, matrix, go: int, ge: int) -> int:
+    qa, ta, ma = _u8(q), _u8(t), _i32(matrix)
+    return int(_load().hyo_brute_score(mode, _ptr(qa, _u8p), len(qa), _ptr(ta, _u8p), len(ta), al,
+                                       _ptr(ma, _i32p), go, ge))
+
+
+def brute_end(mode: int, q, t, al: int, matrix, go: int, ge: int):
+    qa, ta, ma = _u8(q), _u8(t), _i32(matrix)
+    s, qe, te = ctypes.c_int32(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    _load().hyo_brute_end(mode, _ptr(qa, _u8p), len(qa), _ptr(ta, _u8p), len(ta), al,
+                          _ptr(ma, _i32p), go, ge, ctypes.byref(s), ctypes.byref(qe),
+                          ctypes.byref(te))
+    return s.value, _opt(qe.value), _opt(te.value)
+
+
+def flatten(seqs):
+    """(flat uint8 residues, uint64 offsets of length n+1) for a list of sequences."""
+    lens = np.fromiter((len(s) for s in seqs), dtype=np.uint64, count=len(seqs))
+    off = np.zeros(len(seqs) + 1, dtype=np.uint64)
+    np.cumsum(lens, out=off[1:])
+    flat = np.empty(int(off[-1]) if len(seqs) else 0, dtype=np.uint8)
+    for k, s in enumerate(seqs):
+        if len(s):
+            flat[int(off[k]):int(off[k + 1])] = np.asarray(s, dtype=np.uint8)
+    return flat, off
+
+
+def scan_all_flat(flat, off, q, al, matrix, go, ge, mode, ends=True, threads=1):
+    """Per-target (scores, query_ends, target_ends) in database order; ends are -1 for None."""
+    n = len(off) - 1
+    flat = _u8(flat) if len(flat) else np.zeros(1, dtype=np.uint8)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    qa, ma = _u8(q) if len(q) else np.zeros(1, dtype=np.uint8), _i32(matrix)
+    scores = np.empty(max(n, 1), dtype=np.int32)
+    qe = np.empty(max(n, 1), dtype=np.int64) if ends else None
+    te = np.empty(max(n, 1), dtype=np.int64) if ends else None
+    rc = _load().hyo_scan_all(_ptr(flat, _u8p), _ptr(off, _u64p), n, _ptr(qa, _u8p), len(q), al,
+                              _ptr(ma, _i32p), go, ge, mode, _ptr(scores, _i32p),
+                              _ptr(qe, _i64p) if ends else None,
+                              _ptr(te, _i64p) if ends else None, threads)
+    if rc != OK:
+        raise MemoryError("oracle allocation failed")
+    if ends:
+        return scores[:n], qe[:n], te[:n]
+    return scores[:n]
+
+
+def scan_all(seqs, q, al, matrix, go, ge, mode, threads=1):
+    """List of (score, query_end, target_end) per target, None for absent ends."""
+    flat, off = flatten(seqs)
+    s, qe, te = scan_all_flat(flat, off, q, al, matrix, go, ge, mode, ends=True, threads=threads)
+    return [(int(a), _opt(int(b)), _opt(int(c))) for a, b, c in zip(s, qe, te)]
+
+
+def scan_scores(seqs, q, al, matrix, go, ge, mode, threads=1):
+    flat, off = flatten(seqs)
+    return scan_all_flat(flat, off, q, al, matrix, go, ge, mode, ends=False, threads=threads)
+
+
+def scan_flat(flat, off, q, al, matrix, go, ge, mode, tracks_end, threads=1):
+    """Single best hit (score, db_index, query_end, target_end)."""
+    n = len(off) - 1
+    flat = _u8(flat) if len(flat) else np.zeros(1, dtype=np.uint8)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    qa, ma = _u8(q) if len(q) else np.zeros(1, dtype=np.uint8), _i32(matrix)
+    s, idx = ctypes.c_int32(0), ctypes.c_uint64(0)
+    qe, te = ctypes.c_int64(0), ctypes.c_int64(0)
+    rc = _load().hyo_scan(_ptr(flat, _u8p), _ptr(off, _u64p), n, _ptr(qa, _u8p), len(q), al,
+                          _ptr(ma, _i32p), go, ge, mode, 1 if tracks_end else 0, ctypes.byref(s),
+                          ctypes.byref(idx), ctypes.byref(qe), ctypes.byref(te), threads)
+    if rc != OK:
+        raise ValueError(f"oracle scan failed with status {rc}")
+    return s.value, idx.value, _opt(qe.value), _opt(te.value)
+
+
+def scan(seqs, q, al, matrix, go, ge, mode, tracks_end, threads=1):
+    flat, off = flatten(seqs)
+    return scan_flat(flat, off, q, al, matrix, go, ge, mode, tracks_end, threads)
+
+
+def avx2_available() -> bool:
+    return hasattr(_load(), "hyo_avx2_scan_scores")
+
+
+def avx2_scan_scores_flat(flat, off, q, al, matrix, go, ge, mode, threads=1, sort=False):
+    """Per-target scores from the AVX2 16-lane i16 restatement of the reference's inter-sequence
+    kernel (CPU baseline). Only i16-provable databases are accepted."""
+    n = len(off) - 1
+    flat = _u8(flat) if len(flat) else np.zeros(1, dtype=np.uint8)
+    off = np.ascontiguousarray(off, dtype=np.uint64)
+    qa, ma = _u8(q) if len(q) else np.zeros(1, dtype=np.uint8), _i32(matrix)
+    scores = np.empty(max(n, 1), dtype=np.int32)
+    rc = _load().hyo_avx2_scan_scores(_ptr(flat, _u8p), _ptr(off, _u64p), n, _ptr(qa, _u8p),
+                                      len(q), al, _ptr(ma, _i32p), go, ge, mode,
+                                      _ptr(scores, _i32p), threads, 1 if sort else 0)
+    if rc != OK:
+        raise ValueError(f"avx2 baseline failed with status {rc}")
+    return scores[:n]

@@ -1,0 +1,404 @@
+/*
+ * CPU oracle for the database scan — TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the scalar Gotoh reference kernel of hyalite 0.3.0 and of the
+ * database-level reductions built on it. Only the test-suite, the smoke check and the
+ * CPU-baseline leg of the benchmark may load this library, and only as the checker: the
+ * product path (hyalite_amd/csrc) never links, loads or calls it.
+ *
+ * Reference files restated (paths relative to the hyalite source tree):
+ *   scoring validation ........ src/scoring.rs:65-100
+ *   width proof ............... src/width.rs:36-71 (ScoreWidth), 107-169 (magnitude_bound,
+ *                               required_width)
+ *   mode flags ................ src/kernel.rs:43-98 (Flags::for_mode)
+ *   gap penalty ............... src/kernel.rs:103-109
+ *   DP fill ................... src/kernel.rs:168-217 (fill_dp)
+ *   answer reduction .......... src/kernel.rs:113-129 (Best::consider), 223-260 (reduce_answer)
+ *   scan / scan_all / scan_scores (scalar backend) ... src/database.rs:115-144, 405-414,
+ *                               453-462, 511-513
+ *   brute-force enumerators ... tests/common/mod.rs:65-171 (independent structure: exhaustive
+ *                               path / substring enumeration, used to pin the DP)
+ *
+ * Every function below is pinned by the tests in tests/test_oracle.py (hand vectors from the
+ * reference's own tests, exhaustive brute force over short inputs, width-proof boundary KATs).
+ *
+ * Arithmetic: cells are computed in int64 and stored as int32. For every input the width proof
+ * accepts, the reference's wrapping i32 arithmetic never wraps, so the results are identical.
+ */
+#include <stdint.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+
+#include "oracle.h"
+
+/* src/kernel.rs:39 — sentinel for unreachable E/F cells. */
+#define HYO_NEG ((int64_t)(INT32_MIN / 4))
+
+/* src/kernel.rs:43-98 */
+typedef struct {
+    int top_row_free, left_col_free, answer_last_row, answer_last_col, local;
+} hyo_flags;
+
+static hyo_flags flags_for_mode(uint32_t mode) {
+    hyo_flags f = {0, 0, 0, 0, 0};
+    switch (mode) {
+    case HYO_MODE_NW: break;
+    case HYO_MODE_HW: f.top_row_free = 1; f.answer_last_row = 1; break;
+    case HYO_MODE_OV:
+        f.top_row_free = 1; f.left_col_free = 1; f.answer_last_row = 1; f.answer_last_col = 1;
+        break;
+    case HYO_MODE_SW: f.top_row_free = 1; f.left_col_free = 1; f.local = 1; break;
+    case HYO_MODE_SHW: f.left_col_free = 1; f.answer_last_col = 1; break;
+    default: break;
+    }
+    return f;
+}
+
+/* src/kernel.rs:103-109 — Opal convention go + (len-1)*ge, 0 for an empty gap. */
+int32_t hyo_gap_penalty(int32_t go, int32_t ge, uint64_t len) {
+    if (len == 0) return 0;
+    return (int32_t)((int64_t)go + ((int64_t)len - 1) * (int64_t)ge);
+}
+
+/* src/scoring.rs:65-100 — error order: EmptyAlphabet, MatrixShape, NegativeGapPenalty,
+ * GapOpenLessThanExtend. */
+int hyo_scoring_check(uint32_t al, const int32_t* matrix, uint64_t matrix_len, int32_t go,
+                      int32_t ge, int32_t* min_e, int32_t* max_e) {
+    if (al == 0) return HYO_E_EMPTY_ALPHABET;
+    if ((uint64_t)al * (uint64_t)al != matrix_len) return HYO_E_MATRIX_SHAPE;
+    if (go < 0 || ge < 0) return HYO_E_NEGATIVE_GAP;
+    if (go < ge) return HYO_E_GAP_OPEN_LT_EXT;
+    int32_t lo = matrix[0], hi = matrix[0];
+    for (uint64_t k = 1; k < matrix_len; ++k) {
+        if (matrix[k] < lo) lo = matrix[k];
+        if (matrix[k] > hi) hi = matrix[k];
+    }
+    if (min_e) *min_e = lo;
+    if (max_e) *max_e = hi;
+    return HYO_OK;
+}
+
+/* src/width.rs:107-137 — computed in 128-bit so no product can wrap. */
+static __int128 magnitude_bound(uint32_t mode, int32_t min_e, int32_t max_e, int32_t go,
+                                int32_t ge, uint64_t mql, uint64_t mtl) {
+    __int128 m = (__int128)mql, n = (__int128)mtl;
+    __int128 min_ij = m < n ? m : n;
+    __int128 max_pos = max_e > 0 ? (__int128)max_e : 0;
+    __int128 max_neg = min_e < 0 ? -(__int128)min_e : 0;
+    __int128 positive = min_ij * max_pos;
+    __int128 negative;
+    if (mode == HYO_MODE_SW) {
+        negative = go;
+    } else if (mode == HYO_MODE_OV) {
+        negative = min_ij * max_neg + go;
+    } else {
+        __int128 run = m + n - 1;
+        if (run < 0) run = 0;
+        negative = (m + n) * max_neg + go + run * ge;
+    }
+    return positive > negative ? positive : negative;
+}
+
+/* src/width.rs:145-169 with ScoreWidth::max_abs (width.rs:48-55) and narrowest_for (69-71). */
+int hyo_required_width(uint32_t mode, int32_t min_e, int32_t max_e, int32_t go, int32_t ge,
+                       uint64_t mql, uint64_t mtl, int64_t* bound_out, int32_t* width_bits) {
+    __int128 b = magnitude_bound(mode, min_e, max_e, go, ge, mql, mtl);
+    int64_t b64 = b > (__int128)INT64_MAX ? INT64_MAX : (int64_t)b;
+    if (bound_out) *bound_out = b64;
+    if (b64 <= 127) { *width_bits = 8; return HYO_OK; }
+    if (b64 <= 32767) { *width_bits = 16; return HYO_OK; }
+    if (b64 <= 536870911) { *width_bits = 32; return HYO_OK; }
+    *width_bits = 0;
+    return HYO_E_SCORE_RANGE;
+}
+
+/* src/kernel.rs:113-129 — maximise score, then minimise (grid_col, grid_row). */
+typedef struct { int64_t score; uint64_t row, col; } hyo_best;
+
+static void consider(hyo_best* b, int64_t s, uint64_t row, uint64_t col) {
+    if (s > b->score || (s == b->score && (col < b->col || (col == b->col && row < b->row)))) {
+        b->score = s; b->row = row; b->col = col;
+    }
+}
+
+/* src/kernel.rs:168-217 (fill_dp) followed by src/kernel.rs:223-260 (reduce_answer).
+ * h must hold (m+1)*(n+1) int32, f must hold n+1 int32. */
+static void align_core_buf(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n,
+                           uint32_t al, const int32_t* matrix, int32_t go, int32_t ge,
+                           uint32_t mode, int32_t* h, int32_t* f, int32_t* score,
+                           int64_t* qend, int64_t* tend) {
+    hyo_flags fl = flags_for_mode(mode);
+    const uint64_t cols = n + 1;
+#define H(i, j) h[(uint64_t)(i) * cols + (uint64_t)(j)]
+    H(0, 0) = 0;
+    for (uint64_t j = 1; j <= n; ++j) H(0, j) = fl.top_row_free ? 0 : -hyo_gap_penalty(go, ge, j);
+    for (uint64_t i = 1; i <= m; ++i) H(i, 0) = fl.left_col_free ? 0 : -hyo_gap_penalty(go, ge, i);
+    for (uint64_t j = 0; j <= n; ++j) f[j] = (int32_t)HYO_NEG;
+    for (uint64_t i = 1; i <= m; ++i) {
+        int64_t e = HYO_NEG;
+        const int32_t* qrow = matrix + (uint64_t)q[i - 1] * al;
+        for (uint64_t j = 1; j <= n; ++j) {
+            int64_t e1 = (int64_t)H(i, j - 1) - go, e2 = e - ge;
+            e = e1 > e2 ? e1 : e2;
+            int64_t f1 = (int64_t)H(i - 1, j) - go, f2 = (int64_t)f[j] - ge;
+            int64_t fj = f1 > f2 ? f1 : f2;
+            f[j] = (int32_t)fj;
+            int64_t diag = (int64_t)H(i - 1, j - 1) + qrow[t[j - 1]];
+            int64_t cell = diag;
+            if (e > cell) cell = e;
+            if (fj > cell) cell = fj;
+            if (fl.local && cell < 0) cell = 0;
+            H(i, j) = (int32_t)cell;
+        }
+    }
+    hyo_best b = {HYO_NEG, 0, 0};
+    if (fl.local) {
+        for (uint64_t i = 0; i <= m; ++i)
+            for (uint64_t j = 0; j <= n; ++j) consider(&b, H(i, j), i, j);
+    } else {
+        consider(&b, H(m, n), m, n);
+        if (fl.answer_last_row)
+            for (uint64_t j = 0; j <= n; ++j) consider(&b, H(m, j), m, j);
+        if (fl.answer_last_col)
+            for (uint64_t i = 0; i <= m; ++i) consider(&b, H(i, n), i, n);
+    }
+#undef H
+    *score = (int32_t)b.score;
+    *qend = b.row == 0 ? -1 : (int64_t)b.row - 1;
+    *tend = b.col == 0 ? -1 : (int64_t)b.col - 1;
+}
+
+/* src/kernel.rs:267-277 (align_core). */
+int hyo_align_core(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
+                   const int32_t* matrix, int32_t go, int32_t ge, uint32_t mode, int32_t* score,
+                   int64_t* qend, int64_t* tend) {
+    int32_t* h = (int32_t*)malloc(sizeof(int32_t) * (m + 1) * (n + 1));
+    int32_t* f = (int32_t*)malloc(sizeof(int32_t) * (n + 1));
+    if (!h || !f) { free(h); free(f); return HYO_E_NOMEM; }
+    align_core_buf(q, m, t, n, al, matrix, go, ge, mode, h, f, score, qend, tend);
+    free(h);
+    free(f);
+    return HYO_OK;
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* Brute force: tests/common/mod.rs:65-171. Exhaustive path enumeration for global alignment, */
+/* then maxima over substring windows for the other modes. Exponential; tiny inputs only.     */
+/* ---------------------------------------------------------------------------------------- */
+
+#define BRUTE_NEG_INF INT64_MIN
+
+typedef struct {
+    const uint8_t* q; uint64_t m; const uint8_t* t; uint64_t n;
+    const int32_t* mat; uint32_t al; int32_t go, ge;
+} brute_prob;
+
+enum { MV_START = 0, MV_RIGHT = 1, MV_DOWN = 2 };
+
+static int64_t brute_rec(const brute_prob* p, uint64_t i, uint64_t j, int last) {
+    if (i == p->m && j == p->n) return 0;
+    int64_t best = BRUTE_NEG_INF;
+    if (i < p->m && j < p->n) {
+        int64_t r = brute_rec(p, i + 1, j + 1, MV_START);
+        int64_t v = (int64_t)p->mat[(uint64_t)p->q[i] * p->al + p->t[j]] + r;
+        if (v > best) best = v;
+    }
+    if (j < p->n) {
+        int64_t cost = last == MV_RIGHT ? p->ge : p->go;
+        int64_t v = -cost + brute_rec(p, i, j + 1, MV_RIGHT);
+        if (v > best) best = v;
+    }
+    if (i < p->m) {
+        int64_t cost = last == MV_DOWN ? p->ge : p->go;
+        int64_t v = -cost + brute_rec(p, i + 1, j, MV_DOWN);
+        if (v > best) best = v;
+    }
+    return best;
+}
+
+/* Global score of q[a:b] against t[c:d]. */
+static int64_t brute_nw_window(const uint8_t* q, uint64_t a, uint64_t b, const uint8_t* t,
+                               uint64_t c, uint64_t d, uint32_t al, const int32_t* mat,
+                               int32_t go, int32_t ge) {
+    brute_prob p = {q + a, b - a, t + c, d - c, mat, al, go, ge};
+    return brute_rec(&p, 0, 0, MV_START);
+}
+
+int64_t hyo_brute_score(uint32_t mode, const uint8_t* q, uint64_t m, const uint8_t* t,
+                        uint64_t n, uint32_t al, const int32_t* mat, int32_t go, int32_t ge) {
+    int64_t best;
+    switch (mode) {
+    case HYO_MODE_NW:
+        return brute_nw_window(q, 0, m, t, 0, n, al, mat, go, ge);
+    case HYO_MODE_SW:
+    case HYO_MODE_OV:
+        best = 0;
+        for (uint64_t a = 0; a <= m; ++a)
+            for (uint64_t b = a; b <= m; ++b)
+                for (uint64_t c = 0; c <= n; ++c)
+                    for (uint64_t d = c; d <= n; ++d) {
+                        if (mode == HYO_MODE_OV && !((a == 0 || c == 0) && (b == m || d == n)))
+                            continue;
+                        int64_t v = brute_nw_window(q, a, b, t, c, d, al, mat, go, ge);
+                        if (v > best) best = v;
+                    }
+        return best;
+    case HYO_MODE_HW:
+        best = BRUTE_NEG_INF;
+        for (uint64_t c = 0; c <= n; ++c)
+            for (uint64_t d = c; d <= n; ++d) {
+                int64_t v = brute_nw_window(q, 0, m, t, c, d, al, mat, go, ge);
+                if (v > best) best = v;
+            }
+        return best;
+    case HYO_MODE_SHW:
+        best = BRUTE_NEG_INF;
+        for (uint64_t a = 0; a <= m; ++a)
+            for (uint64_t b = a; b <= m; ++b) {
+                int64_t v = brute_nw_window(q, a, b, t, 0, n, al, mat, go, ge);
+                if (v > best) best = v;
+            }
+        return best;
+    default:
+        return BRUTE_NEG_INF;
+    }
+}
+
+/* Brute-force value of the DP cell (i, j): the best global score of q[a:i] against t[c:j]
+ * over the start windows each mode's free borders allow (SW additionally floors at 0). This
+ * extends the reference's whole-pair enumerators (tests/common/mod.rs:98-171) to every end cell
+ * so that end positions can be pinned by enumeration too. */
+static int64_t brute_cell(uint32_t mode, const uint8_t* q, uint64_t i, const uint8_t* t,
+                          uint64_t j, uint32_t al, const int32_t* mat, int32_t go, int32_t ge) {
+    hyo_flags fl = flags_for_mode(mode);
+    int64_t best = fl.local ? 0 : BRUTE_NEG_INF;
+    for (uint64_t a = 0; a <= i; ++a)
+        for (uint64_t c = 0; c <= j; ++c) {
+            int ok;
+            if (fl.local) ok = 1;
+            else if (fl.top_row_free && fl.left_col_free) ok = (a == 0 || c == 0);
+            else if (fl.top_row_free) ok = (a == 0);
+            else if (fl.left_col_free) ok = (c == 0);
+            else ok = (a == 0 && c == 0);
+            if (!ok) continue;
+            int64_t v = brute_nw_window(q, a, i, t, c, j, al, mat, go, ge);
+            if (v > best) best = v;
+        }
+    return best;
+}
+
+/* Score and ends by enumeration: brute-force cell values reduced over each mode's answer
+ * region (src/kernel.rs:13-19) with the (score desc, col asc, row asc) tie-break. */
+void hyo_brute_end(uint32_t mode, const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n,
+                   uint32_t al, const int32_t* mat, int32_t go, int32_t ge, int32_t* score,
+                   int64_t* qend, int64_t* tend) {
+    hyo_flags fl = flags_for_mode(mode);
+    hyo_best b = {BRUTE_NEG_INF, 0, 0};
+    for (uint64_t i = 0; i <= m; ++i)
+        for (uint64_t j = 0; j <= n; ++j) {
+            int in;
+            if (fl.local) in = 1;
+            else in = (i == m && j == n) || (fl.answer_last_row && i == m) ||
+                      (fl.answer_last_col && j == n);
+            if (!in) continue;
+            consider(&b, brute_cell(mode, q, i, t, j, al, mat, go, ge), i, j);
+        }
+    *score = (int32_t)b.score;
+    *qend = b.row == 0 ? -1 : (int64_t)b.row - 1;
+    *tend = b.col == 0 ? -1 : (int64_t)b.col - 1;
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* Database level (scalar backend of src/database.rs).                                        */
+/* The database is flat residues plus n+1 offsets.                                            */
+/* ---------------------------------------------------------------------------------------- */
+
+typedef struct {
+    const uint8_t* flat; const uint64_t* off; uint64_t lo, hi;
+    const uint8_t* q; uint64_t m; uint32_t al; const int32_t* mat; int32_t go, ge;
+    uint32_t mode; int32_t* scores; int64_t* qends; int64_t* tends; int rc;
+} scan_job;
+
+static void* scan_worker(void* arg) {
+    scan_job* jb = (scan_job*)arg;
+    uint64_t maxn = 0;
+    for (uint64_t k = jb->lo; k < jb->hi; ++k) {
+        uint64_t len = jb->off[k + 1] - jb->off[k];
+        if (len > maxn) maxn = len;
+    }
+    int32_t* h = (int32_t*)malloc(sizeof(int32_t) * (jb->m + 1) * (maxn + 1));
+    int32_t* f = (int32_t*)malloc(sizeof(int32_t) * (maxn + 1));
+    if (!h || !f) { free(h); free(f); jb->rc = HYO_E_NOMEM; return NULL; }
+    for (uint64_t k = jb->lo; k < jb->hi; ++k) {
+        int32_t s; int64_t qe, te;
+        align_core_buf(jb->q, jb->m, jb->flat + jb->off[k], jb->off[k + 1] - jb->off[k], jb->al,
+                       jb->mat, jb->go, jb->ge, jb->mode, h, f, &s, &qe, &te);
+        jb->scores[k] = s;
+        if (jb->qends) jb->qends[k] = qe;
+        if (jb->tends) jb->tends[k] = te;
+    }
+    free(h);
+    free(f);
+    jb->rc = HYO_OK;
+    return NULL;
+}
+
+/* Per-target (score, query_end, target_end) in database order: the scalar arm of
+ * Database::scan_all (src/database.rs:405-414) and, with qends/tends NULL, of
+ * Database::scan_scores (src/database.rs:511-513). Split over nthreads contiguous shards. */
+int hyo_scan_all(const uint8_t* flat, const uint64_t* off, uint64_t n, const uint8_t* q,
+                 uint64_t m, uint32_t al, const int32_t* mat, int32_t go, int32_t ge,
+                 uint32_t mode, int32_t* scores, int64_t* qends, int64_t* tends, int nthreads) {
+    if (nthreads < 1) nthreads = 1;
+    if ((uint64_t)nthreads > n) nthreads = (int)(n ? n : 1);
+    scan_job* jobs = (scan_job*)calloc((size_t)nthreads, sizeof(scan_job));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    if (!jobs || !th) { free(jobs); free(th); return HYO_E_NOMEM; }
+    /* Balance shards by residue count. */
+    uint64_t total = off[n] - off[0], k = 0;
+    for (int w = 0; w < nthreads; ++w) {
+        uint64_t target = off[0] + (total * (uint64_t)(w + 1)) / (uint64_t)nthreads;
+        uint64_t lo = k;
+        if (w == nthreads - 1) k = n;
+        else while (k < n && off[k + 1] <= target) ++k;
+        scan_job j = {flat, off, lo, k, q, m, al, mat, go, ge, mode, scores, qends, tends, 0};
+        jobs[w] = j;
+    }
+    for (int w = 0; w < nthreads; ++w) pthread_create(&th[w], NULL, scan_worker, &jobs[w]);
+    int rc = HYO_OK;
+    for (int w = 0; w < nthreads; ++w) {
+        pthread_join(th[w], NULL);
+        if (jobs[w].rc != HYO_OK) rc = jobs[w].rc;
+    }
+    free(jobs);
+    free(th);
+    return rc;
+}
+
+/* Database::scan on the scalar backend (src/database.rs:115-144): per-target align_core, then
+ * the scalar argmax keeping the smallest index on a tie (strictly-greater replaces); ends are
+ * reported only when the search type tracks them. */
+int hyo_scan(const uint8_t* flat, const uint64_t* off, uint64_t n, const uint8_t* q, uint64_t m,
+             uint32_t al, const int32_t* mat, int32_t go, int32_t ge, uint32_t mode,
+             uint32_t tracks_end, int32_t* score, uint64_t* db_index, int64_t* qend,
+             int64_t* tend, int nthreads) {
+    if (n == 0) return HYO_E_EMPTY_DATABASE;
+    int32_t* s = (int32_t*)malloc(sizeof(int32_t) * n);
+    int64_t* qe = (int64_t*)malloc(sizeof(int64_t) * n);
+    int64_t* te = (int64_t*)malloc(sizeof(int64_t) * n);
+    if (!s || !qe || !te) { free(s); free(qe); free(te); return HYO_E_NOMEM; }
+    int rc = hyo_scan_all(flat, off, n, q, m, al, mat, go, ge, mode, s, qe, te, nthreads);
+    if (rc == HYO_OK) {
+        int64_t best = INT64_MIN; uint64_t bi = 0;
+        for (uint64_t k = 0; k < n; ++k)
+            if ((int64_t)s[k] > best) { best = s[k]; bi = k; }
+        *score = (int32_t)best;
+        *db_index = bi;
+        *qend = tracks_end ? qe[bi] : -1;
+        *tend = tracks_end ? te[bi] : -1;
+    }
+    free(s); free(qe); free(te);
+    return rc;
+}

@@ -1,0 +1,49 @@
+/* CPU oracle for the database scan — TEST INFRASTRUCTURE ONLY (see oracle.c). */
+#ifndef HYALITE_ORACLE_H
+#define HYALITE_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Mode codes (src/mode.rs:11-27); the same numbering as the product C ABI. */
+enum { HYO_MODE_SW = 0, HYO_MODE_NW = 1, HYO_MODE_HW = 2, HYO_MODE_OV = 3, HYO_MODE_SHW = 4 };
+
+/* Status codes; the same numbering as the product C ABI's construction errors. */
+enum {
+    HYO_OK = 0,
+    HYO_E_EMPTY_ALPHABET = 1,
+    HYO_E_MATRIX_SHAPE = 2,
+    HYO_E_NEGATIVE_GAP = 3,
+    HYO_E_GAP_OPEN_LT_EXT = 4,
+    HYO_E_SCORE_RANGE = 5,
+    HYO_E_EMPTY_DATABASE = 8,
+    HYO_E_NOMEM = 100
+};
+
+int32_t hyo_gap_penalty(int32_t go, int32_t ge, uint64_t len);
+int hyo_scoring_check(uint32_t al, const int32_t* matrix, uint64_t matrix_len, int32_t go,
+                      int32_t ge, int32_t* min_e, int32_t* max_e);
+int hyo_required_width(uint32_t mode, int32_t min_e, int32_t max_e, int32_t go, int32_t ge,
+                       uint64_t mql, uint64_t mtl, int64_t* bound_out, int32_t* width_bits);
+int hyo_align_core(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
+                   const int32_t* matrix, int32_t go, int32_t ge, uint32_t mode, int32_t* score,
+                   int64_t* qend, int64_t* tend);
+int64_t hyo_brute_score(uint32_t mode, const uint8_t* q, uint64_t m, const uint8_t* t,
+                        uint64_t n, uint32_t al, const int32_t* mat, int32_t go, int32_t ge);
+void hyo_brute_end(uint32_t mode, const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n,
+                   uint32_t al, const int32_t* mat, int32_t go, int32_t ge, int32_t* score,
+                   int64_t* qend, int64_t* tend);
+int hyo_scan_all(const uint8_t* flat, const uint64_t* off, uint64_t n, const uint8_t* q,
+                 uint64_t m, uint32_t al, const int32_t* mat, int32_t go, int32_t ge,
+                 uint32_t mode, int32_t* scores, int64_t* qends, int64_t* tends, int nthreads);
+int hyo_scan(const uint8_t* flat, const uint64_t* off, uint64_t n, const uint8_t* q, uint64_t m,
+             uint32_t al, const int32_t* mat, int32_t go, int32_t ge, uint32_t mode,
+             uint32_t tracks_end, int32_t* score, uint64_t* db_index, int64_t* qend,
+             int64_t* tend, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,268 @@
+"""GPU parity: every scan through the C ABI must equal the CPU oracle bit for bit.
+
+Mirrors the reference's differential tests (tests/database.rs, tests/sweeps.rs,
+tests/properties.rs, tests/opal_regressions.rs, tests/real_data.rs): the Hip backend is forced and
+its (score, db_index, query_end, target_end) compared with the oracle's scalar scan, per target
+(scan_all, scan_scores) and single best (scan), on the same seeded inputs.
+"""
+import threading
+
+import numpy as np
+import pytest
+
+import hyalite_amd as hy
+import oracle as orc
+from common import (ALL_MODES, BLOSUM62, MODE_NW, MODE_OV, MODE_SW, all_sequences,
+                    cr4_adapters, cr4_scheme, dna_scheme, identity_matrix, lcg_next, lcg_seq,
+                    phix)
+
+pytestmark = pytest.mark.gpu
+
+ST = (hy.SearchType.Score, hy.SearchType.ScoreEnd)
+
+
+def build(seqs, scheme, mode, st, mql):
+    al, mat, go, ge = scheme
+    return (hy.Database.builder().sequences(seqs).scoring(hy.Scoring(al, mat, go, ge))
+            .mode(hy.Mode(mode)).search_type(st).max_query_len(mql)
+            .backend(hy.BackendChoice.force(hy.Backend.Hip)).build())
+
+
+def check(seqs, scheme, mode, st, queries, mql=None, all_targets=True):
+    al, mat, go, ge = scheme
+    mql = max([len(q) for q in queries] + [1]) if mql is None else mql
+    db = build(seqs, scheme, mode, st, mql)
+    sc = hy.Scratch(db)
+    flat, off = orc.flatten(seqs)
+    tracks = st == hy.SearchType.ScoreEnd
+    for q in queries:
+        want = orc.scan_flat(flat, off, q, al, mat, go, ge, mode, tracks, threads=4)
+        got = db.scan(sc, q)
+        assert (got.score, got.db_index, got.query_end, got.target_end) == want, \
+            (mode, st, list(q)[:20], len(seqs))
+        if not all_targets:
+            continue
+        ws, wqe, wte = orc.scan_all_flat(flat, off, q, al, mat, go, ge, mode, threads=4)
+        s, qe, te = db.scan_all_arrays(sc, q)
+        assert np.array_equal(s, ws), (mode, st)
+        if tracks:
+            assert np.array_equal(qe, wqe) and np.array_equal(te, wte), (mode, st)
+        else:
+            assert (qe == -1).all() and (te == -1).all()
+        assert np.array_equal(db.scan_scores_array(sc, q), ws)
+
+
+def test_backend_is_available():
+    assert hy.Backend.Hip.is_available()
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_all_short_sequences_against_all_short_queries(mode):
+    """tests/database.rs:67-100 at database scale: all 21 sequences of length <=2 (plus a
+    reversed copy so equal scores tie across indices) against all 85 queries of length <=3."""
+    seqs = all_sequences(4, 2)
+    seqs = seqs + seqs[::-1]
+    for st in ST:
+        check(seqs, dna_scheme(), mode, st, all_sequences(4, 3), mql=3)
+
+
+def test_two_sequence_databases_tie_break_both_orders():
+    rng = np.random.default_rng(5)
+    seqs = all_sequences(4, 2)
+    queries = all_sequences(4, 3)
+    for _ in range(12):
+        a, b = rng.integers(0, len(seqs), 2)
+        for mode in ALL_MODES:
+            check([seqs[a], seqs[b]], dna_scheme(), mode, hy.SearchType.ScoreEnd,
+                  queries[::7], mql=3)
+
+
+def test_database_sizes_around_lane_boundaries():
+    """tests/sweeps.rs:130-158 with the wavefront boundaries added."""
+    state = [0x5EED1234]
+    for n in (1, 2, 15, 16, 17, 31, 32, 33, 63, 64, 65, 127, 128, 129, 191, 192, 193, 257):
+        seqs = [lcg_seq(state, 3 + lcg_next(state) % 8, 4) for _ in range(n)]
+        queries = [lcg_seq(state, 4 + lcg_next(state) % 6, 4) for _ in range(3)]
+        for mode in ALL_MODES:
+            for st in ST:
+                check(seqs, dna_scheme(), mode, st, queries)
+
+
+def test_degenerate_sequences_and_queries():
+    """tests/sweeps.rs:160-194."""
+    cases = [
+        [[]] * 20,
+        [[] if i % 2 == 0 else [0, 1, 2] for i in range(33)],
+        [[0] * 10] * 40,
+        [[i % 4] for i in range(17)],
+        [[0, 1, 2, 3]] * 32 + [[0] * 20],
+        [[]] * 70 + [[1, 2, 3]],
+    ]
+    queries = [[], [0], [0, 1, 2, 3], [2, 2, 2, 2, 2]]
+    for seqs in cases:
+        for mode in ALL_MODES:
+            for st in ST:
+                check(seqs, dna_scheme(), mode, st, queries, mql=5)
+
+
+def test_scoring_parameter_grid():
+    """tests/sweeps.rs:196-236, including free gaps and gap_open at the i8 edge."""
+    state = [0xA11CE]
+    seqs = [lcg_seq(state, lcg_next(state) % 9, 4) for _ in range(20)] + [[]]
+    queries = [lcg_seq(state, lcg_next(state) % 9, 4) for _ in range(5)]
+    for m, x in ((1, -1), (2, -1), (3, -2), (1, 0)):
+        for go, ge in ((1, 1), (2, 1), (3, 0), (5, 5), (4, 2), (0, 0), (100, 1), (127, 1)):
+            scheme = (4, identity_matrix(4, m, x), go, ge)
+            for mode in ALL_MODES:
+                check(seqs, scheme, mode, hy.SearchType.ScoreEnd, queries, mql=9)
+
+
+@pytest.mark.parametrize("al", [16, 17, 24, 64, 255, 256])
+def test_alphabet_sizes(al):
+    """tests/sweeps.rs:238-262 plus protein-sized, large and byte-exhausting alphabets (256 has
+    no free padding symbol and runs entirely in the systolic kernel)."""
+    mat = [-1] * (al * al)
+    for i in range(al):
+        mat[i * al + i] = 1 + (i % 3)
+    state = [0xF00D + al]
+    seqs = [lcg_seq(state, 4 + lcg_next(state) % 40, al) for _ in range(90)]
+    queries = [lcg_seq(state, 6 + lcg_next(state) % 30, al) for _ in range(3)]
+    for mode in ALL_MODES:
+        check(seqs, (al, mat, 2, 1), mode, hy.SearchType.ScoreEnd, queries)
+
+
+def test_score_width_boundaries():
+    """tests/sweeps.rs:264-285."""
+    scheme = (2, [1, -1, -1, 1], 2, 1)
+    for n in (126, 127, 128, 129):
+        check([[0] * n], scheme, MODE_SW, hy.SearchType.ScoreEnd, [[0] * n])
+
+
+def test_overlap_negative_reach_at_the_i8_edge():
+    """tests/sweeps.rs:287-316."""
+    scheme = (2, [1, -1, -1, 1], 1, 1)
+    check([[1] * 126], scheme, MODE_OV, hy.SearchType.ScoreEnd, [[0] * 126])
+    check([[1] * 6], scheme, MODE_OV, hy.SearchType.ScoreEnd, [[0] * 120 + [1] * 6])
+
+
+def test_opal_regressions_and_proptest_seeds():
+    """tests/opal_regressions.rs:68-82 and the proptest regression seeds."""
+    check([[0, 0, 1, 1, 2]], (3, identity_matrix(3, 2, -2), 1, 1), MODE_NW,
+          hy.SearchType.ScoreEnd, [[0, 0, 1, 1]])
+    for mode in ALL_MODES:
+        check([[]], (2, [0, 0, 0, 0], 0, 0), mode, hy.SearchType.ScoreEnd, [[0]])
+        check([[0, 0, 1, 0, 1, 1, 0, 1, 1, 0, 0, 1]], (2, [0, 0, 0, -5], 8, 0), mode,
+              hy.SearchType.ScoreEnd, [[1, 1, 0, 1, 1]])
+        check([[0] * 12], (2, [0, 0, 11, 0], 1, 0), mode, hy.SearchType.ScoreEnd, [[1]])
+
+
+def test_protein_scale_scoring():
+    """tests/opal_regressions.rs:140-216 (24-letter matrix, 13/2 gaps)."""
+    AL = 24
+    mat = [4 + (i % 3) if i == j else -1 - ((i + j) % 4) for i in range(AL) for j in range(AL)]
+    state = [0xB105]
+    seqs = [lcg_seq(state, 30 + lcg_next(state) % 40, AL) for _ in range(12)]
+    queries = [lcg_seq(state, 40 + lcg_next(state) % 30, AL) for _ in range(4)]
+    for mode in ALL_MODES:
+        check(seqs, (AL, mat, 13, 2), mode, hy.SearchType.ScoreEnd, queries)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_random_differential_multi_tile(mode):
+    """Random databases with lengths 0..700 and queries long enough to need several query tiles
+    (inter kernel) and several passes (systolic kernel, > 512 rows)."""
+    rng = np.random.default_rng(100 + mode)
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(0, 700, 400)]
+    queries = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in (0, 1, 31, 33, 600)]
+    check(seqs, (20, BLOSUM62, 11, 1), mode, hy.SearchType.ScoreEnd, queries, mql=700)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_long_targets_take_the_systolic_path(mode):
+    """A few targets far above the long-target threshold next to many short ones, plus
+    mutated copies of the query so high scores tie across indices."""
+    rng = np.random.default_rng(200 + mode)
+    q = rng.integers(0, 4, size=150, dtype=np.uint8)
+    seqs = [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in rng.integers(20, 80, 600)]
+    for k in (3, 77, 301):
+        seqs[k] = np.concatenate([rng.integers(0, 4, 2500, dtype=np.uint8), q,
+                                  rng.integers(0, 4, 900, dtype=np.uint8)])
+    seqs[450] = q.copy()
+    seqs[451] = q.copy()
+    check(seqs, dna_scheme(), mode, hy.SearchType.ScoreEnd, [q, q[:40]], mql=150)
+
+
+def test_real_phix_windows_and_cr4():
+    """tests/real_data.rs:63-103 and 145-201."""
+    genome = np.array(phix(), dtype=np.uint8)
+    windows = [genome[o:o + 60] for o in (0, 500, 1500, 2500, 4000)]
+    queries = [genome[500:560], genome[510:570], genome[3000:3060], genome[1500:1530]]
+    for mode in ALL_MODES:
+        for st in ST:
+            check(windows, dna_scheme(), mode, st, queries, mql=64)
+    seqs = [np.array(s, dtype=np.uint8) for _, s in cr4_adapters()]
+    db = build(seqs, cr4_scheme(), MODE_OV, hy.SearchType.ScoreEnd, 128)
+    sc = hy.Scratch(db)
+    tso_read = np.concatenate([seqs[0], genome[800:840]])
+    hit = db.scan(sc, tso_read)
+    assert (hit.db_index, hit.score) == (0, 30)
+    polya = np.concatenate([genome[1500:1530], np.zeros(25, dtype=np.uint8)])
+    hit = db.scan(sc, polya)
+    assert hit.db_index == 1 and hit.score >= 25
+
+
+def test_concurrent_scans_from_threads():
+    """tests/database.rs:136-176: one database, a Scratch per thread, identical results."""
+    rng = np.random.default_rng(9)
+    seqs = [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in rng.integers(0, 200, 2000)]
+    db = build(seqs, dna_scheme(), MODE_SW, hy.SearchType.ScoreEnd, 100)
+    q = rng.integers(0, 4, size=100, dtype=np.uint8)
+    expected = db.scan(hy.Scratch(db), q)
+    results, errors = [], []
+
+    def work():
+        try:
+            sc = hy.Scratch(db)
+            for _ in range(20):
+                results.append(db.scan(sc, q))
+        except Exception as exc:  # pragma: no cover - surfaced below
+            errors.append(exc)
+
+    threads = [threading.Thread(target=work) for _ in range(4)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors
+    assert len(results) == 80 and all(r == expected for r in results)
+
+
+def test_scratch_reuse_across_query_sizes():
+    """src/database.rs scratch_reuse_across_many_scans_is_consistent."""
+    db = build([[0, 1], [0, 1, 2, 3, 3, 2, 1, 0], [2]], dna_scheme(), MODE_NW,
+               hy.SearchType.ScoreEnd, 8)
+    sc = hy.Scratch(db)
+    qs = [[0, 1, 2, 3], [2], [0, 1, 2, 3, 3, 2, 1, 0], [0, 1, 2, 3]]
+    hits = [db.scan(sc, q) for q in qs]
+    assert hits[0] == hits[3]
+    for q, h in zip(qs, hits):
+        assert db.scan(hy.Scratch(db), q) == h
+
+
+def test_scan_contract_violations_are_errors():
+    db = build([[0, 1, 2]], dna_scheme(), MODE_SW, hy.SearchType.ScoreEnd, 4)
+    sc = hy.Scratch(db)
+    with pytest.raises(hy.ScanContractViolation):
+        db.scan(sc, [0, 1, 2, 3, 0])
+    with pytest.raises(hy.ScanContractViolation):
+        db.scan(sc, [0, 9])
+
+
+def test_batch_equals_individual_scans():
+    rng = np.random.default_rng(21)
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(10, 300, 3000)]
+    db = build(seqs, (20, BLOSUM62, 11, 1), MODE_SW, hy.SearchType.ScoreEnd, 400)
+    sc = hy.Scratch(db)
+    qs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in (5, 50, 399, 0, 120)]
+    batch = db.scan_batch(sc, qs)
+    assert batch == [db.scan(sc, q) for q in qs]
