@@ -336,6 +336,11 @@ def load_library():
     lib.hy_scan_batch.restype = ctypes.c_int
     lib.hy_scan_batch.argtypes = [vp, vp, ctypes.POINTER(_u8p), _sizep, ctypes.c_size_t, _i32p,
                                   _u64p, _i64p, _i64p]
+    lib.hy_scratch_profile.restype = ctypes.c_int
+    lib.hy_scratch_profile.argtypes = [vp, ctypes.c_int]
+    lib.hy_scratch_profile_read.restype = ctypes.c_int
+    lib.hy_scratch_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(ctypes.c_double), _u64p, ctypes.c_int]
     lib.hy_strerror.restype = ctypes.c_char_p
     lib.hy_strerror.argtypes = [ctypes.c_int]
     _lib = lib
@@ -398,6 +403,14 @@ class DatabaseBuilder:
 
     def sequences(self, seqs) -> "DatabaseBuilder":
         self._seqs = [np.ascontiguousarray(np.asarray(s, dtype=np.uint8)) for s in seqs]
+        self._flat = None
+        return self
+
+    def sequences_flat(self, flat: np.ndarray, offsets: np.ndarray) -> "DatabaseBuilder":
+        """Sequences given as one residue buffer plus n+1 offsets (no per-sequence objects)."""
+        self._flat = (np.ascontiguousarray(flat, dtype=np.uint8),
+                      np.ascontiguousarray(offsets, dtype=np.uint64))
+        self._seqs = []
         return self
 
     def scoring(self, scoring: Scoring) -> "DatabaseBuilder":
@@ -431,16 +444,31 @@ class DatabaseBuilder:
             have |= 4
         if self._mql is not None:
             have |= 8
-        seqs = self._seqs or []
-        n = len(seqs)
-        ptrs = (_u8p * max(n, 1))()
-        lens = (ctypes.c_size_t * max(n, 1))()
+        flat = getattr(self, "_flat", None)
         keep = []
-        for k, s in enumerate(seqs):
-            a = s if s.size else np.zeros(1, dtype=np.uint8)
-            keep.append(a)
-            ptrs[k] = a.ctypes.data_as(_u8p)
-            lens[k] = int(s.size)
+        if flat is not None:
+            buf, off = flat
+            if buf.size == 0:
+                buf = np.zeros(1, dtype=np.uint8)
+            keep.append(buf)
+            n = len(off) - 1
+            ptr_arr = (np.uint64(buf.ctypes.data) + off[:-1]).astype(np.uint64)
+            len_arr = np.diff(off).astype(np.uint64)
+            keep += [ptr_arr, len_arr]
+            ptrs = ptr_arr.ctypes.data_as(ctypes.POINTER(_u8p))
+            lens = len_arr.ctypes.data_as(_sizep)
+            seq_lens = len_arr
+        else:
+            seqs = self._seqs or []
+            n = len(seqs)
+            ptrs = (_u8p * max(n, 1))()
+            lens = (ctypes.c_size_t * max(n, 1))()
+            for k, s in enumerate(seqs):
+                a = s if s.size else np.zeros(1, dtype=np.uint8)
+                keep.append(a)
+                ptrs[k] = a.ctypes.data_as(_u8p)
+                lens[k] = int(s.size)
+            seq_lens = [int(s.size) for s in seqs]
         sc = self._scoring
         mat = sc.matrix() if sc is not None else np.zeros(1, dtype=np.int32)
         backend = 0
@@ -459,7 +487,7 @@ class DatabaseBuilder:
         if rc != 0:
             _raise_build_error(rc, detail.value, sc.alphabet_len() if sc else 0,
                                os.environ.get(BACKEND_ENV_VAR))
-        return Database(out, sc, self._mode, self._st, [int(s.size) for s in seqs])
+        return Database(out, sc, self._mode, self._st, seq_lens)
 
 
 class Database:
@@ -612,6 +640,19 @@ class Scratch:
             raise DeviceError(load_library().hy_strerror(rc).decode())
         self._h = out
         self._db = db  # keeps the database alive while the scratch exists
+
+    def profile(self, enable: bool = True) -> None:
+        """Time the per-target kernels of every scan with HIP events on this scratch's stream."""
+        rc = load_library().hy_scratch_profile(self._h, 1 if enable else 0)
+        if rc != 0:
+            raise DeviceError(load_library().hy_strerror(rc).decode())
+
+    def profile_read(self, reset: bool = False) -> tuple[float, float, int]:
+        """(inter-kernel ms, long-target-kernel ms, scans timed), accumulated."""
+        a, b, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_uint64(0)
+        load_library().hy_scratch_profile_read(self._h, ctypes.byref(a), ctypes.byref(b),
+                                               ctypes.byref(n), 1 if reset else 0)
+        return a.value, b.value, n.value
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -142,6 +142,12 @@ struct hy_scratch {
     unsigned long long* d_key = nullptr;
     Result* d_result = nullptr;
     Result* h_result = nullptr;          // pinned
+    // Instrumentation: events around the per-target kernels of the last scan.
+    bool profile = false;
+    bool pending = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double inter_ms = 0.0, tail_ms = 0.0;
+    uint64_t launches = 0;
 
     ~hy_scratch() {
         (void)hipSetDevice(device);
@@ -151,6 +157,8 @@ struct hy_scratch {
         dev_free(d_key); dev_free(d_result);
         if (h_query) (void)hipHostFree(h_query);
         if (h_result) (void)hipHostFree(h_result);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -529,6 +537,7 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     ia.out_qend = s->d_qend;
     ia.out_tend = s->d_tend;
     ia.key = ends ? nullptr : key;
+    if (s->profile && hipEventRecord(s->ev[0], s->stream) != hipSuccess) return HY_E_DEVICE;
     if (hy::launch_inter32((int)db->mode, ends, ia, (uint32_t)db->groups.size(), s->stream) !=
         hipSuccess)
         return HY_E_DEVICE;
@@ -537,10 +546,28 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     sa.out_qend = s->d_qend;
     sa.out_tend = s->d_tend;
     sa.key = ia.key;
+    if (s->profile && hipEventRecord(s->ev[1], s->stream) != hipSuccess) return HY_E_DEVICE;
     if (hy::launch_sys32((int)db->mode, ends, false, sa, (uint32_t)db->sys_list.size(),
                          s->stream) != hipSuccess)
         return HY_E_DEVICE;
+    if (s->profile) {
+        if (hipEventRecord(s->ev[2], s->stream) != hipSuccess) return HY_E_DEVICE;
+        s->pending = true;
+    }
     return HY_OK;
+}
+
+// After the scan's stream synchronisation: fold the recorded kernel times into the counters.
+void collect_profile(hy_scratch* s) {
+    if (!s->profile || !s->pending) return;
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, s->ev[0], s->ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&b, s->ev[1], s->ev[2]) == hipSuccess) {
+        s->inter_ms += a;
+        s->tail_ms += b;
+        s->launches += 1;
+    }
+    s->pending = false;
 }
 
 }  // namespace
@@ -573,6 +600,7 @@ extern "C" int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, siz
                        s->stream) != hipSuccess)
         return HY_E_DEVICE;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    collect_profile(s);
     const Result r = *s->h_result;
     if (r.check != 0) return HY_E_INTERNAL;
     *score = r.score;
@@ -604,6 +632,7 @@ extern "C" int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query,
             return HY_E_DEVICE;
     }
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    collect_profile(s);
     if (!ends) {
         for (size_t k = 0; k < n; ++k) {
             if (query_ends) query_ends[k] = -1;
@@ -625,6 +654,7 @@ extern "C" int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* que
                        hipMemcpyDeviceToHost, s->stream) != hipSuccess)
         return HY_E_DEVICE;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    collect_profile(s);
     return HY_OK;
 }
 
@@ -642,6 +672,31 @@ extern "C" int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* cons
         const int rc = hy_scan(db, s, queries[k], qlens[k], &scores[k], &db_indices[k],
                                &query_ends[k], &target_ends[k]);
         if (rc != HY_OK) return rc;
+    }
+    return HY_OK;
+}
+
+extern "C" int hy_scratch_profile(hy_scratch* s, int enable) {
+    if (!s) return HY_E_NULL_ARGUMENT;
+    if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+    if (enable) {
+        for (auto& e : s->ev)
+            if (!e && hipEventCreate(&e) != hipSuccess) return HY_E_DEVICE;
+    }
+    s->profile = enable != 0;
+    s->pending = false;
+    return HY_OK;
+}
+
+extern "C" int hy_scratch_profile_read(hy_scratch* s, double* inter_ms, double* tail_ms,
+                                       uint64_t* launches, int reset) {
+    if (!s) return HY_E_NULL_ARGUMENT;
+    if (inter_ms) *inter_ms = s->inter_ms;
+    if (tail_ms) *tail_ms = s->tail_ms;
+    if (launches) *launches = s->launches;
+    if (reset) {
+        s->inter_ms = s->tail_ms = 0.0;
+        s->launches = 0;
     }
     return HY_OK;
 }

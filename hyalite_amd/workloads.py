@@ -1,0 +1,164 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+Each generator returns a ``Workload`` with the scoring scheme, mode, search type, the database as
+flat residues plus offsets, and the queries. Choices the configurations leave open are fixed here
+and printed by the benchmark: RNG = numpy PCG64 seeded per configuration; protein background
+frequencies = Robinson & Robinson (1991) for the 20 amino acids plus four rare symbols at 0.1%
+each; log-normal target lengths use sigma = 0.75.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# Robinson & Robinson (1991) amino-acid background frequencies, order ARNDCQEGHILKMFPSTWYV.
+AA_FREQ = np.array([7.805, 5.129, 4.487, 5.364, 1.925, 4.264, 6.295, 7.377, 2.199, 5.142,
+                    9.019, 5.744, 2.243, 3.856, 5.203, 7.120, 5.841, 1.330, 3.216, 6.441])
+
+LOGNORMAL_SIGMA = 0.75
+
+
+@dataclass
+class Workload:
+    name: str
+    al: int
+    matrix: np.ndarray
+    go: int
+    ge: int
+    mode: int
+    search_type: int
+    mql: int
+    flat: np.ndarray
+    off: np.ndarray
+    queries: list = field(default_factory=list)
+    note: str = ""
+
+    @property
+    def n(self) -> int:
+        return len(self.off) - 1
+
+    @property
+    def residues(self) -> int:
+        return int(self.off[-1])
+
+    def lens(self) -> np.ndarray:
+        return np.diff(self.off).astype(np.int64)
+
+    def subsample(self, k: int, seed: int = 0) -> "Workload":
+        """k targets drawn without replacement (for the bounded CPU baseline)."""
+        rng = np.random.default_rng(seed)
+        idx = np.sort(rng.choice(self.n, size=min(k, self.n), replace=False))
+        lens = self.lens()[idx]
+        off = np.zeros(len(idx) + 1, dtype=np.uint64)
+        np.cumsum(lens, out=off[1:])
+        flat = np.concatenate([self.flat[int(self.off[i]):int(self.off[i + 1])] for i in idx]) \
+            if len(idx) else np.zeros(0, dtype=np.uint8)
+        return Workload(self.name + f"[sample {len(idx)}]", self.al, self.matrix, self.go,
+                        self.ge, self.mode, self.search_type, self.mql, flat.astype(np.uint8),
+                        off, self.queries, self.note)
+
+
+def dna_matrix() -> np.ndarray:
+    m = np.full((4, 4), -1, dtype=np.int32)
+    np.fill_diagonal(m, 2)
+    return m.reshape(-1)
+
+
+def protein_freqs() -> np.ndarray:
+    p = np.concatenate([AA_FREQ / AA_FREQ.sum() * 0.996, np.full(4, 0.001)])
+    return p / p.sum()
+
+
+def protein_matrix(seed: int) -> np.ndarray:
+    """Seeded symmetric 24x24 matrix: diagonal U{4..11}, off-diagonal U{-4..3}."""
+    rng = np.random.default_rng(seed)
+    m = rng.integers(-4, 4, size=(24, 24))
+    m = np.triu(m, 1)
+    m = m + m.T
+    np.fill_diagonal(m, rng.integers(4, 12, size=24))
+    return m.astype(np.int32).reshape(-1)
+
+
+def _draw(rng, n, p=None, al=4) -> np.ndarray:
+    if p is None:
+        return rng.integers(0, al, size=n, dtype=np.uint8)
+    cdf = np.cumsum(p)
+    cdf[-1] = 1.0
+    u = rng.random(n, dtype=np.float32) if n > 0 else np.zeros(0, dtype=np.float32)
+    return np.searchsorted(cdf.astype(np.float32), u, side="right").astype(np.uint8).clip(0, len(p) - 1)
+
+
+def _flat_from_lens(rng, lens, p=None, al=4):
+    off = np.zeros(len(lens) + 1, dtype=np.uint64)
+    np.cumsum(lens, out=off[1:])
+    return _draw(rng, int(off[-1]), p, al), off
+
+
+def lognormal_lens(rng, n, median, lo, hi, sigma=LOGNORMAL_SIGMA) -> np.ndarray:
+    x = rng.lognormal(np.log(median), sigma, size=n)
+    return np.clip(np.rint(x), lo, hi).astype(np.int64)
+
+
+def c1_dna(seed: int = 1) -> Workload:
+    """Plain DNA SW ScoreEnd: 100-nt query vs 10,000 targets U{50..300}, +2/-1, 2/1."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(50, 301, size=10_000)
+    flat, off = _flat_from_lens(rng, lens)
+    q = _draw(rng, 100)
+    return Workload("dna_sw_scoreend_100x10k", 4, dna_matrix(), 2, 1, 0, 1, 100, flat, off, [q])
+
+
+def c2_protein(seed: int = 2, n_targets: int = 1_000_000, mql: int = 360) -> Workload:
+    """Protein SW ScoreEnd: 360-aa query vs 1M log-normal(median 300) targets in [30, 5000]."""
+    rng = np.random.default_rng(seed)
+    p = protein_freqs()
+    lens = lognormal_lens(rng, n_targets, 300, 30, 5000)
+    flat, off = _flat_from_lens(rng, lens, p)
+    q = _draw(rng, 360, p)
+    return Workload(f"protein_sw_scoreend_360x{n_targets}", 24, protein_matrix(seed), 11, 1, 0,
+                    1, mql, flat, off, [q],
+                    note=f"lognormal sigma {LOGNORMAL_SIGMA}, Robinson-Robinson background")
+
+
+def c3_dna_modes(seed: int = 3, n_targets: int = 2_000_000, mode: int = 1) -> Workload:
+    """DNA short reads: 150-nt query vs targets U{100..1000}; 5% are mutated query copies."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(100, 1001, size=n_targets)
+    q = _draw(rng, 150)
+    n_mut = n_targets // 20
+    mut_idx = np.sort(rng.choice(n_targets, size=n_mut, replace=False))
+    muts = []
+    for _ in range(n_mut):
+        s = q.copy()
+        k = rng.random(150) < 0.05
+        s[k] = (s[k] + rng.integers(1, 4, size=int(k.sum()))) % 4
+        nd = int(rng.integers(1, 4))
+        pos = int(rng.integers(0, 150))
+        if rng.random() < 0.5:
+            s = np.concatenate([s[:pos], s[pos + nd:]])
+        else:
+            s = np.concatenate([s[:pos], _draw(rng, nd), s[pos:]])
+        muts.append(s.astype(np.uint8))
+    lens = lens.copy()
+    for k, i in enumerate(mut_idx):
+        lens[i] = len(muts[k])
+    flat, off = _flat_from_lens(rng, lens)
+    for k, i in enumerate(mut_idx):
+        flat[int(off[i]):int(off[i + 1])] = muts[k]
+    return Workload(f"dna_mode{mode}_scores_150x{n_targets}", 4, dna_matrix(), 2, 1, mode, 0,
+                    150, flat, off, [q])
+
+
+def c4_batch_queries(seed: int = 4, nq: int = 256) -> list:
+    rng = np.random.default_rng(seed)
+    p = protein_freqs()
+    lens = np.rint(np.exp(rng.uniform(np.log(20), np.log(2000), size=nq))).astype(int)
+    return [_draw(rng, int(L), p) for L in lens]
+
+
+def c5_serving_queries(seed: int = 5, nq: int = 64) -> list:
+    rng = np.random.default_rng(seed)
+    p = protein_freqs()
+    lens = lognormal_lens(rng, nq, 350, 20, 3000)
+    return [_draw(rng, int(L), p) for L in lens]

@@ -136,6 +136,15 @@ int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
                   const size_t* qlens, size_t nq, int32_t* scores, uint64_t* db_indices,
                   int64_t* query_ends, int64_t* target_ends);
 
+/* Instrumentation (no counterpart in the reference; for benchmarks and profilers). When enabled,
+ * every scan records HIP events on the scratch's own stream around the per-target kernel
+ * launches, and hy_scratch_profile_read returns the accumulated device time: `inter` is the
+ * inter-sequence kernel (the dominant one), `tail` the systolic kernel over long targets, and
+ * `launches` the number of scans timed. reset != 0 clears the counters after reading. */
+int hy_scratch_profile(hy_scratch* s, int enable);
+int hy_scratch_profile_read(hy_scratch* s, double* inter_ms, double* tail_ms, uint64_t* launches,
+                            int reset);
+
 /* Human-readable message for a status code (error.rs:98-157 wording where one exists). */
 const char* hy_strerror(int code);
 
