@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cctype>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
@@ -113,6 +114,18 @@ struct hy_db {
     uint32_t* d_lens = nullptr;
     uint32_t* d_gidx = nullptr;
     uint64_t inter_spill = 0;      // int2 entries
+    // Groups [0, groups_score32) hold the targets that are not eligible for the packed kernel;
+    // score-only scans run the 32-bit kernel over those and the packed kernel over the rest.
+    uint32_t groups_score32 = 0;
+    // Packed 16-bit partition (two targets per lane).
+    std::vector<Group> groups16;
+    Group* d_groups16 = nullptr;
+    uint2* d_words16 = nullptr;
+    uint32_t* d_lens16 = nullptr;
+    uint32_t* d_gidx16 = nullptr;
+    uint64_t inter16_spill = 0;    // uint2 entries
+    int q16 = 24;        // query rows per register tile of the packed kernel
+    int waves16 = 4;     // wavefronts per workgroup of the packed kernel
     // Systolic list (long targets, or everything when the alphabet leaves no padding symbol).
     std::vector<uint32_t> sys_list;
     uint32_t* d_sys_list = nullptr;
@@ -122,6 +135,7 @@ struct hy_db {
         (void)hipSetDevice(device);
         dev_free(d_matrix); dev_free(d_flat); dev_free(d_off);
         dev_free(d_groups); dev_free(d_words); dev_free(d_lens); dev_free(d_gidx);
+        dev_free(d_groups16); dev_free(d_words16); dev_free(d_lens16); dev_free(d_gidx16);
         dev_free(d_sys_list);
     }
 };
@@ -297,13 +311,20 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     db->total_residues = total;
     db->widest_bits = widest;
 
-    // Per-sequence widths (src/database.rs:896-903) — reported through hy_db_partitions. All
-    // partitions currently run in the 32-bit kernels, which are exact for every proven width.
+    // Per-sequence widths (src/database.rs:896-903). A target whose proof gives i8 or i16 AND
+    // leaves room for one gap opening below the bound (bound + gap_open <= 32767, so H - go never
+    // saturates) runs in the packed 16-bit kernel; every other target runs at 32 bits. Any width
+    // the proof accepts is exact (DETERMINISM.md §4, §6), so the split changes speed only.
+    const bool pair_profile_fits =
+        db->al_eff <= kNoPadSymbol &&
+        hy::inter16_lds_bytes((int)db->al_eff, 16) <= hy::kPairProfileMaxBytes;
+    std::vector<uint8_t> elig16(n, 0);
     for (size_t k = 0; k < n; ++k) {
+        int64_t b = 0;
         const int w = width_bits_for(
-            magnitude_bound(mode, min_e, max_e, gap_open, gap_ext, max_query_len, lens[k]),
-            nullptr);
+            magnitude_bound(mode, min_e, max_e, gap_open, gap_ext, max_query_len, lens[k]), &b);
         db->count_by_width[w == 8 ? 0 : (w == 16 ? 1 : 2)]++;
+        elig16[k] = pair_profile_fits && w <= 16 && b + (int64_t)gap_open <= 32767;
     }
 
     int rc = HY_OK;
@@ -334,16 +355,25 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     const double mean = (double)total / (double)n;
     const uint64_t long_threshold = std::max<uint64_t>(1024, (uint64_t)(8.0 * mean));
     const bool all_systolic = db->al_eff > kNoPadSymbol;
-    std::vector<uint32_t> inter_idx;
+    // 32-bit inter layout order: [targets not eligible for the packed kernel][eligible ones],
+    // each part length-sorted and starting a fresh group, so score-only scans can run the 32-bit
+    // kernel over the first part alone; end-tracking scans run it over both.
+    std::vector<uint32_t> inter_idx, idx16;
     inter_idx.reserve(n);
     for (uint32_t k : order_idx) {
         if (all_systolic || lens[k] > long_threshold) {
             db->sys_list.push_back(k);
             db->sys_max_len = std::max<uint64_t>(db->sys_max_len, lens[k]);
-        } else {
+        } else if (!elig16[k]) {
             inter_idx.push_back(k);
         }
     }
+    const size_t n_not16 = inter_idx.size();
+    for (uint32_t k : order_idx)
+        if (!(all_systolic || lens[k] > long_threshold) && elig16[k]) {
+            inter_idx.push_back(k);
+            idx16.push_back(k);
+        }
 
     // Inter layout: groups of 64 consecutive (length-sorted) targets; residues interleaved so a
     // wavefront's load of column word w reads 64 consecutive 32-bit words.
@@ -356,17 +386,22 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
             slot_len[s] = (uint32_t)lens[inter_idx[s]];
         }
         uint64_t words = 0, spill = 0;
-        for (size_t s = 0; s < ns; s += 64) {
-            Group g{};
-            g.first = (uint32_t)s;
-            g.count = (uint32_t)std::min<size_t>(64, ns - s);
-            g.ncols = slot_len[s];
-            g.word_off = words;
-            g.spill_off = spill;
-            words += (uint64_t)((g.ncols + 3) / 4) * 64;
-            spill += (uint64_t)(g.ncols + 1) * 64;
-            db->groups.push_back(g);
-        }
+        auto add_groups = [&](size_t lo, size_t hi) {
+            for (size_t s = lo; s < hi; s += 64) {
+                Group g{};
+                g.first = (uint32_t)s;
+                g.count = (uint32_t)std::min<size_t>(64, hi - s);
+                g.ncols = slot_len[s];
+                g.word_off = words;
+                g.spill_off = spill;
+                words += (uint64_t)((g.ncols + 3) / 4) * 64;
+                spill += (uint64_t)(g.ncols + 1) * 64;
+                db->groups.push_back(g);
+            }
+        };
+        add_groups(0, n_not16);
+        db->groups_score32 = (uint32_t)db->groups.size();
+        add_groups(n_not16, ns);
         std::vector<uint32_t> w((size_t)words, 0);
         for (const Group& g : db->groups) {
             const uint32_t nw = (g.ncols + 3) / 4;
@@ -390,6 +425,71 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
         if (dev_upload(&db->d_words, w) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_lens, slot_len) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_gidx, slot_gidx) != hipSuccess) return fail(HY_E_DEVICE);
+    }
+
+    // Packed 16-bit layout: groups of 128 length-sorted targets, lane l holding slots 2l (low
+    // half) and 2l+1 (high half); column word w of a lane is a uint2 (low target's four
+    // residues, high target's four residues).
+    {
+        const uint32_t pad = db->al_eff;
+        const size_t ns = idx16.size();
+        std::vector<uint32_t> slot_len(ns), slot_gidx(ns);
+        for (size_t s = 0; s < ns; ++s) {
+            slot_gidx[s] = idx16[s];
+            slot_len[s] = (uint32_t)lens[idx16[s]];
+        }
+        uint64_t words = 0, spill = 0;
+        for (size_t s = 0; s < ns; s += 128) {
+            Group g{};
+            g.first = (uint32_t)s;
+            g.count = (uint32_t)std::min<size_t>(128, ns - s);
+            g.ncols = slot_len[s];
+            g.word_off = words;
+            g.spill_off = spill;
+            words += (uint64_t)((g.ncols + 3) / 4) * 64;
+            spill += (uint64_t)(g.ncols + 1) * 64;
+            db->groups16.push_back(g);
+        }
+        std::vector<uint2> w((size_t)words, make_uint2(0, 0));
+        for (const Group& g : db->groups16) {
+            const uint32_t nw = (g.ncols + 3) / 4;
+            for (uint32_t lane = 0; lane < 64; ++lane) {
+                for (uint32_t jw = 0; jw < nw; ++jw) {
+                    uint32_t wd[2] = {0, 0};
+                    for (uint32_t half = 0; half < 2; ++half) {
+                        const uint32_t slot = 2 * lane + half;
+                        const bool live = slot < g.count;
+                        const uint32_t k = live ? slot_gidx[g.first + slot] : 0;
+                        const uint32_t L = live ? (uint32_t)lens[k] : 0;
+                        for (uint32_t b = 0; b < 4; ++b) {
+                            const uint32_t col = jw * 4 + b;
+                            const uint32_t sym = col < L ? seqs[k][col] : pad;
+                            wd[half] |= (sym & 0xffu) << (8 * b);
+                        }
+                    }
+                    w[(size_t)g.word_off + (size_t)jw * 64 + lane] = make_uint2(wd[0], wd[1]);
+                }
+            }
+        }
+        db->inter16_spill = spill;
+        // Tile shape: 24 rows and 4 wavefronts per workgroup unless the pair profile would not
+        // fit; HYALITE_HIP_TILE="rows,waves" overrides (used by the tuning sweeps).
+        db->q16 = 24;
+        db->waves16 = 4;
+        if (const char* t = std::getenv("HYALITE_HIP_TILE")) {
+            int q = 0, w = 0;
+            if (std::sscanf(t, "%d,%d", &q, &w) == 2 && q > 0 && w > 0) {
+                db->q16 = hy::inter16_tile_rows(q);
+                db->waves16 = w >= 8 ? 8 : 4;
+            }
+        }
+        while (db->q16 > 16 &&
+               hy::inter16_lds_bytes((int)db->al_eff, db->q16) > hy::kPairProfileMaxBytes)
+            db->q16 -= 8;
+        if (dev_upload(&db->d_groups16, db->groups16) != hipSuccess) return fail(HY_E_DEVICE);
+        if (dev_upload(&db->d_words16, w) != hipSuccess) return fail(HY_E_DEVICE);
+        if (dev_upload(&db->d_lens16, slot_len) != hipSuccess) return fail(HY_E_DEVICE);
+        if (dev_upload(&db->d_gidx16, slot_gidx) != hipSuccess) return fail(HY_E_DEVICE);
     }
     if (dev_upload(&db->d_sys_list, db->sys_list) != hipSuccess) return fail(HY_E_DEVICE);
     (void)rc;
@@ -441,7 +541,8 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     const size_t qcap = std::max<uint64_t>(db->mql, 1);
     if (hipMalloc((void**)&s->d_query, qcap) != hipSuccess) return fail(HY_E_OUT_OF_MEMORY);
     if (hipHostMalloc((void**)&s->h_query, qcap) != hipSuccess) return fail(HY_E_OUT_OF_MEMORY);
-    s->spill_entries = std::max<uint64_t>(db->inter_spill, 1);
+    // One boundary buffer serves both inter kernels: they run one after the other on the stream.
+    s->spill_entries = std::max<uint64_t>(std::max(db->inter_spill, db->inter16_spill), 1);
     if (hipMalloc((void**)&s->d_spill, s->spill_entries * sizeof(int2)) != hipSuccess)
         return fail(HY_E_OUT_OF_MEMORY);
     // Systolic boundary rows are needed only when a query can exceed one pass (512 rows).
@@ -538,9 +639,31 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     ia.out_tend = s->d_tend;
     ia.key = ends ? nullptr : key;
     if (s->profile && hipEventRecord(s->ev[0], s->stream) != hipSuccess) return HY_E_DEVICE;
-    if (hy::launch_inter32((int)db->mode, ends, ia, (uint32_t)db->groups.size(), s->stream) !=
-        hipSuccess)
+    // End tracking runs every non-long target at 32 bits; score-only scans send the targets the
+    // packed kernel accepts there instead.
+    const uint32_t g32 = ends ? (uint32_t)db->groups.size() : db->groups_score32;
+    if (hy::launch_inter32((int)db->mode, ends, ia, g32, s->stream) != hipSuccess)
         return HY_E_DEVICE;
+    if (!ends && !db->groups16.empty()) {
+        hy::Inter16Args pa{};
+        pa.groups = db->d_groups16;
+        pa.ngroups = (uint32_t)db->groups16.size();
+        pa.words = db->d_words16;
+        pa.lens = db->d_lens16;
+        pa.gidx = db->d_gidx16;
+        pa.query = s->d_query;
+        pa.m = m;
+        pa.matrix = db->d_matrix;
+        pa.al = (int)db->al;
+        pa.al_eff = (int)db->al_eff;
+        pa.go = db->go;
+        pa.ge = db->ge;
+        pa.spill = reinterpret_cast<uint2*>(s->d_spill);
+        pa.out_score = ia.out_score;
+        pa.key = key;
+        if (hy::launch_inter16((int)db->mode, pa, db->q16, db->waves16, s->stream) != hipSuccess)
+            return HY_E_DEVICE;
+    }
     hy::SysArgs sa = sys_args(db, s, m);
     sa.out_score = ia.out_score;
     sa.out_qend = s->d_qend;

@@ -66,6 +66,27 @@ struct InterArgs {
     unsigned long long* key; // fused argmax key (may be null)
 };
 
+// Packed 16-bit inter-sequence kernel: two targets per lane (low half = even slot, high half =
+// odd slot of the lane's pair), 128 targets per group.
+struct Inter16Args {
+    const Group* groups;     // count <= 128
+    uint32_t ngroups;
+    const uint2* words;      // words[word_off + (col/4)*64 + lane] = (low target, high target)
+    const uint32_t* lens;    // per slot
+    const uint32_t* gidx;    // per slot -> database index
+    const uint8_t* query;
+    int m;
+    const int32_t* matrix;   // al x al (clamped to the 32-bit range; saturated to 16 bits here)
+    int al, al_eff;
+    int go, ge;
+    uint2* spill;            // spill[spill_off + col*64 + lane] = (H - go pair, F pair)
+    int32_t* out_score;      // database order (may be null)
+    unsigned long long* key; // fused argmax (may be null)
+};
+
+// Largest pair-profile footprint the packed kernel stages in LDS (bytes).
+constexpr uint32_t kPairProfileMaxBytes = 150u * 1024u;
+
 struct SysArgs {
     const uint8_t* flat;      // database residues in original order
     const uint64_t* off;      // n+1 offsets
@@ -92,6 +113,13 @@ hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngro
                           hipStream_t stream);
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
                         hipStream_t stream);
+// q: query rows per register tile (16, 24 or 32); waves: wavefronts per workgroup sharing one
+// pair profile (4 or 8).
+hipError_t launch_inter16(int mode, const Inter16Args& a, int q, int waves, hipStream_t stream);
+// Tile rows the packed kernel actually uses for a requested q.
+int inter16_tile_rows(int q);
+// LDS bytes of the pair profile for an alphabet of al_eff symbols (plus the padding symbol).
+uint32_t inter16_lds_bytes(int al_eff, int q);
 hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream);
 
 }  // namespace hy

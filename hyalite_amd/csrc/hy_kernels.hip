@@ -247,12 +247,192 @@ __global__ __launch_bounds__(64) void k_inter32(InterArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Inter-sequence kernel, packed 16-bit lanes: two targets per lane.
+// ------------------------------------------------------------------------------------------------
+//
+// The same recurrence as k_inter32 on pairs of 16-bit values with saturating packed arithmetic
+// (v_pk_add_i16/v_pk_sub_i16 with clamp, v_pk_max_i16): the i8/i16 realisation of
+// DETERMINISM.md §1, sentinel = i16::MIN. A lane's two targets read different residues, so the
+// profile is indexed by the residue PAIR: pp[(a * A + b) * S + r] holds (sub(q_r, a) + go,
+// sub(q_r, b) + go) saturated to 16 bits, A = al_eff + 1 symbols including padding. The tile of
+// Q query rows is shared by all wavefronts of the workgroup.
+//
+// Exactness: the host sends a target here only when its width proof gives |cell| <= bound with
+// bound + gap_open <= 32767, so H - gap_open never saturates on a real cell and
+// HG[i-1][j-1] + (sub + go) equals H[i-1][j-1] + sub whenever the sum is in range (a saturated
+// diagonal is below every real cell and loses the max). Padding cells are bounded by saturation
+// and never flow into real cells; in local mode they never exceed a real cell.
+typedef short s2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ s2v as_s2(uint32_t u) { return __builtin_bit_cast(s2v, u); }
+__device__ __forceinline__ uint32_t as_u32(s2v v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ s2v p_add(s2v a, s2v b) { return __builtin_elementwise_add_sat(a, b); }
+__device__ __forceinline__ s2v p_sub(s2v a, s2v b) { return __builtin_elementwise_sub_sat(a, b); }
+__device__ __forceinline__ s2v p_max(s2v a, s2v b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ int sat16(long long v) {
+    return v < -32768 ? -32768 : (v > 32767 ? 32767 : (int)v);
+}
+__device__ __forceinline__ s2v splat2(int v) {
+    const uint32_t u = (uint32_t)(uint16_t)(short)v;
+    return as_s2(u | (u << 16));
+}
+__device__ __forceinline__ s2v pair2(int lo, int hi) {
+    return as_s2((uint32_t)(uint16_t)(short)lo | ((uint32_t)(uint16_t)(short)hi << 16));
+}
+// Keep v in the halves selected by (lo, hi), the 16-bit sentinel elsewhere.
+__device__ __forceinline__ s2v keep2(s2v v, bool lo, bool hi) {
+    const uint32_t m = (lo ? 0x0000FFFFu : 0u) | (hi ? 0xFFFF0000u : 0u);
+    return as_s2((as_u32(v) & m) | (0x80008000u & ~m));
+}
+
+template <int MODE>
+__device__ __forceinline__ int border16(int i, int j, int go, int ge) {
+    if (i == 0) return (j == 0 || MF<MODE>::top) ? 0 : sat16(-(long long)gap32(go, ge, j));
+    return MF<MODE>::left ? 0 : sat16(-(long long)gap32(go, ge, i));
+}
+
+template <int MODE, int Q, int NW>
+__global__ __launch_bounds__(64 * NW) void k_inter16(Inter16Args a) {
+    using F = MF<MODE>;
+    extern __shared__ uint32_t pp[];
+    constexpr int S = Q + 4;
+    const int lane = threadIdx.x & 63;
+    const uint32_t gix = blockIdx.x * NW + (threadIdx.x >> 6);
+    const bool wlive = gix < a.ngroups;
+    Group g{};
+    if (wlive) g = a.groups[gix];
+    const bool liveA = 2 * lane < (int)g.count, liveB = 2 * lane + 1 < (int)g.count;
+    const uint32_t slotA = g.first + 2 * lane, slotB = slotA + 1;
+    const int nA = liveA ? (int)a.lens[slotA] : 0;
+    const int nB = liveB ? (int)a.lens[slotB] : 0;
+    const int L = (int)g.ncols;
+    const int m = a.m, go = a.go, ge = a.ge;
+    const int A = a.al_eff + 1;
+
+    int bsA, bsB, dj, di;
+    border_best<MODE>(m, nA, go, ge, bsA, dj, di);
+    border_best<MODE>(m, nB, go, ge, bsB, dj, di);
+    s2v run = splat2(-32768);
+
+    const s2v go2 = splat2(go), ge2 = splat2(ge), zero2 = splat2(0);
+    const int nchunks = (m + Q - 1) / Q;
+    uint2* sp = a.spill + g.spill_off + lane;
+    const uint2* wp = a.words + g.word_off + lane;
+
+    for (int c = 0; c < nchunks; ++c) {
+        const int i0 = c * Q + 1;
+        const bool lastc = c == nchunks - 1;
+        const int rm = m - i0;
+        __syncthreads();
+        for (int k = threadIdx.x; k < A * A * Q; k += 64 * NW) {
+            const int pr = k / Q, r = k - pr * Q, i = i0 + r;
+            const int ca = pr / A, cb = pr - ca * A;
+            int va = -32768, vb = -32768;
+            if (i <= m) {
+                const int* row = a.matrix + (int)a.query[i - 1] * a.al;
+                if (ca < a.al_eff) va = sat16((long long)row[ca] + go);
+                if (cb < a.al_eff) vb = sat16((long long)row[cb] + go);
+            }
+            pp[pr * S + r] = (uint32_t)(uint16_t)(short)va | ((uint32_t)(uint16_t)(short)vb << 16);
+        }
+        __syncthreads();
+        if (!wlive) continue;
+
+        s2v HG[Q], E[Q];
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            HG[r] = splat2(sat16((long long)border16<MODE>(i0 + r, 0, go, ge) - go));
+            E[r] = splat2(-32768);
+        }
+        s2v HGd = splat2(sat16((long long)border16<MODE>(i0 - 1, 0, go, ge) - go));
+
+        for (int jb = 0; jb < L; jb += 4) {
+            const uint2 w = wp[(size_t)(jb >> 2) * 64];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = jb + b + 1;
+                if (j > L) break;
+                s2v hgup, f;
+                if (c == 0) {
+                    hgup = splat2(sat16((long long)border16<MODE>(0, j, go, ge) - go));
+                    f = splat2(-32768);
+                } else {
+                    const uint2 t = sp[(size_t)j * 64];
+                    hgup = as_s2(t.x);
+                    f = as_s2(t.y);
+                }
+                const int ca = (int)((w.x >> (8 * b)) & 0xffu);
+                const int cb = (int)((w.y >> (8 * b)) & 0xffu);
+                const uint4* prow = reinterpret_cast<const uint4*>(&pp[(ca * A + cb) * S]);
+                s2v sg[Q];
+#pragma unroll
+                for (int r4 = 0; r4 < Q / 4; ++r4) {
+                    const uint4 v = prow[r4];
+                    sg[4 * r4 + 0] = as_s2(v.x); sg[4 * r4 + 1] = as_s2(v.y);
+                    sg[4 * r4 + 2] = as_s2(v.z); sg[4 * r4 + 3] = as_s2(v.w);
+                }
+                s2v hgd = HGd;
+                HGd = hgup;
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    const s2v diag = p_add(hgd, sg[r]);
+                    const s2v e = p_max(HG[r], p_sub(E[r], ge2));
+                    f = p_max(hgup, p_sub(f, ge2));
+                    s2v h = p_max(p_max(diag, e), f);
+                    if (F::local) h = p_max(h, zero2);
+                    hgd = HG[r];
+                    E[r] = e;
+                    HG[r] = p_sub(h, go2);
+                    hgup = HG[r];
+                    if (F::local) run = p_max(run, h);
+                }
+                if (c + 1 < nchunks) sp[(size_t)j * 64] = make_uint2(as_u32(HG[Q - 1]), as_u32(f));
+                if (!F::local && lastc) {
+                    const bool la = F::lrow ? j <= nA : j == nA;
+                    const bool lb = F::lrow ? j <= nB : j == nB;
+                    if (la || lb) {
+                        s2v v = HG[0];
+#pragma unroll
+                        for (int r = 1; r < Q; ++r)
+                            if (r == rm) v = HG[r];
+                        run = p_max(run, keep2(p_add(v, go2), la, lb));
+                    }
+                }
+                if (F::lcol && (j == nA || j == nB)) {
+#pragma unroll
+                    for (int r = 0; r < Q; ++r)
+                        if (i0 + r <= m)
+                            run = p_max(run, keep2(p_add(HG[r], go2), j == nA, j == nB));
+                }
+            }
+        }
+    }
+    if (!wlive) return;
+    const int sA = max(bsA, (int)run.x), sB = max(bsB, (int)run.y);
+    unsigned long long k = 0ull;
+    if (liveA) {
+        const uint32_t gi = a.gidx[slotA];
+        if (a.out_score) a.out_score[gi] = sA;
+        k = make_key(sA, gi);
+    }
+    if (liveB) {
+        const uint32_t gi = a.gidx[slotB];
+        if (a.out_score) a.out_score[gi] = sB;
+        const unsigned long long kb = make_key(sB, gi);
+        k = kb > k ? kb : k;
+    }
+    if (a.key) wave_argmax(k, a.key);
+}
+
+// ------------------------------------------------------------------------------------------------
 // Intra-sequence systolic kernel, 32-bit lanes, one (query, target) pair per wavefront.
 // ------------------------------------------------------------------------------------------------
 template <int MODE, int R, bool ENDS, bool WINNER>
 __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
     using F = MF<MODE>;
     extern __shared__ int smat[];
+    __shared__ uint8_t tring[256];
+    __shared__ int2 sring[256];
     const int lane = threadIdx.x;
     // Winner mode: an all-zero key means no hit was recorded; fall back to index 0 and let the
     // self-check below flag it.
@@ -287,22 +467,39 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
         }
         int HGd = wsub(border<MODE>(r0 - 1, 0, go, ge), go);
         int topHG = 0, topF = kNeg32;
+        // Residues and the previous pass's boundary row are staged through LDS rings of four
+        // 64-entry blocks: block e+1 is fetched into registers at the start of 64-step epoch e
+        // and stored at its end, so global latency overlaps a whole epoch of compute. At step s
+        // lane l reads residue s-l (blocks e-1, e) and lane 0 reads boundary column s+1.
+        tring[lane] = lane < n ? t[lane] : (uint8_t)0;
+        if (p > 0) sring[lane] = lane + 1 <= n ? sp[lane + 1] : make_int2(0, kNeg32);
+        __syncthreads();
+        uint8_t tnext = 0;
+        int2 snext = make_int2(0, kNeg32);
         const int steps = n + 63;
         for (int s = 0; s < steps; ++s) {
+            if ((s & 63) == 0) {
+                const int idx = s + 64 + lane;
+                tnext = idx < n ? t[idx] : (uint8_t)0;
+                if (p > 0) {
+                    const int col = s + 65 + lane;
+                    snext = col <= n ? sp[col] : make_int2(0, kNeg32);
+                }
+            }
             const int j = s - lane + 1;
             if (lane == 0 && j <= n) {
                 if (p == 0) {
                     topHG = wsub(border<MODE>(0, j, go, ge), go);
                     topF = kNeg32;
                 } else {
-                    const int2 v = sp[j];
+                    const int2 v = sring[s & 255];
                     topHG = v.x;
                     topF = v.y;
                 }
             }
             int outHG = 0, outF = 0;
             if (j >= 1 && j <= n) {
-                const int c = t[j - 1];
+                const int c = tring[(j - 1) & 255];
                 int f = topF, hgup = topHG, hgd = HGd;
                 HGd = topHG;
 #pragma unroll
@@ -341,6 +538,11 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
             // Lane l receives lane l-1's bottom row (wave_shr:1); lane 0 keeps its own value.
             topHG = __builtin_amdgcn_update_dpp(topHG, outHG, 0x138, 0xf, 0xf, false);
             topF = __builtin_amdgcn_update_dpp(topF, outF, 0x138, 0xf, 0xf, false);
+            if ((s & 63) == 63) {
+                tring[(s + 1 + lane) & 255] = tnext;
+                if (p > 0) sring[(s + 1 + lane) & 255] = snext;
+                __syncthreads();
+            }
         }
         __threadfence_block();
         __syncthreads();
@@ -390,10 +592,9 @@ hipError_t launch_inter32_m(bool ends, const InterArgs& a, uint32_t ngroups, hip
     return hipGetLastError();
 }
 
-template <int MODE>
-hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
-                          hipStream_t st) {
-    constexpr int R = 8;
+template <int MODE, int R>
+hipError_t launch_sys32_mr(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                           hipStream_t st) {
     const size_t lds = a.al_eff * a.al_eff <= 16384 ? (size_t)a.al_eff * a.al_eff * sizeof(int) : 0;
     if (winner) {
         if (ends) hipLaunchKernelGGL((k_sys32<MODE, R, true, true>), dim3(1), dim3(64), lds, st, a);
@@ -408,7 +609,64 @@ hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nbl
     return hipGetLastError();
 }
 
+// Rows per lane: the smallest of 2, 4, 6, 8 whose 64-lane pass covers the query (one pass up to
+// 512 rows); longer queries take several passes of 8 rows per lane.
+template <int MODE>
+hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                          hipStream_t st) {
+    if (a.m <= 128) return launch_sys32_mr<MODE, 2>(ends, winner, a, nblocks, st);
+    if (a.m <= 256) return launch_sys32_mr<MODE, 4>(ends, winner, a, nblocks, st);
+    if (a.m <= 384) return launch_sys32_mr<MODE, 6>(ends, winner, a, nblocks, st);
+    return launch_sys32_mr<MODE, 8>(ends, winner, a, nblocks, st);
+}
+
+template <int MODE, int Q, int NW>
+hipError_t launch_inter16_qw(const Inter16Args& a, hipStream_t st) {
+    const uint32_t lds = inter16_lds_bytes(a.al_eff, Q);
+    const uint32_t blocks = (a.ngroups + NW - 1) / NW;
+    auto kern = k_inter16<MODE, Q, NW>;
+    if (lds > 64u * 1024u) {
+        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * NW), lds, st, a);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_inter16_m(const Inter16Args& a, int q, int waves, hipStream_t st) {
+    if (waves >= 8) {
+        if (q <= 16) return launch_inter16_qw<MODE, 16, 8>(a, st);
+        if (q <= 24) return launch_inter16_qw<MODE, 24, 8>(a, st);
+        return launch_inter16_qw<MODE, 32, 8>(a, st);
+    }
+    if (q <= 16) return launch_inter16_qw<MODE, 16, 4>(a, st);
+    if (q <= 24) return launch_inter16_qw<MODE, 24, 4>(a, st);
+    return launch_inter16_qw<MODE, 32, 4>(a, st);
+}
+
 }  // namespace
+
+int inter16_tile_rows(int q) { return q <= 16 ? 16 : (q <= 24 ? 24 : 32); }
+
+uint32_t inter16_lds_bytes(int al_eff, int q) {
+    const uint32_t A = (uint32_t)al_eff + 1;
+    return A * A * (uint32_t)(inter16_tile_rows(q) + 4) * 4u;
+}
+
+hipError_t launch_inter16(int mode, const Inter16Args& a, int q, int waves, hipStream_t stream) {
+    if (a.ngroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_inter16_m<kSW>(a, q, waves, stream);
+    case kNW: return launch_inter16_m<kNW>(a, q, waves, stream);
+    case kHW: return launch_inter16_m<kHW>(a, q, waves, stream);
+    case kOV: return launch_inter16_m<kOV>(a, q, waves, stream);
+    case kSHW: return launch_inter16_m<kSHW>(a, q, waves, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
 
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
                            hipStream_t stream) {

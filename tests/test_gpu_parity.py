@@ -145,6 +145,21 @@ def test_overlap_negative_reach_at_the_i8_edge():
     check([[1] * 6], scheme, MODE_OV, hy.SearchType.ScoreEnd, [[0] * 120 + [1] * 6])
 
 
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_packed_kernel_eligibility_edge(mode):
+    """Targets whose width-proof bound sits just inside (packed 16-bit kernel) and just outside
+    (32-bit kernel) bound + gap_open <= 32767, with mismatch-heavy content so that real cells run
+    deep negative and non-real intermediates saturate."""
+    scheme = (2, [1, -8, -8, 1], 6, 2)
+    # NW/HW/SHW bound = 10 (m + n) + 4; OV/SW bounds are far smaller at these lengths.
+    m = 1000
+    rng = np.random.default_rng(300 + mode)
+    lens = [2270, 2275, 2276, 2277, 2290, 5, 0, 1500]
+    seqs = [np.where(rng.random(n) < 0.9, 1, 0).astype(np.uint8) for n in lens]
+    q = np.where(rng.random(m) < 0.9, 0, 1).astype(np.uint8)
+    check(seqs, scheme, mode, hy.SearchType.ScoreEnd, [q, q[:400]], mql=m)
+
+
 def test_opal_regressions_and_proptest_seeds():
     """tests/opal_regressions.rs:68-82 and the proptest regression seeds."""
     check([[0, 0, 1, 1, 2]], (3, identity_matrix(3, 2, -2), 1, 1), MODE_NW,
