@@ -27,9 +27,18 @@ sys.path.insert(0, ROOT)
 import hyalite_amd as hy  # noqa: E402
 from hyalite_amd import workloads as wl  # noqa: E402
 
-# MI355X: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 32-bit VALU lane-ops/s.
+# MI355X: 256 CUs x 64 lanes x 2.4 GHz = 39.3e12 32-bit VALU lane-ops/s; packed 16-bit
+# instructions carry two elements, 78.6e12 element-ops/s.
 VALU_PEAK_TOPS_I32 = 256 * 64 * 2.4e9 / 1e12
+VALU_PEAK_TOPS_PK16 = 2 * VALU_PEAK_TOPS_I32
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, the factors a 256 MiB device copy gives); the run and the counters are in
+# profiles/k_inter16_protein_sw_360x1M.md.
+MEASURED_TRAFFIC_BYTES = {
+    ("protein_sw_scoreend_360x1000000", "k_inter16<SW,24,1>"): 50.14e9,
+}
 
 
 def ops_per_cell(mode: int) -> int:
@@ -49,35 +58,71 @@ def cpu_model() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(w: wl.Workload, budget_s: float = 12.0):
-    """The C oracle (scalar restatement of the reference kernel) on a bounded random sample of
-    the same database, all host threads. Its hit on the sample is checked against itself only;
-    parity of the GPU path is the job of the test-suite."""
+def host_threads() -> int:
+    try:
+        return max(1, len(os.sched_getaffinity(0)))
+    except AttributeError:
+        return max(1, os.cpu_count() or 1)
+
+
+def cpu_baseline(w: wl.Workload, budget_s: float = 3.0, max_targets: int = 60_000):
+    """The AVX2 restatement of the reference's avx2 i16 inter-sequence backend (16 saturating
+    lanes, database-order batches, Precomputed table built before the timed scan) on a bounded
+    random sample of the same database, all host threads, plus a 1-thread and a length-sorted
+    run. Every timed result is checked against the scalar oracle on the same sample first."""
     import oracle
-    threads = max(1, min(os.cpu_count() or 1, 64))
+    threads = host_threads()
     q = w.queries[0]
-    k = 2000
-    elapsed = 0.0
-    cells = 0
+    k = 4000
     while True:
         sub = w.subsample(k, seed=17)
-        t0 = time.perf_counter()
-        oracle.scan_flat(sub.flat, sub.off, q, w.al, w.matrix, w.go, w.ge, w.mode,
-                         w.search_type == 1, threads=threads)
-        elapsed = time.perf_counter() - t0
-        cells = len(q) * sub.residues
-        if elapsed >= budget_s / 4 or k >= w.n:
+        scores, secs = oracle.avx2_scan_scores_flat(sub.flat, sub.off, q, w.al, w.matrix, w.go,
+                                                    w.ge, w.mode, threads=threads)
+        if secs >= budget_s / 3 or k >= min(w.n, max_targets):
             break
-        k = min(w.n, int(k * max(2.0, (budget_s / 2) / max(elapsed, 1e-3))))
+        k = min(w.n, max_targets, int(k * max(2.0, budget_s / max(secs, 1e-3))))
+    want = oracle.scan_all_flat(sub.flat, sub.off, q, w.al, w.matrix, w.go, w.ge, w.mode,
+                                ends=False, threads=threads)
+    if not np.array_equal(scores, want):
+        raise RuntimeError("AVX2 baseline disagrees with the scalar oracle")
+    cells = len(q) * sub.residues
+    sorted_scores, sorted_secs = oracle.avx2_scan_scores_flat(
+        sub.flat, sub.off, q, w.al, w.matrix, w.go, w.ge, w.mode, threads=threads, sort=True)
+    if not np.array_equal(sorted_scores, want):
+        raise RuntimeError("sorted AVX2 baseline disagrees with the scalar oracle")
+    one = sub.subsample(max(200, sub.n // max(threads, 1)), seed=3)
+    one_scores, one_secs = oracle.avx2_scan_scores_flat(one.flat, one.off, q, w.al, w.matrix,
+                                                        w.go, w.ge, w.mode, threads=1)
     return {
-        "value": round(cells / elapsed / 1e9, 4),
+        "value": round(cells / secs / 1e9, 3),
         "unit": "GCUPS",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sub.n} targets ({sub.residues} residues) drawn from the same database, "
-                  f"scalar C oracle (src/kernel.rs fill_dp restated), {threads} threads on "
-                  f"{cpu_model()}, {elapsed:.2f} s",
+        "sample": f"{sub.n} random targets ({sub.residues} residues) of the same database; AVX2 "
+                  f"16 x i16 restatement of the reference avx2 backend, database-order batches, "
+                  f"{threads} threads on {cpu_model()}, scan {secs:.2f} s; checked against the "
+                  f"scalar oracle",
+        "one_thread_gcups": round(len(q) * one.residues / one_secs / 1e9, 3),
+        "length_sorted_gcups": round(cells / sorted_secs / 1e9, 3),
     }
+
+
+def copy_calibration(mib: int) -> int:
+    """One device-to-device copy of `mib` MiB (for scaling HBM byte counters: a plain copy
+    fetches and writes exactly that many bytes)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(n)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(n)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(n)) == 0
+    assert hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) == 0  # device to device
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": n}))
+    return 0
 
 
 def main() -> int:
@@ -87,7 +132,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--targets", type=int, default=1_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the scan would shard targets across GPUs; see DESIGN.md)", file=sys.stderr)
@@ -132,6 +181,9 @@ def main() -> int:
     # Dominant kernel: the inter-sequence kernel over all but the long-target tail.
     lens = w.lens()
     inter_ms_avg = inter_ms / max(nscan, 1)
+    tile = (24, 1)
+    if os.environ.get("HYALITE_HIP_TILE"):
+        tile = tuple(int(x) for x in os.environ["HYALITE_HIP_TILE"].split(","))
     tail_ms_avg = tail_ms / max(nscan, 1)
     roof = None
     if inter_ms_avg > 0:
@@ -140,14 +192,16 @@ def main() -> int:
         thr = max(1024, int(8.0 * lens.mean()))
         inter_cells = len(q) * int(lens[lens <= thr].sum())
         achieved = inter_cells * ops_per_cell(w.mode) / (inter_ms_avg / 1e3) / 1e12
+        kernel = f"k_inter16<SW,{tile[0]},{tile[1]}>"
         roof = {
             "bound": "valu",
             "achieved": round(achieved, 3),
-            "peak": round(VALU_PEAK_TOPS_I32, 2),
+            "peak": round(VALU_PEAK_TOPS_PK16, 2),
             "unit": "TOP/s",
-            "frac": round(achieved / VALU_PEAK_TOPS_I32, 4),
-            "traffic": None,
-            "kernel": "k_inter32",
+            "frac": round(achieved / VALU_PEAK_TOPS_PK16, 4),
+            "traffic": MEASURED_TRAFFIC_BYTES.get((w.name, kernel)),
+            "kernel": kernel,
+            "peak_note": "packed 16-bit VALU element-ops (2 per lane per instruction)",
             "kernel_ms": round(inter_ms_avg, 3),
             "ops_per_cell": ops_per_cell(w.mode),
             "cells_per_launch": inter_cells,
@@ -165,7 +219,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int32",
+        "dtype": "int16 (packed pairs, saturating)",
         "data": "synthetic (seeded; see hyalite_amd/workloads.py)",
         "config": {"workload": w.name, "query_len": len(q), "targets": w.n,
                    "residues": w.residues, "mode": "SW", "search_type": "ScoreEnd",

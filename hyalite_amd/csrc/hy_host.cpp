@@ -159,7 +159,11 @@ struct hy_scratch {
     // Instrumentation: events around the per-target kernels of the last scan.
     bool profile = false;
     bool pending = false;
+    bool pending_tail = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // Auxiliary stream for the long-target kernel, with its fork/join events.
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     double inter_ms = 0.0, tail_ms = 0.0;
     uint64_t launches = 0;
 
@@ -171,8 +175,12 @@ struct hy_scratch {
         dev_free(d_key); dev_free(d_result);
         if (h_query) (void)hipHostFree(h_query);
         if (h_result) (void)hipHostFree(h_result);
+        if (aux) (void)hipStreamSynchronize(aux);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (aux) (void)hipStreamDestroy(aux);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -315,9 +323,26 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     // leaves room for one gap opening below the bound (bound + gap_open <= 32767, so H - go never
     // saturates) runs in the packed 16-bit kernel; every other target runs at 32 bits. Any width
     // the proof accepts is exact (DETERMINISM.md §4, §6), so the split changes speed only.
+    // Tile shape of the packed kernel: 24 query rows per register tile, one wavefront per
+    // workgroup with a per-symbol profile; HYALITE_HIP_TILE="rows,waves" overrides it (waves 4 or
+    // 8 select workgroups sharing a residue-pair profile), as the tuning sweeps do.
+    db->q16 = 24;
+    db->waves16 = 1;
+    if (const char* t = std::getenv("HYALITE_HIP_TILE")) {
+        int q = 0, w = 0;
+        if (std::sscanf(t, "%d,%d", &q, &w) == 2 && q > 0 && w > 0) {
+            db->q16 = hy::inter16_tile_rows(q);
+            db->waves16 = w >= 8 ? 8 : (w >= 4 ? 4 : 1);
+        }
+    }
+    if (db->waves16 > 1)
+        while (db->q16 > 16 &&
+               hy::inter16_lds_bytes((int)db->al_eff, db->q16) > hy::kPairProfileMaxBytes)
+            db->q16 -= 8;
     const bool pair_profile_fits =
         db->al_eff <= kNoPadSymbol &&
-        hy::inter16_lds_bytes((int)db->al_eff, 16) <= hy::kPairProfileMaxBytes;
+        (db->waves16 == 1 ||
+         hy::inter16_lds_bytes((int)db->al_eff, db->q16) <= hy::kPairProfileMaxBytes);
     std::vector<uint8_t> elig16(n, 0);
     for (size_t k = 0; k < n; ++k) {
         int64_t b = 0;
@@ -391,7 +416,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
                 Group g{};
                 g.first = (uint32_t)s;
                 g.count = (uint32_t)std::min<size_t>(64, hi - s);
-                g.ncols = slot_len[s];
+                g.ncols = (slot_len[s] + 3u) & ~3u;  // whole four-column words
                 g.word_off = words;
                 g.spill_off = spill;
                 words += (uint64_t)((g.ncols + 3) / 4) * 64;
@@ -443,7 +468,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
             Group g{};
             g.first = (uint32_t)s;
             g.count = (uint32_t)std::min<size_t>(128, ns - s);
-            g.ncols = slot_len[s];
+            g.ncols = (slot_len[s] + 3u) & ~3u;  // whole four-column words
             g.word_off = words;
             g.spill_off = spill;
             words += (uint64_t)((g.ncols + 3) / 4) * 64;
@@ -472,20 +497,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
             }
         }
         db->inter16_spill = spill;
-        // Tile shape: 24 rows and 4 wavefronts per workgroup unless the pair profile would not
-        // fit; HYALITE_HIP_TILE="rows,waves" overrides (used by the tuning sweeps).
-        db->q16 = 24;
-        db->waves16 = 4;
-        if (const char* t = std::getenv("HYALITE_HIP_TILE")) {
-            int q = 0, w = 0;
-            if (std::sscanf(t, "%d,%d", &q, &w) == 2 && q > 0 && w > 0) {
-                db->q16 = hy::inter16_tile_rows(q);
-                db->waves16 = w >= 8 ? 8 : 4;
-            }
-        }
-        while (db->q16 > 16 &&
-               hy::inter16_lds_bytes((int)db->al_eff, db->q16) > hy::kPairProfileMaxBytes)
-            db->q16 -= 8;
+
         if (dev_upload(&db->d_groups16, db->groups16) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_words16, w) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_lens16, slot_len) != hipSuccess) return fail(HY_E_DEVICE);
@@ -536,7 +548,10 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     s->device = db->device;
     s->db = db;
     auto fail = [&](int code) { delete s; return code; };
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->join, hipEventDisableTiming) != hipSuccess)
         return fail(HY_E_DEVICE);
     const size_t qcap = std::max<uint64_t>(db->mql, 1);
     if (hipMalloc((void**)&s->d_query, qcap) != hipSuccess) return fail(HY_E_OUT_OF_MEMORY);
@@ -638,6 +653,27 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     ia.out_qend = s->d_qend;
     ia.out_tend = s->d_tend;
     ia.key = ends ? nullptr : key;
+
+    // Long targets run on the auxiliary stream, forked from and joined back into the main one,
+    // so their few long wavefronts overlap the inter-sequence kernels instead of trailing them.
+    const bool tail = !db->sys_list.empty();
+    if (tail) {
+        if (hipEventRecord(s->fork, s->stream) != hipSuccess ||
+            hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess)
+            return HY_E_DEVICE;
+        hy::SysArgs sa = sys_args(db, s, m);
+        sa.out_score = ia.out_score;
+        sa.out_qend = s->d_qend;
+        sa.out_tend = s->d_tend;
+        sa.key = ia.key;
+        if (s->profile && hipEventRecord(s->ev[2], s->aux) != hipSuccess) return HY_E_DEVICE;
+        if (hy::launch_sys32((int)db->mode, ends, false, sa, (uint32_t)db->sys_list.size(),
+                             s->aux) != hipSuccess)
+            return HY_E_DEVICE;
+        if (s->profile && hipEventRecord(s->ev[3], s->aux) != hipSuccess) return HY_E_DEVICE;
+        if (hipEventRecord(s->join, s->aux) != hipSuccess) return HY_E_DEVICE;
+    }
+
     if (s->profile && hipEventRecord(s->ev[0], s->stream) != hipSuccess) return HY_E_DEVICE;
     // End tracking runs every non-long target at 32 bits; score-only scans send the targets the
     // packed kernel accepts there instead.
@@ -664,18 +700,11 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
         if (hy::launch_inter16((int)db->mode, pa, db->q16, db->waves16, s->stream) != hipSuccess)
             return HY_E_DEVICE;
     }
-    hy::SysArgs sa = sys_args(db, s, m);
-    sa.out_score = ia.out_score;
-    sa.out_qend = s->d_qend;
-    sa.out_tend = s->d_tend;
-    sa.key = ia.key;
     if (s->profile && hipEventRecord(s->ev[1], s->stream) != hipSuccess) return HY_E_DEVICE;
-    if (hy::launch_sys32((int)db->mode, ends, false, sa, (uint32_t)db->sys_list.size(),
-                         s->stream) != hipSuccess)
-        return HY_E_DEVICE;
+    if (tail && hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess) return HY_E_DEVICE;
     if (s->profile) {
-        if (hipEventRecord(s->ev[2], s->stream) != hipSuccess) return HY_E_DEVICE;
         s->pending = true;
+        s->pending_tail = tail;
     }
     return HY_OK;
 }
@@ -685,7 +714,7 @@ void collect_profile(hy_scratch* s) {
     if (!s->profile || !s->pending) return;
     float a = 0.f, b = 0.f;
     if (hipEventElapsedTime(&a, s->ev[0], s->ev[1]) == hipSuccess &&
-        hipEventElapsedTime(&b, s->ev[1], s->ev[2]) == hipSuccess) {
+        (!s->pending_tail || hipEventElapsedTime(&b, s->ev[2], s->ev[3]) == hipSuccess)) {
         s->inter_ms += a;
         s->tail_ms += b;
         s->launches += 1;

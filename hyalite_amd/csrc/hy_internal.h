@@ -113,13 +113,15 @@ hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngro
                           hipStream_t stream);
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
                         hipStream_t stream);
-// q: query rows per register tile (16, 24 or 32); waves: wavefronts per workgroup sharing one
-// pair profile (4 or 8).
+// q: query rows per register tile (16, 24 or 32); waves: 1 = one-wavefront workgroups with a
+// per-symbol profile, 4 or 8 = wavefronts per workgroup sharing one residue-pair profile.
 hipError_t launch_inter16(int mode, const Inter16Args& a, int q, int waves, hipStream_t stream);
 // Tile rows the packed kernel actually uses for a requested q.
 int inter16_tile_rows(int q);
 // LDS bytes of the pair profile for an alphabet of al_eff symbols (plus the padding symbol).
 uint32_t inter16_lds_bytes(int al_eff, int q);
+// LDS bytes of the per-symbol profile used by one-wavefront workgroups.
+uint32_t inter16p_lds_bytes(int al_eff, int q);
 hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream);
 
 }  // namespace hy

@@ -51,8 +51,11 @@ __device__ __forceinline__ int border(int i, int j, int go, int ge) {
 
 // Best::consider (src/kernel.rs:113-129): maximise score, then minimise (col, row).
 __device__ __forceinline__ void consider(int& bs, int& bj, int& bi, int s, int j, int i) {
-    const bool take = s > bs || (s == bs && (j < bj || (j == bj && i < bi)));
-    if (take) { bs = s; bj = j; bi = i; }
+    // Bitwise, not short-circuit, so the comparison stays branch-free in the cell loops.
+    const bool take = (s > bs) | ((s == bs) & ((j < bj) | ((j == bj) & (i < bi))));
+    bs = take ? s : bs;
+    bj = take ? j : bj;
+    bi = take ? i : bi;
 }
 
 // The answer-region cells that lie on the DP border, reduced in closed form. Borders never
@@ -86,7 +89,8 @@ __device__ __forceinline__ void wave_argmax(unsigned long long k, unsigned long 
         const unsigned long long o = __shfl_xor(k, off);
         k = o > k ? o : k;
     }
-    if (threadIdx.x == 0 && k != 0ull) atomicMax(dst, k);
+    // Lane 0 of every wavefront (workgroups may hold several).
+    if ((threadIdx.x & 63) == 0 && k != 0ull) atomicMax(dst, k);
 }
 
 __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
@@ -161,8 +165,26 @@ __global__ __launch_bounds__(64) void k_inter32(InterArgs a) {
         int HGd = wsub(border<MODE>(i0 - 1, 0, go, ge), go);
         int cbs = INT_MIN, cbj = 0, cbi = 0;  // tile-local best (local mode, ends)
 
+        // Column words and the previous tile's boundary values are fetched one four-column
+        // block ahead, so their HBM latency overlaps the current block's compute.
+        uint32_t w = L > 0 ? wp[0] : 0u;
+        int2 spv[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            spv[b] = (c > 0 && b + 1 <= L) ? sp[(size_t)(b + 1) * 64] : make_int2(0, 0);
         for (int jb = 0; jb < L; jb += 4) {
-            const uint32_t w = wp[(size_t)(jb >> 2) * 64];
+            uint32_t wn = 0u;
+            int2 spn[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) spn[b] = make_int2(0, 0);
+            if (jb + 4 < L) {
+                wn = wp[(size_t)((jb >> 2) + 1) * 64];
+                if (c > 0) {
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        if (jb + 5 + b <= L) spn[b] = sp[(size_t)(jb + 5 + b) * 64];
+                }
+            }
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = jb + b + 1;
@@ -172,9 +194,8 @@ __global__ __launch_bounds__(64) void k_inter32(InterArgs a) {
                     hgup = wsub(border<MODE>(0, j, go, ge), go);
                     f = kNeg32;
                 } else {
-                    const int2 t = sp[(size_t)j * 64];
-                    hgup = t.x;
-                    f = t.y;
+                    hgup = spv[b].x;
+                    f = spv[b].y;
                 }
                 const int sym = (int)((w >> (8 * b)) & 0xffu);
                 const int4* pr = reinterpret_cast<const int4*>(&prof[sym * S]);
@@ -229,6 +250,9 @@ __global__ __launch_bounds__(64) void k_inter32(InterArgs a) {
                     }
                 }
             }
+            w = wn;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) spv[b] = spn[b];
         }
         if (F::local && ENDS && cbs != INT_MIN) consider(bs, bj, bi, cbs, cbj, cbi);
     }
@@ -291,11 +315,23 @@ __device__ __forceinline__ int border16(int i, int j, int go, int ge) {
     return MF<MODE>::left ? 0 : sat16(-(long long)gap32(go, ge, i));
 }
 
-template <int MODE, int Q, int NW>
-__global__ __launch_bounds__(64 * NW) void k_inter16(Inter16Args a) {
+// PAIR = true: the residue-pair profile above, shared by NW wavefronts. PAIR = false: a plain
+// per-symbol profile of the tile (two query rows per 32-bit word), private to a one-wavefront
+// workgroup; each lane reads its two targets' rows and pairs them with one v_perm_b32 per query
+// row. The second form needs ~1 KB of LDS instead of ~70 KB (protein), so occupancy is bounded
+// by registers only and no barrier couples wavefronts.
+#ifndef HY_PACKED_WAVES_PER_EU
+#define HY_PACKED_WAVES_PER_EU 1
+#endif
+template <int MODE, int Q, int NW, bool PAIR>
+__global__ __launch_bounds__(64 * NW)
+__attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inter16Args a) {
     using F = MF<MODE>;
     extern __shared__ uint32_t pp[];
     constexpr int S = Q + 4;
+    // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
+    // different symbols start in different 4-bank groups.
+    constexpr int RD = ((Q / 2) % 8 == 4) ? Q / 2 : Q / 2 + 4;
     const int lane = threadIdx.x & 63;
     const uint32_t gix = blockIdx.x * NW + (threadIdx.x >> 6);
     const bool wlive = gix < a.ngroups;
@@ -313,6 +349,9 @@ __global__ __launch_bounds__(64 * NW) void k_inter16(Inter16Args a) {
     border_best<MODE>(m, nA, go, ge, bsA, dj, di);
     border_best<MODE>(m, nB, go, ge, bsB, dj, di);
     s2v run = splat2(-32768);
+    // Local mode keeps four independent running maxima (by row mod 4) so the tracker is not one
+    // long chain of dependent packed maxima.
+    s2v runq[4] = {run, run, run, run};
 
     const s2v go2 = splat2(go), ge2 = splat2(ge), zero2 = splat2(0);
     const int nchunks = (m + Q - 1) / Q;
@@ -324,16 +363,30 @@ __global__ __launch_bounds__(64 * NW) void k_inter16(Inter16Args a) {
         const bool lastc = c == nchunks - 1;
         const int rm = m - i0;
         __syncthreads();
-        for (int k = threadIdx.x; k < A * A * Q; k += 64 * NW) {
-            const int pr = k / Q, r = k - pr * Q, i = i0 + r;
-            const int ca = pr / A, cb = pr - ca * A;
-            int va = -32768, vb = -32768;
-            if (i <= m) {
-                const int* row = a.matrix + (int)a.query[i - 1] * a.al;
-                if (ca < a.al_eff) va = sat16((long long)row[ca] + go);
-                if (cb < a.al_eff) vb = sat16((long long)row[cb] + go);
+        if (PAIR) {
+            for (int k = threadIdx.x; k < A * A * Q; k += 64 * NW) {
+                const int pr = k / Q, r = k - pr * Q, i = i0 + r;
+                const int ca = pr / A, cb = pr - ca * A;
+                int va = -32768, vb = -32768;
+                if (i <= m) {
+                    const int* row = a.matrix + (int)a.query[i - 1] * a.al;
+                    if (ca < a.al_eff) va = sat16((long long)row[ca] + go);
+                    if (cb < a.al_eff) vb = sat16((long long)row[cb] + go);
+                }
+                pp[pr * S + r] =
+                    (uint32_t)(uint16_t)(short)va | ((uint32_t)(uint16_t)(short)vb << 16);
             }
-            pp[pr * S + r] = (uint32_t)(uint16_t)(short)va | ((uint32_t)(uint16_t)(short)vb << 16);
+        } else {
+            for (int k = threadIdx.x; k < A * (Q / 2); k += 64 * NW) {
+                const int sym = k / (Q / 2), rr = k - sym * (Q / 2);
+                const int i = i0 + 2 * rr;  // rows i (low half) and i + 1 (high half)
+                int v0 = -32768, v1 = -32768;
+                if (sym < a.al_eff) {
+                    if (i <= m) v0 = sat16((long long)a.matrix[(int)a.query[i - 1] * a.al + sym] + go);
+                    if (i + 1 <= m) v1 = sat16((long long)a.matrix[(int)a.query[i] * a.al + sym] + go);
+                }
+                pp[sym * RD + rr] = (uint32_t)(uint16_t)(short)v0 | ((uint32_t)(uint16_t)(short)v1 << 16);
+            }
         }
         __syncthreads();
         if (!wlive) continue;
@@ -346,31 +399,111 @@ __global__ __launch_bounds__(64 * NW) void k_inter16(Inter16Args a) {
         }
         s2v HGd = splat2(sat16((long long)border16<MODE>(i0 - 1, 0, go, ge) - go));
 
+        // Column words and the top input of each column (the previous tile's boundary row, or
+        // the DP border for the first tile) are fetched one four-column block ahead, so their HBM
+        // latency overlaps the current block's compute. The host pads every group's column count
+        // to a multiple of four with the inert padding symbol, so the four-column body has no
+        // exits and the packed registers stay packed across it.
+        auto fetch_block = [&](int jb, uint2* dst) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int jn = jb + b + 1;
+                if (c == 0)
+                    dst[b] = make_uint2(
+                        as_u32(splat2(sat16((long long)border16<MODE>(0, jn, go, ge) - go))),
+                        0x80008000u);
+                else
+                    dst[b] = sp[(size_t)jn * 64];
+            }
+        };
+        uint2 w = make_uint2(0, 0);
+        uint2 spv[4];
+        if (L > 0) {
+            w = wp[0];
+            fetch_block(0, spv);
+        }
         for (int jb = 0; jb < L; jb += 4) {
-            const uint2 w = wp[(size_t)(jb >> 2) * 64];
+            uint2 wn = make_uint2(0, 0);
+            uint2 spn[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) spn[b] = make_uint2(0, 0);
+            if (jb + 4 < L) {
+                wn = wp[(size_t)((jb >> 2) + 1) * 64];
+                fetch_block(jb + 4, spn);
+            }
+            uint2 outb[4];
+            // Substitution pairs of column b of the block: (sub(q_r, low target), sub(q_r, high
+            // target)) + go for the tile's rows.
+            auto load_sg = [&](int b, s2v* sg) {
+                const int ca = (int)((w.x >> (8 * b)) & 0xffu);
+                const int cb = (int)((w.y >> (8 * b)) & 0xffu);
+                if (PAIR) {
+                    const uint4* prow = reinterpret_cast<const uint4*>(&pp[(ca * A + cb) * S]);
+#pragma unroll
+                    for (int r4 = 0; r4 < Q / 4; ++r4) {
+                        const uint4 v = prow[r4];
+                        sg[4 * r4 + 0] = as_s2(v.x); sg[4 * r4 + 1] = as_s2(v.y);
+                        sg[4 * r4 + 2] = as_s2(v.z); sg[4 * r4 + 3] = as_s2(v.w);
+                    }
+                } else {
+                    const uint4* ra = reinterpret_cast<const uint4*>(&pp[ca * RD]);
+                    const uint4* rb = reinterpret_cast<const uint4*>(&pp[cb * RD]);
+#pragma unroll
+                    for (int r8 = 0; r8 < Q / 8; ++r8) {
+                        const uint4 va = ra[r8], vb = rb[r8];
+                        const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
+                        const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            // (A_r, B_r) and (A_r+1, B_r+1) from (A_r, A_r+1) and (B_r, B_r+1).
+                            sg[8 * r8 + 2 * k] =
+                                as_s2(__builtin_amdgcn_perm(wb[k], wa[k], 0x05040100u));
+                            sg[8 * r8 + 2 * k + 1] =
+                                as_s2(__builtin_amdgcn_perm(wb[k], wa[k], 0x07060302u));
+                        }
+                    }
+                }
+            };
+            // Local mode: columns are processed in pairs skewed by one row — row r of column j
+            // and row r-1 of column j+1 are independent — so two dependency chains interleave
+            // and hide the latency between dependent packed operations. The cell update and the
+            // order in which each column's cells are computed are unchanged.
+            if (F::local) {
+                auto cell = [&](int r, s2v sgv, s2v& hgd, s2v& hgup, s2v& f) {
+                    const s2v diag = p_add(hgd, sgv);
+                    const s2v e = p_max(HG[r], p_sub(E[r], ge2));
+                    f = p_max(hgup, p_sub(f, ge2));
+                    const s2v h = p_max(p_max(p_max(diag, e), f), zero2);
+                    hgd = HG[r];
+                    E[r] = e;
+                    HG[r] = p_sub(h, go2);
+                    hgup = HG[r];
+                    runq[r & 3] = p_max(runq[r & 3], HG[r]);
+                };
+#pragma unroll
+                for (int bp = 0; bp < 4; bp += 2) {
+                    s2v sgA[Q], sgB[Q];
+                    load_sg(bp, sgA);
+                    load_sg(bp + 1, sgB);
+                    s2v hgupA = as_s2(spv[bp].x), fA = as_s2(spv[bp].y);
+                    s2v hgupB = as_s2(spv[bp + 1].x), fB = as_s2(spv[bp + 1].y);
+                    s2v hgdA = HGd, hgdB = hgupA;
+                    HGd = hgupB;
+#pragma unroll
+                    for (int r = 0; r <= Q; ++r) {
+                        if (r < Q) cell(r, sgA[r], hgdA, hgupA, fA);
+                        if (r == Q - 1) outb[bp] = make_uint2(as_u32(HG[Q - 1]), as_u32(fA));
+                        if (r >= 1) cell(r - 1, sgB[r - 1], hgdB, hgupB, fB);
+                    }
+                    outb[bp + 1] = make_uint2(as_u32(HG[Q - 1]), as_u32(fB));
+                }
+            } else {
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = jb + b + 1;
-                if (j > L) break;
-                s2v hgup, f;
-                if (c == 0) {
-                    hgup = splat2(sat16((long long)border16<MODE>(0, j, go, ge) - go));
-                    f = splat2(-32768);
-                } else {
-                    const uint2 t = sp[(size_t)j * 64];
-                    hgup = as_s2(t.x);
-                    f = as_s2(t.y);
-                }
-                const int ca = (int)((w.x >> (8 * b)) & 0xffu);
-                const int cb = (int)((w.y >> (8 * b)) & 0xffu);
-                const uint4* prow = reinterpret_cast<const uint4*>(&pp[(ca * A + cb) * S]);
+                s2v hgup = as_s2(spv[b].x), f = as_s2(spv[b].y);
                 s2v sg[Q];
-#pragma unroll
-                for (int r4 = 0; r4 < Q / 4; ++r4) {
-                    const uint4 v = prow[r4];
-                    sg[4 * r4 + 0] = as_s2(v.x); sg[4 * r4 + 1] = as_s2(v.y);
-                    sg[4 * r4 + 2] = as_s2(v.z); sg[4 * r4 + 3] = as_s2(v.w);
-                }
+                load_sg(b, sg);
                 s2v hgd = HGd;
                 HGd = hgup;
 #pragma unroll
@@ -384,19 +517,18 @@ __global__ __launch_bounds__(64 * NW) void k_inter16(Inter16Args a) {
                     E[r] = e;
                     HG[r] = p_sub(h, go2);
                     hgup = HG[r];
-                    if (F::local) run = p_max(run, h);
+                    // Local mode tracks max(H - go) through the HG value that stays live anyway:
+                    // H >= 0 there, so H - go never saturates and max H = max(H - go) + go.
+                    if (F::local) run = p_max(run, HG[r]);
                 }
-                if (c + 1 < nchunks) sp[(size_t)j * 64] = make_uint2(as_u32(HG[Q - 1]), as_u32(f));
+                outb[b] = make_uint2(as_u32(HG[Q - 1]), as_u32(f));
                 if (!F::local && lastc) {
                     const bool la = F::lrow ? j <= nA : j == nA;
                     const bool lb = F::lrow ? j <= nB : j == nB;
-                    if (la || lb) {
-                        s2v v = HG[0];
+                    s2v v = HG[0];
 #pragma unroll
-                        for (int r = 1; r < Q; ++r)
-                            if (r == rm) v = HG[r];
-                        run = p_max(run, keep2(p_add(v, go2), la, lb));
-                    }
+                    for (int r = 1; r < Q; ++r) v = r == rm ? HG[r] : v;
+                    run = p_max(run, keep2(p_add(v, go2), la, lb));
                 }
                 if (F::lcol && (j == nA || j == nB)) {
 #pragma unroll
@@ -405,10 +537,20 @@ __global__ __launch_bounds__(64 * NW) void k_inter16(Inter16Args a) {
                             run = p_max(run, keep2(p_add(HG[r], go2), j == nA, j == nB));
                 }
             }
+            }
+            if (c + 1 < nchunks) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) sp[(size_t)(jb + b + 1) * 64] = outb[b];
+            }
+            w = wn;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) spv[b] = spn[b];
         }
     }
     if (!wlive) return;
-    const int sA = max(bsA, (int)run.x), sB = max(bsB, (int)run.y);
+    run = p_max(run, p_max(p_max(runq[0], runq[1]), p_max(runq[2], runq[3])));
+    const int lift = F::local ? go : 0;
+    const int sA = max(bsA, (int)run.x + lift), sB = max(bsB, (int)run.y + lift);
     unsigned long long k = 0ull;
     if (liveA) {
         const uint32_t gi = a.gidx[slotA];
@@ -427,7 +569,9 @@ __global__ __launch_bounds__(64 * NW) void k_inter16(Inter16Args a) {
 // ------------------------------------------------------------------------------------------------
 // Intra-sequence systolic kernel, 32-bit lanes, one (query, target) pair per wavefront.
 // ------------------------------------------------------------------------------------------------
-template <int MODE, int R, bool ENDS, bool WINNER>
+// LDSMAT: the substitution matrix (al_eff^2 entries) is staged in LDS; otherwise it is read from
+// global memory (alphabets above 128 symbols).
+template <int MODE, int R, bool ENDS, bool WINNER, bool LDSMAT>
 __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
     using F = MF<MODE>;
     extern __shared__ int smat[];
@@ -441,11 +585,12 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
     const uint8_t* t = a.flat + a.off[gi];
     const int n = (int)(a.off[gi + 1] - a.off[gi]);
     const int m = a.m, go = a.go, ge = a.ge, ale = a.al_eff;
-    const bool lds_mat = ale * ale <= 16384;
-    const int rowstride = lds_mat ? ale : a.al;
-    const int* mat = lds_mat ? smat : a.matrix;
-    if (lds_mat) {
-        for (int k = lane; k < ale * ale; k += 64) smat[k] = a.matrix[(k / ale) * a.al + (k % ale)];
+    const int rowstride = LDSMAT ? ale : a.al;
+    // LDS matrix rows 0..ale-1 hold the scores; row ale is the padding row (kPad32), used by query
+    // rows past the end so the lookup needs no branch.
+    if (LDSMAT) {
+        for (int k = lane; k < (ale + 1) * ale; k += 64)
+            smat[k] = k < ale * ale ? a.matrix[(k / ale) * a.al + (k % ale)] : kPad32;
     }
     __syncthreads();
 
@@ -461,7 +606,7 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = r0 + r;
-            qb[r] = i <= m ? (int)a.query[i - 1] * rowstride : -1;
+            qb[r] = i <= m ? (int)a.query[i - 1] * rowstride : (LDSMAT ? ale * ale : -1);
             HG[r] = wsub(border<MODE>(i, 0, go, ge), go);
             E[r] = kNeg32;
         }
@@ -504,7 +649,8 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
                 HGd = topHG;
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    const int sub = qb[r] >= 0 ? mat[qb[r] + c] : kPad32;
+                    const int sub = LDSMAT ? smat[qb[r] + c]
+                                           : (qb[r] >= 0 ? a.matrix[qb[r] + c] : kPad32);
                     const int diag = wadd(wadd(hgd, go), sub);
                     const int e = max(HG[r], wsub(E[r], ge));
                     f = max(hgup, wsub(f, ge));
@@ -514,22 +660,14 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
                     E[r] = e;
                     HG[r] = wsub(h, go);
                     hgup = HG[r];
+                    // Candidate cells of the mode's answer region, selected without branches;
+                    // a non-candidate offers INT_MIN, which never beats a real cell.
                     const int i = r0 + r;
-                    if (F::local) {
-                        if (i <= m) {
-                            if (ENDS) consider(bs, bj, bi, h, j, i);
-                            else bs = max(bs, h);
-                        }
-                    } else {
-                        if (F::lcol && j == n && i <= m) {
-                            if (ENDS) consider(bs, bj, bi, h, j, i);
-                            else bs = max(bs, h);
-                        }
-                        if (i == m && (F::lrow || j == n)) {
-                            if (ENDS) consider(bs, bj, bi, h, j, i);
-                            else bs = max(bs, h);
-                        }
-                    }
+                    const bool cand =
+                        i <= m && (F::local || (F::lcol && j == n) || (i == m && (F::lrow || j == n)));
+                    const int v = cand ? h : INT_MIN;
+                    if (ENDS) consider(bs, bj, bi, v, j, i);
+                    else bs = max(bs, v);
                 }
                 outHG = HG[R - 1];
                 outF = f;
@@ -592,21 +730,33 @@ hipError_t launch_inter32_m(bool ends, const InterArgs& a, uint32_t ngroups, hip
     return hipGetLastError();
 }
 
+template <int MODE, int R, bool LDSMAT>
+hipError_t launch_sys32_mrl(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                            hipStream_t st) {
+    const size_t lds = LDSMAT ? (size_t)(a.al_eff + 1) * a.al_eff * sizeof(int) : 0;
+    if (winner) {
+        if (ends)
+            hipLaunchKernelGGL((k_sys32<MODE, R, true, true, LDSMAT>), dim3(1), dim3(64), lds, st, a);
+        else
+            hipLaunchKernelGGL((k_sys32<MODE, R, false, true, LDSMAT>), dim3(1), dim3(64), lds, st,
+                               a);
+    } else {
+        if (ends)
+            hipLaunchKernelGGL((k_sys32<MODE, R, true, false, LDSMAT>), dim3(nblocks), dim3(64), lds,
+                               st, a);
+        else
+            hipLaunchKernelGGL((k_sys32<MODE, R, false, false, LDSMAT>), dim3(nblocks), dim3(64),
+                               lds, st, a);
+    }
+    return hipGetLastError();
+}
+
 template <int MODE, int R>
 hipError_t launch_sys32_mr(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
                            hipStream_t st) {
-    const size_t lds = a.al_eff * a.al_eff <= 16384 ? (size_t)a.al_eff * a.al_eff * sizeof(int) : 0;
-    if (winner) {
-        if (ends) hipLaunchKernelGGL((k_sys32<MODE, R, true, true>), dim3(1), dim3(64), lds, st, a);
-        else hipLaunchKernelGGL((k_sys32<MODE, R, false, true>), dim3(1), dim3(64), lds, st, a);
-    } else {
-        if (ends)
-            hipLaunchKernelGGL((k_sys32<MODE, R, true, false>), dim3(nblocks), dim3(64), lds, st, a);
-        else
-            hipLaunchKernelGGL((k_sys32<MODE, R, false, false>), dim3(nblocks), dim3(64), lds, st,
-                               a);
-    }
-    return hipGetLastError();
+    if ((a.al_eff + 1) * a.al_eff <= 16384)
+        return launch_sys32_mrl<MODE, R, true>(ends, winner, a, nblocks, st);
+    return launch_sys32_mrl<MODE, R, false>(ends, winner, a, nblocks, st);
 }
 
 // Rows per lane: the smallest of 2, 4, 6, 8 whose 64-lane pass covers the query (one pass up to
@@ -620,11 +770,11 @@ hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nbl
     return launch_sys32_mr<MODE, 8>(ends, winner, a, nblocks, st);
 }
 
-template <int MODE, int Q, int NW>
+template <int MODE, int Q, int NW, bool PAIR>
 hipError_t launch_inter16_qw(const Inter16Args& a, hipStream_t st) {
-    const uint32_t lds = inter16_lds_bytes(a.al_eff, Q);
+    const uint32_t lds = PAIR ? inter16_lds_bytes(a.al_eff, Q) : inter16p_lds_bytes(a.al_eff, Q);
     const uint32_t blocks = (a.ngroups + NW - 1) / NW;
-    auto kern = k_inter16<MODE, Q, NW>;
+    auto kern = k_inter16<MODE, Q, NW, PAIR>;
     if (lds > 64u * 1024u) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -635,21 +785,34 @@ hipError_t launch_inter16_qw(const Inter16Args& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+// waves: 1 = one-wavefront workgroups with the per-symbol profile; 4 or 8 = workgroups sharing
+// the residue-pair profile.
 template <int MODE>
 hipError_t launch_inter16_m(const Inter16Args& a, int q, int waves, hipStream_t st) {
-    if (waves >= 8) {
-        if (q <= 16) return launch_inter16_qw<MODE, 16, 8>(a, st);
-        if (q <= 24) return launch_inter16_qw<MODE, 24, 8>(a, st);
-        return launch_inter16_qw<MODE, 32, 8>(a, st);
+    if (waves <= 1) {
+        if (q <= 16) return launch_inter16_qw<MODE, 16, 1, false>(a, st);
+        if (q <= 24) return launch_inter16_qw<MODE, 24, 1, false>(a, st);
+        return launch_inter16_qw<MODE, 32, 1, false>(a, st);
     }
-    if (q <= 16) return launch_inter16_qw<MODE, 16, 4>(a, st);
-    if (q <= 24) return launch_inter16_qw<MODE, 24, 4>(a, st);
-    return launch_inter16_qw<MODE, 32, 4>(a, st);
+    if (waves >= 8) {
+        if (q <= 16) return launch_inter16_qw<MODE, 16, 8, true>(a, st);
+        if (q <= 24) return launch_inter16_qw<MODE, 24, 8, true>(a, st);
+        return launch_inter16_qw<MODE, 32, 8, true>(a, st);
+    }
+    if (q <= 16) return launch_inter16_qw<MODE, 16, 4, true>(a, st);
+    if (q <= 24) return launch_inter16_qw<MODE, 24, 4, true>(a, st);
+    return launch_inter16_qw<MODE, 32, 4, true>(a, st);
 }
 
 }  // namespace
 
 int inter16_tile_rows(int q) { return q <= 16 ? 16 : (q <= 24 ? 24 : 32); }
+
+uint32_t inter16p_lds_bytes(int al_eff, int q) {
+    const uint32_t rows = (uint32_t)inter16_tile_rows(q) / 2;
+    const uint32_t rd = rows % 8 == 4 ? rows : rows + 4;
+    return ((uint32_t)al_eff + 1) * rd * 4u;
+}
 
 uint32_t inter16_lds_bytes(int al_eff, int q) {
     const uint32_t A = (uint32_t)al_eff + 1;

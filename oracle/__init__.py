@@ -79,12 +79,12 @@ def _load():
                              ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32,
                              ctypes.c_uint32, ctypes.c_uint32, _i32p, _u64p, _i64p, _i64p,
                              ctypes.c_int]
-    if hasattr(lib, "hyo_avx2_scan_scores"):
-        lib.hyo_avx2_scan_scores.restype = ctypes.c_int
-        lib.hyo_avx2_scan_scores.argtypes = [_u8p, _u64p, ctypes.c_uint64, _u8p, ctypes.c_uint64,
-                                             ctypes.c_uint32, _i32p, ctypes.c_int32,
-                                             ctypes.c_int32, ctypes.c_uint32, _i32p,
-                                             ctypes.c_int, ctypes.c_int]
+    if hasattr(lib, "hyo_avx2_scan_scores_timed"):
+        lib.hyo_avx2_scan_scores_timed.restype = ctypes.c_int
+        lib.hyo_avx2_scan_scores_timed.argtypes = [
+            _u8p, _u64p, ctypes.c_uint64, _u8p, ctypes.c_uint64, ctypes.c_uint32, _i32p,
+            ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _i32p, ctypes.c_int, ctypes.c_int,
+            ctypes.POINTER(ctypes.c_double)]
     _lib = lib
     return lib
 
@@ -219,20 +219,24 @@ def scan(seqs, q, al, matrix, go, ge, mode, tracks_end, threads=1):
 
 
 def avx2_available() -> bool:
-    return hasattr(_load(), "hyo_avx2_scan_scores")
+    return hasattr(_load(), "hyo_avx2_scan_scores_timed")
 
 
 def avx2_scan_scores_flat(flat, off, q, al, matrix, go, ge, mode, threads=1, sort=False):
-    """Per-target scores from the AVX2 16-lane i16 restatement of the reference's inter-sequence
-    kernel (CPU baseline). Only i16-provable databases are accepted."""
+    """(per-target scores, scan seconds) from the AVX2 16-lane i16 restatement of the reference's
+    inter-sequence kernel (the CPU baseline). Only databases whose width proof gives <= 16 bits
+    are accepted; the per-batch tables are built before the timed scan, as the reference builds
+    them once per database."""
     n = len(off) - 1
     flat = _u8(flat) if len(flat) else np.zeros(1, dtype=np.uint8)
     off = np.ascontiguousarray(off, dtype=np.uint64)
     qa, ma = _u8(q) if len(q) else np.zeros(1, dtype=np.uint8), _i32(matrix)
     scores = np.empty(max(n, 1), dtype=np.int32)
-    rc = _load().hyo_avx2_scan_scores(_ptr(flat, _u8p), _ptr(off, _u64p), n, _ptr(qa, _u8p),
-                                      len(q), al, _ptr(ma, _i32p), go, ge, mode,
-                                      _ptr(scores, _i32p), threads, 1 if sort else 0)
+    secs = ctypes.c_double(0.0)
+    rc = _load().hyo_avx2_scan_scores_timed(_ptr(flat, _u8p), _ptr(off, _u64p), n,
+                                            _ptr(qa, _u8p), len(q), al, _ptr(ma, _i32p), go, ge,
+                                            mode, _ptr(scores, _i32p), threads,
+                                            1 if sort else 0, ctypes.byref(secs))
     if rc != OK:
         raise ValueError(f"avx2 baseline failed with status {rc}")
-    return scores[:n]
+    return scores[:n], secs.value

@@ -88,6 +88,20 @@ def test_database_sizes_around_lane_boundaries():
                 check(seqs, dna_scheme(), mode, st, queries)
 
 
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_winner_in_every_wavefront_of_a_workgroup(mode):
+    """Regression: the single-best key must collect every wavefront of a multi-wavefront
+    workgroup. The winner (an exact query copy) is planted among the shortest targets, so after
+    the length sort it lands in the last group, for each position of that group in its block."""
+    rng = np.random.default_rng(400 + mode)
+    q = rng.integers(0, 4, size=12, dtype=np.uint8)
+    for ngroups in range(1, 10):
+        n = 128 * ngroups
+        seqs = [rng.integers(0, 4, size=int(k), dtype=np.uint8) for k in rng.integers(30, 60, n)]
+        seqs[n // 3] = q.copy()
+        check(seqs, dna_scheme(), mode, hy.SearchType.Score, [q], all_targets=False)
+
+
 def test_degenerate_sequences_and_queries():
     """tests/sweeps.rs:160-194."""
     cases = [

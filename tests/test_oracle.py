@@ -319,3 +319,28 @@ def test_scan_all_threads_agree():
         best = max(range(len(a)), key=lambda k: (a[k][0], -k))
         assert orc.scan(seqs, q, al, mat, go, ge, mode, True, threads=3) == \
             (a[best][0], best, a[best][1], a[best][2])
+
+
+# ---- the AVX2 CPU baseline must agree with the scalar oracle before it is timed -----------------
+
+@pytest.mark.parametrize("sort", [False, True])
+def test_avx2_baseline_matches_scalar_oracle(sort):
+    rng = np.random.default_rng(41)
+    from common import BLOSUM62 as B62
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(0, 120, 333)]
+    flat, off = orc.flatten(seqs)
+    for q in (rng.integers(0, 20, size=90, dtype=np.uint8), np.zeros(0, dtype=np.uint8),
+              rng.integers(0, 20, size=1, dtype=np.uint8)):
+        for mode in ALL_MODES:
+            want = orc.scan_all_flat(flat, off, q, 20, B62, 11, 1, mode, ends=False, threads=3)
+            got, secs = orc.avx2_scan_scores_flat(flat, off, q, 20, B62, 11, 1, mode, threads=3,
+                                                  sort=sort)
+            assert np.array_equal(got, want), (mode, sort, len(q))
+            assert secs >= 0.0
+    al, mat, go, ge = dna_scheme()
+    for mode in ALL_MODES:
+        q = rng.integers(0, 4, size=40, dtype=np.uint8)
+        want = orc.scan_all_flat(flat % 4, off, q, al, mat, go, ge, mode, ends=False)
+        got, _ = orc.avx2_scan_scores_flat(flat % 4, off, q, al, mat, go, ge, mode, threads=2,
+                                           sort=sort)
+        assert np.array_equal(got, want), mode
