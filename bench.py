@@ -138,8 +138,9 @@ def main() -> int:
     if args.copy_calibration:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the scan would shard targets across GPUs; see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU: this build has no multi-GPU path (the scan would "
+              "shard targets across GPUs and all-reduce one 64-bit key per query)",
+              file=sys.stderr)
         return 2
 
     t0 = time.perf_counter()
@@ -159,11 +160,14 @@ def main() -> int:
 
     for _ in range(args.warmup):
         hit = db.scan(sc, q)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:  # torch is plumbing only; the scan itself synchronises its stream
-        sync = lambda: None  # noqa: E731
+    # Device-wide synchronisation around the timed region (what torch.cuda.synchronize does),
+    # called directly so the benchmark does not pay for importing torch.
+    import ctypes
+    _hip = ctypes.CDLL("libamdhip64.so")
+
+    def sync():
+        if _hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipDeviceSynchronize failed")
     sc.profile(True)
     sc.profile_read(reset=True)
     sync()

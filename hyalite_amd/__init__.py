@@ -341,6 +341,11 @@ def load_library():
     lib.hy_scratch_profile_read.restype = ctypes.c_int
     lib.hy_scratch_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double),
                                             ctypes.POINTER(ctypes.c_double), _u64p, ctypes.c_int]
+    lib.hy_scratch_reserve_batch.restype = ctypes.c_int
+    lib.hy_scratch_reserve_batch.argtypes = [vp, ctypes.c_size_t]
+    lib.hy_scratch_batch_profile_read.restype = ctypes.c_int
+    lib.hy_scratch_batch_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), _u64p,
+                                                  ctypes.c_int]
     lib.hy_strerror.restype = ctypes.c_char_p
     lib.hy_strerror.argtypes = [ctypes.c_int]
     _lib = lib
@@ -646,6 +651,19 @@ class Scratch:
         rc = load_library().hy_scratch_profile(self._h, 1 if enable else 0)
         if rc != 0:
             raise DeviceError(load_library().hy_strerror(rc).decode())
+
+    def reserve_batch(self, max_queries: int) -> None:
+        """Reserve buffers for batches of up to max_queries queries (batched form)."""
+        rc = load_library().hy_scratch_reserve_batch(self._h, max_queries)
+        if rc != 0:
+            raise DeviceError(load_library().hy_strerror(rc).decode())
+
+    def batch_profile_read(self, reset: bool = False) -> tuple[float, int]:
+        """(persistent batch kernel ms, batches timed), accumulated while profiling."""
+        a, n = ctypes.c_double(0), ctypes.c_uint64(0)
+        load_library().hy_scratch_batch_profile_read(self._h, ctypes.byref(a), ctypes.byref(n),
+                                                     1 if reset else 0)
+        return a.value, n.value
 
     def profile_read(self, reset: bool = False) -> tuple[float, float, int]:
         """(inter-kernel ms, long-target-kernel ms, scans timed), accumulated."""

@@ -126,10 +126,17 @@ struct hy_db {
     uint64_t inter16_spill = 0;    // uint2 entries
     int q16 = 24;        // query rows per register tile of the packed kernel
     int waves16 = 4;     // wavefronts per workgroup of the packed kernel
+    // Groups [0, groups16_short_first) hold long targets; one-query scans start after them.
+    uint32_t groups16_short_first = 0;
+    uint32_t max_ncols16 = 0;
     // Systolic list (long targets, or everything when the alphabet leaves no padding symbol).
     std::vector<uint32_t> sys_list;
     uint32_t* d_sys_list = nullptr;
     uint64_t sys_max_len = 0;
+    // Long targets the packed kernel cannot take: the batched form's systolic list.
+    std::vector<uint32_t> sys_list_batch;
+    uint32_t* d_sys_list_batch = nullptr;
+    uint64_t sys_batch_max_len = 0;
 
     ~hy_db() {
         (void)hipSetDevice(device);
@@ -137,6 +144,7 @@ struct hy_db {
         dev_free(d_groups); dev_free(d_words); dev_free(d_lens); dev_free(d_gidx);
         dev_free(d_groups16); dev_free(d_words16); dev_free(d_lens16); dev_free(d_gidx16);
         dev_free(d_sys_list);
+        dev_free(d_sys_list_batch);
     }
 };
 
@@ -166,6 +174,26 @@ struct hy_scratch {
     hipEvent_t fork = nullptr, join = nullptr;
     double inter_ms = 0.0, tail_ms = 0.0;
     uint64_t launches = 0;
+    // Batched form (reserved on first use or by hy_scratch_reserve_batch).
+    uint32_t bcap = 0;                   // queries per submission
+    uint32_t bq_stride = 0;              // bytes reserved per query (max_query_len, >= 1)
+    uint8_t* d_bq = nullptr;             // concatenated queries
+    uint8_t* h_bq = nullptr;             // pinned staging
+    uint32_t* d_bmeta = nullptr;         // qoff[cap] | qlen[cap] | qorder[cap] | nq | counter
+    uint32_t* h_bmeta = nullptr;         // pinned
+    unsigned long long* d_bkeys = nullptr;
+    hy::Result* d_bres = nullptr;
+    hy::Result* h_bres = nullptr;        // pinned
+    uint2* d_slots = nullptr;            // persistent kernel boundary rows
+    uint64_t slot_stride = 0;
+    uint32_t nslots = 0;
+    int2* d_bwin_spill = nullptr;        // winner pass, multi-pass queries
+    int2* d_btail_spill = nullptr;       // long-target tail, one query at a time
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double batch_ms = 0.0;               // instrumentation: persistent kernel time
+    uint64_t batch_launches = 0;
+    hipEvent_t bev[2] = {nullptr, nullptr};
 
     ~hy_scratch() {
         (void)hipSetDevice(device);
@@ -173,6 +201,15 @@ struct hy_scratch {
         dev_free(d_query); dev_free(d_spill); dev_free(d_sys_spill);
         dev_free(d_scores); dev_free(d_qend); dev_free(d_tend);
         dev_free(d_key); dev_free(d_result);
+        dev_free(d_bq); dev_free(d_bmeta); dev_free(d_bkeys); dev_free(d_bres);
+        dev_free(d_slots); dev_free(d_bwin_spill); dev_free(d_btail_spill);
+        if (h_bq) (void)hipHostFree(h_bq);
+        if (h_bmeta) (void)hipHostFree(h_bmeta);
+        if (h_bres) (void)hipHostFree(h_bres);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : bev)
+            if (e) (void)hipEventDestroy(e);
         if (h_query) (void)hipHostFree(h_query);
         if (h_result) (void)hipHostFree(h_result);
         if (aux) (void)hipStreamSynchronize(aux);
@@ -383,17 +420,28 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     // 32-bit inter layout order: [targets not eligible for the packed kernel][eligible ones],
     // each part length-sorted and starting a fresh group, so score-only scans can run the 32-bit
     // kernel over the first part alone; end-tracking scans run it over both.
+    // The packed layout holds every eligible target, long ones first in groups of their own:
+    // one-query scans skip those groups (the systolic kernel takes the long targets), while the
+    // batched form runs them in the persistent queue, where many queries keep them balanced.
     std::vector<uint32_t> inter_idx, idx16;
     inter_idx.reserve(n);
     for (uint32_t k : order_idx) {
-        if (all_systolic || lens[k] > long_threshold) {
+        const bool is_long = all_systolic || lens[k] > long_threshold;
+        if (is_long) {
             db->sys_list.push_back(k);
             db->sys_max_len = std::max<uint64_t>(db->sys_max_len, lens[k]);
+            if (elig16[k] && !all_systolic) {
+                idx16.push_back(k);
+            } else {
+                db->sys_list_batch.push_back(k);
+                db->sys_batch_max_len = std::max<uint64_t>(db->sys_batch_max_len, lens[k]);
+            }
         } else if (!elig16[k]) {
             inter_idx.push_back(k);
         }
     }
     const size_t n_not16 = inter_idx.size();
+    const size_t n16_long = idx16.size();
     for (uint32_t k : order_idx)
         if (!(all_systolic || lens[k] > long_threshold) && elig16[k]) {
             inter_idx.push_back(k);
@@ -464,17 +512,23 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
             slot_len[s] = (uint32_t)lens[idx16[s]];
         }
         uint64_t words = 0, spill = 0;
-        for (size_t s = 0; s < ns; s += 128) {
-            Group g{};
-            g.first = (uint32_t)s;
-            g.count = (uint32_t)std::min<size_t>(128, ns - s);
-            g.ncols = (slot_len[s] + 3u) & ~3u;  // whole four-column words
-            g.word_off = words;
-            g.spill_off = spill;
-            words += (uint64_t)((g.ncols + 3) / 4) * 64;
-            spill += (uint64_t)(g.ncols + 1) * 64;
-            db->groups16.push_back(g);
-        }
+        auto add_groups16 = [&](size_t lo, size_t hi) {
+            for (size_t s = lo; s < hi; s += 128) {
+                Group g{};
+                g.first = (uint32_t)s;
+                g.count = (uint32_t)std::min<size_t>(128, hi - s);
+                g.ncols = (slot_len[s] + 3u) & ~3u;  // whole four-column words
+                g.word_off = words;
+                g.spill_off = spill;
+                words += (uint64_t)((g.ncols + 3) / 4) * 64;
+                spill += (uint64_t)(g.ncols + 1) * 64;
+                db->groups16.push_back(g);
+            }
+        };
+        add_groups16(0, n16_long);
+        db->groups16_short_first = (uint32_t)db->groups16.size();
+        add_groups16(n16_long, ns);
+        db->max_ncols16 = db->groups16.empty() ? 0 : db->groups16.front().ncols;
         std::vector<uint2> w((size_t)words, make_uint2(0, 0));
         for (const Group& g : db->groups16) {
             const uint32_t nw = (g.ncols + 3) / 4;
@@ -504,6 +558,8 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
         if (dev_upload(&db->d_gidx16, slot_gidx) != hipSuccess) return fail(HY_E_DEVICE);
     }
     if (dev_upload(&db->d_sys_list, db->sys_list) != hipSuccess) return fail(HY_E_DEVICE);
+    if (dev_upload(&db->d_sys_list_batch, db->sys_list_batch) != hipSuccess)
+        return fail(HY_E_DEVICE);
     (void)rc;
     *out = db;
     return HY_OK;
@@ -680,10 +736,11 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     const uint32_t g32 = ends ? (uint32_t)db->groups.size() : db->groups_score32;
     if (hy::launch_inter32((int)db->mode, ends, ia, g32, s->stream) != hipSuccess)
         return HY_E_DEVICE;
-    if (!ends && !db->groups16.empty()) {
+    if (!ends && db->groups16.size() > db->groups16_short_first) {
         hy::Inter16Args pa{};
         pa.groups = db->d_groups16;
-        pa.ngroups = (uint32_t)db->groups16.size();
+        pa.group_first = db->groups16_short_first;
+        pa.ngroups = (uint32_t)db->groups16.size() - db->groups16_short_first;
         pa.words = db->d_words16;
         pa.lens = db->d_lens16;
         pa.gidx = db->d_gidx16;
@@ -810,20 +867,284 @@ extern "C" int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* que
     return HY_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Batched form
+// ---------------------------------------------------------------------------------------------
+
+extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
+    if (!s) return HY_E_NULL_ARGUMENT;
+    if (max_queries == 0 || max_queries > 65536) return HY_E_INVALID_ARGUMENT;
+    if (s->bcap >= max_queries) return HY_OK;
+    const hy_db* db = s->db;
+    if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    // Release a smaller reservation (and the graph captured for it).
+    dev_free(s->d_bq); dev_free(s->d_bmeta); dev_free(s->d_bkeys); dev_free(s->d_bres);
+    dev_free(s->d_slots); dev_free(s->d_bwin_spill); dev_free(s->d_btail_spill);
+    if (s->h_bq) (void)hipHostFree(s->h_bq);
+    if (s->h_bmeta) (void)hipHostFree(s->h_bmeta);
+    if (s->h_bres) (void)hipHostFree(s->h_bres);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->graph) (void)hipGraphDestroy(s->graph);
+    s->d_bq = nullptr; s->d_bmeta = nullptr; s->d_bkeys = nullptr; s->d_bres = nullptr;
+    s->d_slots = nullptr; s->d_bwin_spill = nullptr; s->d_btail_spill = nullptr;
+    s->h_bq = nullptr; s->h_bmeta = nullptr; s->h_bres = nullptr;
+    s->gexec = nullptr; s->graph = nullptr;
+    s->bcap = 0;
+
+    const uint32_t cap = (uint32_t)max_queries;
+    const uint32_t stride = (uint32_t)std::max<uint64_t>(db->mql, 1);
+    const size_t qbytes = (size_t)cap * stride;
+    const size_t meta = (size_t)3 * cap + 2;
+    if (hipMalloc((void**)&s->d_bq, qbytes) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_bq, qbytes) != hipSuccess ||
+        hipMalloc((void**)&s->d_bmeta, meta * sizeof(uint32_t)) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_bmeta, meta * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc((void**)&s->d_bkeys, cap * sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void**)&s->d_bres, cap * sizeof(hy::Result)) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_bres, cap * sizeof(hy::Result)) != hipSuccess)
+        return HY_E_OUT_OF_MEMORY;
+    std::memset(s->h_bq, 0, qbytes);
+    std::memset(s->h_bmeta, 0, meta * sizeof(uint32_t));
+    if (!db->groups16.empty()) {
+        s->nslots = hy::inter16_batch_workgroups((int)db->mode, db->q16, (int)db->al_eff);
+        s->slot_stride = (uint64_t)(db->max_ncols16 + 1) * 64;
+        if (s->nslots == 0 ||
+            hipMalloc((void**)&s->d_slots, (size_t)s->nslots * s->slot_stride * sizeof(uint2)) !=
+                hipSuccess)
+            return HY_E_OUT_OF_MEMORY;
+    }
+    const bool multipass = db->mql > 512;
+    if (multipass) {
+        if (hipMalloc((void**)&s->d_bwin_spill, (size_t)cap * (db->mtl + 1) * sizeof(int2)) !=
+            hipSuccess)
+            return HY_E_OUT_OF_MEMORY;
+        if (!db->sys_list_batch.empty() &&
+            hipMalloc((void**)&s->d_btail_spill, db->sys_list_batch.size() *
+                                                     (db->sys_batch_max_len + 1) * sizeof(int2)) !=
+                hipSuccess)
+            return HY_E_OUT_OF_MEMORY;
+    }
+    for (auto& e : s->bev)
+        if (!e && hipEventCreate(&e) != hipSuccess) return HY_E_DEVICE;
+    s->bq_stride = stride;
+    s->bcap = cap;
+    return HY_OK;
+}
+
+namespace {
+
+// Enqueue one batched submission of up to bcap queries whose bytes and metadata are already in
+// the pinned staging buffers. Every shape is fixed by the reservation, so the same sequence can
+// be captured once as a graph and replayed: per-batch values live in device memory.
+int enqueue_batch(const hy_db* db, hy_scratch* s, bool timed) {
+    const uint32_t cap = s->bcap;
+    uint32_t* d_qoff = s->d_bmeta;
+    uint32_t* d_qlen = s->d_bmeta + cap;
+    uint32_t* d_qorder = s->d_bmeta + 2 * cap;
+    uint32_t* d_nq = s->d_bmeta + 3 * cap;
+    unsigned int* d_counter = s->d_bmeta + 3 * cap + 1;
+    const hipStream_t st = s->stream;
+    if (hipMemcpyAsync(s->d_bq, s->h_bq, (size_t)cap * s->bq_stride, hipMemcpyHostToDevice, st) !=
+            hipSuccess ||
+        hipMemcpyAsync(s->d_bmeta, s->h_bmeta, ((size_t)3 * cap + 1) * sizeof(uint32_t),
+                       hipMemcpyHostToDevice, st) != hipSuccess)
+        return HY_E_DEVICE;
+    if (hy::launch_batch_reset(s->d_bkeys, cap, d_counter, st) != hipSuccess) return HY_E_DEVICE;
+
+    // Long targets the packed kernel cannot take: one systolic launch per query slot on the
+    // auxiliary stream (slots past the batch size exit at once).
+    const bool tail = !db->sys_list_batch.empty();
+    if (tail) {
+        if (hipEventRecord(s->fork, st) != hipSuccess ||
+            hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess)
+            return HY_E_DEVICE;
+        hy::SysArgs sa{};
+        sa.flat = db->d_flat;
+        sa.off = db->d_off;
+        sa.list = db->d_sys_list_batch;
+        sa.m = (int)db->mql;
+        sa.matrix = db->d_matrix;
+        sa.al = (int)db->al;
+        sa.al_eff = (int)db->al_eff;
+        sa.go = db->go;
+        sa.ge = db->ge;
+        sa.spill = s->d_btail_spill;
+        sa.spill_stride = db->sys_batch_max_len + 1;
+        sa.key = s->d_bkeys;
+        sa.queries = s->d_bq;
+        sa.qoff = d_qoff;
+        sa.qlen = d_qlen;
+        sa.nq = d_nq;
+        for (uint32_t k = 0; k < cap; ++k) {
+            sa.qsel = (int)k;
+            if (hy::launch_sys32((int)db->mode, false, false, sa,
+                                 (uint32_t)db->sys_list_batch.size(), s->aux) != hipSuccess)
+                return HY_E_DEVICE;
+        }
+        if (hipEventRecord(s->join, s->aux) != hipSuccess) return HY_E_DEVICE;
+    }
+
+    if (!db->groups16.empty()) {
+        hy::Inter16Args pa{};
+        pa.groups = db->d_groups16;
+        pa.group_first = 0;
+        pa.ngroups = (uint32_t)db->groups16.size();
+        pa.words = db->d_words16;
+        pa.lens = db->d_lens16;
+        pa.gidx = db->d_gidx16;
+        pa.matrix = db->d_matrix;
+        pa.al = (int)db->al;
+        pa.al_eff = (int)db->al_eff;
+        pa.go = db->go;
+        pa.ge = db->ge;
+        hy::BatchArgs b{};
+        b.queries = s->d_bq;
+        b.qoff = d_qoff;
+        b.qlen = d_qlen;
+        b.qorder = d_qorder;
+        b.nq = d_nq;
+        b.counter = d_counter;
+        b.keys = s->d_bkeys;
+        b.slots = s->d_slots;
+        b.slot_stride = s->slot_stride;
+        if (timed && hipEventRecord(s->bev[0], st) != hipSuccess) return HY_E_DEVICE;
+        if (hy::launch_inter16_batch((int)db->mode, pa, b, db->q16, s->nslots, st) != hipSuccess)
+            return HY_E_DEVICE;
+        if (timed && hipEventRecord(s->bev[1], st) != hipSuccess) return HY_E_DEVICE;
+    }
+    if (tail && hipStreamWaitEvent(st, s->join, 0) != hipSuccess) return HY_E_DEVICE;
+
+    if (db->search_type == HY_SEARCH_SCORE_END) {
+        hy::SysArgs sa{};
+        sa.flat = db->d_flat;
+        sa.off = db->d_off;
+        sa.winner_key = s->d_bkeys;
+        sa.m = (int)db->mql;
+        sa.matrix = db->d_matrix;
+        sa.al = (int)db->al;
+        sa.al_eff = (int)db->al_eff;
+        sa.go = db->go;
+        sa.ge = db->ge;
+        sa.spill = s->d_bwin_spill;
+        sa.spill_stride = db->mtl + 1;
+        sa.result = s->d_bres;
+        sa.queries = s->d_bq;
+        sa.qoff = d_qoff;
+        sa.qlen = d_qlen;
+        sa.nq = d_nq;
+        sa.qsel = -1;
+        if (hy::launch_sys32((int)db->mode, true, true, sa, 1, st, cap) != hipSuccess)
+            return HY_E_DEVICE;
+    } else {
+        if (hy::launch_key_results(s->d_bkeys, s->d_bres, d_nq, cap, st) != hipSuccess)
+            return HY_E_DEVICE;
+    }
+    if (hipMemcpyAsync(s->h_bres, s->d_bres, (size_t)cap * sizeof(hy::Result),
+                       hipMemcpyDeviceToHost, st) != hipSuccess)
+        return HY_E_DEVICE;
+    return HY_OK;
+}
+
+}  // namespace
+
 extern "C" int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
                              const size_t* qlens, size_t nq, int32_t* scores,
                              uint64_t* db_indices, int64_t* query_ends, int64_t* target_ends) {
     if (!db || !s || (nq && (!queries || !qlens || !scores || !db_indices || !query_ends ||
                              !target_ends)))
         return HY_E_NULL_ARGUMENT;
+    if (s->db != db) return HY_E_INVALID_ARGUMENT;
     for (size_t k = 0; k < nq; ++k) {
         const int rc = check_query(db, queries[k], qlens[k]);
         if (rc != HY_OK) return rc;
     }
-    for (size_t k = 0; k < nq; ++k) {
-        const int rc = hy_scan(db, s, queries[k], qlens[k], &scores[k], &db_indices[k],
-                               &query_ends[k], &target_ends[k]);
-        if (rc != HY_OK) return rc;
+    if (nq == 0) return HY_OK;
+    // Targets not eligible for the packed kernel and not long (a 32-bit inter partition) have no
+    // batched kernel: such databases run the queries one scan at a time.
+    if (db->groups_score32 > 0) {
+        for (size_t k = 0; k < nq; ++k) {
+            const int rc = hy_scan(db, s, queries[k], qlens[k], &scores[k], &db_indices[k],
+                                   &query_ends[k], &target_ends[k]);
+            if (rc != HY_OK) return rc;
+        }
+        return HY_OK;
+    }
+    int rc = HY_OK;
+    if (s->bcap < std::min<size_t>(nq, 256) &&
+        (rc = hy_scratch_reserve_batch(s, std::min<size_t>(nq, 256))) != HY_OK)
+        return rc;
+    if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
+    const uint32_t cap = s->bcap;
+    std::vector<uint32_t> order;
+    for (size_t base = 0; base < nq; base += cap) {
+        const uint32_t cnt = (uint32_t)std::min<size_t>(cap, nq - base);
+        uint32_t* qoff = s->h_bmeta;
+        uint32_t* qlen = s->h_bmeta + cap;
+        uint32_t* qord = s->h_bmeta + 2 * cap;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            qoff[k] = k * s->bq_stride;
+            qlen[k] = (uint32_t)qlens[base + k];
+            if (qlen[k]) std::memcpy(s->h_bq + qoff[k], queries[base + k], qlen[k]);
+        }
+        order.resize(cnt);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(),
+                         [&](uint32_t a, uint32_t b) { return qlen[a] > qlen[b]; });
+        for (uint32_t k = 0; k < cnt; ++k) qord[k] = order[k];
+        s->h_bmeta[3 * cap] = cnt;
+        // Profiling runs the sequence directly (with events); otherwise the first submission is
+        // captured as a graph and every later one replays it.
+        if (s->profile) {
+            if ((rc = enqueue_batch(db, s, true)) != HY_OK) return rc;
+        } else {
+            if (!s->gexec) {
+                if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                    return HY_E_DEVICE;
+                rc = enqueue_batch(db, s, false);
+                hipGraph_t g = nullptr;
+                const hipError_t e = hipStreamEndCapture(s->stream, &g);
+                if (rc != HY_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                if (e != hipSuccess) return HY_E_DEVICE;
+                s->graph = g;
+                if (hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+                    s->gexec = nullptr;
+                    return HY_E_DEVICE;
+                }
+            }
+            if (hipGraphLaunch(s->gexec, s->stream) != hipSuccess) return HY_E_DEVICE;
+        }
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+        if (s->profile && !db->groups16.empty()) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, s->bev[0], s->bev[1]) == hipSuccess) {
+                s->batch_ms += ms;
+                s->batch_launches += 1;
+            }
+        }
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const hy::Result& r = s->h_bres[k];
+            if (r.check != 0) return HY_E_INTERNAL;
+            scores[base + k] = r.score;
+            db_indices[base + k] = r.db_index;
+            query_ends[base + k] = r.query_end;
+            target_ends[base + k] = r.target_end;
+        }
+    }
+    return HY_OK;
+}
+
+extern "C" int hy_scratch_batch_profile_read(hy_scratch* s, double* kernel_ms, uint64_t* launches,
+                                             int reset) {
+    if (!s) return HY_E_NULL_ARGUMENT;
+    if (kernel_ms) *kernel_ms = s->batch_ms;
+    if (launches) *launches = s->batch_launches;
+    if (reset) {
+        s->batch_ms = 0.0;
+        s->batch_launches = 0;
     }
     return HY_OK;
 }

@@ -70,7 +70,8 @@ struct InterArgs {
 // odd slot of the lane's pair), 128 targets per group.
 struct Inter16Args {
     const Group* groups;     // count <= 128
-    uint32_t ngroups;
+    uint32_t group_first;    // first group of the launch
+    uint32_t ngroups;        // groups of the launch
     const uint2* words;      // words[word_off + (col/4)*64 + lane] = (low target, high target)
     const uint32_t* lens;    // per slot
     const uint32_t* gidx;    // per slot -> database index
@@ -82,6 +83,19 @@ struct Inter16Args {
     uint2* spill;            // spill[spill_off + col*64 + lane] = (H - go pair, F pair)
     int32_t* out_score;      // database order (may be null)
     unsigned long long* key; // fused argmax (may be null)
+};
+
+// Batched form: queries concatenated in one device buffer, processed by the persistent kernel.
+struct BatchArgs {
+    const uint8_t* queries;     // concatenated encoded queries
+    const uint32_t* qoff;       // query k starts at queries + qoff[k]
+    const uint32_t* qlen;
+    const uint32_t* qorder;     // query indices by descending length
+    const uint32_t* nq;         // number of queries (device memory, so a graph can be replayed)
+    unsigned int* counter;      // work-queue head, zeroed before each launch
+    unsigned long long* keys;   // per-query argmax keys, zeroed before each launch
+    uint2* slots;               // boundary rows: one slot of slot_stride entries per workgroup
+    uint64_t slot_stride;
 };
 
 // Largest pair-profile footprint the packed kernel stages in LDS (bytes).
@@ -104,6 +118,13 @@ struct SysArgs {
     int64_t* out_tend;
     unsigned long long* key;  // list mode: fused argmax key (may be null)
     Result* result;           // winner mode output
+    // Batched form (qoff != null): blockIdx.y selects the query; key, winner_key and result are
+    // per-query arrays, and blocks with blockIdx.y >= *nq exit at once.
+    const uint8_t* queries;
+    const uint32_t* qoff;
+    const uint32_t* qlen;
+    const uint32_t* nq;
+    int qsel;                 // >= 0: every block uses query qsel instead of blockIdx.y
 };
 
 // Launchers (hy_kernels.hip). All launch on `stream` and never synchronise.
@@ -111,11 +132,17 @@ hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
                            hipStream_t stream);
 hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngroups,
                           hipStream_t stream);
+// nq_grid > 1: batched form, a grid of nblocks x nq_grid blocks; a.m is then the longest query
+// of the batch (it fixes the rows per lane).
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
-                        hipStream_t stream);
+                        hipStream_t stream, uint32_t nq_grid = 1);
 // q: query rows per register tile (16, 24 or 32); waves: 1 = one-wavefront workgroups with a
 // per-symbol profile, 4 or 8 = wavefronts per workgroup sharing one residue-pair profile.
 hipError_t launch_inter16(int mode, const Inter16Args& a, int q, int waves, hipStream_t stream);
+// Persistent batch kernel (one-wavefront workgroups, per-symbol profile): `workgroups` resident
+// workgroups drain nq * a.ngroups (group, query) items.
+hipError_t launch_inter16_batch(int mode, const Inter16Args& a, const BatchArgs& b, int q,
+                                uint32_t workgroups, hipStream_t stream);
 // Tile rows the packed kernel actually uses for a requested q.
 int inter16_tile_rows(int q);
 // LDS bytes of the pair profile for an alphabet of al_eff symbols (plus the padding symbol).
@@ -123,5 +150,12 @@ uint32_t inter16_lds_bytes(int al_eff, int q);
 // LDS bytes of the per-symbol profile used by one-wavefront workgroups.
 uint32_t inter16p_lds_bytes(int al_eff, int q);
 hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream);
+// Batched form: zero n keys and the queue counter; write a Result per query from its key.
+hipError_t launch_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter,
+                              hipStream_t stream);
+hipError_t launch_key_results(const unsigned long long* keys, Result* r, const uint32_t* nq,
+                              uint32_t cap, hipStream_t stream);
+// Resident one-wavefront workgroups of the persistent batch kernel on this device.
+uint32_t inter16_batch_workgroups(int mode, int q, int al_eff);
 
 }  // namespace hy

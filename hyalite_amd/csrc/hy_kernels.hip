@@ -98,6 +98,25 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     if (i < n) p[i] = v;
 }
 
+__global__ void k_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = 0ull;
+    if (i == 0) *counter = 0u;
+}
+
+__global__ void k_key_results(const unsigned long long* keys, Result* r, const uint32_t* nq,
+                              uint32_t cap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cap || i >= *nq) return;
+    const unsigned long long k = keys[i];
+    r[i].score = key_score(k);
+    r[i].db_index = key_index(k);
+    r[i].query_end = -1;
+    r[i].target_end = -1;
+    r[i].check = k == 0ull ? 1 : 0;
+    r[i].pad = 0;
+}
+
 __global__ void k_key_result(const unsigned long long* key, Result* r) {
     if (threadIdx.x == 0) {
         const unsigned long long k = *key;
@@ -323,232 +342,272 @@ __device__ __forceinline__ int border16(int i, int j, int go, int ge) {
 #ifndef HY_PACKED_WAVES_PER_EU
 #define HY_PACKED_WAVES_PER_EU 1
 #endif
-template <int MODE, int Q, int NW, bool PAIR>
-__global__ __launch_bounds__(64 * NW)
-__attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inter16Args a) {
-    using F = MF<MODE>;
-    extern __shared__ uint32_t pp[];
-    constexpr int S = Q + 4;
-    // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
-    // different symbols start in different 4-bank groups.
-    constexpr int RD = ((Q / 2) % 8 == 4) ? Q / 2 : Q / 2 + 4;
-    const int lane = threadIdx.x & 63;
-    const uint32_t gix = blockIdx.x * NW + (threadIdx.x >> 6);
-    const bool wlive = gix < a.ngroups;
-    Group g{};
-    if (wlive) g = a.groups[gix];
-    const bool liveA = 2 * lane < (int)g.count, liveB = 2 * lane + 1 < (int)g.count;
-    const uint32_t slotA = g.first + 2 * lane, slotB = slotA + 1;
-    const int nA = liveA ? (int)a.lens[slotA] : 0;
-    const int nB = liveB ? (int)a.lens[slotB] : 0;
-    const int L = (int)g.ncols;
-    const int m = a.m, go = a.go, ge = a.ge;
-    const int A = a.al_eff + 1;
-
-    int bsA, bsB, dj, di;
-    border_best<MODE>(m, nA, go, ge, bsA, dj, di);
-    border_best<MODE>(m, nB, go, ge, bsB, dj, di);
-    s2v run = splat2(-32768);
+// Running state of one wavefront's two-targets-per-lane pass over one group, carried across the
+// query tiles.
+struct PairRun {
+    s2v run;
     // Local mode keeps four independent running maxima (by row mod 4) so the tracker is not one
     // long chain of dependent packed maxima.
-    s2v runq[4] = {run, run, run, run};
+    s2v runq[4];
+};
 
+// One tile of QT query rows starting at row i0 over every column of the group: the profile of
+// those rows is staged in LDS, the rows' H/E columns live in registers, and the tile's bottom row
+// goes to `sp` (column j of this wavefront at sp[j * 64]) for the next tile unless this is the last.
+template <int MODE, int QT, int NW, bool PAIR>
+__device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& g, bool wlive,
+                                             const uint8_t* query, int m, int i0, bool firstc,
+                                             bool lastc, int nA, int nB, uint2* sp, const uint2* wp,
+                                             uint32_t* pp, PairRun& st) {
+    using F = MF<MODE>;
+    constexpr int S = QT + 4;
+    // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
+    // different symbols start in different 4-bank groups.
+    constexpr int RD = ((QT / 2) % 8 == 4) ? QT / 2 : QT / 2 + 4;
+    const int L = (int)g.ncols;
+    const int go = a.go, ge = a.ge;
+    const int A = a.al_eff + 1;
+    const int rm = m - i0;
     const s2v go2 = splat2(go), ge2 = splat2(ge), zero2 = splat2(0);
-    const int nchunks = (m + Q - 1) / Q;
-    uint2* sp = a.spill + g.spill_off + lane;
-    const uint2* wp = a.words + g.word_off + lane;
-
-    for (int c = 0; c < nchunks; ++c) {
-        const int i0 = c * Q + 1;
-        const bool lastc = c == nchunks - 1;
-        const int rm = m - i0;
-        __syncthreads();
-        if (PAIR) {
-            for (int k = threadIdx.x; k < A * A * Q; k += 64 * NW) {
-                const int pr = k / Q, r = k - pr * Q, i = i0 + r;
-                const int ca = pr / A, cb = pr - ca * A;
-                int va = -32768, vb = -32768;
-                if (i <= m) {
-                    const int* row = a.matrix + (int)a.query[i - 1] * a.al;
-                    if (ca < a.al_eff) va = sat16((long long)row[ca] + go);
-                    if (cb < a.al_eff) vb = sat16((long long)row[cb] + go);
-                }
-                pp[pr * S + r] =
-                    (uint32_t)(uint16_t)(short)va | ((uint32_t)(uint16_t)(short)vb << 16);
+    __syncthreads();
+    if (PAIR) {
+        for (int k = threadIdx.x; k < A * A * QT; k += 64 * NW) {
+            const int pr = k / QT, r = k - pr * QT, i = i0 + r;
+            const int ca = pr / A, cb = pr - ca * A;
+            int va = -32768, vb = -32768;
+            if (i <= m) {
+                const int* row = a.matrix + (int)query[i - 1] * a.al;
+                if (ca < a.al_eff) va = sat16((long long)row[ca] + go);
+                if (cb < a.al_eff) vb = sat16((long long)row[cb] + go);
             }
-        } else {
-            for (int k = threadIdx.x; k < A * (Q / 2); k += 64 * NW) {
-                const int sym = k / (Q / 2), rr = k - sym * (Q / 2);
-                const int i = i0 + 2 * rr;  // rows i (low half) and i + 1 (high half)
-                int v0 = -32768, v1 = -32768;
-                if (sym < a.al_eff) {
-                    if (i <= m) v0 = sat16((long long)a.matrix[(int)a.query[i - 1] * a.al + sym] + go);
-                    if (i + 1 <= m) v1 = sat16((long long)a.matrix[(int)a.query[i] * a.al + sym] + go);
-                }
-                pp[sym * RD + rr] = (uint32_t)(uint16_t)(short)v0 | ((uint32_t)(uint16_t)(short)v1 << 16);
-            }
+            pp[pr * S + r] =
+                (uint32_t)(uint16_t)(short)va | ((uint32_t)(uint16_t)(short)vb << 16);
         }
-        __syncthreads();
-        if (!wlive) continue;
-
-        s2v HG[Q], E[Q];
-#pragma unroll
-        for (int r = 0; r < Q; ++r) {
-            HG[r] = splat2(sat16((long long)border16<MODE>(i0 + r, 0, go, ge) - go));
-            E[r] = splat2(-32768);
-        }
-        s2v HGd = splat2(sat16((long long)border16<MODE>(i0 - 1, 0, go, ge) - go));
-
-        // Column words and the top input of each column (the previous tile's boundary row, or
-        // the DP border for the first tile) are fetched one four-column block ahead, so their HBM
-        // latency overlaps the current block's compute. The host pads every group's column count
-        // to a multiple of four with the inert padding symbol, so the four-column body has no
-        // exits and the packed registers stay packed across it.
-        auto fetch_block = [&](int jb, uint2* dst) {
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int jn = jb + b + 1;
-                if (c == 0)
-                    dst[b] = make_uint2(
-                        as_u32(splat2(sat16((long long)border16<MODE>(0, jn, go, ge) - go))),
-                        0x80008000u);
-                else
-                    dst[b] = sp[(size_t)jn * 64];
+    } else {
+        for (int k = threadIdx.x; k < A * (QT / 2); k += 64 * NW) {
+            const int sym = k / (QT / 2), rr = k - sym * (QT / 2);
+            const int i = i0 + 2 * rr;  // rows i (low half) and i + 1 (high half)
+            int v0 = -32768, v1 = -32768;
+            if (sym < a.al_eff) {
+                if (i <= m) v0 = sat16((long long)a.matrix[(int)query[i - 1] * a.al + sym] + go);
+                if (i + 1 <= m) v1 = sat16((long long)a.matrix[(int)query[i] * a.al + sym] + go);
             }
-        };
-        uint2 w = make_uint2(0, 0);
-        uint2 spv[4];
-        if (L > 0) {
-            w = wp[0];
-            fetch_block(0, spv);
+            pp[sym * RD + rr] = (uint32_t)(uint16_t)(short)v0 | ((uint32_t)(uint16_t)(short)v1 << 16);
         }
-        for (int jb = 0; jb < L; jb += 4) {
-            uint2 wn = make_uint2(0, 0);
-            uint2 spn[4];
+    }
+    __syncthreads();
+    if (!wlive) return;
+
+    s2v HG[QT], E[QT];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) spn[b] = make_uint2(0, 0);
-            if (jb + 4 < L) {
-                wn = wp[(size_t)((jb >> 2) + 1) * 64];
-                fetch_block(jb + 4, spn);
-            }
-            uint2 outb[4];
-            // Substitution pairs of column b of the block: (sub(q_r, low target), sub(q_r, high
-            // target)) + go for the tile's rows.
-            auto load_sg = [&](int b, s2v* sg) {
-                const int ca = (int)((w.x >> (8 * b)) & 0xffu);
-                const int cb = (int)((w.y >> (8 * b)) & 0xffu);
-                if (PAIR) {
-                    const uint4* prow = reinterpret_cast<const uint4*>(&pp[(ca * A + cb) * S]);
+    for (int r = 0; r < QT; ++r) {
+        HG[r] = splat2(sat16((long long)border16<MODE>(i0 + r, 0, go, ge) - go));
+        E[r] = splat2(-32768);
+    }
+    s2v HGd = splat2(sat16((long long)border16<MODE>(i0 - 1, 0, go, ge) - go));
+    s2v run = st.run;
+    s2v runq[4] = {st.runq[0], st.runq[1], st.runq[2], st.runq[3]};
+
+    // Column words and the top input of each column (the previous tile's boundary row, or the DP
+    // border for the first tile) are fetched one four-column block ahead, so their HBM latency
+    // overlaps the current block's compute. The host pads every group's column count to a
+    // multiple of four with the inert padding symbol, so the four-column body has no exits and
+    // the packed registers stay packed across it.
+    auto fetch_block = [&](int jb, uint2* dst) {
 #pragma unroll
-                    for (int r4 = 0; r4 < Q / 4; ++r4) {
-                        const uint4 v = prow[r4];
-                        sg[4 * r4 + 0] = as_s2(v.x); sg[4 * r4 + 1] = as_s2(v.y);
-                        sg[4 * r4 + 2] = as_s2(v.z); sg[4 * r4 + 3] = as_s2(v.w);
-                    }
-                } else {
-                    const uint4* ra = reinterpret_cast<const uint4*>(&pp[ca * RD]);
-                    const uint4* rb = reinterpret_cast<const uint4*>(&pp[cb * RD]);
+        for (int b = 0; b < 4; ++b) {
+            const int jn = jb + b + 1;
+            if (firstc)
+                dst[b] = make_uint2(
+                    as_u32(splat2(sat16((long long)border16<MODE>(0, jn, go, ge) - go))),
+                    0x80008000u);
+            else
+                dst[b] = sp[(size_t)jn * 64];
+        }
+    };
+    uint2 w = make_uint2(0, 0);
+    uint2 spv[4];
+    if (L > 0) {
+        w = wp[0];
+        fetch_block(0, spv);
+    }
+    for (int jb = 0; jb < L; jb += 4) {
+        uint2 wn = make_uint2(0, 0);
+        uint2 spn[4];
 #pragma unroll
-                    for (int r8 = 0; r8 < Q / 8; ++r8) {
-                        const uint4 va = ra[r8], vb = rb[r8];
-                        const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
-                        const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
+        for (int b = 0; b < 4; ++b) spn[b] = make_uint2(0, 0);
+        if (jb + 4 < L) {
+            wn = wp[(size_t)((jb >> 2) + 1) * 64];
+            fetch_block(jb + 4, spn);
+        }
+        uint2 outb[4];
+        // Substitution pairs of column b of the block: (sub(q_r, low target), sub(q_r, high
+        // target)) + go for the tile's rows.
+        auto load_sg = [&](int b, s2v* sg) {
+            const int ca = (int)((w.x >> (8 * b)) & 0xffu);
+            const int cb = (int)((w.y >> (8 * b)) & 0xffu);
+            if (PAIR) {
+                const uint4* prow = reinterpret_cast<const uint4*>(&pp[(ca * A + cb) * S]);
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            // (A_r, B_r) and (A_r+1, B_r+1) from (A_r, A_r+1) and (B_r, B_r+1).
-                            sg[8 * r8 + 2 * k] =
-                                as_s2(__builtin_amdgcn_perm(wb[k], wa[k], 0x05040100u));
-                            sg[8 * r8 + 2 * k + 1] =
-                                as_s2(__builtin_amdgcn_perm(wb[k], wa[k], 0x07060302u));
-                        }
-                    }
-                }
-            };
-            // Local mode: columns are processed in pairs skewed by one row — row r of column j
-            // and row r-1 of column j+1 are independent — so two dependency chains interleave
-            // and hide the latency between dependent packed operations. The cell update and the
-            // order in which each column's cells are computed are unchanged.
-            if (F::local) {
-                auto cell = [&](int r, s2v sgv, s2v& hgd, s2v& hgup, s2v& f) {
-                    const s2v diag = p_add(hgd, sgv);
-                    const s2v e = p_max(HG[r], p_sub(E[r], ge2));
-                    f = p_max(hgup, p_sub(f, ge2));
-                    const s2v h = p_max(p_max(p_max(diag, e), f), zero2);
-                    hgd = HG[r];
-                    E[r] = e;
-                    HG[r] = p_sub(h, go2);
-                    hgup = HG[r];
-                    runq[r & 3] = p_max(runq[r & 3], HG[r]);
-                };
-#pragma unroll
-                for (int bp = 0; bp < 4; bp += 2) {
-                    s2v sgA[Q], sgB[Q];
-                    load_sg(bp, sgA);
-                    load_sg(bp + 1, sgB);
-                    s2v hgupA = as_s2(spv[bp].x), fA = as_s2(spv[bp].y);
-                    s2v hgupB = as_s2(spv[bp + 1].x), fB = as_s2(spv[bp + 1].y);
-                    s2v hgdA = HGd, hgdB = hgupA;
-                    HGd = hgupB;
-#pragma unroll
-                    for (int r = 0; r <= Q; ++r) {
-                        if (r < Q) cell(r, sgA[r], hgdA, hgupA, fA);
-                        if (r == Q - 1) outb[bp] = make_uint2(as_u32(HG[Q - 1]), as_u32(fA));
-                        if (r >= 1) cell(r - 1, sgB[r - 1], hgdB, hgupB, fB);
-                    }
-                    outb[bp + 1] = make_uint2(as_u32(HG[Q - 1]), as_u32(fB));
+                for (int r4 = 0; r4 < QT / 4; ++r4) {
+                    const uint4 v = prow[r4];
+                    sg[4 * r4 + 0] = as_s2(v.x); sg[4 * r4 + 1] = as_s2(v.y);
+                    sg[4 * r4 + 2] = as_s2(v.z); sg[4 * r4 + 3] = as_s2(v.w);
                 }
             } else {
+                const uint4* ra = reinterpret_cast<const uint4*>(&pp[ca * RD]);
+                const uint4* rb = reinterpret_cast<const uint4*>(&pp[cb * RD]);
+#pragma unroll
+                for (int r8 = 0; r8 < QT / 8; ++r8) {
+                    const uint4 va = ra[r8], vb = rb[r8];
+                    const uint32_t wa[4] = {va.x, va.y, va.z, va.w};
+                    const uint32_t wb[4] = {vb.x, vb.y, vb.z, vb.w};
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        // (A_r, B_r) and (A_r+1, B_r+1) from (A_r, A_r+1) and (B_r, B_r+1).
+                        sg[8 * r8 + 2 * k] =
+                            as_s2(__builtin_amdgcn_perm(wb[k], wa[k], 0x05040100u));
+                        sg[8 * r8 + 2 * k + 1] =
+                            as_s2(__builtin_amdgcn_perm(wb[k], wa[k], 0x07060302u));
+                    }
+                }
+            }
+        };
+        // Local mode: columns are processed in pairs skewed by one row — row r of column j and
+        // row r-1 of column j+1 are independent — so two dependency chains interleave and hide
+        // the latency between dependent packed operations. The cell update and the order in
+        // which each column's cells are computed are unchanged.
+        if (F::local) {
+            auto cell = [&](int r, s2v sgv, s2v& hgd, s2v& hgup, s2v& f) {
+                const s2v diag = p_add(hgd, sgv);
+                const s2v e = p_max(HG[r], p_sub(E[r], ge2));
+                f = p_max(hgup, p_sub(f, ge2));
+                const s2v h = p_max(p_max(p_max(diag, e), f), zero2);
+                hgd = HG[r];
+                E[r] = e;
+                HG[r] = p_sub(h, go2);
+                hgup = HG[r];
+                runq[r & 3] = p_max(runq[r & 3], HG[r]);
+            };
+#pragma unroll
+            for (int bp = 0; bp < 4; bp += 2) {
+                s2v sgA[QT], sgB[QT];
+                load_sg(bp, sgA);
+                load_sg(bp + 1, sgB);
+                s2v hgupA = as_s2(spv[bp].x), fA = as_s2(spv[bp].y);
+                s2v hgupB = as_s2(spv[bp + 1].x), fB = as_s2(spv[bp + 1].y);
+                s2v hgdA = HGd, hgdB = hgupA;
+                HGd = hgupB;
+#pragma unroll
+                for (int r = 0; r <= QT; ++r) {
+                    if (r < QT) cell(r, sgA[r], hgdA, hgupA, fA);
+                    if (r == QT - 1) outb[bp] = make_uint2(as_u32(HG[QT - 1]), as_u32(fA));
+                    if (r >= 1) cell(r - 1, sgB[r - 1], hgdB, hgupB, fB);
+                }
+                outb[bp + 1] = make_uint2(as_u32(HG[QT - 1]), as_u32(fB));
+            }
+        } else {
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = jb + b + 1;
                 s2v hgup = as_s2(spv[b].x), f = as_s2(spv[b].y);
-                s2v sg[Q];
+                s2v sg[QT];
                 load_sg(b, sg);
                 s2v hgd = HGd;
                 HGd = hgup;
 #pragma unroll
-                for (int r = 0; r < Q; ++r) {
+                for (int r = 0; r < QT; ++r) {
                     const s2v diag = p_add(hgd, sg[r]);
                     const s2v e = p_max(HG[r], p_sub(E[r], ge2));
                     f = p_max(hgup, p_sub(f, ge2));
-                    s2v h = p_max(p_max(diag, e), f);
-                    if (F::local) h = p_max(h, zero2);
+                    const s2v h = p_max(p_max(diag, e), f);
                     hgd = HG[r];
                     E[r] = e;
                     HG[r] = p_sub(h, go2);
                     hgup = HG[r];
-                    // Local mode tracks max(H - go) through the HG value that stays live anyway:
-                    // H >= 0 there, so H - go never saturates and max H = max(H - go) + go.
-                    if (F::local) run = p_max(run, HG[r]);
                 }
-                outb[b] = make_uint2(as_u32(HG[Q - 1]), as_u32(f));
-                if (!F::local && lastc) {
+                outb[b] = make_uint2(as_u32(HG[QT - 1]), as_u32(f));
+                if (lastc) {
                     const bool la = F::lrow ? j <= nA : j == nA;
                     const bool lb = F::lrow ? j <= nB : j == nB;
                     s2v v = HG[0];
 #pragma unroll
-                    for (int r = 1; r < Q; ++r) v = r == rm ? HG[r] : v;
+                    for (int r = 1; r < QT; ++r) v = r == rm ? HG[r] : v;
                     run = p_max(run, keep2(p_add(v, go2), la, lb));
                 }
                 if (F::lcol && (j == nA || j == nB)) {
 #pragma unroll
-                    for (int r = 0; r < Q; ++r)
+                    for (int r = 0; r < QT; ++r)
                         if (i0 + r <= m)
                             run = p_max(run, keep2(p_add(HG[r], go2), j == nA, j == nB));
                 }
             }
-            }
-            if (c + 1 < nchunks) {
-#pragma unroll
-                for (int b = 0; b < 4; ++b) sp[(size_t)(jb + b + 1) * 64] = outb[b];
-            }
-            w = wn;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) spv[b] = spn[b];
         }
+        if (!lastc) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) sp[(size_t)(jb + b + 1) * 64] = outb[b];
+        }
+        w = wn;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) spv[b] = spn[b];
+    }
+    st.run = run;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) st.runq[k] = runq[k];
+}
+
+// One group of 128 targets against one query: full tiles of Q rows, then one tile of the fewest
+// rows (8, 16, ... Q) that covers the remainder, so short queries waste at most 7 padded rows.
+// The boundary rows go through `spill` (column j of this wavefront at spill[j * 64 + lane]); the
+// two targets' scores go to out_score and their best key is folded into *key. Called by the
+// one-shot kernel (one group per wavefront) and by the persistent batch kernel (a queue of
+// (group, query) items).
+template <int MODE, int Q, int NW, bool PAIR>
+__device__ __forceinline__ void inter16_group(const Inter16Args& a, const Group& g, bool wlive,
+                                              const uint8_t* query, int m, uint2* spill,
+                                              unsigned long long* key, uint32_t* pp) {
+    using F = MF<MODE>;
+    static_assert(Q % 8 == 0 && Q >= 8 && Q <= 32, "tile rows");
+    const int lane = threadIdx.x & 63;
+    const bool liveA = 2 * lane < (int)g.count, liveB = 2 * lane + 1 < (int)g.count;
+    const uint32_t slotA = g.first + 2 * lane, slotB = slotA + 1;
+    const int nA = liveA ? (int)a.lens[slotA] : 0;
+    const int nB = liveB ? (int)a.lens[slotB] : 0;
+    const int go = a.go, ge = a.ge;
+
+    int bsA, bsB, dj, di;
+    border_best<MODE>(m, nA, go, ge, bsA, dj, di);
+    border_best<MODE>(m, nB, go, ge, bsB, dj, di);
+    PairRun st;
+    st.run = splat2(-32768);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) st.runq[k] = st.run;
+
+    uint2* sp = spill + lane;
+    const uint2* wp = a.words + g.word_off + lane;
+    const int nfull = m / Q, rem = m - nfull * Q;
+    for (int c = 0; c < nfull; ++c)
+        inter16_tile<MODE, Q, NW, PAIR>(a, g, wlive, query, m, c * Q + 1, c == 0,
+                                        rem == 0 && c == nfull - 1, nA, nB, sp, wp, pp, st);
+    if (rem > 0) {
+        const int i0 = nfull * Q + 1;
+        const bool first = nfull == 0;
+        if (rem <= 8)
+            inter16_tile<MODE, 8, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp, wp,
+                                            pp, st);
+        else if (Q > 16 && rem <= 16)
+            inter16_tile<MODE, 16, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
+                                             wp, pp, st);
+        else if (Q > 24 && rem <= 24)
+            inter16_tile<MODE, 24, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
+                                             wp, pp, st);
+        else
+            inter16_tile<MODE, Q, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
+                                            wp, pp, st);
     }
     if (!wlive) return;
-    run = p_max(run, p_max(p_max(runq[0], runq[1]), p_max(runq[2], runq[3])));
+    const s2v run =
+        p_max(st.run, p_max(p_max(st.runq[0], st.runq[1]), p_max(st.runq[2], st.runq[3])));
     const int lift = F::local ? go : 0;
     const int sA = max(bsA, (int)run.x + lift), sB = max(bsB, (int)run.y + lift);
     unsigned long long k = 0ull;
@@ -563,8 +622,48 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inte
         const unsigned long long kb = make_key(sB, gi);
         k = kb > k ? kb : k;
     }
-    if (a.key) wave_argmax(k, a.key);
+    if (key) wave_argmax(k, key);
 }
+
+template <int MODE, int Q, int NW, bool PAIR>
+__global__ __launch_bounds__(64 * NW)
+__attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inter16Args a) {
+    extern __shared__ uint32_t pp[];
+    const uint32_t gix = a.group_first + blockIdx.x * NW + (threadIdx.x >> 6);
+    const bool wlive = gix < a.group_first + a.ngroups;
+    Group g{};
+    if (wlive) g = a.groups[gix];
+    inter16_group<MODE, Q, NW, PAIR>(a, g, wlive, a.query, a.m, a.spill + g.spill_off, a.key, pp);
+}
+
+// Persistent batch kernel: each one-wavefront workgroup takes (group, query) items from a device
+// counter until the queue is empty. Items run group-major over groups in descending column count
+// and, within a group, queries in descending length, so the longest work starts first. Each
+// workgroup owns one boundary-row slot of bt.slot_stride entries.
+template <int MODE, int Q>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_batch(Inter16Args a,
+                                                                                   BatchArgs bt) {
+    extern __shared__ uint32_t pp[];
+    __shared__ uint32_t item_s;
+    uint2* slot = bt.slots + (size_t)blockIdx.x * bt.slot_stride;
+    const uint32_t nq = *bt.nq;
+    const uint32_t n_items = nq * a.ngroups;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) item_s = atomicAdd(bt.counter, 1u);
+        __syncthreads();
+        const uint32_t item = item_s;
+        if (item >= n_items) break;
+        const uint32_t gix = a.group_first + item / nq;
+        const uint32_t qi = bt.qorder[item % nq];
+        const Group g = a.groups[gix];
+        inter16_group<MODE, Q, 1, false>(a, g, true, bt.queries + bt.qoff[qi], (int)bt.qlen[qi],
+                                         slot, bt.keys + qi, pp);
+    }
+}
+
+
 
 // ------------------------------------------------------------------------------------------------
 // Intra-sequence systolic kernel, 32-bit lanes, one (query, target) pair per wavefront.
@@ -580,11 +679,15 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
     const int lane = threadIdx.x;
     // Winner mode: an all-zero key means no hit was recorded; fall back to index 0 and let the
     // self-check below flag it.
-    const unsigned long long wk = WINNER ? *a.winner_key : 0ull;
+    const bool batch = a.qoff != nullptr;
+    const uint32_t qi = batch ? (a.qsel >= 0 ? (uint32_t)a.qsel : blockIdx.y) : 0u;
+    if (batch && qi >= *a.nq) return;
+    const uint8_t* query = batch ? a.queries + a.qoff[qi] : a.query;
+    const unsigned long long wk = WINNER ? a.winner_key[qi] : 0ull;
     const uint32_t gi = WINNER ? (wk != 0ull ? key_index(wk) : 0u) : a.list[blockIdx.x];
     const uint8_t* t = a.flat + a.off[gi];
     const int n = (int)(a.off[gi + 1] - a.off[gi]);
-    const int m = a.m, go = a.go, ge = a.ge, ale = a.al_eff;
+    const int m = batch ? (int)a.qlen[qi] : a.m, go = a.go, ge = a.ge, ale = a.al_eff;
     const int rowstride = LDSMAT ? ale : a.al;
     // LDS matrix rows 0..ale-1 hold the scores; row ale is the padding row (kPad32), used by query
     // rows past the end so the lookup needs no branch.
@@ -599,14 +702,16 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
 
     constexpr int P = 64 * R;
     const int npass = (m + P - 1) / P;
-    int2* sp = a.spill + (WINNER ? 0 : (size_t)blockIdx.x * a.spill_stride);
+    const size_t spill_block =
+        WINNER ? qi : (a.qsel >= 0 ? blockIdx.x : (size_t)qi * gridDim.x + blockIdx.x);
+    int2* sp = a.spill + spill_block * a.spill_stride;
     for (int p = 0; p < npass; ++p) {
         const int r0 = p * P + lane * R + 1;
         int qb[R], HG[R], E[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = r0 + r;
-            qb[r] = i <= m ? (int)a.query[i - 1] * rowstride : (LDSMAT ? ale * ale : -1);
+            qb[r] = i <= m ? (int)query[i - 1] * rowstride : (LDSMAT ? ale * ale : -1);
             HG[r] = wsub(border<MODE>(i, 0, go, ge), go);
             E[r] = kNeg32;
         }
@@ -699,7 +804,7 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
     if (WINNER) {
         if (lane == 0) {
             const int ks = key_score(wk);
-            Result* r = a.result;
+            Result* r = a.result + qi;
             r->score = bs;
             r->db_index = gi;
             r->query_end = ENDS ? (bi == 0 ? -1 : (int64_t)bi - 1) : -1;
@@ -714,7 +819,7 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
                 a.out_qend[gi] = bi == 0 ? -1 : (int64_t)bi - 1;
                 a.out_tend[gi] = bj == 0 ? -1 : (int64_t)bj - 1;
             }
-            if (a.key) atomicMax(a.key, make_key(bs, gi));
+            if (a.key) atomicMax(a.key + qi, make_key(bs, gi));
         }
     }
 }
@@ -732,42 +837,43 @@ hipError_t launch_inter32_m(bool ends, const InterArgs& a, uint32_t ngroups, hip
 
 template <int MODE, int R, bool LDSMAT>
 hipError_t launch_sys32_mrl(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
-                            hipStream_t st) {
+                            hipStream_t st, uint32_t nq) {
     const size_t lds = LDSMAT ? (size_t)(a.al_eff + 1) * a.al_eff * sizeof(int) : 0;
     if (winner) {
         if (ends)
-            hipLaunchKernelGGL((k_sys32<MODE, R, true, true, LDSMAT>), dim3(1), dim3(64), lds, st, a);
-        else
-            hipLaunchKernelGGL((k_sys32<MODE, R, false, true, LDSMAT>), dim3(1), dim3(64), lds, st,
-                               a);
-    } else {
-        if (ends)
-            hipLaunchKernelGGL((k_sys32<MODE, R, true, false, LDSMAT>), dim3(nblocks), dim3(64), lds,
+            hipLaunchKernelGGL((k_sys32<MODE, R, true, true, LDSMAT>), dim3(1, nq), dim3(64), lds,
                                st, a);
         else
-            hipLaunchKernelGGL((k_sys32<MODE, R, false, false, LDSMAT>), dim3(nblocks), dim3(64),
+            hipLaunchKernelGGL((k_sys32<MODE, R, false, true, LDSMAT>), dim3(1, nq), dim3(64), lds,
+                               st, a);
+    } else {
+        if (ends)
+            hipLaunchKernelGGL((k_sys32<MODE, R, true, false, LDSMAT>), dim3(nblocks, nq), dim3(64),
                                lds, st, a);
+        else
+            hipLaunchKernelGGL((k_sys32<MODE, R, false, false, LDSMAT>), dim3(nblocks, nq),
+                               dim3(64), lds, st, a);
     }
     return hipGetLastError();
 }
 
 template <int MODE, int R>
 hipError_t launch_sys32_mr(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
-                           hipStream_t st) {
+                           hipStream_t st, uint32_t nq) {
     if ((a.al_eff + 1) * a.al_eff <= 16384)
-        return launch_sys32_mrl<MODE, R, true>(ends, winner, a, nblocks, st);
-    return launch_sys32_mrl<MODE, R, false>(ends, winner, a, nblocks, st);
+        return launch_sys32_mrl<MODE, R, true>(ends, winner, a, nblocks, st, nq);
+    return launch_sys32_mrl<MODE, R, false>(ends, winner, a, nblocks, st, nq);
 }
 
 // Rows per lane: the smallest of 2, 4, 6, 8 whose 64-lane pass covers the query (one pass up to
 // 512 rows); longer queries take several passes of 8 rows per lane.
 template <int MODE>
 hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
-                          hipStream_t st) {
-    if (a.m <= 128) return launch_sys32_mr<MODE, 2>(ends, winner, a, nblocks, st);
-    if (a.m <= 256) return launch_sys32_mr<MODE, 4>(ends, winner, a, nblocks, st);
-    if (a.m <= 384) return launch_sys32_mr<MODE, 6>(ends, winner, a, nblocks, st);
-    return launch_sys32_mr<MODE, 8>(ends, winner, a, nblocks, st);
+                          hipStream_t st, uint32_t nq) {
+    if (a.m <= 128) return launch_sys32_mr<MODE, 2>(ends, winner, a, nblocks, st, nq);
+    if (a.m <= 256) return launch_sys32_mr<MODE, 4>(ends, winner, a, nblocks, st, nq);
+    if (a.m <= 384) return launch_sys32_mr<MODE, 6>(ends, winner, a, nblocks, st, nq);
+    return launch_sys32_mr<MODE, 8>(ends, winner, a, nblocks, st, nq);
 }
 
 template <int MODE, int Q, int NW, bool PAIR>
@@ -806,6 +912,35 @@ hipError_t launch_inter16_m(const Inter16Args& a, int q, int waves, hipStream_t 
 
 }  // namespace
 
+template <int MODE, int Q>
+hipError_t launch_inter16_batch_q(const Inter16Args& a, const BatchArgs& b, uint32_t wgs,
+                                  hipStream_t st) {
+    const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
+    hipLaunchKernelGGL((k_inter16_batch<MODE, Q>), dim3(wgs), dim3(64), lds, st, a, b);
+    return hipGetLastError();
+}
+
+template <int MODE>
+hipError_t launch_inter16_batch_m(const Inter16Args& a, const BatchArgs& b, int q, uint32_t wgs,
+                                  hipStream_t st) {
+    if (q <= 16) return launch_inter16_batch_q<MODE, 16>(a, b, wgs, st);
+    if (q <= 24) return launch_inter16_batch_q<MODE, 24>(a, b, wgs, st);
+    return launch_inter16_batch_q<MODE, 32>(a, b, wgs, st);
+}
+
+hipError_t launch_inter16_batch(int mode, const Inter16Args& a, const BatchArgs& b, int q,
+                                uint32_t workgroups, hipStream_t stream) {
+    if (a.ngroups == 0 || workgroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_inter16_batch_m<kSW>(a, b, q, workgroups, stream);
+    case kNW: return launch_inter16_batch_m<kNW>(a, b, q, workgroups, stream);
+    case kHW: return launch_inter16_batch_m<kHW>(a, b, q, workgroups, stream);
+    case kOV: return launch_inter16_batch_m<kOV>(a, b, q, workgroups, stream);
+    case kSHW: return launch_inter16_batch_m<kSHW>(a, b, q, workgroups, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 int inter16_tile_rows(int q) { return q <= 16 ? 16 : (q <= 24 ? 24 : 32); }
 
 uint32_t inter16p_lds_bytes(int al_eff, int q) {
@@ -839,6 +974,48 @@ hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     return hipGetLastError();
 }
 
+hipError_t launch_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter,
+                              hipStream_t stream) {
+    const uint32_t blocks = (n + 255) / 256 + (n == 0);
+    hipLaunchKernelGGL(k_batch_reset, dim3(blocks), dim3(256), 0, stream, keys, n, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_key_results(const unsigned long long* keys, Result* r, const uint32_t* nq,
+                              uint32_t cap, hipStream_t stream) {
+    if (cap == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_key_results, dim3((cap + 255) / 256), dim3(256), 0, stream, keys, r, nq,
+                       cap);
+    return hipGetLastError();
+}
+
+uint32_t inter16_batch_workgroups(int mode, int q, int al_eff) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    const int rows = inter16_tile_rows(q);
+    const size_t lds = inter16p_lds_bytes(al_eff, rows) + sizeof(uint32_t);
+    hipError_t e = hipErrorInvalidValue;
+#define HY_OCC(M, Q)                                                                              \
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                             \
+        &per_cu, reinterpret_cast<const void*>(k_inter16_batch<M, Q>), 64, lds)
+#define HY_OCC_Q(M)                                                                               \
+    if (rows == 16) HY_OCC(M, 16); else if (rows == 24) HY_OCC(M, 24); else HY_OCC(M, 32)
+    switch (mode) {
+    case kSW: HY_OCC_Q(kSW); break;
+    case kNW: HY_OCC_Q(kNW); break;
+    case kHW: HY_OCC_Q(kHW); break;
+    case kOV: HY_OCC_Q(kOV); break;
+    case kSHW: HY_OCC_Q(kSHW); break;
+    default: break;
+    }
+#undef HY_OCC_Q
+#undef HY_OCC
+    if (e != hipSuccess || per_cu <= 0) per_cu = 2;
+    return (uint32_t)(per_cu * cus);
+}
+
 hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream) {
     hipLaunchKernelGGL(k_key_result, dim3(1), dim3(64), 0, stream, key, r);
     return hipGetLastError();
@@ -858,14 +1035,14 @@ hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngro
 }
 
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
-                        hipStream_t stream) {
-    if (!winner && nblocks == 0) return hipSuccess;
+                        hipStream_t stream, uint32_t nq_grid) {
+    if ((!winner && nblocks == 0) || nq_grid == 0) return hipSuccess;
     switch (mode) {
-    case kSW: return launch_sys32_m<kSW>(ends, winner, a, nblocks, stream);
-    case kNW: return launch_sys32_m<kNW>(ends, winner, a, nblocks, stream);
-    case kHW: return launch_sys32_m<kHW>(ends, winner, a, nblocks, stream);
-    case kOV: return launch_sys32_m<kOV>(ends, winner, a, nblocks, stream);
-    case kSHW: return launch_sys32_m<kSHW>(ends, winner, a, nblocks, stream);
+    case kSW: return launch_sys32_m<kSW>(ends, winner, a, nblocks, stream, nq_grid);
+    case kNW: return launch_sys32_m<kNW>(ends, winner, a, nblocks, stream, nq_grid);
+    case kHW: return launch_sys32_m<kHW>(ends, winner, a, nblocks, stream, nq_grid);
+    case kOV: return launch_sys32_m<kOV>(ends, winner, a, nblocks, stream, nq_grid);
+    case kSHW: return launch_sys32_m<kSHW>(ends, winner, a, nblocks, stream, nq_grid);
     default: return hipErrorInvalidValue;
     }
 }

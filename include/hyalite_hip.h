@@ -131,10 +131,19 @@ int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qle
 int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
                    int32_t* scores);
 
-/* Batched form: nq queries in one submission; hit k equals hy_scan of query k. */
+/* Batched form: nq queries in one submission; hit k equals hy_scan of query k. The queries run
+ * in one persistent kernel over a (target group, query) work queue, the winners' end positions
+ * in one launch, and the whole submission is captured once as a HIP graph and replayed. Batches
+ * larger than the scratch's reservation run as several submissions. No reference counterpart:
+ * the reference loops Database::scan (src/database.rs:83). */
 int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
                   const size_t* qlens, size_t nq, int32_t* scores, uint64_t* db_indices,
                   int64_t* query_ends, int64_t* target_ends);
+
+/* Reserve device and pinned buffers for batches of up to max_queries queries (done on first use
+ * with min(nq, 256) otherwise). The persistent kernel's boundary rows need one slot of
+ * (longest packed target + 1) x 512 bytes per resident workgroup. */
+int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries);
 
 /* Instrumentation (no counterpart in the reference; for benchmarks and profilers). When enabled,
  * every scan records HIP events on the scratch's own stream around the per-target kernel
@@ -144,6 +153,8 @@ int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
 int hy_scratch_profile(hy_scratch* s, int enable);
 int hy_scratch_profile_read(hy_scratch* s, double* inter_ms, double* tail_ms, uint64_t* launches,
                             int reset);
+/* Device time of the batched form's persistent kernel (profiling runs submit without a graph). */
+int hy_scratch_batch_profile_read(hy_scratch* s, double* kernel_ms, uint64_t* launches, int reset);
 
 /* Human-readable message for a status code (error.rs:98-157 wording where one exists). */
 const char* hy_strerror(int code);

@@ -295,3 +295,74 @@ def test_batch_equals_individual_scans():
     qs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in (5, 50, 399, 0, 120)]
     batch = db.scan_batch(sc, qs)
     assert batch == [db.scan(sc, q) for q in qs]
+
+
+# ---- batched form -------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_batch_matches_oracle_all_modes(mode):
+    """The persistent batch kernel + batched winner pass: every hit equals the oracle's scan, for
+    both search types, with long targets (packed and, at 600-row queries, multi-pass)."""
+    rng = np.random.default_rng(500 + mode)
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(5, 200, 900)]
+    for k in (7, 300, 899):
+        seqs[k] = rng.integers(0, 20, size=int(rng.integers(2500, 3200)), dtype=np.uint8)
+    qs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in (0, 1, 17, 200, 600, 33, 90)]
+    flat, off = orc.flatten(seqs)
+    for st in ST:
+        db = build(seqs, (20, BLOSUM62, 11, 1), mode, st, 600)
+        sc = hy.Scratch(db)
+        got = db.scan_batch(sc, qs)
+        for q, h in zip(qs, got):
+            want = orc.scan_flat(flat, off, q, 20, BLOSUM62, 11, 1, mode,
+                                 st == hy.SearchType.ScoreEnd, threads=4)
+            assert (h.score, h.db_index, h.query_end, h.target_end) == want, (mode, st, len(q))
+
+
+def test_batch_graph_replays_and_chunking():
+    """Several batches through one scratch (the first captures the graph, later ones replay it),
+    a batch larger than the reservation (several submissions), each hit == the one-query scan."""
+    rng = np.random.default_rng(77)
+    seqs = [rng.integers(0, 24, size=int(n), dtype=np.uint8) for n in rng.integers(10, 400, 4000)]
+    mat = [4 + (i % 8) if i == j else -4 + ((i * 7 + j * 3) % 8) for i in range(24)
+           for j in range(24)]
+    db = build(seqs, (24, mat, 11, 1), MODE_SW, hy.SearchType.ScoreEnd, 300)
+    sc = hy.Scratch(db)
+    sc.reserve_batch(8)
+    one = hy.Scratch(db)
+    for rep in range(3):
+        qs = [rng.integers(0, 24, size=int(n), dtype=np.uint8)
+              for n in rng.integers(1, 300, 5 + 4 * rep)]
+        got = db.scan_batch(sc, qs)
+        assert got == [db.scan(one, q) for q in qs], rep
+
+
+def test_batch_with_unpackable_long_targets():
+    """Long targets whose proof needs 32 bits go through the batched systolic tail."""
+    rng = np.random.default_rng(88)
+    scheme = (4, [40 if a == b else -3 for a in range(4) for b in range(4)], 6, 2)
+    seqs = [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in rng.integers(20, 120, 700)]
+    q = rng.integers(0, 4, size=900, dtype=np.uint8)
+    seqs[5] = np.concatenate([q[:800], rng.integers(0, 4, 400, dtype=np.uint8)])  # i32, long
+    seqs[6] = rng.integers(0, 4, size=1500, dtype=np.uint8)
+    db = build(seqs, scheme, MODE_SW, hy.SearchType.ScoreEnd, 900)
+    assert db.partition_sizes()[hy.ScoreWidth.I32] >= 1
+    sc = hy.Scratch(db)
+    qs = [q, q[:100], q[300:], rng.integers(0, 4, size=40, dtype=np.uint8)]
+    flat, off = orc.flatten(seqs)
+    got = db.scan_batch(sc, qs)
+    for qq, h in zip(qs, got):
+        want = orc.scan_flat(flat, off, qq, *scheme, MODE_SW, True, threads=4)
+        assert (h.score, h.db_index, h.query_end, h.target_end) == want
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_every_remainder_tile_size(mode):
+    """Query lengths 1..73 take every remainder tile (8, 16 and full rows) after 0, 1 and 2 full
+    tiles; per-target scores and ends equal the oracle's."""
+    rng = np.random.default_rng(600 + mode)
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(0, 90, 300)]
+    base = rng.integers(0, 20, size=73, dtype=np.uint8)
+    queries = [base[:m] for m in range(1, 74)]
+    check(seqs, (20, BLOSUM62, 11, 1), mode, hy.SearchType.ScoreEnd, queries, mql=73)
+    check(seqs, (20, BLOSUM62, 11, 1), mode, hy.SearchType.Score, queries[::5], mql=73)
