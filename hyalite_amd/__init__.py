@@ -34,7 +34,8 @@ __all__ = [
     "NegativeGapPenalty", "GapOpenLessThanExtend", "ScoreRangeTooWide", "SymbolOutOfRange",
     "IncompleteBuilder", "EmptyDatabase", "BackendUnavailable", "InvalidBackendName",
     "ScanContractViolation", "DeviceError", "HipLibraryMissing", "BACKEND_ENV_VAR", "build",
-    "library_path", "load_library",
+    "library_path", "load_library", "Tuning", "Server", "ServedHit", "ServerStats",
+    "ServerTimeout",
 ]
 
 BACKEND_ENV_VAR = "HYALITE_BACKEND"
@@ -42,8 +43,12 @@ BACKEND_ENV_VAR = "HYALITE_BACKEND"
 _PKG = os.path.dirname(os.path.abspath(__file__))
 _ROOT = os.path.dirname(_PKG)
 _LIB_NAME = "libhyalite_hip.so"
-_SOURCES = ["csrc/hy_host.cpp", "csrc/hy_kernels.hip"]
-HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+# Translation units: the host library, the autotuner, the mode-independent kernels and dispatch,
+# and the mode-templated kernels compiled once per alignment mode (SW, NW, HW, OV, SHW).
+_UNITS = [("csrc/hy_host.cpp", []), ("csrc/hy_tune.cpp", []), ("csrc/hy_serve.cpp", []),
+          ("csrc/hy_launch.hip", [])] + [
+    ("csrc/hy_kernels.hip", [f"-DHY_KMODE={k}"]) for k in range(5)]
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
@@ -51,19 +56,41 @@ def library_path() -> str:
     return os.path.join(_PKG, _LIB_NAME)
 
 
-def build(verbose: bool = False) -> str:
-    """Compile libhyalite_hip.so for gfx950 in-tree with hipcc."""
+def build(verbose: bool = False, jobs: Optional[int] = None) -> str:
+    """Compile libhyalite_hip.so for gfx950 in-tree with hipcc (translation units in parallel)."""
+    from concurrent.futures import ThreadPoolExecutor
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     if not os.path.exists(hipcc):
         hipcc = "hipcc"
+    objdir = os.path.join(_PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    cmds, objs = [], []
+    for src, extra in _UNITS:
+        stem = os.path.splitext(os.path.basename(src))[0] + "".join(
+            "_" + e.split("=")[-1] for e in extra)
+        obj = os.path.join(objdir, stem + ".o")
+        objs.append(obj)
+        cmds.append([hipcc, *HIPCC_FLAGS, *extra, "-c", os.path.join(_PKG, src), "-o", obj])
+    jobs = jobs or max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", "0") or 0)
+                              or os.cpu_count() or 1))
+
+    def run(cmd):
+        return cmd, subprocess.run(cmd, capture_output=True, text=True)
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        for cmd, res in pool.map(run, cmds):
+            if res.returncode != 0:
+                raise RuntimeError("hipcc failed: " + " ".join(cmd) + "\n" + res.stdout +
+                                   res.stderr)
+            if verbose:
+                print(" ".join(cmd))
     out = library_path()
     tmp = out + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, *[os.path.join(_PKG, s) for s in _SOURCES], "-o", tmp]
-    res = subprocess.run(cmd, capture_output=True, text=True)
+    link = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    res = subprocess.run(link, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + res.stdout + res.stderr)
+        raise RuntimeError("hipcc link failed:\n" + res.stdout + res.stderr)
     if verbose:
-        print(" ".join(cmd))
+        print(" ".join(link))
     os.replace(tmp, out)
     return out
 
@@ -294,6 +321,51 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _sizep = ctypes.POINTER(ctypes.c_size_t)
 
 
+class _HyServerConfig(ctypes.Structure):
+    _fields_ = [("scratch_budget_bytes", ctypes.c_uint64), ("max_lanes", ctypes.c_uint32),
+                ("max_batch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+
+
+class _HyServerResult(ctypes.Structure):
+    _fields_ = [("score", ctypes.c_int32), ("status", ctypes.c_int32),
+                ("db_index", ctypes.c_uint64), ("query_end", ctypes.c_int64),
+                ("target_end", ctypes.c_int64), ("submit_ns", ctypes.c_uint64),
+                ("admit_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64),
+                ("batch_size", ctypes.c_uint32), ("lane", ctypes.c_uint32)]
+
+
+class _HyServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("query_residues", ctypes.c_uint64),
+                ("lanes", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("lane_bytes", ctypes.c_uint64)]
+
+
+class _HyTuning(ctypes.Structure):
+    _fields_ = [("tile_rows", ctypes.c_int32), ("waves", ctypes.c_int32),
+                ("batch_tile_rows", ctypes.c_int32), ("batch_wgs_per_cu", ctypes.c_int32),
+                ("batch_chunk", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+@dataclass(frozen=True)
+class Tuning:
+    """Launch shapes of the packed kernels held by a Scratch (hy_tuning). Shapes change speed
+    only: every shape gives the same hits."""
+    tile_rows: int = 32
+    waves: int = 1
+    batch_tile_rows: int = 32
+    batch_wgs_per_cu: int = 0
+    batch_chunk: int = 256
+
+    def _c(self) -> _HyTuning:
+        return _HyTuning(self.tile_rows, self.waves, self.batch_tile_rows, self.batch_wgs_per_cu,
+                         self.batch_chunk, 0)
+
+    @staticmethod
+    def _from(c: _HyTuning) -> "Tuning":
+        return Tuning(c.tile_rows, c.waves, c.batch_tile_rows, c.batch_wgs_per_cu, c.batch_chunk)
+
+
 def load_library():
     """Load libhyalite_hip.so (raises HipLibraryMissing when it has not been built)."""
     global _lib
@@ -346,6 +418,24 @@ def load_library():
     lib.hy_scratch_batch_profile_read.restype = ctypes.c_int
     lib.hy_scratch_batch_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), _u64p,
                                                   ctypes.c_int]
+    lib.hy_scratch_get_tuning.restype = ctypes.c_int
+    lib.hy_scratch_get_tuning.argtypes = [vp, ctypes.POINTER(_HyTuning)]
+    lib.hy_scratch_set_tuning.restype = ctypes.c_int
+    lib.hy_scratch_set_tuning.argtypes = [vp, ctypes.POINTER(_HyTuning)]
+    lib.hy_scratch_autotune.restype = ctypes.c_int
+    lib.hy_scratch_autotune.argtypes = [vp, vp, ctypes.POINTER(_u8p), _sizep, ctypes.c_size_t,
+                                        ctypes.c_uint32, ctypes.POINTER(_HyTuning)]
+    lib.hy_server_new.restype = ctypes.c_int
+    lib.hy_server_new.argtypes = [vp, ctypes.POINTER(_HyServerConfig), ctypes.POINTER(vp)]
+    lib.hy_server_submit.restype = ctypes.c_int
+    lib.hy_server_submit.argtypes = [vp, _u8p, ctypes.c_size_t, _u64p]
+    lib.hy_server_wait.restype = ctypes.c_int
+    lib.hy_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.c_int64,
+                                   ctypes.POINTER(_HyServerResult)]
+    lib.hy_server_stats_read.restype = ctypes.c_int
+    lib.hy_server_stats_read.argtypes = [vp, ctypes.POINTER(_HyServerStats)]
+    lib.hy_server_free.restype = None
+    lib.hy_server_free.argtypes = [vp]
     lib.hy_strerror.restype = ctypes.c_char_p
     lib.hy_strerror.argtypes = [ctypes.c_int]
     _lib = lib
@@ -665,6 +755,38 @@ class Scratch:
                                                      1 if reset else 0)
         return a.value, n.value
 
+    def tuning(self) -> Tuning:
+        c = _HyTuning()
+        rc = load_library().hy_scratch_get_tuning(self._h, ctypes.byref(c))
+        if rc != 0:
+            raise DeviceError(load_library().hy_strerror(rc).decode())
+        return Tuning._from(c)
+
+    def set_tuning(self, t: Tuning) -> None:
+        """Set launch shapes; invalid shapes raise ValueError."""
+        rc = load_library().hy_scratch_set_tuning(self._h, ctypes.byref(t._c()))
+        if rc == 103:
+            raise ValueError(f"invalid tuning {t}")
+        if rc != 0:
+            raise DeviceError(load_library().hy_strerror(rc).decode())
+
+    def autotune(self, queries, plain: bool = True, batch: bool = True) -> Tuning:
+        """Time candidate shapes on a sample of `queries` against this scratch's database and keep
+        the fastest (hy_scratch_autotune; HYALITE_HIP_TUNE_CACHE names an optional cache file)."""
+        raw = [np.asarray(q, dtype=np.uint8) for q in queries]
+        qs = [_as_u8(q) for q in raw]
+        n = len(qs)
+        ptrs = (_u8p * max(n, 1))()
+        lens = (ctypes.c_size_t * max(n, 1))()
+        for k in range(n):
+            ptrs[k] = qs[k].ctypes.data_as(_u8p)
+            lens[k] = raw[k].size
+        c = _HyTuning()
+        forms = (1 if plain else 0) | (2 if batch else 0)
+        _check_scan(load_library().hy_scratch_autotune(self._db._h, self._h, ptrs, lens, n, forms,
+                                                       ctypes.byref(c)))
+        return Tuning._from(c)
+
     def profile_read(self, reset: bool = False) -> tuple[float, float, int]:
         """(inter-kernel ms, long-target-kernel ms, scans timed), accumulated."""
         a, b, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_uint64(0)
@@ -677,3 +799,96 @@ class Scratch:
         if h is not None and _lib is not None:
             _lib.hy_scratch_free(h)
             self._h = None
+
+
+# ---- serving form -------------------------------------------------------------------------------
+
+@dataclass(frozen=True)
+class ServedHit:
+    """One served query: its hit (equal to Database.scan of the query alone) and its timeline
+    in seconds on the steady clock: queued (submit -> admitted to a batch) and latency
+    (submit -> result ready)."""
+    hit: BestHit
+    queued_s: float
+    latency_s: float
+    batch_size: int
+    lane: int
+
+
+@dataclass(frozen=True)
+class ServerStats:
+    submitted: int
+    completed: int
+    batches: int
+    query_residues: int
+    lanes: int
+    max_batch: int
+    lane_bytes: int
+
+
+class ServerTimeout(TimeoutError):
+    pass
+
+
+class Server:
+    """Serving form over a resident database (hy_server_*): queries are queued, admitted by free
+    lanes in batches of up to max_batch and scanned while other batches run; the number of lanes
+    is the most that fit scratch_budget_bytes (0 = no limit), at most max_lanes."""
+
+    def __init__(self, db: Database, scratch_budget_bytes: int = 0, max_lanes: int = 2,
+                 max_batch: int = 64):
+        cfg = _HyServerConfig(scratch_budget_bytes, max_lanes, max_batch)
+        out = ctypes.c_void_p(None)
+        rc = load_library().hy_server_new(db._h, ctypes.byref(cfg), ctypes.byref(out))
+        if rc == 103:
+            raise ValueError("invalid server configuration")
+        if rc != 0:
+            raise DeviceError(load_library().hy_strerror(rc).decode())
+        self._h = out
+        self._db = db
+
+    def submit(self, query) -> int:
+        q = np.asarray(query, dtype=np.uint8)
+        buf = _as_u8(q)
+        t = ctypes.c_uint64(0)
+        _check_scan(load_library().hy_server_submit(self._h, buf.ctypes.data_as(_u8p), q.size,
+                                                    ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket: int, timeout: Optional[float] = None) -> ServedHit:
+        r = _HyServerResult()
+        us = -1 if timeout is None else int(timeout * 1e6)
+        rc = load_library().hy_server_wait(self._h, ticket, us, ctypes.byref(r))
+        if rc == 107:
+            raise ServerTimeout(f"ticket {ticket} not completed within {timeout} s")
+        if rc == 103:
+            raise KeyError(ticket)
+        _check_scan(rc)
+        _check_scan(r.status)
+        hit = BestHit(int(r.score), int(r.db_index),
+                      None if r.query_end < 0 else int(r.query_end),
+                      None if r.target_end < 0 else int(r.target_end))
+        return ServedHit(hit, (r.admit_ns - r.submit_ns) / 1e9, (r.done_ns - r.submit_ns) / 1e9,
+                         int(r.batch_size), int(r.lane))
+
+    def stats(self) -> ServerStats:
+        st = _HyServerStats()
+        load_library().hy_server_stats_read(self._h, ctypes.byref(st))
+        return ServerStats(st.submitted, st.completed, st.batches, st.query_residues, st.lanes,
+                           st.max_batch, st.lane_bytes)
+
+    def close(self) -> None:
+        """Finish every queued query, then stop the lanes."""
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.hy_server_free(h)
+        self._h = None
+
+    def __enter__(self) -> "Server":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
+    def __del__(self):
+        self.close()

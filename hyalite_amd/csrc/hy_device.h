@@ -1,0 +1,82 @@
+// Device helpers shared by the kernel translation units: wrapping 32-bit arithmetic, the mode
+// flags, DP borders, the answer tie-break and the 64-bit argmax key.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <climits>
+
+#include "hy_internal.h"
+
+namespace hy {
+namespace {
+
+__device__ __forceinline__ int wadd(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
+__device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
+
+// Flags::for_mode (src/kernel.rs:43-98).
+template <int MODE>
+struct MF {
+    static constexpr bool top = MODE == kHW || MODE == kOV || MODE == kSW;
+    static constexpr bool left = MODE == kOV || MODE == kSW || MODE == kSHW;
+    static constexpr bool lrow = MODE == kHW || MODE == kOV;
+    static constexpr bool lcol = MODE == kOV || MODE == kSHW;
+    static constexpr bool local = MODE == kSW;
+};
+
+// gap_penalty (src/kernel.rs:103-109).
+__device__ __forceinline__ int gap32(int go, int ge, int len) {
+    return len == 0 ? 0 : (int)((long long)go + (long long)(len - 1) * (long long)ge);
+}
+
+// H on the DP border: row 0 (top) or column 0 (left).
+template <int MODE>
+__device__ __forceinline__ int border(int i, int j, int go, int ge) {
+    if (i == 0) return (j == 0 || MF<MODE>::top) ? 0 : -gap32(go, ge, j);
+    return MF<MODE>::left ? 0 : -gap32(go, ge, i);
+}
+
+// Best::consider (src/kernel.rs:113-129): maximise score, then minimise (col, row).
+__device__ __forceinline__ void consider(int& bs, int& bj, int& bi, int s, int j, int i) {
+    // Bitwise, not short-circuit, so the comparison stays branch-free in the cell loops.
+    const bool take = (s > bs) | ((s == bs) & ((j < bj) | ((j == bj) & (i < bi))));
+    bs = take ? s : bs;
+    bj = take ? j : bj;
+    bi = take ? i : bi;
+}
+
+// The answer-region cells that lie on the DP border, reduced in closed form. Borders never
+// increase away from the origin (gaps are non-negative), so on each border segment of the region
+// the cell nearest the origin dominates or ties-and-wins.
+template <int MODE>
+__device__ __forceinline__ void border_best(int m, int n, int go, int ge, int& bs, int& bj,
+                                            int& bi) {
+    bs = INT_MIN; bj = INT_MAX; bi = INT_MAX;
+    if (MF<MODE>::local) { consider(bs, bj, bi, 0, 0, 0); return; }
+    if (m == 0 || n == 0) consider(bs, bj, bi, border<MODE>(m, n, go, ge), n, m);
+    if (MF<MODE>::lrow) consider(bs, bj, bi, border<MODE>(m, 0, go, ge), 0, m);
+    if (MF<MODE>::lcol) consider(bs, bj, bi, border<MODE>(0, n, go, ge), n, 0);
+}
+
+// 64-bit argmax key: higher score first, then the smaller database index.
+__device__ __forceinline__ unsigned long long make_key(int score, uint32_t gi) {
+    return ((unsigned long long)((uint32_t)score ^ 0x80000000u) << 32) |
+           (unsigned long long)(0xFFFFFFFFu - gi);
+}
+__device__ __forceinline__ int key_score(unsigned long long k) {
+    return (int)((uint32_t)(k >> 32) ^ 0x80000000u);
+}
+__device__ __forceinline__ uint32_t key_index(unsigned long long k) {
+    return 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFull);
+}
+
+__device__ __forceinline__ void wave_argmax(unsigned long long k, unsigned long long* dst) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const unsigned long long o = __shfl_xor(k, off);
+        k = o > k ? o : k;
+    }
+    // Lane 0 of every wavefront (workgroups may hold several).
+    if ((threadIdx.x & 63) == 0 && k != 0ull) atomicMax(dst, k);
+}
+
+}  // namespace
+}  // namespace hy

@@ -20,9 +20,11 @@
 
 #include "../../include/hyalite_hip.h"
 #include "hy_internal.h"
+#include "hy_state.h"
 
 using hy::Group;
 using hy::Result;
+using hy::dev_free;
 
 namespace {
 
@@ -42,9 +44,6 @@ hipError_t dev_upload(T** dst, const std::vector<T>& src) {
     return hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
 }
 
-void dev_free(void* p) {
-    if (p) (void)hipFree(p);
-}
 
 // ScoreWidth::max_abs (src/width.rs:48-55) and required_width (src/width.rs:145-169).
 int width_bits_for(__int128 bound, int64_t* bound_out) {
@@ -92,135 +91,6 @@ std::string lower_trim(const char* s) {
 // ---------------------------------------------------------------------------------------------
 // Database and Scratch
 // ---------------------------------------------------------------------------------------------
-
-struct hy_db {
-    std::atomic<int> refs{1};
-    int device = 0;
-    // Validated scheme (src/scoring.rs:41-48) and builder fields.
-    uint32_t al = 0, al_eff = 0;
-    int32_t go = 0, ge = 0, min_e = 0, max_e = 0;
-    uint32_t mode = 0, search_type = 0;
-    uint64_t n = 0, mql = 0, mtl = 0, total_residues = 0;
-    int widest_bits = 8;
-    uint64_t count_by_width[3] = {0, 0, 0};
-    // Device copies.
-    int32_t* d_matrix = nullptr;   // al x al, entries clamped to +-kEntryMax32
-    uint8_t* d_flat = nullptr;     // residues in database order
-    uint64_t* d_off = nullptr;     // n+1 offsets
-    // Inter-sequence partition (one target per lane).
-    std::vector<Group> groups;
-    Group* d_groups = nullptr;
-    uint32_t* d_words = nullptr;
-    uint32_t* d_lens = nullptr;
-    uint32_t* d_gidx = nullptr;
-    uint64_t inter_spill = 0;      // int2 entries
-    // Groups [0, groups_score32) hold the targets that are not eligible for the packed kernel;
-    // score-only scans run the 32-bit kernel over those and the packed kernel over the rest.
-    uint32_t groups_score32 = 0;
-    // Packed 16-bit partition (two targets per lane).
-    std::vector<Group> groups16;
-    Group* d_groups16 = nullptr;
-    uint2* d_words16 = nullptr;
-    uint32_t* d_lens16 = nullptr;
-    uint32_t* d_gidx16 = nullptr;
-    uint64_t inter16_spill = 0;    // uint2 entries
-    int q16 = 24;        // query rows per register tile of the packed kernel
-    int waves16 = 4;     // wavefronts per workgroup of the packed kernel
-    // Groups [0, groups16_short_first) hold long targets; one-query scans start after them.
-    uint32_t groups16_short_first = 0;
-    uint32_t max_ncols16 = 0;
-    // Systolic list (long targets, or everything when the alphabet leaves no padding symbol).
-    std::vector<uint32_t> sys_list;
-    uint32_t* d_sys_list = nullptr;
-    uint64_t sys_max_len = 0;
-    // Long targets the packed kernel cannot take: the batched form's systolic list.
-    std::vector<uint32_t> sys_list_batch;
-    uint32_t* d_sys_list_batch = nullptr;
-    uint64_t sys_batch_max_len = 0;
-
-    ~hy_db() {
-        (void)hipSetDevice(device);
-        dev_free(d_matrix); dev_free(d_flat); dev_free(d_off);
-        dev_free(d_groups); dev_free(d_words); dev_free(d_lens); dev_free(d_gidx);
-        dev_free(d_groups16); dev_free(d_words16); dev_free(d_lens16); dev_free(d_gidx16);
-        dev_free(d_sys_list);
-        dev_free(d_sys_list_batch);
-    }
-};
-
-struct hy_scratch {
-    int device = 0;
-    const hy_db* db = nullptr;  // sized for this database (not owned)
-    hipStream_t stream = nullptr;
-    uint8_t* d_query = nullptr;
-    uint8_t* h_query = nullptr;          // pinned staging
-    int2* d_spill = nullptr;
-    uint64_t spill_entries = 0;
-    int2* d_sys_spill = nullptr;
-    uint64_t sys_spill_stride = 0;
-    int32_t* d_scores = nullptr;
-    int64_t* d_qend = nullptr;
-    int64_t* d_tend = nullptr;
-    unsigned long long* d_key = nullptr;
-    Result* d_result = nullptr;
-    Result* h_result = nullptr;          // pinned
-    // Instrumentation: events around the per-target kernels of the last scan.
-    bool profile = false;
-    bool pending = false;
-    bool pending_tail = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    // Auxiliary stream for the long-target kernel, with its fork/join events.
-    hipStream_t aux = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    double inter_ms = 0.0, tail_ms = 0.0;
-    uint64_t launches = 0;
-    // Batched form (reserved on first use or by hy_scratch_reserve_batch).
-    uint32_t bcap = 0;                   // queries per submission
-    uint32_t bq_stride = 0;              // bytes reserved per query (max_query_len, >= 1)
-    uint8_t* d_bq = nullptr;             // concatenated queries
-    uint8_t* h_bq = nullptr;             // pinned staging
-    uint32_t* d_bmeta = nullptr;         // qoff[cap] | qlen[cap] | qorder[cap] | nq | counter
-    uint32_t* h_bmeta = nullptr;         // pinned
-    unsigned long long* d_bkeys = nullptr;
-    hy::Result* d_bres = nullptr;
-    hy::Result* h_bres = nullptr;        // pinned
-    uint2* d_slots = nullptr;            // persistent kernel boundary rows
-    uint64_t slot_stride = 0;
-    uint32_t nslots = 0;
-    int2* d_bwin_spill = nullptr;        // winner pass, multi-pass queries
-    int2* d_btail_spill = nullptr;       // long-target tail, one query at a time
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    double batch_ms = 0.0;               // instrumentation: persistent kernel time
-    uint64_t batch_launches = 0;
-    hipEvent_t bev[2] = {nullptr, nullptr};
-
-    ~hy_scratch() {
-        (void)hipSetDevice(device);
-        if (stream) (void)hipStreamSynchronize(stream);
-        dev_free(d_query); dev_free(d_spill); dev_free(d_sys_spill);
-        dev_free(d_scores); dev_free(d_qend); dev_free(d_tend);
-        dev_free(d_key); dev_free(d_result);
-        dev_free(d_bq); dev_free(d_bmeta); dev_free(d_bkeys); dev_free(d_bres);
-        dev_free(d_slots); dev_free(d_bwin_spill); dev_free(d_btail_spill);
-        if (h_bq) (void)hipHostFree(h_bq);
-        if (h_bmeta) (void)hipHostFree(h_bmeta);
-        if (h_bres) (void)hipHostFree(h_bres);
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
-        for (auto& e : bev)
-            if (e) (void)hipEventDestroy(e);
-        if (h_query) (void)hipHostFree(h_query);
-        if (h_result) (void)hipHostFree(h_result);
-        if (aux) (void)hipStreamSynchronize(aux);
-        for (auto& e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
-        if (aux) (void)hipStreamDestroy(aux);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 // ---------------------------------------------------------------------------------------------
 // Backend resolution (src/backend.rs)
@@ -360,10 +230,10 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     // leaves room for one gap opening below the bound (bound + gap_open <= 32767, so H - go never
     // saturates) runs in the packed 16-bit kernel; every other target runs at 32 bits. Any width
     // the proof accepts is exact (DETERMINISM.md §4, §6), so the split changes speed only.
-    // Tile shape of the packed kernel: 24 query rows per register tile, one wavefront per
+    // Tile shape of the packed kernel: 32 query rows per register tile, one wavefront per
     // workgroup with a per-symbol profile; HYALITE_HIP_TILE="rows,waves" overrides it (waves 4 or
-    // 8 select workgroups sharing a residue-pair profile), as the tuning sweeps do.
-    db->q16 = 24;
+    // 8 select workgroups sharing a residue-pair profile). A Scratch can change it (hy_tuning).
+    db->q16 = 32;
     db->waves16 = 1;
     if (const char* t = std::getenv("HYALITE_HIP_TILE")) {
         int q = 0, w = 0;
@@ -438,6 +308,10 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
             }
         } else if (!elig16[k]) {
             inter_idx.push_back(k);
+            // The batched form has no 32-bit inter-sequence queue: these targets join its
+            // systolic list.
+            db->sys_list_batch.push_back(k);
+            db->sys_batch_max_len = std::max<uint64_t>(db->sys_batch_max_len, lens[k]);
         }
     }
     const size_t n_not16 = inter_idx.size();
@@ -510,6 +384,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
         for (size_t s = 0; s < ns; ++s) {
             slot_gidx[s] = idx16[s];
             slot_len[s] = (uint32_t)lens[idx16[s]];
+            db->residues16 += slot_len[s];
         }
         uint64_t words = 0, spill = 0;
         auto add_groups16 = [&](size_t lo, size_t hi) {
@@ -603,6 +478,9 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     if (!s) return HY_E_OUT_OF_MEMORY;
     s->device = db->device;
     s->db = db;
+    s->q16 = db->q16;
+    s->waves16 = db->waves16;
+    s->bq16 = db->q16;
     auto fail = [&](int code) { delete s; return code; };
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
@@ -633,6 +511,9 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
         hipMalloc((void**)&s->d_result, sizeof(Result)) != hipSuccess ||
         hipHostMalloc((void**)&s->h_result, sizeof(Result)) != hipSuccess)
         return fail(HY_E_OUT_OF_MEMORY);
+    s->dev_bytes = qcap + s->spill_entries * sizeof(int2) + sys_entries * sizeof(int2) +
+                   n * (sizeof(int32_t) + 2 * sizeof(int64_t)) + sizeof(unsigned long long) +
+                   sizeof(Result);
     *out = s;
     return HY_OK;
 }
@@ -754,7 +635,7 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
         pa.spill = reinterpret_cast<uint2*>(s->d_spill);
         pa.out_score = ia.out_score;
         pa.key = key;
-        if (hy::launch_inter16((int)db->mode, pa, db->q16, db->waves16, s->stream) != hipSuccess)
+        if (hy::launch_inter16((int)db->mode, pa, s->q16, s->waves16, s->stream) != hipSuccess)
             return HY_E_DEVICE;
     }
     if (s->profile && hipEventRecord(s->ev[1], s->stream) != hipSuccess) return HY_E_DEVICE;
@@ -871,14 +752,10 @@ extern "C" int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* que
 // Batched form
 // ---------------------------------------------------------------------------------------------
 
-extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
-    if (!s) return HY_E_NULL_ARGUMENT;
-    if (max_queries == 0 || max_queries > 65536) return HY_E_INVALID_ARGUMENT;
-    if (s->bcap >= max_queries) return HY_OK;
-    const hy_db* db = s->db;
-    if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
-    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
-    // Release a smaller reservation (and the graph captured for it).
+namespace {
+
+// Release the batched form's buffers and the graph captured for them.
+void release_batch(hy_scratch* s) {
     dev_free(s->d_bq); dev_free(s->d_bmeta); dev_free(s->d_bkeys); dev_free(s->d_bres);
     dev_free(s->d_slots); dev_free(s->d_bwin_spill); dev_free(s->d_btail_spill);
     if (s->h_bq) (void)hipHostFree(s->h_bq);
@@ -891,6 +768,20 @@ extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
     s->h_bq = nullptr; s->h_bmeta = nullptr; s->h_bres = nullptr;
     s->gexec = nullptr; s->graph = nullptr;
     s->bcap = 0;
+    s->nslots = 0;
+    s->batch_bytes = 0;
+}
+
+}  // namespace
+
+extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
+    if (!s) return HY_E_NULL_ARGUMENT;
+    if (max_queries == 0 || max_queries > 65536) return HY_E_INVALID_ARGUMENT;
+    if (s->bcap >= max_queries) return HY_OK;
+    const hy_db* db = s->db;
+    if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    release_batch(s);  // a smaller reservation
 
     const uint32_t cap = (uint32_t)max_queries;
     const uint32_t stride = (uint32_t)std::max<uint64_t>(db->mql, 1);
@@ -907,7 +798,9 @@ extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
     std::memset(s->h_bq, 0, qbytes);
     std::memset(s->h_bmeta, 0, meta * sizeof(uint32_t));
     if (!db->groups16.empty()) {
-        s->nslots = hy::inter16_batch_workgroups((int)db->mode, db->q16, (int)db->al_eff);
+        uint32_t per_cu = hy::inter16_batch_occupancy((int)db->mode, s->bq16, (int)db->al_eff);
+        if (s->bwpc > 0) per_cu = std::min<uint32_t>(per_cu, (uint32_t)s->bwpc);
+        s->nslots = per_cu * hy::device_cus();
         s->slot_stride = (uint64_t)(db->max_ncols16 + 1) * 64;
         if (s->nslots == 0 ||
             hipMalloc((void**)&s->d_slots, (size_t)s->nslots * s->slot_stride * sizeof(uint2)) !=
@@ -929,6 +822,12 @@ extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
         if (!e && hipEventCreate(&e) != hipSuccess) return HY_E_DEVICE;
     s->bq_stride = stride;
     s->bcap = cap;
+    s->batch_bytes = qbytes + meta * sizeof(uint32_t) +
+                     cap * (sizeof(unsigned long long) + sizeof(hy::Result)) +
+                     (uint64_t)s->nslots * s->slot_stride * sizeof(uint2);
+    if (multipass)
+        s->batch_bytes += (uint64_t)cap * (db->mtl + 1) * sizeof(int2) +
+                          db->sys_list_batch.size() * (db->sys_batch_max_len + 1) * sizeof(int2);
     return HY_OK;
 }
 
@@ -1009,7 +908,7 @@ int enqueue_batch(const hy_db* db, hy_scratch* s, bool timed) {
         b.slots = s->d_slots;
         b.slot_stride = s->slot_stride;
         if (timed && hipEventRecord(s->bev[0], st) != hipSuccess) return HY_E_DEVICE;
-        if (hy::launch_inter16_batch((int)db->mode, pa, b, db->q16, s->nslots, st) != hipSuccess)
+        if (hy::launch_inter16_batch((int)db->mode, pa, b, s->bq16, s->nslots, st) != hipSuccess)
             return HY_E_DEVICE;
         if (timed && hipEventRecord(s->bev[1], st) != hipSuccess) return HY_E_DEVICE;
     }
@@ -1060,25 +959,15 @@ extern "C" int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* cons
         if (rc != HY_OK) return rc;
     }
     if (nq == 0) return HY_OK;
-    // Targets not eligible for the packed kernel and not long (a 32-bit inter partition) have no
-    // batched kernel: such databases run the queries one scan at a time.
-    if (db->groups_score32 > 0) {
-        for (size_t k = 0; k < nq; ++k) {
-            const int rc = hy_scan(db, s, queries[k], qlens[k], &scores[k], &db_indices[k],
-                                   &query_ends[k], &target_ends[k]);
-            if (rc != HY_OK) return rc;
-        }
-        return HY_OK;
-    }
     int rc = HY_OK;
-    if (s->bcap < std::min<size_t>(nq, 256) &&
-        (rc = hy_scratch_reserve_batch(s, std::min<size_t>(nq, 256))) != HY_OK)
-        return rc;
+    const size_t want = std::min<size_t>(nq, s->bchunk);
+    if (s->bcap < want && (rc = hy_scratch_reserve_batch(s, want)) != HY_OK) return rc;
     if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
     const uint32_t cap = s->bcap;
+    const uint32_t per_sub = std::min<uint32_t>(cap, s->bchunk);
     std::vector<uint32_t> order;
-    for (size_t base = 0; base < nq; base += cap) {
-        const uint32_t cnt = (uint32_t)std::min<size_t>(cap, nq - base);
+    for (size_t base = 0; base < nq; base += per_sub) {
+        const uint32_t cnt = (uint32_t)std::min<size_t>(per_sub, nq - base);
         uint32_t* qoff = s->h_bmeta;
         uint32_t* qlen = s->h_bmeta + cap;
         uint32_t* qord = s->h_bmeta + 2 * cap;
@@ -1149,6 +1038,42 @@ extern "C" int hy_scratch_batch_profile_read(hy_scratch* s, double* kernel_ms, u
     return HY_OK;
 }
 
+extern "C" int hy_scratch_get_tuning(const hy_scratch* s, hy_tuning* out) {
+    if (!s || !out) return HY_E_NULL_ARGUMENT;
+    out->tile_rows = s->q16;
+    out->waves = s->waves16;
+    out->batch_tile_rows = s->bq16;
+    out->batch_wgs_per_cu = s->bwpc;
+    out->batch_chunk = (int32_t)s->bchunk;
+    out->reserved = 0;
+    return HY_OK;
+}
+
+extern "C" int hy_scratch_set_tuning(hy_scratch* s, const hy_tuning* t) {
+    if (!s || !t) return HY_E_NULL_ARGUMENT;
+    const hy_db* db = s->db;
+    auto rows_ok = [](int q) { return q == 16 || q == 24 || q == 32; };
+    if (!rows_ok(t->tile_rows) || !rows_ok(t->batch_tile_rows) || t->reserved != 0 ||
+        !(t->waves == 1 || t->waves == 4 || t->waves == 8) || t->batch_wgs_per_cu < 0 ||
+        t->batch_wgs_per_cu > 64 || t->batch_chunk < 1 || t->batch_chunk > 65536)
+        return HY_E_INVALID_ARGUMENT;
+    // Shared-profile workgroups need the residue-pair profile to fit in LDS.
+    if (t->waves > 1 &&
+        hy::inter16_lds_bytes((int)db->al_eff, t->tile_rows) > hy::kPairProfileMaxBytes)
+        return HY_E_INVALID_ARGUMENT;
+    if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+    if (t->batch_tile_rows != s->bq16 || t->batch_wgs_per_cu != s->bwpc) {
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+        release_batch(s);
+    }
+    s->q16 = t->tile_rows;
+    s->waves16 = t->waves;
+    s->bq16 = t->batch_tile_rows;
+    s->bwpc = t->batch_wgs_per_cu;
+    s->bchunk = (uint32_t)t->batch_chunk;
+    return HY_OK;
+}
+
 extern "C" int hy_scratch_profile(hy_scratch* s, int enable) {
     if (!s) return HY_E_NULL_ARGUMENT;
     if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
@@ -1198,6 +1123,7 @@ extern "C" const char* hy_strerror(int code) {
     case HY_E_DEVICE: return "a HIP runtime call failed";
     case HY_E_OUT_OF_MEMORY: return "out of memory";
     case HY_E_INTERNAL: return "device self-check failed";
+    case HY_E_TIMEOUT: return "timed out waiting for a served query";
     default: return "unknown status code";
     }
 }

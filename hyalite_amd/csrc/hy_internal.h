@@ -127,7 +127,7 @@ struct SysArgs {
     int qsel;                 // >= 0: every block uses query qsel instead of blockIdx.y
 };
 
-// Launchers (hy_kernels.hip). All launch on `stream` and never synchronise.
+// Launchers (hy_launch.hip). All launch on `stream` and never synchronise.
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
                            hipStream_t stream);
 hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngroups,
@@ -155,7 +155,23 @@ hipError_t launch_batch_reset(unsigned long long* keys, uint32_t n, unsigned int
                               hipStream_t stream);
 hipError_t launch_key_results(const unsigned long long* keys, Result* r, const uint32_t* nq,
                               uint32_t cap, hipStream_t stream);
-// Resident one-wavefront workgroups of the persistent batch kernel on this device.
-uint32_t inter16_batch_workgroups(int mode, int q, int al_eff);
+// Resident one-wavefront workgroups of the batch kernel per CU at the occupancy limit.
+uint32_t inter16_batch_occupancy(int mode, int q, int al_eff);
+// CUs of the current device.
+uint32_t device_cus();
+
+// Per-mode launchers, instantiated by hy_kernels.hip once per alignment mode.
+template <int MODE>
+hipError_t launch_inter32_m(bool ends, const InterArgs& a, uint32_t ngroups, hipStream_t st);
+template <int MODE>
+hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                          hipStream_t st, uint32_t nq);
+template <int MODE>
+hipError_t launch_inter16_m(const Inter16Args& a, int q, int waves, hipStream_t st);
+template <int MODE>
+hipError_t launch_inter16_batch_m(const Inter16Args& a, const BatchArgs& b, int q, uint32_t wgs,
+                                  hipStream_t st);
+template <int MODE>
+uint32_t inter16_batch_occupancy_m(int q, int al_eff);
 
 }  // namespace hy

@@ -16,118 +16,18 @@
 //
 // Arithmetic is the 32-bit realisation of DETERMINISM.md §1: sentinel i32::MIN/4, wrapping
 // add/sub, exact for every input the width proof accepts.
-#include <hip/hip_runtime.h>
-#include <climits>
+//
+// Every kernel and launcher here is a template over the alignment mode; this file is compiled
+// once per mode (-DHY_KMODE=0..4, the numbering of hy::kSW..kSHW) and each object exports that
+// mode's launchers. hy_launch.hip maps a runtime mode onto them.
+#include "hy_device.h"
 
-#include "hy_internal.h"
+#ifndef HY_KMODE
+#error "hy_kernels.hip is compiled once per alignment mode with -DHY_KMODE=0..4"
+#endif
 
 namespace hy {
 namespace {
-
-__device__ __forceinline__ int wadd(int a, int b) { return (int)((unsigned)a + (unsigned)b); }
-__device__ __forceinline__ int wsub(int a, int b) { return (int)((unsigned)a - (unsigned)b); }
-
-// Flags::for_mode (src/kernel.rs:43-98).
-template <int MODE>
-struct MF {
-    static constexpr bool top = MODE == kHW || MODE == kOV || MODE == kSW;
-    static constexpr bool left = MODE == kOV || MODE == kSW || MODE == kSHW;
-    static constexpr bool lrow = MODE == kHW || MODE == kOV;
-    static constexpr bool lcol = MODE == kOV || MODE == kSHW;
-    static constexpr bool local = MODE == kSW;
-};
-
-// gap_penalty (src/kernel.rs:103-109).
-__device__ __forceinline__ int gap32(int go, int ge, int len) {
-    return len == 0 ? 0 : (int)((long long)go + (long long)(len - 1) * (long long)ge);
-}
-
-// H on the DP border: row 0 (top) or column 0 (left).
-template <int MODE>
-__device__ __forceinline__ int border(int i, int j, int go, int ge) {
-    if (i == 0) return (j == 0 || MF<MODE>::top) ? 0 : -gap32(go, ge, j);
-    return MF<MODE>::left ? 0 : -gap32(go, ge, i);
-}
-
-// Best::consider (src/kernel.rs:113-129): maximise score, then minimise (col, row).
-__device__ __forceinline__ void consider(int& bs, int& bj, int& bi, int s, int j, int i) {
-    // Bitwise, not short-circuit, so the comparison stays branch-free in the cell loops.
-    const bool take = (s > bs) | ((s == bs) & ((j < bj) | ((j == bj) & (i < bi))));
-    bs = take ? s : bs;
-    bj = take ? j : bj;
-    bi = take ? i : bi;
-}
-
-// The answer-region cells that lie on the DP border, reduced in closed form. Borders never
-// increase away from the origin (gaps are non-negative), so on each border segment of the region
-// the cell nearest the origin dominates or ties-and-wins.
-template <int MODE>
-__device__ __forceinline__ void border_best(int m, int n, int go, int ge, int& bs, int& bj,
-                                            int& bi) {
-    bs = INT_MIN; bj = INT_MAX; bi = INT_MAX;
-    if (MF<MODE>::local) { consider(bs, bj, bi, 0, 0, 0); return; }
-    if (m == 0 || n == 0) consider(bs, bj, bi, border<MODE>(m, n, go, ge), n, m);
-    if (MF<MODE>::lrow) consider(bs, bj, bi, border<MODE>(m, 0, go, ge), 0, m);
-    if (MF<MODE>::lcol) consider(bs, bj, bi, border<MODE>(0, n, go, ge), n, 0);
-}
-
-// 64-bit argmax key: higher score first, then the smaller database index.
-__device__ __forceinline__ unsigned long long make_key(int score, uint32_t gi) {
-    return ((unsigned long long)((uint32_t)score ^ 0x80000000u) << 32) |
-           (unsigned long long)(0xFFFFFFFFu - gi);
-}
-__device__ __forceinline__ int key_score(unsigned long long k) {
-    return (int)((uint32_t)(k >> 32) ^ 0x80000000u);
-}
-__device__ __forceinline__ uint32_t key_index(unsigned long long k) {
-    return 0xFFFFFFFFu - (uint32_t)(k & 0xFFFFFFFFull);
-}
-
-__device__ __forceinline__ void wave_argmax(unsigned long long k, unsigned long long* dst) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        const unsigned long long o = __shfl_xor(k, off);
-        k = o > k ? o : k;
-    }
-    // Lane 0 of every wavefront (workgroups may hold several).
-    if ((threadIdx.x & 63) == 0 && k != 0ull) atomicMax(dst, k);
-}
-
-__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__global__ void k_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = 0ull;
-    if (i == 0) *counter = 0u;
-}
-
-__global__ void k_key_results(const unsigned long long* keys, Result* r, const uint32_t* nq,
-                              uint32_t cap) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cap || i >= *nq) return;
-    const unsigned long long k = keys[i];
-    r[i].score = key_score(k);
-    r[i].db_index = key_index(k);
-    r[i].query_end = -1;
-    r[i].target_end = -1;
-    r[i].check = k == 0ull ? 1 : 0;
-    r[i].pad = 0;
-}
-
-__global__ void k_key_result(const unsigned long long* key, Result* r) {
-    if (threadIdx.x == 0) {
-        const unsigned long long k = *key;
-        r->score = key_score(k);
-        r->db_index = key_index(k);
-        r->query_end = -1;
-        r->target_end = -1;
-        r->check = k == 0ull ? 1 : 0;
-        r->pad = 0;
-    }
-}
 
 // ------------------------------------------------------------------------------------------------
 // Inter-sequence kernel, 32-bit lanes.
@@ -339,8 +239,14 @@ __device__ __forceinline__ int border16(int i, int j, int go, int ge) {
 // workgroup; each lane reads its two targets' rows and pairs them with one v_perm_b32 per query
 // row. The second form needs ~1 KB of LDS instead of ~70 KB (protein), so occupancy is bounded
 // by registers only and no barrier couples wavefronts.
+// Register budget of the packed kernels: at least two resident wavefronts per SIMD (at most 256
+// VGPRs), measured faster than one wavefront with more registers for the 32-row local tile.
 #ifndef HY_PACKED_WAVES_PER_EU
-#define HY_PACKED_WAVES_PER_EU 1
+#define HY_PACKED_WAVES_PER_EU 2
+#endif
+// Local mode: 1 = columns in skewed pairs (below), 0 = one column at a time.
+#ifndef HY_SW_SKEW
+#define HY_SW_SKEW 1
 #endif
 // Running state of one wavefront's two-targets-per-lane pass over one group, carried across the
 // query tiles.
@@ -413,16 +319,18 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     // overlaps the current block's compute. The host pads every group's column count to a
     // multiple of four with the inert padding symbol, so the four-column body has no exits and
     // the packed registers stay packed across it.
+    // The boundary inputs are loaded for the first tile too (in bounds, then replaced by the DP
+    // border) and the last block re-reads its own inputs, so the loop body has no branch around
+    // its loads and the wait for them does not also wait for the previous block's stores.
     auto fetch_block = [&](int jb, uint2* dst) {
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const int jn = jb + b + 1;
-            if (firstc)
-                dst[b] = make_uint2(
-                    as_u32(splat2(sat16((long long)border16<MODE>(0, jn, go, ge) - go))),
-                    0x80008000u);
-            else
-                dst[b] = sp[(size_t)jn * 64];
+            const uint2 v = sp[(size_t)jn * 64];
+            dst[b] = firstc ? make_uint2(as_u32(splat2(sat16(
+                                             (long long)border16<MODE>(0, jn, go, ge) - go))),
+                                         0x80008000u)
+                            : v;
         }
     };
     uint2 w = make_uint2(0, 0);
@@ -432,14 +340,10 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
         fetch_block(0, spv);
     }
     for (int jb = 0; jb < L; jb += 4) {
-        uint2 wn = make_uint2(0, 0);
+        const int jnx = jb + 4 < L ? jb + 4 : jb;
+        const uint2 wn = wp[(size_t)(jnx >> 2) * 64];
         uint2 spn[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) spn[b] = make_uint2(0, 0);
-        if (jb + 4 < L) {
-            wn = wp[(size_t)((jb >> 2) + 1) * 64];
-            fetch_block(jb + 4, spn);
-        }
+        fetch_block(jnx, spn);
         uint2 outb[4];
         // Substitution pairs of column b of the block: (sub(q_r, low target), sub(q_r, high
         // target)) + go for the tile's rows.
@@ -477,7 +381,7 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
         // row r-1 of column j+1 are independent — so two dependency chains interleave and hide
         // the latency between dependent packed operations. The cell update and the order in
         // which each column's cells are computed are unchanged.
-        if (F::local) {
+        if (F::local && HY_SW_SKEW) {
             auto cell = [&](int r, s2v sgv, s2v& hgd, s2v& hgup, s2v& f) {
                 const s2v diag = p_add(hgd, sgv);
                 const s2v e = p_max(HG[r], p_sub(E[r], ge2));
@@ -520,14 +424,16 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                     const s2v diag = p_add(hgd, sg[r]);
                     const s2v e = p_max(HG[r], p_sub(E[r], ge2));
                     f = p_max(hgup, p_sub(f, ge2));
-                    const s2v h = p_max(p_max(diag, e), f);
+                    s2v h = p_max(p_max(diag, e), f);
+                    if (F::local) h = p_max(h, zero2);
                     hgd = HG[r];
                     E[r] = e;
                     HG[r] = p_sub(h, go2);
                     hgup = HG[r];
+                    if (F::local) runq[r & 3] = p_max(runq[r & 3], HG[r]);
                 }
                 outb[b] = make_uint2(as_u32(HG[QT - 1]), as_u32(f));
-                if (lastc) {
+                if (!F::local && lastc) {
                     const bool la = F::lrow ? j <= nA : j == nA;
                     const bool lb = F::lrow ? j <= nB : j == nB;
                     s2v v = HG[0];
@@ -824,17 +730,6 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
     }
 }
 
-template <int MODE>
-hipError_t launch_inter32_m(bool ends, const InterArgs& a, uint32_t ngroups, hipStream_t st) {
-    constexpr int Q = 32;
-    const size_t lds = (size_t)(a.al_eff + 1) * (Q + 4) * sizeof(int);
-    if (ends)
-        hipLaunchKernelGGL((k_inter32<MODE, Q, true>), dim3(ngroups), dim3(64), lds, st, a);
-    else
-        hipLaunchKernelGGL((k_inter32<MODE, Q, false>), dim3(ngroups), dim3(64), lds, st, a);
-    return hipGetLastError();
-}
-
 template <int MODE, int R, bool LDSMAT>
 hipError_t launch_sys32_mrl(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
                             hipStream_t st, uint32_t nq) {
@@ -865,17 +760,6 @@ hipError_t launch_sys32_mr(bool ends, bool winner, const SysArgs& a, uint32_t nb
     return launch_sys32_mrl<MODE, R, false>(ends, winner, a, nblocks, st, nq);
 }
 
-// Rows per lane: the smallest of 2, 4, 6, 8 whose 64-lane pass covers the query (one pass up to
-// 512 rows); longer queries take several passes of 8 rows per lane.
-template <int MODE>
-hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
-                          hipStream_t st, uint32_t nq) {
-    if (a.m <= 128) return launch_sys32_mr<MODE, 2>(ends, winner, a, nblocks, st, nq);
-    if (a.m <= 256) return launch_sys32_mr<MODE, 4>(ends, winner, a, nblocks, st, nq);
-    if (a.m <= 384) return launch_sys32_mr<MODE, 6>(ends, winner, a, nblocks, st, nq);
-    return launch_sys32_mr<MODE, 8>(ends, winner, a, nblocks, st, nq);
-}
-
 template <int MODE, int Q, int NW, bool PAIR>
 hipError_t launch_inter16_qw(const Inter16Args& a, hipStream_t st) {
     const uint32_t lds = PAIR ? inter16_lds_bytes(a.al_eff, Q) : inter16p_lds_bytes(a.al_eff, Q);
@@ -889,6 +773,38 @@ hipError_t launch_inter16_qw(const Inter16Args& a, hipStream_t st) {
     }
     hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * NW), lds, st, a);
     return hipGetLastError();
+}
+
+template <int MODE, int Q>
+hipError_t launch_inter16_batch_q(const Inter16Args& a, const BatchArgs& b, uint32_t wgs,
+                                  hipStream_t st) {
+    const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
+    hipLaunchKernelGGL((k_inter16_batch<MODE, Q>), dim3(wgs), dim3(64), lds, st, a, b);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+template <int MODE>
+hipError_t launch_inter32_m(bool ends, const InterArgs& a, uint32_t ngroups, hipStream_t st) {
+    constexpr int Q = 32;
+    const size_t lds = (size_t)(a.al_eff + 1) * (Q + 4) * sizeof(int);
+    if (ends)
+        hipLaunchKernelGGL((k_inter32<MODE, Q, true>), dim3(ngroups), dim3(64), lds, st, a);
+    else
+        hipLaunchKernelGGL((k_inter32<MODE, Q, false>), dim3(ngroups), dim3(64), lds, st, a);
+    return hipGetLastError();
+}
+
+// Rows per lane: the smallest of 2, 4, 6, 8 whose 64-lane pass covers the query (one pass up to
+// 512 rows); longer queries take several passes of 8 rows per lane.
+template <int MODE>
+hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                          hipStream_t st, uint32_t nq) {
+    if (a.m <= 128) return launch_sys32_mr<MODE, 2>(ends, winner, a, nblocks, st, nq);
+    if (a.m <= 256) return launch_sys32_mr<MODE, 4>(ends, winner, a, nblocks, st, nq);
+    if (a.m <= 384) return launch_sys32_mr<MODE, 6>(ends, winner, a, nblocks, st, nq);
+    return launch_sys32_mr<MODE, 8>(ends, winner, a, nblocks, st, nq);
 }
 
 // waves: 1 = one-wavefront workgroups with the per-symbol profile; 4 or 8 = workgroups sharing
@@ -910,16 +826,6 @@ hipError_t launch_inter16_m(const Inter16Args& a, int q, int waves, hipStream_t 
     return launch_inter16_qw<MODE, 32, 4, true>(a, st);
 }
 
-}  // namespace
-
-template <int MODE, int Q>
-hipError_t launch_inter16_batch_q(const Inter16Args& a, const BatchArgs& b, uint32_t wgs,
-                                  hipStream_t st) {
-    const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
-    hipLaunchKernelGGL((k_inter16_batch<MODE, Q>), dim3(wgs), dim3(64), lds, st, a, b);
-    return hipGetLastError();
-}
-
 template <int MODE>
 hipError_t launch_inter16_batch_m(const Inter16Args& a, const BatchArgs& b, int q, uint32_t wgs,
                                   hipStream_t st) {
@@ -928,123 +834,29 @@ hipError_t launch_inter16_batch_m(const Inter16Args& a, const BatchArgs& b, int 
     return launch_inter16_batch_q<MODE, 32>(a, b, wgs, st);
 }
 
-hipError_t launch_inter16_batch(int mode, const Inter16Args& a, const BatchArgs& b, int q,
-                                uint32_t workgroups, hipStream_t stream) {
-    if (a.ngroups == 0 || workgroups == 0) return hipSuccess;
-    switch (mode) {
-    case kSW: return launch_inter16_batch_m<kSW>(a, b, q, workgroups, stream);
-    case kNW: return launch_inter16_batch_m<kNW>(a, b, q, workgroups, stream);
-    case kHW: return launch_inter16_batch_m<kHW>(a, b, q, workgroups, stream);
-    case kOV: return launch_inter16_batch_m<kOV>(a, b, q, workgroups, stream);
-    case kSHW: return launch_inter16_batch_m<kSHW>(a, b, q, workgroups, stream);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-int inter16_tile_rows(int q) { return q <= 16 ? 16 : (q <= 24 ? 24 : 32); }
-
-uint32_t inter16p_lds_bytes(int al_eff, int q) {
-    const uint32_t rows = (uint32_t)inter16_tile_rows(q) / 2;
-    const uint32_t rd = rows % 8 == 4 ? rows : rows + 4;
-    return ((uint32_t)al_eff + 1) * rd * 4u;
-}
-
-uint32_t inter16_lds_bytes(int al_eff, int q) {
-    const uint32_t A = (uint32_t)al_eff + 1;
-    return A * A * (uint32_t)(inter16_tile_rows(q) + 4) * 4u;
-}
-
-hipError_t launch_inter16(int mode, const Inter16Args& a, int q, int waves, hipStream_t stream) {
-    if (a.ngroups == 0) return hipSuccess;
-    switch (mode) {
-    case kSW: return launch_inter16_m<kSW>(a, q, waves, stream);
-    case kNW: return launch_inter16_m<kNW>(a, q, waves, stream);
-    case kHW: return launch_inter16_m<kHW>(a, q, waves, stream);
-    case kOV: return launch_inter16_m<kOV>(a, q, waves, stream);
-    case kSHW: return launch_inter16_m<kSHW>(a, q, waves, stream);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
-                           hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, stream, p, n, v);
-    return hipGetLastError();
-}
-
-hipError_t launch_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter,
-                              hipStream_t stream) {
-    const uint32_t blocks = (n + 255) / 256 + (n == 0);
-    hipLaunchKernelGGL(k_batch_reset, dim3(blocks), dim3(256), 0, stream, keys, n, counter);
-    return hipGetLastError();
-}
-
-hipError_t launch_key_results(const unsigned long long* keys, Result* r, const uint32_t* nq,
-                              uint32_t cap, hipStream_t stream) {
-    if (cap == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_key_results, dim3((cap + 255) / 256), dim3(256), 0, stream, keys, r, nq,
-                       cap);
-    return hipGetLastError();
-}
-
-uint32_t inter16_batch_workgroups(int mode, int q, int al_eff) {
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return 0;
+// Resident one-wavefront workgroups of the batch kernel per CU at the occupancy limit.
+template <int MODE>
+uint32_t inter16_batch_occupancy_m(int q, int al_eff) {
+    int per_cu = 0;
     const int rows = inter16_tile_rows(q);
     const size_t lds = inter16p_lds_bytes(al_eff, rows) + sizeof(uint32_t);
     hipError_t e = hipErrorInvalidValue;
-#define HY_OCC(M, Q)                                                                              \
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(                                             \
-        &per_cu, reinterpret_cast<const void*>(k_inter16_batch<M, Q>), 64, lds)
-#define HY_OCC_Q(M)                                                                               \
-    if (rows == 16) HY_OCC(M, 16); else if (rows == 24) HY_OCC(M, 24); else HY_OCC(M, 32)
-    switch (mode) {
-    case kSW: HY_OCC_Q(kSW); break;
-    case kNW: HY_OCC_Q(kNW); break;
-    case kHW: HY_OCC_Q(kHW); break;
-    case kOV: HY_OCC_Q(kOV); break;
-    case kSHW: HY_OCC_Q(kSHW); break;
-    default: break;
-    }
-#undef HY_OCC_Q
-#undef HY_OCC
+    auto occ = [&](const void* f) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, lds);
+    };
+    if (rows == 16) occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 16>));
+    else if (rows == 24) occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 24>));
+    else occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 32>));
     if (e != hipSuccess || per_cu <= 0) per_cu = 2;
-    return (uint32_t)(per_cu * cus);
+    return (uint32_t)per_cu;
 }
 
-hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream) {
-    hipLaunchKernelGGL(k_key_result, dim3(1), dim3(64), 0, stream, key, r);
-    return hipGetLastError();
-}
-
-hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngroups,
-                          hipStream_t stream) {
-    if (ngroups == 0) return hipSuccess;
-    switch (mode) {
-    case kSW: return launch_inter32_m<kSW>(ends, a, ngroups, stream);
-    case kNW: return launch_inter32_m<kNW>(ends, a, ngroups, stream);
-    case kHW: return launch_inter32_m<kHW>(ends, a, ngroups, stream);
-    case kOV: return launch_inter32_m<kOV>(ends, a, ngroups, stream);
-    case kSHW: return launch_inter32_m<kSHW>(ends, a, ngroups, stream);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
-                        hipStream_t stream, uint32_t nq_grid) {
-    if ((!winner && nblocks == 0) || nq_grid == 0) return hipSuccess;
-    switch (mode) {
-    case kSW: return launch_sys32_m<kSW>(ends, winner, a, nblocks, stream, nq_grid);
-    case kNW: return launch_sys32_m<kNW>(ends, winner, a, nblocks, stream, nq_grid);
-    case kHW: return launch_sys32_m<kHW>(ends, winner, a, nblocks, stream, nq_grid);
-    case kOV: return launch_sys32_m<kOV>(ends, winner, a, nblocks, stream, nq_grid);
-    case kSHW: return launch_sys32_m<kSHW>(ends, winner, a, nblocks, stream, nq_grid);
-    default: return hipErrorInvalidValue;
-    }
-}
+template hipError_t launch_inter32_m<HY_KMODE>(bool, const InterArgs&, uint32_t, hipStream_t);
+template hipError_t launch_sys32_m<HY_KMODE>(bool, bool, const SysArgs&, uint32_t, hipStream_t,
+                                             uint32_t);
+template hipError_t launch_inter16_m<HY_KMODE>(const Inter16Args&, int, int, hipStream_t);
+template hipError_t launch_inter16_batch_m<HY_KMODE>(const Inter16Args&, const BatchArgs&, int,
+                                                     uint32_t, hipStream_t);
+template uint32_t inter16_batch_occupancy_m<HY_KMODE>(int, int);
 
 }  // namespace hy

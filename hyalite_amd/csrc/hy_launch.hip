@@ -1,0 +1,158 @@
+// Mode-independent kernels (argmax keys, results, fills) and the dispatch from a runtime
+// alignment mode to the per-mode launchers that hy_kernels.hip exports (one object per mode).
+#include "hy_device.h"
+
+namespace hy {
+namespace {
+
+__global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = 0ull;
+    if (i == 0) *counter = 0u;
+}
+
+__global__ void k_key_results(const unsigned long long* keys, Result* r, const uint32_t* nq,
+                              uint32_t cap) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cap || i >= *nq) return;
+    const unsigned long long k = keys[i];
+    r[i].score = key_score(k);
+    r[i].db_index = key_index(k);
+    r[i].query_end = -1;
+    r[i].target_end = -1;
+    r[i].check = k == 0ull ? 1 : 0;
+    r[i].pad = 0;
+}
+
+__global__ void k_key_result(const unsigned long long* key, Result* r) {
+    if (threadIdx.x == 0) {
+        const unsigned long long k = *key;
+        r->score = key_score(k);
+        r->db_index = key_index(k);
+        r->query_end = -1;
+        r->target_end = -1;
+        r->check = k == 0ull ? 1 : 0;
+        r->pad = 0;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_inter16_batch(int mode, const Inter16Args& a, const BatchArgs& b, int q,
+                                uint32_t workgroups, hipStream_t stream) {
+    if (a.ngroups == 0 || workgroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_inter16_batch_m<kSW>(a, b, q, workgroups, stream);
+    case kNW: return launch_inter16_batch_m<kNW>(a, b, q, workgroups, stream);
+    case kHW: return launch_inter16_batch_m<kHW>(a, b, q, workgroups, stream);
+    case kOV: return launch_inter16_batch_m<kOV>(a, b, q, workgroups, stream);
+    case kSHW: return launch_inter16_batch_m<kSHW>(a, b, q, workgroups, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int inter16_tile_rows(int q) { return q <= 16 ? 16 : (q <= 24 ? 24 : 32); }
+
+uint32_t inter16p_lds_bytes(int al_eff, int q) {
+    const uint32_t rows = (uint32_t)inter16_tile_rows(q) / 2;
+    const uint32_t rd = rows % 8 == 4 ? rows : rows + 4;
+    return ((uint32_t)al_eff + 1) * rd * 4u;
+}
+
+uint32_t inter16_lds_bytes(int al_eff, int q) {
+    const uint32_t A = (uint32_t)al_eff + 1;
+    return A * A * (uint32_t)(inter16_tile_rows(q) + 4) * 4u;
+}
+
+hipError_t launch_inter16(int mode, const Inter16Args& a, int q, int waves, hipStream_t stream) {
+    if (a.ngroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_inter16_m<kSW>(a, q, waves, stream);
+    case kNW: return launch_inter16_m<kNW>(a, q, waves, stream);
+    case kHW: return launch_inter16_m<kHW>(a, q, waves, stream);
+    case kOV: return launch_inter16_m<kOV>(a, q, waves, stream);
+    case kSHW: return launch_inter16_m<kSHW>(a, q, waves, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
+                           hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter,
+                              hipStream_t stream) {
+    const uint32_t blocks = (n + 255) / 256 + (n == 0);
+    hipLaunchKernelGGL(k_batch_reset, dim3(blocks), dim3(256), 0, stream, keys, n, counter);
+    return hipGetLastError();
+}
+
+hipError_t launch_key_results(const unsigned long long* keys, Result* r, const uint32_t* nq,
+                              uint32_t cap, hipStream_t stream) {
+    if (cap == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_key_results, dim3((cap + 255) / 256), dim3(256), 0, stream, keys, r, nq,
+                       cap);
+    return hipGetLastError();
+}
+
+uint32_t device_cus() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+        return 0;
+    return (uint32_t)cus;
+}
+
+uint32_t inter16_batch_occupancy(int mode, int q, int al_eff) {
+    switch (mode) {
+    case kSW: return inter16_batch_occupancy_m<kSW>(q, al_eff);
+    case kNW: return inter16_batch_occupancy_m<kNW>(q, al_eff);
+    case kHW: return inter16_batch_occupancy_m<kHW>(q, al_eff);
+    case kOV: return inter16_batch_occupancy_m<kOV>(q, al_eff);
+    case kSHW: return inter16_batch_occupancy_m<kSHW>(q, al_eff);
+    default: return 2;
+    }
+}
+
+hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream) {
+    hipLaunchKernelGGL(k_key_result, dim3(1), dim3(64), 0, stream, key, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngroups,
+                          hipStream_t stream) {
+    if (ngroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_inter32_m<kSW>(ends, a, ngroups, stream);
+    case kNW: return launch_inter32_m<kNW>(ends, a, ngroups, stream);
+    case kHW: return launch_inter32_m<kHW>(ends, a, ngroups, stream);
+    case kOV: return launch_inter32_m<kOV>(ends, a, ngroups, stream);
+    case kSHW: return launch_inter32_m<kSHW>(ends, a, ngroups, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
+                        hipStream_t stream, uint32_t nq_grid) {
+    if ((!winner && nblocks == 0) || nq_grid == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_sys32_m<kSW>(ends, winner, a, nblocks, stream, nq_grid);
+    case kNW: return launch_sys32_m<kNW>(ends, winner, a, nblocks, stream, nq_grid);
+    case kHW: return launch_sys32_m<kHW>(ends, winner, a, nblocks, stream, nq_grid);
+    case kOV: return launch_sys32_m<kOV>(ends, winner, a, nblocks, stream, nq_grid);
+    case kSHW: return launch_sys32_m<kSHW>(ends, winner, a, nblocks, stream, nq_grid);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace hy

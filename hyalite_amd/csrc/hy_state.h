@@ -1,0 +1,154 @@
+// Internal state of the C ABI objects: the device-resident database (hy_db) and a thread's
+// scratch (hy_scratch). Shared by the host-side translation units of the library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/hyalite_hip.h"
+#include "hy_internal.h"
+
+namespace hy {
+inline void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+}  // namespace hy
+
+struct hy_db {
+    std::atomic<int> refs{1};
+    int device = 0;
+    // Validated scheme (src/scoring.rs:41-48) and builder fields.
+    uint32_t al = 0, al_eff = 0;
+    int32_t go = 0, ge = 0, min_e = 0, max_e = 0;
+    uint32_t mode = 0, search_type = 0;
+    uint64_t n = 0, mql = 0, mtl = 0, total_residues = 0;
+    int widest_bits = 8;
+    uint64_t count_by_width[3] = {0, 0, 0};
+    // Device copies.
+    int32_t* d_matrix = nullptr;   // al x al, entries clamped to +-kEntryMax32
+    uint8_t* d_flat = nullptr;     // residues in database order
+    uint64_t* d_off = nullptr;     // n+1 offsets
+    // Inter-sequence partition (one target per lane).
+    std::vector<hy::Group> groups;
+    hy::Group* d_groups = nullptr;
+    uint32_t* d_words = nullptr;
+    uint32_t* d_lens = nullptr;
+    uint32_t* d_gidx = nullptr;
+    uint64_t inter_spill = 0;      // int2 entries
+    // Groups [0, groups_score32) hold the targets that are not eligible for the packed kernel;
+    // score-only scans run the 32-bit kernel over those and the packed kernel over the rest.
+    uint32_t groups_score32 = 0;
+    // Packed 16-bit partition (two targets per lane).
+    std::vector<hy::Group> groups16;
+    hy::Group* d_groups16 = nullptr;
+    uint2* d_words16 = nullptr;
+    uint32_t* d_lens16 = nullptr;
+    uint32_t* d_gidx16 = nullptr;
+    uint64_t inter16_spill = 0;    // uint2 entries
+    int q16 = 32;        // query rows per register tile of the packed kernel
+    int waves16 = 4;     // wavefronts per workgroup of the packed kernel
+    // Groups [0, groups16_short_first) hold long targets; one-query scans start after them.
+    uint32_t groups16_short_first = 0;
+    uint32_t max_ncols16 = 0;
+    uint64_t residues16 = 0;       // residues of the packed targets
+    // Systolic list (long targets, or everything when the alphabet leaves no padding symbol).
+    std::vector<uint32_t> sys_list;
+    uint32_t* d_sys_list = nullptr;
+    uint64_t sys_max_len = 0;
+    // Long targets the packed kernel cannot take: the batched form's systolic list.
+    std::vector<uint32_t> sys_list_batch;
+    uint32_t* d_sys_list_batch = nullptr;
+    uint64_t sys_batch_max_len = 0;
+
+    ~hy_db() {
+        (void)hipSetDevice(device);
+        hy::dev_free(d_matrix); hy::dev_free(d_flat); hy::dev_free(d_off);
+        hy::dev_free(d_groups); hy::dev_free(d_words); hy::dev_free(d_lens); hy::dev_free(d_gidx);
+        hy::dev_free(d_groups16); hy::dev_free(d_words16); hy::dev_free(d_lens16); hy::dev_free(d_gidx16);
+        hy::dev_free(d_sys_list);
+        hy::dev_free(d_sys_list_batch);
+    }
+};
+
+struct hy_scratch {
+    int device = 0;
+    const hy_db* db = nullptr;  // sized for this database (not owned)
+    hipStream_t stream = nullptr;
+    uint8_t* d_query = nullptr;
+    uint8_t* h_query = nullptr;          // pinned staging
+    int2* d_spill = nullptr;
+    uint64_t spill_entries = 0;
+    int2* d_sys_spill = nullptr;
+    uint64_t sys_spill_stride = 0;
+    int32_t* d_scores = nullptr;
+    int64_t* d_qend = nullptr;
+    int64_t* d_tend = nullptr;
+    unsigned long long* d_key = nullptr;
+    hy::Result* d_result = nullptr;
+    hy::Result* h_result = nullptr;          // pinned
+    // Instrumentation: events around the per-target kernels of the last scan.
+    bool profile = false;
+    bool pending = false;
+    bool pending_tail = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // Auxiliary stream for the long-target kernel, with its fork/join events.
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    double inter_ms = 0.0, tail_ms = 0.0;
+    uint64_t launches = 0;
+    // Batched form (reserved on first use or by hy_scratch_reserve_batch).
+    uint32_t bcap = 0;                   // queries per submission
+    uint32_t bq_stride = 0;              // bytes reserved per query (max_query_len, >= 1)
+    uint8_t* d_bq = nullptr;             // concatenated queries
+    uint8_t* h_bq = nullptr;             // pinned staging
+    uint32_t* d_bmeta = nullptr;         // qoff[cap] | qlen[cap] | qorder[cap] | nq | counter
+    uint32_t* h_bmeta = nullptr;         // pinned
+    unsigned long long* d_bkeys = nullptr;
+    hy::Result* d_bres = nullptr;
+    hy::Result* h_bres = nullptr;        // pinned
+    uint2* d_slots = nullptr;            // persistent kernel boundary rows
+    uint64_t slot_stride = 0;
+    uint32_t nslots = 0;
+    int2* d_bwin_spill = nullptr;        // winner pass, multi-pass queries
+    int2* d_btail_spill = nullptr;       // long-target tail, one query at a time
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double batch_ms = 0.0;               // instrumentation: persistent kernel time
+    uint64_t batch_launches = 0;
+    hipEvent_t bev[2] = {nullptr, nullptr};
+    // Launch shapes (hy_tuning), initialised from the database.
+    int q16 = 32, waves16 = 1;           // one-query scans
+    int bq16 = 32, bwpc = 0;             // batched form: tile rows, workgroups per CU (0 = max)
+    uint32_t bchunk = 256;               // batched form: queries per submission
+    uint64_t dev_bytes = 0;              // device memory held by the one-query buffers
+    uint64_t batch_bytes = 0;            // device memory held by the batched-form reservation
+
+    ~hy_scratch() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        hy::dev_free(d_query); hy::dev_free(d_spill); hy::dev_free(d_sys_spill);
+        hy::dev_free(d_scores); hy::dev_free(d_qend); hy::dev_free(d_tend);
+        hy::dev_free(d_key); hy::dev_free(d_result);
+        hy::dev_free(d_bq); hy::dev_free(d_bmeta); hy::dev_free(d_bkeys); hy::dev_free(d_bres);
+        hy::dev_free(d_slots); hy::dev_free(d_bwin_spill); hy::dev_free(d_btail_spill);
+        if (h_bq) (void)hipHostFree(h_bq);
+        if (h_bmeta) (void)hipHostFree(h_bmeta);
+        if (h_bres) (void)hipHostFree(h_bres);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& e : bev)
+            if (e) (void)hipEventDestroy(e);
+        if (h_query) (void)hipHostFree(h_query);
+        if (h_result) (void)hipHostFree(h_result);
+        if (aux) (void)hipStreamSynchronize(aux);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (aux) (void)hipStreamDestroy(aux);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+

@@ -162,3 +162,24 @@ def c5_serving_queries(seed: int = 5, nq: int = 64) -> list:
     p = protein_freqs()
     lens = lognormal_lens(rng, nq, 350, 20, 3000)
     return [_draw(rng, int(L), p) for L in lens]
+
+
+def c4_batch(n_targets: int = 1_000_000, nq: int = 256) -> Workload:
+    """Batched form: 256 protein queries, lengths log-uniform in [20, 2000], against the protein
+    database of the single-query protein workload rebuilt with max_query_len 2000."""
+    w = c2_protein(n_targets=n_targets, mql=2000)
+    w.name = f"protein_sw_scoreend_batch{nq}x{n_targets}"
+    w.queries = c4_batch_queries(nq=nq)
+    w.note += "; queries log-uniform [20, 2000] (seed 4)"
+    return w
+
+
+def c5_serving(n_targets: int = 2_000_000, nq: int = 64) -> Workload:
+    """Serving form: protein queries with log-normal lengths (median 350, clipped [20, 3000])
+    against 2M targets drawn like the single-query protein database (seed 5), max_query_len 3000,
+    so targets of 2979 residues or more need the 32-bit width."""
+    w = c2_protein(seed=5, n_targets=n_targets, mql=3000)
+    w.name = f"protein_sw_scoreend_serving{nq}x{n_targets}"
+    w.queries = c5_serving_queries(nq=nq)
+    w.note += "; queries log-normal median 350 [20, 3000] (seed 5)"
+    return w

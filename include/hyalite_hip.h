@@ -71,7 +71,8 @@ enum {
     HY_E_INVALID_ARGUMENT = 103,
     HY_E_DEVICE = 104,               /* a HIP runtime call failed */
     HY_E_OUT_OF_MEMORY = 105,
-    HY_E_INTERNAL = 106              /* a device-side self-check failed */
+    HY_E_INTERNAL = 106,             /* a device-side self-check failed */
+    HY_E_TIMEOUT = 107               /* hy_server_wait: not completed within the timeout */
 };
 
 /* Field flags for hy_db_build's `have` mask: which builder fields were set. A field left out
@@ -145,6 +146,35 @@ int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
  * (longest packed target + 1) x 512 bytes per resident workgroup. */
 int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries);
 
+/* Launch shapes of the packed 16-bit kernels, held per scratch (defaults come from the database:
+ * 24-row tiles, one wavefront per workgroup, the occupancy limit of resident workgroups and
+ * 256 queries per batched submission). Shapes change speed only, never results. No reference
+ * counterpart (the reference picks its SIMD layout in src/inter.rs:277-386). */
+typedef struct hy_tuning {
+    int32_t tile_rows;        /* one-query scans: query rows per register tile (16, 24 or 32) */
+    int32_t waves;            /* one-query scans: wavefronts per workgroup (1, 4 or 8; 4 and 8
+                                 share a residue-pair profile in LDS) */
+    int32_t batch_tile_rows;  /* batched form: query rows per register tile (16, 24 or 32) */
+    int32_t batch_wgs_per_cu; /* batched form: resident workgroups per CU (0 = occupancy limit) */
+    int32_t batch_chunk;      /* batched form: queries per submission (1..65536) */
+    int32_t reserved;         /* must be 0 */
+} hy_tuning;
+
+int hy_scratch_get_tuning(const hy_scratch* s, hy_tuning* out);
+/* Validates every field (HY_E_INVALID_ARGUMENT otherwise); a change of the batched form's
+ * shapes releases its reservation, which the next batched scan makes again. */
+int hy_scratch_set_tuning(hy_scratch* s, const hy_tuning* t);
+
+/* Autotuner: times candidate shapes on the given sample queries (at most 8 are used, spread over
+ * their length range) against the whole database and keeps the fastest in the scratch. forms:
+ * bit 0 = one-query scans (tile rows x waves), bit 1 = batched form (tile rows x resident
+ * workgroups; the submission size is the smallest whose predicted makespan, from the measured
+ * rates, is within 3% of the ideal). With HYALITE_HIP_TUNE_CACHE naming a file, results are
+ * looked up there first (key: device, mode, alphabet, database shape, sample lengths) and
+ * appended after tuning. `out` (optional) receives the chosen shapes. */
+int hy_scratch_autotune(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
+                        const size_t* qlens, size_t nq, uint32_t forms, hy_tuning* out);
+
 /* Instrumentation (no counterpart in the reference; for benchmarks and profilers). When enabled,
  * every scan records HIP events on the scratch's own stream around the per-target kernel
  * launches, and hy_scratch_profile_read returns the accumulated device time: `inter` is the
@@ -155,6 +185,52 @@ int hy_scratch_profile_read(hy_scratch* s, double* inter_ms, double* tail_ms, ui
                             int reset);
 /* Device time of the batched form's persistent kernel (profiling runs submit without a graph). */
 int hy_scratch_batch_profile_read(hy_scratch* s, double* kernel_ms, uint64_t* launches, int reset);
+
+/* Serving form (no reference counterpart: the reference is a library without a request loop).
+ * A server keeps the database resident and owns `lanes` scratches, each with its own stream and
+ * worker thread. Submitted queries wait in one FIFO; a free lane admits the queries at its head,
+ * up to max_batch, and runs them as one batched scan (hy_scan_batch), so requests that arrive
+ * while other batches are on the GPU are admitted as soon as a lane is free and their work
+ * overlaps on the device. Admission is bounded by device memory: the number of lanes is the
+ * largest that fits scratch_budget_bytes (at most max_lanes; HY_E_OUT_OF_MEMORY if not even one
+ * fits), each lane's batched-form reservation sized for max_batch queries. Every hit equals
+ * Database::scan of that query alone (src/database.rs:83). */
+typedef struct hy_server hy_server;
+
+typedef struct hy_server_config {
+    uint64_t scratch_budget_bytes; /* device bytes the lanes may hold; 0 = no limit */
+    uint32_t max_lanes;            /* concurrent batches (1..64) */
+    uint32_t max_batch;            /* queries per batch (1..65536) */
+    uint32_t reserved[4];          /* must be 0 */
+} hy_server_config;
+
+typedef struct hy_server_result {
+    int32_t score;
+    int32_t status;                /* HY_OK, or the scan's error code */
+    uint64_t db_index;
+    int64_t query_end;             /* -1 = None */
+    int64_t target_end;
+    uint64_t submit_ns, admit_ns, done_ns;  /* steady-clock timestamps */
+    uint32_t batch_size;           /* queries in the batch that carried this one */
+    uint32_t lane;
+} hy_server_result;
+
+typedef struct hy_server_stats {
+    uint64_t submitted, completed, batches;
+    uint64_t query_residues;       /* sum of completed query lengths */
+    uint32_t lanes, max_batch;
+    uint64_t lane_bytes;           /* device bytes one lane holds */
+} hy_server_stats;
+
+int hy_server_new(const hy_db* db, const hy_server_config* cfg, hy_server** out);
+/* Copies the query; validates it as hy_scan does (codes 100, 101) before queueing. */
+int hy_server_submit(hy_server* srv, const uint8_t* query, size_t qlen, uint64_t* ticket);
+/* Waits for a ticket (timeout_us < 0: no limit). HY_E_TIMEOUT if it has not completed in time
+ * (the ticket stays valid); HY_E_INVALID_ARGUMENT for an unknown or already collected ticket. */
+int hy_server_wait(hy_server* srv, uint64_t ticket, int64_t timeout_us, hy_server_result* out);
+int hy_server_stats_read(hy_server* srv, hy_server_stats* out);
+/* Finishes every queued query, then stops the lanes and frees the server. */
+void hy_server_free(hy_server* srv);
 
 /* Human-readable message for a status code (error.rs:98-157 wording where one exists). */
 const char* hy_strerror(int code);

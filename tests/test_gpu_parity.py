@@ -338,13 +338,17 @@ def test_batch_graph_replays_and_chunking():
 
 
 def test_batch_with_unpackable_long_targets():
-    """Long targets whose proof needs 32 bits go through the batched systolic tail."""
+    """Targets whose proof needs 32 bits (long, and short ones of the 32-bit partition) go through
+    the batched systolic list."""
     rng = np.random.default_rng(88)
     scheme = (4, [40 if a == b else -3 for a in range(4) for b in range(4)], 6, 2)
     seqs = [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in rng.integers(20, 120, 700)]
     q = rng.integers(0, 4, size=900, dtype=np.uint8)
     seqs[5] = np.concatenate([q[:800], rng.integers(0, 4, 400, dtype=np.uint8)])  # i32, long
     seqs[6] = rng.integers(0, 4, size=1500, dtype=np.uint8)
+    # Short (below the long-target threshold) but beyond the packed bound: the 32-bit partition.
+    seqs[7] = np.concatenate([rng.integers(0, 4, 60, dtype=np.uint8), q[100:900]])
+    seqs[8] = rng.integers(0, 4, size=850, dtype=np.uint8)
     db = build(seqs, scheme, MODE_SW, hy.SearchType.ScoreEnd, 900)
     assert db.partition_sizes()[hy.ScoreWidth.I32] >= 1
     sc = hy.Scratch(db)
@@ -366,3 +370,151 @@ def test_every_remainder_tile_size(mode):
     queries = [base[:m] for m in range(1, 74)]
     check(seqs, (20, BLOSUM62, 11, 1), mode, hy.SearchType.ScoreEnd, queries, mql=73)
     check(seqs, (20, BLOSUM62, 11, 1), mode, hy.SearchType.Score, queries[::5], mql=73)
+
+
+# ---- launch shapes and the autotuner ------------------------------------------------------------
+
+def _tuning_db(mode, st):
+    rng = np.random.default_rng(700 + mode)
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(1, 300, 1500)]
+    seqs[11] = rng.integers(0, 20, size=2600, dtype=np.uint8)  # beyond the long-target threshold
+    qs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in (5, 23, 64, 150)]
+    return seqs, qs, build(seqs, (20, BLOSUM62, 11, 1), mode, st, 150)
+
+
+@pytest.mark.parametrize("mode", [MODE_SW, MODE_NW, MODE_OV])
+def test_every_launch_shape_gives_the_same_hits(mode):
+    """Tile rows x workgroup shape of the one-query kernel and tile rows x resident workgroups x
+    submission size of the batch kernel change speed only: every shape equals the oracle."""
+    st = hy.SearchType.ScoreEnd
+    seqs, qs, db = _tuning_db(mode, st)
+    flat, off = orc.flatten(seqs)
+    want = [orc.scan_flat(flat, off, q, 20, BLOSUM62, 11, 1, mode, True, threads=4) for q in qs]
+    want_scores = [orc.scan_all_flat(flat, off, q, 20, BLOSUM62, 11, 1, mode, threads=4)[0]
+                   for q in qs[:2]]
+    sc = hy.Scratch(db)
+    assert sc.tuning() == hy.Tuning()
+    for rows in (16, 24, 32):
+        for waves in (1, 4, 8):
+            sc.set_tuning(hy.Tuning(tile_rows=rows, waves=waves))
+            for q, w in zip(qs, want):
+                h = db.scan(sc, q)
+                assert (h.score, h.db_index, h.query_end, h.target_end) == w, (rows, waves)
+            for q, w in zip(qs, want_scores):
+                assert np.array_equal(db.scan_scores_array(sc, q), w), (rows, waves)
+    for rows in (16, 24, 32):
+        for wpc in (0, 1):
+            for chunk in (3, 256):
+                sc.set_tuning(hy.Tuning(batch_tile_rows=rows, batch_wgs_per_cu=wpc,
+                                        batch_chunk=chunk))
+                got = db.scan_batch(sc, qs)
+                assert [(h.score, h.db_index, h.query_end, h.target_end) for h in got] == want, \
+                    (rows, wpc, chunk)
+
+
+def test_invalid_launch_shapes_are_rejected():
+    seqs, qs, db = _tuning_db(MODE_SW, hy.SearchType.Score)
+    sc = hy.Scratch(db)
+    for bad in (hy.Tuning(tile_rows=20), hy.Tuning(waves=2), hy.Tuning(batch_tile_rows=8),
+                hy.Tuning(batch_chunk=0), hy.Tuning(batch_chunk=65537),
+                hy.Tuning(batch_wgs_per_cu=-1)):
+        with pytest.raises(ValueError):
+            sc.set_tuning(bad)
+    assert sc.tuning() == hy.Tuning()
+
+
+def test_autotune_keeps_results_and_uses_its_cache(tmp_path, monkeypatch):
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("HYALITE_HIP_TUNE_CACHE", str(cache))
+    seqs, qs, db = _tuning_db(MODE_SW, hy.SearchType.ScoreEnd)
+    flat, off = orc.flatten(seqs)
+    want = [orc.scan_flat(flat, off, q, 20, BLOSUM62, 11, 1, MODE_SW, True, threads=4) for q in qs]
+    sc = hy.Scratch(db)
+    t = sc.autotune(qs)
+    assert sc.tuning() == t
+    assert t.tile_rows in (16, 24, 32) and t.waves in (1, 4, 8)
+    assert t.batch_tile_rows in (16, 24, 32) and 1 <= t.batch_chunk <= 65536
+    assert [(h.score, h.db_index, h.query_end, h.target_end) for h in db.scan_batch(sc, qs)] == want
+    for q, w in zip(qs, want):
+        h = db.scan(sc, q)
+        assert (h.score, h.db_index, h.query_end, h.target_end) == w
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1
+    other = hy.Scratch(db)
+    assert other.autotune(qs) == t  # served from the cache
+    assert len(cache.read_text().splitlines()) == 1
+
+
+# ---- serving form -------------------------------------------------------------------------------
+
+def _serving_db(st=hy.SearchType.ScoreEnd):
+    rng = np.random.default_rng(900)
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(1, 400, 3000)]
+    seqs[17] = rng.integers(0, 20, size=3500, dtype=np.uint8)
+    qs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(1, 300, 40)]
+    qs[5] = seqs[1234][10:200].copy()  # a strong hit
+    db = build(seqs, (20, BLOSUM62, 11, 1), MODE_SW, st, 300)
+    flat, off = orc.flatten(seqs)
+    want = [orc.scan_flat(flat, off, q, 20, BLOSUM62, 11, 1, MODE_SW,
+                          st == hy.SearchType.ScoreEnd, threads=4) for q in qs]
+    return db, qs, want
+
+
+def _tuple(h):
+    return (h.score, h.db_index, h.query_end, h.target_end)
+
+
+@pytest.mark.parametrize("st", ST)
+def test_served_hit_is_the_same_alone_and_in_the_mix(st):
+    """Each served hit equals the oracle's scan of that query, whether it ran alone, in a burst
+    where it shares batches with others, or under Poisson arrivals while batches overlap."""
+    from hyalite_amd import loadgen
+    db, qs, want = _serving_db(st)
+    with hy.Server(db, max_lanes=2, max_batch=8) as srv:
+        alone = [srv.wait(srv.submit(q)) for q in qs[:6]]
+        assert all(a.batch_size == 1 for a in alone)
+        assert [_tuple(a.hit) for a in alone] == want[:6]
+        tickets = [srv.submit(q) for q in qs]
+        burst = [srv.wait(t) for t in tickets]
+        assert [_tuple(b.hit) for b in burst] == want
+        assert max(b.batch_size for b in burst) > 1
+        res = loadgen.run_poisson(srv, qs, qps=400.0, seed=1)
+        assert [_tuple(s.hit) for s in res.served] == want
+        assert all(s.latency_s >= s.queued_s >= 0 for s in res.served)
+        stt = srv.stats()
+        assert stt.completed == stt.submitted == 6 + 2 * len(qs)
+        assert stt.lanes == 2
+
+
+def test_server_memory_budget_sets_the_lanes():
+    db, qs, want = _serving_db()
+    with hy.Server(db, max_lanes=4, max_batch=8) as srv:
+        lane = srv.stats().lane_bytes
+        assert srv.stats().lanes == 4 and lane > 0
+    with hy.Server(db, scratch_budget_bytes=int(lane * 2.5), max_lanes=4, max_batch=8) as srv:
+        assert srv.stats().lanes == 2
+        assert [_tuple(srv.wait(srv.submit(q)).hit) for q in qs[:3]] == want[:3]
+    with pytest.raises(hy.DeviceError):
+        hy.Server(db, scratch_budget_bytes=lane // 2, max_lanes=4, max_batch=8)
+    for bad in (dict(max_lanes=0), dict(max_batch=0), dict(max_lanes=65)):
+        with pytest.raises(ValueError):
+            hy.Server(db, **bad)
+
+
+def test_server_contract_errors():
+    db, qs, want = _serving_db()
+    with hy.Server(db, max_lanes=1, max_batch=4) as srv:
+        with pytest.raises(hy.ScanContractViolation):
+            srv.submit(np.zeros(301, dtype=np.uint8))
+        with pytest.raises(hy.ScanContractViolation):
+            srv.submit(np.array([0, 20], dtype=np.uint8))
+        with pytest.raises(KeyError):
+            srv.wait(12345)
+        t = srv.submit(qs[0])
+        try:
+            got = srv.wait(t, timeout=0.0)
+        except hy.ServerTimeout:
+            got = srv.wait(t)
+        assert _tuple(got.hit) == want[0]
+        with pytest.raises(KeyError):
+            srv.wait(t)  # already collected
