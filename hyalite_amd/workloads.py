@@ -33,6 +33,7 @@ class Workload:
     off: np.ndarray
     queries: list = field(default_factory=list)
     note: str = ""
+    extra: dict = field(default_factory=dict)
 
     @property
     def n(self) -> int:
@@ -169,6 +170,7 @@ def c4_batch(n_targets: int = 1_000_000, nq: int = 256) -> Workload:
     database of the single-query protein workload rebuilt with max_query_len 2000."""
     w = c2_protein(n_targets=n_targets, mql=2000)
     w.name = f"protein_sw_scoreend_batch{nq}x{n_targets}"
+    w.extra["single_query"] = w.queries[0]  # the 360-aa query of the single-query workload
     w.queries = c4_batch_queries(nq=nq)
     w.note += "; queries log-uniform [20, 2000] (seed 4)"
     return w
