@@ -194,11 +194,11 @@ __global__ __launch_bounds__(64) void k_inter32(InterArgs a) {
 // ------------------------------------------------------------------------------------------------
 //
 // The same recurrence as k_inter32 on pairs of 16-bit values with saturating packed arithmetic
-// (v_pk_add_i16/v_pk_sub_i16 with clamp, v_pk_max_i16): the i8/i16 realisation of
-// DETERMINISM.md §1, sentinel = i16::MIN. A lane's two targets read different residues, so the
-// profile is indexed by the residue PAIR: pp[(a * A + b) * S + r] holds (sub(q_r, a) + go,
-// sub(q_r, b) + go) saturated to 16 bits, A = al_eff + 1 symbols including padding. The tile of
-// Q query rows is shared by all wavefronts of the workgroup.
+// (v_pk_add_i16/v_pk_sub_i16/v_pk_sub_u16 with clamp, v_pk_max_i16): the i8/i16 realisation of
+// DETERMINISM.md §1, sentinel = i16::MIN. A lane's two targets read different residues; the
+// profile of a tile holds sub(q_r, s) + go (local mode: sub(q_r, s)) saturated to 16 bits for
+// every symbol s < A = al_eff + 1 (the padding symbol's entries are i16::MIN), either per symbol
+// (paired per lane with v_perm_b32) or per residue pair (PAIR: pp[(a * A + b) * S + r]).
 //
 // Exactness: the host sends a target here only when its width proof gives |cell| <= bound with
 // bound + gap_open <= 32767, so H - gap_open never saturates on a real cell and
@@ -212,6 +212,12 @@ __device__ __forceinline__ uint32_t as_u32(s2v v) { return __builtin_bit_cast(ui
 __device__ __forceinline__ s2v p_add(s2v a, s2v b) { return __builtin_elementwise_add_sat(a, b); }
 __device__ __forceinline__ s2v p_sub(s2v a, s2v b) { return __builtin_elementwise_sub_sat(a, b); }
 __device__ __forceinline__ s2v p_max(s2v a, s2v b) { return __builtin_elementwise_max(a, b); }
+// Unsigned saturating subtraction (v_pk_sub_u16 with clamp): max(a - b, 0) for a, b in [0, 32767].
+typedef unsigned short u2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s2v p_subu(s2v a, s2v b) {
+    return __builtin_bit_cast(s2v, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2v, a),
+                                                                 __builtin_bit_cast(u2v, b)));
+}
 __device__ __forceinline__ int sat16(long long v) {
     return v < -32768 ? -32768 : (v > 32767 ? 32767 : (int)v);
 }
@@ -244,10 +250,6 @@ __device__ __forceinline__ int border16(int i, int j, int go, int ge) {
 #ifndef HY_PACKED_WAVES_PER_EU
 #define HY_PACKED_WAVES_PER_EU 2
 #endif
-// Local mode: 1 = columns in skewed pairs (below), 0 = one column at a time.
-#ifndef HY_SW_SKEW
-#define HY_SW_SKEW 1
-#endif
 // Running state of one wavefront's two-targets-per-lane pass over one group, carried across the
 // query tiles.
 struct PairRun {
@@ -274,7 +276,10 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     const int go = a.go, ge = a.ge;
     const int A = a.al_eff + 1;
     const int rm = m - i0;
-    const s2v go2 = splat2(go), ge2 = splat2(ge), zero2 = splat2(0);
+    const s2v go2 = splat2(go), ge2 = splat2(ge);
+    // Local mode keeps H itself per row and the profile holds sub; the other modes keep
+    // HG = H - go and the profile holds sub + go.
+    const int gadd = F::local ? 0 : go;
     __syncthreads();
     if (PAIR) {
         for (int k = threadIdx.x; k < A * A * QT; k += 64 * NW) {
@@ -283,8 +288,8 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
             int va = -32768, vb = -32768;
             if (i <= m) {
                 const int* row = a.matrix + (int)query[i - 1] * a.al;
-                if (ca < a.al_eff) va = sat16((long long)row[ca] + go);
-                if (cb < a.al_eff) vb = sat16((long long)row[cb] + go);
+                if (ca < a.al_eff) va = sat16((long long)row[ca] + gadd);
+                if (cb < a.al_eff) vb = sat16((long long)row[cb] + gadd);
             }
             pp[pr * S + r] =
                 (uint32_t)(uint16_t)(short)va | ((uint32_t)(uint16_t)(short)vb << 16);
@@ -295,8 +300,8 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
             const int i = i0 + 2 * rr;  // rows i (low half) and i + 1 (high half)
             int v0 = -32768, v1 = -32768;
             if (sym < a.al_eff) {
-                if (i <= m) v0 = sat16((long long)a.matrix[(int)query[i - 1] * a.al + sym] + go);
-                if (i + 1 <= m) v1 = sat16((long long)a.matrix[(int)query[i] * a.al + sym] + go);
+                if (i <= m) v0 = sat16((long long)a.matrix[(int)query[i - 1] * a.al + sym] + gadd);
+                if (i + 1 <= m) v1 = sat16((long long)a.matrix[(int)query[i] * a.al + sym] + gadd);
             }
             pp[sym * RD + rr] = (uint32_t)(uint16_t)(short)v0 | ((uint32_t)(uint16_t)(short)v1 << 16);
         }
@@ -304,13 +309,17 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     __syncthreads();
     if (!wlive) return;
 
+    // Per row: HG = H - go (local mode: H) of the last column, and E of the next column (local
+    // mode: E floored at 0). Column 0 is the DP border.
     s2v HG[QT], E[QT];
 #pragma unroll
     for (int r = 0; r < QT; ++r) {
-        HG[r] = splat2(sat16((long long)border16<MODE>(i0 + r, 0, go, ge) - go));
-        E[r] = splat2(-32768);
+        HG[r] = F::local ? splat2(0)
+                         : splat2(sat16((long long)border16<MODE>(i0 + r, 0, go, ge) - go));
+        E[r] = splat2(F::local ? 0 : -32768);
     }
-    s2v HGd = splat2(sat16((long long)border16<MODE>(i0 - 1, 0, go, ge) - go));
+    s2v HGd = F::local ? splat2(0)
+                       : splat2(sat16((long long)border16<MODE>(i0 - 1, 0, go, ge) - go));
     s2v run = st.run;
     s2v runq[4] = {st.runq[0], st.runq[1], st.runq[2], st.runq[3]};
 
@@ -327,10 +336,13 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
         for (int b = 0; b < 4; ++b) {
             const int jn = jb + b + 1;
             const uint2 v = sp[(size_t)jn * 64];
-            dst[b] = firstc ? make_uint2(as_u32(splat2(sat16(
-                                             (long long)border16<MODE>(0, jn, go, ge) - go))),
-                                         0x80008000u)
-                            : v;
+            // Row 0: local mode (H, F of row 1) = (0, 0); otherwise (H - go, F) of row 0.
+            const uint2 top = F::local
+                                  ? make_uint2(0u, 0u)
+                                  : make_uint2(as_u32(splat2(sat16(
+                                                   (long long)border16<MODE>(0, jn, go, ge) - go))),
+                                               0x80008000u);
+            dst[b] = firstc ? top : v;
         }
     };
     uint2 w = make_uint2(0, 0);
@@ -377,36 +389,38 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                 }
             }
         };
-        // Local mode: columns are processed in pairs skewed by one row — row r of column j and
-        // row r-1 of column j+1 are independent — so two dependency chains interleave and hide
-        // the latency between dependent packed operations. The cell update and the order in
-        // which each column's cells are computed are unchanged.
-        if (F::local && HY_SW_SKEW) {
-            auto cell = [&](int r, s2v sgv, s2v& hgd, s2v& hgup, s2v& f) {
-                const s2v diag = p_add(hgd, sgv);
-                const s2v e = p_max(HG[r], p_sub(E[r], ge2));
-                f = p_max(hgup, p_sub(f, ge2));
-                const s2v h = p_max(p_max(p_max(diag, e), f), zero2);
-                hgd = HG[r];
-                E[r] = e;
-                HG[r] = p_sub(h, go2);
-                hgup = HG[r];
-                runq[r & 3] = p_max(runq[r & 3], HG[r]);
+        // Local mode. Every quantity is kept floored at 0: with H >= 0, E and F only matter
+        // through max(H, 0), and max(a - b, 0) is one unsigned saturating subtraction, so
+        //     H = max(H[i-1][j-1] + sub, E', F')        (>= 0: the local clamp comes free)
+        //     E'[i][j+1] = max(H - go, E' - ge)+,  F'[i+1][j] = max(H - go, F' - ge)+
+        // equals the reference's recurrence for every cell that can win (H is exact; E', F' are
+        // max(E, 0), max(F, 0)). 9 packed operations per cell pair including the running max.
+        // Columns are processed in pairs skewed by one row — row r of column j and row r-1 of
+        // column j+1 are independent — so two dependency chains interleave. The boundary row of a
+        // tile is (H of its last row, F' of the row below).
+        if (F::local) {
+            auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f) {
+                const s2v h = p_max(p_max(p_add(hd, sgv), E[r]), f);
+                hd = HG[r];
+                HG[r] = h;
+                const s2v hg = p_subu(h, go2);
+                E[r] = p_max(hg, p_subu(E[r], ge2));
+                f = p_max(hg, p_subu(f, ge2));
+                runq[r & 3] = p_max(runq[r & 3], h);
             };
 #pragma unroll
             for (int bp = 0; bp < 4; bp += 2) {
                 s2v sgA[QT], sgB[QT];
                 load_sg(bp, sgA);
                 load_sg(bp + 1, sgB);
-                s2v hgupA = as_s2(spv[bp].x), fA = as_s2(spv[bp].y);
-                s2v hgupB = as_s2(spv[bp + 1].x), fB = as_s2(spv[bp + 1].y);
-                s2v hgdA = HGd, hgdB = hgupA;
-                HGd = hgupB;
+                s2v fA = as_s2(spv[bp].y), fB = as_s2(spv[bp + 1].y);
+                s2v hdA = HGd, hdB = as_s2(spv[bp].x);
+                HGd = as_s2(spv[bp + 1].x);
 #pragma unroll
                 for (int r = 0; r <= QT; ++r) {
-                    if (r < QT) cell(r, sgA[r], hgdA, hgupA, fA);
+                    if (r < QT) cell(r, sgA[r], hdA, fA);
                     if (r == QT - 1) outb[bp] = make_uint2(as_u32(HG[QT - 1]), as_u32(fA));
-                    if (r >= 1) cell(r - 1, sgB[r - 1], hgdB, hgupB, fB);
+                    if (r >= 1) cell(r - 1, sgB[r - 1], hdB, fB);
                 }
                 outb[bp + 1] = make_uint2(as_u32(HG[QT - 1]), as_u32(fB));
             }
@@ -424,16 +438,14 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                     const s2v diag = p_add(hgd, sg[r]);
                     const s2v e = p_max(HG[r], p_sub(E[r], ge2));
                     f = p_max(hgup, p_sub(f, ge2));
-                    s2v h = p_max(p_max(diag, e), f);
-                    if (F::local) h = p_max(h, zero2);
+                    const s2v h = p_max(p_max(diag, e), f);
                     hgd = HG[r];
                     E[r] = e;
                     HG[r] = p_sub(h, go2);
                     hgup = HG[r];
-                    if (F::local) runq[r & 3] = p_max(runq[r & 3], HG[r]);
                 }
                 outb[b] = make_uint2(as_u32(HG[QT - 1]), as_u32(f));
-                if (!F::local && lastc) {
+                if (lastc) {
                     const bool la = F::lrow ? j <= nA : j == nA;
                     const bool lb = F::lrow ? j <= nB : j == nB;
                     s2v v = HG[0];
@@ -514,8 +526,7 @@ __device__ __forceinline__ void inter16_group(const Inter16Args& a, const Group&
     if (!wlive) return;
     const s2v run =
         p_max(st.run, p_max(p_max(st.runq[0], st.runq[1]), p_max(st.runq[2], st.runq[3])));
-    const int lift = F::local ? go : 0;
-    const int sA = max(bsA, (int)run.x + lift), sB = max(bsB, (int)run.y + lift);
+    const int sA = max(bsA, (int)run.x), sB = max(bsB, (int)run.y);
     unsigned long long k = 0ull;
     if (liveA) {
         const uint32_t gi = a.gidx[slotA];
