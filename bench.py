@@ -329,7 +329,8 @@ def main() -> int:
     roof = None
     if launches:
         k_ms = kernel_ms / launches
-        k_cells = cells / launches * 1.0  # every target is in the packed partition here
+        # One launch per step; every target is in the packed partition here.
+        k_cells = cells * args.steps / launches
         achieved = k_cells * ops_per_cell(w.mode) / (k_ms / 1e3) / 1e12
         roof = {
             "bound": "valu",

@@ -262,10 +262,10 @@ struct PairRun {
 // One tile of QT query rows starting at row i0 over every column of the group: the profile of
 // those rows is staged in LDS, the rows' H/E columns live in registers, and the tile's bottom row
 // goes to `sp` (column j of this wavefront at sp[j * 64]) for the next tile unless this is the last.
-template <int MODE, int QT, int NW, bool PAIR>
+template <int MODE, int QT, int NW, bool PAIR, bool FIRST>
 __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& g, bool wlive,
-                                             const uint8_t* query, int m, int i0, bool firstc,
-                                             bool lastc, int nA, int nB, uint2* sp, const uint2* wp,
+                                             const uint8_t* query, int m, int i0, bool lastc,
+                                             int nA, int nB, uint2* sp, const uint2* wp,
                                              uint32_t* pp, PairRun& st) {
     using F = MF<MODE>;
     constexpr int S = QT + 4;
@@ -323,39 +323,32 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     s2v run = st.run;
     s2v runq[4] = {st.runq[0], st.runq[1], st.runq[2], st.runq[3]};
 
-    // Column words and the top input of each column (the previous tile's boundary row, or the DP
-    // border for the first tile) are fetched one four-column block ahead, so their HBM latency
-    // overlaps the current block's compute. The host pads every group's column count to a
-    // multiple of four with the inert padding symbol, so the four-column body has no exits and
-    // the packed registers stay packed across it.
-    // The boundary inputs are loaded for the first tile too (in bounds, then replaced by the DP
-    // border) and the last block re-reads its own inputs, so the loop body has no branch around
-    // its loads and the wait for them does not also wait for the previous block's stores.
-    auto fetch_block = [&](int jb, uint2* dst) {
+    // Column words and boundary inputs stream one four-column block ahead: the loads for block
+    // k+1 are issued before block k's compute and pinned there by a scheduling barrier (the
+    // register pressure of the tile would otherwise make the scheduler sink them next to their
+    // use), so their latency overlaps a whole block. The first tile takes its top inputs from the
+    // DP border instead. The host pads every group's column count to a multiple of four with the
+    // inert padding symbol, so a block has no exits and the packed registers stay packed across
+    // it; the prefetch after the last block re-reads that block's inputs (in bounds).
+    auto fetch = [&](int jb, uint2& wq, uint2* top) {
+        wq = wp[(size_t)(jb >> 2) * 64];
+        if (!FIRST) {
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int jn = jb + b + 1;
-            const uint2 v = sp[(size_t)jn * 64];
-            // Row 0: local mode (H, F of row 1) = (0, 0); otherwise (H - go, F) of row 0.
-            const uint2 top = F::local
-                                  ? make_uint2(0u, 0u)
-                                  : make_uint2(as_u32(splat2(sat16(
-                                                   (long long)border16<MODE>(0, jn, go, ge) - go))),
-                                               0x80008000u);
-            dst[b] = firstc ? top : v;
+            for (int b = 0; b < 4; ++b) top[b] = sp[(size_t)(jb + b + 1) * 64];
         }
     };
-    uint2 w = make_uint2(0, 0);
-    uint2 spv[4];
-    if (L > 0) {
-        w = wp[0];
-        fetch_block(0, spv);
-    }
-    for (int jb = 0; jb < L; jb += 4) {
-        const int jnx = jb + 4 < L ? jb + 4 : jb;
-        const uint2 wn = wp[(size_t)(jnx >> 2) * 64];
-        uint2 spn[4];
-        fetch_block(jnx, spn);
+    auto block = [&](int jb, const uint2 w, const uint2* topin) {
+        // Top inputs of the block's columns: row 0 of the DP (local mode: (H, F' of row 1) =
+        // (0, 0); otherwise (H - go, F) of row 0) or the previous tile's boundary row.
+        uint2 spv[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            spv[b] = !FIRST ? topin[b]
+                     : F::local
+                         ? make_uint2(0u, 0u)
+                         : make_uint2(as_u32(splat2(sat16(
+                                          (long long)border16<MODE>(0, jb + b + 1, go, ge) - go))),
+                                      0x80008000u);
         uint2 outb[4];
         // Substitution pairs of column b of the block: (sub(q_r, low target), sub(q_r, high
         // target)) + go for the tile's rows.
@@ -465,13 +458,38 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
 #pragma unroll
             for (int b = 0; b < 4; ++b) sp[(size_t)(jb + b + 1) * 64] = outb[b];
         }
+    };
+    uint2 w = make_uint2(0, 0), wn = make_uint2(0, 0);
+    uint2 top[4], topn[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) top[b] = topn[b] = make_uint2(0, 0);
+    if (L > 0) fetch(0, w, top);
+    for (int jb = 0; jb < L; jb += 4) {
+        fetch(jb + 4 < L ? jb + 4 : jb, wn, topn);
+        __builtin_amdgcn_sched_barrier(0);
+        block(jb, w, top);
         w = wn;
 #pragma unroll
-        for (int b = 0; b < 4; ++b) spv[b] = spn[b];
+        for (int b = 0; b < 4; ++b) top[b] = topn[b];
     }
     st.run = run;
 #pragma unroll
     for (int k = 0; k < 4; ++k) st.runq[k] = runq[k];
+}
+
+// The tile above with FIRST chosen at run time (the first tile of a group starts from the DP
+// border instead of a boundary row).
+template <int MODE, int QT, int NW, bool PAIR>
+__device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Group& g, bool wlive,
+                                                const uint8_t* query, int m, int i0, bool first,
+                                                bool lastc, int nA, int nB, uint2* sp,
+                                                const uint2* wp, uint32_t* pp, PairRun& st) {
+    if (first)
+        inter16_tile<MODE, QT, NW, PAIR, true>(a, g, wlive, query, m, i0, lastc, nA, nB, sp, wp,
+                                               pp, st);
+    else
+        inter16_tile<MODE, QT, NW, PAIR, false>(a, g, wlive, query, m, i0, lastc, nA, nB, sp, wp,
+                                                pp, st);
 }
 
 // One group of 128 targets against one query: full tiles of Q rows, then one tile of the fewest
@@ -505,23 +523,23 @@ __device__ __forceinline__ void inter16_group(const Inter16Args& a, const Group&
     const uint2* wp = a.words + g.word_off + lane;
     const int nfull = m / Q, rem = m - nfull * Q;
     for (int c = 0; c < nfull; ++c)
-        inter16_tile<MODE, Q, NW, PAIR>(a, g, wlive, query, m, c * Q + 1, c == 0,
-                                        rem == 0 && c == nfull - 1, nA, nB, sp, wp, pp, st);
+        inter16_tile_at<MODE, Q, NW, PAIR>(a, g, wlive, query, m, c * Q + 1, c == 0,
+                                           rem == 0 && c == nfull - 1, nA, nB, sp, wp, pp, st);
     if (rem > 0) {
         const int i0 = nfull * Q + 1;
         const bool first = nfull == 0;
         if (rem <= 8)
-            inter16_tile<MODE, 8, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp, wp,
-                                            pp, st);
+            inter16_tile_at<MODE, 8, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
+                                               wp, pp, st);
         else if (Q > 16 && rem <= 16)
-            inter16_tile<MODE, 16, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
-                                             wp, pp, st);
+            inter16_tile_at<MODE, 16, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB,
+                                                sp, wp, pp, st);
         else if (Q > 24 && rem <= 24)
-            inter16_tile<MODE, 24, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
-                                             wp, pp, st);
+            inter16_tile_at<MODE, 24, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB,
+                                                sp, wp, pp, st);
         else
-            inter16_tile<MODE, Q, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
-                                            wp, pp, st);
+            inter16_tile_at<MODE, Q, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
+                                               wp, pp, st);
     }
     if (!wlive) return;
     const s2v run =
@@ -545,7 +563,7 @@ __device__ __forceinline__ void inter16_group(const Inter16Args& a, const Group&
 template <int MODE, int Q, int NW, bool PAIR>
 __global__ __launch_bounds__(64 * NW)
 __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inter16Args a) {
-    extern __shared__ uint32_t pp[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t pp[];
     const uint32_t gix = a.group_first + blockIdx.x * NW + (threadIdx.x >> 6);
     const bool wlive = gix < a.group_first + a.ngroups;
     Group g{};
@@ -561,7 +579,7 @@ template <int MODE, int Q>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_batch(Inter16Args a,
                                                                                    BatchArgs bt) {
-    extern __shared__ uint32_t pp[];
+    extern __shared__ __attribute__((aligned(16))) uint32_t pp[];
     __shared__ uint32_t item_s;
     uint2* slot = bt.slots + (size_t)blockIdx.x * bt.slot_stride;
     const uint32_t nq = *bt.nq;
