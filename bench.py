@@ -51,15 +51,18 @@ VALU_PEAK_TOPS_PK16 = 2 * VALU_PEAK_TOPS_I32
 # counter values are in profiles/.
 MEASURED_TRAFFIC_BYTES = {
     ("protein_sw_scoreend_batch256x1000000", "k_inter16_batch<SW,32>"): 1.355e13,
+    ("protein_sw_scoreend_batch256x1000000", "k_inter16q_batch<32>"): None,
 }
 
 SERVING_QPS = 10.0
 
 
 def ops_per_cell(mode: int) -> int:
-    """Algorithmic integer ops per DP cell: E = 2 sub + max, F = 2 sub + max, diag add,
-    H = 2 max, plus the local-mode clamp."""
-    return 10 if mode == 0 else 9
+    """Integer ops per DP cell of the Gotoh recurrence with H - go computed once and shared by E
+    and F: diag add, HG = H - go, E = max(HG, E - ge) (2), F = max(HG, F - ge) (2), H = 2 max,
+    plus the local-mode clamp. (Counting H - go separately for E and F, as a scalar restatement
+    does, gives 10 and 9.)"""
+    return 9 if mode == 0 else 8
 
 
 def cpu_model() -> str:
@@ -325,7 +328,8 @@ def main() -> int:
         "hit": {"score": h360.score, "db_index": h360.db_index, "query_end": h360.query_end,
                 "target_end": h360.target_end}}
 
-    kernel = f"k_inter16_batch<SW,{tuning.batch_tile_rows}>"
+    kernel = (f"k_inter16q_batch<{tuning.batch_tile_rows}>" if w.mode == 0 and tuning.batch_pairs
+              else f"k_inter16_batch<SW,{tuning.batch_tile_rows}>")
     roof = None
     if launches:
         k_ms = kernel_ms / launches

@@ -344,7 +344,7 @@ class _HyServerStats(ctypes.Structure):
 class _HyTuning(ctypes.Structure):
     _fields_ = [("tile_rows", ctypes.c_int32), ("waves", ctypes.c_int32),
                 ("batch_tile_rows", ctypes.c_int32), ("batch_wgs_per_cu", ctypes.c_int32),
-                ("batch_chunk", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("batch_chunk", ctypes.c_int32), ("batch_pairs", ctypes.c_int32)]
 
 
 @dataclass(frozen=True)
@@ -356,14 +356,16 @@ class Tuning:
     batch_tile_rows: int = 32
     batch_wgs_per_cu: int = 0
     batch_chunk: int = 256
+    batch_pairs: int = 1
 
     def _c(self) -> _HyTuning:
         return _HyTuning(self.tile_rows, self.waves, self.batch_tile_rows, self.batch_wgs_per_cu,
-                         self.batch_chunk, 0)
+                         self.batch_chunk, self.batch_pairs)
 
     @staticmethod
     def _from(c: _HyTuning) -> "Tuning":
-        return Tuning(c.tile_rows, c.waves, c.batch_tile_rows, c.batch_wgs_per_cu, c.batch_chunk)
+        return Tuning(c.tile_rows, c.waves, c.batch_tile_rows, c.batch_wgs_per_cu, c.batch_chunk,
+                      c.batch_pairs)
 
 
 def load_library():

@@ -284,8 +284,19 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     std::iota(order_idx.begin(), order_idx.end(), 0u);
     std::stable_sort(order_idx.begin(), order_idx.end(),
                      [&](uint32_t a, uint32_t b) { return lens[a] > lens[b]; });
-    const double mean = (double)total / (double)n;
-    const uint64_t long_threshold = std::max<uint64_t>(1024, (uint64_t)(8.0 * mean));
+    // A one-query scan lasts as long as its slowest wavefront: a packed group's time grows with
+    // its longest target, so targets longer than 1.6x the average column share of a resident
+    // wavefront (residues / 128 per group / resident workgroups) go to the systolic kernel on a
+    // second stream instead (measured best on the 1M-target protein database: 6.6k GCUPS at
+    // 2,400 columns vs 5.3k at 8x the mean length). HYALITE_HIP_LONG_THRESHOLD overrides it.
+    const uint32_t resident =
+        hy::inter16_batch_occupancy((int)mode, db->q16, (int)db->al_eff) * hy::device_cus();
+    const double share = (double)total / 128.0 / (double)std::max<uint32_t>(resident, 1);
+    uint64_t long_threshold = std::max<uint64_t>(1024, (uint64_t)(1.6 * share));
+    if (const char* t = std::getenv("HYALITE_HIP_LONG_THRESHOLD")) {
+        const long long v = std::atoll(t);
+        if (v > 0) long_threshold = (uint64_t)v;
+    }
     const bool all_systolic = db->al_eff > kNoPadSymbol;
     // 32-bit inter layout order: [targets not eligible for the packed kernel][eligible ones],
     // each part length-sorted and starting a fresh group, so score-only scans can run the 32-bit
@@ -754,6 +765,12 @@ extern "C" int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* que
 
 namespace {
 
+// The batched form pairs queries in the lanes (k_inter16q_batch) in local mode unless the
+// scratch's launch shapes say otherwise.
+bool batch_query_pairs(const hy_db* db, const hy_scratch* s) {
+    return db->mode == HY_MODE_SW && s->bpairs != 0;
+}
+
 // Release the batched form's buffers and the graph captured for them.
 void release_batch(hy_scratch* s) {
     dev_free(s->d_bq); dev_free(s->d_bmeta); dev_free(s->d_bkeys); dev_free(s->d_bres);
@@ -798,7 +815,10 @@ extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
     std::memset(s->h_bq, 0, qbytes);
     std::memset(s->h_bmeta, 0, meta * sizeof(uint32_t));
     if (!db->groups16.empty()) {
-        uint32_t per_cu = hy::inter16_batch_occupancy((int)db->mode, s->bq16, (int)db->al_eff);
+        uint32_t per_cu =
+            batch_query_pairs(db, s)
+                ? hy::inter16q_batch_occupancy(s->bq16, (int)db->al_eff)
+                : hy::inter16_batch_occupancy((int)db->mode, s->bq16, (int)db->al_eff);
         if (s->bwpc > 0) per_cu = std::min<uint32_t>(per_cu, (uint32_t)s->bwpc);
         s->nslots = per_cu * hy::device_cus();
         s->slot_stride = (uint64_t)(db->max_ncols16 + 1) * 64;
@@ -908,8 +928,11 @@ int enqueue_batch(const hy_db* db, hy_scratch* s, bool timed) {
         b.slots = s->d_slots;
         b.slot_stride = s->slot_stride;
         if (timed && hipEventRecord(s->bev[0], st) != hipSuccess) return HY_E_DEVICE;
-        if (hy::launch_inter16_batch((int)db->mode, pa, b, s->bq16, s->nslots, st) != hipSuccess)
-            return HY_E_DEVICE;
+        const hipError_t e =
+            batch_query_pairs(db, s)
+                ? hy::launch_inter16q_batch(pa, b, s->bq16, s->nslots, st)
+                : hy::launch_inter16_batch((int)db->mode, pa, b, s->bq16, s->nslots, st);
+        if (e != hipSuccess) return HY_E_DEVICE;
         if (timed && hipEventRecord(s->bev[1], st) != hipSuccess) return HY_E_DEVICE;
     }
     if (tail && hipStreamWaitEvent(st, s->join, 0) != hipSuccess) return HY_E_DEVICE;
@@ -1045,7 +1068,7 @@ extern "C" int hy_scratch_get_tuning(const hy_scratch* s, hy_tuning* out) {
     out->batch_tile_rows = s->bq16;
     out->batch_wgs_per_cu = s->bwpc;
     out->batch_chunk = (int32_t)s->bchunk;
-    out->reserved = 0;
+    out->batch_pairs = s->bpairs;
     return HY_OK;
 }
 
@@ -1053,7 +1076,8 @@ extern "C" int hy_scratch_set_tuning(hy_scratch* s, const hy_tuning* t) {
     if (!s || !t) return HY_E_NULL_ARGUMENT;
     const hy_db* db = s->db;
     auto rows_ok = [](int q) { return q == 16 || q == 24 || q == 32; };
-    if (!rows_ok(t->tile_rows) || !rows_ok(t->batch_tile_rows) || t->reserved != 0 ||
+    if (!rows_ok(t->tile_rows) || !rows_ok(t->batch_tile_rows) ||
+        !(t->batch_pairs == 0 || t->batch_pairs == 1) ||
         !(t->waves == 1 || t->waves == 4 || t->waves == 8) || t->batch_wgs_per_cu < 0 ||
         t->batch_wgs_per_cu > 64 || t->batch_chunk < 1 || t->batch_chunk > 65536)
         return HY_E_INVALID_ARGUMENT;
@@ -1062,7 +1086,8 @@ extern "C" int hy_scratch_set_tuning(hy_scratch* s, const hy_tuning* t) {
         hy::inter16_lds_bytes((int)db->al_eff, t->tile_rows) > hy::kPairProfileMaxBytes)
         return HY_E_INVALID_ARGUMENT;
     if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
-    if (t->batch_tile_rows != s->bq16 || t->batch_wgs_per_cu != s->bwpc) {
+    if (t->batch_tile_rows != s->bq16 || t->batch_wgs_per_cu != s->bwpc ||
+        t->batch_pairs != s->bpairs) {
         if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
         release_batch(s);
     }
@@ -1071,6 +1096,7 @@ extern "C" int hy_scratch_set_tuning(hy_scratch* s, const hy_tuning* t) {
     s->bq16 = t->batch_tile_rows;
     s->bwpc = t->batch_wgs_per_cu;
     s->bchunk = (uint32_t)t->batch_chunk;
+    s->bpairs = t->batch_pairs;
     return HY_OK;
 }
 

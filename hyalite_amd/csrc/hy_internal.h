@@ -159,6 +159,12 @@ hipError_t launch_key_results(const unsigned long long* keys, Result* r, const u
 uint32_t inter16_batch_occupancy(int mode, int q, int al_eff);
 // CUs of the current device.
 uint32_t device_cus();
+// Batched form, local mode, two queries per lane (hy_kernels.hip compiled for SW): launch,
+// per-CU occupancy and LDS bytes of the paired profile.
+hipError_t launch_inter16q_batch(const Inter16Args& a, const BatchArgs& b, int q, uint32_t wgs,
+                                 hipStream_t st);
+uint32_t inter16q_batch_occupancy(int q, int al_eff);
+uint32_t inter16q_lds_bytes(int al_eff, int q);
 
 // Per-mode launchers, instantiated by hy_kernels.hip once per alignment mode.
 template <int MODE>

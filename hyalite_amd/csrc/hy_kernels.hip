@@ -601,6 +601,197 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_batc
 
 
 // ------------------------------------------------------------------------------------------------
+// Batched form, local mode: two QUERIES per lane.
+// ------------------------------------------------------------------------------------------------
+// Each lane takes one target at a time (the lane's two targets of a group in two passes) against
+// a pair of queries held in the two 16-bit halves, so the tile profile is stored already paired,
+// pq[s * S + r] = (sub(qa_r, s), sub(qb_r, s)), and a column's substitution pairs are read with
+// no per-lane pairing: one v_perm_b32 per cell pair less than the two-targets kernel. The cell
+// update is the local-mode one of inter16_tile; the rows of the shorter query past its end hold
+// i16::MIN in the profile, so their cells only carry gap extensions of real cells and never
+// exceed the real maximum.
+template <int QT, bool FIRST>
+__device__ __forceinline__ void inter16q_tile(const Inter16Args& a, const Group& g, int pass,
+                                              const uint8_t* qa, int ma, const uint8_t* qb,
+                                              int mb, int i0, bool lastc, uint2* sp,
+                                              const uint2* wp, uint32_t* pp, s2v (&runq)[4]) {
+    constexpr int S = QT + 4;  // words per symbol row: 16-byte aligned, rows 4 banks apart
+    const int L = (int)g.ncols;
+    const int A = a.al_eff + 1;
+    const s2v go2 = splat2(a.go), ge2 = splat2(a.ge);
+    __syncthreads();
+    for (int k = threadIdx.x; k < A * QT; k += 64) {
+        const int sym = k / QT, r = k - sym * QT, i = i0 + r;
+        int va = -32768, vb = -32768;
+        if (sym < a.al_eff) {
+            if (i <= ma) va = sat16((long long)a.matrix[(int)qa[i - 1] * a.al + sym]);
+            if (i <= mb) vb = sat16((long long)a.matrix[(int)qb[i - 1] * a.al + sym]);
+        }
+        pp[sym * S + r] = (uint32_t)(uint16_t)(short)va | ((uint32_t)(uint16_t)(short)vb << 16);
+    }
+    __syncthreads();
+
+    s2v HG[QT], E[QT];  // per row: H of the last column and E' of the next one
+#pragma unroll
+    for (int r = 0; r < QT; ++r) HG[r] = E[r] = splat2(0);
+    s2v HGd = splat2(0);
+
+    // Prefetch of the next four-column block pinned ahead of the current block's compute, as in
+    // inter16_tile; a lane's residues are the low (pass 0) or high (pass 1) word of the pair.
+    auto fetch = [&](int jb, uint2& wq, uint2* top) {
+        wq = wp[(size_t)(jb >> 2) * 64];
+        if (!FIRST) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) top[b] = sp[(size_t)(jb + b + 1) * 64];
+        }
+    };
+    auto block = [&](int jb, const uint2 wv, const uint2* topin) {
+        const uint32_t w = pass ? wv.y : wv.x;
+        uint2 spv[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) spv[b] = FIRST ? make_uint2(0u, 0u) : topin[b];
+        uint2 outb[4];
+        auto load_sg = [&](int b, s2v* sg) {
+            const int c = (int)((w >> (8 * b)) & 0xffu);
+            const uint4* row = reinterpret_cast<const uint4*>(&pp[c * S]);
+#pragma unroll
+            for (int r4 = 0; r4 < QT / 4; ++r4) {
+                const uint4 v = row[r4];
+                sg[4 * r4 + 0] = as_s2(v.x); sg[4 * r4 + 1] = as_s2(v.y);
+                sg[4 * r4 + 2] = as_s2(v.z); sg[4 * r4 + 3] = as_s2(v.w);
+            }
+        };
+        auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f) {
+            const s2v h = p_max(p_max(p_add(hd, sgv), E[r]), f);
+            hd = HG[r];
+            HG[r] = h;
+            const s2v hg = p_subu(h, go2);
+            E[r] = p_max(hg, p_subu(E[r], ge2));
+            f = p_max(hg, p_subu(f, ge2));
+            runq[r & 3] = p_max(runq[r & 3], h);
+        };
+#pragma unroll
+        for (int bp = 0; bp < 4; bp += 2) {
+            s2v sgA[QT], sgB[QT];
+            load_sg(bp, sgA);
+            load_sg(bp + 1, sgB);
+            s2v fA = as_s2(spv[bp].y), fB = as_s2(spv[bp + 1].y);
+            s2v hdA = HGd, hdB = as_s2(spv[bp].x);
+            HGd = as_s2(spv[bp + 1].x);
+#pragma unroll
+            for (int r = 0; r <= QT; ++r) {
+                if (r < QT) cell(r, sgA[r], hdA, fA);
+                if (r == QT - 1) outb[bp] = make_uint2(as_u32(HG[QT - 1]), as_u32(fA));
+                if (r >= 1) cell(r - 1, sgB[r - 1], hdB, fB);
+            }
+            outb[bp + 1] = make_uint2(as_u32(HG[QT - 1]), as_u32(fB));
+        }
+        if (!lastc) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) sp[(size_t)(jb + b + 1) * 64] = outb[b];
+        }
+    };
+    uint2 w = make_uint2(0, 0), wn = make_uint2(0, 0);
+    uint2 top[4], topn[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) top[b] = topn[b] = make_uint2(0, 0);
+    if (L > 0) fetch(0, w, top);
+    for (int jb = 0; jb < L; jb += 4) {
+        fetch(jb + 4 < L ? jb + 4 : jb, wn, topn);
+        __builtin_amdgcn_sched_barrier(0);
+        block(jb, w, top);
+        w = wn;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) top[b] = topn[b];
+    }
+}
+
+template <int QT>
+__device__ __forceinline__ void inter16q_tile_at(const Inter16Args& a, const Group& g, int pass,
+                                                 const uint8_t* qa, int ma, const uint8_t* qb,
+                                                 int mb, int i0, bool first, bool lastc,
+                                                 uint2* sp, const uint2* wp, uint32_t* pp,
+                                                 s2v (&runq)[4]) {
+    if (first)
+        inter16q_tile<QT, true>(a, g, pass, qa, ma, qb, mb, i0, lastc, sp, wp, pp, runq);
+    else
+        inter16q_tile<QT, false>(a, g, pass, qa, ma, qb, mb, i0, lastc, sp, wp, pp, runq);
+}
+
+// One group of 128 targets against a query pair (ma >= mb; qb absent when kb is ~0u): two passes,
+// each folding the pass's 64 targets into both queries' argmax keys.
+template <int Q>
+__device__ __forceinline__ void inter16q_group(const Inter16Args& a, const Group& g,
+                                               const uint8_t* qa, int ma, const uint8_t* qb,
+                                               int mb, uint32_t ka, uint32_t kb, uint2* spill,
+                                               unsigned long long* keys, uint32_t* pp) {
+    const int lane = threadIdx.x & 63;
+    uint2* sp = spill + lane;
+    const uint2* wp = a.words + g.word_off + lane;
+    const int nfull = ma / Q, rem = ma - nfull * Q;
+    for (int pass = 0; pass < 2; ++pass) {
+        s2v runq[4] = {splat2(0), splat2(0), splat2(0), splat2(0)};
+        for (int c = 0; c < nfull; ++c)
+            inter16q_tile_at<Q>(a, g, pass, qa, ma, qb, mb, c * Q + 1, c == 0,
+                                rem == 0 && c == nfull - 1, sp, wp, pp, runq);
+        if (rem > 0) {
+            const int i0 = nfull * Q + 1;
+            const bool first = nfull == 0;
+            if (rem <= 8)
+                inter16q_tile_at<8>(a, g, pass, qa, ma, qb, mb, i0, first, true, sp, wp, pp, runq);
+            else if (Q > 16 && rem <= 16)
+                inter16q_tile_at<16>(a, g, pass, qa, ma, qb, mb, i0, first, true, sp, wp, pp,
+                                     runq);
+            else if (Q > 24 && rem <= 24)
+                inter16q_tile_at<24>(a, g, pass, qa, ma, qb, mb, i0, first, true, sp, wp, pp,
+                                     runq);
+            else
+                inter16q_tile_at<Q>(a, g, pass, qa, ma, qb, mb, i0, first, true, sp, wp, pp, runq);
+        }
+        const s2v run = p_max(p_max(runq[0], runq[1]), p_max(runq[2], runq[3]));
+        const int t = 2 * lane + pass;
+        unsigned long long key_a = 0ull, key_b = 0ull;
+        if (t < (int)g.count) {
+            const uint32_t gi = a.gidx[g.first + t];
+            key_a = make_key((int)run.x, gi);
+            key_b = make_key((int)run.y, gi);
+        }
+        wave_argmax(key_a, keys + ka);
+        if (kb != ~0u) wave_argmax(key_b, keys + kb);
+    }
+}
+
+// Persistent batch kernel over (group, query pair) items: queries pair up in descending length
+// order (qorder[2p], qorder[2p + 1]); an odd last query pairs with an empty one.
+template <int Q>
+__global__ __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16q_batch(Inter16Args a,
+                                                                                    BatchArgs bt) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t pp[];
+    __shared__ uint32_t item_s;
+    uint2* slot = bt.slots + (size_t)blockIdx.x * bt.slot_stride;
+    const uint32_t nq = *bt.nq;
+    const uint32_t npairs = (nq + 1) / 2;
+    const uint32_t n_items = npairs * a.ngroups;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) item_s = atomicAdd(bt.counter, 1u);
+        __syncthreads();
+        const uint32_t item = item_s;
+        if (item >= n_items) break;
+        const uint32_t gix = a.group_first + item / npairs;
+        const uint32_t p = item % npairs;
+        const uint32_t ia = bt.qorder[2 * p];
+        const bool hasb = 2 * p + 1 < nq;
+        const uint32_t ib = hasb ? bt.qorder[2 * p + 1] : ia;
+        const Group g = a.groups[gix];
+        inter16q_group<Q>(a, g, bt.queries + bt.qoff[ia], (int)bt.qlen[ia],
+                          bt.queries + bt.qoff[ib], hasb ? (int)bt.qlen[ib] : 0, ia,
+                          hasb ? ib : ~0u, slot, bt.keys, pp);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
 // Intra-sequence systolic kernel, 32-bit lanes, one (query, target) pair per wavefront.
 // ------------------------------------------------------------------------------------------------
 // LDSMAT: the substitution matrix (al_eff^2 entries) is staged in LDS; otherwise it is read from
@@ -879,6 +1070,41 @@ uint32_t inter16_batch_occupancy_m(int q, int al_eff) {
     if (e != hipSuccess || per_cu <= 0) per_cu = 2;
     return (uint32_t)per_cu;
 }
+
+#if HY_KMODE == 0
+uint32_t inter16q_lds_bytes(int al_eff, int q) {
+    return ((uint32_t)al_eff + 1) * ((uint32_t)inter16_tile_rows(q) + 4) * 4u;
+}
+
+hipError_t launch_inter16q_batch(const Inter16Args& a, const BatchArgs& b, int q, uint32_t wgs,
+                                 hipStream_t st) {
+    if (a.ngroups == 0 || wgs == 0) return hipSuccess;
+    const uint32_t lds = inter16q_lds_bytes(a.al_eff, q);
+    const int rows = inter16_tile_rows(q);
+    if (rows == 16)
+        hipLaunchKernelGGL((k_inter16q_batch<16>), dim3(wgs), dim3(64), lds, st, a, b);
+    else if (rows == 24)
+        hipLaunchKernelGGL((k_inter16q_batch<24>), dim3(wgs), dim3(64), lds, st, a, b);
+    else
+        hipLaunchKernelGGL((k_inter16q_batch<32>), dim3(wgs), dim3(64), lds, st, a, b);
+    return hipGetLastError();
+}
+
+uint32_t inter16q_batch_occupancy(int q, int al_eff) {
+    int per_cu = 0;
+    const int rows = inter16_tile_rows(q);
+    const size_t lds = inter16q_lds_bytes(al_eff, rows);
+    hipError_t e = hipErrorInvalidValue;
+    auto occ = [&](const void* f) {
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, lds);
+    };
+    if (rows == 16) occ(reinterpret_cast<const void*>(k_inter16q_batch<16>));
+    else if (rows == 24) occ(reinterpret_cast<const void*>(k_inter16q_batch<24>));
+    else occ(reinterpret_cast<const void*>(k_inter16q_batch<32>));
+    if (e != hipSuccess || per_cu <= 0) per_cu = 2;
+    return (uint32_t)per_cu;
+}
+#endif
 
 template hipError_t launch_inter32_m<HY_KMODE>(bool, const InterArgs&, uint32_t, hipStream_t);
 template hipError_t launch_sys32_m<HY_KMODE>(bool, bool, const SysArgs&, uint32_t, hipStream_t,

@@ -122,6 +122,7 @@ struct hy_scratch {
     int q16 = 32, waves16 = 1;           // one-query scans
     int bq16 = 32, bwpc = 0;             // batched form: tile rows, workgroups per CU (0 = max)
     uint32_t bchunk = 256;               // batched form: queries per submission
+    int bpairs = 1;                      // batched form, local mode: two queries per lane
     uint64_t dev_bytes = 0;              // device memory held by the one-query buffers
     uint64_t batch_bytes = 0;            // device memory held by the batched-form reservation
 

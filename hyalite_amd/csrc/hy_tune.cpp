@@ -118,11 +118,11 @@ bool cache_lookup(const char* path, const std::string& key, hy_tuning* t) {
     char k[512];
     hy_tuning v{};
     bool found = false;
-    while (std::fscanf(f, "%511s %d %d %d %d %d", k, &v.tile_rows, &v.waves, &v.batch_tile_rows,
-                       &v.batch_wgs_per_cu, &v.batch_chunk) == 6) {
+    while (std::fscanf(f, "%511s %d %d %d %d %d %d", k, &v.tile_rows, &v.waves,
+                       &v.batch_tile_rows, &v.batch_wgs_per_cu, &v.batch_chunk,
+                       &v.batch_pairs) == 7) {
         if (key == k) {
             *t = v;
-            t->reserved = 0;
             found = true;  // the last entry for a key wins
         }
     }
@@ -133,8 +133,8 @@ bool cache_lookup(const char* path, const std::string& key, hy_tuning* t) {
 void cache_store(const char* path, const std::string& key, const hy_tuning& t) {
     FILE* f = std::fopen(path, "a");
     if (!f) return;
-    std::fprintf(f, "%s %d %d %d %d %d\n", key.c_str(), t.tile_rows, t.waves, t.batch_tile_rows,
-                 t.batch_wgs_per_cu, t.batch_chunk);
+    std::fprintf(f, "%s %d %d %d %d %d %d\n", key.c_str(), t.tile_rows, t.waves,
+                 t.batch_tile_rows, t.batch_wgs_per_cu, t.batch_chunk, t.batch_pairs);
     std::fclose(f);
 }
 
@@ -207,14 +207,21 @@ extern "C" int hy_scratch_autotune(const hy_db* db, hy_scratch* s, const uint8_t
         double best_ms = std::numeric_limits<double>::infinity(), best_wall = 0.0;
         uint32_t best_slots = 0;
         hy_tuning pick = best;
+        std::vector<int> pair_opts{best.batch_pairs};
+        if (db->mode == HY_MODE_SW) pair_opts = {1, 0};
+        for (int pairs : pair_opts)
         for (int q : kRows) {
-            const int occ = (int)hy::inter16_batch_occupancy((int)db->mode, q, (int)db->al_eff);
+            const int occ =
+                (int)(db->mode == HY_MODE_SW && pairs
+                          ? hy::inter16q_batch_occupancy(q, (int)db->al_eff)
+                          : hy::inter16_batch_occupancy((int)db->mode, q, (int)db->al_eff));
             std::vector<int> wpcs{occ};
             if (occ / 2 >= 1) wpcs.push_back(occ / 2);
             for (int wpc : wpcs) {
                 hy_tuning t = best;
                 t.batch_tile_rows = q;
                 t.batch_wgs_per_cu = wpc;
+                t.batch_pairs = pairs;
                 t.batch_chunk = (int32_t)smp.q.size();
                 if ((rc = hy_scratch_set_tuning(s, &t)) != HY_OK) return finish(rc);
                 double kms = 0.0, wall = 0.0;
@@ -252,6 +259,7 @@ extern "C" int hy_scratch_autotune(const hy_db* db, hy_scratch* s, const uint8_t
         }
         best.batch_tile_rows = pick.batch_tile_rows;
         best.batch_wgs_per_cu = pick.batch_wgs_per_cu;
+        best.batch_pairs = pick.batch_pairs;
         best.batch_chunk = (int32_t)chunk;
         if ((rc = hy_scratch_set_tuning(s, &best)) != HY_OK) return finish(rc);
     }

@@ -157,7 +157,8 @@ typedef struct hy_tuning {
     int32_t batch_tile_rows;  /* batched form: query rows per register tile (16, 24 or 32) */
     int32_t batch_wgs_per_cu; /* batched form: resident workgroups per CU (0 = occupancy limit) */
     int32_t batch_chunk;      /* batched form: queries per submission (1..65536) */
-    int32_t reserved;         /* must be 0 */
+    int32_t batch_pairs;      /* batched form, local mode: 1 = two queries per lane (paired
+                                 profile), 0 = two targets per lane; other modes ignore it */
 } hy_tuning;
 
 int hy_scratch_get_tuning(const hy_scratch* s, hy_tuning* out);
@@ -168,7 +169,7 @@ int hy_scratch_set_tuning(hy_scratch* s, const hy_tuning* t);
 /* Autotuner: times candidate shapes on the given sample queries (at most 8 are used, spread over
  * their length range) against the whole database and keeps the fastest in the scratch. forms:
  * bit 0 = one-query scans (tile rows x waves), bit 1 = batched form (tile rows x resident
- * workgroups; the submission size is the smallest whose predicted makespan, from the measured
+ * workgroups x, in local mode, query pairs or target pairs per lane; the submission size is the smallest whose predicted makespan, from the measured
  * rates, is within 3% of the ideal). With HYALITE_HIP_TUNE_CACHE naming a file, results are
  * looked up there first (key: device, mode, alphabet, database shape, sample lengths) and
  * appended after tuning. `out` (optional) receives the chosen shapes. */

@@ -405,11 +405,12 @@ def test_every_launch_shape_gives_the_same_hits(mode):
     for rows in (16, 24, 32):
         for wpc in (0, 1):
             for chunk in (3, 256):
-                sc.set_tuning(hy.Tuning(batch_tile_rows=rows, batch_wgs_per_cu=wpc,
-                                        batch_chunk=chunk))
-                got = db.scan_batch(sc, qs)
-                assert [(h.score, h.db_index, h.query_end, h.target_end) for h in got] == want, \
-                    (rows, wpc, chunk)
+                for pairs in (0, 1):
+                    sc.set_tuning(hy.Tuning(batch_tile_rows=rows, batch_wgs_per_cu=wpc,
+                                            batch_chunk=chunk, batch_pairs=pairs))
+                    got = db.scan_batch(sc, qs)
+                    assert [(h.score, h.db_index, h.query_end, h.target_end)
+                            for h in got] == want, (rows, wpc, chunk, pairs)
 
 
 def test_invalid_launch_shapes_are_rejected():
@@ -417,7 +418,7 @@ def test_invalid_launch_shapes_are_rejected():
     sc = hy.Scratch(db)
     for bad in (hy.Tuning(tile_rows=20), hy.Tuning(waves=2), hy.Tuning(batch_tile_rows=8),
                 hy.Tuning(batch_chunk=0), hy.Tuning(batch_chunk=65537),
-                hy.Tuning(batch_wgs_per_cu=-1)):
+                hy.Tuning(batch_wgs_per_cu=-1), hy.Tuning(batch_pairs=2)):
         with pytest.raises(ValueError):
             sc.set_tuning(bad)
     assert sc.tuning() == hy.Tuning()
