@@ -51,7 +51,7 @@ VALU_PEAK_TOPS_PK16 = 2 * VALU_PEAK_TOPS_I32
 # counter values are in profiles/.
 MEASURED_TRAFFIC_BYTES = {
     ("protein_sw_scoreend_batch256x1000000", "k_inter16_batch<SW,32>"): 1.355e13,
-    ("protein_sw_scoreend_batch256x1000000", "k_inter16q_batch<32>"): None,
+    ("protein_sw_scoreend_batch256x1000000", "k_inter16q_batch<32>"): 1.390e13,
 }
 
 SERVING_QPS = 10.0

@@ -285,14 +285,18 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     std::stable_sort(order_idx.begin(), order_idx.end(),
                      [&](uint32_t a, uint32_t b) { return lens[a] > lens[b]; });
     // A one-query scan lasts as long as its slowest wavefront: a packed group's time grows with
-    // its longest target, so targets longer than 1.6x the average column share of a resident
-    // wavefront (residues / 128 per group / resident workgroups) go to the systolic kernel on a
-    // second stream instead (measured best on the 1M-target protein database: 6.6k GCUPS at
-    // 2,400 columns vs 5.3k at 8x the mean length). HYALITE_HIP_LONG_THRESHOLD overrides it.
+    // its longest target, so long targets go to the systolic kernel on a second stream instead.
+    // Long = beyond 1.6x the average column share of a resident wavefront (residues / 128 per
+    // group / resident workgroups) or beyond 8x the mean length, whichever is shorter, and at
+    // least 1,024. Measured: 1M protein targets (share 1.5k columns) 6.6k GCUPS at 2.4k vs
+    // 5.3k at 3.2k; 2M targets (share 3.0k) 71 ms per query at 3.2k vs 85 ms at 4.9k.
+    // HYALITE_HIP_LONG_THRESHOLD overrides it.
     const uint32_t resident =
         hy::inter16_batch_occupancy((int)mode, db->q16, (int)db->al_eff) * hy::device_cus();
     const double share = (double)total / 128.0 / (double)std::max<uint32_t>(resident, 1);
-    uint64_t long_threshold = std::max<uint64_t>(1024, (uint64_t)(1.6 * share));
+    const double mean = (double)total / (double)n;
+    uint64_t long_threshold =
+        std::max<uint64_t>(1024, (uint64_t)std::min(1.6 * share, 8.0 * mean));
     if (const char* t = std::getenv("HYALITE_HIP_LONG_THRESHOLD")) {
         const long long v = std::atoll(t);
         if (v > 0) long_threshold = (uint64_t)v;
