@@ -995,6 +995,14 @@ extern "C" int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* cons
     std::vector<uint32_t> order;
     for (size_t base = 0; base < nq; base += per_sub) {
         const uint32_t cnt = (uint32_t)std::min<size_t>(per_sub, nq - base);
+        // A submission of one query has nothing to share a launch with: the one-query path (with
+        // its long-target split) is faster, and the query-pair kernel would idle half its lanes.
+        if (cnt == 1) {
+            if ((rc = hy_scan(db, s, queries[base], qlens[base], &scores[base], &db_indices[base],
+                              &query_ends[base], &target_ends[base])) != HY_OK)
+                return rc;
+            continue;
+        }
         uint32_t* qoff = s->h_bmeta;
         uint32_t* qlen = s->h_bmeta + cap;
         uint32_t* qord = s->h_bmeta + 2 * cap;

@@ -133,10 +133,12 @@ int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t 
                    int32_t* scores);
 
 /* Batched form: nq queries in one submission; hit k equals hy_scan of query k. The queries run
- * in one persistent kernel over a (target group, query) work queue, the winners' end positions
- * in one launch, and the whole submission is captured once as a HIP graph and replayed. Batches
- * larger than the scratch's reservation run as several submissions. No reference counterpart:
- * the reference loops Database::scan (src/database.rs:83). */
+ * in one persistent kernel over a (target group, query) work queue — in local mode two queries
+ * share each lane, a (target group, query pair) queue — the winners' end positions in one
+ * launch, and the whole submission is captured once as a HIP graph and replayed. Batches larger
+ * than the scratch's reservation (or than its batch_chunk) run as several submissions; a
+ * submission of a single query takes the one-query path. No reference counterpart: the
+ * reference loops Database::scan (src/database.rs:83). */
 int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
                   const size_t* qlens, size_t nq, int32_t* scores, uint64_t* db_indices,
                   int64_t* query_ends, int64_t* target_ends);
