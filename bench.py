@@ -8,13 +8,15 @@ symbols; seeded symmetric matrix, diagonal 4..11, off-diagonal -4..3; gap_open 1
 SW ScoreEnd; max_query_len 2000). A step is one Database::scan_batch of all 256 queries over the
 resident database (best hit with end positions per query).
 
-Beside it, on the same GPU in the same run:
+Beside it, on the same GPU in the same run (each part can be skipped with a flag):
   * plain: the same 256 queries scanned one Database::scan at a time (one query per launch);
   * single: the 360-aa query of the single-query protein workload on the same targets;
   * serving: Poisson arrivals at a fixed QPS of 64 queries (log-normal lengths, median 350,
     clipped [20, 3000]) against 2,000,000 targets (max_query_len 3000, so the longest targets
     need 32 bits), p50/p99 latency and sustained GCUPS, every served hit checked against the
     plain scan of that query alone;
+  * other_configs: the DNA configurations — 100-nt SW ScoreEnd scan of 10,000 targets and the
+    150-nt NW/HW/SHW/OV scan_scores sweep over 2,000,000 targets (5% mutated query copies);
   * cpu_baseline: the AVX2 16 x i16 restatement of the reference avx2 backend on a bounded
     sample of the batched workload, on this host's cores.
 
@@ -217,6 +219,54 @@ def serving(args, sync) -> dict:
     }
 
 
+def other_configs(sync) -> dict:
+    """The DNA configurations on the same GPU: the 100-nt SW ScoreEnd scan of 10,000 targets
+    (single best with ends) and the 150-nt mode sweep over 2,000,000 targets (scan_scores per
+    mode, the per-target score vector copied to the host)."""
+    out = {}
+    w = wl.c1_dna()
+    db = build_db(w)
+    sc = hy.Scratch(db)
+    q = w.queries[0]
+    for _ in range(3):
+        db.scan(sc, q)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(50):
+        hit = db.scan(sc, q)
+    sync()
+    dt = (time.perf_counter() - t0) / 50
+    out[w.name] = {"op": "scan (ScoreEnd)", "ms": round(dt * 1e3, 3),
+                   "gcups": round(len(q) * w.residues / dt / 1e9, 1),
+                   "partitions": {str(k): v for k, v in db.partition_sizes().items()},
+                   "hit": {"score": hit.score, "db_index": hit.db_index,
+                           "query_end": hit.query_end, "target_end": hit.target_end}}
+    del sc, db
+    base = wl.c3_dna_modes()
+    for mode, name in ((1, "NW"), (2, "HW"), (4, "SHW"), (3, "OV")):
+        w = wl.Workload(base.name.replace("mode1", name), base.al, base.matrix, base.go, base.ge,
+                        mode, 0, base.mql, base.flat, base.off, base.queries)
+        db = build_db(w)
+        sc = hy.Scratch(db)
+        q = w.queries[0]
+        for _ in range(2):
+            scores = db.scan_scores_array(sc, q)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            scores = db.scan_scores_array(sc, q)
+        sync()
+        dt = (time.perf_counter() - t0) / 5
+        out[w.name] = {"op": "scan_scores", "ms": round(dt * 1e3, 3),
+                       "gcups": round(len(q) * w.residues / dt / 1e9, 1),
+                       "partitions": {str(k): v for k, v in db.partition_sizes().items()},
+                       "best_score": int(scores.max()),
+                       "targets_at_best": int((scores == scores.max()).sum())}
+        log(f"{w.name}: {out[w.name]['gcups']} GCUPS")
+        del sc, db
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +277,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-query loop")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the DNA configurations (10k-target scan, 2M-target mode sweep)")
     ap.add_argument("--autotune", action="store_true",
                     help="pick launch shapes with the autotuner before timing")
     ap.add_argument("--qps", type=float, default=SERVING_QPS)
@@ -353,6 +405,7 @@ def main() -> int:
     del sc, db
 
     serve = None if args.no_serving else serving(args, sync)
+    others = None if args.no_configs else other_configs(sync)
 
     out = {
         "metric": METRIC,
@@ -377,6 +430,7 @@ def main() -> int:
         "plain": plain,
         "single_query": single,
         "serving": serve,
+        "other_configs": others,
         "roofline": roof,
     }
     if not args.no_cpu:

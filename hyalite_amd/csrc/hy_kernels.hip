@@ -205,6 +205,10 @@ __global__ __launch_bounds__(64) void k_inter32(InterArgs a) {
 // HG[i-1][j-1] + (sub + go) equals H[i-1][j-1] + sub whenever the sum is in range (a saturated
 // diagonal is below every real cell and loses the max). Padding cells are bounded by saturation
 // and never flow into real cells; in local mode they never exceed a real cell.
+// The tile code is written with lambdas over the tile's register arrays; every one must be
+// inlined, or the arrays they capture are moved to scratch memory.
+#define HY_INLINE __attribute__((always_inline))
+
 typedef short s2v __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ s2v as_s2(uint32_t u) { return __builtin_bit_cast(s2v, u); }
@@ -330,14 +334,14 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     // DP border instead. The host pads every group's column count to a multiple of four with the
     // inert padding symbol, so a block has no exits and the packed registers stay packed across
     // it; the prefetch after the last block re-reads that block's inputs (in bounds).
-    auto fetch = [&](int jb, uint2& wq, uint2* top) {
+    auto fetch = [&](int jb, uint2& wq, uint2* top) HY_INLINE {
         wq = wp[(size_t)(jb >> 2) * 64];
         if (!FIRST) {
 #pragma unroll
             for (int b = 0; b < 4; ++b) top[b] = sp[(size_t)(jb + b + 1) * 64];
         }
     };
-    auto block = [&](int jb, const uint2 w, const uint2* topin) {
+    auto block = [&](int jb, const uint2 w, const uint2* topin) HY_INLINE {
         // Top inputs of the block's columns: row 0 of the DP (local mode: (H, F' of row 1) =
         // (0, 0); otherwise (H - go, F) of row 0) or the previous tile's boundary row.
         uint2 spv[4];
@@ -352,7 +356,7 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
         uint2 outb[4];
         // Substitution pairs of column b of the block: (sub(q_r, low target), sub(q_r, high
         // target)) + go for the tile's rows.
-        auto load_sg = [&](int b, s2v* sg) {
+        auto load_sg = [&](int b, s2v* sg) HY_INLINE {
             const int ca = (int)((w.x >> (8 * b)) & 0xffu);
             const int cb = (int)((w.y >> (8 * b)) & 0xffu);
             if (PAIR) {
@@ -392,7 +396,7 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
         // column j+1 are independent — so two dependency chains interleave. The boundary row of a
         // tile is (H of its last row, F' of the row below).
         if (F::local) {
-            auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f) {
+            auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f) HY_INLINE {
                 const s2v h = p_max(p_max(p_add(hd, sgv), E[r]), f);
                 hd = HG[r];
                 HG[r] = h;
@@ -418,14 +422,19 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                 outb[bp + 1] = make_uint2(as_u32(HG[QT - 1]), as_u32(fB));
             }
         } else {
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
+            // One column with its answer cells: the last row in the last tile (every column up
+            // to the target's end, or only its end column), and the whole column at j == n.
+            auto column = [&](int b) HY_INLINE {
                 const int j = jb + b + 1;
                 s2v hgup = as_s2(spv[b].x), f = as_s2(spv[b].y);
                 s2v sg[QT];
                 load_sg(b, sg);
                 s2v hgd = HGd;
                 HGd = hgup;
+                // Row m of the query (last tile only) is taken as it is computed: selecting it
+                // from the row array afterwards would index the array at run time and move it
+                // to scratch memory.
+                s2v v = splat2(-32768);
 #pragma unroll
                 for (int r = 0; r < QT; ++r) {
                     const s2v diag = p_add(hgd, sg[r]);
@@ -436,14 +445,12 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                     E[r] = e;
                     HG[r] = p_sub(h, go2);
                     hgup = HG[r];
+                    if (lastc && r == rm) v = hgup;
                 }
                 outb[b] = make_uint2(as_u32(HG[QT - 1]), as_u32(f));
                 if (lastc) {
                     const bool la = F::lrow ? j <= nA : j == nA;
                     const bool lb = F::lrow ? j <= nB : j == nB;
-                    s2v v = HG[0];
-#pragma unroll
-                    for (int r = 1; r < QT; ++r) v = r == rm ? HG[r] : v;
                     run = p_max(run, keep2(p_add(v, go2), la, lb));
                 }
                 if (F::lcol && (j == nA || j == nB)) {
@@ -451,6 +458,44 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                     for (int r = 0; r < QT; ++r)
                         if (i0 + r <= m)
                             run = p_max(run, keep2(p_add(HG[r], go2), j == nA, j == nB));
+                }
+            };
+            // Columns without answer cells run in skewed pairs (row r of column j with row r-1
+            // of column j+1), as in local mode, so two dependency chains interleave; the last
+            // tile and the few column pairs holding a target's end column run one column at a
+            // time.
+            auto cell = [&](int r, s2v sgv, s2v& hgd, s2v& hgup, s2v& f) HY_INLINE {
+                const s2v diag = p_add(hgd, sgv);
+                const s2v e = p_max(HG[r], p_sub(E[r], ge2));
+                f = p_max(hgup, p_sub(f, ge2));
+                const s2v h = p_max(p_max(diag, e), f);
+                hgd = HG[r];
+                E[r] = e;
+                HG[r] = p_sub(h, go2);
+                hgup = HG[r];
+            };
+#pragma unroll
+            for (int bp = 0; bp < 4; bp += 2) {
+                const int jA = jb + bp + 1, jB = jA + 1;
+                const bool ends = F::lcol && (jA == nA || jA == nB || jB == nA || jB == nB);
+                if (lastc || __builtin_amdgcn_ballot_w64(ends) != 0) {
+                    column(bp);
+                    column(bp + 1);
+                } else {
+                    s2v sgA[QT], sgB[QT];
+                    load_sg(bp, sgA);
+                    load_sg(bp + 1, sgB);
+                    s2v hgupA = as_s2(spv[bp].x), fA = as_s2(spv[bp].y);
+                    s2v hgupB = as_s2(spv[bp + 1].x), fB = as_s2(spv[bp + 1].y);
+                    s2v hgdA = HGd, hgdB = hgupA;
+                    HGd = hgupB;
+#pragma unroll
+                    for (int r = 0; r <= QT; ++r) {
+                        if (r < QT) cell(r, sgA[r], hgdA, hgupA, fA);
+                        if (r == QT - 1) outb[bp] = make_uint2(as_u32(HG[QT - 1]), as_u32(fA));
+                        if (r >= 1) cell(r - 1, sgB[r - 1], hgdB, hgupB, fB);
+                    }
+                    outb[bp + 1] = make_uint2(as_u32(HG[QT - 1]), as_u32(fB));
                 }
             }
         }
@@ -638,20 +683,20 @@ __device__ __forceinline__ void inter16q_tile(const Inter16Args& a, const Group&
 
     // Prefetch of the next four-column block pinned ahead of the current block's compute, as in
     // inter16_tile; a lane's residues are the low (pass 0) or high (pass 1) word of the pair.
-    auto fetch = [&](int jb, uint2& wq, uint2* top) {
+    auto fetch = [&](int jb, uint2& wq, uint2* top) HY_INLINE {
         wq = wp[(size_t)(jb >> 2) * 64];
         if (!FIRST) {
 #pragma unroll
             for (int b = 0; b < 4; ++b) top[b] = sp[(size_t)(jb + b + 1) * 64];
         }
     };
-    auto block = [&](int jb, const uint2 wv, const uint2* topin) {
+    auto block = [&](int jb, const uint2 wv, const uint2* topin) HY_INLINE {
         const uint32_t w = pass ? wv.y : wv.x;
         uint2 spv[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b) spv[b] = FIRST ? make_uint2(0u, 0u) : topin[b];
         uint2 outb[4];
-        auto load_sg = [&](int b, s2v* sg) {
+        auto load_sg = [&](int b, s2v* sg) HY_INLINE {
             const int c = (int)((w >> (8 * b)) & 0xffu);
             const uint4* row = reinterpret_cast<const uint4*>(&pp[c * S]);
 #pragma unroll
@@ -661,7 +706,7 @@ __device__ __forceinline__ void inter16q_tile(const Inter16Args& a, const Group&
                 sg[4 * r4 + 2] = as_s2(v.z); sg[4 * r4 + 3] = as_s2(v.w);
             }
         };
-        auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f) {
+        auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f) HY_INLINE {
             const s2v h = p_max(p_max(p_add(hd, sgv), E[r]), f);
             hd = HG[r];
             HG[r] = h;
