@@ -79,6 +79,12 @@ def _load():
                              ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32,
                              ctypes.c_uint32, ctypes.c_uint32, _i32p, _u64p, _i64p, _i64p,
                              ctypes.c_int]
+    lib.hyo_traceback.restype = ctypes.c_int
+    lib.hyo_traceback.argtypes = [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64, ctypes.c_uint32,
+                                  _i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _i32p,
+                                  _u64p, _u64p, _u64p, _u64p, _u8p, _u64p]
+    lib.hyo_traceback_min_bytes_for_database.restype = ctypes.c_uint64
+    lib.hyo_traceback_min_bytes_for_database.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     if hasattr(lib, "hyo_avx2_scan_scores_timed"):
         lib.hyo_avx2_scan_scores_timed.restype = ctypes.c_int
         lib.hyo_avx2_scan_scores_timed.argtypes = [
@@ -136,6 +142,49 @@ def align_core(q, t, al: int, matrix, go: int, ge: int, mode: int):
     if rc != OK:
         raise MemoryError("oracle allocation failed")
     return s.value, _opt(qe.value), _opt(te.value)
+
+
+# Alignment operations (src/align.rs:47-57).
+OP_MATCH, OP_MISMATCH, OP_INS, OP_DEL = 0, 1, 2, 3
+
+
+def traceback(q, t, al: int, matrix, go: int, ge: int, mode: int):
+    """The canonical alignment of one pair (src/align.rs:169-420): (score, query_start, query_end,
+    target_start, target_end, ops) with half-open spans and ops in forward order."""
+    qa = _u8(q) if len(q) else np.zeros(1, dtype=np.uint8)
+    ta = _u8(t) if len(t) else np.zeros(1, dtype=np.uint8)
+    ma = _i32(matrix)
+    ops = np.zeros(len(q) + len(t) + 1, dtype=np.uint8)
+    s = ctypes.c_int32(0)
+    qs, qe, ts, te, k = (ctypes.c_uint64(0) for _ in range(5))
+    rc = _load().hyo_traceback(_ptr(qa, _u8p), len(q), _ptr(ta, _u8p), len(t), al,
+                               _ptr(ma, _i32p), go, ge, mode, ctypes.byref(s), ctypes.byref(qs),
+                               ctypes.byref(qe), ctypes.byref(ts), ctypes.byref(te),
+                               _ptr(ops, _u8p), ctypes.byref(k))
+    if rc != OK:
+        raise MemoryError("oracle allocation failed")
+    return s.value, qs.value, qe.value, ts.value, te.value, [int(x) for x in ops[:k.value]]
+
+
+def traceback_min_bytes_for_database(max_m: int, max_n: int) -> int:
+    return int(_load().hyo_traceback_min_bytes_for_database(max_m, max_n))
+
+
+def cigar(ops, extended: bool = False) -> str:
+    """Run-length CIGAR of the ops (src/align.rs:96-160): M/I/D, or =/X/I/D when extended."""
+    letters = ("=", "X", "I", "D") if extended else ("M", "M", "I", "D")
+    out, prev, run = [], None, 0
+    for op in ops:
+        c = letters[op]
+        if c == prev:
+            run += 1
+        else:
+            if prev is not None:
+                out.append(f"{run}{prev}")
+            prev, run = c, 1
+    if prev is not None:
+        out.append(f"{run}{prev}")
+    return "".join(out)
 
 
 def brute_score(mode: int, q, t, al: int, matrix, go: int, ge: int) -> int:

@@ -16,6 +16,8 @@
  *   answer reduction .......... src/kernel.rs:113-129 (Best::consider), 223-260 (reduce_answer)
  *   scan / scan_all / scan_scores (scalar backend) ... src/database.rs:115-144, 405-414,
  *                               453-462, 511-513
+ *   traceback ................. src/align.rs:169-420 (full-matrix DP, canonical walk),
+ *                               230-290 (budget arithmetic)
  *   brute-force enumerators ... tests/common/mod.rs:65-171 (independent structure: exhaustive
  *                               path / substring enumeration, used to pin the DP)
  *
@@ -401,4 +403,135 @@ int hyo_scan(const uint8_t* flat, const uint64_t* off, uint64_t n, const uint8_t
     }
     free(s); free(qe); free(te);
     return rc;
+}
+
+/* ---------------------------------------------------------------------------------------- */
+/* Traceback: src/align.rs:169-420 (traceback, full-matrix DP, canonical backward walk) and   */
+/* src/align.rs:230-290 (budget arithmetic). Full H/E/F matrices; the checkpoint path of the  */
+/* reference returns byte-identical alignments, so one path suffices for the checker.         */
+/* ---------------------------------------------------------------------------------------- */
+
+static uint64_t sat_mul(uint64_t a, uint64_t b) {
+    if (a != 0 && b > UINT64_MAX / a) return UINT64_MAX;
+    return a * b;
+}
+
+static uint64_t full_matrix_bytes(uint64_t m, uint64_t n) {
+    return sat_mul(sat_mul(sat_mul(m + 1, n + 1), 3), 4);
+}
+
+static uint64_t isqrt_u64(uint64_t v) {
+    uint64_t r = 0;
+    for (uint64_t bit = (uint64_t)1 << 62; bit; bit >>= 2) {
+        if (v >= r + bit) { v -= r + bit; r = (r >> 1) + bit; } else { r >>= 1; }
+    }
+    return r;
+}
+
+static uint64_t traceback_min_bytes(uint64_t m, uint64_t n) {
+    uint64_t full = full_matrix_bytes(m, n);
+    if (m == 0 || n == 0) return full;
+    uint64_t k = isqrt_u64(m);
+    if (k < 1) k = 1;
+    uint64_t ckpt = sat_mul(sat_mul(2 * (m / k + 1) + 3 * (k + 1), n + 1), 4);
+    return full < ckpt ? full : ckpt;
+}
+
+uint64_t hyo_traceback_min_bytes_for_database(uint64_t max_m, uint64_t max_n) {
+    uint64_t v = traceback_min_bytes(max_m, max_n);
+    uint64_t a = full_matrix_bytes(max_m, 0), b = full_matrix_bytes(0, max_n);
+    if (a > v) v = a;
+    if (b > v) v = b;
+    return v;
+}
+
+int hyo_traceback(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
+                  const int32_t* matrix, int32_t go, int32_t ge, uint32_t mode, int32_t* score,
+                  uint64_t* qstart, uint64_t* qend, uint64_t* tstart, uint64_t* tend,
+                  uint8_t* ops, uint64_t* nops) {
+    hyo_flags fl = flags_for_mode(mode);
+    const uint64_t cols = n + 1, cells = (m + 1) * cols;
+    int64_t* h = (int64_t*)malloc(sizeof(int64_t) * cells);
+    int64_t* e = (int64_t*)malloc(sizeof(int64_t) * cells);
+    int64_t* f = (int64_t*)malloc(sizeof(int64_t) * cells);
+    if (!h || !e || !f) { free(h); free(e); free(f); return HYO_E_NOMEM; }
+#define IX(i, j) ((uint64_t)(i) * cols + (uint64_t)(j))
+    for (uint64_t k = 0; k < cells; ++k) { h[k] = 0; e[k] = HYO_NEG; f[k] = HYO_NEG; }
+    /* Borders. E on row 0 and F on column 0 carry the penalised border gap so the walk can trace
+     * and close it; both stay at the sentinel where the border is free. */
+    for (uint64_t j = 1; j <= n; ++j) {
+        h[IX(0, j)] = fl.top_row_free ? 0 : -(int64_t)hyo_gap_penalty(go, ge, j);
+        e[IX(0, j)] = fl.top_row_free ? HYO_NEG : h[IX(0, j)];
+    }
+    for (uint64_t i = 1; i <= m; ++i) {
+        h[IX(i, 0)] = fl.left_col_free ? 0 : -(int64_t)hyo_gap_penalty(go, ge, i);
+        f[IX(i, 0)] = fl.left_col_free ? HYO_NEG : h[IX(i, 0)];
+        e[IX(i, 0)] = HYO_NEG;
+        const int32_t* qrow = matrix + (uint64_t)q[i - 1] * al;
+        for (uint64_t j = 1; j <= n; ++j) {
+            int64_t e1 = h[IX(i, j - 1)] - go, e2 = e[IX(i, j - 1)] - ge;
+            e[IX(i, j)] = e1 > e2 ? e1 : e2;
+            int64_t f1 = h[IX(i - 1, j)] - go, f2 = f[IX(i - 1, j)] - ge;
+            f[IX(i, j)] = f1 > f2 ? f1 : f2;
+            int64_t cell = h[IX(i - 1, j - 1)] + qrow[t[j - 1]];
+            if (e[IX(i, j)] > cell) cell = e[IX(i, j)];
+            if (f[IX(i, j)] > cell) cell = f[IX(i, j)];
+            if (fl.local && cell < 0) cell = 0;
+            h[IX(i, j)] = cell;
+        }
+    }
+    /* End cell: the answer region with the (score, col, row) tie-break, row by row. */
+    hyo_best b = {HYO_NEG, 0, 0};
+    for (uint64_t i = 0; i <= m; ++i) {
+        if (fl.local) {
+            for (uint64_t j = 0; j <= n; ++j) consider(&b, h[IX(i, j)], i, j);
+        } else {
+            if (fl.answer_last_col) consider(&b, h[IX(i, n)], i, n);
+            if (i == m) {
+                consider(&b, h[IX(m, n)], m, n);
+                if (fl.answer_last_row)
+                    for (uint64_t j = 0; j <= n; ++j) consider(&b, h[IX(m, j)], m, j);
+            }
+        }
+    }
+    /* Backward walk: diagonal before a gap, a target-consuming gap (Del) before a query-consuming
+     * one (Ins), closing a gap before extending it; local alignments stop at the first 0 cell. */
+    enum { ST_H, ST_E, ST_F } st = ST_H;
+    uint64_t i = b.row, j = b.col, k = 0;
+    for (;;) {
+        if (st == ST_H) {
+            int64_t hij = h[IX(i, j)];
+            if (fl.local && hij == 0) break;
+            if (i == 0 && j == 0) break;
+            if (i == 0) { if (fl.top_row_free) break; st = ST_E; continue; }
+            if (j == 0) { if (fl.left_col_free) break; st = ST_F; continue; }
+            int64_t sub = matrix[(uint64_t)q[i - 1] * al + t[j - 1]];
+            if (hij == h[IX(i - 1, j - 1)] + sub) {
+                ops[k++] = q[i - 1] == t[j - 1] ? HYO_OP_MATCH : HYO_OP_MISMATCH;
+                --i; --j;
+            } else if (hij == e[IX(i, j)]) {
+                st = ST_E;
+            } else {
+                st = ST_F;
+            }
+        } else if (st == ST_E) {
+            ops[k++] = HYO_OP_DEL;
+            int64_t ev = e[IX(i, j)];
+            --j;
+            if (ev == h[IX(i, j)] - go) st = ST_H;
+        } else {
+            ops[k++] = HYO_OP_INS;
+            int64_t fv = f[IX(i, j)];
+            --i;
+            if (fv == h[IX(i, j)] - go) st = ST_H;
+        }
+    }
+#undef IX
+    for (uint64_t a = 0, z = k; a + 1 < z; ++a, --z) {  /* forward order */
+        uint8_t tmp = ops[a]; ops[a] = ops[z - 1]; ops[z - 1] = tmp;
+    }
+    *score = (int32_t)b.score;
+    *qstart = i; *qend = b.row; *tstart = j; *tend = b.col; *nops = k;
+    free(h); free(e); free(f);
+    return HYO_OK;
 }

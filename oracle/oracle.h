@@ -22,6 +22,9 @@ enum {
     HYO_E_NOMEM = 100
 };
 
+/* Alignment operations (src/align.rs:47-57), in declaration order. */
+enum { HYO_OP_MATCH = 0, HYO_OP_MISMATCH = 1, HYO_OP_INS = 2, HYO_OP_DEL = 3 };
+
 int32_t hyo_gap_penalty(int32_t go, int32_t ge, uint64_t len);
 int hyo_scoring_check(uint32_t al, const int32_t* matrix, uint64_t matrix_len, int32_t go,
                       int32_t ge, int32_t* min_e, int32_t* max_e);
@@ -42,6 +45,15 @@ int hyo_scan(const uint8_t* flat, const uint64_t* off, uint64_t n, const uint8_t
              uint32_t al, const int32_t* mat, int32_t go, int32_t ge, uint32_t mode,
              uint32_t tracks_end, int32_t* score, uint64_t* db_index, int64_t* qend,
              int64_t* tend, int nthreads);
+
+/* src/align.rs:169-420: the canonical alignment (ops in forward order, at most m + n of them;
+ * spans half-open). */
+int hyo_traceback(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
+                  const int32_t* matrix, int32_t go, int32_t ge, uint32_t mode, int32_t* score,
+                  uint64_t* qstart, uint64_t* qend, uint64_t* tstart, uint64_t* tend,
+                  uint8_t* ops, uint64_t* nops);
+/* src/align.rs:230-275 (traceback_min_bytes_for_database). */
+uint64_t hyo_traceback_min_bytes_for_database(uint64_t max_m, uint64_t max_n);
 
 #ifdef __cplusplus
 }
