@@ -35,7 +35,7 @@ __all__ = [
     "IncompleteBuilder", "EmptyDatabase", "BackendUnavailable", "InvalidBackendName",
     "ScanContractViolation", "DeviceError", "HipLibraryMissing", "BACKEND_ENV_VAR", "build",
     "library_path", "load_library", "Tuning", "Server", "ServedHit", "ServerStats",
-    "ServerTimeout",
+    "ServerTimeout", "AlignOp", "Alignment", "AlignedHit", "TracebackBudgetExceeded",
 ]
 
 BACKEND_ENV_VAR = "HYALITE_BACKEND"
@@ -46,8 +46,10 @@ _LIB_NAME = "libhyalite_hip.so"
 # Translation units: the host library, the autotuner, the mode-independent kernels and dispatch,
 # and the mode-templated kernels compiled once per alignment mode (SW, NW, HW, OV, SHW).
 _UNITS = [("csrc/hy_host.cpp", []), ("csrc/hy_tune.cpp", []), ("csrc/hy_serve.cpp", []),
+          ("csrc/hy_align.cpp", []),
           ("csrc/hy_launch.hip", [])] + [
-    ("csrc/hy_kernels.hip", [f"-DHY_KMODE={k}"]) for k in range(5)]
+    (src, [f"-DHY_KMODE={k}"]) for src in ("csrc/hy_kernels.hip", "csrc/hy_trace.hip")
+    for k in range(5)]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -142,6 +144,16 @@ class EmptyDatabase(HyaliteError):
     pass
 
 
+class TracebackBudgetExceeded(HyaliteError):
+    """src/error.rs:90-95, 147-154."""
+
+    def __init__(self, needed_bytes: int, budget_bytes: int):
+        super().__init__(f"traceback needs {needed_bytes} bytes but the budget is {budget_bytes}; "
+                         "raise max_bytes (the linear-space path is not yet available)")
+        self.needed_bytes = needed_bytes
+        self.budget_bytes = budget_bytes
+
+
 class BackendUnavailable(HyaliteError):
     def __init__(self, backend: str = "hip"):
         super().__init__(f"backend {backend} is not available on this build/device")
@@ -189,12 +201,100 @@ class Mode(enum.IntEnum):
 
 
 class SearchType(enum.IntEnum):
-    """src/search.rs:13-27 (Alignment is outside the scan backend)."""
+    """src/search.rs:13-44. ``SearchType.Score``, ``SearchType.ScoreEnd``, or
+    ``SearchType.Alignment(max_bytes)`` (an :class:`AlignmentSearch`)."""
     Score = 0
     ScoreEnd = 1
 
     def tracks_end(self) -> bool:
         return self is SearchType.ScoreEnd
+
+    def max_bytes(self) -> Optional[int]:
+        return None
+
+
+@dataclass(frozen=True)
+class AlignmentSearch:
+    """SearchType::Alignment { max_bytes } (src/search.rs:23-26): ends are tracked and the
+    aligned scans are available; the build proves the traceback budget once."""
+    budget: int
+
+    def tracks_end(self) -> bool:
+        return True
+
+    def max_bytes(self) -> Optional[int]:
+        return self.budget
+
+    def __int__(self) -> int:
+        return 2
+
+
+SearchType.Alignment = staticmethod(lambda max_bytes: AlignmentSearch(int(max_bytes)))  # type: ignore
+
+
+class AlignOp(enum.IntEnum):
+    """src/align.rs:47-57. Ins consumes a query symbol, Del a target symbol."""
+    Match = 0
+    Mismatch = 1
+    Ins = 2
+    Del = 3
+
+
+@dataclass(frozen=True, eq=False)
+class Alignment:
+    """src/align.rs:58-160: score, half-open spans, ops in forward order (a uint8 array of
+    AlignOp values; ``ops`` gives them as AlignOp)."""
+    score: int
+    query_start: int
+    query_end: int
+    target_start: int
+    target_end: int
+    op_codes: np.ndarray
+
+    @property
+    def ops(self) -> list:
+        return [AlignOp(int(v)) for v in self.op_codes]
+
+    def _encode(self, extended: bool) -> str:
+        letters = "=XID" if extended else "MMID"
+        out, prev, run = [], None, 0
+        for v in self.op_codes.tolist():
+            c = letters[v]
+            if c == prev:
+                run += 1
+            else:
+                if prev is not None:
+                    out.append(f"{run}{prev}")
+                prev, run = c, 1
+        if prev is not None:
+            out.append(f"{run}{prev}")
+        return "".join(out)
+
+    def cigar(self) -> str:
+        return self._encode(False)
+
+    def cigar_extended(self) -> str:
+        return self._encode(True)
+
+    def __eq__(self, other) -> bool:
+        if not isinstance(other, Alignment):
+            return NotImplemented
+        return ((self.score, self.query_start, self.query_end, self.target_start,
+                 self.target_end) == (other.score, other.query_start, other.query_end,
+                                      other.target_start, other.target_end)
+                and np.array_equal(self.op_codes, other.op_codes))
+
+    def as_tuple(self):
+        """(score, query_start, query_end, target_start, target_end, [op values])."""
+        return (self.score, self.query_start, self.query_end, self.target_start,
+                self.target_end, [int(v) for v in self.op_codes])
+
+
+@dataclass(frozen=True)
+class AlignedHit:
+    """src/align.rs:83-90."""
+    db_index: int
+    alignment: Alignment
 
 
 class ScoreWidth(enum.IntEnum):
@@ -321,6 +421,18 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _sizep = ctypes.POINTER(ctypes.c_size_t)
 
 
+class _HyAlignment(ctypes.Structure):
+    _fields_ = [("score", ctypes.c_int32), ("reserved", ctypes.c_uint32),
+                ("query_start", ctypes.c_uint64), ("query_end", ctypes.c_uint64),
+                ("target_start", ctypes.c_uint64), ("target_end", ctypes.c_uint64),
+                ("n_ops", ctypes.c_uint64), ("ops_offset", ctypes.c_uint64)]
+
+
+_ALIGNMENT_DTYPE = np.dtype([("score", "<i4"), ("reserved", "<u4"), ("query_start", "<u8"),
+                             ("query_end", "<u8"), ("target_start", "<u8"),
+                             ("target_end", "<u8"), ("n_ops", "<u8"), ("ops_offset", "<u8")])
+
+
 class _HyServerConfig(ctypes.Structure):
     _fields_ = [("scratch_budget_bytes", ctypes.c_uint64), ("max_lanes", ctypes.c_uint32),
                 ("max_batch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
@@ -386,8 +498,8 @@ def load_library():
     lib.hy_db_build.restype = ctypes.c_int
     lib.hy_db_build.argtypes = [ctypes.POINTER(_u8p), _sizep, ctypes.c_size_t, ctypes.c_uint32,
                                 _i32p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int32,
-                                ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_size_t,
-                                ctypes.c_uint32, ctypes.POINTER(vp), _i64p]
+                                ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint64,
+                                ctypes.c_size_t, ctypes.c_uint32, ctypes.POINTER(vp), _i64p]
     lib.hy_db_retain.restype = ctypes.c_int
     lib.hy_db_retain.argtypes = [vp]
     lib.hy_db_free.restype = None
@@ -407,6 +519,14 @@ def load_library():
     lib.hy_scan_all.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _i32p, _i64p, _i64p]
     lib.hy_scan_scores.restype = ctypes.c_int
     lib.hy_scan_scores.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _i32p]
+    lib.hy_scan_aligned.restype = ctypes.c_int
+    lib.hy_scan_aligned.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _u64p,
+                                    ctypes.POINTER(_HyAlignment), _u8p, ctypes.c_size_t]
+    lib.hy_db_aligned_ops_bytes.restype = ctypes.c_int
+    lib.hy_db_aligned_ops_bytes.argtypes = [vp, ctypes.c_size_t, _u64p]
+    lib.hy_scan_all_aligned.restype = ctypes.c_int
+    lib.hy_scan_all_aligned.argtypes = [vp, vp, _u8p, ctypes.c_size_t, ctypes.c_void_p, _u8p,
+                                        ctypes.c_size_t]
     lib.hy_scan_batch.restype = ctypes.c_int
     lib.hy_scan_batch.argtypes = [vp, vp, ctypes.POINTER(_u8p), _sizep, ctypes.c_size_t, _i32p,
                                   _u64p, _i64p, _i64p]
@@ -447,7 +567,8 @@ def load_library():
 _FIELDS = ["sequences", "scoring", "mode", "max_query_len"]
 
 
-def _raise_build_error(rc: int, detail: int, alphabet_len: int, env_name: Optional[str]):
+def _raise_build_error(rc: int, detail: int, alphabet_len: int, env_name: Optional[str],
+                       budget: int = 0):
     lib = load_library()
     msg = lib.hy_strerror(rc).decode()
     if rc == 1:
@@ -470,6 +591,8 @@ def _raise_build_error(rc: int, detail: int, alphabet_len: int, env_name: Option
         raise BackendUnavailable()
     if rc == 10:
         raise InvalidBackendName(env_name or "")
+    if rc == 11:
+        raise TracebackBudgetExceeded(detail & ((1 << 64) - 1), budget)
     raise DeviceError(f"hy_db_build failed: {msg} (status {rc})")
 
 
@@ -477,7 +600,7 @@ def _check_scan(rc: int):
     if rc == 0:
         return
     msg = load_library().hy_strerror(rc).decode()
-    if rc in (100, 101):
+    if rc in (100, 101, 108):
         raise ScanContractViolation(msg)
     raise DeviceError(f"{msg} (status {rc})")
 
@@ -518,8 +641,8 @@ class DatabaseBuilder:
         self._mode = Mode(mode)
         return self
 
-    def search_type(self, st: SearchType) -> "DatabaseBuilder":
-        self._st = SearchType(st)
+    def search_type(self, st) -> "DatabaseBuilder":
+        self._st = st if isinstance(st, AlignmentSearch) else SearchType(st)
         return self
 
     def max_query_len(self, n: int) -> "DatabaseBuilder":
@@ -579,11 +702,12 @@ class DatabaseBuilder:
                              sc.gap_open() if sc is not None else 0,
                              sc.gap_ext() if sc is not None else 0,
                              int(self._mode) if self._mode is not None else 0, int(self._st),
+                             min(self._st.max_bytes() or 0, (1 << 64) - 1),
                              self._mql if self._mql is not None else 0, backend,
                              ctypes.byref(out), ctypes.byref(detail))
         if rc != 0:
             _raise_build_error(rc, detail.value, sc.alphabet_len() if sc else 0,
-                               os.environ.get(BACKEND_ENV_VAR))
+                               os.environ.get(BACKEND_ENV_VAR), self._st.max_bytes() or 0)
         return Database(out, sc, self._mode, self._st, seq_lens)
 
 
@@ -702,6 +826,49 @@ class Database:
             out.extend(s)
             return out
         return s
+
+    def scan_aligned(self, scratch: "Scratch", query) -> AlignedHit:
+        """src/database.rs:528-540: the winner of the score pass with its canonical alignment
+        (the alignment align::traceback gives for the pair), traced back on the device."""
+        q = np.asarray(query, dtype=np.uint8)
+        qa = _as_u8(q)
+        ops = np.empty(max(int(q.size) + int(self._mtl), 1), dtype=np.uint8)
+        idx = ctypes.c_uint64(0)
+        al = _HyAlignment()
+        _check_scan(load_library().hy_scan_aligned(
+            self._h, scratch._h, qa.ctypes.data_as(_u8p), q.size, ctypes.byref(idx),
+            ctypes.byref(al), ops.ctypes.data_as(_u8p), ops.size))
+        return AlignedHit(int(idx.value), Alignment(
+            int(al.score), int(al.query_start), int(al.query_end), int(al.target_start),
+            int(al.target_end), ops[:al.n_ops].copy()))
+
+    def scan_all_aligned_arrays(self, scratch: "Scratch", query):
+        """(records, ops): one hy_alignment record per target in database order (a structured
+        array with score, spans, n_ops, ops_offset) and the op bytes they index."""
+        q = np.asarray(query, dtype=np.uint8)
+        qa = _as_u8(q)
+        lib = load_library()
+        need = ctypes.c_uint64(0)
+        lib.hy_db_aligned_ops_bytes(self._h, q.size, ctypes.byref(need))
+        ops = np.empty(max(need.value, 1), dtype=np.uint8)
+        recs = np.zeros(self._n, dtype=_ALIGNMENT_DTYPE)
+        _check_scan(lib.hy_scan_all_aligned(self._h, scratch._h, qa.ctypes.data_as(_u8p), q.size,
+                                            recs.ctypes.data, ops.ctypes.data_as(_u8p),
+                                            ops.size))
+        return recs, ops
+
+    def scan_all_aligned(self, scratch: "Scratch", query, out: Optional[list] = None) -> list:
+        """src/database.rs:551-559: one Alignment per database sequence, in database order."""
+        recs, ops = self.scan_all_aligned_arrays(scratch, query)
+        res = [Alignment(int(r["score"]), int(r["query_start"]), int(r["query_end"]),
+                         int(r["target_start"]), int(r["target_end"]),
+                         ops[int(r["ops_offset"]):int(r["ops_offset"]) + int(r["n_ops"])])
+               for r in recs]
+        if out is not None:
+            out.clear()
+            out.extend(res)
+            return out
+        return res
 
     def scan_batch(self, scratch: "Scratch", queries) -> list:
         """Batched form: one hit per query, each equal to scan() of that query."""

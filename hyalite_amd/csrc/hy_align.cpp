@@ -1,0 +1,209 @@
+// Aligned scans on the device (Database::scan_aligned / scan_all_aligned, src/database.rs:528-580).
+// scan_aligned runs the score pass, whose fused argmax key names the winner on the device, then
+// the traceback kernels (hy_trace.hip) on that one pair; scan_all_aligned runs the traceback
+// kernels over every target in database order, in chunks whose move-code slabs fit the pool.
+// Both give the alignment align::traceback gives for the pair (src/align.rs:192-226).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstddef>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../../include/hyalite_hip.h"
+#include "hy_internal.h"
+#include "hy_state.h"
+
+static_assert(sizeof(hy::TraceOut) == sizeof(hy_alignment), "TraceOut mirrors hy_alignment");
+static_assert(offsetof(hy::TraceOut, n_ops) == offsetof(hy_alignment, n_ops),
+              "TraceOut mirrors hy_alignment");
+
+namespace {
+
+constexpr uint64_t kDefaultTracePool = 4ull << 30;  // bytes of move codes per chunk
+
+uint64_t trace_pool_bytes() {
+    if (const char* v = std::getenv("HYALITE_HIP_TRACE_POOL_BYTES")) {
+        char* end = nullptr;
+        const unsigned long long b = std::strtoull(v, &end, 10);
+        if (end != v) return (uint64_t)b;
+    }
+    return kDefaultTracePool;
+}
+
+// Grow a device buffer to at least `need` elements; the old contents are dropped. Every entry
+// point synchronises its stream before returning, so nothing is in flight on the buffer here.
+template <typename T>
+int grow(T** p, uint64_t* cap, uint64_t need) {
+    need = std::max<uint64_t>(need, 1);
+    if (*cap >= need) return HY_OK;
+    hy::dev_free(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc((void**)p, need * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return HY_E_OUT_OF_MEMORY;
+    }
+    *cap = need;
+    return HY_OK;
+}
+
+void account(hy_scratch* s) {
+    s->trace_bytes = s->codes_words * 4 + s->tends_cap * sizeof(int4) +
+                     s->tout_cap * sizeof(hy::TraceOut) + s->ops_cap +
+                     (s->d_slab_off ? (s->db->n + 1) * sizeof(uint64_t) : 0);
+}
+
+hy::TraceArgs trace_args(const hy_db* db, hy_scratch* s, int m) {
+    hy::TraceArgs a{};
+    a.flat = db->d_flat;
+    a.off = db->d_off;
+    a.query = s->d_query;
+    a.m = m;
+    a.matrix = db->d_matrix;
+    a.al = (int)db->al;
+    a.al_eff = (int)db->al_eff;
+    a.go = db->go;
+    a.ge = db->ge;
+    a.rows = hy::trace_rows(m);
+    a.codes = s->d_codes;
+    a.ends = s->d_tends;
+    a.ops = s->d_ops;
+    a.out = s->d_tout;
+    return a;
+}
+
+int aligned_preamble(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen) {
+    if (s->db != db) return HY_E_INVALID_ARGUMENT;
+    if (db->search_type != HY_SEARCH_ALIGNMENT) return HY_E_NOT_ALIGNMENT;
+    return hy::check_query(db, query, qlen);
+}
+
+}  // namespace
+
+extern "C" int hy_db_aligned_ops_bytes(const hy_db* db, size_t qlen, uint64_t* bytes) {
+    if (!db || !bytes) return HY_E_NULL_ARGUMENT;
+    *bytes = db->n * (uint64_t)qlen + db->total_residues;
+    return HY_OK;
+}
+
+extern "C" int hy_scan_aligned(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
+                               uint64_t* db_index, hy_alignment* out, uint8_t* ops,
+                               size_t ops_cap) {
+    if (!db || !s || !db_index || !out || !ops) return HY_E_NULL_ARGUMENT;
+    int rc = aligned_preamble(db, s, query, qlen);
+    if (rc != HY_OK) return rc;
+    if ((uint64_t)ops_cap < (uint64_t)qlen + db->mtl) return HY_E_INVALID_ARGUMENT;
+    const int m = (int)qlen;
+    if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
+    if ((rc = grow(&s->d_codes, &s->codes_words, hy::trace_slab_words(m, db->mtl))) != HY_OK ||
+        (rc = grow(&s->d_tends, &s->tends_cap, 1)) != HY_OK ||
+        (rc = grow(&s->d_tout, &s->tout_cap, 1)) != HY_OK ||
+        (rc = grow(&s->d_ops, &s->ops_cap, db->mql + db->mtl)) != HY_OK)
+        return rc;
+    account(s);
+    if (!s->h_tout && hipHostMalloc((void**)&s->h_tout, sizeof(hy::TraceOut)) != hipSuccess)
+        return HY_E_OUT_OF_MEMORY;
+    if (!s->h_key && hipHostMalloc((void**)&s->h_key, sizeof(unsigned long long)) != hipSuccess)
+        return HY_E_OUT_OF_MEMORY;
+    if (!s->h_ops &&
+        hipHostMalloc((void**)&s->h_ops, std::max<uint64_t>(db->mql + db->mtl, 1)) != hipSuccess)
+        return HY_E_OUT_OF_MEMORY;
+
+    if ((rc = hy::stage_query(db, s, query, qlen)) != HY_OK) return rc;
+    if (hy::launch_fill_u64(s->d_key, 1, 0ull, s->stream) != hipSuccess) return HY_E_DEVICE;
+    if ((rc = hy::enqueue_targets(db, s, m, false, false, s->d_key)) != HY_OK) return rc;
+    hy::TraceArgs a = trace_args(db, s, m);
+    a.winner_key = s->d_key;
+    a.nitems = 1;
+    if (hy::launch_trace((int)db->mode, a, s->stream) != hipSuccess) return HY_E_DEVICE;
+    if (hipMemcpyAsync(s->h_tout, s->d_tout, sizeof(hy::TraceOut), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipMemcpyAsync(s->h_key, s->d_key, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipMemcpyAsync(s->h_ops, s->d_ops, (size_t)(qlen + db->mtl), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    hy::collect_profile(s);
+    const hy::TraceOut o = *s->h_tout;
+    if (o.check != 0 || o.ops_offset + o.n_ops > (uint64_t)qlen + db->mtl) return HY_E_INTERNAL;
+    std::memcpy(ops, s->h_ops + o.ops_offset, (size_t)o.n_ops);
+    *db_index = 0xFFFFFFFFull - (*s->h_key & 0xFFFFFFFFull);
+    out->score = o.score;
+    out->reserved = 0;
+    out->query_start = o.query_start;
+    out->query_end = o.query_end;
+    out->target_start = o.target_start;
+    out->target_end = o.target_end;
+    out->n_ops = o.n_ops;
+    out->ops_offset = 0;
+    return HY_OK;
+}
+
+extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t* query,
+                                   size_t qlen, hy_alignment* out, uint8_t* ops, size_t ops_cap) {
+    if (!db || !s || !out || !ops) return HY_E_NULL_ARGUMENT;
+    int rc = aligned_preamble(db, s, query, qlen);
+    if (rc != HY_OK) return rc;
+    const uint64_t n = db->n;
+    const uint64_t ops_need = n * (uint64_t)qlen + db->total_residues;
+    if ((uint64_t)ops_cap < ops_need) return HY_E_INVALID_ARGUMENT;
+    const int m = (int)qlen;
+    if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
+
+    // Slab offsets per database index for this query length (R and the pass count follow m).
+    if (s->slab_m != m || !s->d_slab_off) {
+        s->h_slab_off.assign(n + 1, 0);
+        for (uint64_t k = 0; k < n; ++k)
+            s->h_slab_off[k + 1] =
+                s->h_slab_off[k] + hy::trace_slab_words(m, db->h_off[k + 1] - db->h_off[k]);
+        if (!s->d_slab_off &&
+            hipMalloc((void**)&s->d_slab_off, (n + 1) * sizeof(uint64_t)) != hipSuccess) {
+            s->d_slab_off = nullptr;
+            return HY_E_OUT_OF_MEMORY;
+        }
+        if (hipMemcpy(s->d_slab_off, s->h_slab_off.data(), (n + 1) * sizeof(uint64_t),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            s->slab_m = -1;
+            return HY_E_DEVICE;
+        }
+        s->slab_m = m;
+    }
+    const std::vector<uint64_t>& so = s->h_slab_off;
+    const uint64_t one = hy::trace_slab_words(m, db->mtl);
+    const uint64_t pool = std::min<uint64_t>(so[n], std::max<uint64_t>(one, trace_pool_bytes() / 4));
+    if ((rc = grow(&s->d_codes, &s->codes_words, std::max<uint64_t>(pool, 1))) != HY_OK ||
+        (rc = grow(&s->d_tends, &s->tends_cap, n)) != HY_OK ||
+        (rc = grow(&s->d_tout, &s->tout_cap, n)) != HY_OK ||
+        (rc = grow(&s->d_ops, &s->ops_cap, std::max<uint64_t>(ops_need, 1))) != HY_OK)
+        return rc;
+    account(s);
+
+    if ((rc = hy::stage_query(db, s, query, qlen)) != HY_OK) return rc;
+    hy::TraceArgs a = trace_args(db, s, m);
+    a.slab_off = s->d_slab_off;
+    // Chunks of consecutive targets whose slabs fit the pool (at least one target each); the
+    // stream orders each chunk's DP after the previous chunk's walk, which reads the same pool.
+    for (uint64_t k0 = 0; k0 < n;) {
+        uint64_t k1 = k0 + 1;
+        while (k1 < n && so[k1 + 1] - so[k0] <= s->codes_words) ++k1;
+        a.first = (uint32_t)k0;
+        a.nitems = (uint32_t)(k1 - k0);
+        a.slab_base = so[k0];
+        if (hy::launch_trace((int)db->mode, a, s->stream) != hipSuccess) return HY_E_DEVICE;
+        k0 = k1;
+    }
+    if (hipMemcpyAsync(out, s->d_tout, n * sizeof(hy::TraceOut), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipMemcpyAsync(ops, s->d_ops, (size_t)ops_need, hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess)
+        return HY_E_DEVICE;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    for (uint64_t k = 0; k < n; ++k) {
+        if (out[k].reserved != 0) return HY_E_INTERNAL;
+        if (out[k].ops_offset + out[k].n_ops > ops_need) return HY_E_INTERNAL;
+    }
+    return HY_OK;
+}

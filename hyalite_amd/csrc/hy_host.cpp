@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cctype>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -76,6 +77,37 @@ __int128 magnitude_bound(uint32_t mode, int32_t min_e, int32_t max_e, int32_t go
     return positive > negative ? positive : negative;
 }
 
+// The traceback budget of a database (src/align.rs:228-268): the smallest max_bytes with which
+// the reference's traceback serves every (m, n) in [0, max_m] x [0, max_n]. A full traceback
+// holds three i32 matrices of (m+1)(n+1) cells; the checkpoint path, with strip height
+// k = isqrt(m), holds H and F at every k-th row plus one (k+1)-row strip of H, E and F. The
+// cheaper of the two at (max_m, max_n), raised to the single-row/column full footprints of an
+// empty query or target, which take the full path unconditionally. Saturating u64 arithmetic.
+uint64_t sat_mul(uint64_t a, uint64_t b) {
+    return (a != 0 && b > UINT64_MAX / a) ? UINT64_MAX : a * b;
+}
+
+uint64_t full_traceback_bytes(uint64_t m, uint64_t n) {
+    return sat_mul(sat_mul(sat_mul(m + 1, n + 1), 3), 4);
+}
+
+uint64_t isqrt64(uint64_t v) {
+    uint64_t r = (uint64_t)std::sqrt((double)v);
+    while (r > 0 && r * r > v) --r;
+    while ((r + 1) * (r + 1) <= v) ++r;
+    return r;
+}
+
+uint64_t traceback_budget_for_database(uint64_t max_m, uint64_t max_n) {
+    uint64_t need = full_traceback_bytes(max_m, max_n);
+    if (max_m > 0 && max_n > 0) {
+        const uint64_t k = std::max<uint64_t>(isqrt64(max_m), 1);
+        const uint64_t ckpt = sat_mul(sat_mul(2 * (max_m / k + 1) + 3 * (k + 1), max_n + 1), 4);
+        need = std::min(need, ckpt);
+    }
+    return std::max({need, full_traceback_bytes(max_m, 0), full_traceback_bytes(0, max_n)});
+}
+
 std::string lower_trim(const char* s) {
     std::string v(s ? s : "");
     size_t a = 0, b = v.size();
@@ -137,8 +169,9 @@ static int first_gfx950_device() {
 extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_t n,
                            uint32_t have, const int32_t* matrix, size_t matrix_len,
                            uint32_t alphabet_len, int32_t gap_open, int32_t gap_ext, uint32_t mode,
-                           uint32_t search_type, size_t max_query_len, uint32_t backend,
-                           hy_db** out, int64_t* err_detail) {
+                           uint32_t search_type, uint64_t traceback_max_bytes,
+                           size_t max_query_len, uint32_t backend, hy_db** out,
+                           int64_t* err_detail) {
     if (!out) return HY_E_NULL_ARGUMENT;
     *out = nullptr;
     if (err_detail) *err_detail = 0;
@@ -169,7 +202,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
             return HY_E_INCOMPLETE_BUILDER;
         }
     }
-    if (mode > HY_MODE_SHW || search_type > HY_SEARCH_SCORE_END) return HY_E_INVALID_ARGUMENT;
+    if (mode > HY_MODE_SHW || search_type > HY_SEARCH_ALIGNMENT) return HY_E_INVALID_ARGUMENT;
     if (n == 0) return HY_E_EMPTY_DATABASE;
     if (!seqs || !lens) return HY_E_NULL_ARGUMENT;
     // SymbolOutOfRange: the first bad symbol in database order (src/database.rs:756-766).
@@ -194,6 +227,15 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     if (widest == 0) {
         if (err_detail) *err_detail = bound;
         return HY_E_SCORE_RANGE_TOO_WIDE;
+    }
+    // An Alignment search proves its traceback budget once for the whole problem box
+    // (database.rs:773-787), so the aligned scans never fail on it later.
+    if (search_type == HY_SEARCH_ALIGNMENT) {
+        const uint64_t needed = traceback_budget_for_database(max_query_len, mtl);
+        if (needed > traceback_max_bytes) {
+            if (err_detail) std::memcpy(err_detail, &needed, sizeof(needed));
+            return HY_E_TRACEBACK_BUDGET;
+        }
     }
     // Backend: explicit choice, else HYALITE_BACKEND, else Auto (src/database.rs:789-794).
     if (backend != HY_BACKEND_AUTO && backend != HY_BACKEND_HIP) return HY_E_INVALID_ARGUMENT;
@@ -220,6 +262,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     db->max_e = max_e;
     db->mode = mode;
     db->search_type = search_type;
+    db->traceback_max_bytes = search_type == HY_SEARCH_ALIGNMENT ? traceback_max_bytes : 0;
     db->n = n;
     db->mql = max_query_len;
     db->mtl = mtl;
@@ -276,6 +319,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
             if (lens[k]) std::memcpy(flat.data() + off[k], seqs[k], lens[k]);
         if (dev_upload(&db->d_off, off) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_flat, flat) != hipSuccess) return fail(HY_E_DEVICE);
+        db->h_off = std::move(off);
     }
 
     // Split: long targets go to the systolic kernel, the rest to the inter-sequence kernel. A
@@ -539,7 +583,7 @@ extern "C" void hy_scratch_free(hy_scratch* s) { delete s; }
 // Scans
 // ---------------------------------------------------------------------------------------------
 
-namespace {
+namespace hy {
 
 // Contract checks the reference turns into panics (src/database.rs:79-89): an over-long query or
 // a symbol outside the alphabet.
@@ -675,7 +719,14 @@ void collect_profile(hy_scratch* s) {
     s->pending = false;
 }
 
-}  // namespace
+}  // namespace hy
+
+using hy::check_query;
+using hy::collect_profile;
+using hy::inter_args;
+using hy::sys_args;
+using hy::enqueue_targets;
+using hy::stage_query;
 
 extern "C" int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
                        int32_t* score, uint64_t* db_index, int64_t* query_end,
@@ -688,7 +739,7 @@ extern "C" int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, siz
     const int m = (int)qlen;
     if (hy::launch_fill_u64(s->d_key, 1, 0ull, s->stream) != hipSuccess) return HY_E_DEVICE;
     if ((rc = enqueue_targets(db, s, m, false, false, s->d_key)) != HY_OK) return rc;
-    const bool ends = db->search_type == HY_SEARCH_SCORE_END;
+    const bool ends = db->search_type != HY_SEARCH_SCORE;
     if (ends) {
         // The winner's end positions: one systolic pass over the winning target only (the role
         // of align_core on the winner in src/database.rs:183-197), chosen on the device.
@@ -722,7 +773,7 @@ extern "C" int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query,
     int rc = check_query(db, query, qlen);
     if (rc != HY_OK) return rc;
     if ((rc = stage_query(db, s, query, qlen)) != HY_OK) return rc;
-    const bool ends = db->search_type == HY_SEARCH_SCORE_END;
+    const bool ends = db->search_type != HY_SEARCH_SCORE;
     if ((rc = enqueue_targets(db, s, (int)qlen, ends, true, nullptr)) != HY_OK) return rc;
     const size_t n = (size_t)db->n;
     if (hipMemcpyAsync(scores, s->d_scores, n * sizeof(int32_t), hipMemcpyDeviceToHost,
@@ -941,7 +992,7 @@ int enqueue_batch(const hy_db* db, hy_scratch* s, bool timed) {
     }
     if (tail && hipStreamWaitEvent(st, s->join, 0) != hipSuccess) return HY_E_DEVICE;
 
-    if (db->search_type == HY_SEARCH_SCORE_END) {
+    if (db->search_type != HY_SEARCH_SCORE) {
         hy::SysArgs sa{};
         sa.flat = db->d_flat;
         sa.off = db->d_off;
@@ -1162,6 +1213,8 @@ extern "C" const char* hy_strerror(int code) {
     case HY_E_OUT_OF_MEMORY: return "out of memory";
     case HY_E_INTERNAL: return "device self-check failed";
     case HY_E_TIMEOUT: return "timed out waiting for a served query";
+    case HY_E_NOT_ALIGNMENT:
+        return "scan_aligned/scan_all_aligned require a database built with SearchType::Alignment";
     default: return "unknown status code";
     }
 }

@@ -127,6 +127,51 @@ struct SysArgs {
     int qsel;                 // >= 0: every block uses query qsel instead of blockIdx.y
 };
 
+// Traceback (src/align.rs:169-420), one (query, target) pair per item. The DP kernel writes a
+// 4-bit move code per cell into the item's slab and the pair's end cell into ends[item]; the walk
+// kernel reads them back, one lane per item, and writes the alignment's ops and a TraceOut.
+//
+// Slab layout (32-bit words): the query is cut into passes of 64*R rows (R rows per lane); pass
+// p, step s, lane l holds the R nibbles of rows p*64R + l*R + 1 .. + R at column j = s - l + 1,
+// at word (p * (n + 63) + s) * 64 + l. With more than one pass the boundary rows between passes
+// (int2 per column, n + 1 of them) follow the codes.
+struct TraceOut {          // same layout as hy_alignment
+    int32_t score;
+    int32_t check;         // 0 when the walk ended where the DP says it must
+    uint64_t query_start, query_end, target_start, target_end;
+    uint64_t n_ops, ops_offset;
+};
+
+struct TraceArgs {
+    const uint8_t* flat;
+    const uint64_t* off;
+    const unsigned long long* winner_key;  // non-null: one item, the key's target
+    uint32_t first;           // list mode: item k is database index first + k
+    uint32_t nitems;
+    const uint8_t* query;
+    int m;
+    const int32_t* matrix;
+    int al, al_eff;
+    int go, ge;
+    int rows;                 // R (rows per lane) of the DP launch
+    uint32_t* codes;          // slab pool
+    const uint64_t* slab_off; // per database index (words); null: every item at offset 0
+    uint64_t slab_base;       // slab_off of the first item of the launch
+    int4* ends;               // per item: (score, end row, end column, database index)
+    uint8_t* ops;             // winner: ops at [0, m + n); list: target g at g * m + off[g]
+    TraceOut* out;            // winner: out[0]; list: out[database index]
+};
+
+// Rows per lane of the traceback DP for a query of m rows (2, 4, 6 or 8; 8 beyond 384 rows).
+inline int trace_rows(int m) { return m <= 128 ? 2 : (m <= 256 ? 4 : (m <= 384 ? 6 : 8)); }
+// Slab words of one pair, rounded to whole 64-word (256-byte) lines.
+inline uint64_t trace_slab_words(int m, uint64_t n) {
+    const uint64_t P = 64ull * (uint64_t)trace_rows(m);
+    const uint64_t npass = ((uint64_t)m + P - 1) / P;
+    uint64_t w = npass * (n + 63) * 64 + (npass > 1 ? 2 * (n + 1) : 0);
+    return (w + 63) & ~63ull;
+}
+
 // Launchers (hy_launch.hip). All launch on `stream` and never synchronise.
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
                            hipStream_t stream);
@@ -134,6 +179,8 @@ hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngro
                           hipStream_t stream);
 // nq_grid > 1: batched form, a grid of nblocks x nq_grid blocks; a.m is then the longest query
 // of the batch (it fixes the rows per lane).
+// Traceback: the DP over a.nitems items, then the walk (one lane per item).
+hipError_t launch_trace(int mode, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
                         hipStream_t stream, uint32_t nq_grid = 1);
 // q: query rows per register tile (16, 24 or 32); waves: 1 = one-wavefront workgroups with a
@@ -179,5 +226,7 @@ hipError_t launch_inter16_batch_m(const Inter16Args& a, const BatchArgs& b, int 
                                   hipStream_t st);
 template <int MODE>
 uint32_t inter16_batch_occupancy_m(int q, int al_eff);
+template <int MODE>
+hipError_t launch_trace_m(const TraceArgs& a, hipStream_t st);
 
 }  // namespace hy

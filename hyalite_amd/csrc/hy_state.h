@@ -23,6 +23,7 @@ struct hy_db {
     uint32_t al = 0, al_eff = 0;
     int32_t go = 0, ge = 0, min_e = 0, max_e = 0;
     uint32_t mode = 0, search_type = 0;
+    uint64_t traceback_max_bytes = 0;  // SearchType::Alignment's budget (0 otherwise)
     uint64_t n = 0, mql = 0, mtl = 0, total_residues = 0;
     int widest_bits = 8;
     uint64_t count_by_width[3] = {0, 0, 0};
@@ -30,6 +31,7 @@ struct hy_db {
     int32_t* d_matrix = nullptr;   // al x al, entries clamped to +-kEntryMax32
     uint8_t* d_flat = nullptr;     // residues in database order
     uint64_t* d_off = nullptr;     // n+1 offsets
+    std::vector<uint64_t> h_off;   // host copy of the offsets (traceback slab and op layout)
     // Inter-sequence partition (one target per lane).
     std::vector<hy::Group> groups;
     hy::Group* d_groups = nullptr;
@@ -125,6 +127,23 @@ struct hy_scratch {
     int bpairs = 1;                      // batched form, local mode: two queries per lane
     uint64_t dev_bytes = 0;              // device memory held by the one-query buffers
     uint64_t batch_bytes = 0;            // device memory held by the batched-form reservation
+    // Traceback (grown on first use): move-code slab pool, per-item ends, per-target outputs and
+    // op bytes, slab offsets per database index for one rows-per-lane / pass count.
+    uint32_t* d_codes = nullptr;
+    uint64_t codes_words = 0;
+    int4* d_tends = nullptr;
+    uint64_t tends_cap = 0;
+    hy::TraceOut* d_tout = nullptr;
+    uint64_t tout_cap = 0;
+    uint8_t* d_ops = nullptr;
+    uint64_t ops_cap = 0;
+    uint64_t* d_slab_off = nullptr;
+    int slab_m = -1;                     // query length the slab offsets were laid out for
+    std::vector<uint64_t> h_slab_off;
+    hy::TraceOut* h_tout = nullptr;      // pinned, one entry (winner)
+    uint8_t* h_ops = nullptr;            // pinned, max_query_len + max_target_len (winner)
+    unsigned long long* h_key = nullptr; // pinned (winner)
+    uint64_t trace_bytes = 0;            // device memory held by the traceback buffers
 
     ~hy_scratch() {
         (void)hipSetDevice(device);
@@ -134,6 +153,11 @@ struct hy_scratch {
         hy::dev_free(d_key); hy::dev_free(d_result);
         hy::dev_free(d_bq); hy::dev_free(d_bmeta); hy::dev_free(d_bkeys); hy::dev_free(d_bres);
         hy::dev_free(d_slots); hy::dev_free(d_bwin_spill); hy::dev_free(d_btail_spill);
+        hy::dev_free(d_codes); hy::dev_free(d_tends); hy::dev_free(d_tout); hy::dev_free(d_ops);
+        hy::dev_free(d_slab_off);
+        if (h_tout) (void)hipHostFree(h_tout);
+        if (h_ops) (void)hipHostFree(h_ops);
+        if (h_key) (void)hipHostFree(h_key);
         if (h_bq) (void)hipHostFree(h_bq);
         if (h_bmeta) (void)hipHostFree(h_bmeta);
         if (h_bres) (void)hipHostFree(h_bres);
@@ -153,3 +177,14 @@ struct hy_scratch {
     }
 };
 
+namespace hy {
+// Scan plumbing shared by the host translation units (defined in hy_host.cpp).
+// check_query: the scan contract (database.rs:79-89). stage_query: copy the query to the device
+// on the scratch's stream. enqueue_targets: the per-target pass (see hy_host.cpp).
+// collect_profile: fold the last scan's kernel times into the scratch's counters.
+int check_query(const hy_db* db, const uint8_t* q, size_t qlen);
+int stage_query(const hy_db* db, hy_scratch* s, const uint8_t* q, size_t qlen);
+int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write_scores,
+                    unsigned long long* key);
+void collect_profile(hy_scratch* s);
+}  // namespace hy

@@ -14,6 +14,9 @@
  *                                                          src/inter.rs:1226-1262
  *   hy_scan_scores   <- Database::scan_scores / fill_scores src/database.rs:305-329, 481-515,
  *                                                          src/inter.rs:1161-1205
+ *   hy_scan_aligned  <- Database::scan_aligned            src/database.rs:528-540,
+ *                                                          src/align.rs:192-226, 487-545
+ *   hy_scan_all_aligned <- Database::scan_all_aligned     src/database.rs:551-580
  *   hy_scan_batch    <- a loop of Database::scan over many queries (batched form; no single
  *                       reference counterpart — each hit equals Database::scan of that query)
  *   hy_backend_available <- Backend::is_available         src/backend.rs:77-84
@@ -45,8 +48,12 @@ typedef struct hy_scratch hy_scratch;
 
 /* Mode (src/mode.rs:11-27). */
 enum { HY_MODE_SW = 0, HY_MODE_NW = 1, HY_MODE_HW = 2, HY_MODE_OV = 3, HY_MODE_SHW = 4 };
-/* SearchType (src/search.rs:13-27); Alignment is outside this backend. */
-enum { HY_SEARCH_SCORE = 0, HY_SEARCH_SCORE_END = 1 };
+/* SearchType (src/search.rs:13-27). ALIGNMENT carries its traceback budget in hy_db_build's
+ * traceback_max_bytes and tracks ends like SCORE_END (search.rs:33-35). */
+enum { HY_SEARCH_SCORE = 0, HY_SEARCH_SCORE_END = 1, HY_SEARCH_ALIGNMENT = 2 };
+/* AlignOp (src/align.rs:47-57): the values of an alignment's op bytes. Ins consumes a query
+ * symbol, Del a target symbol (the target is the reference, as in SAM). */
+enum { HY_OP_MATCH = 0, HY_OP_MISMATCH = 1, HY_OP_INS = 2, HY_OP_DEL = 3 };
 /* Backend choice: AUTO or FORCE(HIP). The CPU backends stay in the reference crate. */
 enum { HY_BACKEND_AUTO = 0, HY_BACKEND_HIP = 9 };
 
@@ -63,7 +70,7 @@ enum {
     HY_E_EMPTY_DATABASE = 8,
     HY_E_BACKEND_UNAVAILABLE = 9,
     HY_E_INVALID_BACKEND_NAME = 10,
-    HY_E_TRACEBACK_BUDGET = 11,      /* never produced: Alignment search is not offered here */
+    HY_E_TRACEBACK_BUDGET = 11,      /* err_detail = bytes needed (as uint64_t bits) */
     /* Scan-time contract violations and runtime failures. */
     HY_E_QUERY_TOO_LONG = 100,       /* err: query longer than max_query_len */
     HY_E_QUERY_SYMBOL = 101,         /* query symbol >= alphabet_len */
@@ -72,7 +79,8 @@ enum {
     HY_E_DEVICE = 104,               /* a HIP runtime call failed */
     HY_E_OUT_OF_MEMORY = 105,
     HY_E_INTERNAL = 106,             /* a device-side self-check failed */
-    HY_E_TIMEOUT = 107               /* hy_server_wait: not completed within the timeout */
+    HY_E_TIMEOUT = 107,              /* hy_server_wait: not completed within the timeout */
+    HY_E_NOT_ALIGNMENT = 108         /* an aligned scan on a database built without ALIGNMENT */
 };
 
 /* Field flags for hy_db_build's `have` mask: which builder fields were set. A field left out
@@ -96,11 +104,14 @@ int hy_backend_parse(const char* name, int32_t* out);
 /* Build a device-resident database. Validation order, error codes and the per-sequence width
  * proof follow DatabaseBuilder::build (database.rs:737-877). Sequences are copied; the caller's
  * buffers may be freed afterwards. `backend` is HY_BACKEND_AUTO or HY_BACKEND_HIP; AUTO also
- * consults HYALITE_BACKEND. err_detail (may be NULL) receives the payload documented above. */
+ * consults HYALITE_BACKEND. traceback_max_bytes is SearchType::Alignment's max_bytes (ignored for
+ * the other search types): the build fails with HY_E_TRACEBACK_BUDGET when the reference's
+ * traceback could not serve the whole [0, max_query_len] x [0, max_target_len] box within it
+ * (database.rs:773-787). err_detail (may be NULL) receives the payload documented above. */
 int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_t n, uint32_t have,
                 const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len, int32_t gap_open,
-                int32_t gap_ext, uint32_t mode, uint32_t search_type, size_t max_query_len,
-                uint32_t backend, hy_db** out, int64_t* err_detail);
+                int32_t gap_ext, uint32_t mode, uint32_t search_type, uint64_t traceback_max_bytes,
+                size_t max_query_len, uint32_t backend, hy_db** out, int64_t* err_detail);
 
 /* Reference-count a database (a Rust `Clone`); every retain needs one hy_db_free. */
 int hy_db_retain(hy_db* db);
@@ -131,6 +142,35 @@ int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qle
 /* Per-target scores in database order (Database::scan_scores; SearchType-independent). */
 int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
                    int32_t* scores);
+
+/* Alignment (src/align.rs:60-95): score, half-open spans [start, end) of query and target, and
+ * n_ops op bytes (HY_OP_*, forward order) at ops + ops_offset of the buffer given to the scan. */
+typedef struct hy_alignment {
+    int32_t score;
+    uint32_t reserved;
+    uint64_t query_start, query_end, target_start, target_end;
+    uint64_t n_ops, ops_offset;
+} hy_alignment;
+
+/* The best target with its canonical alignment (Database::scan_aligned): the winner of the
+ * score pass (the hit hy_scan reports), traced back on the device with the reference's walk
+ * preferences, so the alignment equals align::traceback of the query against that target.
+ * ops needs ops_cap >= qlen + max_target_len bytes; the ops start at ops[out->ops_offset]. The
+ * database must be built with HY_SEARCH_ALIGNMENT (else HY_E_NOT_ALIGNMENT). */
+int hy_scan_aligned(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
+                    uint64_t* db_index, hy_alignment* out, uint8_t* ops, size_t ops_cap);
+
+/* Bytes of the ops buffer hy_scan_all_aligned needs for a query of qlen symbols:
+ * n * qlen + the database's total residues (target k's ops lie in [k * qlen + offset_k,
+ * (k + 1) * qlen + offset_k + len_k), offset_k = residues of the targets before k). */
+int hy_db_aligned_ops_bytes(const hy_db* db, size_t qlen, uint64_t* bytes);
+
+/* One alignment per target in database order (Database::scan_all_aligned): out has n entries,
+ * out[k].ops_offset locates target k's ops in `ops` (ops_cap >= hy_db_aligned_ops_bytes). The
+ * pairs run in chunks whose move-code slabs fit the scratch's traceback pool (4 GiB, or
+ * HYALITE_HIP_TRACE_POOL_BYTES when set). */
+int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
+                        hy_alignment* out, uint8_t* ops, size_t ops_cap);
 
 /* Batched form: nq queries in one submission; hit k equals hy_scan of query k. The queries run
  * in one persistent kernel over a (target group, query) work queue — in local mode two queries

@@ -76,6 +76,56 @@ def test_builder_empty_symbol_width_errors():
     assert e.value.bound == 2_000_000_001
 
 
+def test_alignment_budget_is_validated_at_build():
+    """tests/database_alignment.rs:84-104: a one-byte budget cannot cover an 8 x 8 traceback.
+    The budget check runs before the backend is resolved (database.rs:773-794), so it is decided
+    here without a device. The needed bytes are the database budget of the (8, 8) box: the
+    checkpoint footprint (2 * 5 + 3 * 3) * 9 * 4 = 684."""
+    seqs = [[0, 1, 2, 3, 0, 1, 2, 3]]
+    with pytest.raises(hy.TracebackBudgetExceeded) as e:
+        hy.Database.builder().sequences(seqs).scoring(scoring()).mode(hy.Mode.Nw) \
+            .search_type(hy.SearchType.Alignment(1)).max_query_len(8).build()
+    assert (e.value.needed_bytes, e.value.budget_bytes) == (684, 1)
+    assert "traceback needs 684 bytes but the budget is 1" in str(e.value)
+    # One byte short of the exact budget fails; the exact budget passes validation (and then
+    # stops at the backend, which this container lacks, or builds on a device).
+    with pytest.raises(hy.TracebackBudgetExceeded):
+        hy.Database.builder().sequences(seqs).scoring(scoring()).mode(hy.Mode.Nw) \
+            .search_type(hy.SearchType.Alignment(683)).max_query_len(8).build()
+    try:
+        hy.Database.builder().sequences(seqs).scoring(scoring()).mode(hy.Mode.Nw) \
+            .search_type(hy.SearchType.Alignment(684)).max_query_len(8).build()
+    except hy.BackendUnavailable:
+        pass
+    # Width and symbol errors come first (database.rs:756-771).
+    with pytest.raises(hy.SymbolOutOfRange):
+        hy.Database.builder().sequences([[9]]).scoring(scoring()).mode(hy.Mode.Nw) \
+            .search_type(hy.SearchType.Alignment(1)).max_query_len(8).build()
+    st = hy.SearchType.Alignment(4096)
+    assert st.tracks_end() and st.max_bytes() == 4096
+    assert hy.SearchType.ScoreEnd.max_bytes() is None
+
+
+def test_alignment_budget_matches_the_oracle_over_a_box_sweep():
+    """The library's budget arithmetic against the pinned oracle, through the build check: for
+    each box the oracle's exact budget passes validation and one byte less fails."""
+    import oracle as orc
+    for m, n in [(0, 0), (0, 5), (5, 0), (1, 30), (30, 1), (8, 8), (40, 55), (400, 400),
+                 (1000, 35000)]:
+        need = orc.traceback_min_bytes_for_database(m, n)
+        seqs = [[0] * n] if n else [[]]
+        b = lambda budget: (hy.Database.builder().sequences(seqs).scoring(scoring())  # noqa
+                            .mode(hy.Mode.Sw).search_type(hy.SearchType.Alignment(budget))
+                            .max_query_len(m))
+        with pytest.raises(hy.TracebackBudgetExceeded) as e:
+            b(need - 1).build()
+        assert e.value.needed_bytes == need, (m, n)
+        try:
+            b(need).build()
+        except hy.BackendUnavailable:
+            pass
+
+
 def test_backend_parse_and_env_var():
     assert hy.BackendChoice.parse("AUTO") == hy.BackendChoice.AUTO
     assert hy.BackendChoice.parse("  Hip ") == hy.BackendChoice.force(hy.Backend.Hip)
@@ -119,5 +169,5 @@ def test_env_var_is_read_in_its_own_process():
 
 def test_strerror_covers_every_code():
     lib = hy.load_library()
-    for code in list(range(0, 12)) + list(range(100, 107)):
+    for code in list(range(0, 12)) + list(range(100, 109)):
         assert lib.hy_strerror(code) not in (None, b"unknown status code")
