@@ -17,8 +17,12 @@ Beside it, on the same GPU in the same run (each part can be skipped with a flag
     plain scan of that query alone;
   * other_configs: the DNA configurations — 100-nt SW ScoreEnd scan of 10,000 targets and the
     150-nt NW/HW/SHW/OV scan_scores sweep over 2,000,000 targets (5% mutated query copies);
+  * traceback: the same 1,000,000 targets built with SearchType::Alignment — scan_aligned of
+    the 360-aa query (winner plus its canonical alignment) against scan alone, and
+    scan_all_aligned (one alignment per target, ops copied to the host);
   * cpu_baseline: the AVX2 16 x i16 restatement of the reference avx2 backend on a bounded
-    sample of the batched workload, on this host's cores.
+    sample of the batched workload, on this host's cores; beside it the scalar traceback
+    restatement on a sample of the traceback pairs, each checked against the device's result.
 
 GCUPS = sum over queries of |q| * sum|t| / seconds / 1e9, real cells only. The roofline is the
 VALU throughput of the persistent batch kernel, timed with HIP events on its stream.
@@ -267,6 +271,88 @@ def other_configs(sync) -> dict:
     return out
 
 
+def traceback(w: wl.Workload, sync) -> tuple[dict, dict]:
+    """Aligned scans on the batched workload's targets: an Alignment-search database of the same
+    sequences, the single-query workload's 360-aa query. Returns the report and a sample of the
+    device's per-target alignments for the CPU leg to check and time against."""
+    q = w.extra["single_query"]
+    t0 = time.perf_counter()
+    db = (hy.Database.builder().sequences_flat(w.flat, w.off)
+          .scoring(hy.Scoring(w.al, w.matrix, w.go, w.ge)).mode(hy.Mode(w.mode))
+          .search_type(hy.SearchType.Alignment(1 << 62)).max_query_len(len(q))
+          .backend(hy.BackendChoice.force(hy.Backend.Hip)).build())
+    build_s = time.perf_counter() - t0
+    sc = hy.Scratch(db)
+
+    def timed(f, k):
+        f()
+        sync()
+        t = time.perf_counter()
+        for _ in range(k):
+            r = f()
+        sync()
+        return (time.perf_counter() - t) / k, r
+
+    scan_s, hit = timed(lambda: db.scan(sc, q), 5)
+    al_s, ah = timed(lambda: db.scan_aligned(sc, q), 5)
+    if (ah.db_index, ah.alignment.score) != (hit.db_index, hit.score):
+        raise RuntimeError("scan_aligned disagrees with scan")
+    all_s, (recs, ops) = timed(lambda: db.scan_all_aligned_arrays(sc, q), 2)
+    if recs["score"][hit.db_index] != hit.score:
+        raise RuntimeError("scan_all_aligned disagrees with scan")
+    cells = len(q) * w.residues
+    rng = np.random.default_rng(23)
+    pick = np.concatenate([[hit.db_index], rng.choice(w.n, size=299, replace=False)])
+    sample = {int(k): (int(recs["score"][k]), int(recs["query_start"][k]),
+                       int(recs["query_end"][k]), int(recs["target_start"][k]),
+                       int(recs["target_end"][k]),
+                       ops[int(recs["ops_offset"][k]):int(recs["ops_offset"][k]) +
+                           int(recs["n_ops"][k])].tolist()) for k in pick}
+    a = ah.alignment
+    out = {
+        "database": f"{w.n} targets of {w.name}, SearchType::Alignment, max_query_len {len(q)}",
+        "build_s": round(build_s, 2),
+        "query_len": len(q),
+        "scan_ms": round(scan_s * 1e3, 3),
+        "scan_aligned_ms": round(al_s * 1e3, 3),
+        "traceback_overhead_ms": round((al_s - scan_s) * 1e3, 3),
+        "winner": {"db_index": ah.db_index, "score": a.score,
+                   "query_span": [a.query_start, a.query_end],
+                   "target_span": [a.target_start, a.target_end], "ops": len(a.op_codes),
+                   "cigar_prefix": a.cigar()[:60]},
+        "scan_all_aligned_s": round(all_s, 4),
+        "alignments_per_s": round(w.n / all_s, 1),
+        "gcups_equivalent": round(cells / all_s / 1e9, 1),
+        "ops_bytes_to_host": int(ops.size),
+    }
+    log(f"traceback: scan_aligned {out['scan_aligned_ms']} ms (scan {out['scan_ms']} ms), "
+        f"scan_all_aligned {all_s:.3f} s = {out['alignments_per_s']:.0f} alignments/s")
+    del sc, db
+    return out, {"query": q, "sample": sample}
+
+
+def cpu_traceback(w: wl.Workload, tb: dict) -> dict:
+    """The scalar restatement of the reference's traceback (full H/E/F matrices and the canonical
+    walk) on the sampled pairs, one thread, each result checked against the device's."""
+    import oracle
+    q = tb["query"]
+    secs, cells = 0.0, 0
+    for k, got in tb["sample"].items():
+        t = w.flat[w.off[k]:w.off[k + 1]]
+        t0 = time.perf_counter()
+        want = oracle.traceback(q, t, w.al, w.matrix, w.go, w.ge, w.mode)
+        secs += time.perf_counter() - t0
+        cells += len(q) * len(t)
+        if tuple(want) != got:
+            raise RuntimeError(f"device traceback of target {k} disagrees with the oracle")
+    n = len(tb["sample"])
+    return {"value": round(cells / secs / 1e9, 4), "unit": "GCUPS", "cores": 1, "kind": "port",
+            "alignments_per_s": round(n / secs, 1),
+            "sample": f"{n} (query, target) pairs of the traceback database (the winner and "
+                      f"{n - 1} random targets), scalar full-matrix traceback restatement, "
+                      f"{secs:.2f} s; every alignment equal to the device's"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,6 +363,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-query loop")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-traceback", action="store_true", help="skip the aligned scans")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the DNA configurations (10k-target scan, 2M-target mode sweep)")
     ap.add_argument("--autotune", action="store_true",
@@ -404,6 +491,7 @@ def main() -> int:
         }
     del sc, db
 
+    tb, tb_sample = (None, None) if args.no_traceback else traceback(w, sync)
     serve = None if args.no_serving else serving(args, sync)
     others = None if args.no_configs else other_configs(sync)
 
@@ -431,11 +519,14 @@ def main() -> int:
         "single_query": single,
         "serving": serve,
         "other_configs": others,
+        "traceback": tb,
         "roofline": roof,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, qs)
         out["gpu_vs_cpu"] = round(gcups / out["cpu_baseline"]["value"], 1)
+        if tb_sample is not None:
+            out["cpu_baseline"]["traceback"] = cpu_traceback(w, tb_sample)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -46,9 +46,12 @@ __global__ __launch_bounds__(64) void k_trace_dp(TraceArgs a) {
     const int n = (int)(a.off[gi + 1] - a.off[gi]);
     const int m = a.m, go = a.go, ge = a.ge, ale = a.al_eff;
     const int rowstride = LDSMAT ? ale : a.al;
+    // The LDS copy of the matrix carries +go, so the diagonal candidate H(i-1, j-1) + sub is one
+    // add from the stored H - go. Matrix entries are clamped to +-(2^29 - 1) and go < 2^29, so
+    // the biased entries cannot wrap.
     if (LDSMAT) {
         for (int k = lane; k < (ale + 1) * ale; k += 64)
-            smat[k] = k < ale * ale ? a.matrix[(k / ale) * a.al + (k % ale)] : kPad32;
+            smat[k] = k < ale * ale ? a.matrix[(k / ale) * a.al + (k % ale)] + go : kPad32;
     }
     __syncthreads();
 
@@ -72,6 +75,10 @@ __global__ __launch_bounds__(64) void k_trace_dp(TraceArgs a) {
         }
         int HGd = wsub(border<MODE>(r0 - 1, 0, go, ge), go);
         int topHG = 0, topF = kNeg32;
+        // Best answer-region cell of this lane in this pass. A lane meets its cells in
+        // (column, row) order, the tie-break's order, so a strictly greater score is the only
+        // update; passes are merged with the full tie-break below.
+        int pbs = INT_MIN, pbj = INT_MAX, pbi = INT_MAX;
         // Residues and the previous pass's boundary row go through LDS rings of four 64-entry
         // blocks, fetched one 64-step epoch ahead (the scheme of the long-target scan kernel).
         tring[lane] = lane < n ? t[lane] : (uint8_t)0;
@@ -109,8 +116,8 @@ __global__ __launch_bounds__(64) void k_trace_dp(TraceArgs a) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     const int sub = LDSMAT ? smat[qb[r] + c]
-                                           : (qb[r] >= 0 ? a.matrix[qb[r] + c] : kPad32);
-                    const int diag = wadd(wadd(hgd, go), sub);
+                                           : (qb[r] >= 0 ? a.matrix[qb[r] + c] + go : kPad32);
+                    const int diag = wadd(hgd, sub);
                     const int e = max(HG[r], wsub(E[r], ge));
                     const int fv = max(hgup, wsub(f, ge));
                     int h = max(max(diag, e), fv);
@@ -126,7 +133,10 @@ __global__ __launch_bounds__(64) void k_trace_dp(TraceArgs a) {
                     const int i = r0 + r;
                     const bool cand =
                         i <= m && (F::local || (F::lcol && j == n) || (i == m && (F::lrow || j == n)));
-                    consider(bs, bj, bi, cand ? h : INT_MIN, j, i);
+                    const bool take = cand & (h > pbs);
+                    pbs = take ? h : pbs;
+                    pbj = take ? j : pbj;
+                    pbi = take ? i : pbi;
                 }
                 outHG = HG[R - 1];
                 outF = f;
@@ -141,6 +151,7 @@ __global__ __launch_bounds__(64) void k_trace_dp(TraceArgs a) {
                 __syncthreads();
             }
         }
+        consider(bs, bj, bi, pbs, pbj, pbi);
         __threadfence_block();
         __syncthreads();
     }
