@@ -83,6 +83,17 @@ def _load():
     lib.hyo_traceback.argtypes = [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64, ctypes.c_uint32,
                                   _i32p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32, _i32p,
                                   _u64p, _u64p, _u64p, _u64p, _u8p, _u64p]
+    lib.hyo_position_max.restype = ctypes.c_int
+    lib.hyo_position_max.argtypes = [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64, ctypes.c_uint32,
+                                     _i32p, ctypes.c_int32, ctypes.c_int32, _i32p,
+                                     ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
+    lib.hyo_score2.restype = ctypes.c_int
+    lib.hyo_score2.argtypes = [_i32p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_uint64,
+                               ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
+                               ctypes.POINTER(ctypes.c_uint64)]
+    lib.hyo_brute_colmax.restype = None
+    lib.hyo_brute_colmax.argtypes = [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64,
+                                     ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32, _i32p]
     lib.hyo_traceback_min_bytes_for_database.restype = ctypes.c_uint64
     lib.hyo_traceback_min_bytes_for_database.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
     if hasattr(lib, "hyo_avx2_scan_scores_timed"):
@@ -168,6 +179,41 @@ def traceback(q, t, al: int, matrix, go: int, ge: int, mode: int):
 
 def traceback_min_bytes_for_database(max_m: int, max_n: int) -> int:
     return int(_load().hyo_traceback_min_bytes_for_database(max_m, max_n))
+
+
+def position_max(q, t, al: int, matrix, go: int, ge: int):
+    """SW per-target-position maxima (src/kernel.rs:474-575): (colmax list, score, target_end
+    or None)."""
+    qa = _u8(q) if len(q) else np.zeros(1, dtype=np.uint8)
+    ta = _u8(t) if len(t) else np.zeros(1, dtype=np.uint8)
+    out = np.zeros(max(len(t), 1), dtype=np.int32)
+    s, te = ctypes.c_int32(0), ctypes.c_int64(0)
+    rc = _load().hyo_position_max(_ptr(qa, _u8p), len(q), _ptr(ta, _u8p), len(t), al,
+                                  _ptr(_i32(matrix), _i32p), go, ge, _ptr(out, _i32p),
+                                  ctypes.byref(s), ctypes.byref(te))
+    if rc != OK:
+        raise MemoryError("oracle allocation failed")
+    return [int(v) for v in out[:len(t)]], s.value, _opt(te.value)
+
+
+def score2(colmax, best_score: int, best_target_end: int, matrix_max: int, min_score: int):
+    """src/kernel.rs:741-792: (score2, position) or None."""
+    a = np.ascontiguousarray(np.asarray(colmax, dtype=np.int32))
+    if a.size == 0:
+        a = np.zeros(1, dtype=np.int32)
+    s, p = ctypes.c_int32(0), ctypes.c_uint64(0)
+    found = _load().hyo_score2(_ptr(a, _i32p), len(colmax), best_score, best_target_end,
+                               matrix_max, min_score, ctypes.byref(s), ctypes.byref(p))
+    return (s.value, p.value) if found else None
+
+
+def brute_colmax(q, t, al: int, matrix, go: int, ge: int):
+    qa = _u8(q) if len(q) else np.zeros(1, dtype=np.uint8)
+    ta = _u8(t) if len(t) else np.zeros(1, dtype=np.uint8)
+    out = np.zeros(max(len(t), 1), dtype=np.int32)
+    _load().hyo_brute_colmax(_ptr(qa, _u8p), len(q), _ptr(ta, _u8p), len(t), al,
+                             _ptr(_i32(matrix), _i32p), go, ge, _ptr(out, _i32p))
+    return [int(v) for v in out[:len(t)]]
 
 
 def cigar(ops, extended: bool = False) -> str:

@@ -535,3 +535,93 @@ int hyo_traceback(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, ui
     free(h); free(e); free(f);
     return HYO_OK;
 }
+
+/* ---------------------------------------------------------------------------------------- */
+/* Pairwise extras: src/kernel.rs:474-575 (align_pair_position_max, scalar arm) and          */
+/* src/kernel.rs:741-792 (score2).                                                            */
+/* ---------------------------------------------------------------------------------------- */
+
+/* Per-target-position maxima of the SW DP: out[t] = max over rows of H[i][t+1] (floor 0), the
+ * best score = the array maximum, the target end = its first position (-1 when the best is 0). */
+int hyo_position_max(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
+                     const int32_t* matrix, int32_t go, int32_t ge, int32_t* out, int32_t* score,
+                     int64_t* tend) {
+    int64_t* h = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+    int64_t* f = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+    if (!h || !f) { free(h); free(f); return HYO_E_NOMEM; }
+    for (uint64_t j = 0; j <= n; ++j) { h[j] = 0; f[j] = HYO_NEG; }
+    for (uint64_t j = 0; j < n; ++j) out[j] = 0;
+    for (uint64_t i = 1; i <= m; ++i) {
+        int64_t diag = h[0], left = 0, ev = HYO_NEG;  /* H[i][0] = 0, E[i][0] = sentinel */
+        const int32_t* row = matrix + (uint64_t)q[i - 1] * al;
+        for (uint64_t j = 1; j <= n; ++j) {
+            const int64_t e1 = left - go, e2 = ev - ge;
+            ev = e1 > e2 ? e1 : e2;
+            const int64_t f1 = h[j] - go, f2 = f[j] - ge;
+            f[j] = f1 > f2 ? f1 : f2;
+            int64_t c = diag + row[t[j - 1]];
+            if (ev > c) c = ev;
+            if (f[j] > c) c = f[j];
+            if (c < 0) c = 0;
+            diag = h[j];
+            h[j] = c;
+            left = c;
+            if (c > out[j - 1]) out[j - 1] = (int32_t)c;
+        }
+    }
+    int32_t best = 0;
+    int64_t pos = -1;
+    for (uint64_t j = 0; j < n; ++j)
+        if (out[j] > best) { best = out[j]; pos = (int64_t)j; }
+    *score = best;
+    *tend = best > 0 ? pos : -1;
+    free(h); free(f);
+    return HYO_OK;
+}
+
+/* bwa's score2 recipe as the reference streams it: above-threshold columns contiguous with the
+ * open peak's maximum position extend it, anything else closes it; a closed peak outside
+ * [te - w, te + w], w = ceil(best / matrix_max), replaces the runner-up only when strictly
+ * higher. Returns 1 with (score, pos) set, or 0 for None. */
+int hyo_score2(const int32_t* colmax, uint64_t n, int32_t best_score, uint64_t best_tend,
+               int32_t matrix_max, int32_t min_score, int32_t* s2, uint64_t* pos) {
+    if (matrix_max <= 0) return 0;
+    const int64_t bsc = best_score > 0 ? best_score : 0;
+    int64_t w = (bsc + matrix_max - 1) / matrix_max;
+    if (w < 0) w = 0;
+    const int64_t lo = (int64_t)best_tend - w, hi = (int64_t)best_tend + w;
+    int have2 = 0, open = 0;
+    int32_t b2s = 0, cs = 0;
+    uint64_t b2p = 0, cp = 0;
+    for (uint64_t i = 0; i <= n; ++i) {
+        const int close_now = i == n || (colmax[i] >= min_score && !(open && cp + 1 == i));
+        if (i < n && colmax[i] < min_score) continue;
+        if (open && close_now) {
+            const int64_t p = (int64_t)cp;
+            if ((p < lo || p > hi) && (!have2 || cs > b2s)) { have2 = 1; b2s = cs; b2p = cp; }
+            open = 0;
+        }
+        if (i == n) break;
+        if (open) {
+            if (cs < colmax[i]) { cs = colmax[i]; cp = i; }
+        } else {
+            open = 1; cs = colmax[i]; cp = i;
+        }
+    }
+    if (have2) { *s2 = b2s; *pos = b2p; }
+    return have2;
+}
+
+/* Brute-force column maxima: out[t] = max over query ends i of the best local alignment ending
+ * exactly at cell (i, t+1), by enumerating start windows (tiny inputs only). */
+void hyo_brute_colmax(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
+                      const int32_t* mat, int32_t go, int32_t ge, int32_t* out) {
+    for (uint64_t j = 1; j <= n; ++j) {
+        int64_t best = 0;
+        for (uint64_t i = 0; i <= m; ++i) {
+            const int64_t v = brute_cell(HYO_MODE_SW, q, i, t, j, al, mat, go, ge);
+            if (v > best) best = v;
+        }
+        out[j - 1] = (int32_t)best;
+    }
+}

@@ -55,6 +55,18 @@ int hyo_traceback(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, ui
 /* src/align.rs:230-275 (traceback_min_bytes_for_database). */
 uint64_t hyo_traceback_min_bytes_for_database(uint64_t max_m, uint64_t max_n);
 
+/* src/kernel.rs:474-575: SW per-target-position maxima (out has n entries), best score, first
+ * target position of the best (-1 when the best is 0). */
+int hyo_position_max(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
+                     const int32_t* matrix, int32_t go, int32_t ge, int32_t* out, int32_t* score,
+                     int64_t* tend);
+/* src/kernel.rs:741-792: 1 with (score2, pos) when a secondary peak qualifies, else 0. */
+int hyo_score2(const int32_t* colmax, uint64_t n, int32_t best_score, uint64_t best_tend,
+               int32_t matrix_max, int32_t min_score, int32_t* s2, uint64_t* pos);
+/* Column maxima by enumeration of local alignments ending in each column (tiny inputs). */
+void hyo_brute_colmax(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
+                      const int32_t* mat, int32_t go, int32_t ge, int32_t* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,113 @@
+"""Pins the oracle's pairwise extras before the GPU pair kernels are checked against them:
+align_pair_position_max (src/kernel.rs:474-575) and score2 (src/kernel.rs:741-792).
+
+Checks come from the reference's own tests (hand vectors of src/kernel.rs:1028-1059 and
+1245-1276), from enumeration (column maxima as the best local alignment ending in each column,
+by brute force over start windows), from the relation to the pinned scan oracle (best score and
+target end equal align_core in SW mode), and, for score2, from the reference's structurally
+different transcription (build the explicit peak list, then filter; src/kernel.rs:1207-1243).
+"""
+import random
+
+import pytest
+
+import oracle as orc
+from common import all_sequences, dna_scheme, identity_matrix
+
+MODE_SW = 0
+
+
+def pm(q, t, scheme=None):
+    al, mat, go, ge = scheme or dna_scheme()
+    return orc.position_max(q, t, al, mat, go, ge)
+
+
+# ---- position_max: src/kernel.rs:1028-1059 ---------------------------------------------------
+
+def test_position_max_hand_computed_cases():
+    assert pm([0, 1, 2, 3], [0, 1, 2, 3]) == ([2, 4, 6, 8], 8, 3)
+    assert pm([0], [1, 1, 0]) == ([0, 0, 2], 2, 2)
+    assert pm([0, 0], [1, 1]) == ([0, 0], 0, None)
+    assert pm([0, 1], []) == ([], 0, None)
+    assert pm([], [0, 1, 2]) == ([0, 0, 0], 0, None)
+
+
+SCHEMES = [
+    (2, identity_matrix(2, 2, -1), 2, 1),
+    (2, identity_matrix(2, 1, -1), 3, 3),
+    (3, identity_matrix(3, 3, -2), 4, 2),
+    (3, identity_matrix(3, 2, -1), 1, 0),
+]
+
+
+@pytest.mark.parametrize("scheme", SCHEMES)
+def test_position_max_equals_enumeration_over_short_pairs(scheme):
+    al, mat, go, ge = scheme
+    seqs = all_sequences(al, 3)
+    for q in seqs:
+        for t in seqs:
+            col, s, te = orc.position_max(q, t, al, mat, go, ge)
+            assert col == orc.brute_colmax(q, t, al, mat, go, ge), (q, t)
+            assert all(v >= 0 for v in col) and s == max(col + [0])
+
+
+def test_position_max_agrees_with_the_scan_oracle_on_random_schemes():
+    """src/kernel.rs:1076-1102: score and target end equal align_pair(SW, ScoreEnd); the array
+    maximum is the score. Asymmetric random matrices catch a transposed lookup."""
+    rng = random.Random(11)
+    for _ in range(400):
+        al = rng.randint(2, 5)
+        mat = [rng.randint(-6, 6) for _ in range(al * al)]
+        go = rng.randint(0, 7)
+        ge = rng.randint(0, go)
+        q = [rng.randrange(al) for _ in range(rng.randrange(0, 14))]
+        t = [rng.randrange(al) for _ in range(rng.randrange(0, 14))]
+        col, s, te = orc.position_max(q, t, al, mat, go, ge)
+        assert len(col) == len(t)
+        ws, _, wte = orc.align_core(q, t, al, mat, go, ge, MODE_SW)
+        assert (s, te) == (ws, wte), (q, t, al, mat, go, ge)
+        if len(q) <= 5 and len(t) <= 5:
+            assert col == orc.brute_colmax(q, t, al, mat, go, ge)
+
+
+# ---- score2: src/kernel.rs:1245-1276 and the peak-list transcription of 1207-1243 --------------
+
+def score2_peak_list(colmax, best_score, te, matrix_max, min_score):
+    if matrix_max <= 0:
+        return None
+    peaks = []
+    for i, v in enumerate(colmax):
+        if v < min_score:
+            continue
+        if peaks and peaks[-1][1] + 1 == i:
+            if peaks[-1][0] < v:
+                peaks[-1] = (v, i)
+        else:
+            peaks.append((v, i))
+    w = max((max(best_score, 0) + matrix_max - 1) // matrix_max, 0)
+    lo, hi = te - w, te + w
+    best2 = None
+    for score, pos in peaks:
+        if (pos < lo or pos > hi) and (best2 is None or score > best2[0]):
+            best2 = (score, pos)
+    return best2
+
+
+def test_score2_hand_computed_cases():
+    assert orc.score2([0, 5, 0, 0, 0, 0, 0, 0, 4, 0], 5, 1, 1, 1) == (4, 8)
+    assert orc.score2([0, 3, 5, 4, 0], 5, 2, 5, 1) is None
+    assert orc.score2([3, 5, 4, 6], 1, 20, 1, 1) == (6, 3)
+    assert orc.score2([7, 0, 7], 1, 10, 1, 1) == (7, 0)
+    assert orc.score2([0, 0, 0], 0, 0, 1, 1) is None
+    assert orc.score2([5, 5], 5, 0, 0, 1) is None
+    colmax = [2, 0, 0, 0, 0, 0, 9]
+    assert orc.score2(colmax, 9, 6, 9, 5) is None
+    assert orc.score2(colmax, 9, 6, 9, 1) == (2, 0)
+
+
+def test_score2_matches_the_peak_list_transcription():
+    rng = random.Random(5)
+    for _ in range(3000):
+        colmax = [rng.randint(0, 20) for _ in range(rng.randint(0, 40))]
+        args = (rng.randint(0, 40), rng.randint(0, 40), rng.randint(-2, 8), rng.randint(1, 6))
+        assert orc.score2(colmax, *args) == score2_peak_list(colmax, *args), (colmax, args)
