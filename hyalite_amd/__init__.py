@@ -36,6 +36,8 @@ __all__ = [
     "ScanContractViolation", "DeviceError", "HipLibraryMissing", "BACKEND_ENV_VAR", "build",
     "library_path", "load_library", "Tuning", "Server", "ServedHit", "ServerStats",
     "ServerTimeout", "AlignOp", "Alignment", "AlignedHit", "TracebackBudgetExceeded",
+    "PairScratch", "align_pair", "align_pair_with", "align_pairs", "align_pair_position_max",
+    "align_pair_position_max_with", "score2",
 ]
 
 BACKEND_ENV_VAR = "HYALITE_BACKEND"
@@ -46,9 +48,10 @@ _LIB_NAME = "libhyalite_hip.so"
 # Translation units: the host library, the autotuner, the mode-independent kernels and dispatch,
 # and the mode-templated kernels compiled once per alignment mode (SW, NW, HW, OV, SHW).
 _UNITS = [("csrc/hy_host.cpp", []), ("csrc/hy_tune.cpp", []), ("csrc/hy_serve.cpp", []),
-          ("csrc/hy_align.cpp", []),
+          ("csrc/hy_align.cpp", []), ("csrc/hy_pair.cpp", []),
           ("csrc/hy_launch.hip", [])] + [
-    (src, [f"-DHY_KMODE={k}"]) for src in ("csrc/hy_kernels.hip", "csrc/hy_trace.hip")
+    (src, [f"-DHY_KMODE={k}"])
+    for src in ("csrc/hy_kernels.hip", "csrc/hy_trace.hip", "csrc/hy_pair.hip")
     for k in range(5)]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -527,6 +530,28 @@ def load_library():
     lib.hy_scan_all_aligned.restype = ctypes.c_int
     lib.hy_scan_all_aligned.argtypes = [vp, vp, _u8p, ctypes.c_size_t, ctypes.c_void_p, _u8p,
                                         ctypes.c_size_t]
+    lib.hy_pair_scratch_new.restype = ctypes.c_int
+    lib.hy_pair_scratch_new.argtypes = [ctypes.POINTER(vp)]
+    lib.hy_pair_scratch_free.restype = None
+    lib.hy_pair_scratch_free.argtypes = [vp]
+    lib.hy_align_pair.restype = ctypes.c_int
+    lib.hy_align_pair.argtypes = [vp, _u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t, _i32p,
+                                  ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int32,
+                                  ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, _i32p, _i64p,
+                                  _i64p, _i64p]
+    lib.hy_align_pairs.restype = ctypes.c_int
+    lib.hy_align_pairs.argtypes = [vp, ctypes.POINTER(_u8p), _sizep, ctypes.POINTER(_u8p), _sizep,
+                                   ctypes.c_size_t, _i32p, ctypes.c_size_t, ctypes.c_uint32,
+                                   ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
+                                   ctypes.c_uint32, _i32p, _i64p, _i64p, _i64p, _u64p]
+    lib.hy_align_pair_position_max.restype = ctypes.c_int
+    lib.hy_align_pair_position_max.argtypes = [vp, _u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t,
+                                               _i32p, ctypes.c_size_t, ctypes.c_uint32,
+                                               ctypes.c_int32, ctypes.c_int32, _i32p, _i32p,
+                                               _i64p, _i64p]
+    lib.hy_score2.restype = ctypes.c_int
+    lib.hy_score2.argtypes = [_i32p, ctypes.c_size_t, ctypes.c_int32, ctypes.c_uint64,
+                              ctypes.c_int32, ctypes.c_int32, _i32p, _i32p, _u64p]
     lib.hy_scan_batch.restype = ctypes.c_int
     lib.hy_scan_batch.argtypes = [vp, vp, ctypes.POINTER(_u8p), _sizep, ctypes.c_size_t, _i32p,
                                   _u64p, _i64p, _i64p]
@@ -997,6 +1022,146 @@ class ServerStats:
 
 class ServerTimeout(TimeoutError):
     pass
+
+
+# ---- pairwise alignment (src/kernel.rs:285-575, 741-792) ------------------------------------------
+
+class PairScratch:
+    """PairScratch (src/kernel.rs:285-311): a HIP stream and device buffers that grow to the
+    largest problem seen and are reused. One per thread."""
+
+    def __init__(self):
+        out = ctypes.c_void_p(None)
+        rc = load_library().hy_pair_scratch_new(ctypes.byref(out))
+        if rc == 9:
+            raise BackendUnavailable()
+        if rc != 0:
+            raise DeviceError(load_library().hy_strerror(rc).decode())
+        self._h = out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.hy_pair_scratch_free(h)
+            self._h = None
+
+
+def _raise_pair_error(rc: int, detail: int, scoring: Scoring):
+    if rc in (1, 2, 3, 4, 5, 6):
+        _raise_build_error(rc, detail, scoring.alphabet_len(), None)
+    raise DeviceError(f"{load_library().hy_strerror(rc).decode()} (status {rc})")
+
+
+def _st_code(search_type) -> int:
+    return int(search_type)
+
+
+def align_pair_with(scratch: PairScratch, query, target, scoring: Scoring, mode: Mode,
+                    search_type) -> BestHit:
+    """src/kernel.rs:354-420: one pair on the device (one wavefront); errors as the reference's
+    (SymbolOutOfRange, ScoreRangeTooWide)."""
+    q, t = np.asarray(query, dtype=np.uint8), np.asarray(target, dtype=np.uint8)
+    qa, ta = _as_u8(q), _as_u8(t)
+    mat = scoring.matrix()
+    s, qe, te, det = ctypes.c_int32(0), ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    rc = load_library().hy_align_pair(
+        scratch._h, qa.ctypes.data_as(_u8p), q.size, ta.ctypes.data_as(_u8p), t.size,
+        mat.ctypes.data_as(_i32p), mat.size, scoring.alphabet_len(), scoring.gap_open(),
+        scoring.gap_ext(), int(Mode(mode)), _st_code(search_type), ctypes.byref(s),
+        ctypes.byref(qe), ctypes.byref(te), ctypes.byref(det))
+    if rc != 0:
+        _raise_pair_error(rc, det.value, scoring)
+    return BestHit(s.value, 0, None if qe.value < 0 else qe.value,
+                   None if te.value < 0 else te.value)
+
+
+def align_pair(query, target, scoring: Scoring, mode: Mode, search_type) -> BestHit:
+    """src/kernel.rs:331-352 (a fresh PairScratch per call; reuse one with align_pair_with)."""
+    return align_pair_with(PairScratch(), query, target, scoring, mode, search_type)
+
+
+def align_pairs(pairs, scoring: Scoring, mode: Mode, search_type, out: Optional[list] = None,
+                scratch: Optional[PairScratch] = None) -> list:
+    """src/kernel.rs:429-452: one BestHit per (query, target) pair, db_index = pair index; the
+    first failing pair's error is raised."""
+    qs = [np.ascontiguousarray(np.asarray(q, dtype=np.uint8)) for q, _ in pairs]
+    ts = [np.ascontiguousarray(np.asarray(t, dtype=np.uint8)) for _, t in pairs]
+    n = len(qs)
+    qp, tp = (_u8p * max(n, 1))(), (_u8p * max(n, 1))()
+    ql, tl = (ctypes.c_size_t * max(n, 1))(), (ctypes.c_size_t * max(n, 1))()
+    keep = []
+    for k in range(n):
+        a = qs[k] if qs[k].size else np.zeros(1, dtype=np.uint8)
+        b = ts[k] if ts[k].size else np.zeros(1, dtype=np.uint8)
+        keep += [a, b]
+        qp[k], tp[k] = a.ctypes.data_as(_u8p), b.ctypes.data_as(_u8p)
+        ql[k], tl[k] = qs[k].size, ts[k].size
+    scores = np.empty(max(n, 1), dtype=np.int32)
+    qe = np.empty(max(n, 1), dtype=np.int64)
+    te = np.empty(max(n, 1), dtype=np.int64)
+    mat = scoring.matrix()
+    det, bad = ctypes.c_int64(0), ctypes.c_uint64(0)
+    sc = scratch or PairScratch()
+    rc = load_library().hy_align_pairs(
+        sc._h, qp, ql, tp, tl, n, mat.ctypes.data_as(_i32p), mat.size, scoring.alphabet_len(),
+        scoring.gap_open(), scoring.gap_ext(), int(Mode(mode)), _st_code(search_type),
+        scores.ctypes.data_as(_i32p), qe.ctypes.data_as(_i64p), te.ctypes.data_as(_i64p),
+        ctypes.byref(det), ctypes.byref(bad))
+    if out is not None:
+        out.clear()
+    if rc != 0:
+        _raise_pair_error(rc, det.value, scoring)
+    hits = [BestHit(int(scores[k]), k, None if qe[k] < 0 else int(qe[k]),
+                    None if te[k] < 0 else int(te[k])) for k in range(n)]
+    if out is not None:
+        out.extend(hits)
+        return out
+    return hits
+
+
+def align_pair_position_max_with(scratch: PairScratch, query, target, scoring: Scoring,
+                                 out: Optional[list] = None) -> BestHit:
+    """src/kernel.rs:490-575: SW per-target-position maxima into `out` (cleared and refilled);
+    the hit's score is their maximum, its target_end the first position holding it (None when
+    the best is 0); query_end is not tracked."""
+    q, t = np.asarray(query, dtype=np.uint8), np.asarray(target, dtype=np.uint8)
+    qa, ta = _as_u8(q), _as_u8(t)
+    col = np.zeros(max(t.size, 1), dtype=np.int32)
+    mat = scoring.matrix()
+    s, te, det = ctypes.c_int32(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    if out is not None:
+        out.clear()
+    rc = load_library().hy_align_pair_position_max(
+        scratch._h, qa.ctypes.data_as(_u8p), q.size, ta.ctypes.data_as(_u8p), t.size,
+        mat.ctypes.data_as(_i32p), mat.size, scoring.alphabet_len(), scoring.gap_open(),
+        scoring.gap_ext(), col.ctypes.data_as(_i32p), ctypes.byref(s), ctypes.byref(te),
+        ctypes.byref(det))
+    if rc != 0:
+        _raise_pair_error(rc, det.value, scoring)
+    if out is not None:
+        out.extend(int(v) for v in col[:t.size])
+    return BestHit(s.value, 0, None, None if te.value < 0 else te.value)
+
+
+def align_pair_position_max(query, target, scoring: Scoring,
+                            out: Optional[list] = None) -> BestHit:
+    return align_pair_position_max_with(PairScratch(), query, target, scoring, out)
+
+
+def score2(colmax, best_score: int, best_target_end: int, matrix_max: int,
+           min_score: int) -> Optional[tuple]:
+    """src/kernel.rs:741-792: (score2, target position) of the best secondary peak, or None."""
+    a = np.ascontiguousarray(np.asarray(colmax, dtype=np.int32))
+    n = a.size
+    if n == 0:
+        a = np.zeros(1, dtype=np.int32)
+    found, s2, pos = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_uint64(0)
+    rc = load_library().hy_score2(a.ctypes.data_as(_i32p), n, best_score, best_target_end,
+                                  matrix_max, min_score, ctypes.byref(found), ctypes.byref(s2),
+                                  ctypes.byref(pos))
+    if rc != 0:
+        raise DeviceError(load_library().hy_strerror(rc).decode())
+    return (s2.value, pos.value) if found.value else None
 
 
 class Server:

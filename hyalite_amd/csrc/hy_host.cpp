@@ -46,6 +46,10 @@ hipError_t dev_upload(T** dst, const std::vector<T>& src) {
 }
 
 
+}  // namespace
+
+namespace hy {
+
 // ScoreWidth::max_abs (src/width.rs:48-55) and required_width (src/width.rs:145-169).
 int width_bits_for(__int128 bound, int64_t* bound_out) {
     const int64_t b = bound > (__int128)INT64_MAX ? INT64_MAX : (int64_t)bound;
@@ -76,6 +80,33 @@ __int128 magnitude_bound(uint32_t mode, int32_t min_e, int32_t max_e, int32_t go
     }
     return positive > negative ? positive : negative;
 }
+
+// Scoring::new (src/scoring.rs:65-100): alphabet, matrix shape, gap signs and order, in that
+// order; on success the matrix's entry bounds.
+int validate_scoring(const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len,
+                     int32_t gap_open, int32_t gap_ext, int32_t* min_e, int32_t* max_e,
+                     int64_t* err_detail) {
+    if (alphabet_len == 0) return HY_E_EMPTY_ALPHABET;
+    if ((uint64_t)alphabet_len * (uint64_t)alphabet_len != (uint64_t)matrix_len) {
+        if (err_detail) *err_detail = (int64_t)matrix_len;
+        return HY_E_MATRIX_SHAPE;
+    }
+    if (!matrix) return HY_E_NULL_ARGUMENT;
+    if (gap_open < 0 || gap_ext < 0) return HY_E_NEGATIVE_GAP_PENALTY;
+    if (gap_open < gap_ext) return HY_E_GAP_OPEN_LESS_THAN_EXTEND;
+    *min_e = *max_e = matrix[0];
+    for (size_t k = 1; k < matrix_len; ++k) {
+        *min_e = std::min(*min_e, matrix[k]);
+        *max_e = std::max(*max_e, matrix[k]);
+    }
+    return HY_OK;
+}
+
+}  // namespace hy
+
+namespace {
+using hy::magnitude_bound;
+using hy::width_bits_for;
 
 // The traceback budget of a database (src/align.rs:228-268): the smallest max_bytes with which
 // the reference's traceback serves every (m, n) in [0, max_m] x [0, max_n]. A full traceback
@@ -151,7 +182,7 @@ extern "C" int hy_backend_parse(const char* name, int32_t* out) {
     return HY_E_INVALID_BACKEND_NAME;
 }
 
-static int first_gfx950_device() {
+int hy::first_gfx950_device() {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess) return -1;
     for (int d = 0; d < count; ++d) {
@@ -179,19 +210,9 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     // Scoring::new (src/scoring.rs:65-100) runs before the builder sees the scheme.
     int32_t min_e = 0, max_e = 0;
     if (have & HY_HAVE_SCORING) {
-        if (alphabet_len == 0) return HY_E_EMPTY_ALPHABET;
-        if ((uint64_t)alphabet_len * (uint64_t)alphabet_len != (uint64_t)matrix_len) {
-            if (err_detail) *err_detail = (int64_t)matrix_len;
-            return HY_E_MATRIX_SHAPE;
-        }
-        if (!matrix) return HY_E_NULL_ARGUMENT;
-        if (gap_open < 0 || gap_ext < 0) return HY_E_NEGATIVE_GAP_PENALTY;
-        if (gap_open < gap_ext) return HY_E_GAP_OPEN_LESS_THAN_EXTEND;
-        min_e = max_e = matrix[0];
-        for (size_t k = 1; k < matrix_len; ++k) {
-            min_e = std::min(min_e, matrix[k]);
-            max_e = std::max(max_e, matrix[k]);
-        }
+        const int rc = hy::validate_scoring(matrix, matrix_len, alphabet_len, gap_open, gap_ext,
+                                            &min_e, &max_e, err_detail);
+        if (rc != HY_OK) return rc;
     }
     // IncompleteBuilder in field order (src/database.rs:738-749).
     const uint32_t order[4] = {HY_HAVE_SEQUENCES, HY_HAVE_SCORING, HY_HAVE_MODE,
@@ -247,7 +268,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
             if (b < 0) return HY_E_BACKEND_UNAVAILABLE;  // a CPU backend was requested
         }
     }
-    const int device = first_gfx950_device();
+    const int device = hy::first_gfx950_device();
     if (device < 0) return HY_E_BACKEND_UNAVAILABLE;
     if (hipSetDevice(device) != hipSuccess) return HY_E_DEVICE;
 

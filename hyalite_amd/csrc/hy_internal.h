@@ -125,6 +125,7 @@ struct SysArgs {
     const uint32_t* qlen;
     const uint32_t* nq;
     int qsel;                 // >= 0: every block uses query qsel instead of blockIdx.y
+    int pairs;                // nonzero: block b aligns pair list[b] (its own query and target)
 };
 
 // Traceback (src/align.rs:169-420), one (query, target) pair per item. The DP kernel writes a
@@ -172,6 +173,67 @@ inline uint64_t trace_slab_words(int m, uint64_t n) {
     return (w + 63) & ~63ull;
 }
 
+// Independent (query, target) pairs (align_pairs, src/kernel.rs:429-452), one pair per lane.
+// Pairs are sorted by (query length, target length), descending, and cut into groups of 64;
+// each group's queries and targets are lane-interleaved like the database layout (one 32-bit
+// word carries four symbols of a lane; the padding symbol al_eff fills past each length).
+struct PairGroup {
+    uint32_t first;      // first slot of the group
+    uint32_t count;      // live lanes, <= 64
+    uint32_t ncols;      // longest target
+    uint32_t nrows;      // longest query
+    uint64_t tword_off;  // target words: twords[tword_off + (col/4)*64 + lane]
+    uint64_t qword_off;  // query words: qwords[qword_off + (row/4)*64 + lane], whole tiles
+    uint64_t spill_off;  // tile boundary rows: spill[spill_off + col*64 + lane], col <= ncols
+};
+
+struct PairArgs {
+    const PairGroup* groups;
+    const uint32_t* twords;
+    const uint32_t* qwords;
+    const uint32_t* tlens;    // per slot
+    const uint32_t* qlens;    // per slot
+    const uint32_t* pidx;     // per slot -> pair index
+    const int32_t* matrix;    // al x al, entries clamped to +-kEntryMax32
+    int al, al_eff;
+    int go, ge;
+    int2* spill;
+    int32_t* out_score;       // per pair
+    int64_t* out_qend;
+    int64_t* out_tend;
+};
+
+struct PairPackArgs {
+    const PairGroup* groups;
+    const uint8_t* qflat;
+    const uint64_t* qoff;     // per pair
+    const uint8_t* tflat;
+    const uint64_t* toff;
+    const uint32_t* pidx;     // per slot -> pair index
+    const uint32_t* qlens;    // per slot
+    const uint32_t* tlens;    // per slot
+    uint32_t* qwords;
+    uint32_t* twords;
+    uint32_t pad_symbol;
+    int qtile;                // rows per tile: each group's query words cover whole tiles
+};
+
+// Local-mode per-target-position maxima of one pair (align_pair_position_max,
+// src/kernel.rs:474-575): the systolic shape with every lane folding its rows' maxima into
+// colmax[j - 1] (zeroed beforehand), then the best and its first column.
+struct PosMaxArgs {
+    const uint8_t* query;
+    int m;
+    const uint8_t* target;
+    int n;
+    const int32_t* matrix;
+    int al, al_eff;
+    int go, ge;
+    int2* spill;              // n + 1 entries (queries beyond one pass)
+    int32_t* colmax;          // n entries
+    int2* best;               // (score, target end or -1)
+};
+
 // Launchers (hy_launch.hip). All launch on `stream` and never synchronise.
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
                            hipStream_t stream);
@@ -179,6 +241,14 @@ hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngro
                           hipStream_t stream);
 // nq_grid > 1: batched form, a grid of nblocks x nq_grid blocks; a.m is then the longest query
 // of the batch (it fixes the rows per lane).
+// Pairs: lane-interleave a batch's queries and targets (one workgroup per group).
+hipError_t launch_pair_pack(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream);
+// Pairs: one launch over `ngroups` groups. q: rows per tile (16 or 32).
+hipError_t launch_pairs32(int mode, bool ends, const PairArgs& a, uint32_t ngroups, int q,
+                          hipStream_t stream);
+// Column maxima of one local-mode pair (zeroes colmax first).
+hipError_t launch_posmax(const PosMaxArgs& a, hipStream_t stream);
+hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t stream);
 // Traceback: the DP over a.nitems items, then the walk (one lane per item).
 hipError_t launch_trace(int mode, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
@@ -228,5 +298,9 @@ template <int MODE>
 uint32_t inter16_batch_occupancy_m(int q, int al_eff);
 template <int MODE>
 hipError_t launch_trace_m(const TraceArgs& a, hipStream_t st);
+template <int MODE>
+hipError_t launch_pairs32_m(bool ends, const PairArgs& a, uint32_t ngroups, int q,
+                            hipStream_t st);
+hipError_t launch_posmax_sw(const PosMaxArgs& a, hipStream_t st);
 
 }  // namespace hy

@@ -851,7 +851,9 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
     // Winner mode: an all-zero key means no hit was recorded; fall back to index 0 and let the
     // self-check below flag it.
     const bool batch = a.qoff != nullptr;
-    const uint32_t qi = batch ? (a.qsel >= 0 ? (uint32_t)a.qsel : blockIdx.y) : 0u;
+    // Pair mode: block b aligns pair list[b], whose query is qoff/qlen[list[b]].
+    const uint32_t qi =
+        batch ? (a.qsel >= 0 ? (uint32_t)a.qsel : (a.pairs ? a.list[blockIdx.x] : blockIdx.y)) : 0u;
     if (batch && qi >= *a.nq) return;
     const uint8_t* query = batch ? a.queries + a.qoff[qi] : a.query;
     const unsigned long long wk = WINNER ? a.winner_key[qi] : 0ull;
@@ -874,7 +876,8 @@ __global__ __launch_bounds__(64) void k_sys32(SysArgs a) {
     constexpr int P = 64 * R;
     const int npass = (m + P - 1) / P;
     const size_t spill_block =
-        WINNER ? qi : (a.qsel >= 0 ? blockIdx.x : (size_t)qi * gridDim.x + blockIdx.x);
+        WINNER ? qi
+               : ((a.qsel >= 0 || a.pairs) ? blockIdx.x : (size_t)qi * gridDim.x + blockIdx.x);
     int2* sp = a.spill + spill_block * a.spill_stride;
     for (int p = 0; p < npass; ++p) {
         const int r0 = p * P + lane * R + 1;

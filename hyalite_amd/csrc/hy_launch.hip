@@ -10,6 +10,47 @@ __global__ void k_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     if (i < n) p[i] = v;
 }
 
+__global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// One workgroup per pair group: lane l gathers four symbols of its pair per word (the padding
+// symbol past the pair's length, and for lanes past the group's count) and the group's words
+// are written lane-interleaved, so the pair kernel's per-column load is one coalesced line.
+__global__ __launch_bounds__(64) void k_pair_pack(PairPackArgs a) {
+    const PairGroup g = a.groups[blockIdx.x];
+    const int lane = threadIdx.x;
+    const bool live = lane < (int)g.count;
+    const uint32_t slot = g.first + (live ? lane : 0);
+    const uint32_t pair = a.pidx[slot];
+    const uint32_t pad = a.pad_symbol & 0xffu;
+    const uint64_t tn = live ? a.tlens[slot] : 0, qn = live ? a.qlens[slot] : 0;
+    const uint8_t* t = a.tflat + a.toff[pair];
+    const uint8_t* q = a.qflat + a.qoff[pair];
+    const uint32_t tw = (g.ncols + 3) / 4;
+    for (uint32_t w = 0; w < tw; ++w) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t c = (uint64_t)w * 4 + k;
+            v |= (c < tn ? (uint32_t)t[c] : pad) << (8 * k);
+        }
+        a.twords[g.tword_off + (uint64_t)w * 64 + lane] = v;
+    }
+    const uint32_t qt = (uint32_t)a.qtile;
+    const uint32_t qw = (g.nrows + qt - 1) / qt * qt / 4;
+    for (uint32_t w = 0; w < qw; ++w) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t r = (uint64_t)w * 4 + k;
+            v |= (r < qn ? (uint32_t)q[r] : pad) << (8 * k);
+        }
+        a.qwords[g.qword_off + (uint64_t)w * 64 + lane] = v;
+    }
+}
+
 __global__ void k_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) keys[i] = 0ull;
@@ -87,6 +128,38 @@ hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, stream, p, n, v);
     return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_pack(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream) {
+    if (ngroups == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_pack, dim3(ngroups), dim3(64), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pairs32(int mode, bool ends, const PairArgs& a, uint32_t ngroups, int q,
+                          hipStream_t stream) {
+    if (ngroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_pairs32_m<kSW>(ends, a, ngroups, q, stream);
+    case kNW: return launch_pairs32_m<kNW>(ends, a, ngroups, q, stream);
+    case kHW: return launch_pairs32_m<kHW>(ends, a, ngroups, q, stream);
+    case kOV: return launch_pairs32_m<kOV>(ends, a, ngroups, q, stream);
+    case kSHW: return launch_pairs32_m<kSHW>(ends, a, ngroups, q, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_posmax(const PosMaxArgs& a, hipStream_t stream) {
+    const hipError_t e = launch_fill_i32(a.colmax, (uint64_t)a.n, 0, stream);
+    if (e != hipSuccess) return e;
+    return launch_posmax_sw(a, stream);
 }
 
 hipError_t launch_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter,

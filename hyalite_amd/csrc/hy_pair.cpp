@@ -1,0 +1,636 @@
+// Pairwise alignment on the device: align_pair, align_pairs, align_pair_position_max and score2
+// (src/kernel.rs:285-575, 741-792). No database: every call carries its scoring scheme and
+// sequences, stages them through a pinned buffer of the caller's hy_pair_scratch and runs
+//   * one pair, or the long pairs of a batch: k_sys32 in pair mode (one wavefront per pair);
+//   * the rest of a large batch: k_pair_pack + k_pairs32 (one pair per lane, hy_pair.hip);
+//   * position maxima: k_posmax (one wavefront).
+// Validation follows the reference: Scoring::new's checks, then per pair the symbols (query, then
+// target) and the width proof for the pair's own lengths.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/hyalite_hip.h"
+#include "hy_internal.h"
+#include "hy_state.h"
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct hy_pair_scratch {
+    int device = 0;
+    hipStream_t stream = nullptr, aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    DevBuf mat, qflat, tflat, qoff, toff, qoff32, qlen32, list, nq, groups, twords, qwords, tlens,
+        qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best;
+    uint8_t* stage = nullptr;  // pinned
+    size_t stage_bytes = 0;
+
+    ~hy_pair_scratch() {
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (aux) (void)hipStreamSynchronize(aux);
+        for (DevBuf* b : {&mat, &qflat, &tflat, &qoff, &toff, &qoff32, &qlen32, &list, &nq,
+                          &groups, &twords, &qwords, &tlens, &qlens, &pidx, &spill, &sys_spill,
+                          &scores, &qend, &tend, &colmax, &best})
+            hy::dev_free(b->p);
+        if (stage) (void)hipHostFree(stage);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (aux) (void)hipStreamDestroy(aux);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Pairs past these lengths always run one per wavefront: a lane would sweep too many cells.
+constexpr size_t kLaneMaxQuery = 4096;
+constexpr size_t kLaneMaxTarget = 65536;
+// A batch runs one pair per lane when it has at least this many short pairs (enough 64-lane
+// groups to keep every SIMD busy); smaller batches run one pair per wavefront.
+constexpr size_t kLaneMinPairs = 32768;
+// Boundary rows of one launch of long multi-pass pairs.
+constexpr size_t kSysSpillBudget = 512ull << 20;
+
+// Grow a device buffer (contents dropped; every entry point synchronises before returning).
+int grow(DevBuf& b, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 16);
+    if (b.bytes >= bytes) return HY_OK;
+    hy::dev_free(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        b.p = nullptr;
+        return HY_E_OUT_OF_MEMORY;
+    }
+    b.bytes = bytes;
+    return HY_OK;
+}
+
+int grow_stage(hy_pair_scratch* s, size_t bytes) {
+    bytes = std::max<size_t>(bytes, 64);
+    if (s->stage_bytes >= bytes) return HY_OK;
+    if (s->stage) (void)hipHostFree(s->stage);
+    s->stage = nullptr;
+    s->stage_bytes = 0;
+    if (hipHostMalloc((void**)&s->stage, bytes) != hipSuccess) {
+        s->stage = nullptr;
+        return HY_E_OUT_OF_MEMORY;
+    }
+    s->stage_bytes = bytes;
+    return HY_OK;
+}
+
+// Sequential layout of one call's host-to-device data inside the pinned stage.
+struct Stager {
+    hy_pair_scratch* s;
+    size_t used = 0;
+    struct Piece { size_t off, bytes; DevBuf* dst; };
+    std::vector<Piece> pieces;
+    // Reserve `bytes` for `dst` (16-byte aligned) and return the host pointer to fill later.
+    size_t add(DevBuf* dst, size_t bytes) {
+        const size_t off = (used + 15) & ~size_t(15);
+        pieces.push_back({off, bytes, dst});
+        used = off + bytes;
+        return off;
+    }
+    uint8_t* at(size_t off) { return s->stage + off; }
+    int commit() {
+        for (const Piece& p : pieces) {
+            int rc = grow(*p.dst, p.bytes);
+            if (rc != HY_OK) return rc;
+        }
+        for (const Piece& p : pieces) {
+            if (p.bytes && hipMemcpyAsync(p.dst->p, s->stage + p.off, p.bytes,
+                                          hipMemcpyHostToDevice, s->stream) != hipSuccess)
+                return HY_E_DEVICE;
+        }
+        return HY_OK;
+    }
+};
+
+struct Scheme {
+    int32_t min_e = 0, max_e = 0;
+};
+
+int check_scheme(const int32_t* matrix, size_t matrix_len, uint32_t al, int32_t go, int32_t ge,
+                 Scheme* sc, int64_t* err_detail) {
+    return hy::validate_scoring(matrix, matrix_len, al, go, ge, &sc->min_e, &sc->max_e,
+                                err_detail);
+}
+
+// The per-pair checks of align_pair_with (src/kernel.rs:362-374): symbols (query, then target),
+// then the width proof for the pair's lengths.
+int check_pair(const uint8_t* q, size_t m, const uint8_t* t, size_t n, uint32_t al, uint32_t mode,
+               int32_t go, int32_t ge, const Scheme& sc, int64_t* err_detail) {
+    if ((m && !q) || (n && !t)) return HY_E_NULL_ARGUMENT;
+    for (size_t k = 0; k < m; ++k)
+        if ((uint32_t)q[k] >= al) {
+            if (err_detail) *err_detail = q[k];
+            return HY_E_SYMBOL_OUT_OF_RANGE;
+        }
+    for (size_t k = 0; k < n; ++k)
+        if ((uint32_t)t[k] >= al) {
+            if (err_detail) *err_detail = t[k];
+            return HY_E_SYMBOL_OUT_OF_RANGE;
+        }
+    int64_t bound = 0;
+    if (hy::width_bits_for(hy::magnitude_bound(mode, sc.min_e, sc.max_e, go, ge, m, n), &bound) ==
+        0) {
+        if (err_detail) *err_detail = bound;
+        return HY_E_SCORE_RANGE_TOO_WIDE;
+    }
+    if (m > 0x7FFFFFF0ull || n > 0x7FFFFFF0ull) return HY_E_INVALID_ARGUMENT;
+    return HY_OK;
+}
+
+// The matrix as the 32-bit kernels take it: entries clamped to +-kEntryMax32.
+void stage_matrix(Stager& st, hy_pair_scratch* s, const int32_t* matrix, size_t len) {
+    const size_t off = st.add(&s->mat, len * sizeof(int32_t));
+    int32_t* d = reinterpret_cast<int32_t*>(st.at(off));
+    for (size_t k = 0; k < len; ++k)
+        d[k] = std::max(-hy::kEntryMax32, std::min(hy::kEntryMax32, matrix[k]));
+}
+
+bool tracks_end(uint32_t search_type) { return search_type != HY_SEARCH_SCORE; }
+
+int lane_choice() {
+    const char* v = std::getenv("HYALITE_HIP_PAIRS_KERNEL");
+    if (!v) return 0;
+    if (std::strcmp(v, "lanes") == 0) return 1;
+    if (std::strcmp(v, "waves") == 0) return 2;
+    return 0;
+}
+
+// One wavefront per pair for the pairs in `idx` (k_sys32 pair mode): one launch per rows-per-
+// lane class (query up to 128, 256, 384 rows, and longer), the longest class in chunks whose
+// boundary rows fit kSysSpillBudget.
+int enqueue_waves(hy_pair_scratch* s, const std::vector<uint32_t>& idx,
+                  const std::vector<size_t>& qlens, const std::vector<size_t>& tlens,
+                  const uint32_t* d_list, uint32_t al, int32_t go, int32_t ge, uint32_t mode,
+                  bool ends, hipStream_t stream) {
+    // idx is ordered by class; find the class boundaries.
+    auto cls = [&](uint32_t k) { const size_t m = qlens[k];
+                                 return m <= 128 ? 0 : (m <= 256 ? 1 : (m <= 384 ? 2 : 3)); };
+    size_t b = 0;
+    while (b < idx.size()) {
+        size_t e = b;
+        const int c = cls(idx[b]);
+        size_t maxm = 0, maxn = 0;
+        while (e < idx.size() && cls(idx[e]) == c) {
+            maxm = std::max(maxm, qlens[idx[e]]);
+            maxn = std::max(maxn, tlens[idx[e]]);
+            ++e;
+        }
+        const bool multipass = maxm > 512;
+        const size_t per = multipass ? (maxn + 1) * sizeof(int2) : 0;
+        const size_t chunk = multipass ? std::max<size_t>(1, kSysSpillBudget / per) : (e - b);
+        if (multipass) {
+            const int rc = grow(s->sys_spill, std::min(chunk, e - b) * per);
+            if (rc != HY_OK) return rc;
+        }
+        for (size_t c0 = b; c0 < e; c0 += chunk) {
+            const size_t cnt = std::min(chunk, e - c0);
+            hy::SysArgs a{};
+            a.flat = static_cast<const uint8_t*>(s->tflat.p);
+            a.off = static_cast<const uint64_t*>(s->toff.p);
+            a.list = d_list + c0;
+            a.m = (int)maxm;
+            a.matrix = static_cast<const int32_t*>(s->mat.p);
+            a.al = (int)al;
+            a.al_eff = (int)std::min<uint32_t>(al, 256);
+            a.go = go;
+            a.ge = ge;
+            a.spill = static_cast<int2*>(s->sys_spill.p);
+            a.spill_stride = maxn + 1;
+            a.out_score = static_cast<int32_t*>(s->scores.p);
+            a.out_qend = static_cast<int64_t*>(s->qend.p);
+            a.out_tend = static_cast<int64_t*>(s->tend.p);
+            a.queries = static_cast<const uint8_t*>(s->qflat.p);
+            a.qoff = static_cast<const uint32_t*>(s->qoff32.p);
+            a.qlen = static_cast<const uint32_t*>(s->qlen32.p);
+            a.nq = static_cast<const uint32_t*>(s->nq.p);
+            a.qsel = -1;
+            a.pairs = 1;
+            if (hy::launch_sys32((int)mode, ends, false, a, (uint32_t)cnt, stream) != hipSuccess)
+                return HY_E_DEVICE;
+        }
+        b = e;
+    }
+    return HY_OK;
+}
+
+}  // namespace
+
+extern "C" int hy_pair_scratch_new(hy_pair_scratch** out) {
+    if (!out) return HY_E_NULL_ARGUMENT;
+    *out = nullptr;
+    const int device = hy::first_gfx950_device();
+    if (device < 0) return HY_E_BACKEND_UNAVAILABLE;
+    if (hipSetDevice(device) != hipSuccess) return HY_E_DEVICE;
+    hy_pair_scratch* s = new (std::nothrow) hy_pair_scratch();
+    if (!s) return HY_E_OUT_OF_MEMORY;
+    s->device = device;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s->join, hipEventDisableTiming) != hipSuccess) {
+        delete s;
+        return HY_E_DEVICE;
+    }
+    *out = s;
+    return HY_OK;
+}
+
+extern "C" void hy_pair_scratch_free(hy_pair_scratch* s) { delete s; }
+
+extern "C" int hy_align_pair(hy_pair_scratch* s, const uint8_t* query, size_t qlen,
+                             const uint8_t* target, size_t tlen, const int32_t* matrix,
+                             size_t matrix_len, uint32_t alphabet_len, int32_t gap_open,
+                             int32_t gap_ext, uint32_t mode, uint32_t search_type, int32_t* score,
+                             int64_t* query_end, int64_t* target_end, int64_t* err_detail) {
+    if (!s || !score || !query_end || !target_end) return HY_E_NULL_ARGUMENT;
+    if (err_detail) *err_detail = 0;
+    if (mode > HY_MODE_SHW || search_type > HY_SEARCH_ALIGNMENT) return HY_E_INVALID_ARGUMENT;
+    Scheme sc;
+    int rc = check_scheme(matrix, matrix_len, alphabet_len, gap_open, gap_ext, &sc, err_detail);
+    if (rc != HY_OK) return rc;
+    rc = check_pair(query, qlen, target, tlen, alphabet_len, mode, gap_open, gap_ext, sc,
+                    err_detail);
+    if (rc != HY_OK) return rc;
+    if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+
+    Stager st{s};
+    const size_t o_q = st.add(&s->qflat, qlen);
+    const size_t o_t = st.add(&s->tflat, tlen);
+    const size_t o_off = st.add(&s->toff, 2 * sizeof(uint64_t));
+    const size_t o_list = st.add(&s->list, sizeof(uint32_t));
+    const size_t o_qoff = st.add(&s->qoff32, sizeof(uint32_t));
+    const size_t o_qlen = st.add(&s->qlen32, sizeof(uint32_t));
+    const size_t o_nq = st.add(&s->nq, sizeof(uint32_t));
+    const size_t mat_at = st.used;
+    (void)mat_at;
+    if ((rc = grow_stage(s, st.used + matrix_len * sizeof(int32_t) + 64)) != HY_OK) return rc;
+    stage_matrix(st, s, matrix, matrix_len);
+    if (qlen) std::memcpy(st.at(o_q), query, qlen);
+    if (tlen) std::memcpy(st.at(o_t), target, tlen);
+    const uint64_t off2[2] = {0, (uint64_t)tlen};
+    std::memcpy(st.at(o_off), off2, sizeof(off2));
+    const uint32_t zero = 0, one = 1, m32 = (uint32_t)qlen;
+    std::memcpy(st.at(o_list), &zero, 4);
+    std::memcpy(st.at(o_qoff), &zero, 4);
+    std::memcpy(st.at(o_qlen), &m32, 4);
+    std::memcpy(st.at(o_nq), &one, 4);
+    if ((rc = st.commit()) != HY_OK) return rc;
+    if ((rc = grow(s->scores, sizeof(int32_t))) != HY_OK ||
+        (rc = grow(s->qend, sizeof(int64_t))) != HY_OK ||
+        (rc = grow(s->tend, sizeof(int64_t))) != HY_OK)
+        return rc;
+    const bool ends = tracks_end(search_type);
+    std::vector<uint32_t> idx{0};
+    std::vector<size_t> ql{qlen}, tl{tlen};
+    if ((rc = enqueue_waves(s, idx, ql, tl, static_cast<const uint32_t*>(s->list.p),
+                            alphabet_len, gap_open, gap_ext, mode, ends, s->stream)) != HY_OK)
+        return rc;
+    // Results come back through the stage (its inputs are already on the device in stream order).
+    int32_t* h_s = reinterpret_cast<int32_t*>(s->stage);
+    int64_t* h_e = reinterpret_cast<int64_t*>(s->stage + 16);
+    if (hipMemcpyAsync(h_s, s->scores.p, sizeof(int32_t), hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(h_e, s->qend.p, sizeof(int64_t), hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess ||
+        hipMemcpyAsync(h_e + 1, s->tend.p, sizeof(int64_t), hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess)
+        return HY_E_DEVICE;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    *score = *h_s;
+    *query_end = ends ? h_e[0] : -1;
+    *target_end = ends ? h_e[1] : -1;
+    return HY_OK;
+}
+
+extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
+                              const size_t* qlens, const uint8_t* const* targets,
+                              const size_t* tlens, size_t npairs, const int32_t* matrix,
+                              size_t matrix_len, uint32_t alphabet_len, int32_t gap_open,
+                              int32_t gap_ext, uint32_t mode, uint32_t search_type,
+                              int32_t* scores, int64_t* query_ends, int64_t* target_ends,
+                              int64_t* err_detail, uint64_t* err_pair) {
+    if (!s || (npairs && (!queries || !qlens || !targets || !tlens || !scores)))
+        return HY_E_NULL_ARGUMENT;
+    if (err_detail) *err_detail = 0;
+    if (err_pair) *err_pair = 0;
+    if (mode > HY_MODE_SHW || search_type > HY_SEARCH_ALIGNMENT) return HY_E_INVALID_ARGUMENT;
+    Scheme sc;
+    int rc = check_scheme(matrix, matrix_len, alphabet_len, gap_open, gap_ext, &sc, err_detail);
+    if (rc != HY_OK) return rc;
+    uint64_t qtot = 0, ttot = 0;
+    for (size_t k = 0; k < npairs; ++k) {
+        rc = check_pair(queries[k], qlens[k], targets[k], tlens[k], alphabet_len, mode, gap_open,
+                        gap_ext, sc, err_detail);
+        if (rc != HY_OK) {
+            if (err_pair) *err_pair = k;
+            return rc;
+        }
+        qtot += qlens[k];
+        ttot += tlens[k];
+    }
+    if (npairs == 0) return HY_OK;
+    if (npairs > 0xFFFFFFF0ull || qtot > 0xFFFFFFF0ull) return HY_E_INVALID_ARGUMENT;
+    if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+    const bool ends = tracks_end(search_type);
+    const uint32_t al_eff = std::min<uint32_t>(alphabet_len, 256);
+
+    // Split: one pair per lane for the short pairs of a large batch, one per wavefront else.
+    std::vector<size_t> ql(qlens, qlens + npairs), tl(tlens, tlens + npairs);
+    const int force = lane_choice();
+    std::vector<uint32_t> lanes, waves;
+    size_t nshort = 0;
+    for (size_t k = 0; k < npairs; ++k)
+        nshort += ql[k] <= kLaneMaxQuery && tl[k] <= kLaneMaxTarget;
+    const bool use_lanes =
+        al_eff <= 255 && (force == 1 || (force == 0 && nshort >= kLaneMinPairs));
+    for (uint32_t k = 0; k < (uint32_t)npairs; ++k) {
+        const bool shortp = ql[k] <= kLaneMaxQuery && tl[k] <= kLaneMaxTarget;
+        if (use_lanes && (shortp || force == 1)) lanes.push_back(k);
+        else waves.push_back(k);
+    }
+    // Lanes: sorted by (query length, target length) descending, cut into 64-lane groups.
+    std::stable_sort(lanes.begin(), lanes.end(), [&](uint32_t a, uint32_t b) {
+        return ql[a] != ql[b] ? ql[a] > ql[b] : tl[a] > tl[b];
+    });
+    // Waves: ordered by rows-per-lane class for one launch per class.
+    std::stable_sort(waves.begin(), waves.end(), [&](uint32_t a, uint32_t b) {
+        auto c = [&](uint32_t k) { return ql[k] <= 128 ? 0 : (ql[k] <= 256 ? 1 : (ql[k] <= 384 ? 2 : 3)); };
+        return c(a) < c(b);
+    });
+    int qtile = 32;
+    if (!lanes.empty() && ql[lanes.front()] <= 16) qtile = 16;
+    std::vector<hy::PairGroup> groups;
+    uint64_t twords = 0, qwords = 0, spill = 0;
+    for (size_t g0 = 0; g0 < lanes.size(); g0 += 64) {
+        hy::PairGroup g{};
+        g.first = (uint32_t)g0;
+        g.count = (uint32_t)std::min<size_t>(64, lanes.size() - g0);
+        for (uint32_t l = 0; l < g.count; ++l) {
+            g.ncols = std::max<uint32_t>(g.ncols, (uint32_t)tl[lanes[g0 + l]]);
+            g.nrows = std::max<uint32_t>(g.nrows, (uint32_t)ql[lanes[g0 + l]]);
+        }
+        const uint64_t tiles = (g.nrows + qtile - 1) / qtile;
+        g.tword_off = twords;
+        twords += (uint64_t)((g.ncols + 3) / 4) * 64;
+        g.qword_off = qwords;
+        qwords += tiles * (uint64_t)(qtile / 4) * 64;
+        g.spill_off = tiles > 1 ? spill : 0;
+        if (tiles > 1) spill += ((uint64_t)g.ncols + 1) * 64;
+        groups.push_back(g);
+    }
+
+    Stager st{s};
+    const size_t o_q = st.add(&s->qflat, qtot);
+    const size_t o_t = st.add(&s->tflat, ttot);
+    const size_t o_qoff = st.add(&s->qoff, (npairs + 1) * sizeof(uint64_t));
+    const size_t o_toff = st.add(&s->toff, (npairs + 1) * sizeof(uint64_t));
+    size_t o_q32 = 0, o_l32 = 0, o_list = 0, o_nq = 0, o_groups = 0, o_tl = 0, o_ql = 0, o_pi = 0;
+    if (!waves.empty()) {
+        o_q32 = st.add(&s->qoff32, npairs * sizeof(uint32_t));
+        o_l32 = st.add(&s->qlen32, npairs * sizeof(uint32_t));
+        o_list = st.add(&s->list, waves.size() * sizeof(uint32_t));
+        o_nq = st.add(&s->nq, sizeof(uint32_t));
+    }
+    if (!lanes.empty()) {
+        o_groups = st.add(&s->groups, groups.size() * sizeof(hy::PairGroup));
+        o_tl = st.add(&s->tlens, lanes.size() * sizeof(uint32_t));
+        o_ql = st.add(&s->qlens, lanes.size() * sizeof(uint32_t));
+        o_pi = st.add(&s->pidx, lanes.size() * sizeof(uint32_t));
+    }
+    if ((rc = grow_stage(s, st.used + matrix_len * sizeof(int32_t) + 64)) != HY_OK) return rc;
+    stage_matrix(st, s, matrix, matrix_len);
+    {
+        uint8_t* qd = st.at(o_q);
+        uint8_t* td = st.at(o_t);
+        uint64_t* qo = reinterpret_cast<uint64_t*>(st.at(o_qoff));
+        uint64_t* to = reinterpret_cast<uint64_t*>(st.at(o_toff));
+        uint64_t qa = 0, ta = 0;
+        for (size_t k = 0; k < npairs; ++k) {
+            qo[k] = qa;
+            to[k] = ta;
+            if (ql[k]) std::memcpy(qd + qa, queries[k], ql[k]);
+            if (tl[k]) std::memcpy(td + ta, targets[k], tl[k]);
+            qa += ql[k];
+            ta += tl[k];
+        }
+        qo[npairs] = qa;
+        to[npairs] = ta;
+        if (!waves.empty()) {
+            uint32_t* q32 = reinterpret_cast<uint32_t*>(st.at(o_q32));
+            uint32_t* l32 = reinterpret_cast<uint32_t*>(st.at(o_l32));
+            for (size_t k = 0; k < npairs; ++k) {
+                q32[k] = (uint32_t)qo[k];
+                l32[k] = (uint32_t)ql[k];
+            }
+            std::memcpy(st.at(o_list), waves.data(), waves.size() * sizeof(uint32_t));
+            const uint32_t np = (uint32_t)npairs;
+            std::memcpy(st.at(o_nq), &np, sizeof(np));
+        }
+        if (!lanes.empty()) {
+            std::memcpy(st.at(o_groups), groups.data(), groups.size() * sizeof(hy::PairGroup));
+            uint32_t* tls = reinterpret_cast<uint32_t*>(st.at(o_tl));
+            uint32_t* qls = reinterpret_cast<uint32_t*>(st.at(o_ql));
+            for (size_t k = 0; k < lanes.size(); ++k) {
+                tls[k] = (uint32_t)tl[lanes[k]];
+                qls[k] = (uint32_t)ql[lanes[k]];
+            }
+            std::memcpy(st.at(o_pi), lanes.data(), lanes.size() * sizeof(uint32_t));
+        }
+    }
+    if ((rc = st.commit()) != HY_OK) return rc;
+    if ((rc = grow(s->scores, npairs * sizeof(int32_t))) != HY_OK ||
+        (rc = grow(s->qend, npairs * sizeof(int64_t))) != HY_OK ||
+        (rc = grow(s->tend, npairs * sizeof(int64_t))) != HY_OK)
+        return rc;
+
+    // Long pairs and small batches: the aux stream, forked after the uploads.
+    const bool both = !waves.empty() && !lanes.empty();
+    hipStream_t wstream = both ? s->aux : s->stream;
+    if (both && (hipEventRecord(s->fork, s->stream) != hipSuccess ||
+                 hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess))
+        return HY_E_DEVICE;
+    if (!waves.empty() &&
+        (rc = enqueue_waves(s, waves, ql, tl, static_cast<const uint32_t*>(s->list.p),
+                            alphabet_len, gap_open, gap_ext, mode, ends, wstream)) != HY_OK)
+        return rc;
+    if (!lanes.empty()) {
+        if ((rc = grow(s->twords, twords * 4)) != HY_OK ||
+            (rc = grow(s->qwords, qwords * 4)) != HY_OK ||
+            (rc = grow(s->spill, std::max<uint64_t>(spill, 1) * sizeof(int2))) != HY_OK)
+            return rc;
+        hy::PairPackArgs pk{};
+        pk.groups = static_cast<const hy::PairGroup*>(s->groups.p);
+        pk.qflat = static_cast<const uint8_t*>(s->qflat.p);
+        pk.qoff = static_cast<const uint64_t*>(s->qoff.p);
+        pk.tflat = static_cast<const uint8_t*>(s->tflat.p);
+        pk.toff = static_cast<const uint64_t*>(s->toff.p);
+        pk.pidx = static_cast<const uint32_t*>(s->pidx.p);
+        pk.qlens = static_cast<const uint32_t*>(s->qlens.p);
+        pk.tlens = static_cast<const uint32_t*>(s->tlens.p);
+        pk.qwords = static_cast<uint32_t*>(s->qwords.p);
+        pk.twords = static_cast<uint32_t*>(s->twords.p);
+        pk.pad_symbol = al_eff;
+        pk.qtile = qtile;
+        if (hy::launch_pair_pack(pk, (uint32_t)groups.size(), s->stream) != hipSuccess)
+            return HY_E_DEVICE;
+        hy::PairArgs pa{};
+        pa.groups = pk.groups;
+        pa.twords = pk.twords;
+        pa.qwords = pk.qwords;
+        pa.tlens = pk.tlens;
+        pa.qlens = pk.qlens;
+        pa.pidx = pk.pidx;
+        pa.matrix = static_cast<const int32_t*>(s->mat.p);
+        pa.al = (int)alphabet_len;
+        pa.al_eff = (int)al_eff;
+        pa.go = gap_open;
+        pa.ge = gap_ext;
+        pa.spill = static_cast<int2*>(s->spill.p);
+        pa.out_score = static_cast<int32_t*>(s->scores.p);
+        pa.out_qend = static_cast<int64_t*>(s->qend.p);
+        pa.out_tend = static_cast<int64_t*>(s->tend.p);
+        if (hy::launch_pairs32((int)mode, ends, pa, (uint32_t)groups.size(), qtile, s->stream) !=
+            hipSuccess)
+            return HY_E_DEVICE;
+    }
+    if (both && (hipEventRecord(s->join, s->aux) != hipSuccess ||
+                 hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess))
+        return HY_E_DEVICE;
+    if (hipMemcpyAsync(scores, s->scores.p, npairs * sizeof(int32_t), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    if (ends) {
+        if (query_ends && hipMemcpyAsync(query_ends, s->qend.p, npairs * sizeof(int64_t),
+                                         hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+            return HY_E_DEVICE;
+        if (target_ends && hipMemcpyAsync(target_ends, s->tend.p, npairs * sizeof(int64_t),
+                                          hipMemcpyDeviceToHost, s->stream) != hipSuccess)
+            return HY_E_DEVICE;
+    }
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    if (!ends) {
+        for (size_t k = 0; k < npairs; ++k) {
+            if (query_ends) query_ends[k] = -1;
+            if (target_ends) target_ends[k] = -1;
+        }
+    }
+    return HY_OK;
+}
+
+extern "C" int hy_align_pair_position_max(hy_pair_scratch* s, const uint8_t* query, size_t qlen,
+                                          const uint8_t* target, size_t tlen,
+                                          const int32_t* matrix, size_t matrix_len,
+                                          uint32_t alphabet_len, int32_t gap_open,
+                                          int32_t gap_ext, int32_t* colmax, int32_t* score,
+                                          int64_t* target_end, int64_t* err_detail) {
+    if (!s || !score || !target_end || (tlen && !colmax)) return HY_E_NULL_ARGUMENT;
+    if (err_detail) *err_detail = 0;
+    Scheme sc;
+    int rc = check_scheme(matrix, matrix_len, alphabet_len, gap_open, gap_ext, &sc, err_detail);
+    if (rc != HY_OK) return rc;
+    rc = check_pair(query, qlen, target, tlen, alphabet_len, HY_MODE_SW, gap_open, gap_ext, sc,
+                    err_detail);
+    if (rc != HY_OK) return rc;
+    if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+    Stager st{s};
+    const size_t o_q = st.add(&s->qflat, qlen);
+    const size_t o_t = st.add(&s->tflat, tlen);
+    if ((rc = grow_stage(s, st.used + matrix_len * sizeof(int32_t) + 64)) != HY_OK) return rc;
+    stage_matrix(st, s, matrix, matrix_len);
+    if (qlen) std::memcpy(st.at(o_q), query, qlen);
+    if (tlen) std::memcpy(st.at(o_t), target, tlen);
+    if ((rc = st.commit()) != HY_OK) return rc;
+    if ((rc = grow(s->colmax, tlen * sizeof(int32_t))) != HY_OK ||
+        (rc = grow(s->best, sizeof(int2))) != HY_OK ||
+        (rc = grow(s->sys_spill, qlen > 512 ? (tlen + 1) * sizeof(int2) : 16)) != HY_OK)
+        return rc;
+    hy::PosMaxArgs pa{};
+    pa.query = static_cast<const uint8_t*>(s->qflat.p);
+    pa.m = (int)qlen;
+    pa.target = static_cast<const uint8_t*>(s->tflat.p);
+    pa.n = (int)tlen;
+    pa.matrix = static_cast<const int32_t*>(s->mat.p);
+    pa.al = (int)alphabet_len;
+    pa.al_eff = (int)std::min<uint32_t>(alphabet_len, 256);
+    pa.go = gap_open;
+    pa.ge = gap_ext;
+    pa.spill = static_cast<int2*>(s->sys_spill.p);
+    pa.colmax = static_cast<int32_t*>(s->colmax.p);
+    pa.best = static_cast<int2*>(s->best.p);
+    if (hy::launch_posmax(pa, s->stream) != hipSuccess) return HY_E_DEVICE;
+    int2* h_best = reinterpret_cast<int2*>(s->stage);
+    if ((tlen && hipMemcpyAsync(colmax, s->colmax.p, tlen * sizeof(int32_t),
+                                hipMemcpyDeviceToHost, s->stream) != hipSuccess) ||
+        hipMemcpyAsync(h_best, s->best.p, sizeof(int2), hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess)
+        return HY_E_DEVICE;
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    *score = h_best->x;
+    *target_end = h_best->y;
+    return HY_OK;
+}
+
+// score2 (src/kernel.rs:741-792): w = ceil(best / matrix_max); columns at or above min_score
+// that continue the open peak's maximum position extend it (keeping the higher score), any other
+// such column closes it and opens a new one; a closed peak outside [te - w, te + w] becomes the
+// runner-up only when strictly higher, so ties keep the smallest position.
+extern "C" int hy_score2(const int32_t* colmax, size_t n, int32_t best_score,
+                         uint64_t best_target_end, int32_t matrix_max, int32_t min_score,
+                         int32_t* found, int32_t* score2, uint64_t* position) {
+    if (!found || !score2 || !position || (n && !colmax)) return HY_E_NULL_ARGUMENT;
+    *found = 0;
+    if (matrix_max <= 0) return HY_OK;
+    const int64_t w = std::max<int64_t>(
+        ((int64_t)std::max(best_score, 0) + (int64_t)matrix_max - 1) / (int64_t)matrix_max, 0);
+    const int64_t low = (int64_t)best_target_end - w, high = (int64_t)best_target_end + w;
+    bool have = false, open = false;
+    int32_t rs = 0, ps = 0;
+    uint64_t rp = 0, pp = 0;
+    auto close = [&]() {
+        const int64_t p = (int64_t)pp;
+        if ((p < low || p > high) && (!have || ps > rs)) {
+            have = true;
+            rs = ps;
+            rp = pp;
+        }
+    };
+    for (size_t i = 0; i < n; ++i) {
+        const int32_t v = colmax[i];
+        if (v < min_score) continue;
+        if (open && pp + 1 == i) {
+            if (ps < v) { ps = v; pp = i; }
+        } else {
+            if (open) close();
+            open = true;
+            ps = v;
+            pp = i;
+        }
+    }
+    if (open) close();
+    if (have) {
+        *found = 1;
+        *score2 = rs;
+        *position = rp;
+    }
+    return HY_OK;
+}

@@ -187,4 +187,15 @@ int stage_query(const hy_db* db, hy_scratch* s, const uint8_t* q, size_t qlen);
 int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write_scores,
                     unsigned long long* key);
 void collect_profile(hy_scratch* s);
+// The width proof (src/width.rs:107-169): magnitude bound of a (mode, scheme, lengths) problem
+// and the narrowest width holding it (8, 16, 32; 0 when not even i32 suffices).
+__int128 magnitude_bound(uint32_t mode, int32_t min_e, int32_t max_e, int32_t go, int32_t ge,
+                         uint64_t mql, uint64_t mtl);
+int width_bits_for(__int128 bound, int64_t* bound_out);
+// The first gfx950 device, or -1.
+int first_gfx950_device();
+// Scoring::new's checks (src/scoring.rs:65-100) and the matrix's entry bounds.
+int validate_scoring(const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len,
+                     int32_t gap_open, int32_t gap_ext, int32_t* min_e, int32_t* max_e,
+                     int64_t* err_detail);
 }  // namespace hy

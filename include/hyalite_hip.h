@@ -17,6 +17,11 @@
  *   hy_scan_aligned  <- Database::scan_aligned            src/database.rs:528-540,
  *                                                          src/align.rs:192-226, 487-545
  *   hy_scan_all_aligned <- Database::scan_all_aligned     src/database.rs:551-580
+ *   hy_pair_scratch_new <- PairScratch::new               src/kernel.rs:285-311
+ *   hy_align_pair    <- align_pair / align_pair_with      src/kernel.rs:331-420
+ *   hy_align_pairs   <- align_pairs                       src/kernel.rs:429-452
+ *   hy_align_pair_position_max <- align_pair_position_max(_with) src/kernel.rs:474-575
+ *   hy_score2        <- score2                            src/kernel.rs:741-792
  *   hy_scan_batch    <- a loop of Database::scan over many queries (batched form; no single
  *                       reference counterpart — each hit equals Database::scan of that query)
  *   hy_backend_available <- Backend::is_available         src/backend.rs:77-84
@@ -45,6 +50,7 @@ extern "C" {
 
 typedef struct hy_db hy_db;
 typedef struct hy_scratch hy_scratch;
+typedef struct hy_pair_scratch hy_pair_scratch;
 
 /* Mode (src/mode.rs:11-27). */
 enum { HY_MODE_SW = 0, HY_MODE_NW = 1, HY_MODE_HW = 2, HY_MODE_OV = 3, HY_MODE_SHW = 4 };
@@ -171,6 +177,51 @@ int hy_db_aligned_ops_bytes(const hy_db* db, size_t qlen, uint64_t* bytes);
  * HYALITE_HIP_TRACE_POOL_BYTES when set). */
 int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
                         hy_alignment* out, uint8_t* ops, size_t ops_cap);
+
+/* ---- Pairwise alignment (free functions of src/kernel.rs, no database) ------------------------
+ * Each call takes the scoring scheme as Scoring::new does and validates it in the same order
+ * (HY_E_EMPTY_ALPHABET .. HY_E_GAP_OPEN_LESS_THAN_EXTEND), then every pair's symbols (query
+ * first, then target: HY_E_SYMBOL_OUT_OF_RANGE, err_detail = the symbol) and its width proof
+ * (HY_E_SCORE_RANGE_TOO_WIDE, err_detail = the bound). Results equal the reference's scalar
+ * align_core (score, then the (target_end, query_end) tie-break; ends -1 for SearchType::Score).
+ * A hy_pair_scratch holds a HIP stream and device buffers that grow to the largest problem seen;
+ * one per thread, like PairScratch. */
+int hy_pair_scratch_new(hy_pair_scratch** out);
+void hy_pair_scratch_free(hy_pair_scratch* s);
+
+/* One pair (align_pair_with): one wavefront aligns it in the systolic kernel. */
+int hy_align_pair(hy_pair_scratch* s, const uint8_t* query, size_t qlen, const uint8_t* target,
+                  size_t tlen, const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len,
+                  int32_t gap_open, int32_t gap_ext, uint32_t mode, uint32_t search_type,
+                  int32_t* score, int64_t* query_end, int64_t* target_end, int64_t* err_detail);
+
+/* A batch of independent pairs (align_pairs): hit k is align_pair of pair k (db_index k). Large
+ * batches run one pair per lane (pairs sorted by length into 64-lane groups, a substitution
+ * lookup per cell from LDS); small batches and long pairs run one pair per wavefront.
+ * HYALITE_HIP_PAIRS_KERNEL=lanes|waves forces one of the two for every pair. On an error the
+ * failing pair's index goes to *err_pair (may be NULL) and no hit is written. */
+int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries, const size_t* qlens,
+                   const uint8_t* const* targets, const size_t* tlens, size_t npairs,
+                   const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len,
+                   int32_t gap_open, int32_t gap_ext, uint32_t mode, uint32_t search_type,
+                   int32_t* scores, int64_t* query_ends, int64_t* target_ends,
+                   int64_t* err_detail, uint64_t* err_pair);
+
+/* Local-mode per-target-position maxima (align_pair_position_max_with): colmax[t] = the best
+ * local score ending at target position t (tlen entries); *score = their maximum, *target_end =
+ * its first position or -1 when the best is 0 (query end untracked). */
+int hy_align_pair_position_max(hy_pair_scratch* s, const uint8_t* query, size_t qlen,
+                               const uint8_t* target, size_t tlen, const int32_t* matrix,
+                               size_t matrix_len, uint32_t alphabet_len, int32_t gap_open,
+                               int32_t gap_ext, int32_t* colmax, int32_t* score,
+                               int64_t* target_end, int64_t* err_detail);
+
+/* bwa-mem's second-best score over a column-maxima array (score2): *found = 1 with
+ * (*score2, *position) for the best secondary peak outside the primary's exclusion window,
+ * *found = 0 for None. Host arithmetic on the caller's array. */
+int hy_score2(const int32_t* colmax, size_t n, int32_t best_score, uint64_t best_target_end,
+              int32_t matrix_max, int32_t min_score, int32_t* found, int32_t* score2,
+              uint64_t* position);
 
 /* Batched form: nq queries in one submission; hit k equals hy_scan of query k. The queries run
  * in one persistent kernel over a (target group, query) work queue — in local mode two queries

@@ -1,0 +1,299 @@
+"""GPU parity of the pairwise functions: align_pair, align_pairs, align_pair_position_max and
+score2 must equal the pinned oracle bit for bit.
+
+Mirrors the reference's unit tests of src/kernel.rs (position-max hand cases and proptest,
+align_pairs against per-pair align_pair across modes, search types and widths with empty
+geometries, scratch reuse, error propagation, score2 cases), then widens to every short pair in
+every mode, random schemes, protein and DNA batches large enough for the one-pair-per-lane kernel,
+long and multi-pass pairs, both batch kernels forced in turn.
+"""
+import random
+
+import numpy as np
+import pytest
+
+import hyalite_amd as hy
+import oracle as orc
+from common import (ALL_MODES, BLOSUM62, MODE_SW, all_sequences, dna_scheme, identity_matrix)
+
+pytestmark = pytest.mark.gpu
+
+ST = (hy.SearchType.Score, hy.SearchType.ScoreEnd)
+
+
+def scoring(scheme):
+    al, mat, go, ge = scheme
+    return hy.Scoring(al, mat, go, ge)
+
+
+def want_hit(q, t, scheme, mode, st, index=0):
+    al, mat, go, ge = scheme
+    s, qe, te = orc.align_core(q, t, al, mat, go, ge, mode)
+    if st == hy.SearchType.Score:
+        qe = te = None
+    return hy.BestHit(s, index, qe, te)
+
+
+@pytest.fixture(params=["lanes", "waves"])
+def kernel(request, monkeypatch):
+    monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", request.param)
+    return request.param
+
+
+def check_pairs(pairs, scheme, mode, st, scratch=None):
+    got = hy.align_pairs(pairs, scoring(scheme), hy.Mode(mode), st, scratch=scratch)
+    assert len(got) == len(pairs)
+    for k, (q, t) in enumerate(pairs):
+        assert got[k] == want_hit(list(q), list(t), scheme, mode, st, k), \
+            (hy.Mode(mode), st, k, list(q)[:20], list(t)[:20])
+
+
+# ---- src/kernel.rs unit tests ---------------------------------------------------------------------
+
+def mk(seed, n):
+    return [((seed * (i + 1)) >> 3) % 4 for i in range(n)]
+
+
+def test_align_pairs_matches_per_pair_and_reuses_out(kernel):
+    """src/kernel.rs:1104-1153: lengths {0, 1, 4, 9, 20} crossed, three widths, every mode and
+    search type; a reused `out` is cleared; an out-of-range symbol in any pair surfaces."""
+    s_i8 = (4, identity_matrix(4, 2, -1), 2, 1)
+    s_i16 = (4, identity_matrix(4, 20, -5), 8, 2)
+    s_i32 = (4, identity_matrix(4, 3000, -1000), 500, 100)
+    lens = [0, 1, 4, 9, 20]
+    pairs = [(mk(0x1000 + a, ql), mk(0x2000 + b, tl))
+             for a, ql in enumerate(lens) for b, tl in enumerate(lens)]
+    sc = hy.PairScratch()
+    out = []
+    for scheme in (s_i8, s_i16, s_i32):
+        for mode in ALL_MODES:
+            for st in ST:
+                hy.align_pairs(pairs, scoring(scheme), hy.Mode(mode), st, out=out, scratch=sc)
+                assert len(out) == len(pairs)
+                for k, (q, t) in enumerate(pairs):
+                    assert out[k] == want_hit(q, t, scheme, mode, st, k), (mode, st, k)
+    hy.align_pairs(pairs[:3], scoring(s_i8), hy.Mode.Sw, hy.SearchType.Score, out=out,
+                   scratch=sc)
+    assert len(out) == 3
+    bad = [([0, 1], [0]), ([9], [0])]
+    with pytest.raises(hy.SymbolOutOfRange):
+        hy.align_pairs(bad, scoring(s_i8), hy.Mode.Nw, hy.SearchType.Score, out=out, scratch=sc)
+    assert hy.align_pairs([], scoring(s_i8), hy.Mode.Nw, hy.SearchType.Score) == []
+
+
+def test_pair_scratch_reuse_matches_one_shot():
+    """src/kernel.rs:1156-1204: grow-then-shrink lengths through one scratch."""
+    s_i8 = (4, identity_matrix(4, 2, -1), 2, 1)
+    s_i16 = (4, identity_matrix(4, 20, -5), 8, 2)
+    lens = [40, 3, 55, 1, 30, 0, 12, 48, 2]
+    sc = hy.PairScratch()
+    for k, ql in enumerate(lens):
+        tl = lens[(k + 3) % len(lens)]
+        for scheme in (s_i8, s_i16):
+            q, t = mk(0x9E3779B1 + k, ql), mk(0x12345678 + k, tl)
+            for mode in ALL_MODES:
+                for st in ST:
+                    got = hy.align_pair_with(sc, q, t, scoring(scheme), hy.Mode(mode), st)
+                    assert got == want_hit(q, t, scheme, mode, st), (mode, st, ql, tl)
+                    assert got == hy.align_pair(q, t, scoring(scheme), hy.Mode(mode), st)
+            out_r, out_o = [], []
+            a = hy.align_pair_position_max_with(sc, q, t, scoring(scheme), out_r)
+            b = hy.align_pair_position_max(q, t, scoring(scheme), out_o)
+            assert a == b and out_r == out_o
+            col, s, te = orc.position_max(q, t, *scheme)
+            assert out_r == col and (a.score, a.target_end, a.query_end) == (s, te, None)
+
+
+def test_position_max_hand_computed_cases():
+    """src/kernel.rs:1028-1059."""
+    s = scoring(dna_scheme())
+    out = []
+    hit = hy.align_pair_position_max([0, 1, 2, 3], [0, 1, 2, 3], s, out)
+    assert out == [2, 4, 6, 8] and hit.score == 8 and (hit.query_end, hit.target_end) == (None, 3)
+    hit = hy.align_pair_position_max([0], [1, 1, 0], s, out)
+    assert out == [0, 0, 2] and hit.score == 2 and hit.target_end == 2
+    hit = hy.align_pair_position_max([0, 0], [1, 1], s, out)
+    assert out == [0, 0] and hit.score == 0 and hit.target_end is None
+    hit = hy.align_pair_position_max([0, 1], [], s, out)
+    assert out == [] and hit.score == 0 and hit.target_end is None
+    hit = hy.align_pair_position_max([], [0, 1, 2], s, out)
+    assert out == [0, 0, 0] and hit.target_end is None
+
+
+def test_position_max_out_of_range_symbol_is_reported():
+    """src/kernel.rs:1062-1075: the error carries the symbol; `out` is cleared."""
+    out = [1, 2, 3]
+    with pytest.raises(hy.SymbolOutOfRange) as e:
+        hy.align_pair_position_max([0, 5], [0], scoring(dna_scheme()), out)
+    assert (e.value.symbol, e.value.alphabet_len) == (5, 4)
+    assert out == []
+
+
+def test_position_max_random_schemes():
+    """src/kernel.rs:1076-1102: array, score and target end equal the oracle's."""
+    rng = random.Random(3)
+    sc = hy.PairScratch()
+    for _ in range(150):
+        al = rng.randint(2, 5)
+        mat = [rng.randint(-6, 6) for _ in range(al * al)]
+        go = rng.randint(0, 7)
+        ge = rng.randint(0, go)
+        q = [rng.randrange(al) for _ in range(rng.randrange(0, 30))]
+        t = [rng.randrange(al) for _ in range(rng.randrange(0, 30))]
+        out = []
+        hit = hy.align_pair_position_max_with(sc, q, t, hy.Scoring(al, mat, go, ge), out)
+        col, s, te = orc.position_max(q, t, al, mat, go, ge)
+        assert (out, hit.score, hit.target_end) == (col, s, te), (q, t, al, mat, go, ge)
+
+
+@pytest.mark.parametrize("qlen,tlen", [(100, 3000), (300, 700), (600, 1200), (1100, 5000)])
+def test_position_max_protein_shapes(qlen, tlen):
+    """Every rows-per-lane shape and multi-pass queries (beyond 512 rows)."""
+    rng = np.random.default_rng(qlen)
+    q = rng.integers(0, 20, size=qlen, dtype=np.uint8)
+    t = np.concatenate([rng.integers(0, 20, size=tlen // 2, dtype=np.uint8), q[: qlen // 2],
+                        rng.integers(0, 20, size=tlen // 2, dtype=np.uint8)])
+    out = []
+    hit = hy.align_pair_position_max(q, t, hy.Scoring(20, BLOSUM62, 11, 1), out)
+    col, s, te = orc.position_max(q, t, 20, BLOSUM62, 11, 1)
+    assert out == col and (hit.score, hit.target_end) == (s, te)
+
+
+def test_score2_hand_computed_cases():
+    """src/kernel.rs:1245-1276."""
+    assert hy.score2([0, 5, 0, 0, 0, 0, 0, 0, 4, 0], 5, 1, 1, 1) == (4, 8)
+    assert hy.score2([0, 3, 5, 4, 0], 5, 2, 5, 1) is None
+    assert hy.score2([3, 5, 4, 6], 1, 20, 1, 1) == (6, 3)
+    assert hy.score2([7, 0, 7], 1, 10, 1, 1) == (7, 0)
+    assert hy.score2([0, 0, 0], 0, 0, 1, 1) is None
+    assert hy.score2([5, 5], 5, 0, 0, 1) is None
+    assert hy.score2([2, 0, 0, 0, 0, 0, 9], 9, 6, 9, 5) is None
+    assert hy.score2([2, 0, 0, 0, 0, 0, 9], 9, 6, 9, 1) == (2, 0)
+
+
+def test_score2_matches_the_oracle_on_random_arrays():
+    rng = random.Random(8)
+    for _ in range(2000):
+        colmax = [rng.randint(0, 20) for _ in range(rng.randint(0, 40))]
+        args = (rng.randint(0, 40), rng.randint(0, 40), rng.randint(-2, 8), rng.randint(1, 6))
+        assert hy.score2(colmax, *args) == orc.score2(colmax, *args)
+
+
+def test_score2_on_a_device_position_max():
+    """The mate-rescue flow: position maxima of a read against a window holding two copies."""
+    rng = np.random.default_rng(4)
+    read = rng.integers(0, 4, size=120, dtype=np.uint8)
+    win = np.concatenate([rng.integers(0, 4, size=300, dtype=np.uint8), read,
+                          rng.integers(0, 4, size=200, dtype=np.uint8), read[:100],
+                          rng.integers(0, 4, size=100, dtype=np.uint8)])
+    s = scoring(dna_scheme())
+    out = []
+    hit = hy.align_pair_position_max(read, win, s, out)
+    got = hy.score2(out, hit.score, hit.target_end, 2, 20)
+    col, bs, te = orc.position_max(read, win, *dna_scheme())
+    assert got == orc.score2(col, bs, te, 2, 20) and got is not None and got[0] >= 190
+
+
+# ---- widened parity -------------------------------------------------------------------------------
+
+SCHEMES = [
+    dna_scheme(),
+    (4, identity_matrix(4, 1, -3), 10, 5),
+    (4, identity_matrix(4, 3, -2), 4, 0),
+    (4, identity_matrix(4, 5, -1), 1, 1),
+]
+
+
+@pytest.mark.parametrize("scheme_ix", range(4))
+def test_every_short_pair_in_one_batch(scheme_ix, kernel):
+    scheme = SCHEMES[scheme_ix]
+    seqs = all_sequences(3, 3)
+    pairs = [(q, t) for q in seqs for t in seqs]
+    for mode in ALL_MODES:
+        for st in ST:
+            check_pairs(pairs, scheme, mode, st)
+
+
+def test_random_schemes_and_lengths(kernel):
+    rng = random.Random(21)
+    for _ in range(25):
+        al = rng.randint(2, 6)
+        mat = [rng.randint(-6, 6) for _ in range(al * al)]
+        go = rng.randint(0, 8)
+        ge = rng.randint(0, go)
+        pairs = [([rng.randrange(al) for _ in range(rng.randrange(0, 70))],
+                  [rng.randrange(al) for _ in range(rng.randrange(0, 90))]) for _ in range(150)]
+        for mode in ALL_MODES:
+            check_pairs(pairs, (al, mat, go, ge), mode, hy.SearchType.ScoreEnd)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_protein_pairs_across_tiles_and_long_pairs(mode, kernel):
+    """Queries past several 16/32-row tiles, targets of 0-2000 residues, related pairs (ties and
+    high scores), and pairs long enough to leave the lane kernel in the default split."""
+    rng = np.random.default_rng(40 + mode)
+    pairs = []
+    for _ in range(90):
+        q = rng.integers(0, 20, size=int(rng.integers(0, 400)), dtype=np.uint8)
+        if rng.random() < 0.5 and q.size:
+            t = np.concatenate([rng.integers(0, 20, size=int(rng.integers(0, 300)),
+                                             dtype=np.uint8), q,
+                                rng.integers(0, 20, size=int(rng.integers(0, 300)),
+                                             dtype=np.uint8)])
+        else:
+            t = rng.integers(0, 20, size=int(rng.integers(0, 2000)), dtype=np.uint8)
+        pairs.append((q, t))
+    pairs.append((rng.integers(0, 20, size=700, dtype=np.uint8),
+                  rng.integers(0, 20, size=3000, dtype=np.uint8)))
+    for st in ST:
+        check_pairs(pairs, (20, BLOSUM62, 11, 1), mode, st)
+
+
+def test_default_split_on_a_large_read_batch():
+    """40,000 DNA read pairs (reads 100-150 nt, windows 200-600 nt) take the one-pair-per-lane
+    kernel by default; a few long pairs run one per wavefront beside it. Checked against the
+    oracle on a sample spread over the batch, and on every pair against the forced one-pair-per-
+    wavefront run."""
+    rng = np.random.default_rng(77)
+    n = 40000
+    pairs = []
+    for k in range(n):
+        r = rng.integers(0, 4, size=int(rng.integers(100, 151)), dtype=np.uint8)
+        w = rng.integers(0, 4, size=int(rng.integers(200, 601)), dtype=np.uint8)
+        if k % 3 == 0:
+            at = int(rng.integers(0, w.size - 60))
+            w[at:at + 60] = r[:60]
+        pairs.append((r, w))
+    pairs[5] = (rng.integers(0, 4, size=5000, dtype=np.uint8),
+                rng.integers(0, 4, size=9000, dtype=np.uint8))
+    s = scoring(dna_scheme())
+    for mode in (MODE_SW, 2):
+        got = hy.align_pairs(pairs, s, hy.Mode(mode), hy.SearchType.ScoreEnd)
+        for k in list(range(0, n, 97)) + [5]:
+            q, t = pairs[k]
+            assert got[k] == want_hit(list(q), list(t), dna_scheme(), mode,
+                                      hy.SearchType.ScoreEnd, k), (mode, k)
+        import os
+        os.environ["HYALITE_HIP_PAIRS_KERNEL"] = "waves"
+        try:
+            ref = hy.align_pairs(pairs, s, hy.Mode(mode), hy.SearchType.ScoreEnd)
+        finally:
+            del os.environ["HYALITE_HIP_PAIRS_KERNEL"]
+        assert got == ref
+
+
+def test_pair_errors_follow_the_reference_order():
+    s = scoring(dna_scheme())
+    with pytest.raises(hy.SymbolOutOfRange) as e:
+        hy.align_pair([0, 7], [9], s, hy.Mode.Nw, hy.SearchType.Score)
+    assert e.value.symbol == 7          # the query is checked first
+    with pytest.raises(hy.SymbolOutOfRange) as e:
+        hy.align_pair([0, 1], [0, 8], s, hy.Mode.Nw, hy.SearchType.Score)
+    assert e.value.symbol == 8
+    huge = hy.Scoring(2, [0, 0, 0, 0], 2_000_000_000, 1)
+    with pytest.raises(hy.ScoreRangeTooWide) as e:
+        hy.align_pair([0], [1], huge, hy.Mode.Nw, hy.SearchType.Score)
+    assert e.value.bound == 2_000_000_001
+    with pytest.raises(hy.SymbolOutOfRange):
+        hy.align_pairs([([0], [1]), ([1], [0, 4]), ([5], [0])], s, hy.Mode.Sw,
+                       hy.SearchType.Score)
