@@ -20,9 +20,13 @@ Beside it, on the same GPU in the same run (each part can be skipped with a flag
   * traceback: the same 1,000,000 targets built with SearchType::Alignment — scan_aligned of
     the 360-aa query (winner plus its canonical alignment) against scan alone, and
     scan_all_aligned (one alignment per target, ops copied to the host);
+  * pairs: align_pairs over 200,000 mate-rescue-shaped DNA pairs (150-nt reads x 500-nt
+    windows) and 100,000 protein pairs (250-350 x 300-400), Score and ScoreEnd, device and
+    host-inclusive rates; align_pair and align_pair_position_max latency on single pairs;
   * cpu_baseline: the AVX2 16 x i16 restatement of the reference avx2 backend on a bounded
     sample of the batched workload, on this host's cores; beside it the scalar traceback
-    restatement on a sample of the traceback pairs, each checked against the device's result.
+    restatement on a sample of the traceback pairs and the scalar pair DP on a sample of the
+    pairs, each checked against the device's result.
 
 GCUPS = sum over queries of |q| * sum|t| / seconds / 1e9, real cells only. The roofline is the
 VALU throughput of the persistent batch kernel, timed with HIP events on its stream.
@@ -353,6 +357,87 @@ def cpu_traceback(w: wl.Workload, tb: dict) -> dict:
                       f"{secs:.2f} s; every alignment equal to the device's"}
 
 
+def pairs(sync) -> tuple[dict, dict]:
+    """The pairwise functions: align_pairs on the DNA read-pair and protein pair batches (flat
+    buffers; device time from HIP events around the kernels, wall time including the upload of
+    the sequences and the copy back), and single-pair latencies."""
+    out = {}
+    keep = {}
+    sc = hy.PairScratch()
+    for b in (wl.dna_read_pairs(), wl.protein_pairs()):
+        s = hy.Scoring(b.al, b.matrix, b.go, b.ge)
+        cells = b.cells
+        entry = {"pairs": b.n, "cells": cells, "mode": "SW"}
+        for st in (hy.SearchType.Score, hy.SearchType.ScoreEnd):
+            f = lambda: hy.align_pairs_flat(b.queries, b.qoff, b.targets, b.toff, s, hy.Mode.Sw,
+                                            st, scratch=sc)
+            f()
+            sync()
+            sc.profile_read(reset=True)
+            t0 = time.perf_counter()
+            for _ in range(3):
+                res = f()
+            sync()
+            wall = (time.perf_counter() - t0) / 3
+            dev_ms, calls = sc.profile_read(reset=True)
+            dev = dev_ms / max(calls, 1) / 1e3
+            key = "score" if st == hy.SearchType.Score else "score_end"
+            entry[key] = {"device_ms": round(dev * 1e3, 3), "wall_ms": round(wall * 1e3, 3),
+                          "gcups_device": round(cells / dev / 1e9, 1),
+                          "gcups_wall": round(cells / wall / 1e9, 1),
+                          "pairs_per_s": round(b.n / wall, 1)}
+            if st == hy.SearchType.ScoreEnd:
+                keep[b.name] = (b, res)
+        out[b.name] = entry
+        log(f"pairs {b.name}: {entry['score']['gcups_device']} / "
+            f"{entry['score_end']['gcups_device']} GCUPS on the device (Score / ScoreEnd)")
+    rng = np.random.default_rng(8)
+    p = wl.protein_freqs()
+    prot = hy.Scoring(24, wl.protein_matrix(2), 11, 1)
+    single = []
+    for m, n in ((150, 500), (360, 885), (1000, 10000)):
+        q = wl._draw(rng, m, p)
+        t = wl._draw(rng, n, p)
+
+        def lat(f, k=20):
+            f()
+            sync()
+            t0 = time.perf_counter()
+            for _ in range(k):
+                f()
+            sync()
+            return (time.perf_counter() - t0) / k * 1e3
+        single.append({"pair": [m, n],
+                       "align_pair_ms": round(lat(lambda: hy.align_pair_with(
+                           sc, q, t, prot, hy.Mode.Sw, hy.SearchType.ScoreEnd)), 3),
+                       "position_max_ms": round(lat(lambda: hy.align_pair_position_max_with(
+                           sc, q, t, prot, [])), 3)})
+    out["single_pairs"] = single
+    return out, keep
+
+
+def cpu_pairs(keep: dict) -> dict:
+    """The scalar pair DP (the oracle's align_core) on 1,000 pairs spread over each batch, one
+    thread, every result checked against the device's ScoreEnd run."""
+    import oracle
+    res = {}
+    for name, (b, (scores, qe, te)) in keep.items():
+        secs, cells = 0.0, 0
+        for k in np.linspace(0, b.n - 1, 1000).astype(int):
+            q, t = b.pair(int(k))
+            t0 = time.perf_counter()
+            s, wq, wt = oracle.align_core(q, t, b.al, b.matrix, b.go, b.ge, 0)
+            secs += time.perf_counter() - t0
+            cells += len(q) * len(t)
+            if (s, -1 if wq is None else wq, -1 if wt is None else wt) != (scores[k], qe[k],
+                                                                           te[k]):
+                raise RuntimeError(f"device align_pairs disagrees with the oracle at pair {k}")
+        res[name] = {"value": round(cells / secs / 1e9, 4), "unit": "GCUPS", "cores": 1,
+                     "kind": "port", "sample": f"1000 pairs spread over the batch, scalar "
+                     f"full DP with ends, {secs:.2f} s; every result equal to the device's"}
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,6 +449,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-query loop")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-traceback", action="store_true", help="skip the aligned scans")
+    ap.add_argument("--no-pairs", action="store_true", help="skip the pairwise functions")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the DNA configurations (10k-target scan, 2M-target mode sweep)")
     ap.add_argument("--autotune", action="store_true",
@@ -492,6 +578,7 @@ def main() -> int:
     del sc, db
 
     tb, tb_sample = (None, None) if args.no_traceback else traceback(w, sync)
+    pr, pr_keep = (None, None) if args.no_pairs else pairs(sync)
     serve = None if args.no_serving else serving(args, sync)
     others = None if args.no_configs else other_configs(sync)
 
@@ -520,6 +607,7 @@ def main() -> int:
         "serving": serve,
         "other_configs": others,
         "traceback": tb,
+        "pairs": pr,
         "roofline": roof,
     }
     if not args.no_cpu:
@@ -527,6 +615,8 @@ def main() -> int:
         out["gpu_vs_cpu"] = round(gcups / out["cpu_baseline"]["value"], 1)
         if tb_sample is not None:
             out["cpu_baseline"]["traceback"] = cpu_traceback(w, tb_sample)
+        if pr_keep is not None:
+            out["cpu_baseline"]["pairs"] = cpu_pairs(pr_keep)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -37,7 +37,7 @@ __all__ = [
     "library_path", "load_library", "Tuning", "Server", "ServedHit", "ServerStats",
     "ServerTimeout", "AlignOp", "Alignment", "AlignedHit", "TracebackBudgetExceeded",
     "PairScratch", "align_pair", "align_pair_with", "align_pairs", "align_pair_position_max",
-    "align_pair_position_max_with", "score2",
+    "align_pair_position_max_with", "score2", "align_pairs_flat",
 ]
 
 BACKEND_ENV_VAR = "HYALITE_BACKEND"
@@ -544,6 +544,14 @@ def load_library():
                                    ctypes.c_size_t, _i32p, ctypes.c_size_t, ctypes.c_uint32,
                                    ctypes.c_int32, ctypes.c_int32, ctypes.c_uint32,
                                    ctypes.c_uint32, _i32p, _i64p, _i64p, _i64p, _u64p]
+    lib.hy_align_pairs_flat.restype = ctypes.c_int
+    lib.hy_align_pairs_flat.argtypes = [vp, _u8p, _u64p, _u8p, _u64p, ctypes.c_size_t, _i32p,
+                                        ctypes.c_size_t, ctypes.c_uint32, ctypes.c_int32,
+                                        ctypes.c_int32, ctypes.c_uint32, ctypes.c_uint32, _i32p,
+                                        _i64p, _i64p, _i64p, _u64p]
+    lib.hy_pair_scratch_profile_read.restype = ctypes.c_int
+    lib.hy_pair_scratch_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), _u64p,
+                                                 ctypes.c_int]
     lib.hy_align_pair_position_max.restype = ctypes.c_int
     lib.hy_align_pair_position_max.argtypes = [vp, _u8p, ctypes.c_size_t, _u8p, ctypes.c_size_t,
                                                _i32p, ctypes.c_size_t, ctypes.c_uint32,
@@ -1039,6 +1047,14 @@ class PairScratch:
             raise DeviceError(load_library().hy_strerror(rc).decode())
         self._h = out
 
+    def profile_read(self, reset: bool = False) -> tuple[float, int]:
+        """(device ms, batches) of the align_pairs calls through this scratch: HIP events around
+        the kernels, after the uploads and before the copy back."""
+        ms, n = ctypes.c_double(0), ctypes.c_uint64(0)
+        load_library().hy_pair_scratch_profile_read(self._h, ctypes.byref(ms), ctypes.byref(n),
+                                                    1 if reset else 0)
+        return ms.value, n.value
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and _lib is not None:
@@ -1080,37 +1096,54 @@ def align_pair(query, target, scoring: Scoring, mode: Mode, search_type) -> Best
     return align_pair_with(PairScratch(), query, target, scoring, mode, search_type)
 
 
-def align_pairs(pairs, scoring: Scoring, mode: Mode, search_type, out: Optional[list] = None,
-                scratch: Optional[PairScratch] = None) -> list:
-    """src/kernel.rs:429-452: one BestHit per (query, target) pair, db_index = pair index; the
-    first failing pair's error is raised."""
-    qs = [np.ascontiguousarray(np.asarray(q, dtype=np.uint8)) for q, _ in pairs]
-    ts = [np.ascontiguousarray(np.asarray(t, dtype=np.uint8)) for _, t in pairs]
-    n = len(qs)
-    qp, tp = (_u8p * max(n, 1))(), (_u8p * max(n, 1))()
-    ql, tl = (ctypes.c_size_t * max(n, 1))(), (ctypes.c_size_t * max(n, 1))()
-    keep = []
-    for k in range(n):
-        a = qs[k] if qs[k].size else np.zeros(1, dtype=np.uint8)
-        b = ts[k] if ts[k].size else np.zeros(1, dtype=np.uint8)
-        keep += [a, b]
-        qp[k], tp[k] = a.ctypes.data_as(_u8p), b.ctypes.data_as(_u8p)
-        ql[k], tl[k] = qs[k].size, ts[k].size
+def align_pairs_flat(queries, query_offsets, targets, target_offsets, scoring: Scoring,
+                     mode: Mode, search_type, scratch: Optional[PairScratch] = None):
+    """align_pairs over concatenated sequences (pair k = queries[query_offsets[k]:
+    query_offsets[k+1]] against the same slice of targets): (scores int32[n], query_ends
+    int64[n], target_ends int64[n]), -1 standing for None."""
+    qf = _as_u8(np.asarray(queries, dtype=np.uint8).ravel())
+    tf = _as_u8(np.asarray(targets, dtype=np.uint8).ravel())
+    qo = np.ascontiguousarray(query_offsets, dtype=np.uint64)
+    to = np.ascontiguousarray(target_offsets, dtype=np.uint64)
+    if qo.size != to.size or qo.size == 0:
+        raise ValueError("query and target offsets need n + 1 entries each")
+    n = qo.size - 1
     scores = np.empty(max(n, 1), dtype=np.int32)
     qe = np.empty(max(n, 1), dtype=np.int64)
     te = np.empty(max(n, 1), dtype=np.int64)
     mat = scoring.matrix()
     det, bad = ctypes.c_int64(0), ctypes.c_uint64(0)
     sc = scratch or PairScratch()
-    rc = load_library().hy_align_pairs(
-        sc._h, qp, ql, tp, tl, n, mat.ctypes.data_as(_i32p), mat.size, scoring.alphabet_len(),
+    rc = load_library().hy_align_pairs_flat(
+        sc._h, qf.ctypes.data_as(_u8p), qo.ctypes.data_as(_u64p), tf.ctypes.data_as(_u8p),
+        to.ctypes.data_as(_u64p), n, mat.ctypes.data_as(_i32p), mat.size, scoring.alphabet_len(),
         scoring.gap_open(), scoring.gap_ext(), int(Mode(mode)), _st_code(search_type),
         scores.ctypes.data_as(_i32p), qe.ctypes.data_as(_i64p), te.ctypes.data_as(_i64p),
         ctypes.byref(det), ctypes.byref(bad))
-    if out is not None:
-        out.clear()
     if rc != 0:
         _raise_pair_error(rc, det.value, scoring)
+    return scores[:n], qe[:n], te[:n]
+
+
+def _flatten(seqs):
+    arrs = [np.asarray(x, dtype=np.uint8).ravel() for x in seqs]
+    off = np.zeros(len(arrs) + 1, dtype=np.uint64)
+    if arrs:
+        np.cumsum([a.size for a in arrs], out=off[1:])
+    flat = np.concatenate(arrs) if arrs else np.zeros(0, dtype=np.uint8)
+    return flat, off
+
+
+def align_pairs(pairs, scoring: Scoring, mode: Mode, search_type, out: Optional[list] = None,
+                scratch: Optional[PairScratch] = None) -> list:
+    """src/kernel.rs:429-452: one BestHit per (query, target) pair, db_index = pair index; the
+    first failing pair's error is raised."""
+    if out is not None:
+        out.clear()
+    qf, qo = _flatten([q for q, _ in pairs])
+    tf, to = _flatten([t for _, t in pairs])
+    scores, qe, te = align_pairs_flat(qf, qo, tf, to, scoring, mode, search_type, scratch)
+    n = len(pairs)
     hits = [BestHit(int(scores[k]), k, None if qe[k] < 0 else int(qe[k]),
                     None if te[k] < 0 else int(te[k])) for k in range(n)]
     if out is not None:

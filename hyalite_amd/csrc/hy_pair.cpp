@@ -33,6 +33,9 @@ struct hy_pair_scratch {
     int device = 0;
     hipStream_t stream = nullptr, aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // around a batch's device work
+    double device_ms = 0.0;
+    uint64_t calls = 0;
     DevBuf mat, qflat, tflat, qoff, toff, qoff32, qlen32, list, nq, groups, twords, qwords, tlens,
         qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best;
     uint8_t* stage = nullptr;  // pinned
@@ -49,6 +52,8 @@ struct hy_pair_scratch {
         if (stage) (void)hipHostFree(stage);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
         if (aux) (void)hipStreamDestroy(aux);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -64,6 +69,8 @@ constexpr size_t kLaneMaxTarget = 65536;
 constexpr size_t kLaneMinPairs = 32768;
 // Boundary rows of one launch of long multi-pass pairs.
 constexpr size_t kSysSpillBudget = 512ull << 20;
+// Returned by the exact per-pair checks when every pair passes.
+constexpr int kChecksPassed = -1;
 
 // Grow a device buffer (contents dropped; every entry point synchronises before returning).
 int grow(DevBuf& b, size_t bytes) {
@@ -247,7 +254,8 @@ extern "C" int hy_pair_scratch_new(hy_pair_scratch** out) {
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&s->join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&s->join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
         delete s;
         return HY_E_DEVICE;
     }
@@ -322,31 +330,75 @@ extern "C" int hy_align_pair(hy_pair_scratch* s, const uint8_t* query, size_t ql
     return HY_OK;
 }
 
-extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
-                              const size_t* qlens, const uint8_t* const* targets,
-                              const size_t* tlens, size_t npairs, const int32_t* matrix,
-                              size_t matrix_len, uint32_t alphabet_len, int32_t gap_open,
-                              int32_t gap_ext, uint32_t mode, uint32_t search_type,
-                              int32_t* scores, int64_t* query_ends, int64_t* target_ends,
-                              int64_t* err_detail, uint64_t* err_pair) {
-    if (!s || (npairs && (!queries || !qlens || !targets || !tlens || !scores)))
-        return HY_E_NULL_ARGUMENT;
+namespace {
+
+// The sequences of a batch: per-pair pointers and lengths, or one buffer per side with n + 1
+// offsets.
+struct PairSource {
+    const uint8_t* const* qp = nullptr;
+    const size_t* qlen = nullptr;
+    const uint8_t* const* tp = nullptr;
+    const size_t* tlen = nullptr;
+    const uint8_t* qflat = nullptr;
+    const uint64_t* qoff = nullptr;
+    const uint8_t* tflat = nullptr;
+    const uint64_t* toff = nullptr;
+    size_t m(size_t k) const { return qp ? qlen[k] : (size_t)(qoff[k + 1] - qoff[k]); }
+    size_t n(size_t k) const { return tp ? tlen[k] : (size_t)(toff[k + 1] - toff[k]); }
+    const uint8_t* q(size_t k) const { return qp ? qp[k] : qflat + qoff[k]; }
+    const uint8_t* t(size_t k) const { return tp ? tp[k] : tflat + toff[k]; }
+};
+
+uint8_t max_byte(const uint8_t* p, size_t n) {
+    uint8_t mx = 0;
+    for (size_t i = 0; i < n; ++i) mx = p[i] > mx ? p[i] : mx;
+    return mx;
+}
+
+int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
+                     const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len,
+                     int32_t gap_open, int32_t gap_ext, uint32_t mode, uint32_t search_type,
+                     int32_t* scores, int64_t* query_ends, int64_t* target_ends,
+                     int64_t* err_detail, uint64_t* err_pair) {
+    if (!s || (npairs && !scores)) return HY_E_NULL_ARGUMENT;
     if (err_detail) *err_detail = 0;
     if (err_pair) *err_pair = 0;
     if (mode > HY_MODE_SHW || search_type > HY_SEARCH_ALIGNMENT) return HY_E_INVALID_ARGUMENT;
     Scheme sc;
     int rc = check_scheme(matrix, matrix_len, alphabet_len, gap_open, gap_ext, &sc, err_detail);
     if (rc != HY_OK) return rc;
-    uint64_t qtot = 0, ttot = 0;
-    for (size_t k = 0; k < npairs; ++k) {
-        rc = check_pair(queries[k], qlens[k], targets[k], tlens[k], alphabet_len, mode, gap_open,
-                        gap_ext, sc, err_detail);
-        if (rc != HY_OK) {
-            if (err_pair) *err_pair = k;
-            return rc;
+    // The exact per-pair checks in the reference's order (pair by pair: symbols, then width)
+    // run only when a whole-batch check fails: the largest symbol, and the width proof at the
+    // largest lengths (the bound grows with both lengths, so it covers every pair).
+    auto exact_checks = [&]() -> int {
+        for (size_t k = 0; k < npairs; ++k) {
+            const int r = check_pair(src.q(k), src.m(k), src.t(k), src.n(k), alphabet_len, mode,
+                                     gap_open, gap_ext, sc, err_detail);
+            if (r != HY_OK) {
+                if (err_pair) *err_pair = k;
+                return r;
+            }
         }
-        qtot += qlens[k];
-        ttot += tlens[k];
+        return kChecksPassed;
+    };
+    uint64_t qtot = 0, ttot = 0;
+    size_t maxm = 0, maxn = 0;
+    for (size_t k = 0; k < npairs; ++k) {
+        const size_t m = src.m(k), n = src.n(k);
+        if ((m && !src.q(k)) || (n && !src.t(k))) return HY_E_NULL_ARGUMENT;
+        qtot += m;
+        ttot += n;
+        maxm = std::max(maxm, m);
+        maxn = std::max(maxn, n);
+    }
+    {
+        int64_t bound = 0;
+        if (hy::width_bits_for(hy::magnitude_bound(mode, sc.min_e, sc.max_e, gap_open, gap_ext,
+                                                   maxm, maxn), &bound) == 0 ||
+            maxm > 0x7FFFFFF0ull || maxn > 0x7FFFFFF0ull) {
+            rc = exact_checks();
+            if (rc != kChecksPassed) return rc;
+        }
     }
     if (npairs == 0) return HY_OK;
     if (npairs > 0xFFFFFFF0ull || qtot > 0xFFFFFFF0ull) return HY_E_INVALID_ARGUMENT;
@@ -355,7 +407,11 @@ extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
     const uint32_t al_eff = std::min<uint32_t>(alphabet_len, 256);
 
     // Split: one pair per lane for the short pairs of a large batch, one per wavefront else.
-    std::vector<size_t> ql(qlens, qlens + npairs), tl(tlens, tlens + npairs);
+    std::vector<size_t> ql(npairs), tl(npairs);
+    for (size_t k = 0; k < npairs; ++k) {
+        ql[k] = src.m(k);
+        tl[k] = src.n(k);
+    }
     const int force = lane_choice();
     std::vector<uint32_t> lanes, waves;
     size_t nshort = 0;
@@ -368,17 +424,31 @@ extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
         if (use_lanes && (shortp || force == 1)) lanes.push_back(k);
         else waves.push_back(k);
     }
-    // Lanes: sorted by (query length, target length) descending, cut into 64-lane groups.
-    std::stable_sort(lanes.begin(), lanes.end(), [&](uint32_t a, uint32_t b) {
-        return ql[a] != ql[b] ? ql[a] > ql[b] : tl[a] > tl[b];
-    });
+    // Lanes: sorted by (query length, target length) descending, cut into 64-lane groups. The
+    // sort key packs both lengths (complemented, 13 + 17 bits: lane pairs are at most 4096 x
+    // 65536 unless the lane kernel is forced) above the pair index.
+    {
+        std::vector<uint64_t> keys(lanes.size());
+        for (size_t i = 0; i < lanes.size(); ++i) {
+            const uint64_t k = lanes[i];
+            const uint64_t qk = 0x1FFFull - std::min<uint64_t>(ql[k], 0x1FFF);
+            const uint64_t tk = 0x1FFFFull - std::min<uint64_t>(tl[k], 0x1FFFF);
+            keys[i] = (qk << 51) | (tk << 34) | k;
+        }
+        std::sort(keys.begin(), keys.end());
+        for (size_t i = 0; i < lanes.size(); ++i) lanes[i] = (uint32_t)(keys[i] & 0x3FFFFFFFFull);
+    }
     // Waves: ordered by rows-per-lane class for one launch per class.
     std::stable_sort(waves.begin(), waves.end(), [&](uint32_t a, uint32_t b) {
         auto c = [&](uint32_t k) { return ql[k] <= 128 ? 0 : (ql[k] <= 256 ? 1 : (ql[k] <= 384 ? 2 : 3)); };
         return c(a) < c(b);
     });
-    int qtile = 32;
-    if (!lanes.empty() && ql[lanes.front()] <= 16) qtile = 16;
+    // Rows per register tile of the lane kernel. Score-only runs take 32 (fewer boundary-row
+    // trips per cell); end tracking carries three more values per cell and fits twice the
+    // resident wavefronts at 16. Measured on 200,000 150 x 500 DNA pairs: 2.37k vs 2.13k GCUPS
+    // (Score), 1.78k vs 1.62k (ScoreEnd). HYALITE_HIP_PAIRS_TILE=16|32 overrides it.
+    int qtile = ends ? 16 : 32;
+    if (const char* v = std::getenv("HYALITE_HIP_PAIRS_TILE")) qtile = std::atoi(v) >= 32 ? 32 : 16;
     std::vector<hy::PairGroup> groups;
     uint64_t twords = 0, qwords = 0, spill = 0;
     for (size_t g0 = 0; g0 < lanes.size(); g0 += 64) {
@@ -425,16 +495,36 @@ extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
         uint64_t* qo = reinterpret_cast<uint64_t*>(st.at(o_qoff));
         uint64_t* to = reinterpret_cast<uint64_t*>(st.at(o_toff));
         uint64_t qa = 0, ta = 0;
-        for (size_t k = 0; k < npairs; ++k) {
-            qo[k] = qa;
-            to[k] = ta;
-            if (ql[k]) std::memcpy(qd + qa, queries[k], ql[k]);
-            if (tl[k]) std::memcpy(td + ta, targets[k], tl[k]);
-            qa += ql[k];
-            ta += tl[k];
+        if (src.qp) {
+            for (size_t k = 0; k < npairs; ++k) {
+                qo[k] = qa;
+                if (ql[k]) std::memcpy(qd + qa, src.q(k), ql[k]);
+                qa += ql[k];
+            }
+        } else {
+            for (size_t k = 0; k < npairs; ++k) qo[k] = src.qoff[k] - src.qoff[0];
+            if (qtot) std::memcpy(qd, src.qflat + src.qoff[0], qtot);
+            qa = qtot;
+        }
+        if (src.tp) {
+            for (size_t k = 0; k < npairs; ++k) {
+                to[k] = ta;
+                if (tl[k]) std::memcpy(td + ta, src.t(k), tl[k]);
+                ta += tl[k];
+            }
+        } else {
+            for (size_t k = 0; k < npairs; ++k) to[k] = src.toff[k] - src.toff[0];
+            if (ttot) std::memcpy(td, src.tflat + src.toff[0], ttot);
+            ta = ttot;
         }
         qo[npairs] = qa;
         to[npairs] = ta;
+        // Symbols: one pass over the staged bytes; the exact per-pair order only on a failure.
+        if (std::max(max_byte(qd, qa), max_byte(td, ta)) >= alphabet_len) {
+            rc = exact_checks();
+            if (rc != kChecksPassed) return rc;
+            return HY_E_INTERNAL;
+        }
         if (!waves.empty()) {
             uint32_t* q32 = reinterpret_cast<uint32_t*>(st.at(o_q32));
             uint32_t* l32 = reinterpret_cast<uint32_t*>(st.at(o_l32));
@@ -463,6 +553,7 @@ extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
         (rc = grow(s->tend, npairs * sizeof(int64_t))) != HY_OK)
         return rc;
 
+    if (hipEventRecord(s->ev0, s->stream) != hipSuccess) return HY_E_DEVICE;
     // Long pairs and small batches: the aux stream, forked after the uploads.
     const bool both = !waves.empty() && !lanes.empty();
     hipStream_t wstream = both ? s->aux : s->stream;
@@ -516,6 +607,7 @@ extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
     if (both && (hipEventRecord(s->join, s->aux) != hipSuccess ||
                  hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess))
         return HY_E_DEVICE;
+    if (hipEventRecord(s->ev1, s->stream) != hipSuccess) return HY_E_DEVICE;
     if (hipMemcpyAsync(scores, s->scores.p, npairs * sizeof(int32_t), hipMemcpyDeviceToHost,
                        s->stream) != hipSuccess)
         return HY_E_DEVICE;
@@ -528,11 +620,72 @@ extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
             return HY_E_DEVICE;
     }
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
+    {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s->ev0, s->ev1) == hipSuccess) {
+            s->device_ms += ms;
+            s->calls += 1;
+        }
+    }
     if (!ends) {
         for (size_t k = 0; k < npairs; ++k) {
             if (query_ends) query_ends[k] = -1;
             if (target_ends) target_ends[k] = -1;
         }
+    }
+    return HY_OK;
+}
+
+}  // namespace
+
+extern "C" int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries,
+                              const size_t* qlens, const uint8_t* const* targets,
+                              const size_t* tlens, size_t npairs, const int32_t* matrix,
+                              size_t matrix_len, uint32_t alphabet_len, int32_t gap_open,
+                              int32_t gap_ext, uint32_t mode, uint32_t search_type,
+                              int32_t* scores, int64_t* query_ends, int64_t* target_ends,
+                              int64_t* err_detail, uint64_t* err_pair) {
+    if (npairs && (!queries || !qlens || !targets || !tlens)) return HY_E_NULL_ARGUMENT;
+    PairSource src;
+    src.qp = queries;
+    src.qlen = qlens;
+    src.tp = targets;
+    src.tlen = tlens;
+    return align_pairs_core(s, src, npairs, matrix, matrix_len, alphabet_len, gap_open, gap_ext,
+                            mode, search_type, scores, query_ends, target_ends, err_detail,
+                            err_pair);
+}
+
+extern "C" int hy_align_pairs_flat(hy_pair_scratch* s, const uint8_t* queries,
+                                   const uint64_t* query_offsets, const uint8_t* targets,
+                                   const uint64_t* target_offsets, size_t npairs,
+                                   const int32_t* matrix, size_t matrix_len,
+                                   uint32_t alphabet_len, int32_t gap_open, int32_t gap_ext,
+                                   uint32_t mode, uint32_t search_type, int32_t* scores,
+                                   int64_t* query_ends, int64_t* target_ends,
+                                   int64_t* err_detail, uint64_t* err_pair) {
+    if (npairs && (!query_offsets || !target_offsets)) return HY_E_NULL_ARGUMENT;
+    for (size_t k = 0; k < npairs; ++k)
+        if (query_offsets[k + 1] < query_offsets[k] || target_offsets[k + 1] < target_offsets[k])
+            return HY_E_INVALID_ARGUMENT;
+    PairSource src;
+    src.qflat = queries;
+    src.qoff = query_offsets;
+    src.tflat = targets;
+    src.toff = target_offsets;
+    return align_pairs_core(s, src, npairs, matrix, matrix_len, alphabet_len, gap_open, gap_ext,
+                            mode, search_type, scores, query_ends, target_ends, err_detail,
+                            err_pair);
+}
+
+extern "C" int hy_pair_scratch_profile_read(hy_pair_scratch* s, double* device_ms,
+                                            uint64_t* calls, int reset) {
+    if (!s || !device_ms || !calls) return HY_E_NULL_ARGUMENT;
+    *device_ms = s->device_ms;
+    *calls = s->calls;
+    if (reset) {
+        s->device_ms = 0.0;
+        s->calls = 0;
     }
     return HY_OK;
 }

@@ -185,3 +185,66 @@ def c5_serving(n_targets: int = 2_000_000, nq: int = 64) -> Workload:
     w.queries = c5_serving_queries(nq=nq)
     w.note += "; queries log-normal median 350 [20, 3000] (seed 5)"
     return w
+
+
+@dataclass
+class PairBatch:
+    """Independent (query, target) pairs as one buffer per side with n + 1 offsets each."""
+    name: str
+    al: int
+    matrix: np.ndarray
+    go: int
+    ge: int
+    queries: np.ndarray
+    qoff: np.ndarray
+    targets: np.ndarray
+    toff: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return len(self.qoff) - 1
+
+    @property
+    def cells(self) -> int:
+        return int((np.diff(self.qoff).astype(np.int64) * np.diff(self.toff).astype(np.int64)).sum())
+
+    def pair(self, k: int):
+        return (self.queries[int(self.qoff[k]):int(self.qoff[k + 1])],
+                self.targets[int(self.toff[k]):int(self.toff[k + 1])])
+
+
+def dna_read_pairs(n: int = 200_000, seed: int = 6, read_len: int = 150,
+                   window_len: int = 500) -> PairBatch:
+    """Mate-rescue-shaped DNA pairs for align_pairs: reads of read_len against windows of
+    window_len (uniform i.i.d. ACGT), every other window holding its read at a random offset
+    with 5% substitutions; the DNA scheme (+2 / -1, gap_open 2, gap_ext 1)."""
+    rng = np.random.default_rng(seed)
+    q = rng.integers(0, 4, size=n * read_len, dtype=np.uint8)
+    t = rng.integers(0, 4, size=n * window_len, dtype=np.uint8)
+    for k in range(0, n, 2):
+        at = k * window_len + int(rng.integers(0, window_len - read_len + 1))
+        r = q[k * read_len:(k + 1) * read_len].copy()
+        sub = rng.random(read_len) < 0.05
+        r[sub] = (r[sub] + rng.integers(1, 4, size=int(sub.sum()), dtype=np.uint8)) % 4
+        t[at:at + read_len] = r
+    qoff = (np.arange(n + 1, dtype=np.uint64) * read_len)
+    toff = (np.arange(n + 1, dtype=np.uint64) * window_len)
+    return PairBatch(f"dna_read_pairs_{read_len}x{window_len}x{n}", 4, dna_matrix(), 2, 1, q,
+                     qoff, t, toff)
+
+
+def protein_pairs(n: int = 100_000, seed: int = 7) -> PairBatch:
+    """Protein pairs: queries of 250-350 and targets of 300-400 residues drawn at background
+    frequencies, the protein matrix of the single-query workload (gap_open 11, gap_ext 1)."""
+    rng = np.random.default_rng(seed)
+    p = protein_freqs()
+    lq = rng.integers(250, 351, size=n)
+    lt = rng.integers(300, 401, size=n)
+    q = _draw(rng, int(lq.sum()), p)
+    t = _draw(rng, int(lt.sum()), p)
+    qoff = np.zeros(n + 1, dtype=np.uint64)
+    toff = np.zeros(n + 1, dtype=np.uint64)
+    np.cumsum(lq, out=qoff[1:])
+    np.cumsum(lt, out=toff[1:])
+    return PairBatch(f"protein_pairs_250-350x300-400x{n}", 24, protein_matrix(2), 11, 1, q, qoff,
+                     t, toff)

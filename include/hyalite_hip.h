@@ -207,6 +207,23 @@ int hy_align_pairs(hy_pair_scratch* s, const uint8_t* const* queries, const size
                    int32_t* scores, int64_t* query_ends, int64_t* target_ends,
                    int64_t* err_detail, uint64_t* err_pair);
 
+/* hy_align_pairs over concatenated sequences: pair k is queries[query_offsets[k] ..
+ * query_offsets[k + 1]) against targets[target_offsets[k] .. target_offsets[k + 1]) (n + 1
+ * offsets per side). Same results and errors as hy_align_pairs; no per-pair pointers. */
+int hy_align_pairs_flat(hy_pair_scratch* s, const uint8_t* queries,
+                        const uint64_t* query_offsets, const uint8_t* targets,
+                        const uint64_t* target_offsets, size_t npairs, const int32_t* matrix,
+                        size_t matrix_len, uint32_t alphabet_len, int32_t gap_open,
+                        int32_t gap_ext, uint32_t mode, uint32_t search_type, int32_t* scores,
+                        int64_t* query_ends, int64_t* target_ends, int64_t* err_detail,
+                        uint64_t* err_pair);
+
+/* Device milliseconds of the batches run through this scratch (HIP events around the kernels,
+ * after the uploads and before the copy back) and their count; reset zeroes them. No reference
+ * counterpart: instrumentation. */
+int hy_pair_scratch_profile_read(hy_pair_scratch* s, double* device_ms, uint64_t* calls,
+                                 int reset);
+
 /* Local-mode per-target-position maxima (align_pair_position_max_with): colmax[t] = the best
  * local score ending at target position t (tlen entries); *score = their maximum, *target_end =
  * its first position or -1 when the best is 0 (query end untracked). */

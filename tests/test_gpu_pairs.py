@@ -297,3 +297,31 @@ def test_pair_errors_follow_the_reference_order():
     with pytest.raises(hy.SymbolOutOfRange):
         hy.align_pairs([([0], [1]), ([1], [0, 4]), ([5], [0])], s, hy.Mode.Sw,
                        hy.SearchType.Score)
+
+
+def test_flat_batch_equals_the_pair_list():
+    """align_pairs_flat over one buffer per side (offsets not starting at 0, as a slice of a
+    larger buffer would give) equals align_pairs over the same pairs."""
+    rng = np.random.default_rng(12)
+    lens_q = rng.integers(0, 200, size=300)
+    lens_t = rng.integers(0, 500, size=300)
+    qf = rng.integers(0, 4, size=int(lens_q.sum()) + 7, dtype=np.uint8)
+    tf = rng.integers(0, 4, size=int(lens_t.sum()) + 5, dtype=np.uint8)
+    qo = np.concatenate([[7], 7 + np.cumsum(lens_q)]).astype(np.uint64)
+    to = np.concatenate([[5], 5 + np.cumsum(lens_t)]).astype(np.uint64)
+    pairs = [(qf[qo[k]:qo[k + 1]], tf[to[k]:to[k + 1]]) for k in range(300)]
+    s = scoring(dna_scheme())
+    sc = hy.PairScratch()
+    for mode in ALL_MODES:
+        sco, qe, te = hy.align_pairs_flat(qf, qo, tf, to, s, hy.Mode(mode),
+                                          hy.SearchType.ScoreEnd, scratch=sc)
+        hits = hy.align_pairs(pairs, s, hy.Mode(mode), hy.SearchType.ScoreEnd, scratch=sc)
+        for k, h in enumerate(hits):
+            assert (h.score, -1 if h.query_end is None else h.query_end,
+                    -1 if h.target_end is None else h.target_end) == (sco[k], qe[k], te[k])
+            assert h == want_hit(list(pairs[k][0]), list(pairs[k][1]), dna_scheme(), mode,
+                                 hy.SearchType.ScoreEnd, k)
+    ms, calls = sc.profile_read(reset=True)
+    assert calls == 10 and ms > 0
+    with pytest.raises(ValueError):
+        hy.align_pairs_flat(qf, qo[:-1], tf, to, s, hy.Mode.Sw, hy.SearchType.Score)
