@@ -20,6 +20,7 @@ Beside it, on the same GPU in the same run (each part can be skipped with a flag
   * traceback: the same 1,000,000 targets built with SearchType::Alignment — scan_aligned of
     the 360-aa query (winner plus its canonical alignment) against scan alone, and
     scan_all_aligned (one alignment per target, ops copied to the host);
+  * db_image: the database saved to an on-disk image and loaded back (sizes and times);
   * pairs: align_pairs over 200,000 mate-rescue-shaped DNA pairs (150-nt reads x 500-nt
     windows) and 100,000 protein pairs (250-350 x 300-400), Score and ScoreEnd, device and
     host-inclusive rates; align_pair and align_pair_position_max latency on single pairs;
@@ -438,6 +439,28 @@ def cpu_pairs(keep: dict) -> dict:
     return res
 
 
+def db_image(db: hy.Database, w: wl.Workload, build_s: float) -> dict:
+    """The on-disk image of the batched workload's database: save, load back (checksum, the
+    build's validation, residue upload, layouts gathered on the device), one scan compared."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "db.img")
+        t0 = time.perf_counter()
+        db.save(path)
+        save_s = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        ld = hy.Database.load(path)
+        load_s = time.perf_counter() - t0
+    q = w.extra["single_query"]
+    if ld.scan(hy.Scratch(ld), q) != db.scan(hy.Scratch(db), q):
+        raise RuntimeError("loaded database disagrees with the original")
+    log(f"image: {size / 1e6:.0f} MB, save {save_s:.2f} s, load {load_s:.2f} s "
+        f"(build from memory {build_s:.2f} s)")
+    return {"image_bytes": size, "save_s": round(save_s, 3), "load_s": round(load_s, 3),
+            "build_from_memory_s": round(build_s, 3), "targets": w.n, "residues": w.residues}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -450,6 +473,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-traceback", action="store_true", help="skip the aligned scans")
     ap.add_argument("--no-pairs", action="store_true", help="skip the pairwise functions")
+    ap.add_argument("--no-image", action="store_true", help="skip the database image round trip")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the DNA configurations (10k-target scan, 2M-target mode sweep)")
     ap.add_argument("--autotune", action="store_true",
@@ -486,6 +510,7 @@ def main() -> int:
     build_s = time.perf_counter() - t0
     parts = {str(k): v for k, v in db.partition_sizes().items()}
     qs = w.queries
+    image = None if args.no_image else db_image(db, w, build_s)
     sc = hy.Scratch(db)
     log(f"workload {w.name}: {w.n} targets, {w.residues} residues, {len(qs)} queries "
         f"({sum(len(q) for q in qs)} residues), gen {gen_s:.1f}s, build+upload {build_s:.1f}s, "
@@ -608,6 +633,7 @@ def main() -> int:
         "other_configs": others,
         "traceback": tb,
         "pairs": pr,
+        "db_image": image,
         "roofline": roof,
     }
     if not args.no_cpu:

@@ -37,7 +37,7 @@ __all__ = [
     "library_path", "load_library", "Tuning", "Server", "ServedHit", "ServerStats",
     "ServerTimeout", "AlignOp", "Alignment", "AlignedHit", "TracebackBudgetExceeded",
     "PairScratch", "align_pair", "align_pair_with", "align_pairs", "align_pair_position_max",
-    "align_pair_position_max_with", "score2", "align_pairs_flat",
+    "align_pair_position_max_with", "score2", "align_pairs_flat", "DatabaseImageError",
 ]
 
 BACKEND_ENV_VAR = "HYALITE_BACKEND"
@@ -48,7 +48,7 @@ _LIB_NAME = "libhyalite_hip.so"
 # Translation units: the host library, the autotuner, the mode-independent kernels and dispatch,
 # and the mode-templated kernels compiled once per alignment mode (SW, NW, HW, OV, SHW).
 _UNITS = [("csrc/hy_host.cpp", []), ("csrc/hy_tune.cpp", []), ("csrc/hy_serve.cpp", []),
-          ("csrc/hy_align.cpp", []), ("csrc/hy_pair.cpp", []),
+          ("csrc/hy_align.cpp", []), ("csrc/hy_pair.cpp", []), ("csrc/hy_image.cpp", []),
           ("csrc/hy_launch.hip", [])] + [
     (src, [f"-DHY_KMODE={k}"])
     for src in ("csrc/hy_kernels.hip", "csrc/hy_trace.hip", "csrc/hy_pair.hip")
@@ -173,6 +173,10 @@ class InvalidBackendName(HyaliteError):
 class ScanContractViolation(Exception):
     """A scan-time contract breach (over-long query, symbol out of range). The reference panics
     here (src/database.rs:79-89)."""
+
+
+class DatabaseImageError(ValueError):
+    """Database.load: the file is not a database image, or a damaged one (size or checksum)."""
 
 
 class DeviceError(RuntimeError):
@@ -510,6 +514,12 @@ def load_library():
     lib.hy_db_info.restype = ctypes.c_int
     lib.hy_db_info.argtypes = [vp, ctypes.POINTER(ctypes.c_uint32), _u64p, _u64p, _u64p,
                                ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
+    lib.hy_db_save.restype = ctypes.c_int
+    lib.hy_db_save.argtypes = [vp, ctypes.c_char_p]
+    lib.hy_db_load.restype = ctypes.c_int
+    lib.hy_db_load.argtypes = [ctypes.c_char_p, ctypes.c_uint32, ctypes.POINTER(vp), _i64p]
+    lib.hy_db_scoring.restype = ctypes.c_int
+    lib.hy_db_scoring.argtypes = [vp, _i32p, ctypes.POINTER(ctypes.c_uint32), _i32p, _i32p, _u64p]
     lib.hy_db_partitions.restype = ctypes.c_int
     lib.hy_db_partitions.argtypes = [vp, _u64p, _u64p, _u64p]
     lib.hy_scratch_new.restype = ctypes.c_int
@@ -768,6 +778,43 @@ class Database:
     @staticmethod
     def builder() -> DatabaseBuilder:
         return DatabaseBuilder()
+
+    def save(self, path: str) -> None:
+        """Write the database's image (scheme, builder fields, residues; checksummed)."""
+        rc = load_library().hy_db_save(self._h, os.fsencode(path))
+        if rc != 0:
+            raise OSError(f"cannot write database image {path!r}: "
+                          f"{load_library().hy_strerror(rc).decode()}")
+
+    @staticmethod
+    def load(path: str, backend: Optional[BackendChoice] = None) -> "Database":
+        """Rebuild a database from an image written by save(); the build's validation runs
+        again on its contents."""
+        lib = load_library()
+        out = ctypes.c_void_p(None)
+        detail = ctypes.c_int64(0)
+        b = 9 if backend is not None and backend.forced is not None else 0
+        rc = lib.hy_db_load(os.fsencode(path), b, ctypes.byref(out), ctypes.byref(detail))
+        if rc == 109:
+            raise DatabaseImageError(f"{path!r}: {lib.hy_strerror(rc).decode()}")
+        if rc == 103:
+            raise OSError(f"cannot read database image {path!r}")
+        if rc != 0:
+            with open(path, "rb") as f:  # the header names the alphabet for the error message
+                head = f.read(16)
+            al0 = int.from_bytes(head[12:16], "little") if len(head) == 16 else 0
+            _raise_build_error(rc, detail.value, al0, os.environ.get(BACKEND_ENV_VAR))
+        al, go, ge, budget = ctypes.c_uint32(0), ctypes.c_int32(0), ctypes.c_int32(0), \
+            ctypes.c_uint64(0)
+        lib.hy_db_scoring(out, None, ctypes.byref(al), ctypes.byref(go), ctypes.byref(ge),
+                          ctypes.byref(budget))
+        mat = np.zeros(al.value * al.value, dtype=np.int32)
+        lib.hy_db_scoring(out, mat.ctypes.data_as(_i32p), None, None, None, None)
+        mode, st = ctypes.c_uint32(0), ctypes.c_uint32(0)
+        lib.hy_db_info(out, None, None, None, None, ctypes.byref(mode), ctypes.byref(st))
+        stype = AlignmentSearch(budget.value) if st.value == 2 else SearchType(st.value)
+        return Database(out, Scoring(al.value, mat, go.value, ge.value), Mode(mode.value), stype,
+                        None)
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -326,7 +326,8 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     int rc = HY_OK;
     auto fail = [&](int code) { delete db; return code; };
 
-    // Matrix, clamped for the 32-bit kernels.
+    // Matrix, clamped for the 32-bit kernels; the scheme as given stays on the host.
+    db->h_matrix.assign(matrix, matrix + matrix_len);
     std::vector<int32_t> mat(matrix, matrix + matrix_len);
     for (auto& v : mat) v = std::max(-hy::kEntryMax32, std::min(hy::kEntryMax32, v));
     if (dev_upload(&db->d_matrix, mat) != hipSuccess) return fail(HY_E_DEVICE);
@@ -429,29 +430,19 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
         add_groups(0, n_not16);
         db->groups_score32 = (uint32_t)db->groups.size();
         add_groups(n_not16, ns);
-        std::vector<uint32_t> w((size_t)words, 0);
-        for (const Group& g : db->groups) {
-            const uint32_t nw = (g.ncols + 3) / 4;
-            for (uint32_t lane = 0; lane < 64; ++lane) {
-                const bool live = lane < g.count;
-                const uint32_t k = live ? slot_gidx[g.first + lane] : 0;
-                const uint32_t L = live ? (uint32_t)lens[k] : 0;
-                for (uint32_t jw = 0; jw < nw; ++jw) {
-                    uint32_t word = 0;
-                    for (uint32_t b = 0; b < 4; ++b) {
-                        const uint32_t col = jw * 4 + b;
-                        const uint32_t sym = col < L ? seqs[k][col] : pad;
-                        word |= (sym & 0xffu) << (8 * b);
-                    }
-                    w[(size_t)g.word_off + (size_t)jw * 64 + lane] = word;
-                }
-            }
-        }
         db->inter_spill = spill;
         if (dev_upload(&db->d_groups, db->groups) != hipSuccess) return fail(HY_E_DEVICE);
-        if (dev_upload(&db->d_words, w) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_lens, slot_len) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_gidx, slot_gidx) != hipSuccess) return fail(HY_E_DEVICE);
+        // The interleaved words are gathered on the device from the residues.
+        if (hipMalloc((void**)&db->d_words, std::max<uint64_t>(words, 1) * sizeof(uint32_t)) !=
+            hipSuccess) {
+            db->d_words = nullptr;
+            return fail(HY_E_OUT_OF_MEMORY);
+        }
+        if (hy::launch_db_pack32(db->d_groups, (uint32_t)db->groups.size(), db->d_gidx, db->d_off,
+                                 db->d_flat, db->d_words, pad, nullptr) != hipSuccess)
+            return fail(HY_E_DEVICE);
     }
 
     // Packed 16-bit layout: groups of 128 length-sorted targets, lane l holding slots 2l (low
@@ -484,34 +475,22 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
         db->groups16_short_first = (uint32_t)db->groups16.size();
         add_groups16(n16_long, ns);
         db->max_ncols16 = db->groups16.empty() ? 0 : db->groups16.front().ncols;
-        std::vector<uint2> w((size_t)words, make_uint2(0, 0));
-        for (const Group& g : db->groups16) {
-            const uint32_t nw = (g.ncols + 3) / 4;
-            for (uint32_t lane = 0; lane < 64; ++lane) {
-                for (uint32_t jw = 0; jw < nw; ++jw) {
-                    uint32_t wd[2] = {0, 0};
-                    for (uint32_t half = 0; half < 2; ++half) {
-                        const uint32_t slot = 2 * lane + half;
-                        const bool live = slot < g.count;
-                        const uint32_t k = live ? slot_gidx[g.first + slot] : 0;
-                        const uint32_t L = live ? (uint32_t)lens[k] : 0;
-                        for (uint32_t b = 0; b < 4; ++b) {
-                            const uint32_t col = jw * 4 + b;
-                            const uint32_t sym = col < L ? seqs[k][col] : pad;
-                            wd[half] |= (sym & 0xffu) << (8 * b);
-                        }
-                    }
-                    w[(size_t)g.word_off + (size_t)jw * 64 + lane] = make_uint2(wd[0], wd[1]);
-                }
-            }
-        }
         db->inter16_spill = spill;
 
         if (dev_upload(&db->d_groups16, db->groups16) != hipSuccess) return fail(HY_E_DEVICE);
-        if (dev_upload(&db->d_words16, w) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_lens16, slot_len) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_gidx16, slot_gidx) != hipSuccess) return fail(HY_E_DEVICE);
+        if (hipMalloc((void**)&db->d_words16, std::max<uint64_t>(words, 1) * sizeof(uint2)) !=
+            hipSuccess) {
+            db->d_words16 = nullptr;
+            return fail(HY_E_OUT_OF_MEMORY);
+        }
+        if (hy::launch_db_pack16(db->d_groups16, (uint32_t)db->groups16.size(), db->d_gidx16,
+                                 db->d_off, db->d_flat, db->d_words16, pad, nullptr) !=
+            hipSuccess)
+            return fail(HY_E_DEVICE);
     }
+    if (hipDeviceSynchronize() != hipSuccess) return fail(HY_E_DEVICE);
     if (dev_upload(&db->d_sys_list, db->sys_list) != hipSuccess) return fail(HY_E_DEVICE);
     if (dev_upload(&db->d_sys_list_batch, db->sys_list_batch) != hipSuccess)
         return fail(HY_E_DEVICE);
@@ -1234,6 +1213,7 @@ extern "C" const char* hy_strerror(int code) {
     case HY_E_OUT_OF_MEMORY: return "out of memory";
     case HY_E_INTERNAL: return "device self-check failed";
     case HY_E_TIMEOUT: return "timed out waiting for a served query";
+    case HY_E_BAD_IMAGE: return "not a database image, or a damaged one";
     case HY_E_NOT_ALIGNMENT:
         return "scan_aligned/scan_all_aligned require a database built with SearchType::Alignment";
     default: return "unknown status code";

@@ -241,6 +241,14 @@ hipError_t launch_inter32(int mode, bool ends, const InterArgs& a, uint32_t ngro
                           hipStream_t stream);
 // nq_grid > 1: batched form, a grid of nblocks x nq_grid blocks; a.m is then the longest query
 // of the batch (it fixes the rows per lane).
+// Database layouts on the device: lane-interleaved words of the 32-bit groups (64 targets) and
+// of the packed groups (128 targets, two per lane), from the residues and n + 1 offsets.
+hipError_t launch_db_pack32(const Group* groups, uint32_t ngroups, const uint32_t* gidx,
+                           const uint64_t* off, const uint8_t* flat, uint32_t* words,
+                           uint32_t pad, hipStream_t stream);
+hipError_t launch_db_pack16(const Group* groups, uint32_t ngroups, const uint32_t* gidx,
+                           const uint64_t* off, const uint8_t* flat, uint2* words, uint32_t pad,
+                           hipStream_t stream);
 // Pairs: lane-interleave a batch's queries and targets (one workgroup per group).
 hipError_t launch_pair_pack(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream);
 // Pairs: one launch over `ngroups` groups. q: rows per tile (16 or 32).

@@ -51,6 +51,60 @@ __global__ __launch_bounds__(64) void k_pair_pack(PairPackArgs a) {
     }
 }
 
+// Database layouts built on the device from the residues in database order: one workgroup per
+// group, lane l gathering its target's four residues per word (the padding symbol past its
+// length, and for lanes past the group's count).
+__global__ __launch_bounds__(64) void k_db_pack32(const Group* groups, const uint32_t* gidx,
+                                                  const uint64_t* off, const uint8_t* flat,
+                                                  uint32_t* words, uint32_t pad) {
+    const Group g = groups[blockIdx.x];
+    const int lane = threadIdx.x;
+    const bool live = lane < (int)g.count;
+    const uint32_t k = live ? gidx[g.first + lane] : 0u;
+    const uint64_t len = live ? off[k + 1] - off[k] : 0;
+    const uint8_t* t = flat + off[k];
+    const uint32_t nw = (g.ncols + 3) / 4;
+    for (uint32_t w = 0; w < nw; ++w) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const uint64_t c = (uint64_t)w * 4 + b;
+            v |= (c < len ? (uint32_t)t[c] : pad) << (8 * b);
+        }
+        words[g.word_off + (uint64_t)w * 64 + lane] = v;
+    }
+}
+
+// The packed layout: lane l holds slots 2l (low word) and 2l + 1 (high word) of its group.
+__global__ __launch_bounds__(64) void k_db_pack16(const Group* groups, const uint32_t* gidx,
+                                                  const uint64_t* off, const uint8_t* flat,
+                                                  uint2* words, uint32_t pad) {
+    const Group g = groups[blockIdx.x];
+    const int lane = threadIdx.x;
+    uint64_t len[2];
+    const uint8_t* t[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t slot = 2 * lane + h;
+        const bool live = slot < g.count;
+        const uint32_t k = live ? gidx[g.first + slot] : 0u;
+        len[h] = live ? off[k + 1] - off[k] : 0;
+        t[h] = flat + off[k];
+    }
+    const uint32_t nw = (g.ncols + 3) / 4;
+    for (uint32_t w = 0; w < nw; ++w) {
+        uint32_t v[2] = {0, 0};
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint64_t c = (uint64_t)w * 4 + b;
+                v[h] |= (c < len[h] ? (uint32_t)t[h][c] : pad) << (8 * b);
+            }
+        words[g.word_off + (uint64_t)w * 64 + lane] = make_uint2(v[0], v[1]);
+    }
+}
+
 __global__ void k_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) keys[i] = 0ull;
@@ -127,6 +181,24 @@ hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long
     if (n == 0) return hipSuccess;
     const unsigned blocks = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(k_fill_u64, dim3(blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_db_pack32(const Group* groups, uint32_t ngroups, const uint32_t* gidx,
+                           const uint64_t* off, const uint8_t* flat, uint32_t* words,
+                           uint32_t pad, hipStream_t stream) {
+    if (ngroups == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_db_pack32, dim3(ngroups), dim3(64), 0, stream, groups, gidx, off, flat,
+                       words, pad);
+    return hipGetLastError();
+}
+
+hipError_t launch_db_pack16(const Group* groups, uint32_t ngroups, const uint32_t* gidx,
+                           const uint64_t* off, const uint8_t* flat, uint2* words, uint32_t pad,
+                           hipStream_t stream) {
+    if (ngroups == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_db_pack16, dim3(ngroups), dim3(64), 0, stream, groups, gidx, off, flat,
+                       words, pad);
     return hipGetLastError();
 }
 

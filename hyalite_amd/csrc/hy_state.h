@@ -24,6 +24,7 @@ struct hy_db {
     int32_t go = 0, ge = 0, min_e = 0, max_e = 0;
     uint32_t mode = 0, search_type = 0;
     uint64_t traceback_max_bytes = 0;  // SearchType::Alignment's budget (0 otherwise)
+    std::vector<int32_t> h_matrix;     // the scheme's matrix as given (al x al)
     uint64_t n = 0, mql = 0, mtl = 0, total_residues = 0;
     int widest_bits = 8;
     uint64_t count_by_width[3] = {0, 0, 0};

@@ -86,7 +86,8 @@ enum {
     HY_E_OUT_OF_MEMORY = 105,
     HY_E_INTERNAL = 106,             /* a device-side self-check failed */
     HY_E_TIMEOUT = 107,              /* hy_server_wait: not completed within the timeout */
-    HY_E_NOT_ALIGNMENT = 108         /* an aligned scan on a database built without ALIGNMENT */
+    HY_E_NOT_ALIGNMENT = 108,        /* an aligned scan on a database built without ALIGNMENT */
+    HY_E_BAD_IMAGE = 109             /* hy_db_load: not a database image, or a damaged one */
 };
 
 /* Field flags for hy_db_build's `have` mask: which builder fields were set. A field left out
@@ -127,6 +128,20 @@ void hy_db_free(hy_db* db);
  * search_type; database.rs:581-656). widest_width_bits is 8, 16 or 32. */
 int hy_db_info(const hy_db* db, uint32_t* widest_width_bits, uint64_t* n, uint64_t* max_query_len,
                uint64_t* max_target_len, uint32_t* mode, uint32_t* search_type);
+
+/* On-disk image (no reference counterpart: the reference keeps sequences in memory only). The
+ * image holds the scheme, the builder fields and the residues in database order with a
+ * checksum; hy_db_load re-runs hy_db_build on them (the interleaved layouts are gathered on the
+ * device), so a loaded database equals the saved one and gives the same hits. hy_db_save needs a
+ * writable path (HY_E_INVALID_ARGUMENT otherwise); hy_db_load returns HY_E_BAD_IMAGE for a
+ * file that is not an image or fails its size or checksum checks, then the build's own codes. */
+int hy_db_save(const hy_db* db, const char* path);
+int hy_db_load(const char* path, uint32_t backend, hy_db** out, int64_t* err_detail);
+
+/* The scheme a database was built with: matrix (alphabet_len^2 entries as given; may be NULL),
+ * gaps and the traceback budget (0 unless built for SearchType::Alignment). */
+int hy_db_scoring(const hy_db* db, int32_t* matrix, uint32_t* alphabet_len, int32_t* gap_open,
+                  int32_t* gap_ext, uint64_t* traceback_max_bytes);
 
 /* Per-width partition sizes (number of targets proven i8, i16, i32). */
 int hy_db_partitions(const hy_db* db, uint64_t* n_i8, uint64_t* n_i16, uint64_t* n_i32);
