@@ -257,6 +257,8 @@ hipError_t launch_pairs32(int mode, bool ends, const PairArgs& a, uint32_t ngrou
 // Column maxima of one local-mode pair (zeroes colmax first).
 hipError_t launch_posmax(const PosMaxArgs& a, hipStream_t stream);
 hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t stream);
+// The largest byte of p[0, n) folded into *out (an atomic max; zero *out first).
+hipError_t launch_max_byte(const uint8_t* p, uint64_t n, uint32_t* out, hipStream_t stream);
 // Traceback: the DP over a.nitems items, then the walk (one lane per item).
 hipError_t launch_trace(int mode, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,

@@ -1,5 +1,7 @@
 // Mode-independent kernels (argmax keys, results, fills) and the dispatch from a runtime
 // alignment mode to the per-mode launchers that hy_kernels.hip exports (one object per mode).
+#include <algorithm>
+
 #include "hy_device.h"
 
 namespace hy {
@@ -105,6 +107,31 @@ __global__ __launch_bounds__(64) void k_db_pack16(const Group* groups, const uin
     }
 }
 
+// The largest byte of a buffer, folded into *out with an atomic max (grid-stride, 16 bytes per
+// load where the buffer allows).
+__global__ __launch_bounds__(256) void k_max_byte(const uint8_t* p, uint64_t n, uint32_t* out) {
+    uint32_t mx = 0;
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t head = (16 - ((uintptr_t)p & 15)) & 15;
+    const uint64_t h = head < n ? head : n;
+    for (uint64_t i = tid; i < h; i += stride) mx = max(mx, (uint32_t)p[i]);
+    const uint4* v = reinterpret_cast<const uint4*>(p + h);
+    const uint64_t nv = (n - h) / 16;
+    for (uint64_t i = tid; i < nv; i += stride) {
+        const uint4 w = v[i];
+        const uint32_t a[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) mx = max(mx, (a[k] >> (8 * b)) & 0xffu);
+    }
+    for (uint64_t i = h + nv * 16 + tid; i < n; i += stride) mx = max(mx, (uint32_t)p[i]);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+    if ((threadIdx.x & 63) == 0 && mx) atomicMax(out, mx);
+}
+
 __global__ void k_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) keys[i] = 0ull;
@@ -199,6 +226,14 @@ hipError_t launch_db_pack16(const Group* groups, uint32_t ngroups, const uint32_
     if (ngroups == 0) return hipSuccess;
     hipLaunchKernelGGL(k_db_pack16, dim3(ngroups), dim3(64), 0, stream, groups, gidx, off, flat,
                        words, pad);
+    return hipGetLastError();
+}
+
+hipError_t launch_max_byte(const uint8_t* p, uint64_t n, uint32_t* out, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint64_t want = (n / 16 + 255) / 256;
+    const unsigned blocks = (unsigned)std::min<uint64_t>(std::max<uint64_t>(want, 1), 4096);
+    hipLaunchKernelGGL(k_max_byte, dim3(blocks), dim3(256), 0, stream, p, n, out);
     return hipGetLastError();
 }
 

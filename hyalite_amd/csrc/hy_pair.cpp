@@ -37,7 +37,8 @@ struct hy_pair_scratch {
     double device_ms = 0.0;
     uint64_t calls = 0;
     DevBuf mat, qflat, tflat, qoff, toff, qoff32, qlen32, list, nq, groups, twords, qwords, tlens,
-        qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best;
+        qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best, flag;
+    uint32_t* h_flag = nullptr;  // pinned: the largest symbol of a batch
     uint8_t* stage = nullptr;  // pinned
     size_t stage_bytes = 0;
 
@@ -47,8 +48,9 @@ struct hy_pair_scratch {
         if (aux) (void)hipStreamSynchronize(aux);
         for (DevBuf* b : {&mat, &qflat, &tflat, &qoff, &toff, &qoff32, &qlen32, &list, &nq,
                           &groups, &twords, &qwords, &tlens, &qlens, &pidx, &spill, &sys_spill,
-                          &scores, &qend, &tend, &colmax, &best})
+                          &scores, &qend, &tend, &colmax, &best, &flag})
             hy::dev_free(b->p);
+        if (h_flag) (void)hipHostFree(h_flag);
         if (stage) (void)hipHostFree(stage);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
@@ -105,14 +107,18 @@ int grow_stage(hy_pair_scratch* s, size_t bytes) {
 struct Stager {
     hy_pair_scratch* s;
     size_t used = 0;
-    struct Piece { size_t off, bytes; DevBuf* dst; };
+    struct Piece { size_t off, bytes; DevBuf* dst; const void* ext; };
     std::vector<Piece> pieces;
     // Reserve `bytes` for `dst` (16-byte aligned) and return the host pointer to fill later.
     size_t add(DevBuf* dst, size_t bytes) {
         const size_t off = (used + 15) & ~size_t(15);
-        pieces.push_back({off, bytes, dst});
+        pieces.push_back({off, bytes, dst, nullptr});
         used = off + bytes;
         return off;
+    }
+    // Upload `bytes` from caller memory straight to `dst`, without passing through the stage.
+    void add_external(DevBuf* dst, const void* src, size_t bytes) {
+        pieces.push_back({0, bytes, dst, src});
     }
     uint8_t* at(size_t off) { return s->stage + off; }
     int commit() {
@@ -121,8 +127,9 @@ struct Stager {
             if (rc != HY_OK) return rc;
         }
         for (const Piece& p : pieces) {
-            if (p.bytes && hipMemcpyAsync(p.dst->p, s->stage + p.off, p.bytes,
-                                          hipMemcpyHostToDevice, s->stream) != hipSuccess)
+            const void* from = p.ext ? p.ext : static_cast<const void*>(s->stage + p.off);
+            if (p.bytes && hipMemcpyAsync(p.dst->p, from, p.bytes, hipMemcpyHostToDevice,
+                                          s->stream) != hipSuccess)
                 return HY_E_DEVICE;
         }
         return HY_OK;
@@ -255,7 +262,8 @@ extern "C" int hy_pair_scratch_new(hy_pair_scratch** out) {
         hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s->join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_flag, sizeof(uint32_t)) != hipSuccess) {
         delete s;
         return HY_E_DEVICE;
     }
@@ -348,12 +356,6 @@ struct PairSource {
     const uint8_t* q(size_t k) const { return qp ? qp[k] : qflat + qoff[k]; }
     const uint8_t* t(size_t k) const { return tp ? tp[k] : tflat + toff[k]; }
 };
-
-uint8_t max_byte(const uint8_t* p, size_t n) {
-    uint8_t mx = 0;
-    for (size_t i = 0; i < n; ++i) mx = p[i] > mx ? p[i] : mx;
-    return mx;
-}
 
 int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
                      const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len,
@@ -470,8 +472,13 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     }
 
     Stager st{s};
-    const size_t o_q = st.add(&s->qflat, qtot);
-    const size_t o_t = st.add(&s->tflat, ttot);
+    // Flat sequences go to the device straight from the caller's buffers; per-pair pointers are
+    // concatenated in the stage first.
+    size_t o_q = 0, o_t = 0;
+    if (src.qp) o_q = st.add(&s->qflat, qtot);
+    else st.add_external(&s->qflat, src.qflat + src.qoff[0], qtot);
+    if (src.tp) o_t = st.add(&s->tflat, ttot);
+    else st.add_external(&s->tflat, src.tflat + src.toff[0], ttot);
     const size_t o_qoff = st.add(&s->qoff, (npairs + 1) * sizeof(uint64_t));
     const size_t o_toff = st.add(&s->toff, (npairs + 1) * sizeof(uint64_t));
     size_t o_q32 = 0, o_l32 = 0, o_list = 0, o_nq = 0, o_groups = 0, o_tl = 0, o_ql = 0, o_pi = 0;
@@ -490,8 +497,8 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     if ((rc = grow_stage(s, st.used + matrix_len * sizeof(int32_t) + 64)) != HY_OK) return rc;
     stage_matrix(st, s, matrix, matrix_len);
     {
-        uint8_t* qd = st.at(o_q);
-        uint8_t* td = st.at(o_t);
+        uint8_t* qd = src.qp ? st.at(o_q) : nullptr;
+        uint8_t* td = src.tp ? st.at(o_t) : nullptr;
         uint64_t* qo = reinterpret_cast<uint64_t*>(st.at(o_qoff));
         uint64_t* to = reinterpret_cast<uint64_t*>(st.at(o_toff));
         uint64_t qa = 0, ta = 0;
@@ -503,7 +510,6 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
             }
         } else {
             for (size_t k = 0; k < npairs; ++k) qo[k] = src.qoff[k] - src.qoff[0];
-            if (qtot) std::memcpy(qd, src.qflat + src.qoff[0], qtot);
             qa = qtot;
         }
         if (src.tp) {
@@ -514,17 +520,11 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
             }
         } else {
             for (size_t k = 0; k < npairs; ++k) to[k] = src.toff[k] - src.toff[0];
-            if (ttot) std::memcpy(td, src.tflat + src.toff[0], ttot);
             ta = ttot;
         }
         qo[npairs] = qa;
         to[npairs] = ta;
-        // Symbols: one pass over the staged bytes; the exact per-pair order only on a failure.
-        if (std::max(max_byte(qd, qa), max_byte(td, ta)) >= alphabet_len) {
-            rc = exact_checks();
-            if (rc != kChecksPassed) return rc;
-            return HY_E_INTERNAL;
-        }
+
         if (!waves.empty()) {
             uint32_t* q32 = reinterpret_cast<uint32_t*>(st.at(o_q32));
             uint32_t* l32 = reinterpret_cast<uint32_t*>(st.at(o_l32));
@@ -548,6 +548,24 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         }
     }
     if ((rc = st.commit()) != HY_OK) return rc;
+    // Symbols: the largest byte of both sides, found on the device right after the upload and
+    // checked before any kernel indexes the matrix with them; the exact per-pair order of the
+    // reference's errors is recovered on the host only when one is out of range.
+    if ((rc = grow(s->flag, sizeof(uint32_t))) != HY_OK) return rc;
+    if (hy::launch_fill_i32(static_cast<int32_t*>(s->flag.p), 1, 0, s->stream) != hipSuccess ||
+        hy::launch_max_byte(static_cast<const uint8_t*>(s->qflat.p), qtot,
+                            static_cast<uint32_t*>(s->flag.p), s->stream) != hipSuccess ||
+        hy::launch_max_byte(static_cast<const uint8_t*>(s->tflat.p), ttot,
+                            static_cast<uint32_t*>(s->flag.p), s->stream) != hipSuccess ||
+        hipMemcpyAsync(s->h_flag, s->flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                       s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    if (*s->h_flag >= alphabet_len) {
+        rc = exact_checks();
+        if (rc != kChecksPassed) return rc;
+        return HY_E_INTERNAL;
+    }
     if ((rc = grow(s->scores, npairs * sizeof(int32_t))) != HY_OK ||
         (rc = grow(s->qend, npairs * sizeof(int64_t))) != HY_OK ||
         (rc = grow(s->tend, npairs * sizeof(int64_t))) != HY_OK)
