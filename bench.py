@@ -571,10 +571,22 @@ def main() -> int:
     s360 = (time.perf_counter() - t0) / 10
     inter_ms, tail_ms, n360 = sc.profile_read(reset=True)
     sc.profile(False)
+    # Per-target hits with their ends (scan_all, ScoreEnd) for the same query.
+    db.scan_all_arrays(sc, q360)
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        all_s, all_qe, all_te = db.scan_all_arrays(sc, q360)
+    sync()
+    sall = (time.perf_counter() - t0) / 5
+    if (all_s[h360.db_index], all_te[h360.db_index]) != (h360.score, h360.target_end):
+        raise RuntimeError("scan_all disagrees with scan on the winner")
     single = {"workload": f"protein_sw_scoreend_360x{w.n}", "gcups": round(
         len(q360) * w.residues / s360 / 1e9, 1), "ms_per_scan": round(s360 * 1e3, 3),
         "inter_kernel_ms": round(inter_ms / max(n360, 1), 3),
         "tail_kernel_ms": round(tail_ms / max(n360, 1), 3),
+        "scan_all_ends_ms": round(sall * 1e3, 3),
+        "scan_all_ends_gcups": round(len(q360) * w.residues / sall / 1e9, 1),
         "hit": {"score": h360.score, "db_index": h360.db_index, "query_end": h360.query_end,
                 "target_end": h360.target_end}}
 

@@ -671,12 +671,18 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     }
 
     if (s->profile && hipEventRecord(s->ev[0], s->stream) != hipSuccess) return HY_E_DEVICE;
-    // End tracking runs every non-long target at 32 bits; score-only scans send the targets the
-    // packed kernel accepts there instead.
-    const uint32_t g32 = ends ? (uint32_t)db->groups.size() : db->groups_score32;
+    // Score-only scans run the targets the packed kernel accepts there and the rest at 32 bits.
+    // End tracking does the same in local mode (the packed kernel's end-tracking form keeps
+    // columns in 16 bits, so groups must stay within 65,535 columns); the other modes track ends
+    // for every non-long target at 32 bits.
+    const bool has16 = db->groups16.size() > db->groups16_short_first;
+    const bool packed_ends = ends && db->mode == HY_MODE_SW && has16 &&
+                             db->groups16[db->groups16_short_first].ncols <= 65535u;
+    const uint32_t g32 =
+        (ends && !packed_ends) ? (uint32_t)db->groups.size() : db->groups_score32;
     if (hy::launch_inter32((int)db->mode, ends, ia, g32, s->stream) != hipSuccess)
         return HY_E_DEVICE;
-    if (!ends && db->groups16.size() > db->groups16_short_first) {
+    if ((!ends || packed_ends) && has16) {
         hy::Inter16Args pa{};
         pa.groups = db->d_groups16;
         pa.group_first = db->groups16_short_first;
@@ -694,8 +700,15 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
         pa.spill = reinterpret_cast<uint2*>(s->d_spill);
         pa.out_score = ia.out_score;
         pa.key = key;
-        if (hy::launch_inter16((int)db->mode, pa, s->q16, s->waves16, s->stream) != hipSuccess)
+        if (ends) {
+            pa.key = nullptr;
+            pa.out_qend = s->d_qend;
+            pa.out_tend = s->d_tend;
+            if (hy::launch_inter16_ends_sw(pa, s->stream) != hipSuccess) return HY_E_DEVICE;
+        } else if (hy::launch_inter16((int)db->mode, pa, s->q16, s->waves16, s->stream) !=
+                   hipSuccess) {
             return HY_E_DEVICE;
+        }
     }
     if (s->profile && hipEventRecord(s->ev[1], s->stream) != hipSuccess) return HY_E_DEVICE;
     if (tail && hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess) return HY_E_DEVICE;

@@ -83,6 +83,8 @@ struct Inter16Args {
     uint2* spill;            // spill[spill_off + col*64 + lane] = (H - go pair, F pair)
     int32_t* out_score;      // database order (may be null)
     unsigned long long* key; // fused argmax (may be null)
+    int64_t* out_qend;       // end-tracking launches (local mode): database order
+    int64_t* out_tend;
 };
 
 // Batched form: queries concatenated in one device buffer, processed by the persistent kernel.
@@ -312,5 +314,8 @@ template <int MODE>
 hipError_t launch_pairs32_m(bool ends, const PairArgs& a, uint32_t ngroups, int q,
                             hipStream_t st);
 hipError_t launch_posmax_sw(const PosMaxArgs& a, hipStream_t st);
+// The packed one-query kernel with end tracking (local mode; hy_kernels.hip compiled for SW):
+// score and ends of every target of the launch's groups into out_score / out_qend / out_tend.
+hipError_t launch_inter16_ends_sw(const Inter16Args& a, hipStream_t st);
 
 }  // namespace hy

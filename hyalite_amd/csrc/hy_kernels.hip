@@ -222,6 +222,23 @@ __device__ __forceinline__ s2v p_subu(s2v a, s2v b) {
     return __builtin_bit_cast(s2v, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2v, a),
                                                                  __builtin_bit_cast(u2v, b)));
 }
+__device__ __forceinline__ s2v p_minu(s2v a, s2v b) {
+    return __builtin_bit_cast(s2v, __builtin_elementwise_min(__builtin_bit_cast(u2v, a),
+                                                             __builtin_bit_cast(u2v, b)));
+}
+__device__ __forceinline__ s2v p_maxu(s2v a, s2v b) {
+    return __builtin_bit_cast(s2v, __builtin_elementwise_max(__builtin_bit_cast(u2v, a),
+                                                             __builtin_bit_cast(u2v, b)));
+}
+// a - b per 16-bit half, wrapping (v_pk_sub_u16 without clamp).
+__device__ __forceinline__ s2v p_subw(s2v a, s2v b) {
+    return __builtin_bit_cast(s2v, __builtin_bit_cast(u2v, a) - __builtin_bit_cast(u2v, b));
+}
+// a * b + c per 16-bit half, wrapping (v_pk_mul_lo_u16 / v_pk_mad_u16).
+__device__ __forceinline__ s2v p_madu(s2v a, s2v b, s2v c) {
+    return __builtin_bit_cast(s2v, __builtin_bit_cast(u2v, a) * __builtin_bit_cast(u2v, b) +
+                                       __builtin_bit_cast(u2v, c));
+}
 __device__ __forceinline__ int sat16(long long v) {
     return v < -32768 ? -32768 : (v > 32767 ? 32767 : (int)v);
 }
@@ -261,17 +278,21 @@ struct PairRun {
     // Local mode keeps four independent running maxima (by row mod 4) so the tracker is not one
     // long chain of dependent packed maxima.
     s2v runq[4];
+    // End tracking (local mode): each target's best (score, column, row) so far, merged tile by
+    // tile with the answer tie-break.
+    int bsA, bjA, biA, bsB, bjB, biB;
 };
 
 // One tile of QT query rows starting at row i0 over every column of the group: the profile of
 // those rows is staged in LDS, the rows' H/E columns live in registers, and the tile's bottom row
 // goes to `sp` (column j of this wavefront at sp[j * 64]) for the next tile unless this is the last.
-template <int MODE, int QT, int NW, bool PAIR, bool FIRST>
+template <int MODE, int QT, int NW, bool PAIR, bool FIRST, bool ENDS = false>
 __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& g, bool wlive,
                                              const uint8_t* query, int m, int i0, bool lastc,
                                              int nA, int nB, uint2* sp, const uint2* wp,
                                              uint32_t* pp, PairRun& st) {
     using F = MF<MODE>;
+    static_assert(!ENDS || F::local, "packed end tracking is the local-mode form");
     constexpr int S = QT + 4;
     // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
     // different symbols start in different 4-bank groups.
@@ -326,6 +347,12 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                        : splat2(sat16((long long)border16<MODE>(i0 - 1, 0, go, ge) - go));
     s2v run = st.run;
     s2v runq[4] = {st.runq[0], st.runq[1], st.runq[2], st.runq[3]};
+    // End tracking: the tile's best score per target, its first column and, in that column, its
+    // first row (0-based in the tile). Cells are met column by column in increasing order and, in
+    // a column, row by row, which is the answer tie-break's order, so strictly greater is the only
+    // update at every level.
+    s2v tb = splat2(0), tc = splat2(0), tr = splat2(0);
+    const s2v one2 = splat2(1);
 
     // Column words and boundary inputs stream one four-column block ahead: the loads for block
     // k+1 are issued before block k's compute and pinned there by a scheduling barrier (the
@@ -396,14 +423,30 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
         // column j+1 are independent — so two dependency chains interleave. The boundary row of a
         // tile is (H of its last row, F' of the row below).
         if (F::local) {
-            auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f) HY_INLINE {
+            // With ENDS the running maximum becomes a per-column one that also keeps the first
+            // row reaching it: cm = max(cm, h); cr = max(cr, [h > cm] * r) (rows only grow within
+            // a column, so the latest strict improvement has the largest row).
+            auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f, s2v& cm, s2v& cr) HY_INLINE {
                 const s2v h = p_max(p_max(p_add(hd, sgv), E[r]), f);
                 hd = HG[r];
                 HG[r] = h;
                 const s2v hg = p_subu(h, go2);
                 E[r] = p_max(hg, p_subu(E[r], ge2));
                 f = p_max(hg, p_subu(f, ge2));
-                runq[r & 3] = p_max(runq[r & 3], h);
+                if constexpr (ENDS) {
+                    const s2v up = p_minu(p_subu(h, cm), one2);
+                    cm = p_max(cm, h);
+                    cr = p_maxu(cr, p_madu(up, splat2(r), splat2(0)));
+                } else {
+                    runq[r & 3] = p_max(runq[r & 3], h);
+                }
+            };
+            // A column's (max, first row) folded into the tile's best: strictly greater only.
+            auto close_column = [&](int j, s2v cm, s2v cr) HY_INLINE {
+                const s2v up = p_minu(p_subu(cm, tb), one2);
+                tb = p_max(tb, cm);
+                tc = p_maxu(tc, p_madu(up, splat2(j), splat2(0)));
+                tr = p_madu(up, p_subw(cr, tr), tr);
             };
 #pragma unroll
             for (int bp = 0; bp < 4; bp += 2) {
@@ -413,13 +456,18 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                 s2v fA = as_s2(spv[bp].y), fB = as_s2(spv[bp + 1].y);
                 s2v hdA = HGd, hdB = as_s2(spv[bp].x);
                 HGd = as_s2(spv[bp + 1].x);
+                s2v cmA = splat2(0), crA = splat2(0), cmB = splat2(0), crB = splat2(0);
 #pragma unroll
                 for (int r = 0; r <= QT; ++r) {
-                    if (r < QT) cell(r, sgA[r], hdA, fA);
+                    if (r < QT) cell(r, sgA[r], hdA, fA, cmA, crA);
                     if (r == QT - 1) outb[bp] = make_uint2(as_u32(HG[QT - 1]), as_u32(fA));
-                    if (r >= 1) cell(r - 1, sgB[r - 1], hdB, fB);
+                    if (r >= 1) cell(r - 1, sgB[r - 1], hdB, fB, cmB, crB);
                 }
                 outb[bp + 1] = make_uint2(as_u32(HG[QT - 1]), as_u32(fB));
+                if constexpr (ENDS) {
+                    close_column(jb + bp + 1, cmA, crA);
+                    close_column(jb + bp + 2, cmB, crB);
+                }
             }
         } else {
             // One column with its answer cells: the last row in the last tile (every column up
@@ -520,21 +568,29 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     st.run = run;
 #pragma unroll
     for (int k = 0; k < 4; ++k) st.runq[k] = runq[k];
+    if constexpr (ENDS) {
+        // The tile's best of each target joins the target's best with the full tie-break: a
+        // later tile can hold an equal score in an earlier column.
+        const uint32_t b = as_u32(tb), c = as_u32(tc), r = as_u32(tr);
+        const int sa = (int)(b & 0xFFFFu), sb = (int)(b >> 16);
+        if (sa > 0) consider(st.bsA, st.bjA, st.biA, sa, (int)(c & 0xFFFFu), i0 + (int)(r & 0xFFFFu));
+        if (sb > 0) consider(st.bsB, st.bjB, st.biB, sb, (int)(c >> 16), i0 + (int)(r >> 16));
+    }
 }
 
 // The tile above with FIRST chosen at run time (the first tile of a group starts from the DP
 // border instead of a boundary row).
-template <int MODE, int QT, int NW, bool PAIR>
+template <int MODE, int QT, int NW, bool PAIR, bool ENDS = false>
 __device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Group& g, bool wlive,
                                                 const uint8_t* query, int m, int i0, bool first,
                                                 bool lastc, int nA, int nB, uint2* sp,
                                                 const uint2* wp, uint32_t* pp, PairRun& st) {
     if (first)
-        inter16_tile<MODE, QT, NW, PAIR, true>(a, g, wlive, query, m, i0, lastc, nA, nB, sp, wp,
-                                               pp, st);
+        inter16_tile<MODE, QT, NW, PAIR, true, ENDS>(a, g, wlive, query, m, i0, lastc, nA, nB,
+                                                     sp, wp, pp, st);
     else
-        inter16_tile<MODE, QT, NW, PAIR, false>(a, g, wlive, query, m, i0, lastc, nA, nB, sp, wp,
-                                                pp, st);
+        inter16_tile<MODE, QT, NW, PAIR, false, ENDS>(a, g, wlive, query, m, i0, lastc, nA, nB,
+                                                      sp, wp, pp, st);
 }
 
 // One group of 128 targets against one query: full tiles of Q rows, then one tile of the fewest
@@ -543,7 +599,7 @@ __device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Grou
 // two targets' scores go to out_score and their best key is folded into *key. Called by the
 // one-shot kernel (one group per wavefront) and by the persistent batch kernel (a queue of
 // (group, query) items).
-template <int MODE, int Q, int NW, bool PAIR>
+template <int MODE, int Q, int NW, bool PAIR, bool ENDS = false>
 __device__ __forceinline__ void inter16_group(const Inter16Args& a, const Group& g, bool wlive,
                                               const uint8_t* query, int m, uint2* spill,
                                               unsigned long long* key, uint32_t* pp) {
@@ -557,9 +613,11 @@ __device__ __forceinline__ void inter16_group(const Inter16Args& a, const Group&
     const int go = a.go, ge = a.ge;
 
     int bsA, bsB, dj, di;
-    border_best<MODE>(m, nA, go, ge, bsA, dj, di);
-    border_best<MODE>(m, nB, go, ge, bsB, dj, di);
     PairRun st;
+    border_best<MODE>(m, nA, go, ge, bsA, dj, di);
+    st.bsA = bsA; st.bjA = dj; st.biA = di;
+    border_best<MODE>(m, nB, go, ge, bsB, dj, di);
+    st.bsB = bsB; st.bjB = dj; st.biB = di;
     st.run = splat2(-32768);
 #pragma unroll
     for (int k = 0; k < 4; ++k) st.runq[k] = st.run;
@@ -568,25 +626,42 @@ __device__ __forceinline__ void inter16_group(const Inter16Args& a, const Group&
     const uint2* wp = a.words + g.word_off + lane;
     const int nfull = m / Q, rem = m - nfull * Q;
     for (int c = 0; c < nfull; ++c)
-        inter16_tile_at<MODE, Q, NW, PAIR>(a, g, wlive, query, m, c * Q + 1, c == 0,
-                                           rem == 0 && c == nfull - 1, nA, nB, sp, wp, pp, st);
+        inter16_tile_at<MODE, Q, NW, PAIR, ENDS>(a, g, wlive, query, m, c * Q + 1, c == 0,
+                                                 rem == 0 && c == nfull - 1, nA, nB, sp, wp, pp,
+                                                 st);
     if (rem > 0) {
         const int i0 = nfull * Q + 1;
         const bool first = nfull == 0;
         if (rem <= 8)
-            inter16_tile_at<MODE, 8, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
-                                               wp, pp, st);
+            inter16_tile_at<MODE, 8, NW, PAIR, ENDS>(a, g, wlive, query, m, i0, first, true, nA,
+                                                     nB, sp, wp, pp, st);
         else if (Q > 16 && rem <= 16)
-            inter16_tile_at<MODE, 16, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB,
-                                                sp, wp, pp, st);
+            inter16_tile_at<MODE, 16, NW, PAIR, ENDS>(a, g, wlive, query, m, i0, first, true, nA,
+                                                      nB, sp, wp, pp, st);
         else if (Q > 24 && rem <= 24)
-            inter16_tile_at<MODE, 24, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB,
-                                                sp, wp, pp, st);
+            inter16_tile_at<MODE, 24, NW, PAIR, ENDS>(a, g, wlive, query, m, i0, first, true, nA,
+                                                      nB, sp, wp, pp, st);
         else
-            inter16_tile_at<MODE, Q, NW, PAIR>(a, g, wlive, query, m, i0, first, true, nA, nB, sp,
-                                               wp, pp, st);
+            inter16_tile_at<MODE, Q, NW, PAIR, ENDS>(a, g, wlive, query, m, i0, first, true, nA,
+                                                     nB, sp, wp, pp, st);
     }
     if (!wlive) return;
+    if constexpr (ENDS) {
+        // Per target: score, then the 0-based inclusive ends (-1 on the border).
+        if (liveA) {
+            const uint32_t gi = a.gidx[slotA];
+            a.out_score[gi] = st.bsA;
+            a.out_qend[gi] = st.biA == 0 ? -1 : (int64_t)st.biA - 1;
+            a.out_tend[gi] = st.bjA == 0 ? -1 : (int64_t)st.bjA - 1;
+        }
+        if (liveB) {
+            const uint32_t gi = a.gidx[slotB];
+            a.out_score[gi] = st.bsB;
+            a.out_qend[gi] = st.biB == 0 ? -1 : (int64_t)st.biB - 1;
+            a.out_tend[gi] = st.bjB == 0 ? -1 : (int64_t)st.bjB - 1;
+        }
+        return;
+    }
     const s2v run =
         p_max(st.run, p_max(p_max(st.runq[0], st.runq[1]), p_max(st.runq[2], st.runq[3])));
     const int sA = max(bsA, (int)run.x), sB = max(bsB, (int)run.y);
@@ -605,7 +680,7 @@ __device__ __forceinline__ void inter16_group(const Inter16Args& a, const Group&
     if (key) wave_argmax(k, key);
 }
 
-template <int MODE, int Q, int NW, bool PAIR>
+template <int MODE, int Q, int NW, bool PAIR, bool ENDS = false>
 __global__ __launch_bounds__(64 * NW)
 __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inter16Args a) {
     extern __shared__ __attribute__((aligned(16))) uint32_t pp[];
@@ -613,7 +688,8 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inte
     const bool wlive = gix < a.group_first + a.ngroups;
     Group g{};
     if (wlive) g = a.groups[gix];
-    inter16_group<MODE, Q, NW, PAIR>(a, g, wlive, a.query, a.m, a.spill + g.spill_off, a.key, pp);
+    inter16_group<MODE, Q, NW, PAIR, ENDS>(a, g, wlive, a.query, a.m, a.spill + g.spill_off,
+                                           a.key, pp);
 }
 
 // Persistent batch kernel: each one-wavefront workgroup takes (group, query) items from a device
@@ -1120,6 +1196,17 @@ uint32_t inter16_batch_occupancy_m(int q, int al_eff) {
 }
 
 #if HY_KMODE == 0
+// End tracking in the packed one-query kernel: 32-row tiles, one wavefront per workgroup with
+// the per-symbol profile (the one-query default shape).
+hipError_t launch_inter16_ends_sw(const Inter16Args& a, hipStream_t st) {
+    if (a.ngroups == 0) return hipSuccess;
+    constexpr int Q = 32;
+    const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
+    hipLaunchKernelGGL((k_inter16<kSW, Q, 1, false, true>), dim3(a.ngroups), dim3(64), lds, st,
+                       a);
+    return hipGetLastError();
+}
+
 uint32_t inter16q_lds_bytes(int al_eff, int q) {
     return ((uint32_t)al_eff + 1) * ((uint32_t)inter16_tile_rows(q) + 4) * 4u;
 }
