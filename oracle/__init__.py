@@ -91,6 +91,11 @@ def _load():
     lib.hyo_score2.argtypes = [_i32p, ctypes.c_uint64, ctypes.c_int32, ctypes.c_uint64,
                                ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32),
                                ctypes.POINTER(ctypes.c_uint64)]
+    lib.hyo_sse_striped_sw_pairs.restype = ctypes.c_int
+    lib.hyo_sse_striped_sw_pairs.argtypes = [_u8p, _u64p, _u8p, _u64p, ctypes.c_uint64,
+                                             ctypes.c_uint32, _i32p, ctypes.c_int32,
+                                             ctypes.c_int32, _i32p,
+                                             ctypes.POINTER(ctypes.c_double)]
     lib.hyo_brute_colmax.restype = None
     lib.hyo_brute_colmax.argtypes = [_u8p, ctypes.c_uint64, _u8p, ctypes.c_uint64,
                                      ctypes.c_uint32, _i32p, ctypes.c_int32, ctypes.c_int32, _i32p]
@@ -205,6 +210,24 @@ def score2(colmax, best_score: int, best_target_end: int, matrix_max: int, min_s
     found = _load().hyo_score2(_ptr(a, _i32p), len(colmax), best_score, best_target_end,
                                matrix_max, min_score, ctypes.byref(s), ctypes.byref(p))
     return (s.value, p.value) if found else None
+
+
+def sse_striped_sw_pairs(qflat, qoff, tflat, toff, al: int, matrix, go: int, ge: int):
+    """(SW scores of the pairs, seconds): the SSE4.1 striped restatement of the reference's
+    single-pair path, one thread. Only for pairs whose width proof gives at most 16 bits."""
+    qf = _u8(qflat) if len(qflat) else np.zeros(1, dtype=np.uint8)
+    tf = _u8(tflat) if len(tflat) else np.zeros(1, dtype=np.uint8)
+    qo = np.ascontiguousarray(qoff, dtype=np.uint64)
+    to = np.ascontiguousarray(toff, dtype=np.uint64)
+    n = len(qo) - 1
+    out = np.zeros(max(n, 1), dtype=np.int32)
+    secs = ctypes.c_double(0)
+    rc = _load().hyo_sse_striped_sw_pairs(_ptr(qf, _u8p), _ptr(qo, _u64p), _ptr(tf, _u8p),
+                                          _ptr(to, _u64p), n, al, _ptr(_i32(matrix), _i32p), go,
+                                          ge, _ptr(out, _i32p), ctypes.byref(secs))
+    if rc != 0:
+        raise MemoryError("oracle allocation failed")
+    return out[:n], secs.value
 
 
 def brute_colmax(q, t, al: int, matrix, go: int, ge: int):

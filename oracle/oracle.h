@@ -63,6 +63,12 @@ int hyo_position_max(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n,
 /* src/kernel.rs:741-792: 1 with (score2, pos) when a secondary peak qualifies, else 0. */
 int hyo_score2(const int32_t* colmax, uint64_t n, int32_t best_score, uint64_t best_tend,
                int32_t matrix_max, int32_t min_score, int32_t* s2, uint64_t* pos);
+/* SSE4.1 striped (Farrar) local-mode scores of n pairs, one thread (the CPU baseline of the
+ * single-pair path); 0 or -1 on allocation failure. */
+int hyo_sse_striped_sw_pairs(const uint8_t* qflat, const uint64_t* qoff, const uint8_t* tflat,
+                             const uint64_t* toff, uint64_t npairs, uint32_t al,
+                             const int32_t* mat, int32_t go, int32_t ge, int32_t* scores,
+                             double* seconds);
 /* Column maxima by enumeration of local alignments ending in each column (tiny inputs). */
 void hyo_brute_colmax(const uint8_t* q, uint64_t m, const uint8_t* t, uint64_t n, uint32_t al,
                       const int32_t* mat, int32_t go, int32_t ge, int32_t* out);

@@ -111,3 +111,26 @@ def test_score2_matches_the_peak_list_transcription():
         colmax = [rng.randint(0, 20) for _ in range(rng.randint(0, 40))]
         args = (rng.randint(0, 40), rng.randint(0, 40), rng.randint(-2, 8), rng.randint(1, 6))
         assert orc.score2(colmax, *args) == score2_peak_list(colmax, *args), (colmax, args)
+
+
+# ---- the striped CPU baseline of the single-pair path --------------------------------------------
+
+def test_sse_striped_baseline_matches_the_scalar_oracle():
+    """The SSE4.1 striped (Farrar) restatement used as the pairs CPU baseline: every score equals
+    the scalar oracle's, including the lazy-F carries across stripes (queries of 1-80 rows over 8
+    lanes), empty sides and pairs that saturate nothing at 16 bits."""
+    import numpy as np
+    rng = np.random.default_rng(2)
+    for al, m_hi, go, ge in ((4, 80, 2, 1), (4, 80, 5, 1), (20, 60, 11, 1), (3, 40, 1, 1),
+                             (4, 40, 3, 0)):
+        mat = rng.integers(-4, 6, size=al * al).astype(np.int32)
+        lq = rng.integers(0, m_hi, size=300)
+        lt = rng.integers(0, 120, size=300)
+        qf = rng.integers(0, al, size=int(lq.sum()), dtype=np.uint8)
+        tf = rng.integers(0, al, size=int(lt.sum()), dtype=np.uint8)
+        qo = np.concatenate([[0], np.cumsum(lq)]).astype(np.uint64)
+        to = np.concatenate([[0], np.cumsum(lt)]).astype(np.uint64)
+        got, _ = orc.sse_striped_sw_pairs(qf, qo, tf, to, al, mat, go, ge)
+        for k in range(300):
+            q, t = qf[qo[k]:qo[k + 1]], tf[to[k]:to[k + 1]]
+            assert got[k] == orc.align_core(q, t, al, mat, go, ge, MODE_SW)[0], (al, k)
