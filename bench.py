@@ -393,29 +393,109 @@ def pairs(sync) -> tuple[dict, dict]:
         out[b.name] = entry
         log(f"pairs {b.name}: {entry['score']['gcups_device']} / "
             f"{entry['score_end']['gcups_device']} GCUPS on the device (Score / ScoreEnd)")
-    rng = np.random.default_rng(8)
-    p = wl.protein_freqs()
-    prot = hy.Scoring(24, wl.protein_matrix(2), 11, 1)
-    single = []
-    for m, n in ((150, 500), (360, 885), (1000, 10000)):
-        q = wl._draw(rng, m, p)
-        t = wl._draw(rng, n, p)
-
-        def lat(f, k=20):
-            f()
-            sync()
-            t0 = time.perf_counter()
-            for _ in range(k):
-                f()
-            sync()
-            return (time.perf_counter() - t0) / k * 1e3
-        single.append({"pair": [m, n],
-                       "align_pair_ms": round(lat(lambda: hy.align_pair_with(
-                           sc, q, t, prot, hy.Mode.Sw, hy.SearchType.ScoreEnd)), 3),
-                       "position_max_ms": round(lat(lambda: hy.align_pair_position_max_with(
-                           sc, q, t, prot, [])), 3)})
+    single, small = single_pairs(sc, sync)
     out["single_pairs"] = single
+    keep["_single"] = small
     return out, keep
+
+
+# The reference's pairwise bench (benches/pairwise.rs): DNA, match +2 / mismatch -1, gap_open 2,
+# gap_ext 1, random sequences; align_pair at 30 x 30, 150 x 150 and 150 x 1,400 (Score and
+# ScoreEnd), align_pair_position_max at 150 x 1,400, align_pairs over 256 pairs of 30 x 30. One
+# long pair (1,000 x 10,000) is added.
+REF_PAIR_SHAPES = ((30, 30), (150, 150), (150, 1400), (1000, 10000))
+
+
+def dna_pair_scoring():
+    return hy.Scoring(4, [2 if i == j else -1 for i in range(4) for j in range(4)], 2, 1)
+
+
+def single_pairs(sc, sync):
+    """Median latency (host call to result, us) of align_pair_with (Score, ScoreEnd) and
+    align_pair_position_max_with per shape, after a second of warm-up, and of align_pairs over
+    the 256 x 30 x 30 batch. Returns the rows and the sequences for the CPU leg."""
+    rng = np.random.default_rng(8)
+    s = dna_pair_scoring()
+    q0 = rng.integers(0, 4, 1000, dtype=np.uint8)
+    t0 = rng.integers(0, 4, 20000, dtype=np.uint8)
+    end = time.time() + 1.0
+    while time.time() < end:
+        hy.align_pair_with(sc, q0, t0, s, hy.Mode.Sw, hy.SearchType.Score)
+
+    def med(f, k):
+        f()
+        sync()
+        ts = []
+        for _ in range(k):
+            t1 = time.perf_counter()
+            f()
+            ts.append(time.perf_counter() - t1)
+        return round(float(np.median(ts)) * 1e6, 1)
+
+    rows, small = [], {}
+    for m, n in REF_PAIR_SHAPES:
+        q = rng.integers(0, 4, m, dtype=np.uint8)
+        t = rng.integers(0, 4, n, dtype=np.uint8)
+        k = 50 if m * n < 1_000_000 else 10
+        row = {"pair": [m, n],
+               "score_us": med(lambda: hy.align_pair_with(sc, q, t, s, hy.Mode.Sw,
+                                                          hy.SearchType.Score), k),
+               "score_end_us": med(lambda: hy.align_pair_with(sc, q, t, s, hy.Mode.Sw,
+                                                              hy.SearchType.ScoreEnd), k),
+               "position_max_us": med(lambda: hy.align_pair_position_max_with(sc, q, t, s, []),
+                                      k)}
+        row["score_end_gcups"] = round(m * n / row["score_end_us"] / 1e3, 3)
+        rows.append(row)
+        small[(m, n)] = (q, t)
+    qs = rng.integers(0, 4, 256 * 30, dtype=np.uint8)
+    ts = rng.integers(0, 4, 256 * 30, dtype=np.uint8)
+    off = np.arange(257, dtype=np.uint64) * 30
+    rows.append({"batch": "align_pairs 256 x 30 x 30", "score_us": med(
+        lambda: hy.align_pairs_flat(qs, off, ts, off, s, hy.Mode.Sw, hy.SearchType.Score,
+                                    scratch=sc), 50)})
+    small["batch"] = (qs, ts, off)
+    log("single pairs: " + "; ".join(
+        f"{r['pair'][0]}x{r['pair'][1]} {r['score_us']}/{r['score_end_us']}/"
+        f"{r['position_max_us']} us" for r in rows if "pair" in r))
+    return rows, small
+
+
+def cpu_single_pairs(small: dict) -> dict:
+    """The CPU side of single_pairs on the same sequences, one thread: Score by the SSE4.1
+    striped restatement at the width the proof gives (repeated to 50 ms), ScoreEnd and the column
+    maxima by the scalar restatements (the reference's CPU path for them at these widths is
+    scalar for ScoreEnd; its position maxima run striped, so that figure is a scalar bound)."""
+    import oracle
+    s = dna_pair_scoring()
+    al, mat = 4, np.asarray(s.matrix(), dtype=np.int32)
+    out = []
+    for (m, n), (q, t) in ((k, v) for k, v in small.items() if k != "batch"):
+        qo = np.array([0, m], dtype=np.uint64)
+        to = np.array([0, n], dtype=np.uint64)
+        reps, secs = 0, 0.0
+        while secs < 0.05:
+            _, dt = oracle.sse_striped_sw_pairs(q, qo, t, to, al, mat, 2, 1)
+            secs += dt
+            reps += 1
+        t1 = time.perf_counter()
+        oracle.align_core(q, t, al, mat, 2, 1, 0)
+        se = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        oracle.position_max(q, t, al, mat, 2, 1)
+        pm = time.perf_counter() - t1
+        out.append({"pair": [m, n], "score_striped_us": round(secs / reps * 1e6, 1),
+                    "score_end_scalar_us": round(se * 1e6, 1),
+                    "position_max_scalar_us": round(pm * 1e6, 1)})
+    qs, ts, off = small["batch"]
+    reps, secs = 0, 0.0
+    while secs < 0.05:
+        _, dt = oracle.sse_striped_sw_pairs(qs, off, ts, off, al, mat, 2, 1)
+        secs += dt
+        reps += 1
+    out.append({"batch": "align_pairs 256 x 30 x 30",
+                "score_striped_us": round(secs / reps * 1e6, 1)})
+    return {"rows": out, "cores": 1, "kind": "port",
+            "sample": "the single pairs and 256-pair batch of the device rows, same sequences"}
 
 
 def cpu_pairs(keep: dict) -> dict:
@@ -425,7 +505,9 @@ def cpu_pairs(keep: dict) -> dict:
     oracle's align_core) on 1,000 pairs spread over the batch."""
     import oracle
     res = {}
-    for name, (b, (scores, qe, te)) in keep.items():
+    if "_single" in keep:
+        res["single_pairs"] = cpu_single_pairs(keep["_single"])
+    for name, (b, (scores, qe, te)) in ((k, v) for k, v in keep.items() if k != "_single"):
         n = min(b.n, 40_000 if b.al <= 4 else 20_000)
         qo, to = b.qoff[:n + 1], b.toff[:n + 1]
         got, ssecs = oracle.sse_striped_sw_pairs(b.queries[:int(qo[-1])], qo,

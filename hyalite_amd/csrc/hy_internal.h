@@ -236,6 +236,43 @@ struct PosMaxArgs {
     int2* best;               // (score, target end or -1)
 };
 
+// Multi-wavefront pair kernel (k_band32): block b aligns pair list[b] with W wavefronts of one
+// workgroup, wavefront w holding query rows w*64R+1 .. (w+1)*64R of each pass of 64*R*W rows and
+// running a fixed lag behind wavefront w-1, whose bottom row it reads from an LDS ring. Outputs:
+// the score (and ends) per pair, or the local-mode column maxima of a single pair (zeroed by the
+// kernel) with the best and its first column.
+enum { kBandScore = 0, kBandEnds = 1, kBandColmax = 2 };
+struct BandArgs {
+    const uint8_t* queries;
+    const uint32_t* qoff;     // per pair
+    const uint32_t* qlen;
+    const uint8_t* targets;
+    const uint64_t* toff;     // per pair, n + 1 entries
+    const uint32_t* list;     // block b -> pair index
+    const int32_t* matrix;
+    int al, go, ge;
+    int2* spill;              // per block: spill[blockIdx.x * spill_stride + col], longer queries
+    uint64_t spill_stride;
+    int32_t* out_score;       // per pair
+    int64_t* out_qend;
+    int64_t* out_tend;
+    int32_t* colmax;          // kBandColmax: n entries of the one pair
+    int2* best;               // kBandColmax: (score, target end or -1)
+};
+// Alphabets the band kernel takes (its substitution matrix and rings fit 64 KB of LDS).
+constexpr uint32_t kBandMaxAlphabet = 63;
+// Launch shape of the band kernel for a longest query of m rows: r rows per lane, w wavefronts
+// (one pass covers 64 * r * w rows). HYALITE_HIP_BAND_SHAPE="r,w" overrides it (shapes the
+// launcher instantiates only).
+struct BandShape {
+    int r, w;
+};
+BandShape band_shape(int m);
+inline int band_rows_per_pass(int m) {
+    const BandShape s = band_shape(m);
+    return 64 * s.r * s.w;
+}
+
 // Launchers (hy_launch.hip). All launch on `stream` and never synchronise.
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
                            hipStream_t stream);
@@ -258,6 +295,10 @@ hipError_t launch_pairs32(int mode, bool ends, const PairArgs& a, uint32_t ngrou
                           hipStream_t stream);
 // Column maxima of one local-mode pair (zeroes colmax first).
 hipError_t launch_posmax(const PosMaxArgs& a, hipStream_t stream);
+// Band kernel over nblocks pairs whose longest query has m rows (it fixes W and R); alphabets up
+// to kBandMaxAlphabet. out: kBandScore, kBandEnds, or kBandColmax (local mode, one block).
+hipError_t launch_band32(int mode, int out, const BandArgs& a, uint32_t nblocks, int m,
+                         hipStream_t stream);
 hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t stream);
 // The largest byte of p[0, n) folded into *out (an atomic max; zero *out first).
 hipError_t launch_max_byte(const uint8_t* p, uint64_t n, uint32_t* out, hipStream_t stream);
@@ -314,6 +355,8 @@ template <int MODE>
 hipError_t launch_pairs32_m(bool ends, const PairArgs& a, uint32_t ngroups, int q,
                             hipStream_t st);
 hipError_t launch_posmax_sw(const PosMaxArgs& a, hipStream_t st);
+template <int MODE>
+hipError_t launch_band32_m(int out, const BandArgs& a, uint32_t nblocks, int m, hipStream_t st);
 // The packed one-query kernel with end tracking (local mode; hy_kernels.hip compiled for SW):
 // score and ends of every target of the launch's groups into out_score / out_qend / out_tend.
 hipError_t launch_inter16_ends_sw(const Inter16Args& a, hipStream_t st);

@@ -1,6 +1,8 @@
 // Mode-independent kernels (argmax keys, results, fills) and the dispatch from a runtime
 // alignment mode to the per-mode launchers that hy_kernels.hip exports (one object per mode).
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 
 #include "hy_device.h"
 
@@ -261,6 +263,50 @@ hipError_t launch_pairs32(int mode, bool ends, const PairArgs& a, uint32_t ngrou
     case kSHW: return launch_pairs32_m<kSHW>(ends, a, ngroups, q, stream);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_band32(int mode, int out, const BandArgs& a, uint32_t nblocks, int m,
+                         hipStream_t stream) {
+    if (nblocks == 0) return hipSuccess;
+    if (out == kBandColmax && (mode != kSW || nblocks != 1)) return hipErrorInvalidValue;
+    switch (mode) {
+    case kSW: return launch_band32_m<kSW>(out, a, nblocks, m, stream);
+    case kNW: return launch_band32_m<kNW>(out, a, nblocks, m, stream);
+    case kHW: return launch_band32_m<kHW>(out, a, nblocks, m, stream);
+    case kOV: return launch_band32_m<kOV>(out, a, nblocks, m, stream);
+    case kSHW: return launch_band32_m<kSHW>(out, a, nblocks, m, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+namespace {
+bool band_shape_known(int r, int w) {
+    switch (r * 100 + w) {
+    case 101: case 102: case 104: case 201: case 202: case 204: case 208: case 216:
+    case 401: case 402: case 404: case 408: case 416:
+        return true;
+    default:
+        return false;
+    }
+}
+}  // namespace
+
+BandShape band_shape(int m) {
+    if (const char* v = std::getenv("HYALITE_HIP_BAND_SHAPE")) {
+        int r = 0, w = 0;
+        if (std::sscanf(v, "%d,%d", &r, &w) == 2 && band_shape_known(r, w)) return {r, w};
+    }
+    // Measured on single DNA pairs (SW; 30 x 30 to 3,000 x 10,000): up to 256 rows one
+    // wavefront of up to four rows per lane is fastest (a lone wavefront is latency-bound, but
+    // the band pipeline's lag costs more on short targets); longer queries split into bands of
+    // 128-256 rows so that two to four wavefronts share each SIMD.
+    if (m <= 64) return {1, 1};
+    if (m <= 128) return {2, 1};
+    if (m <= 256) return {4, 1};
+    if (m <= 512) return {2, 4};
+    if (m <= 1024) return {2, 8};
+    if (m <= 2048) return {4, 8};
+    return {4, 16};
 }
 
 hipError_t launch_posmax(const PosMaxArgs& a, hipStream_t stream) {

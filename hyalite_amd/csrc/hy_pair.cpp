@@ -37,7 +37,7 @@ struct hy_pair_scratch {
     double device_ms = 0.0;
     uint64_t calls = 0;
     DevBuf mat, qflat, tflat, qoff, toff, qoff32, qlen32, list, nq, groups, twords, qwords, tlens,
-        qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best, flag;
+        qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best, flag, arena;
     uint32_t* h_flag = nullptr;  // pinned: the largest symbol of a batch
     uint8_t* stage = nullptr;  // pinned
     size_t stage_bytes = 0;
@@ -48,7 +48,7 @@ struct hy_pair_scratch {
         if (aux) (void)hipStreamSynchronize(aux);
         for (DevBuf* b : {&mat, &qflat, &tflat, &qoff, &toff, &qoff32, &qlen32, &list, &nq,
                           &groups, &twords, &qwords, &tlens, &qlens, &pidx, &spill, &sys_spill,
-                          &scores, &qend, &tend, &colmax, &best, &flag})
+                          &scores, &qend, &tend, &colmax, &best, &flag, &arena})
             hy::dev_free(b->p);
         if (h_flag) (void)hipHostFree(h_flag);
         if (stage) (void)hipHostFree(stage);
@@ -181,12 +181,147 @@ void stage_matrix(Stager& st, hy_pair_scratch* s, const int32_t* matrix, size_t 
 
 bool tracks_end(uint32_t search_type) { return search_type != HY_SEARCH_SCORE; }
 
-int lane_choice() {
+// HYALITE_HIP_PAIRS_KERNEL: "lanes" puts every pair of a batch one per lane (k_pairs32),
+// "waves" runs every pair one per wavefront (k_sys32 pair mode), "band" one per multi-wavefront
+// workgroup (k_band32, alphabets up to kBandMaxAlphabet). Unset: the choice below.
+enum { kPickAuto = 0, kPickLanes = 1, kPickWaves = 2, kPickBand = 3 };
+int kernel_choice() {
     const char* v = std::getenv("HYALITE_HIP_PAIRS_KERNEL");
-    if (!v) return 0;
-    if (std::strcmp(v, "lanes") == 0) return 1;
-    if (std::strcmp(v, "waves") == 0) return 2;
-    return 0;
+    if (!v) return kPickAuto;
+    if (std::strcmp(v, "lanes") == 0) return kPickLanes;
+    if (std::strcmp(v, "waves") == 0) return kPickWaves;
+    if (std::strcmp(v, "band") == 0) return kPickBand;
+    return kPickAuto;
+}
+
+bool band_eligible(uint32_t al) { return al >= 1 && al <= hy::kBandMaxAlphabet; }
+
+// Pairs that do not run one per lane go to the band kernel when the alphabet fits it, except
+// score-only batches of more than kBandMaxPairs: there one wavefront per pair fills the chip and
+// its leaner score-only loop wins. Measured (device time, SW, DNA): 16,384 pairs of 150 x 1,400
+// take 2.23 ms one per wavefront vs 2.71 ms in bands (Score) but 3.29 vs 2.94 ms with ends;
+// 2,048 pairs of 1,000 x 5,000 take 5.6 ms in bands vs 6.2 ms (Score) and 6.0 vs 10.3 ms (ends).
+constexpr size_t kBandMaxPairs = 4096;
+bool use_band(int pick, uint32_t al, size_t npairs, bool ends) {
+    if (!band_eligible(al) || pick == kPickWaves) return false;
+    return pick == kPickBand || ends || npairs <= kBandMaxPairs;
+}
+
+// Band kernel for the pairs in `idx` (ordered by rows-per-pass class): one launch per class,
+// the classes whose queries exceed one pass chunked so their boundary rows fit
+// kSysSpillBudget.
+int enqueue_band(hy_pair_scratch* s, const std::vector<uint32_t>& idx,
+                 const std::vector<size_t>& qlens, const std::vector<size_t>& tlens,
+                 const uint32_t* d_list, uint32_t al, int32_t go, int32_t ge, uint32_t mode,
+                 bool ends, hipStream_t stream) {
+    size_t b = 0;
+    while (b < idx.size()) {
+        const int c = hy::band_rows_per_pass((int)qlens[idx[b]]);
+        size_t e = b, maxm = 0, maxn = 0;
+        while (e < idx.size() && hy::band_rows_per_pass((int)qlens[idx[e]]) == c) {
+            maxm = std::max(maxm, qlens[idx[e]]);
+            maxn = std::max(maxn, tlens[idx[e]]);
+            ++e;
+        }
+        const bool multipass = maxm > (size_t)c;
+        const size_t per = multipass ? (maxn + 1) * sizeof(int2) : 0;
+        const size_t chunk = multipass ? std::max<size_t>(1, kSysSpillBudget / per) : (e - b);
+        if (multipass) {
+            const int rc = grow(s->sys_spill, std::min(chunk, e - b) * per);
+            if (rc != HY_OK) return rc;
+        }
+        for (size_t c0 = b; c0 < e; c0 += chunk) {
+            const size_t cnt = std::min(chunk, e - c0);
+            hy::BandArgs a{};
+            a.queries = static_cast<const uint8_t*>(s->qflat.p);
+            a.qoff = static_cast<const uint32_t*>(s->qoff32.p);
+            a.qlen = static_cast<const uint32_t*>(s->qlen32.p);
+            a.targets = static_cast<const uint8_t*>(s->tflat.p);
+            a.toff = static_cast<const uint64_t*>(s->toff.p);
+            a.list = d_list + c0;
+            a.matrix = static_cast<const int32_t*>(s->mat.p);
+            a.al = (int)al;
+            a.go = go;
+            a.ge = ge;
+            a.spill = static_cast<int2*>(s->sys_spill.p);
+            a.spill_stride = multipass ? maxn + 1 : 0;
+            a.out_score = static_cast<int32_t*>(s->scores.p);
+            a.out_qend = static_cast<int64_t*>(s->qend.p);
+            a.out_tend = static_cast<int64_t*>(s->tend.p);
+            if (hy::launch_band32((int)mode, ends ? hy::kBandEnds : hy::kBandScore, a,
+                                  (uint32_t)cnt, (int)maxm, stream) != hipSuccess)
+                return HY_E_DEVICE;
+        }
+        b = e;
+    }
+    return HY_OK;
+}
+
+// One pair through the band kernel with one upload and one copy back. The arena (device) and
+// its image at the start of the pinned stage hold: target offsets {0, n} at 0; list {0}, query
+// offset {0} and length {m} at 16; the matrix (clamped for the 32-bit kernels) at 32; the query
+// and the target; then, 16-byte aligned, the output: out_bytes written by the kernel and copied
+// back into the stage at the same offset.
+struct Single {
+    size_t q = 0, t = 0, out = 0, in_bytes = 0, total = 0;
+};
+
+int band_single(hy_pair_scratch* s, const uint8_t* query, size_t qlen, const uint8_t* target,
+                size_t tlen, const int32_t* matrix, size_t matrix_len, uint32_t al, int32_t go,
+                int32_t ge, uint32_t mode, int out, size_t out_bytes, Single* lay) {
+    Single L;
+    L.q = 32 + matrix_len * sizeof(int32_t);
+    L.t = L.q + qlen;
+    L.in_bytes = L.t + tlen;
+    L.out = (L.in_bytes + 15) & ~size_t(15);
+    L.total = L.out + out_bytes;
+    int rc;
+    if ((rc = grow_stage(s, L.total)) != HY_OK || (rc = grow(s->arena, L.total)) != HY_OK)
+        return rc;
+    const bool multipass = qlen > (size_t)hy::band_rows_per_pass((int)qlen);
+    if (multipass && (rc = grow(s->sys_spill, (tlen + 1) * sizeof(int2))) != HY_OK) return rc;
+    uint8_t* h = s->stage;
+    const uint64_t off2[2] = {0, (uint64_t)tlen};
+    const uint32_t meta[4] = {0, 0, (uint32_t)qlen, 0};
+    std::memcpy(h, off2, sizeof(off2));
+    std::memcpy(h + 16, meta, sizeof(meta));
+    int32_t* hm = reinterpret_cast<int32_t*>(h + 32);
+    for (size_t k = 0; k < matrix_len; ++k)
+        hm[k] = std::max(-hy::kEntryMax32, std::min(hy::kEntryMax32, matrix[k]));
+    if (qlen) std::memcpy(h + L.q, query, qlen);
+    if (tlen) std::memcpy(h + L.t, target, tlen);
+    uint8_t* d = static_cast<uint8_t*>(s->arena.p);
+    if (hipMemcpyAsync(d, h, L.in_bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    hy::BandArgs a{};
+    a.queries = d + L.q;
+    a.qoff = reinterpret_cast<const uint32_t*>(d + 20);
+    a.qlen = reinterpret_cast<const uint32_t*>(d + 24);
+    a.targets = d + L.t;
+    a.toff = reinterpret_cast<const uint64_t*>(d);
+    a.list = reinterpret_cast<const uint32_t*>(d + 16);
+    a.matrix = reinterpret_cast<const int32_t*>(d + 32);
+    a.al = (int)al;
+    a.go = go;
+    a.ge = ge;
+    a.spill = static_cast<int2*>(s->sys_spill.p);
+    a.spill_stride = 0;
+    if (out == hy::kBandColmax) {
+        a.best = reinterpret_cast<int2*>(d + L.out);
+        a.colmax = reinterpret_cast<int32_t*>(d + L.out + 8);
+    } else {
+        a.out_score = reinterpret_cast<int32_t*>(d + L.out);
+        a.out_qend = reinterpret_cast<int64_t*>(d + L.out + 8);
+        a.out_tend = reinterpret_cast<int64_t*>(d + L.out + 16);
+    }
+    if (hy::launch_band32((int)mode, out, a, 1, (int)qlen, s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    if (hipMemcpyAsync(h + L.out, d + L.out, out_bytes, hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    *lay = L;
+    return HY_OK;
 }
 
 // One wavefront per pair for the pairs in `idx` (k_sys32 pair mode): one launch per rows-per-
@@ -288,6 +423,23 @@ extern "C" int hy_align_pair(hy_pair_scratch* s, const uint8_t* query, size_t ql
                     err_detail);
     if (rc != HY_OK) return rc;
     if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+    const bool ends = tracks_end(search_type);
+    if (band_eligible(alphabet_len) && kernel_choice() != kPickWaves) {
+        Single L;
+        if ((rc = band_single(s, query, qlen, target, tlen, matrix, matrix_len, alphabet_len,
+                              gap_open, gap_ext, mode, ends ? hy::kBandEnds : hy::kBandScore, 24,
+                              &L)) != HY_OK)
+            return rc;
+        const uint8_t* o = s->stage + L.out;
+        int32_t sv = 0;
+        int64_t ev[2] = {-1, -1};
+        std::memcpy(&sv, o, sizeof(sv));
+        std::memcpy(ev, o + 8, sizeof(ev));
+        *score = sv;
+        *query_end = ends ? ev[0] : -1;
+        *target_end = ends ? ev[1] : -1;
+        return HY_OK;
+    }
 
     Stager st{s};
     const size_t o_q = st.add(&s->qflat, qlen);
@@ -315,7 +467,6 @@ extern "C" int hy_align_pair(hy_pair_scratch* s, const uint8_t* query, size_t ql
         (rc = grow(s->qend, sizeof(int64_t))) != HY_OK ||
         (rc = grow(s->tend, sizeof(int64_t))) != HY_OK)
         return rc;
-    const bool ends = tracks_end(search_type);
     std::vector<uint32_t> idx{0};
     std::vector<size_t> ql{qlen}, tl{tlen};
     if ((rc = enqueue_waves(s, idx, ql, tl, static_cast<const uint32_t*>(s->list.p),
@@ -414,18 +565,19 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         ql[k] = src.m(k);
         tl[k] = src.n(k);
     }
-    const int force = lane_choice();
+    const int pick = kernel_choice();
     std::vector<uint32_t> lanes, waves;
     size_t nshort = 0;
     for (size_t k = 0; k < npairs; ++k)
         nshort += ql[k] <= kLaneMaxQuery && tl[k] <= kLaneMaxTarget;
     const bool use_lanes =
-        al_eff <= 255 && (force == 1 || (force == 0 && nshort >= kLaneMinPairs));
+        al_eff <= 255 && (pick == kPickLanes || (pick == kPickAuto && nshort >= kLaneMinPairs));
     for (uint32_t k = 0; k < (uint32_t)npairs; ++k) {
         const bool shortp = ql[k] <= kLaneMaxQuery && tl[k] <= kLaneMaxTarget;
-        if (use_lanes && (shortp || force == 1)) lanes.push_back(k);
+        if (use_lanes && (shortp || pick == kPickLanes)) lanes.push_back(k);
         else waves.push_back(k);
     }
+    const bool band = use_band(pick, alphabet_len, waves.size(), tracks_end(search_type));
     // Lanes: sorted by (query length, target length) descending, cut into 64-lane groups. The
     // sort key packs both lengths (complemented, 13 + 17 bits: lane pairs are at most 4096 x
     // 65536 unless the lane kernel is forced) above the pair index.
@@ -440,9 +592,13 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         std::sort(keys.begin(), keys.end());
         for (size_t i = 0; i < lanes.size(); ++i) lanes[i] = (uint32_t)(keys[i] & 0x3FFFFFFFFull);
     }
-    // Waves: ordered by rows-per-lane class for one launch per class.
+    // The rest: ordered by the kernel's rows class (rows per pass of the band kernel, rows per
+    // lane of the one-wavefront kernel) for one launch per class.
     std::stable_sort(waves.begin(), waves.end(), [&](uint32_t a, uint32_t b) {
-        auto c = [&](uint32_t k) { return ql[k] <= 128 ? 0 : (ql[k] <= 256 ? 1 : (ql[k] <= 384 ? 2 : 3)); };
+        auto c = [&](uint32_t k) {
+            if (band) return hy::band_rows_per_pass((int)ql[k]);
+            return ql[k] <= 128 ? 0 : (ql[k] <= 256 ? 1 : (ql[k] <= 384 ? 2 : 3));
+        };
         return c(a) < c(b);
     });
     // Rows per register tile of the lane kernel. Score-only runs take 32 (fewer boundary-row
@@ -578,10 +734,14 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     if (both && (hipEventRecord(s->fork, s->stream) != hipSuccess ||
                  hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess))
         return HY_E_DEVICE;
-    if (!waves.empty() &&
-        (rc = enqueue_waves(s, waves, ql, tl, static_cast<const uint32_t*>(s->list.p),
-                            alphabet_len, gap_open, gap_ext, mode, ends, wstream)) != HY_OK)
-        return rc;
+    if (!waves.empty()) {
+        const uint32_t* d_list = static_cast<const uint32_t*>(s->list.p);
+        rc = band ? enqueue_band(s, waves, ql, tl, d_list, alphabet_len, gap_open, gap_ext, mode,
+                                 ends, wstream)
+                  : enqueue_waves(s, waves, ql, tl, d_list, alphabet_len, gap_open, gap_ext, mode,
+                                  ends, wstream);
+        if (rc != HY_OK) return rc;
+    }
     if (!lanes.empty()) {
         if ((rc = grow(s->twords, twords * 4)) != HY_OK ||
             (rc = grow(s->qwords, qwords * 4)) != HY_OK ||
@@ -723,6 +883,20 @@ extern "C" int hy_align_pair_position_max(hy_pair_scratch* s, const uint8_t* que
                     err_detail);
     if (rc != HY_OK) return rc;
     if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
+    if (band_eligible(alphabet_len) && kernel_choice() != kPickWaves) {
+        Single L;
+        if ((rc = band_single(s, query, qlen, target, tlen, matrix, matrix_len, alphabet_len,
+                              gap_open, gap_ext, HY_MODE_SW, hy::kBandColmax,
+                              8 + tlen * sizeof(int32_t), &L)) != HY_OK)
+            return rc;
+        const uint8_t* o = s->stage + L.out;
+        int2 b;
+        std::memcpy(&b, o, sizeof(b));
+        if (tlen) std::memcpy(colmax, o + 8, tlen * sizeof(int32_t));
+        *score = b.x;
+        *target_end = b.y;
+        return HY_OK;
+    }
     Stager st{s};
     const size_t o_q = st.add(&s->qflat, qlen);
     const size_t o_t = st.add(&s->tflat, tlen);

@@ -207,6 +207,275 @@ hipError_t launch_pairs_qe(bool ends, const PairArgs& a, uint32_t ngroups, hipSt
                   : launch_pairs_q<Q, false, false>(a, ngroups, st);
 }
 
+// ------------------------------------------------------------------------------------------------
+// Band kernel: one pair per workgroup of W wavefronts.
+// ------------------------------------------------------------------------------------------------
+// A single wavefront sweeping a pair is latency-bound: the R rows of a lane form one dependent
+// chain per step (F runs down the rows), and every step waits on LDS. Here W wavefronts split
+// each pass of 64*R*W rows into bands of 64*R rows. Wavefront w runs the systolic sweep of its
+// band kBandLag steps behind wavefront w-1; the bands land on different SIMDs, and with several
+// per SIMD their chains interleave. Inside a step nothing waits on memory: a lane's residue is
+// its left neighbour's from the step before (DPP), lane 0's comes from registers loaded two
+// 64-step epochs ahead (readlane), and the matrix entries and lane 0's top value for step s + 1
+// are fetched during step s.
+//
+// Lane 0's top values come from an LDS ring per wavefront: ring[w] for w >= 1 holds the bottom
+// row of wavefront w-1 (written by its lane 63), ring[0] the pass's top row (the DP border, or
+// the previous pass's bottom row), which wavefront 0 refills once per epoch, an epoch ahead.
+//
+// Timing: at local step s lane l works column s - l + 1, so the bottom lane of wavefront w-1
+// writes column j at its step j + 62, and lane 0 of wavefront w fetches it one step early, at
+// its step j - 2. On a global clock G wavefront w runs local step G - w*kBandLag, from local
+// step -1 (fetch only), and every wavefront takes a workgroup barrier after each G = 63 mod 64,
+// idle or not, so the barriers pair up. With kBandLag = 128 the write (global (w-1)*128 + j +
+// 62) and the fetch ((w-1)*128 + j + 126) are 64 steps apart, so a barrier always falls between
+// them, and the slot is rewritten (column j + 256) only after two more barriers. kBandLag is a
+// multiple of 64, so every wavefront's epoch ends (local s = 63 mod 64) are barrier steps.
+constexpr int kBandLag = 128;
+
+__device__ __forceinline__ int residue_or0(const uint8_t* t, int idx, int n) {
+    return idx < n ? (int)t[idx] : 0;
+}
+
+// Top row of a pass at column col: the DP border (first pass) or the previous pass's bottom row.
+template <int MODE>
+__device__ __forceinline__ int2 pass_top(const int2* sp, bool first, int col, int n, int go,
+                                         int ge) {
+    if (first) return make_int2(wsub(border<MODE>(0, col, go, ge), go), kNeg32);
+    return col <= n ? sp[col] : make_int2(0, kNeg32);
+}
+
+template <int MODE, int R, int W, int OUT>
+__global__ __launch_bounds__(64 * W) void k_band32(BandArgs a) {
+    using F = MF<MODE>;
+    constexpr bool ENDS = OUT == kBandEnds;
+    constexpr bool COLMAX = OUT == kBandColmax;
+    // Local-mode ends: each row keeps its first best column (ties in a row go to the smaller
+    // column, the lane's rows are merged with the tie-break at the end).
+    constexpr bool ROWBEST = ENDS && F::local;
+    static_assert(!COLMAX || MODE == kSW, "column maxima are a local-mode output");
+    extern __shared__ int smat[];
+    __shared__ int2 ring[W][256];
+    __shared__ int red[3][W];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t pid = a.list[blockIdx.x];
+    const uint8_t* query = a.queries + a.qoff[pid];
+    const uint8_t* t = a.targets + a.toff[pid];
+    const int n = (int)(a.toff[pid + 1] - a.toff[pid]);
+    const int m = (int)a.qlen[pid], go = a.go, ge = a.ge, al = a.al;
+    // The matrix plus go, row al the padding row for query rows past m.
+    for (int k = threadIdx.x; k < (al + 1) * al; k += 64 * W)
+        smat[k] = k < al * al ? a.matrix[k] + go : kPad32;
+    if (COLMAX) {
+        for (int k = threadIdx.x; k < n; k += 64 * W) a.colmax[k] = 0;
+        __threadfence();  // the zeros reach the device-scope atomics below
+    }
+    __syncthreads();
+
+    int bs = INT_MIN, bj = INT_MAX, bi = INT_MAX;
+    if (!COLMAX && threadIdx.x == 0) border_best<MODE>(m, n, go, ge, bs, bj, bi);
+
+    constexpr int P = 64 * R * W;
+    const int npass = n > 0 ? (m + P - 1) / P : 0;
+    int2* sp = a.spill + (size_t)blockIdx.x * a.spill_stride;
+    const int steps = n + 63;
+    for (int p = 0; p < npass; ++p) {
+        const bool first = p == 0;
+        const int r0 = p * P + (w * 64 + lane) * R + 1;
+        int qb[R], HG[R], E[R], sub[R], rb[R], rj[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = r0 + r;
+            qb[r] = (i <= m ? (int)query[i - 1] : al) * al;
+            HG[r] = wsub(border<MODE>(i, 0, go, ge), go);
+            E[r] = kNeg32;
+            sub[r] = 0;
+            rb[r] = i <= m ? INT_MIN : INT_MAX;  // rows past m never record a best
+            rj[r] = INT_MAX;
+        }
+        int HGd = wsub(border<MODE>(r0 - 1, 0, go, ge), go);
+        int topHG = 0, topF = kNeg32, c = 0;
+        const bool anyrow = r0 <= m;
+        // Wavefront 0 seeds its ring with the top row's columns 1-192 (read up to epoch 1);
+        // afterwards the epoch ending at local step s = 64e + 63 stores the columns of epoch
+        // e + 2 from tf, loaded at the end of the epoch before.
+        int2 tf = make_int2(0, kNeg32);
+        if (w == 0) {
+            for (int k = lane; k < 192; k += 64)
+                ring[0][(k + 1) & 255] = pass_top<MODE>(sp, first, k + 1, n, go, ge);
+            tf = pass_top<MODE>(sp, first, 66 + lane, n, go, ge);
+        }
+        // Residue epoch registers: lane k holds residue 64e + k of the current epoch (rc), the
+        // next (rn) and the one after (rf). Local step -1 closes epoch -1.
+        int rc = 0, rn = residue_or0(t, lane, n), rf = residue_or0(t, 64 + lane, n);
+        __syncthreads();
+
+        // Idle lead-in: the barriers of global steps -1 .. w*kBandLag - 2.
+        for (int k = 0; k < 2 * w; ++k) __syncthreads();
+        for (int s = -1; s < steps; ++s) {
+            // Fetch for step s + 1: residues (lane 0 from the epoch registers, the others from
+            // their left neighbour), matrix entries, and lane 0's top (column s + 2).
+            const int k1 = (s + 1) & 63;
+            const int rl = __builtin_amdgcn_readlane(k1 == 0 ? rn : rc, k1);
+            const int cn = __builtin_amdgcn_update_dpp(rl, c, 0x138, 0xf, 0xf, false);
+            int subn[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) subn[r] = smat[qb[r] + cn];
+            const int2 tv = ring[w][(s + 2) & 255];
+
+            const int j = s - lane + 1;
+            int outHG = 0, outF = 0;
+            if (j >= 1 && j <= n) {
+                int f = topF, hgup = topHG, hgd = HGd;
+                HGd = topHG;
+                int cm = 0;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int diag = wadd(hgd, sub[r]);
+                    const int e = max(HG[r], wsub(E[r], ge));
+                    f = max(hgup, wsub(f, ge));
+                    int h = max(max(diag, e), f);
+                    if (F::local) h = max(h, 0);
+                    hgd = HG[r];
+                    E[r] = e;
+                    HG[r] = wsub(h, go);
+                    hgup = HG[r];
+                    const int i = r0 + r;
+                    if (COLMAX) {
+                        // Rows past m would carry real values forward along the gap chains.
+                        cm = i <= m ? max(cm, h) : cm;
+                    } else if (ROWBEST) {
+                        const bool up = h > rb[r];
+                        rb[r] = up ? h : rb[r];
+                        rj[r] = up ? j : rj[r];
+                    } else {
+                        const bool cand = i <= m && (F::local || (F::lcol && j == n) ||
+                                                     (i == m && (F::lrow || j == n)));
+                        const int v = cand ? h : INT_MIN;
+                        if (ENDS) consider(bs, bj, bi, v, j, i);
+                        else bs = max(bs, v);
+                    }
+                }
+                outHG = HG[R - 1];
+                outF = f;
+                if (lane == 63) {
+                    if (w + 1 < W) ring[w + 1][j & 255] = make_int2(outHG, outF);
+                    else if (p + 1 < npass) sp[j] = make_int2(outHG, outF);
+                }
+                if (COLMAX && anyrow && cm > 0) atomicMax(a.colmax + (j - 1), cm);
+            }
+            // Step s + 1's inputs: lane l takes lane l-1's bottom row (wave_shr:1), lane 0 the
+            // fetched top.
+            topHG = __builtin_amdgcn_update_dpp(tv.x, outHG, 0x138, 0xf, 0xf, false);
+            topF = __builtin_amdgcn_update_dpp(tv.y, outF, 0x138, 0xf, 0xf, false);
+            c = cn;
+#pragma unroll
+            for (int r = 0; r < R; ++r) sub[r] = subn[r];
+            if ((s & 63) == 63) {
+                // Epoch e = s >> 6 ends: s + 1 = 64(e + 1).
+                const int s1 = s + 1;
+                rc = rn;
+                rn = rf;
+                rf = residue_or0(t, s1 + 128 + lane, n);  // epoch e + 3
+                if (w == 0) {
+                    // The top row read during epoch e + 2 (columns 64(e+2) + 2 .. + 65).
+                    ring[0][(s1 + 66 + lane) & 255] = tf;
+                    tf = pass_top<MODE>(sp, first, s1 + 130 + lane, n, go, ge);
+                }
+                __syncthreads();
+            }
+        }
+        // Idle tail: the barriers of the global steps after this wavefront's last.
+        for (int k = 0; k < 2 * (W - 1 - w); ++k) __syncthreads();
+        if (ROWBEST) {
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (rb[r] != INT_MAX) consider(bs, bj, bi, rb[r], rj[r], r0 + r);
+        }
+        __threadfence_block();
+        __syncthreads();
+    }
+
+    if (COLMAX) {
+        // Every atomic of the workgroup is done; the best score (>= 0) and its smallest column.
+        __threadfence();
+        __syncthreads();
+        int best = 0, bc = INT_MAX;
+        for (int k = threadIdx.x; k < n; k += 64 * W) {
+            const int v =
+                __hip_atomic_load(a.colmax + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (v > best) { best = v; bc = k; }
+        }
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            const int ob = __shfl_xor(best, off), oc = __shfl_xor(bc, off);
+            const bool take = ob > best || (ob == best && oc < bc);
+            best = take ? ob : best;
+            bc = take ? oc : bc;
+        }
+        if (lane == 0) { red[0][w] = best; red[1][w] = bc; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int k = 1; k < W; ++k) {
+                const bool take = red[0][k] > best || (red[0][k] == best && red[1][k] < bc);
+                best = take ? red[0][k] : best;
+                bc = take ? red[1][k] : bc;
+            }
+            *a.best = make_int2(best, best > 0 ? bc : -1);
+        }
+        return;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const int os = __shfl_xor(bs, off);
+        if (ENDS) consider(bs, bj, bi, os, __shfl_xor(bj, off), __shfl_xor(bi, off));
+        else bs = max(bs, os);
+    }
+    if (lane == 0) { red[0][w] = bs; red[1][w] = bj; red[2][w] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < W; ++k) {
+            if (ENDS) consider(bs, bj, bi, red[0][k], red[1][k], red[2][k]);
+            else bs = max(bs, red[0][k]);
+        }
+        a.out_score[pid] = bs;
+        if (ENDS) {
+            a.out_qend[pid] = bi == 0 ? -1 : (int64_t)bi - 1;
+            a.out_tend[pid] = bj == 0 ? -1 : (int64_t)bj - 1;
+        }
+    }
+}
+
+template <int R, int W, int OUT>
+hipError_t launch_band_rwo(const BandArgs& a, uint32_t nblocks, hipStream_t st) {
+    const size_t lds = (size_t)(a.al + 1) * a.al * sizeof(int);
+    hipLaunchKernelGGL((k_band32<HY_KMODE, R, W, OUT>), dim3(nblocks), dim3(64 * W), lds, st, a);
+    return hipGetLastError();
+}
+
+// Shapes (rows per lane R, wavefronts W) the launcher can pick; band_shape() chooses.
+template <int OUT>
+hipError_t launch_band_o(const BandArgs& a, uint32_t nblocks, int m, hipStream_t st) {
+    const BandShape sh = band_shape(m);
+    switch (sh.r * 100 + sh.w) {
+    case 101: return launch_band_rwo<1, 1, OUT>(a, nblocks, st);
+    case 102: return launch_band_rwo<1, 2, OUT>(a, nblocks, st);
+    case 104: return launch_band_rwo<1, 4, OUT>(a, nblocks, st);
+    case 201: return launch_band_rwo<2, 1, OUT>(a, nblocks, st);
+    case 202: return launch_band_rwo<2, 2, OUT>(a, nblocks, st);
+    case 204: return launch_band_rwo<2, 4, OUT>(a, nblocks, st);
+    case 208: return launch_band_rwo<2, 8, OUT>(a, nblocks, st);
+    case 216: return launch_band_rwo<2, 16, OUT>(a, nblocks, st);
+    case 401: return launch_band_rwo<4, 1, OUT>(a, nblocks, st);
+    case 402: return launch_band_rwo<4, 2, OUT>(a, nblocks, st);
+    case 404: return launch_band_rwo<4, 4, OUT>(a, nblocks, st);
+    case 408: return launch_band_rwo<4, 8, OUT>(a, nblocks, st);
+    case 416: return launch_band_rwo<4, 16, OUT>(a, nblocks, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 #if HY_KMODE == 0
 // Local mode, one pair per wavefront, R query rows per lane (the long-target kernel's shape).
 template <int R, bool LDSMAT>
@@ -345,6 +614,20 @@ hipError_t launch_pairs32_m(bool ends, const PairArgs& a, uint32_t ngroups, int 
 }
 
 template hipError_t launch_pairs32_m<HY_KMODE>(bool, const PairArgs&, uint32_t, int, hipStream_t);
+
+template <int MODE>
+hipError_t launch_band32_m(int out, const BandArgs& a, uint32_t nblocks, int m, hipStream_t st) {
+    static_assert(MODE == HY_KMODE, "one mode per translation unit");
+    if (a.al < 1 || a.al > (int)kBandMaxAlphabet) return hipErrorInvalidValue;
+    if (out == kBandEnds) return launch_band_o<kBandEnds>(a, nblocks, m, st);
+    if (out == kBandScore) return launch_band_o<kBandScore>(a, nblocks, m, st);
+#if HY_KMODE == 0
+    if (out == kBandColmax) return launch_band_o<kBandColmax>(a, nblocks, m, st);
+#endif
+    return hipErrorInvalidValue;
+}
+
+template hipError_t launch_band32_m<HY_KMODE>(int, const BandArgs&, uint32_t, int, hipStream_t);
 
 #if HY_KMODE == 0
 // Rows per lane as for the long-target kernel: 2, 4, 6 or 8 (several passes beyond 512 rows).

@@ -214,7 +214,8 @@ def score2(colmax, best_score: int, best_target_end: int, matrix_max: int, min_s
 
 def sse_striped_sw_pairs(qflat, qoff, tflat, toff, al: int, matrix, go: int, ge: int):
     """(SW scores of the pairs, seconds): the SSE4.1 striped restatement of the reference's
-    single-pair path, one thread. Only for pairs whose width proof gives at most 16 bits."""
+    single-pair path, one thread, each pair at the width its proof gives (16 x i8 or 8 x i16);
+    ValueError when a pair needs 32 bits."""
     qf = _u8(qflat) if len(qflat) else np.zeros(1, dtype=np.uint8)
     tf = _u8(tflat) if len(tflat) else np.zeros(1, dtype=np.uint8)
     qo = np.ascontiguousarray(qoff, dtype=np.uint64)
@@ -225,6 +226,8 @@ def sse_striped_sw_pairs(qflat, qoff, tflat, toff, al: int, matrix, go: int, ge:
     rc = _load().hyo_sse_striped_sw_pairs(_ptr(qf, _u8p), _ptr(qo, _u64p), _ptr(tf, _u8p),
                                           _ptr(to, _u64p), n, al, _ptr(_i32(matrix), _i32p), go,
                                           ge, _ptr(out, _i32p), ctypes.byref(secs))
+    if rc == -2:
+        raise ValueError("a pair needs 32-bit cells; the striped baseline covers 8 and 16 bits")
     if rc != 0:
         raise MemoryError("oracle allocation failed")
     return out[:n], secs.value

@@ -5,7 +5,8 @@ Mirrors the reference's unit tests of src/kernel.rs (position-max hand cases and
 align_pairs against per-pair align_pair across modes, search types and widths with empty
 geometries, scratch reuse, error propagation, score2 cases), then widens to every short pair in
 every mode, random schemes, protein and DNA batches large enough for the one-pair-per-lane kernel,
-long and multi-pass pairs, both batch kernels forced in turn.
+long and multi-pass pairs, each of the three batch kernels forced in turn (one pair per lane, per
+wavefront, per multi-wavefront band), and the band kernel at every wavefront count it launches.
 """
 import random
 
@@ -34,7 +35,7 @@ def want_hit(q, t, scheme, mode, st, index=0):
     return hy.BestHit(s, index, qe, te)
 
 
-@pytest.fixture(params=["lanes", "waves"])
+@pytest.fixture(params=["lanes", "waves", "band"])
 def kernel(request, monkeypatch):
     monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", request.param)
     return request.param
@@ -245,6 +246,9 @@ def test_protein_pairs_across_tiles_and_long_pairs(mode, kernel):
         pairs.append((q, t))
     pairs.append((rng.integers(0, 20, size=700, dtype=np.uint8),
                   rng.integers(0, 20, size=3000, dtype=np.uint8)))
+    q = rng.integers(0, 20, size=2300, dtype=np.uint8)
+    pairs.append((q, np.concatenate([rng.integers(0, 20, size=200, dtype=np.uint8), q[400:1500],
+                                     rng.integers(0, 20, size=150, dtype=np.uint8)])))
     for st in ST:
         check_pairs(pairs, (20, BLOSUM62, 11, 1), mode, st)
 
@@ -325,3 +329,116 @@ def test_flat_batch_equals_the_pair_list():
     assert calls == 10 and ms > 0
     with pytest.raises(ValueError):
         hy.align_pairs_flat(qf, qo[:-1], tf, to, s, hy.Mode.Sw, hy.SearchType.Score)
+
+
+# ---- the band kernel (one pair per multi-wavefront workgroup) -------------------------------------
+
+# Query lengths on both sides of every rows-per-pass class of the default launch shapes (64,
+# 128, 256, 512, 1,024, 2,048 and 4,096 rows) and of multi-pass queries.
+BAND_QLENS = [1, 63, 64, 65, 128, 129, 256, 257, 512, 513, 1024, 1025, 2048, 2049, 4100]
+# Every launch shape (rows per lane, wavefronts) the band launcher instantiates.
+BAND_SHAPES = [(1, 1), (1, 2), (1, 4), (2, 1), (2, 2), (2, 4), (2, 8), (2, 16), (4, 1), (4, 2),
+               (4, 4), (4, 8), (4, 16)]
+
+
+def band_pair(rng, m, n, al):
+    """A query of m and a target of n symbols; the target holds a mutated slice of the query so
+    scores are high and ends sit inside the band boundaries."""
+    q = rng.integers(0, al, size=m, dtype=np.uint8)
+    t = rng.integers(0, al, size=n, dtype=np.uint8)
+    if m and n > 8:
+        k = min(m, n) // 2
+        at = int(rng.integers(0, n - k + 1))
+        src = int(rng.integers(0, m - k + 1))
+        t[at:at + k] = q[src:src + k]
+        t[at:at + k:7] = (t[at:at + k:7] + 1) % al
+    return q, t
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_band_single_pairs_at_every_wavefront_count(mode, monkeypatch):
+    """align_pair (band kernel by default) equals the oracle at every rows-per-pass class and
+    across pass boundaries, for targets shorter than, equal to and longer than one wavefront's
+    64-step skew; the forced one-wavefront kernel gives the same hits."""
+    rng = np.random.default_rng(500 + mode)
+    sc = hy.PairScratch()
+    al, mat, go, ge = (4, identity_matrix(4, 3, -2), 5, 1)
+    s = hy.Scoring(al, mat, go, ge)
+    for m in BAND_QLENS:
+        for n in (1, 63, 64, 65, 300):
+            q, t = band_pair(rng, m, n, al)
+            want = orc.align_core(q, t, al, mat, go, ge, mode)
+            for st in ST:
+                got = hy.align_pair_with(sc, q, t, s, hy.Mode(mode), st)
+                exp = want_hit(q, t, (al, mat, go, ge), mode, st)
+                assert got == exp, (hy.Mode(mode), st, m, n, got, want)
+    for m in (65, 1025, 2049):
+        q, t = band_pair(rng, m, 700, al)
+        monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", "waves")
+        one = hy.align_pair_with(sc, q, t, s, hy.Mode(mode), hy.SearchType.ScoreEnd)
+        monkeypatch.delenv("HYALITE_HIP_PAIRS_KERNEL")
+        assert hy.align_pair_with(sc, q, t, s, hy.Mode(mode), hy.SearchType.ScoreEnd) == one
+
+
+def test_band_position_max_at_every_wavefront_count():
+    """align_pair_position_max (band kernel, column maxima folded by every wavefront) equals the
+    oracle's array, best and first column at every class and across pass boundaries."""
+    rng = np.random.default_rng(600)
+    sc = hy.PairScratch()
+    al, mat, go, ge = dna_scheme()
+    s = hy.Scoring(al, mat, go, ge)
+    for m in BAND_QLENS:
+        for n in (1, 64, 65, 900):
+            q, t = band_pair(rng, m, n, al)
+            out = []
+            hit = hy.align_pair_position_max_with(sc, q, t, s, out)
+            col, bs, te = orc.position_max(q, t, al, mat, go, ge)
+            assert out == col and (hit.score, hit.target_end) == (bs, te), (m, n)
+
+
+def test_band_batch_classes_and_multipass_chunks(monkeypatch):
+    """A batch forced onto the band kernel: pairs of every rows-per-pass class in one call, in
+    shuffled order (one launch per class), multi-pass queries beside single-pass ones; every hit
+    equals the oracle and the one-wavefront kernel's."""
+    rng = np.random.default_rng(700)
+    al, mat, go, ge = (20, BLOSUM62, 11, 1)
+    pairs = []
+    for m in BAND_QLENS + [0, 30, 700, 3000]:
+        for n in (0, 50, 400):
+            pairs.append(band_pair(rng, m, n, al))
+    order = rng.permutation(len(pairs))
+    pairs = [pairs[k] for k in order]
+    s = hy.Scoring(al, mat, go, ge)
+    for mode in ALL_MODES:
+        monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", "band")
+        got = hy.align_pairs(pairs, s, hy.Mode(mode), hy.SearchType.ScoreEnd)
+        for k, (q, t) in enumerate(pairs):
+            assert got[k] == want_hit(q, t, (al, mat, go, ge), mode, hy.SearchType.ScoreEnd, k), \
+                (hy.Mode(mode), k, len(q), len(t))
+        monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", "waves")
+        assert hy.align_pairs(pairs, s, hy.Mode(mode), hy.SearchType.ScoreEnd) == got
+
+
+@pytest.mark.parametrize("shape", BAND_SHAPES)
+def test_band_every_launch_shape_over_several_passes(shape, monkeypatch):
+    """Each launch shape forced in turn on queries of two passes and a ragged third, targets
+    shorter and longer than the 256-slot rings: scores and ends in local and global mode, and the
+    column maxima, equal the oracle's."""
+    r, w = shape
+    monkeypatch.setenv("HYALITE_HIP_BAND_SHAPE", f"{r},{w}")
+    rng = np.random.default_rng(800 + 100 * r + w)
+    al, mat, go, ge = (4, identity_matrix(4, 3, -2), 5, 1)
+    s = hy.Scoring(al, mat, go, ge)
+    sc = hy.PairScratch()
+    rows = 64 * r * w
+    for m in (rows - 1, 2 * rows + 3):
+        for n in (70, 300):
+            q, t = band_pair(rng, m, n, al)
+            for mode in (MODE_SW, 1):
+                for st in ST:
+                    got = hy.align_pair_with(sc, q, t, s, hy.Mode(mode), st)
+                    assert got == want_hit(q, t, (al, mat, go, ge), mode, st), (shape, m, n, mode)
+            out = []
+            hit = hy.align_pair_position_max_with(sc, q, t, s, out)
+            col, bs, te = orc.position_max(q, t, al, mat, go, ge)
+            assert out == col and (hit.score, hit.target_end) == (bs, te), (shape, m, n)

@@ -118,7 +118,8 @@ def test_score2_matches_the_peak_list_transcription():
 def test_sse_striped_baseline_matches_the_scalar_oracle():
     """The SSE4.1 striped (Farrar) restatement used as the pairs CPU baseline: every score equals
     the scalar oracle's, including the lazy-F carries across stripes (queries of 1-80 rows over 8
-    lanes), empty sides and pairs that saturate nothing at 16 bits."""
+    or 16 lanes), empty sides, and pairs at both widths the proof selects (short ones at 8 bits,
+    the rest at 16)."""
     import numpy as np
     rng = np.random.default_rng(2)
     for al, m_hi, go, ge in ((4, 80, 2, 1), (4, 80, 5, 1), (20, 60, 11, 1), (3, 40, 1, 1),
@@ -134,3 +135,26 @@ def test_sse_striped_baseline_matches_the_scalar_oracle():
         for k in range(300):
             q, t = qf[qo[k]:qo[k + 1]], tf[to[k]:to[k + 1]]
             assert got[k] == orc.align_core(q, t, al, mat, go, ge, MODE_SW)[0], (al, k)
+
+
+def test_sse_striped_baseline_widths_and_limits():
+    """The reference's pairwise bench shapes (DNA +2/-1, gaps 2/1): 30 x 30 proves 8 bits, 150 x
+    150 and 150 x 1,400 16 bits; scores equal the scalar oracle at each. A pair needing 32 bits
+    is refused rather than computed at a narrower width."""
+    import numpy as np
+    rng = np.random.default_rng(9)
+    mat = np.array([2 if i == j else -1 for i in range(4) for j in range(4)], dtype=np.int32)
+    for m, n in ((30, 30), (63, 63), (64, 200), (150, 150), (150, 1400)):
+        qs = [rng.integers(0, 4, size=m, dtype=np.uint8) for _ in range(20)]
+        ts = [rng.integers(0, 4, size=n, dtype=np.uint8) for _ in range(20)]
+        qs[0] = ts[0][:m].copy() if n >= m else qs[0]        # a high-scoring pair
+        qo = np.arange(21, dtype=np.uint64) * m
+        to = np.arange(21, dtype=np.uint64) * n
+        got, _ = orc.sse_striped_sw_pairs(np.concatenate(qs), qo, np.concatenate(ts), to, 4, mat,
+                                          2, 1)
+        for k in range(20):
+            assert got[k] == orc.align_core(qs[k], ts[k], 4, mat, 2, 1, MODE_SW)[0], (m, n, k)
+    big = np.array([20000 if i == j else -1 for i in range(4) for j in range(4)], dtype=np.int32)
+    with pytest.raises(ValueError):
+        orc.sse_striped_sw_pairs(np.zeros(5, np.uint8), np.array([0, 5], np.uint64),
+                                 np.zeros(5, np.uint8), np.array([0, 5], np.uint64), 4, big, 2, 1)
