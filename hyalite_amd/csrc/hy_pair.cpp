@@ -209,11 +209,11 @@ bool use_band(int pick, uint32_t al, size_t npairs, bool ends) {
 
 // Band kernel for the pairs in `idx` (ordered by rows-per-pass class): one launch per class,
 // the classes whose queries exceed one pass chunked so their boundary rows fit
-// kSysSpillBudget.
+// kSysSpillBudget. `base` carries the device sequences, matrix and outputs.
 int enqueue_band(hy_pair_scratch* s, const std::vector<uint32_t>& idx,
                  const std::vector<size_t>& qlens, const std::vector<size_t>& tlens,
-                 const uint32_t* d_list, uint32_t al, int32_t go, int32_t ge, uint32_t mode,
-                 bool ends, hipStream_t stream) {
+                 const hy::BandArgs& base, const uint32_t* d_list, uint32_t mode, bool ends,
+                 hipStream_t stream) {
     size_t b = 0;
     while (b < idx.size()) {
         const int c = hy::band_rows_per_pass((int)qlens[idx[b]]);
@@ -232,22 +232,10 @@ int enqueue_band(hy_pair_scratch* s, const std::vector<uint32_t>& idx,
         }
         for (size_t c0 = b; c0 < e; c0 += chunk) {
             const size_t cnt = std::min(chunk, e - c0);
-            hy::BandArgs a{};
-            a.queries = static_cast<const uint8_t*>(s->qflat.p);
-            a.qoff = static_cast<const uint32_t*>(s->qoff32.p);
-            a.qlen = static_cast<const uint32_t*>(s->qlen32.p);
-            a.targets = static_cast<const uint8_t*>(s->tflat.p);
-            a.toff = static_cast<const uint64_t*>(s->toff.p);
+            hy::BandArgs a = base;
             a.list = d_list + c0;
-            a.matrix = static_cast<const int32_t*>(s->mat.p);
-            a.al = (int)al;
-            a.go = go;
-            a.ge = ge;
             a.spill = static_cast<int2*>(s->sys_spill.p);
             a.spill_stride = multipass ? maxn + 1 : 0;
-            a.out_score = static_cast<int32_t*>(s->scores.p);
-            a.out_qend = static_cast<int64_t*>(s->qend.p);
-            a.out_tend = static_cast<int64_t*>(s->tend.p);
             if (hy::launch_band32((int)mode, ends ? hy::kBandEnds : hy::kBandScore, a,
                                   (uint32_t)cnt, (int)maxm, stream) != hipSuccess)
                 return HY_E_DEVICE;
@@ -508,6 +496,106 @@ struct PairSource {
     const uint8_t* t(size_t k) const { return tp ? tp[k] : tflat + toff[k]; }
 };
 
+// Small batches (sequences up to kSmallBatchBytes) in the band kernel with one upload and one
+// copy back: the arena holds target offsets, query offsets and lengths, the pair order (by rows
+// class), the clamped matrix, both sequence buffers and then the outputs. Symbols were checked
+// on the host.
+constexpr uint64_t kSmallBatchBytes = 1ull << 20;
+
+int small_band_batch(hy_pair_scratch* s, const PairSource& src, size_t npairs,
+                     const std::vector<size_t>& ql, const std::vector<size_t>& tl, uint64_t qtot,
+                     uint64_t ttot, const int32_t* matrix, size_t matrix_len, uint32_t al,
+                     int32_t go, int32_t ge, uint32_t mode, bool ends, int32_t* scores,
+                     int64_t* query_ends, int64_t* target_ends) {
+    std::vector<uint32_t> idx(npairs);
+    for (uint32_t k = 0; k < (uint32_t)npairs; ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        return hy::band_rows_per_pass((int)ql[a]) < hy::band_rows_per_pass((int)ql[b]);
+    });
+    size_t used = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = (used + 15) & ~size_t(15);
+        used = o + bytes;
+        return o;
+    };
+    const size_t o_toff = take((npairs + 1) * sizeof(uint64_t));
+    const size_t o_qoff = take(npairs * sizeof(uint32_t));
+    const size_t o_qlen = take(npairs * sizeof(uint32_t));
+    const size_t o_list = take(npairs * sizeof(uint32_t));
+    const size_t o_mat = take(matrix_len * sizeof(int32_t));
+    const size_t o_q = take(qtot);
+    const size_t o_t = take(ttot);
+    const size_t in_bytes = used;
+    const size_t o_sc = take(npairs * sizeof(int32_t));
+    const size_t o_qe = take(npairs * sizeof(int64_t));
+    const size_t o_te = take(npairs * sizeof(int64_t));
+    int rc;
+    if ((rc = grow_stage(s, used)) != HY_OK || (rc = grow(s->arena, used)) != HY_OK) return rc;
+    uint8_t* h = s->stage;
+    uint64_t* toff = reinterpret_cast<uint64_t*>(h + o_toff);
+    uint32_t* qoff = reinterpret_cast<uint32_t*>(h + o_qoff);
+    uint32_t* qlen = reinterpret_cast<uint32_t*>(h + o_qlen);
+    uint64_t qa = 0, ta = 0;
+    for (size_t k = 0; k < npairs; ++k) {
+        qoff[k] = (uint32_t)qa;
+        qlen[k] = (uint32_t)ql[k];
+        toff[k] = ta;
+        if (ql[k]) std::memcpy(h + o_q + qa, src.q(k), ql[k]);
+        if (tl[k]) std::memcpy(h + o_t + ta, src.t(k), tl[k]);
+        qa += ql[k];
+        ta += tl[k];
+    }
+    toff[npairs] = ta;
+    std::memcpy(h + o_list, idx.data(), npairs * sizeof(uint32_t));
+    int32_t* hm = reinterpret_cast<int32_t*>(h + o_mat);
+    for (size_t k = 0; k < matrix_len; ++k)
+        hm[k] = std::max(-hy::kEntryMax32, std::min(hy::kEntryMax32, matrix[k]));
+    uint8_t* d = static_cast<uint8_t*>(s->arena.p);
+    if (hipMemcpyAsync(d, h, in_bytes, hipMemcpyHostToDevice, s->stream) != hipSuccess ||
+        hipEventRecord(s->ev0, s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    hy::BandArgs ba{};
+    ba.queries = d + o_q;
+    ba.qoff = reinterpret_cast<const uint32_t*>(d + o_qoff);
+    ba.qlen = reinterpret_cast<const uint32_t*>(d + o_qlen);
+    ba.targets = d + o_t;
+    ba.toff = reinterpret_cast<const uint64_t*>(d + o_toff);
+    ba.matrix = reinterpret_cast<const int32_t*>(d + o_mat);
+    ba.al = (int)al;
+    ba.go = go;
+    ba.ge = ge;
+    ba.out_score = reinterpret_cast<int32_t*>(d + o_sc);
+    ba.out_qend = reinterpret_cast<int64_t*>(d + o_qe);
+    ba.out_tend = reinterpret_cast<int64_t*>(d + o_te);
+    if ((rc = enqueue_band(s, idx, ql, tl, ba, reinterpret_cast<const uint32_t*>(d + o_list),
+                           mode, ends, s->stream)) != HY_OK)
+        return rc;
+    const size_t back = (ends ? used : o_qe) - o_sc;
+    if (hipEventRecord(s->ev1, s->stream) != hipSuccess ||
+        hipMemcpyAsync(h + o_sc, d + o_sc, back, hipMemcpyDeviceToHost, s->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, s->ev0, s->ev1) == hipSuccess) {
+            s->device_ms += ms;
+            s->calls += 1;
+        }
+    }
+    std::memcpy(scores, h + o_sc, npairs * sizeof(int32_t));
+    for (size_t k = 0; k < npairs; ++k) {
+        int64_t qe = -1, te = -1;
+        if (ends) {
+            std::memcpy(&qe, h + o_qe + k * sizeof(int64_t), sizeof(qe));
+            std::memcpy(&te, h + o_te + k * sizeof(int64_t), sizeof(te));
+        }
+        if (query_ends) query_ends[k] = qe;
+        if (target_ends) target_ends[k] = te;
+    }
+    return HY_OK;
+}
+
 int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
                      const int32_t* matrix, size_t matrix_len, uint32_t alphabet_len,
                      int32_t gap_open, int32_t gap_ext, uint32_t mode, uint32_t search_type,
@@ -566,6 +654,26 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         tl[k] = src.n(k);
     }
     const int pick = kernel_choice();
+    // Small batches that would run in bands: symbols checked here, one upload, one copy back.
+    if (qtot + ttot <= kSmallBatchBytes && pick != kPickLanes &&
+        (pick == kPickBand || npairs < kLaneMinPairs) &&
+        use_band(pick, alphabet_len, npairs, ends)) {
+        uint8_t mx = 0;
+        for (size_t k = 0; k < npairs; ++k) {
+            const uint8_t* q = src.q(k);
+            const uint8_t* t = src.t(k);
+            for (size_t i = 0; i < ql[k]; ++i) mx = std::max(mx, q[i]);
+            for (size_t i = 0; i < tl[k]; ++i) mx = std::max(mx, t[i]);
+        }
+        if (mx >= alphabet_len) {
+            rc = exact_checks();
+            if (rc != kChecksPassed) return rc;
+            return HY_E_INTERNAL;
+        }
+        return small_band_batch(s, src, npairs, ql, tl, qtot, ttot, matrix, matrix_len,
+                                alphabet_len, gap_open, gap_ext, mode, ends, scores, query_ends,
+                                target_ends);
+    }
     std::vector<uint32_t> lanes, waves;
     size_t nshort = 0;
     for (size_t k = 0; k < npairs; ++k)
@@ -736,10 +844,25 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         return HY_E_DEVICE;
     if (!waves.empty()) {
         const uint32_t* d_list = static_cast<const uint32_t*>(s->list.p);
-        rc = band ? enqueue_band(s, waves, ql, tl, d_list, alphabet_len, gap_open, gap_ext, mode,
-                                 ends, wstream)
-                  : enqueue_waves(s, waves, ql, tl, d_list, alphabet_len, gap_open, gap_ext, mode,
-                                  ends, wstream);
+        if (band) {
+            hy::BandArgs ba{};
+            ba.queries = static_cast<const uint8_t*>(s->qflat.p);
+            ba.qoff = static_cast<const uint32_t*>(s->qoff32.p);
+            ba.qlen = static_cast<const uint32_t*>(s->qlen32.p);
+            ba.targets = static_cast<const uint8_t*>(s->tflat.p);
+            ba.toff = static_cast<const uint64_t*>(s->toff.p);
+            ba.matrix = static_cast<const int32_t*>(s->mat.p);
+            ba.al = (int)alphabet_len;
+            ba.go = gap_open;
+            ba.ge = gap_ext;
+            ba.out_score = static_cast<int32_t*>(s->scores.p);
+            ba.out_qend = static_cast<int64_t*>(s->qend.p);
+            ba.out_tend = static_cast<int64_t*>(s->tend.p);
+            rc = enqueue_band(s, waves, ql, tl, ba, d_list, mode, ends, wstream);
+        } else {
+            rc = enqueue_waves(s, waves, ql, tl, d_list, alphabet_len, gap_open, gap_ext, mode,
+                               ends, wstream);
+        }
         if (rc != HY_OK) return rc;
     }
     if (!lanes.empty()) {

@@ -442,3 +442,25 @@ def test_band_every_launch_shape_over_several_passes(shape, monkeypatch):
             hit = hy.align_pair_position_max_with(sc, q, t, s, out)
             col, bs, te = orc.position_max(q, t, al, mat, go, ge)
             assert out == col and (hit.score, hit.target_end) == (bs, te), (shape, m, n)
+
+
+def test_band_batches_on_both_upload_paths(monkeypatch):
+    """Batches forced onto the band kernel through the single-upload path for small batches
+    (under 1 MB of sequences) and the general upload path (a batch of about 2 MB), in local and
+    global mode with ends: every hit of a sample equals the oracle, and the whole batch equals
+    the one-wavefront kernel's."""
+    rng = np.random.default_rng(900)
+    al, mat, go, ge = dna_scheme()
+    s = hy.Scoring(al, mat, go, ge)
+    for npairs in (300, 4000):
+        pairs = [band_pair(rng, int(rng.integers(0, 300)), int(rng.integers(0, 420)), al)
+                 for _ in range(npairs)]
+        for mode in (MODE_SW, 2):
+            monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", "band")
+            got = hy.align_pairs(pairs, s, hy.Mode(mode), hy.SearchType.ScoreEnd)
+            for k in range(0, npairs, 37):
+                q, t = pairs[k]
+                assert got[k] == want_hit(q, t, (al, mat, go, ge), mode, hy.SearchType.ScoreEnd,
+                                          k), (npairs, mode, k)
+            monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", "waves")
+            assert hy.align_pairs(pairs, s, hy.Mode(mode), hy.SearchType.ScoreEnd) == got
