@@ -1121,8 +1121,9 @@ def _st_code(search_type) -> int:
 
 def align_pair_with(scratch: PairScratch, query, target, scoring: Scoring, mode: Mode,
                     search_type) -> BestHit:
-    """src/kernel.rs:354-420: one pair on the device (one wavefront); errors as the reference's
-    (SymbolOutOfRange, ScoreRangeTooWide)."""
+    """src/kernel.rs:354-420: one pair on the device (one workgroup of the band kernel, or one
+    wavefront for alphabets above 63 symbols); errors as the reference's (SymbolOutOfRange,
+    ScoreRangeTooWide)."""
     q, t = np.asarray(query, dtype=np.uint8), np.asarray(target, dtype=np.uint8)
     qa, ta = _as_u8(q), _as_u8(t)
     mat = scoring.matrix()
