@@ -49,6 +49,7 @@ _LIB_NAME = "libhyalite_hip.so"
 # and the mode-templated kernels compiled once per alignment mode (SW, NW, HW, OV, SHW).
 _UNITS = [("csrc/hy_host.cpp", []), ("csrc/hy_tune.cpp", []), ("csrc/hy_serve.cpp", []),
           ("csrc/hy_align.cpp", []), ("csrc/hy_pair.cpp", []), ("csrc/hy_image.cpp", []),
+    ("csrc/hy_xfer.cpp", []),
           ("csrc/hy_launch.hip", [])] + [
     (src, [f"-DHY_KMODE={k}"])
     for src in ("csrc/hy_kernels.hip", "csrc/hy_trace.hip", "csrc/hy_pair.hip")

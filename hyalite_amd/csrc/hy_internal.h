@@ -273,6 +273,26 @@ inline int band_rows_per_pass(int m) {
     return 64 * s.r * s.w;
 }
 
+// Large copies between pageable host memory and the device (hy_xfer.cpp): two pinned slots per
+// worker thread and the event of the DMA that last used each. Uploads return with their DMAs
+// queued on `stream` (after the work already there); downloads return complete.
+struct PinnedRing {
+    static constexpr int kThreads = 8;
+    static constexpr size_t kSlot = 4ull << 20;
+    uint8_t* ring = nullptr;
+    hipEvent_t ev[2 * kThreads] = {};
+    PinnedRing() = default;
+    PinnedRing(const PinnedRing&) = delete;
+    PinnedRing& operator=(const PinnedRing&) = delete;
+    ~PinnedRing();
+};
+// Below this a plain hipMemcpyAsync is as fast.
+constexpr size_t kPinnedRingMin = 8ull << 20;
+int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
+                    hipStream_t stream);
+int download_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
+                      hipStream_t stream);
+
 // Launchers (hy_launch.hip). All launch on `stream` and never synchronise.
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,
                            hipStream_t stream);

@@ -14,7 +14,6 @@
 #include <cstring>
 #include <numeric>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/hyalite_hip.h"
@@ -40,12 +39,7 @@ struct hy_pair_scratch {
     DevBuf mat, qflat, tflat, qoff, toff, qoff32, qlen32, list, nq, groups, twords, qwords, tlens,
         qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best, flag, arena;
     uint32_t* h_flag = nullptr;  // pinned: the largest symbol of a batch
-    // Large pageable uploads: two pinned slots per worker thread, each with the event of the
-    // DMA that last read it.
-    static constexpr int kUpThreads = 8;
-    static constexpr size_t kUpSlot = 4ull << 20;
-    uint8_t* up_ring = nullptr;
-    hipEvent_t up_ev[2 * kUpThreads] = {};
+    hy::PinnedRing ring;  // large pageable uploads
     uint8_t* stage = nullptr;  // pinned
     size_t stage_bytes = 0;
 
@@ -58,9 +52,6 @@ struct hy_pair_scratch {
                           &scores, &qend, &tend, &colmax, &best, &flag, &arena})
             hy::dev_free(b->p);
         if (h_flag) (void)hipHostFree(h_flag);
-        if (up_ring) (void)hipHostFree(up_ring);
-        for (hipEvent_t e : up_ev)
-            if (e) (void)hipEventDestroy(e);
         if (stage) (void)hipHostFree(stage);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
@@ -113,63 +104,6 @@ int grow_stage(hy_pair_scratch* s, size_t bytes) {
     return HY_OK;
 }
 
-// Host-to-device copy of a large pageable buffer. A copy from pageable memory is staged through
-// pinned memory by one runtime thread, which leaves the DMA engine idle half the time; here
-// worker threads each take every kUpThreads-th piece of kUpSlot bytes, copy it into one of their
-// two pinned slots (after the DMA that last read the slot has finished) and issue its DMA, so
-// the host copies run in parallel and overlap the transfers. The DMAs are queued on `stream`
-// before this returns.
-constexpr size_t kParallelUploadMin = 8ull << 20;
-
-int upload_pageable(hy_pair_scratch* s, void* dst, const void* src, size_t bytes,
-                    hipStream_t stream) {
-    constexpr int T = hy_pair_scratch::kUpThreads;
-    constexpr size_t SLOT = hy_pair_scratch::kUpSlot;
-    if (!s->up_ring) {
-        if (hipHostMalloc((void**)&s->up_ring, 2 * T * SLOT) != hipSuccess) {
-            s->up_ring = nullptr;
-            return HY_E_OUT_OF_MEMORY;
-        }
-        for (hipEvent_t& e : s->up_ev)
-            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
-                return HY_E_DEVICE;
-    }
-    const size_t pieces = (bytes + SLOT - 1) / SLOT;
-    const int workers = (int)std::min<size_t>(T, pieces);
-    std::vector<int> status(workers, HY_OK);
-    auto work = [&](int t) {
-        // Set on the worker thread: HIP's current device is per thread.
-        if (hipSetDevice(s->device) != hipSuccess) {
-            status[t] = HY_E_DEVICE;
-            return;
-        }
-        size_t used = 0;
-        for (size_t p = (size_t)t; p < pieces; p += T, ++used) {
-            const int slot = 2 * t + (int)(used & 1);
-            const size_t off = p * SLOT, len = std::min(SLOT, bytes - off);
-            uint8_t* h = s->up_ring + (size_t)slot * SLOT;
-            if (used >= 2 && hipEventSynchronize(s->up_ev[slot]) != hipSuccess) {
-                status[t] = HY_E_DEVICE;
-                return;
-            }
-            std::memcpy(h, static_cast<const uint8_t*>(src) + off, len);
-            if (hipMemcpyAsync(static_cast<uint8_t*>(dst) + off, h, len, hipMemcpyHostToDevice,
-                               stream) != hipSuccess ||
-                hipEventRecord(s->up_ev[slot], stream) != hipSuccess) {
-                status[t] = HY_E_DEVICE;
-                return;
-            }
-        }
-    };
-    std::vector<std::thread> pool;
-    pool.reserve(workers);
-    for (int t = 0; t < workers; ++t) pool.emplace_back(work, t);
-    for (std::thread& th : pool) th.join();
-    for (int st : status)
-        if (st != HY_OK) return st;
-    return HY_OK;
-}
-
 // Sequential layout of one call's host-to-device data inside the pinned stage.
 struct Stager {
     hy_pair_scratch* s;
@@ -194,8 +128,9 @@ struct Stager {
             if (rc != HY_OK) return rc;
         }
         for (const Piece& p : pieces) {
-            if (p.ext && p.bytes >= kParallelUploadMin) {
-                const int rc = upload_pageable(s, p.dst->p, p.ext, p.bytes, s->stream);
+            if (p.ext && p.bytes >= hy::kPinnedRingMin) {
+                const int rc =
+                    hy::upload_pageable(s->ring, s->device, p.dst->p, p.ext, p.bytes, s->stream);
                 if (rc != HY_OK) return rc;
                 continue;
             }

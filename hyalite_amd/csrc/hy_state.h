@@ -138,6 +138,7 @@ struct hy_scratch {
     uint64_t tout_cap = 0;
     uint8_t* d_ops = nullptr;
     uint64_t ops_cap = 0;
+    hy::PinnedRing ring;  // large pageable copies back (scan_all_aligned)
     uint64_t* d_slab_off = nullptr;
     int slab_m = -1;                     // query length the slab offsets were laid out for
     std::vector<uint64_t> h_slab_off;

@@ -1,0 +1,119 @@
+// Large copies between pageable host memory and the device through pinned slots.
+//
+// The runtime stages a copy from or to pageable memory through pinned memory on one thread,
+// which keeps the DMA engine idle about half the time (130 MB up in 6.5 ms, 3.1 ms of it
+// transferring). Here worker threads each own two pinned slots of PinnedRing::kSlot bytes and
+// take every kThreads-th piece of the buffer: on the way up a thread copies a piece into a free
+// slot and queues its DMA; on the way down it queues the DMA of its next piece before copying
+// the current one out, so host copies run in parallel and overlap the transfers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "../../include/hyalite_hip.h"
+#include "hy_internal.h"
+
+namespace hy {
+
+PinnedRing::~PinnedRing() {
+    if (ring) (void)hipHostFree(ring);
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+}
+
+namespace {
+
+int ring_ready(PinnedRing& r) {
+    if (r.ring) return HY_OK;
+    if (hipHostMalloc((void**)&r.ring, 2 * PinnedRing::kThreads * PinnedRing::kSlot) !=
+        hipSuccess) {
+        r.ring = nullptr;
+        return HY_E_OUT_OF_MEMORY;
+    }
+    for (hipEvent_t& e : r.ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+            return HY_E_DEVICE;
+    return HY_OK;
+}
+
+// Runs work(t) on min(kThreads, pieces) threads, each with `device` current; the first
+// failing status, or HY_OK.
+template <class F>
+int run_workers(int device, size_t pieces, F work) {
+    const int workers = (int)std::min<size_t>(PinnedRing::kThreads, pieces);
+    std::vector<int> status(workers, HY_OK);
+    std::vector<std::thread> pool;
+    pool.reserve(workers);
+    for (int t = 0; t < workers; ++t)
+        pool.emplace_back([&, t]() {
+            // HIP's current device is per thread.
+            status[t] = hipSetDevice(device) == hipSuccess ? work(t) : HY_E_DEVICE;
+        });
+    for (std::thread& th : pool) th.join();
+    for (int st : status)
+        if (st != HY_OK) return st;
+    return HY_OK;
+}
+
+}  // namespace
+
+int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
+                    hipStream_t stream) {
+    if (bytes == 0) return HY_OK;
+    int rc = ring_ready(r);
+    if (rc != HY_OK) return rc;
+    constexpr int T = PinnedRing::kThreads;
+    constexpr size_t SLOT = PinnedRing::kSlot;
+    const size_t pieces = (bytes + SLOT - 1) / SLOT;
+    return run_workers(device, pieces, [&](int t) -> int {
+        size_t used = 0;
+        for (size_t p = (size_t)t; p < pieces; p += T, ++used) {
+            const int slot = 2 * t + (int)(used & 1);
+            const size_t off = p * SLOT, len = std::min(SLOT, bytes - off);
+            uint8_t* h = r.ring + (size_t)slot * SLOT;
+            // The slot's previous piece must have left before it is refilled.
+            if (used >= 2 && hipEventSynchronize(r.ev[slot]) != hipSuccess) return HY_E_DEVICE;
+            std::memcpy(h, static_cast<const uint8_t*>(src) + off, len);
+            if (hipMemcpyAsync(static_cast<uint8_t*>(dst) + off, h, len, hipMemcpyHostToDevice,
+                               stream) != hipSuccess ||
+                hipEventRecord(r.ev[slot], stream) != hipSuccess)
+                return HY_E_DEVICE;
+        }
+        return HY_OK;
+    });
+}
+
+int download_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
+                      hipStream_t stream) {
+    if (bytes == 0) return HY_OK;
+    int rc = ring_ready(r);
+    if (rc != HY_OK) return rc;
+    constexpr int T = PinnedRing::kThreads;
+    constexpr size_t SLOT = PinnedRing::kSlot;
+    const size_t pieces = (bytes + SLOT - 1) / SLOT;
+    return run_workers(device, pieces, [&](int t) -> int {
+        auto issue = [&](size_t p, int slot) -> bool {
+            const size_t off = p * SLOT, len = std::min(SLOT, bytes - off);
+            return hipMemcpyAsync(r.ring + (size_t)slot * SLOT,
+                                  static_cast<const uint8_t*>(src) + off, len,
+                                  hipMemcpyDeviceToHost, stream) == hipSuccess &&
+                   hipEventRecord(r.ev[slot], stream) == hipSuccess;
+        };
+        if ((size_t)t < pieces && !issue((size_t)t, 2 * t)) return HY_E_DEVICE;
+        size_t i = 0;
+        for (size_t p = (size_t)t; p < pieces; p += T, ++i) {
+            const int slot = 2 * t + (int)(i & 1);
+            // The next piece goes into the other slot, whose piece was copied out last round.
+            if (p + T < pieces && !issue(p + T, 2 * t + (int)((i + 1) & 1))) return HY_E_DEVICE;
+            if (hipEventSynchronize(r.ev[slot]) != hipSuccess) return HY_E_DEVICE;
+            const size_t off = p * SLOT, len = std::min(SLOT, bytes - off);
+            std::memcpy(static_cast<uint8_t*>(dst) + off, r.ring + (size_t)slot * SLOT, len);
+        }
+        return HY_OK;
+    });
+}
+
+}  // namespace hy
