@@ -195,17 +195,10 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
         if (hy::launch_trace((int)db->mode, a, s->stream) != hipSuccess) return HY_E_DEVICE;
         k0 = k1;
     }
-    // The caller's buffers are pageable; large ones come back through the pinned ring.
-    auto back = [&](void* dst, const void* src, size_t bytes) -> int {
-        if (bytes >= hy::kPinnedRingMin)
-            return hy::download_pageable(s->ring, s->device, dst, src, bytes, s->stream);
-        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream) == hipSuccess
-                   ? HY_OK
-                   : HY_E_DEVICE;
-    };
-    rc = back(out, s->d_tout, n * sizeof(hy::TraceOut));
-    if (rc == HY_OK) rc = back(ops, s->d_ops, (size_t)ops_need);
-    if (rc != HY_OK) return rc;
+    // The caller's buffers are pageable: they come back through the pinned ring when large.
+    const hy::Segment segs[2] = {{out, s->d_tout, n * sizeof(hy::TraceOut)},
+                                 {ops, s->d_ops, (size_t)ops_need}};
+    if ((rc = hy::download(s->ring, s->device, segs, 2, s->stream)) != HY_OK) return rc;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
     for (uint64_t k = 0; k < n; ++k) {
         if (out[k].reserved != 0) return HY_E_INTERNAL;

@@ -789,17 +789,12 @@ extern "C" int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query,
     const bool ends = db->search_type != HY_SEARCH_SCORE;
     if ((rc = enqueue_targets(db, s, (int)qlen, ends, true, nullptr)) != HY_OK) return rc;
     const size_t n = (size_t)db->n;
-    if (hipMemcpyAsync(scores, s->d_scores, n * sizeof(int32_t), hipMemcpyDeviceToHost,
-                       s->stream) != hipSuccess)
-        return HY_E_DEVICE;
-    if (ends) {
-        if (query_ends && hipMemcpyAsync(query_ends, s->d_qend, n * sizeof(int64_t),
-                                         hipMemcpyDeviceToHost, s->stream) != hipSuccess)
-            return HY_E_DEVICE;
-        if (target_ends && hipMemcpyAsync(target_ends, s->d_tend, n * sizeof(int64_t),
-                                          hipMemcpyDeviceToHost, s->stream) != hipSuccess)
-            return HY_E_DEVICE;
-    }
+    hy::Segment segs[3];
+    int nseg = 0;
+    segs[nseg++] = {scores, s->d_scores, n * sizeof(int32_t)};
+    if (ends && query_ends) segs[nseg++] = {query_ends, s->d_qend, n * sizeof(int64_t)};
+    if (ends && target_ends) segs[nseg++] = {target_ends, s->d_tend, n * sizeof(int64_t)};
+    if ((rc = hy::download(s->ring, s->device, segs, nseg, s->stream)) != HY_OK) return rc;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
     collect_profile(s);
     if (!ends) {

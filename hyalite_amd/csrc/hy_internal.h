@@ -290,8 +290,17 @@ struct PinnedRing {
 constexpr size_t kPinnedRingMin = 8ull << 20;
 int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
                     hipStream_t stream);
-int download_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
+struct Segment {
+    void* dst;
+    const void* src;
+    size_t bytes;
+};
+// Every segment (device src -> pageable dst), after the work already queued on `stream`.
+int download_pageable(PinnedRing& r, int device, const Segment* segs, int nseg,
                       hipStream_t stream);
+// The segments back to pageable memory: through the ring when together they hold at least
+// kPinnedRingMin bytes (complete on return), else as plain copies queued on `stream`.
+int download(PinnedRing& r, int device, const Segment* segs, int nseg, hipStream_t stream);
 
 // Launchers (hy_launch.hip). All launch on `stream` and never synchronise.
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,

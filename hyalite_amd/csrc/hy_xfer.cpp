@@ -86,34 +86,54 @@ int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_
     });
 }
 
-int download_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
+int download_pageable(PinnedRing& r, int device, const Segment* segs, int nseg,
                       hipStream_t stream) {
-    if (bytes == 0) return HY_OK;
-    int rc = ring_ready(r);
-    if (rc != HY_OK) return rc;
+    // Pieces of kSlot bytes over all segments, in order.
+    struct Piece {
+        uint8_t* dst;
+        const uint8_t* src;
+        size_t len;
+    };
     constexpr int T = PinnedRing::kThreads;
     constexpr size_t SLOT = PinnedRing::kSlot;
-    const size_t pieces = (bytes + SLOT - 1) / SLOT;
-    return run_workers(device, pieces, [&](int t) -> int {
+    std::vector<Piece> pieces;
+    for (int k = 0; k < nseg; ++k)
+        for (size_t off = 0; off < segs[k].bytes; off += SLOT)
+            pieces.push_back({static_cast<uint8_t*>(segs[k].dst) + off,
+                              static_cast<const uint8_t*>(segs[k].src) + off,
+                              std::min(SLOT, segs[k].bytes - off)});
+    if (pieces.empty()) return HY_OK;
+    int rc = ring_ready(r);
+    if (rc != HY_OK) return rc;
+    const size_t np = pieces.size();
+    return run_workers(device, np, [&](int t) -> int {
         auto issue = [&](size_t p, int slot) -> bool {
-            const size_t off = p * SLOT, len = std::min(SLOT, bytes - off);
-            return hipMemcpyAsync(r.ring + (size_t)slot * SLOT,
-                                  static_cast<const uint8_t*>(src) + off, len,
+            return hipMemcpyAsync(r.ring + (size_t)slot * SLOT, pieces[p].src, pieces[p].len,
                                   hipMemcpyDeviceToHost, stream) == hipSuccess &&
                    hipEventRecord(r.ev[slot], stream) == hipSuccess;
         };
-        if ((size_t)t < pieces && !issue((size_t)t, 2 * t)) return HY_E_DEVICE;
+        if ((size_t)t < np && !issue((size_t)t, 2 * t)) return HY_E_DEVICE;
         size_t i = 0;
-        for (size_t p = (size_t)t; p < pieces; p += T, ++i) {
+        for (size_t p = (size_t)t; p < np; p += T, ++i) {
             const int slot = 2 * t + (int)(i & 1);
             // The next piece goes into the other slot, whose piece was copied out last round.
-            if (p + T < pieces && !issue(p + T, 2 * t + (int)((i + 1) & 1))) return HY_E_DEVICE;
+            if (p + T < np && !issue(p + T, 2 * t + (int)((i + 1) & 1))) return HY_E_DEVICE;
             if (hipEventSynchronize(r.ev[slot]) != hipSuccess) return HY_E_DEVICE;
-            const size_t off = p * SLOT, len = std::min(SLOT, bytes - off);
-            std::memcpy(static_cast<uint8_t*>(dst) + off, r.ring + (size_t)slot * SLOT, len);
+            std::memcpy(pieces[p].dst, r.ring + (size_t)slot * SLOT, pieces[p].len);
         }
         return HY_OK;
     });
+}
+
+int download(PinnedRing& r, int device, const Segment* segs, int nseg, hipStream_t stream) {
+    size_t total = 0;
+    for (int k = 0; k < nseg; ++k) total += segs[k].bytes;
+    if (total >= kPinnedRingMin) return download_pageable(r, device, segs, nseg, stream);
+    for (int k = 0; k < nseg; ++k)
+        if (segs[k].bytes && hipMemcpyAsync(segs[k].dst, segs[k].src, segs[k].bytes,
+                                            hipMemcpyDeviceToHost, stream) != hipSuccess)
+            return HY_E_DEVICE;
+    return HY_OK;
 }
 
 }  // namespace hy
