@@ -281,6 +281,9 @@ struct PinnedRing {
     static constexpr size_t kSlot = 4ull << 20;
     uint8_t* ring = nullptr;
     hipEvent_t ev[2 * kThreads] = {};
+    // A DMA recorded by ev[k] may still read slot k: the slot is not refilled before it ends,
+    // also when that DMA was queued by an earlier call. Slot k belongs to worker k / 2.
+    bool busy[2 * kThreads] = {};
     PinnedRing() = default;
     PinnedRing(const PinnedRing&) = delete;
     PinnedRing& operator=(const PinnedRing&) = delete;

@@ -74,13 +74,18 @@ int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_
             const int slot = 2 * t + (int)(used & 1);
             const size_t off = p * SLOT, len = std::min(SLOT, bytes - off);
             uint8_t* h = r.ring + (size_t)slot * SLOT;
-            // The slot's previous piece must have left before it is refilled.
-            if (used >= 2 && hipEventSynchronize(r.ev[slot]) != hipSuccess) return HY_E_DEVICE;
+            // The slot's previous piece, from this call or an earlier one, must have left
+            // before the slot is refilled.
+            if (r.busy[slot]) {
+                if (hipEventSynchronize(r.ev[slot]) != hipSuccess) return HY_E_DEVICE;
+                r.busy[slot] = false;
+            }
             std::memcpy(h, static_cast<const uint8_t*>(src) + off, len);
             if (hipMemcpyAsync(static_cast<uint8_t*>(dst) + off, h, len, hipMemcpyHostToDevice,
                                stream) != hipSuccess ||
                 hipEventRecord(r.ev[slot], stream) != hipSuccess)
                 return HY_E_DEVICE;
+            r.busy[slot] = true;
         }
         return HY_OK;
     });
@@ -108,6 +113,10 @@ int download_pageable(PinnedRing& r, int device, const Segment* segs, int nseg,
     const size_t np = pieces.size();
     return run_workers(device, np, [&](int t) -> int {
         auto issue = [&](size_t p, int slot) -> bool {
+            // An upload's DMA still reading the slot precedes this one on the stream only if
+            // it was queued on the same stream; wait for it regardless.
+            if (r.busy[slot] && hipEventSynchronize(r.ev[slot]) != hipSuccess) return false;
+            r.busy[slot] = true;
             return hipMemcpyAsync(r.ring + (size_t)slot * SLOT, pieces[p].src, pieces[p].len,
                                   hipMemcpyDeviceToHost, stream) == hipSuccess &&
                    hipEventRecord(r.ev[slot], stream) == hipSuccess;
@@ -119,6 +128,7 @@ int download_pageable(PinnedRing& r, int device, const Segment* segs, int nseg,
             // The next piece goes into the other slot, whose piece was copied out last round.
             if (p + T < np && !issue(p + T, 2 * t + (int)((i + 1) & 1))) return HY_E_DEVICE;
             if (hipEventSynchronize(r.ev[slot]) != hipSuccess) return HY_E_DEVICE;
+            r.busy[slot] = false;
             std::memcpy(pieces[p].dst, r.ring + (size_t)slot * SLOT, pieces[p].len);
         }
         return HY_OK;

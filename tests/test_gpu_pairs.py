@@ -464,3 +464,45 @@ def test_band_batches_on_both_upload_paths(monkeypatch):
                                           k), (npairs, mode, k)
             monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", "waves")
             assert hy.align_pairs(pairs, s, hy.Mode(mode), hy.SearchType.ScoreEnd) == got
+
+
+def test_large_batch_uploads_through_pinned_slots():
+    """A batch whose query and target buffers both exceed 8 MB: both go up through the pinned
+    slots in one call, each worker refilling its slots, the target upload reusing slots whose
+    query pieces may still be in flight, and the next call reusing them again. Every pair equals
+    the same pair aligned in sub-batches small enough for plain copies (a different kernel), and
+    a sample equals the oracle."""
+    rng = np.random.default_rng(1100)
+    n = 100000
+    ql = rng.integers(100, 141, size=n)
+    tl = rng.integers(200, 301, size=n)
+    q = rng.integers(0, 4, size=int(ql.sum()), dtype=np.uint8)
+    t = rng.integers(0, 4, size=int(tl.sum()), dtype=np.uint8)
+    qo = np.concatenate([[0], np.cumsum(ql)]).astype(np.uint64)
+    to = np.concatenate([[0], np.cumsum(tl)]).astype(np.uint64)
+    for k in range(0, n, 2):          # every other window holds its read
+        at = int(to[k]) + int(rng.integers(0, tl[k] - ql[k] + 1))
+        t[at:at + ql[k]] = q[qo[k]:qo[k + 1]]
+    assert q.size > (8 << 20) and t.size > (8 << 20)
+    scheme = dna_scheme()
+    s = scoring(scheme)
+    sc = hy.PairScratch()
+    st = hy.SearchType.ScoreEnd
+    whole = hy.align_pairs_flat(q, qo, t, to, s, hy.Mode.Sw, st, scratch=sc)
+    again = hy.align_pairs_flat(q, qo, t, to, s, hy.Mode.Sw, st, scratch=sc)
+    for a, b in zip(whole, again):
+        assert np.array_equal(a, b)
+    step = 20000
+    for k0 in range(0, n, step):
+        k1 = k0 + step
+        qa, qb, ta, tb = int(qo[k0]), int(qo[k1]), int(to[k0]), int(to[k1])
+        part = hy.align_pairs_flat(q[qa:qb], qo[k0:k1 + 1] - qo[k0], t[ta:tb],
+                                   to[k0:k1 + 1] - to[k0], s, hy.Mode.Sw, st, scratch=sc)
+        for a, b in zip(whole, part):
+            bad = np.flatnonzero(a[k0:k1] != b)
+            assert bad.size == 0, ("pair", k0 + int(bad[0]) if bad.size else None)
+    for k in range(0, n, 4999):
+        want = want_hit(q[qo[k]:qo[k + 1]], t[to[k]:to[k + 1]], scheme, MODE_SW, st)
+        assert (whole[0][k], whole[1][k], whole[2][k]) == (
+            want.score, -1 if want.query_end is None else want.query_end,
+            -1 if want.target_end is None else want.target_end), k
