@@ -205,6 +205,29 @@ struct PairArgs {
     int64_t* out_tend;
 };
 
+// Local-mode pairs two per lane (k_pairs16, alphabets up to kPairs16MaxAlphabet): a group
+// (PairGroup, count <= 128) puts slot first + 2l in the low 16-bit half of lane l and slot
+// first + 2l + 1 in the high half. With S = al + 1 (symbol al pads), a target word holds four
+// columns as bytes ta * S + tb, a query word two rows as 16-bit byte offsets
+// 4 * (qa * S + qb) * S^2 into the kernel's LDS table T[(qa*S + qb)*S^2 + ta*S + tb] =
+// (sub(qa, ta), sub(qb, tb)), padding pairs scoring -32768. Every pair's width proof gives 16
+// bits, its query is at most 4,096 and its target at most 65,535 symbols.
+constexpr uint32_t kPairs16MaxAlphabet = 7;
+struct Pair16Args {
+    const PairGroup* groups;
+    const uint32_t* twords;
+    const uint32_t* qwords;
+    const uint32_t* tlens;    // per slot
+    const uint32_t* qlens;    // per slot
+    const uint32_t* pidx;     // per slot -> pair index
+    const int32_t* matrix;    // al x al
+    int al, go, ge;
+    uint2* spill;             // tile boundary rows (H, F') per column
+    int32_t* out_score;       // per pair
+    int64_t* out_qend;
+    int64_t* out_tend;
+};
+
 struct PairPackArgs {
     const PairGroup* groups;
     const uint8_t* qflat;
@@ -322,6 +345,11 @@ hipError_t launch_db_pack16(const Group* groups, uint32_t ngroups, const uint32_
                            hipStream_t stream);
 // Pairs: lane-interleave a batch's queries and targets (one workgroup per group).
 hipError_t launch_pair_pack(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream);
+// The two-pairs-per-lane words of k_pairs16 (a.pad_symbol = al, a.qtile its tile rows).
+hipError_t launch_pair_pack16(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream);
+// Local-mode pairs two per lane; q: rows per tile (16 or 32).
+hipError_t launch_pairs16_sw(bool ends, const Pair16Args& a, uint32_t ngroups, int q,
+                             hipStream_t stream);
 // Pairs: one launch over `ngroups` groups. q: rows per tile (16 or 32).
 hipError_t launch_pairs32(int mode, bool ends, const PairArgs& a, uint32_t ngroups, int q,
                           hipStream_t stream);

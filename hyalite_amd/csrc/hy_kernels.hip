@@ -205,55 +205,7 @@ __global__ __launch_bounds__(64) void k_inter32(InterArgs a) {
 // HG[i-1][j-1] + (sub + go) equals H[i-1][j-1] + sub whenever the sum is in range (a saturated
 // diagonal is below every real cell and loses the max). Padding cells are bounded by saturation
 // and never flow into real cells; in local mode they never exceed a real cell.
-// The tile code is written with lambdas over the tile's register arrays; every one must be
-// inlined, or the arrays they capture are moved to scratch memory.
-#define HY_INLINE __attribute__((always_inline))
 
-typedef short s2v __attribute__((ext_vector_type(2)));
-
-__device__ __forceinline__ s2v as_s2(uint32_t u) { return __builtin_bit_cast(s2v, u); }
-__device__ __forceinline__ uint32_t as_u32(s2v v) { return __builtin_bit_cast(uint32_t, v); }
-__device__ __forceinline__ s2v p_add(s2v a, s2v b) { return __builtin_elementwise_add_sat(a, b); }
-__device__ __forceinline__ s2v p_sub(s2v a, s2v b) { return __builtin_elementwise_sub_sat(a, b); }
-__device__ __forceinline__ s2v p_max(s2v a, s2v b) { return __builtin_elementwise_max(a, b); }
-// Unsigned saturating subtraction (v_pk_sub_u16 with clamp): max(a - b, 0) for a, b in [0, 32767].
-typedef unsigned short u2v __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ s2v p_subu(s2v a, s2v b) {
-    return __builtin_bit_cast(s2v, __builtin_elementwise_sub_sat(__builtin_bit_cast(u2v, a),
-                                                                 __builtin_bit_cast(u2v, b)));
-}
-__device__ __forceinline__ s2v p_minu(s2v a, s2v b) {
-    return __builtin_bit_cast(s2v, __builtin_elementwise_min(__builtin_bit_cast(u2v, a),
-                                                             __builtin_bit_cast(u2v, b)));
-}
-__device__ __forceinline__ s2v p_maxu(s2v a, s2v b) {
-    return __builtin_bit_cast(s2v, __builtin_elementwise_max(__builtin_bit_cast(u2v, a),
-                                                             __builtin_bit_cast(u2v, b)));
-}
-// a - b per 16-bit half, wrapping (v_pk_sub_u16 without clamp).
-__device__ __forceinline__ s2v p_subw(s2v a, s2v b) {
-    return __builtin_bit_cast(s2v, __builtin_bit_cast(u2v, a) - __builtin_bit_cast(u2v, b));
-}
-// a * b + c per 16-bit half, wrapping (v_pk_mul_lo_u16 / v_pk_mad_u16).
-__device__ __forceinline__ s2v p_madu(s2v a, s2v b, s2v c) {
-    return __builtin_bit_cast(s2v, __builtin_bit_cast(u2v, a) * __builtin_bit_cast(u2v, b) +
-                                       __builtin_bit_cast(u2v, c));
-}
-__device__ __forceinline__ int sat16(long long v) {
-    return v < -32768 ? -32768 : (v > 32767 ? 32767 : (int)v);
-}
-__device__ __forceinline__ s2v splat2(int v) {
-    const uint32_t u = (uint32_t)(uint16_t)(short)v;
-    return as_s2(u | (u << 16));
-}
-__device__ __forceinline__ s2v pair2(int lo, int hi) {
-    return as_s2((uint32_t)(uint16_t)(short)lo | ((uint32_t)(uint16_t)(short)hi << 16));
-}
-// Keep v in the halves selected by (lo, hi), the 16-bit sentinel elsewhere.
-__device__ __forceinline__ s2v keep2(s2v v, bool lo, bool hi) {
-    const uint32_t m = (lo ? 0x0000FFFFu : 0u) | (hi ? 0xFFFF0000u : 0u);
-    return as_s2((as_u32(v) & m) | (0x80008000u & ~m));
-}
 
 template <int MODE>
 __device__ __forceinline__ int border16(int i, int j, int go, int ge) {

@@ -55,6 +55,51 @@ __global__ __launch_bounds__(64) void k_pair_pack(PairPackArgs a) {
     }
 }
 
+// The two-pairs-per-lane layout of k_pairs16: lane l packs slots 2l (low) and 2l + 1 (high) of
+// its group; a target word holds four columns as bytes ta * S + tb, a query word two rows as
+// 16-bit byte offsets 4 * (qa * S + qb) * S^2 (S = al + 1, symbol al pads; symbols past the
+// alphabet become padding, as in k_pair_pack).
+__global__ __launch_bounds__(64) void k_pair_pack16(PairPackArgs a) {
+    const PairGroup g = a.groups[blockIdx.x];
+    const int lane = threadIdx.x;
+    const uint32_t pad = a.pad_symbol, S = pad + 1;
+    const uint32_t sa = 2 * lane, sb = 2 * lane + 1;
+    const bool la = sa < g.count, lb = sb < g.count;
+    const uint32_t slota = g.first + (la ? sa : 0), slotb = g.first + (lb ? sb : 0);
+    const uint32_t pa = a.pidx[slota], pb = a.pidx[slotb];
+    const uint64_t tna = la ? a.tlens[slota] : 0, tnb = lb ? a.tlens[slotb] : 0;
+    const uint64_t qna = la ? a.qlens[slota] : 0, qnb = lb ? a.qlens[slotb] : 0;
+    const uint8_t* ta = a.tflat + a.toff[pa];
+    const uint8_t* tb = a.tflat + a.toff[pb];
+    const uint8_t* qa = a.qflat + a.qoff[pa];
+    const uint8_t* qb = a.qflat + a.qoff[pb];
+    const uint32_t tw = (g.ncols + 3) / 4;
+    for (uint32_t w = 0; w < tw; ++w) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const uint64_t c = (uint64_t)w * 4 + k;
+            const uint32_t x = c < tna ? min((uint32_t)ta[c], pad) : pad;
+            const uint32_t y = c < tnb ? min((uint32_t)tb[c], pad) : pad;
+            v |= (x * S + y) << (8 * k);
+        }
+        a.twords[g.tword_off + (uint64_t)w * 64 + lane] = v;
+    }
+    const uint32_t qt = (uint32_t)a.qtile;
+    const uint32_t qw = (g.nrows + qt - 1) / qt * qt / 2;
+    for (uint32_t w = 0; w < qw; ++w) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const uint64_t r = (uint64_t)w * 2 + k;
+            const uint32_t x = r < qna ? min((uint32_t)qa[r], pad) : pad;
+            const uint32_t y = r < qnb ? min((uint32_t)qb[r], pad) : pad;
+            v |= (4 * (x * S + y) * S * S) << (16 * k);
+        }
+        a.qwords[g.qword_off + (uint64_t)w * 64 + lane] = v;
+    }
+}
+
 // Database layouts built on the device from the residues in database order: one workgroup per
 // group, lane l gathering its target's four residues per word (the padding symbol past its
 // length, and for lanes past the group's count).
@@ -249,6 +294,13 @@ hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t stream
 hipError_t launch_pair_pack(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream) {
     if (ngroups == 0) return hipSuccess;
     hipLaunchKernelGGL(k_pair_pack, dim3(ngroups), dim3(64), 0, stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_pair_pack16(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream) {
+    if (ngroups == 0) return hipSuccess;
+    if (a.pad_symbol > kPairs16MaxAlphabet) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pair_pack16, dim3(ngroups), dim3(64), 0, stream, a);
     return hipGetLastError();
 }
 

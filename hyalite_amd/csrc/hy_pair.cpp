@@ -203,6 +203,11 @@ int kernel_choice() {
 
 bool band_eligible(uint32_t al) { return al >= 1 && al <= hy::kBandMaxAlphabet; }
 
+bool pairs16_enabled() {
+    const char* v = std::getenv("HYALITE_HIP_PAIRS16");
+    return !(v && std::strcmp(v, "0") == 0);
+}
+
 // Pairs that do not run one per lane go to the band kernel when the alphabet fits it, except
 // score-only batches of more than kBandMaxPairs: there one wavefront per pair fills the chip and
 // its leaner score-only loop wins. Measured (device time, SW, DNA): 16,384 pairs of 150 x 1,400
@@ -693,13 +698,33 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         else waves.push_back(k);
     }
     const bool band = use_band(pick, alphabet_len, waves.size(), tracks_end(search_type));
-    // Lanes: sorted by (query length, target length) descending, cut into 64-lane groups. The
-    // sort key packs both lengths (complemented, 13 + 17 bits: lane pairs are at most 4096 x
-    // 65536 unless the lane kernel is forced) above the pair index.
-    {
-        std::vector<uint64_t> keys(lanes.size());
-        for (size_t i = 0; i < lanes.size(); ++i) {
-            const uint64_t k = lanes[i];
+    // Local-mode lane pairs of alphabets up to kPairs16MaxAlphabet whose width proof gives 16
+    // bits (queries up to kLaneMaxQuery, targets below 65,536) run two per lane (k_pairs16).
+    // HYALITE_HIP_PAIRS16=0 keeps them one per lane.
+    std::vector<uint32_t> lanes16;
+    if (mode == HY_MODE_SW && alphabet_len <= hy::kPairs16MaxAlphabet && pairs16_enabled()) {
+        std::vector<uint32_t> rest;
+        rest.reserve(lanes.size());
+        lanes16.reserve(lanes.size());
+        for (uint32_t k : lanes) {
+            int64_t b = 0;
+            const int bits = (ql[k] <= kLaneMaxQuery && tl[k] <= 65535)
+                                 ? hy::width_bits_for(hy::magnitude_bound(HY_MODE_SW, sc.min_e,
+                                                                          sc.max_e, gap_open,
+                                                                          gap_ext, ql[k], tl[k]),
+                                                      &b)
+                                 : 0;
+            (bits != 0 && bits <= 16 ? lanes16 : rest).push_back(k);
+        }
+        lanes.swap(rest);
+    }
+    // Lanes: sorted by (query length, target length) descending, cut into groups. The sort key
+    // packs both lengths (complemented, 13 + 17 bits: lane pairs are at most 4096 x 65536 unless
+    // the lane kernel is forced) above the pair index.
+    auto sort_lanes = [&](std::vector<uint32_t>& list) {
+        std::vector<uint64_t> keys(list.size());
+        for (size_t i = 0; i < list.size(); ++i) {
+            const uint64_t k = list[i];
             const uint64_t qk = 0x1FFFull - std::min<uint64_t>(ql[k], 0x1FFF);
             const uint64_t tk = 0x1FFFFull - std::min<uint64_t>(tl[k], 0x1FFFF);
             keys[i] = (qk << 51) | (tk << 34) | k;
@@ -722,8 +747,11 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
             for (uint64_t k : keys) tmp[cnt[(k >> shift) & 0x7FFF]++] = k;
             keys.swap(tmp);
         }
-        for (size_t i = 0; i < lanes.size(); ++i) lanes[i] = (uint32_t)(keys[i] & 0x3FFFFFFFFull);
-    }
+        for (size_t i = 0; i < list.size(); ++i) list[i] = (uint32_t)(keys[i] & 0x3FFFFFFFFull);
+    };
+    sort_lanes(lanes);
+    sort_lanes(lanes16);
+    const size_t nslots = lanes.size() + lanes16.size();
     // The rest: ordered by the kernel's rows class (rows per pass of the band kernel, rows per
     // lane of the one-wavefront kernel) for one launch per class.
     std::stable_sort(waves.begin(), waves.end(), [&](uint32_t a, uint32_t b) {
@@ -739,25 +767,40 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     // (Score), 1.78k vs 1.62k (ScoreEnd). HYALITE_HIP_PAIRS_TILE=16|32 overrides it.
     int qtile = ends ? 16 : 32;
     if (const char* v = std::getenv("HYALITE_HIP_PAIRS_TILE")) qtile = std::atoi(v) >= 32 ? 32 : 16;
+    // Two pairs per lane: 32-row tiles for both search types (200,000 150 x 500 DNA pairs:
+    // 3.57k vs 3.34k GCUPS Score, 2.76k vs 2.63k ScoreEnd against 16-row tiles).
+    // HYALITE_HIP_PAIRS16_TILE=16|32 overrides it.
+    int qtile16 = 32;
+    if (const char* v = std::getenv("HYALITE_HIP_PAIRS16_TILE"))
+        qtile16 = std::atoi(v) >= 32 ? 32 : 16;
+    // Groups of both lane kernels in one array (k_pairs32's first), their slots in one list
+    // (lanes, then lanes16), their words and boundary rows in shared buffers.
     std::vector<hy::PairGroup> groups;
     uint64_t twords = 0, qwords = 0, spill = 0;
-    for (size_t g0 = 0; g0 < lanes.size(); g0 += 64) {
-        hy::PairGroup g{};
-        g.first = (uint32_t)g0;
-        g.count = (uint32_t)std::min<size_t>(64, lanes.size() - g0);
-        for (uint32_t l = 0; l < g.count; ++l) {
-            g.ncols = std::max<uint32_t>(g.ncols, (uint32_t)tl[lanes[g0 + l]]);
-            g.nrows = std::max<uint32_t>(g.nrows, (uint32_t)ql[lanes[g0 + l]]);
+    auto add_groups = [&](const std::vector<uint32_t>& list, size_t base, uint32_t per, int qt,
+                          int rows_per_word) {
+        for (size_t g0 = 0; g0 < list.size(); g0 += per) {
+            hy::PairGroup g{};
+            g.first = (uint32_t)(base + g0);
+            g.count = (uint32_t)std::min<size_t>(per, list.size() - g0);
+            for (uint32_t l = 0; l < g.count; ++l) {
+                g.ncols = std::max<uint32_t>(g.ncols, (uint32_t)tl[list[g0 + l]]);
+                g.nrows = std::max<uint32_t>(g.nrows, (uint32_t)ql[list[g0 + l]]);
+            }
+            const uint64_t tiles = (g.nrows + qt - 1) / qt;
+            g.tword_off = twords;
+            twords += (uint64_t)((g.ncols + 3) / 4) * 64;
+            g.qword_off = qwords;
+            qwords += tiles * (uint64_t)(qt / rows_per_word) * 64;
+            g.spill_off = tiles > 1 ? spill : 0;
+            if (tiles > 1) spill += ((uint64_t)g.ncols + 1) * 64;
+            groups.push_back(g);
         }
-        const uint64_t tiles = (g.nrows + qtile - 1) / qtile;
-        g.tword_off = twords;
-        twords += (uint64_t)((g.ncols + 3) / 4) * 64;
-        g.qword_off = qwords;
-        qwords += tiles * (uint64_t)(qtile / 4) * 64;
-        g.spill_off = tiles > 1 ? spill : 0;
-        if (tiles > 1) spill += ((uint64_t)g.ncols + 1) * 64;
-        groups.push_back(g);
-    }
+    };
+    add_groups(lanes, 0, 64, qtile, 4);
+    const size_t ng32 = groups.size();
+    add_groups(lanes16, lanes.size(), 128, qtile16, 2);
+    const size_t ng16 = groups.size() - ng32;
 
     Stager st{s};
     // Flat sequences go to the device straight from the caller's buffers; per-pair pointers are
@@ -776,11 +819,11 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         o_list = st.add(&s->list, waves.size() * sizeof(uint32_t));
         o_nq = st.add(&s->nq, sizeof(uint32_t));
     }
-    if (!lanes.empty()) {
+    if (nslots) {
         o_groups = st.add(&s->groups, groups.size() * sizeof(hy::PairGroup));
-        o_tl = st.add(&s->tlens, lanes.size() * sizeof(uint32_t));
-        o_ql = st.add(&s->qlens, lanes.size() * sizeof(uint32_t));
-        o_pi = st.add(&s->pidx, lanes.size() * sizeof(uint32_t));
+        o_tl = st.add(&s->tlens, nslots * sizeof(uint32_t));
+        o_ql = st.add(&s->qlens, nslots * sizeof(uint32_t));
+        o_pi = st.add(&s->pidx, nslots * sizeof(uint32_t));
     }
     if ((rc = grow_stage(s, st.used + matrix_len * sizeof(int32_t) + 64)) != HY_OK) return rc;
     stage_matrix(st, s, matrix, matrix_len);
@@ -824,15 +867,19 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
             const uint32_t np = (uint32_t)npairs;
             std::memcpy(st.at(o_nq), &np, sizeof(np));
         }
-        if (!lanes.empty()) {
+        if (nslots) {
             std::memcpy(st.at(o_groups), groups.data(), groups.size() * sizeof(hy::PairGroup));
             uint32_t* tls = reinterpret_cast<uint32_t*>(st.at(o_tl));
             uint32_t* qls = reinterpret_cast<uint32_t*>(st.at(o_ql));
-            for (size_t k = 0; k < lanes.size(); ++k) {
-                tls[k] = (uint32_t)tl[lanes[k]];
-                qls[k] = (uint32_t)ql[lanes[k]];
-            }
-            std::memcpy(st.at(o_pi), lanes.data(), lanes.size() * sizeof(uint32_t));
+            uint32_t* pis = reinterpret_cast<uint32_t*>(st.at(o_pi));
+            size_t i = 0;
+            for (const std::vector<uint32_t>* list : {&lanes, &lanes16})
+                for (uint32_t k : *list) {
+                    tls[i] = (uint32_t)tl[k];
+                    qls[i] = (uint32_t)ql[k];
+                    pis[i] = k;
+                    ++i;
+                }
         }
     }
     if ((rc = st.commit()) != HY_OK) return rc;
@@ -861,7 +908,7 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
 
     if (hipEventRecord(s->ev0, s->stream) != hipSuccess) return HY_E_DEVICE;
     // Long pairs and small batches: the aux stream, forked after the uploads.
-    const bool both = !waves.empty() && !lanes.empty();
+    const bool both = !waves.empty() && nslots > 0;
     hipStream_t wstream = both ? s->aux : s->stream;
     if (both && (hipEventRecord(s->fork, s->stream) != hipSuccess ||
                  hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess))
@@ -889,7 +936,7 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         }
         if (rc != HY_OK) return rc;
     }
-    if (!lanes.empty()) {
+    if (nslots) {
         if ((rc = grow(s->twords, twords * 4)) != HY_OK ||
             (rc = grow(s->qwords, qwords * 4)) != HY_OK ||
             (rc = grow(s->spill, std::max<uint64_t>(spill, 1) * sizeof(int2))) != HY_OK)
@@ -907,7 +954,7 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         pk.twords = static_cast<uint32_t*>(s->twords.p);
         pk.pad_symbol = al_eff;
         pk.qtile = qtile;
-        if (hy::launch_pair_pack(pk, (uint32_t)groups.size(), s->stream) != hipSuccess)
+        if (hy::launch_pair_pack(pk, (uint32_t)ng32, s->stream) != hipSuccess)
             return HY_E_DEVICE;
         hy::PairArgs pa{};
         pa.groups = pk.groups;
@@ -925,9 +972,34 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         pa.out_score = static_cast<int32_t*>(s->scores.p);
         pa.out_qend = static_cast<int64_t*>(s->qend.p);
         pa.out_tend = static_cast<int64_t*>(s->tend.p);
-        if (hy::launch_pairs32((int)mode, ends, pa, (uint32_t)groups.size(), qtile, s->stream) !=
+        if (hy::launch_pairs32((int)mode, ends, pa, (uint32_t)ng32, qtile, s->stream) !=
             hipSuccess)
             return HY_E_DEVICE;
+        if (ng16) {
+            hy::PairPackArgs pk16 = pk;
+            pk16.groups = pk.groups + ng32;
+            pk16.pad_symbol = alphabet_len;
+            pk16.qtile = qtile16;
+            hy::Pair16Args p16{};
+            p16.groups = pk16.groups;
+            p16.twords = pk.twords;
+            p16.qwords = pk.qwords;
+            p16.tlens = pk.tlens;
+            p16.qlens = pk.qlens;
+            p16.pidx = pk.pidx;
+            p16.matrix = pa.matrix;
+            p16.al = (int)alphabet_len;
+            p16.go = gap_open;
+            p16.ge = gap_ext;
+            p16.spill = static_cast<uint2*>(s->spill.p);
+            p16.out_score = pa.out_score;
+            p16.out_qend = pa.out_qend;
+            p16.out_tend = pa.out_tend;
+            if (hy::launch_pair_pack16(pk16, (uint32_t)ng16, s->stream) != hipSuccess ||
+                hy::launch_pairs16_sw(ends, p16, (uint32_t)ng16, qtile16, s->stream) !=
+                    hipSuccess)
+                return HY_E_DEVICE;
+        }
     }
     if (both && (hipEventRecord(s->join, s->aux) != hipSuccess ||
                  hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess))

@@ -477,6 +477,182 @@ hipError_t launch_band_o(const BandArgs& a, uint32_t nblocks, int m, hipStream_t
 }
 
 #if HY_KMODE == 0
+// ------------------------------------------------------------------------------------------------
+// Local-mode pairs, two per lane in packed 16-bit arithmetic.
+// ------------------------------------------------------------------------------------------------
+// Lane l aligns slot first + 2l in the low half and first + 2l + 1 in the high half of every
+// register. One LDS lookup gives both halves' substitution scores: with S = al + 1 (symbol al
+// pads) the table T[(qa*S + qb)*S^2 + ta*S + tb] holds (sub(qa, ta), sub(qb, tb)); a lane keeps
+// the byte offsets of its tile's query-row pairs in registers and reads the column's target pair
+// from its words. The recurrence is the packed local one of the scan kernels, every quantity
+// floored at 0 (H exact, E' = max(E, 0), F' = max(F, 0)):
+//     H = max(H[i-1][j-1] + sub, E', F'),  E' = max(H - go, E' - ge)+,  F' = max(H - go, F' - ge)+
+// so a cell pair costs the lookup and 9 packed operations. Cells past a pair's own query or
+// target (padding, -32768) never exceed a real cell that precedes them in (column, row) order,
+// so neither the running maximum nor the strictly-greater end tracking needs a mask. With ends,
+// each column keeps its maximum and the first row reaching it; a column that strictly beats the
+// tile's best replaces it; each tile's best is folded into the lane's with the full tie-break.
+template <int Q, bool ENDS>
+__global__ __launch_bounds__(64) void k_pairs16(Pair16Args a) {
+    extern __shared__ uint32_t tab[];
+    const int lane = threadIdx.x;
+    const PairGroup g = a.groups[blockIdx.x];
+    const int al = a.al, S = al + 1, S2 = S * S, go = a.go, ge = a.ge;
+    for (int k = lane; k < S2 * S2; k += 64) {
+        const int qa = k / (S2 * S), qb = (k / S2) % S, ta = (k / S) % S, tb = k % S;
+        const int lo = (qa < al && ta < al) ? sat16(a.matrix[qa * al + ta]) : -32768;
+        const int hi = (qb < al && tb < al) ? sat16(a.matrix[qb * al + tb]) : -32768;
+        tab[k] = as_u32(pair2(lo, hi));
+    }
+    __syncthreads();
+
+    const s2v go2 = splat2(go), ge2 = splat2(ge), zero2 = splat2(0), one2 = splat2(1);
+    const int L = (int)g.ncols;
+    const int ntiles = ((int)g.nrows + Q - 1) / Q;
+    uint2* sp = a.spill + g.spill_off + lane;
+    const uint32_t* wp = a.twords + g.tword_off + lane;
+    const uint32_t* qp = a.qwords + g.qword_off + lane;
+    // Lane bests per half (score, column, row); (0, 0, 0) is the empty alignment.
+    int bs[2] = {0, 0}, bj[2] = {0, 0}, bi[2] = {0, 0};
+    s2v runq[4] = {zero2, zero2, zero2, zero2};
+
+    for (int c = 0; c < ntiles; ++c) {
+        const int i0 = c * Q + 1;
+        uint32_t qrow[Q];
+        s2v HG[Q], E[Q];
+        s2v tb = zero2, tc = zero2, tr = zero2;  // tile best, its column and row in the tile
+        auto cell = [&](int r, s2v sg, s2v& hd, s2v& f, s2v& cm, s2v& cr) HY_INLINE {
+            const s2v h = p_max(p_max(p_add(hd, sg), E[r]), f);
+            hd = HG[r];
+            HG[r] = h;
+            const s2v hg = p_subu(h, go2);
+            E[r] = p_max(hg, p_subu(E[r], ge2));
+            f = p_max(hg, p_subu(f, ge2));
+            if constexpr (ENDS) {
+                const s2v up = p_minu(p_subu(h, cm), one2);
+                cm = p_max(cm, h);
+                cr = p_maxu(cr, p_madu(up, splat2(r), zero2));
+            } else {
+                runq[r & 3] = p_max(runq[r & 3], h);
+            }
+        };
+        auto close_column = [&](int j, s2v cm, s2v cr) HY_INLINE {
+            const s2v up = p_minu(p_subu(cm, tb), one2);
+            tb = p_max(tb, cm);
+            tc = p_maxu(tc, p_madu(up, splat2(j), zero2));
+            tr = p_madu(up, p_subw(cr, tr), tr);
+        };
+#pragma unroll
+        for (int r2 = 0; r2 < Q / 2; ++r2) {
+            const uint32_t qw = qp[(size_t)(c * (Q / 2) + r2) * 64];
+            qrow[2 * r2] = qw & 0xffffu;
+            qrow[2 * r2 + 1] = qw >> 16;
+        }
+#pragma unroll
+        for (int r = 0; r < Q; ++r) {
+            HG[r] = zero2;
+            E[r] = zero2;
+        }
+        s2v HGd = zero2;
+
+        uint32_t w = L > 0 ? wp[0] : 0u;
+        uint2 spv[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            spv[b] = (c > 0 && b + 1 <= L) ? sp[(size_t)(b + 1) * 64] : make_uint2(0u, 0u);
+        for (int jb = 0; jb < L; jb += 4) {
+            uint32_t wn = 0u;
+            uint2 spn[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) spn[b] = make_uint2(0u, 0u);
+            if (jb + 4 < L) {
+                wn = wp[(size_t)((jb >> 2) + 1) * 64];
+                if (c > 0) {
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        if (jb + 5 + b <= L) spn[b] = sp[(size_t)(jb + 5 + b) * 64];
+                }
+            }
+            // Columns in pairs (j, j + 1), skewed by one row: row r of column j and row r - 1
+            // of column j + 1 are independent, so two dependency chains interleave. A column
+            // past the group's end (odd L) computes padding that never wins and is not stored.
+#pragma unroll
+            for (int bp = 0; bp < 4; bp += 2) {
+                const int jA = jb + bp + 1, jB = jA + 1;
+                if (jA > L) break;
+                const uint32_t tpA = 4u * ((w >> (8 * bp)) & 0xffu);
+                const uint32_t tpB = 4u * ((w >> (8 * bp + 8)) & 0xffu);
+                s2v sgA[Q], sgB[Q];
+#pragma unroll
+                for (int r = 0; r < Q; ++r) {
+                    sgA[r] = as_s2(*reinterpret_cast<const uint32_t*>(
+                        reinterpret_cast<const uint8_t*>(tab) + qrow[r] + tpA));
+                    sgB[r] = as_s2(*reinterpret_cast<const uint32_t*>(
+                        reinterpret_cast<const uint8_t*>(tab) + qrow[r] + tpB));
+                }
+                s2v fA = as_s2(spv[bp].y), fB = as_s2(spv[bp + 1].y);
+                s2v hdA = HGd, hdB = as_s2(spv[bp].x);
+                HGd = as_s2(spv[bp + 1].x);
+                s2v cmA = zero2, crA = zero2, cmB = zero2, crB = zero2;
+                uint2 outA = make_uint2(0u, 0u);
+#pragma unroll
+                for (int r = 0; r <= Q; ++r) {
+                    if (r < Q) cell(r, sgA[r], hdA, fA, cmA, crA);
+                    if (r == Q - 1) outA = make_uint2(as_u32(HG[Q - 1]), as_u32(fA));
+                    if (r >= 1) cell(r - 1, sgB[r - 1], hdB, fB, cmB, crB);
+                }
+                if (c + 1 < ntiles) {
+                    sp[(size_t)jA * 64] = outA;
+                    if (jB <= L) sp[(size_t)jB * 64] = make_uint2(as_u32(HG[Q - 1]), as_u32(fB));
+                }
+                if constexpr (ENDS) {
+                    close_column(jA, cmA, crA);
+                    close_column(jB, cmB, crB);
+                }
+            }
+            w = wn;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) spv[b] = spn[b];
+        }
+        if constexpr (ENDS) {
+            const uint32_t ub = as_u32(tb), uc = as_u32(tc), ur = as_u32(tr);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int sc = (int)(short)(uint16_t)(ub >> (16 * h));
+                if (sc > 0)
+                    consider(bs[h], bj[h], bi[h], sc, (int)((uc >> (16 * h)) & 0xffffu),
+                             i0 + (int)((ur >> (16 * h)) & 0xffffu));
+            }
+        }
+    }
+
+    if constexpr (!ENDS) {
+        const uint32_t u = as_u32(p_max(p_max(runq[0], runq[1]), p_max(runq[2], runq[3])));
+        bs[0] = (int)(short)(uint16_t)u;
+        bs[1] = (int)(short)(uint16_t)(u >> 16);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t sl = 2 * lane + h;
+        if (sl < g.count) {
+            const uint32_t pi = a.pidx[g.first + sl];
+            a.out_score[pi] = bs[h];
+            if (ENDS) {
+                a.out_qend[pi] = bi[h] == 0 ? -1 : (int64_t)bi[h] - 1;
+                a.out_tend[pi] = bj[h] == 0 ? -1 : (int64_t)bj[h] - 1;
+            }
+        }
+    }
+}
+
+template <int Q, bool ENDS>
+hipError_t launch_pairs16_q(const Pair16Args& a, uint32_t ngroups, hipStream_t st) {
+    const int S = a.al + 1;
+    const size_t lds = (size_t)S * S * S * S * sizeof(uint32_t);
+    hipLaunchKernelGGL((k_pairs16<Q, ENDS>), dim3(ngroups), dim3(64), lds, st, a);
+    return hipGetLastError();
+}
+
 // Local mode, one pair per wavefront, R query rows per lane (the long-target kernel's shape).
 template <int R, bool LDSMAT>
 __global__ __launch_bounds__(64) void k_posmax(PosMaxArgs a) {
@@ -630,6 +806,16 @@ hipError_t launch_band32_m(int out, const BandArgs& a, uint32_t nblocks, int m, 
 template hipError_t launch_band32_m<HY_KMODE>(int, const BandArgs&, uint32_t, int, hipStream_t);
 
 #if HY_KMODE == 0
+hipError_t launch_pairs16_sw(bool ends, const Pair16Args& a, uint32_t ngroups, int q,
+                             hipStream_t st) {
+    if (ngroups == 0) return hipSuccess;
+    if (a.al < 1 || a.al > (int)kPairs16MaxAlphabet) return hipErrorInvalidValue;
+    if (ends) return q <= 16 ? launch_pairs16_q<16, true>(a, ngroups, st)
+                             : launch_pairs16_q<32, true>(a, ngroups, st);
+    return q <= 16 ? launch_pairs16_q<16, false>(a, ngroups, st)
+                   : launch_pairs16_q<32, false>(a, ngroups, st);
+}
+
 // Rows per lane as for the long-target kernel: 2, 4, 6 or 8 (several passes beyond 512 rows).
 hipError_t launch_posmax_sw(const PosMaxArgs& a, hipStream_t st) {
     if (a.m <= 128) return launch_posmax_r<2>(a, st);

@@ -506,3 +506,47 @@ def test_large_batch_uploads_through_pinned_slots():
         assert (whole[0][k], whole[1][k], whole[2][k]) == (
             want.score, -1 if want.query_end is None else want.query_end,
             -1 if want.target_end is None else want.target_end), k
+
+
+@pytest.mark.parametrize("al", [2, 4, 7])
+def test_two_pairs_per_lane_equal_one_per_lane(al, monkeypatch):
+    """Local-mode lane pairs of small alphabets run two per lane in 16-bit halves (k_pairs16):
+    scores and ends equal the one-pair-per-lane kernel and the oracle, with ties (identity and
+    all-equal matrices), queries across 16/32-row tiles, an odd last group (only low halves
+    live), empty sides, and pairs whose width proof needs 32 bits mixed in (they stay one per
+    lane in the same call)."""
+    rng = np.random.default_rng(1200 + al)
+    schemes = [(al, identity_matrix(al, 2, -1), 2, 1),
+               (al, [int(v) for v in rng.integers(-3, 5, size=al * al)], 3, 1),
+               (al, [1] * (al * al), 1, 0)]
+    monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", "lanes")
+    for scheme in schemes:
+        s = scoring(scheme)
+        pairs = []
+        for k in range(257):
+            m = int(rng.choice([0, 1, 15, 16, 17, 31, 32, 33, 64, 100]))
+            n = int(rng.choice([0, 1, 3, 4, 5, 63, 64, 65, 300]))
+            q = rng.integers(0, al, size=m, dtype=np.uint8)
+            t = rng.integers(0, al, size=n, dtype=np.uint8)
+            if m and n >= m and k % 3 == 0:
+                at = int(rng.integers(0, n - m + 1))
+                t[at:at + m] = q
+            pairs.append((q, t))
+        for st in ST:
+            monkeypatch.delenv("HYALITE_HIP_PAIRS16", raising=False)
+            two = hy.align_pairs(pairs, s, hy.Mode.Sw, st)
+            monkeypatch.setenv("HYALITE_HIP_PAIRS16", "0")
+            one = hy.align_pairs(pairs, s, hy.Mode.Sw, st)
+            assert two == one, (al, scheme[1][:4], st)
+            for k, (q, t) in enumerate(pairs):
+                assert two[k] == want_hit(list(q), list(t), scheme, MODE_SW, st, k), (al, k)
+    # 32-bit pairs beside 16-bit ones: a large match score pushes long pairs past 32,767.
+    big = (al, identity_matrix(al, 300, -1), 5, 1)
+    s = scoring(big)
+    pairs = [(rng.integers(0, al, size=m, dtype=np.uint8), rng.integers(0, al, size=n,
+                                                                        dtype=np.uint8))
+             for m, n in [(20, 40), (150, 200), (10, 10), (120, 130), (3, 500)] * 30]
+    monkeypatch.delenv("HYALITE_HIP_PAIRS16", raising=False)
+    got = hy.align_pairs(pairs, s, hy.Mode.Sw, hy.SearchType.ScoreEnd)
+    for k, (q, t) in enumerate(pairs):
+        assert got[k] == want_hit(list(q), list(t), big, MODE_SW, hy.SearchType.ScoreEnd, k), k
