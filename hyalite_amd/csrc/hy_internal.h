@@ -314,8 +314,12 @@ struct PinnedRing {
 };
 // Below this a plain hipMemcpyAsync is as fast.
 constexpr size_t kPinnedRingMin = 8ull << 20;
+// max_byte (optional) receives the largest byte of src, found while copying it.
 int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
-                    hipStream_t stream);
+                    hipStream_t stream, uint8_t* max_byte = nullptr);
+// The largest byte of p[0, n) (host memory), and a copy that finds it on the way.
+uint8_t host_max_byte(const uint8_t* p, size_t n);
+uint8_t copy_max_byte(uint8_t* dst, const uint8_t* src, size_t n);
 struct Segment {
     void* dst;
     const void* src;
@@ -360,8 +364,6 @@ hipError_t launch_posmax(const PosMaxArgs& a, hipStream_t stream);
 hipError_t launch_band32(int mode, int out, const BandArgs& a, uint32_t nblocks, int m,
                          hipStream_t stream);
 hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t stream);
-// The largest byte of p[0, n) folded into *out (an atomic max; zero *out first).
-hipError_t launch_max_byte(const uint8_t* p, uint64_t n, uint32_t* out, hipStream_t stream);
 // Traceback: the DP over a.nitems items, then the walk (one lane per item).
 hipError_t launch_trace(int mode, const TraceArgs& a, hipStream_t stream);
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,

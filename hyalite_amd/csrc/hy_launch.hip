@@ -59,9 +59,10 @@ __global__ __launch_bounds__(64) void k_pair_pack(PairPackArgs a) {
 // its group; a target word holds four columns as bytes ta * S + tb, a query word two rows as
 // 16-bit byte offsets 4 * (qa * S + qb) * S^2 (S = al + 1, symbol al pads; symbols past the
 // alphabet become padding, as in k_pair_pack).
-__global__ __launch_bounds__(64) void k_pair_pack16(PairPackArgs a) {
+// Four threads per lane (256 per group), each writing every fourth word of the lane.
+__global__ __launch_bounds__(256) void k_pair_pack16(PairPackArgs a) {
     const PairGroup g = a.groups[blockIdx.x];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
     const uint32_t pad = a.pad_symbol, S = pad + 1;
     const uint32_t sa = 2 * lane, sb = 2 * lane + 1;
     const bool la = sa < g.count, lb = sb < g.count;
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(64) void k_pair_pack16(PairPackArgs a) {
     const uint8_t* qa = a.qflat + a.qoff[pa];
     const uint8_t* qb = a.qflat + a.qoff[pb];
     const uint32_t tw = (g.ncols + 3) / 4;
-    for (uint32_t w = 0; w < tw; ++w) {
+    for (uint32_t w = part; w < tw; w += 4) {
         uint32_t v = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -87,7 +88,7 @@ __global__ __launch_bounds__(64) void k_pair_pack16(PairPackArgs a) {
     }
     const uint32_t qt = (uint32_t)a.qtile;
     const uint32_t qw = (g.nrows + qt - 1) / qt * qt / 2;
-    for (uint32_t w = 0; w < qw; ++w) {
+    for (uint32_t w = part; w < qw; w += 4) {
         uint32_t v = 0;
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -152,31 +153,6 @@ __global__ __launch_bounds__(64) void k_db_pack16(const Group* groups, const uin
             }
         words[g.word_off + (uint64_t)w * 64 + lane] = make_uint2(v[0], v[1]);
     }
-}
-
-// The largest byte of a buffer, folded into *out with an atomic max (grid-stride, 16 bytes per
-// load where the buffer allows).
-__global__ __launch_bounds__(256) void k_max_byte(const uint8_t* p, uint64_t n, uint32_t* out) {
-    uint32_t mx = 0;
-    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t head = (16 - ((uintptr_t)p & 15)) & 15;
-    const uint64_t h = head < n ? head : n;
-    for (uint64_t i = tid; i < h; i += stride) mx = max(mx, (uint32_t)p[i]);
-    const uint4* v = reinterpret_cast<const uint4*>(p + h);
-    const uint64_t nv = (n - h) / 16;
-    for (uint64_t i = tid; i < nv; i += stride) {
-        const uint4 w = v[i];
-        const uint32_t a[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) mx = max(mx, (a[k] >> (8 * b)) & 0xffu);
-    }
-    for (uint64_t i = h + nv * 16 + tid; i < n; i += stride) mx = max(mx, (uint32_t)p[i]);
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
-    if ((threadIdx.x & 63) == 0 && mx) atomicMax(out, mx);
 }
 
 __global__ void k_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter) {
@@ -276,13 +252,6 @@ hipError_t launch_db_pack16(const Group* groups, uint32_t ngroups, const uint32_
     return hipGetLastError();
 }
 
-hipError_t launch_max_byte(const uint8_t* p, uint64_t n, uint32_t* out, hipStream_t stream) {
-    if (n == 0) return hipSuccess;
-    const uint64_t want = (n / 16 + 255) / 256;
-    const unsigned blocks = (unsigned)std::min<uint64_t>(std::max<uint64_t>(want, 1), 4096);
-    hipLaunchKernelGGL(k_max_byte, dim3(blocks), dim3(256), 0, stream, p, n, out);
-    return hipGetLastError();
-}
 
 hipError_t launch_fill_i32(int32_t* p, uint64_t n, int32_t v, hipStream_t stream) {
     if (n == 0) return hipSuccess;
@@ -300,7 +269,7 @@ hipError_t launch_pair_pack(const PairPackArgs& a, uint32_t ngroups, hipStream_t
 hipError_t launch_pair_pack16(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream) {
     if (ngroups == 0) return hipSuccess;
     if (a.pad_symbol > kPairs16MaxAlphabet) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_pair_pack16, dim3(ngroups), dim3(64), 0, stream, a);
+    hipLaunchKernelGGL(k_pair_pack16, dim3(ngroups), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

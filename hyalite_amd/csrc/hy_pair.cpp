@@ -37,8 +37,7 @@ struct hy_pair_scratch {
     double device_ms = 0.0;
     uint64_t calls = 0;
     DevBuf mat, qflat, tflat, qoff, toff, qoff32, qlen32, list, nq, groups, twords, qwords, tlens,
-        qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best, flag, arena;
-    uint32_t* h_flag = nullptr;  // pinned: the largest symbol of a batch
+        qlens, pidx, spill, sys_spill, scores, qend, tend, colmax, best, arena;
     hy::PinnedRing ring;  // large pageable uploads
     uint8_t* stage = nullptr;  // pinned
     size_t stage_bytes = 0;
@@ -49,9 +48,8 @@ struct hy_pair_scratch {
         if (aux) (void)hipStreamSynchronize(aux);
         for (DevBuf* b : {&mat, &qflat, &tflat, &qoff, &toff, &qoff32, &qlen32, &list, &nq,
                           &groups, &twords, &qwords, &tlens, &qlens, &pidx, &spill, &sys_spill,
-                          &scores, &qend, &tend, &colmax, &best, &flag, &arena})
+                          &scores, &qend, &tend, &colmax, &best, &arena})
             hy::dev_free(b->p);
-        if (h_flag) (void)hipHostFree(h_flag);
         if (stage) (void)hipHostFree(stage);
         if (fork) (void)hipEventDestroy(fork);
         if (join) (void)hipEventDestroy(join);
@@ -108,18 +106,21 @@ int grow_stage(hy_pair_scratch* s, size_t bytes) {
 struct Stager {
     hy_pair_scratch* s;
     size_t used = 0;
-    struct Piece { size_t off, bytes; DevBuf* dst; const void* ext; };
+    // symbols: the piece holds sequence symbols; commit() folds their largest byte into
+    // max_symbol (on the host, while or before copying them).
+    struct Piece { size_t off, bytes; DevBuf* dst; const void* ext; bool symbols; };
     std::vector<Piece> pieces;
+    uint8_t max_symbol = 0;
     // Reserve `bytes` for `dst` (16-byte aligned) and return the host pointer to fill later.
-    size_t add(DevBuf* dst, size_t bytes) {
+    size_t add(DevBuf* dst, size_t bytes, bool symbols = false) {
         const size_t off = (used + 15) & ~size_t(15);
-        pieces.push_back({off, bytes, dst, nullptr});
+        pieces.push_back({off, bytes, dst, nullptr, symbols});
         used = off + bytes;
         return off;
     }
     // Upload `bytes` from caller memory straight to `dst`, without passing through the stage.
-    void add_external(DevBuf* dst, const void* src, size_t bytes) {
-        pieces.push_back({0, bytes, dst, src});
+    void add_external(DevBuf* dst, const void* src, size_t bytes, bool symbols = false) {
+        pieces.push_back({0, bytes, dst, src, symbols});
     }
     uint8_t* at(size_t off) { return s->stage + off; }
     int commit() {
@@ -129,12 +130,17 @@ struct Stager {
         }
         for (const Piece& p : pieces) {
             if (p.ext && p.bytes >= hy::kPinnedRingMin) {
-                const int rc =
-                    hy::upload_pageable(s->ring, s->device, p.dst->p, p.ext, p.bytes, s->stream);
+                uint8_t mx = 0;
+                const int rc = hy::upload_pageable(s->ring, s->device, p.dst->p, p.ext, p.bytes,
+                                                   s->stream, p.symbols ? &mx : nullptr);
                 if (rc != HY_OK) return rc;
+                max_symbol = std::max(max_symbol, mx);
                 continue;
             }
             const void* from = p.ext ? p.ext : static_cast<const void*>(s->stage + p.off);
+            if (p.symbols)
+                max_symbol = std::max(
+                    max_symbol, hy::host_max_byte(static_cast<const uint8_t*>(from), p.bytes));
             if (p.bytes && hipMemcpyAsync(p.dst->p, from, p.bytes, hipMemcpyHostToDevice,
                                           s->stream) != hipSuccess)
                 return HY_E_DEVICE;
@@ -397,8 +403,7 @@ extern "C" int hy_pair_scratch_new(hy_pair_scratch** out) {
         hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&s->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&s->join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess ||
-        hipHostMalloc((void**)&s->h_flag, sizeof(uint32_t)) != hipSuccess) {
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
         delete s;
         return HY_E_DEVICE;
     }
@@ -702,21 +707,32 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     // bits (queries up to kLaneMaxQuery, targets below 65,536) run two per lane (k_pairs16).
     // HYALITE_HIP_PAIRS16=0 keeps them one per lane.
     std::vector<uint32_t> lanes16;
-    if (mode == HY_MODE_SW && alphabet_len <= hy::kPairs16MaxAlphabet && pairs16_enabled()) {
-        std::vector<uint32_t> rest;
-        rest.reserve(lanes.size());
-        lanes16.reserve(lanes.size());
-        for (uint32_t k : lanes) {
+    if (mode == HY_MODE_SW && alphabet_len <= hy::kPairs16MaxAlphabet && pairs16_enabled() &&
+        !lanes.empty()) {
+        auto bits16 = [&](size_t m, size_t n) {
             int64_t b = 0;
-            const int bits = (ql[k] <= kLaneMaxQuery && tl[k] <= 65535)
-                                 ? hy::width_bits_for(hy::magnitude_bound(HY_MODE_SW, sc.min_e,
-                                                                          sc.max_e, gap_open,
-                                                                          gap_ext, ql[k], tl[k]),
-                                                      &b)
-                                 : 0;
-            (bits != 0 && bits <= 16 ? lanes16 : rest).push_back(k);
+            const int bits = hy::width_bits_for(
+                hy::magnitude_bound(HY_MODE_SW, sc.min_e, sc.max_e, gap_open, gap_ext, m, n), &b);
+            return bits != 0 && bits <= 16;
+        };
+        size_t lm = 0, ln = 0;
+        for (uint32_t k : lanes) {
+            lm = std::max(lm, ql[k]);
+            ln = std::max(ln, tl[k]);
         }
-        lanes.swap(rest);
+        // The bound grows with both lengths: one proof at the longest covers every pair.
+        if (lm <= kLaneMaxQuery && ln <= 65535 && bits16(lm, ln)) {
+            lanes16.swap(lanes);
+        } else {
+            std::vector<uint32_t> rest;
+            rest.reserve(lanes.size());
+            lanes16.reserve(lanes.size());
+            for (uint32_t k : lanes) {
+                const bool fits = ql[k] <= kLaneMaxQuery && tl[k] <= 65535 && bits16(ql[k], tl[k]);
+                (fits ? lanes16 : rest).push_back(k);
+            }
+            lanes.swap(rest);
+        }
     }
     // Lanes: sorted by (query length, target length) descending, cut into groups. The sort key
     // packs both lengths (complemented, 13 + 17 bits: lane pairs are at most 4096 x 65536 unless
@@ -806,10 +822,10 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     // Flat sequences go to the device straight from the caller's buffers; per-pair pointers are
     // concatenated in the stage first.
     size_t o_q = 0, o_t = 0;
-    if (src.qp) o_q = st.add(&s->qflat, qtot);
-    else st.add_external(&s->qflat, src.qflat + src.qoff[0], qtot);
-    if (src.tp) o_t = st.add(&s->tflat, ttot);
-    else st.add_external(&s->tflat, src.tflat + src.toff[0], ttot);
+    if (src.qp) o_q = st.add(&s->qflat, qtot, true);
+    else st.add_external(&s->qflat, src.qflat + src.qoff[0], qtot, true);
+    if (src.tp) o_t = st.add(&s->tflat, ttot, true);
+    else st.add_external(&s->tflat, src.tflat + src.toff[0], ttot, true);
     const size_t o_qoff = st.add(&s->qoff, (npairs + 1) * sizeof(uint64_t));
     const size_t o_toff = st.add(&s->toff, (npairs + 1) * sizeof(uint64_t));
     size_t o_q32 = 0, o_l32 = 0, o_list = 0, o_nq = 0, o_groups = 0, o_tl = 0, o_ql = 0, o_pi = 0;
@@ -883,20 +899,12 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         }
     }
     if ((rc = st.commit()) != HY_OK) return rc;
-    // Symbols: the largest byte of both sides, found on the device right after the upload and
-    // checked before any kernel indexes the matrix with them; the exact per-pair order of the
-    // reference's errors is recovered on the host only when one is out of range.
-    if ((rc = grow(s->flag, sizeof(uint32_t))) != HY_OK) return rc;
-    if (hy::launch_fill_i32(static_cast<int32_t*>(s->flag.p), 1, 0, s->stream) != hipSuccess ||
-        hy::launch_max_byte(static_cast<const uint8_t*>(s->qflat.p), qtot,
-                            static_cast<uint32_t*>(s->flag.p), s->stream) != hipSuccess ||
-        hy::launch_max_byte(static_cast<const uint8_t*>(s->tflat.p), ttot,
-                            static_cast<uint32_t*>(s->flag.p), s->stream) != hipSuccess ||
-        hipMemcpyAsync(s->h_flag, s->flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                       s->stream) != hipSuccess ||
-        hipStreamSynchronize(s->stream) != hipSuccess)
-        return HY_E_DEVICE;
-    if (*s->h_flag >= alphabet_len) {
+    // Symbols: the largest byte of both sides, found on the host while the sequences were copied
+    // for upload and checked before any kernel indexes the matrix with them (their copies are
+    // queued, nothing waits for them here); the exact per-pair order of the reference's errors
+    // is recovered only when one is out of range.
+    if (st.max_symbol >= alphabet_len) {
+        if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
         rc = exact_checks();
         if (rc != kChecksPassed) return rc;
         return HY_E_INTERNAL;

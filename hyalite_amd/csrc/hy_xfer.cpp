@@ -6,6 +6,7 @@
 // take every kThreads-th piece of the buffer: on the way up a thread copies a piece into a free
 // slot and queues its DMA; on the way down it queues the DMA of its next piece before copying
 // the current one out, so host copies run in parallel and overlap the transfers.
+#include <emmintrin.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +18,38 @@
 #include "hy_internal.h"
 
 namespace hy {
+
+uint8_t host_max_byte(const uint8_t* p, size_t n) {
+    __m128i m = _mm_setzero_si128();
+    size_t i = 0;
+    for (; i + 16 <= n; i += 16)
+        m = _mm_max_epu8(m, _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + i)));
+    alignas(16) uint8_t lanes[16];
+    _mm_store_si128(reinterpret_cast<__m128i*>(lanes), m);
+    uint8_t mx = 0;
+    for (uint8_t v : lanes) mx = std::max(mx, v);
+    for (; i < n; ++i) mx = std::max(mx, p[i]);
+    return mx;
+}
+
+uint8_t copy_max_byte(uint8_t* dst, const uint8_t* src, size_t n) {
+    __m128i m = _mm_setzero_si128();
+    size_t i = 0;
+    for (; i + 16 <= n; i += 16) {
+        const __m128i v = _mm_loadu_si128(reinterpret_cast<const __m128i*>(src + i));
+        _mm_storeu_si128(reinterpret_cast<__m128i*>(dst + i), v);
+        m = _mm_max_epu8(m, v);
+    }
+    alignas(16) uint8_t lanes[16];
+    _mm_store_si128(reinterpret_cast<__m128i*>(lanes), m);
+    uint8_t mx = 0;
+    for (uint8_t v : lanes) mx = std::max(mx, v);
+    for (; i < n; ++i) {
+        dst[i] = src[i];
+        mx = std::max(mx, src[i]);
+    }
+    return mx;
+}
 
 PinnedRing::~PinnedRing() {
     if (ring) (void)hipHostFree(ring);
@@ -61,14 +94,16 @@ int run_workers(int device, size_t pieces, F work) {
 }  // namespace
 
 int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
-                    hipStream_t stream) {
+                    hipStream_t stream, uint8_t* max_byte) {
+    if (max_byte) *max_byte = 0;
     if (bytes == 0) return HY_OK;
     int rc = ring_ready(r);
     if (rc != HY_OK) return rc;
     constexpr int T = PinnedRing::kThreads;
     constexpr size_t SLOT = PinnedRing::kSlot;
     const size_t pieces = (bytes + SLOT - 1) / SLOT;
-    return run_workers(device, pieces, [&](int t) -> int {
+    std::vector<uint8_t> wmax(T, 0);
+    rc = run_workers(device, pieces, [&](int t) -> int {
         size_t used = 0;
         for (size_t p = (size_t)t; p < pieces; p += T, ++used) {
             const int slot = 2 * t + (int)(used & 1);
@@ -80,7 +115,9 @@ int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_
                 if (hipEventSynchronize(r.ev[slot]) != hipSuccess) return HY_E_DEVICE;
                 r.busy[slot] = false;
             }
-            std::memcpy(h, static_cast<const uint8_t*>(src) + off, len);
+            const uint8_t* from = static_cast<const uint8_t*>(src) + off;
+            if (max_byte) wmax[t] = std::max(wmax[t], copy_max_byte(h, from, len));
+            else std::memcpy(h, from, len);
             if (hipMemcpyAsync(static_cast<uint8_t*>(dst) + off, h, len, hipMemcpyHostToDevice,
                                stream) != hipSuccess ||
                 hipEventRecord(r.ev[slot], stream) != hipSuccess)
@@ -89,6 +126,9 @@ int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_
         }
         return HY_OK;
     });
+    if (rc == HY_OK && max_byte)
+        for (uint8_t v : wmax) *max_byte = std::max(*max_byte, v);
+    return rc;
 }
 
 int download_pageable(PinnedRing& r, int device, const Segment* segs, int nseg,

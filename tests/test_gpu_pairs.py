@@ -506,6 +506,20 @@ def test_large_batch_uploads_through_pinned_slots():
         assert (whole[0][k], whole[1][k], whole[2][k]) == (
             want.score, -1 if want.query_end is None else want.query_end,
             -1 if want.target_end is None else want.target_end), k
+    # Symbols are checked on the host while the buffers are copied up: a bad one late in either
+    # buffer is still reported, with the reference's error.
+    bad_t = t.copy()
+    bad_t[int(to[n - 2]) + 3] = 7
+    with pytest.raises(hy.SymbolOutOfRange) as e:
+        hy.align_pairs_flat(q, qo, bad_t, to, s, hy.Mode.Sw, st, scratch=sc)
+    assert e.value.symbol == 7
+    bad_q = q.copy()
+    bad_q[int(qo[n // 2]) + 1] = 200
+    with pytest.raises(hy.SymbolOutOfRange) as e:
+        hy.align_pairs_flat(bad_q, qo, t, to, s, hy.Mode.Sw, st, scratch=sc)
+    assert e.value.symbol == 200
+    assert hy.align_pairs_flat(q, qo, t, to, s, hy.Mode.Sw, st, scratch=sc)[0].tolist() == \
+        whole[0].tolist()
 
 
 @pytest.mark.parametrize("al", [2, 4, 7])
