@@ -1,6 +1,9 @@
 // Internal types shared by the host library (hy_host.cpp) and the kernels (hy_kernels.hip).
 #pragma once
 #include <hip/hip_runtime.h>
+
+#include <thread>
+#include <vector>
 #include <stdint.h>
 
 namespace hy {
@@ -317,6 +320,39 @@ constexpr size_t kPinnedRingMin = 8ull << 20;
 // max_byte (optional) receives the largest byte of src, found while copying it.
 int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
                     hipStream_t stream, uint8_t* max_byte = nullptr);
+// An upload of several pageable buffers through the ring that runs on the worker threads while
+// the caller goes on: start() returns at once; finish() joins the workers, after which every
+// DMA is queued on `stream` and max_byte holds the largest byte of the segments marked as
+// symbols. Work that needs the data is queued after finish(). The source buffers must stay
+// valid until finish() or the destructor (which joins) returns; the ring is not used by
+// anything else meanwhile.
+struct UpSegment {
+    void* dst;
+    const void* src;
+    size_t bytes;
+    bool symbols;
+};
+class UploadJob {
+public:
+    UploadJob() = default;
+    UploadJob(const UploadJob&) = delete;
+    UploadJob& operator=(const UploadJob&) = delete;
+    ~UploadJob();
+    int start(PinnedRing& r, int device, const UpSegment* segs, int nseg, hipStream_t stream);
+    int finish(uint8_t* max_byte);
+
+private:
+    struct Piece {
+        uint8_t* dst;
+        const uint8_t* src;
+        size_t len;
+        bool symbols;
+    };
+    std::vector<Piece> pieces_;
+    std::vector<std::thread> pool_;
+    std::vector<int> status_;
+    std::vector<uint8_t> wmax_;
+};
 // The largest byte of p[0, n) (host memory), and a copy that finds it on the way.
 uint8_t host_max_byte(const uint8_t* p, size_t n);
 uint8_t copy_max_byte(uint8_t* dst, const uint8_t* src, size_t n);

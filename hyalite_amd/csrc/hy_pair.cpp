@@ -691,6 +691,18 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
                                 alphabet_len, gap_open, gap_ext, mode, ends, scores, query_ends,
                                 target_ends);
     }
+    // Large flat sequence buffers start going up now, on the pinned-ring workers (which also
+    // find their largest symbol), while the lanes are split, sorted and grouped below; the
+    // kernels are queued only after the workers have joined.
+    hy::UploadJob job;
+    const bool early = !src.qp && !src.tp && qtot + ttot >= hy::kPinnedRingMin;
+    if (early) {
+        if ((rc = grow(s->qflat, qtot)) != HY_OK || (rc = grow(s->tflat, ttot)) != HY_OK)
+            return rc;
+        const hy::UpSegment segs[2] = {{s->qflat.p, src.qflat + src.qoff[0], qtot, true},
+                                       {s->tflat.p, src.tflat + src.toff[0], ttot, true}};
+        if ((rc = job.start(s->ring, s->device, segs, 2, s->stream)) != HY_OK) return rc;
+    }
     std::vector<uint32_t> lanes, waves;
     size_t nshort = 0;
     for (size_t k = 0; k < npairs; ++k)
@@ -822,10 +834,12 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     // Flat sequences go to the device straight from the caller's buffers; per-pair pointers are
     // concatenated in the stage first.
     size_t o_q = 0, o_t = 0;
-    if (src.qp) o_q = st.add(&s->qflat, qtot, true);
-    else st.add_external(&s->qflat, src.qflat + src.qoff[0], qtot, true);
-    if (src.tp) o_t = st.add(&s->tflat, ttot, true);
-    else st.add_external(&s->tflat, src.tflat + src.toff[0], ttot, true);
+    if (!early) {
+        if (src.qp) o_q = st.add(&s->qflat, qtot, true);
+        else st.add_external(&s->qflat, src.qflat + src.qoff[0], qtot, true);
+        if (src.tp) o_t = st.add(&s->tflat, ttot, true);
+        else st.add_external(&s->tflat, src.tflat + src.toff[0], ttot, true);
+    }
     const size_t o_qoff = st.add(&s->qoff, (npairs + 1) * sizeof(uint64_t));
     const size_t o_toff = st.add(&s->toff, (npairs + 1) * sizeof(uint64_t));
     size_t o_q32 = 0, o_l32 = 0, o_list = 0, o_nq = 0, o_groups = 0, o_tl = 0, o_ql = 0, o_pi = 0;
@@ -899,6 +913,11 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
         }
     }
     if ((rc = st.commit()) != HY_OK) return rc;
+    if (early) {
+        uint8_t mx = 0;
+        if ((rc = job.finish(&mx)) != HY_OK) return rc;
+        st.max_symbol = std::max(st.max_symbol, mx);
+    }
     // Symbols: the largest byte of both sides, found on the host while the sequences were copied
     // for upload and checked before any kernel indexes the matrix with them (their copies are
     // queued, nothing waits for them here); the exact per-pair order of the reference's errors

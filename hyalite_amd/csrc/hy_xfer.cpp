@@ -93,42 +93,80 @@ int run_workers(int device, size_t pieces, F work) {
 
 }  // namespace
 
+UploadJob::~UploadJob() {
+    for (std::thread& th : pool_)
+        if (th.joinable()) th.join();
+}
+
+int UploadJob::start(PinnedRing& r, int device, const UpSegment* segs, int nseg,
+                     hipStream_t stream) {
+    pieces_.clear();
+    for (int k = 0; k < nseg; ++k)
+        for (size_t off = 0; off < segs[k].bytes; off += PinnedRing::kSlot)
+            pieces_.push_back({static_cast<uint8_t*>(segs[k].dst) + off,
+                               static_cast<const uint8_t*>(segs[k].src) + off,
+                               std::min(PinnedRing::kSlot, segs[k].bytes - off),
+                               segs[k].symbols});
+    if (pieces_.empty()) return HY_OK;
+    const int rc = ring_ready(r);
+    if (rc != HY_OK) return rc;
+    const int workers = (int)std::min<size_t>(PinnedRing::kThreads, pieces_.size());
+    status_.assign(workers, HY_OK);
+    wmax_.assign(workers, 0);
+    pool_.reserve(workers);
+    for (int t = 0; t < workers; ++t)
+        pool_.emplace_back([this, &r, device, stream, t]() {
+            // HIP's current device is per thread.
+            if (hipSetDevice(device) != hipSuccess) {
+                status_[t] = HY_E_DEVICE;
+                return;
+            }
+            size_t used = 0;
+            for (size_t p = (size_t)t; p < pieces_.size(); p += PinnedRing::kThreads, ++used) {
+                const Piece& pc = pieces_[p];
+                const int slot = 2 * t + (int)(used & 1);
+                uint8_t* h = r.ring + (size_t)slot * PinnedRing::kSlot;
+                // The slot's previous piece, from this job or an earlier one, must have left
+                // before the slot is refilled.
+                if (r.busy[slot]) {
+                    if (hipEventSynchronize(r.ev[slot]) != hipSuccess) {
+                        status_[t] = HY_E_DEVICE;
+                        return;
+                    }
+                    r.busy[slot] = false;
+                }
+                if (pc.symbols) wmax_[t] = std::max(wmax_[t], copy_max_byte(h, pc.src, pc.len));
+                else std::memcpy(h, pc.src, pc.len);
+                if (hipMemcpyAsync(pc.dst, h, pc.len, hipMemcpyHostToDevice, stream) !=
+                        hipSuccess ||
+                    hipEventRecord(r.ev[slot], stream) != hipSuccess) {
+                    status_[t] = HY_E_DEVICE;
+                    return;
+                }
+                r.busy[slot] = true;
+            }
+        });
+    return HY_OK;
+}
+
+int UploadJob::finish(uint8_t* max_byte) {
+    for (std::thread& th : pool_) th.join();
+    pool_.clear();
+    if (max_byte) *max_byte = 0;
+    for (int st : status_)
+        if (st != HY_OK) return st;
+    if (max_byte)
+        for (uint8_t v : wmax_) *max_byte = std::max(*max_byte, v);
+    return HY_OK;
+}
+
 int upload_pageable(PinnedRing& r, int device, void* dst, const void* src, size_t bytes,
                     hipStream_t stream, uint8_t* max_byte) {
-    if (max_byte) *max_byte = 0;
-    if (bytes == 0) return HY_OK;
-    int rc = ring_ready(r);
-    if (rc != HY_OK) return rc;
-    constexpr int T = PinnedRing::kThreads;
-    constexpr size_t SLOT = PinnedRing::kSlot;
-    const size_t pieces = (bytes + SLOT - 1) / SLOT;
-    std::vector<uint8_t> wmax(T, 0);
-    rc = run_workers(device, pieces, [&](int t) -> int {
-        size_t used = 0;
-        for (size_t p = (size_t)t; p < pieces; p += T, ++used) {
-            const int slot = 2 * t + (int)(used & 1);
-            const size_t off = p * SLOT, len = std::min(SLOT, bytes - off);
-            uint8_t* h = r.ring + (size_t)slot * SLOT;
-            // The slot's previous piece, from this call or an earlier one, must have left
-            // before the slot is refilled.
-            if (r.busy[slot]) {
-                if (hipEventSynchronize(r.ev[slot]) != hipSuccess) return HY_E_DEVICE;
-                r.busy[slot] = false;
-            }
-            const uint8_t* from = static_cast<const uint8_t*>(src) + off;
-            if (max_byte) wmax[t] = std::max(wmax[t], copy_max_byte(h, from, len));
-            else std::memcpy(h, from, len);
-            if (hipMemcpyAsync(static_cast<uint8_t*>(dst) + off, h, len, hipMemcpyHostToDevice,
-                               stream) != hipSuccess ||
-                hipEventRecord(r.ev[slot], stream) != hipSuccess)
-                return HY_E_DEVICE;
-            r.busy[slot] = true;
-        }
-        return HY_OK;
-    });
-    if (rc == HY_OK && max_byte)
-        for (uint8_t v : wmax) *max_byte = std::max(*max_byte, v);
-    return rc;
+    UploadJob job;
+    const UpSegment seg{dst, src, bytes, max_byte != nullptr};
+    const int rc = job.start(r, device, &seg, 1, stream);
+    const int fin = job.finish(max_byte);
+    return rc != HY_OK ? rc : fin;
 }
 
 int download_pageable(PinnedRing& r, int device, const Segment* segs, int nseg,
