@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <memory>
 #include <atomic>
 #include <cctype>
 #include <cmath>
@@ -227,13 +228,17 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     if (n == 0) return HY_E_EMPTY_DATABASE;
     if (!seqs || !lens) return HY_E_NULL_ARGUMENT;
     // SymbolOutOfRange: the first bad symbol in database order (src/database.rs:756-766).
+    // Each sequence's largest byte first (16 bytes per step); the exact first bad symbol is
+    // looked for only in a sequence that has one.
     uint64_t mtl = 0, total = 0;
     for (size_t k = 0; k < n; ++k) {
         if (lens[k] && !seqs[k]) return HY_E_NULL_ARGUMENT;
-        for (size_t x = 0; x < lens[k]; ++x) {
-            if ((uint32_t)seqs[k][x] >= alphabet_len) {
-                if (err_detail) *err_detail = seqs[k][x];
-                return HY_E_SYMBOL_OUT_OF_RANGE;
+        if (lens[k] && (uint32_t)hy::host_max_byte(seqs[k], lens[k]) >= alphabet_len) {
+            for (size_t x = 0; x < lens[k]; ++x) {
+                if ((uint32_t)seqs[k][x] >= alphabet_len) {
+                    if (err_detail) *err_detail = seqs[k][x];
+                    return HY_E_SYMBOL_OUT_OF_RANGE;
+                }
             }
         }
         mtl = std::max<uint64_t>(mtl, lens[k]);
@@ -314,13 +319,31 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
         db->al_eff <= kNoPadSymbol &&
         (db->waves16 == 1 ||
          hy::inter16_lds_bytes((int)db->al_eff, db->q16) <= hy::kPairProfileMaxBytes);
+    // The bound grows with the target length, so each width (and the packed kernel's gap-open
+    // headroom) holds up to a length threshold, found once by bisection.
+    auto bound_at = [&](uint64_t len) -> int64_t {
+        int64_t b = 0;
+        (void)width_bits_for(
+            magnitude_bound(mode, min_e, max_e, gap_open, gap_ext, max_query_len, len), &b);
+        return b;
+    };
+    auto longest_within = [&](int64_t limit) -> int64_t {  // -1: not even length 0
+        if (bound_at(0) > limit) return -1;
+        uint64_t lo = 0, hi = mtl;
+        if (bound_at(hi) <= limit) return (int64_t)hi;
+        while (hi - lo > 1) {
+            const uint64_t mid = lo + (hi - lo) / 2;
+            (bound_at(mid) <= limit ? lo : hi) = mid;
+        }
+        return (int64_t)lo;
+    };
+    const int64_t len8 = longest_within(127), len16 = longest_within(32767);
+    const int64_t len_packed = longest_within(32767 - (int64_t)gap_open);
     std::vector<uint8_t> elig16(n, 0);
     for (size_t k = 0; k < n; ++k) {
-        int64_t b = 0;
-        const int w = width_bits_for(
-            magnitude_bound(mode, min_e, max_e, gap_open, gap_ext, max_query_len, lens[k]), &b);
-        db->count_by_width[w == 8 ? 0 : (w == 16 ? 1 : 2)]++;
-        elig16[k] = pair_profile_fits && w <= 16 && b + (int64_t)gap_open <= 32767;
+        const int64_t len = (int64_t)lens[k];
+        db->count_by_width[len <= len8 ? 0 : (len <= len16 ? 1 : 2)]++;
+        elig16[k] = pair_profile_fits && len <= len_packed;
     }
 
     int rc = HY_OK;
@@ -336,20 +359,46 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     {
         std::vector<uint64_t> off(n + 1, 0);
         for (size_t k = 0; k < n; ++k) off[k + 1] = off[k] + lens[k];
-        std::vector<uint8_t> flat(std::max<uint64_t>(total, 1), 0);
+        // Every byte is written below, so the buffer is left uninitialised.
+        std::unique_ptr<uint8_t[]> flat(new (std::nothrow) uint8_t[std::max<uint64_t>(total, 1)]);
+        if (!flat) return fail(HY_E_OUT_OF_MEMORY);
         for (size_t k = 0; k < n; ++k)
-            if (lens[k]) std::memcpy(flat.data() + off[k], seqs[k], lens[k]);
+            if (lens[k]) std::memcpy(flat.get() + off[k], seqs[k], lens[k]);
         if (dev_upload(&db->d_off, off) != hipSuccess) return fail(HY_E_DEVICE);
-        if (dev_upload(&db->d_flat, flat) != hipSuccess) return fail(HY_E_DEVICE);
+        if (hipMalloc(reinterpret_cast<void**>(&db->d_flat), std::max<uint64_t>(total, 1)) !=
+                hipSuccess ||
+            hipMemcpy(db->d_flat, flat.get(), std::max<uint64_t>(total, 1),
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return fail(HY_E_DEVICE);
         db->h_off = std::move(off);
     }
 
     // Split: long targets go to the systolic kernel, the rest to the inter-sequence kernel. A
     // database whose alphabet uses every byte value has no padding symbol and runs systolic.
+    // Longest first, ties in database order: a stable LSD radix sort on the complemented
+    // length (three 11-bit digits cover lengths below 2^33).
     std::vector<uint32_t> order_idx(n);
-    std::iota(order_idx.begin(), order_idx.end(), 0u);
-    std::stable_sort(order_idx.begin(), order_idx.end(),
-                     [&](uint32_t a, uint32_t b) { return lens[a] > lens[b]; });
+    {
+        std::vector<uint32_t> tmp(n);
+        std::iota(order_idx.begin(), order_idx.end(), 0u);
+        std::vector<uint32_t> cnt(1u << 11);
+        const uint64_t top = mtl;
+        for (int pass = 0; pass < 3; ++pass) {
+            const int shift = 11 * pass;
+            if ((top >> shift) == 0 && pass > 0) break;
+            auto digit = [&](uint32_t k) { return (uint32_t)(((top - lens[k]) >> shift) & 0x7FF); };
+            std::fill(cnt.begin(), cnt.end(), 0u);
+            for (uint32_t k : order_idx) ++cnt[digit(k)];
+            uint32_t sum = 0;
+            for (uint32_t& c : cnt) {
+                const uint32_t v = c;
+                c = sum;
+                sum += v;
+            }
+            for (uint32_t k : order_idx) tmp[cnt[digit(k)]++] = k;
+            order_idx.swap(tmp);
+        }
+    }
     // A one-query scan lasts as long as its slowest wavefront: a packed group's time grows with
     // its longest target, so long targets go to the systolic kernel on a second stream instead.
     // Long = beyond 1.6x the average column share of a resident wavefront (residues / 128 per

@@ -16,6 +16,8 @@
 
 #include <cstdio>
 #include <cstring>
+#include <thread>
+#include <memory>
 #include <vector>
 
 #include "../../include/hyalite_hip.h"
@@ -125,32 +127,50 @@ extern "C" int hy_db_load(const char* path, uint32_t backend, hy_db** out, int64
     if (h.total > (uint64_t)size || (uint64_t)size != expect) return HY_E_BAD_IMAGE;
     std::vector<int32_t> mat(mat_n);
     std::vector<uint64_t> off(h.n + 1);
-    std::vector<uint8_t> res(std::max<uint64_t>(h.total, 1));
+    // Every residue byte is read from the file, so the buffer is left uninitialised.
+    std::unique_ptr<uint8_t[]> res(new (std::nothrow) uint8_t[std::max<uint64_t>(h.total, 1)]);
+    if (!res) return HY_E_OUT_OF_MEMORY;
     uint64_t digest = 0;
     if (std::fread(mat.data(), 4, mat_n, in.f) != mat_n ||
         std::fread(off.data(), 8, h.n + 1, in.f) != h.n + 1 ||
-        (h.total && std::fread(res.data(), 1, h.total, in.f) != h.total) ||
+        (h.total && std::fread(res.get(), 1, h.total, in.f) != h.total) ||
         std::fread(&digest, 8, 1, in.f) != 1)
         return HY_E_BAD_IMAGE;
-    Fnv sum;
-    sum.add(&h, sizeof(h));
-    sum.add(mat.data(), mat_n * 4);
-    sum.add(off.data(), (h.n + 1) * 8);
-    sum.add(res.data(), h.total);
-    if (sum.digest() != digest) return HY_E_BAD_IMAGE;
-    // Offsets: from 0, non-decreasing, ending at the residue count.
+    // Offsets: from 0, non-decreasing, ending at the residue count (the build below indexes the
+    // residues with them).
     if (off[0] != 0 || off[h.n] != h.total) return HY_E_BAD_IMAGE;
     std::vector<const uint8_t*> ptrs(h.n);
     std::vector<size_t> lens(h.n);
     for (uint64_t k = 0; k < h.n; ++k) {
         if (off[k + 1] < off[k]) return HY_E_BAD_IMAGE;
-        ptrs[k] = res.data() + off[k];
+        ptrs[k] = res.get() + off[k];
         lens[k] = (size_t)(off[k + 1] - off[k]);
     }
+    // The checksum (one serial multiply chain over the file) runs beside the build, which only
+    // reads the same buffers; a mismatch discards the build's result, whatever it was.
+    bool sum_ok = false;
+    std::thread check([&]() {
+        Fnv sum;
+        sum.add(&h, sizeof(h));
+        sum.add(mat.data(), mat_n * 4);
+        sum.add(off.data(), (h.n + 1) * 8);
+        sum.add(res.get(), h.total);
+        sum_ok = sum.digest() == digest;
+    });
     // The build validates the scheme, every symbol, the width proof and the traceback budget
     // again, exactly as for sequences given in memory.
-    return hy_db_build(ptrs.data(), lens.data(), h.n, HY_HAVE_ALL, mat.data(), mat_n, h.al, h.go,
-                       h.ge, h.mode, h.search_type, h.budget, h.mql, backend, out, err_detail);
+    int64_t detail = 0;
+    const int rc = hy_db_build(ptrs.data(), lens.data(), h.n, HY_HAVE_ALL, mat.data(), mat_n,
+                               h.al, h.go, h.ge, h.mode, h.search_type, h.budget, h.mql, backend,
+                               out, &detail);
+    check.join();
+    if (!sum_ok) {
+        if (*out) hy_db_free(*out);
+        *out = nullptr;
+        return HY_E_BAD_IMAGE;
+    }
+    if (err_detail) *err_detail = detail;
+    return rc;
 }
 
 extern "C" int hy_db_scoring(const hy_db* db, int32_t* matrix, uint32_t* alphabet_len,
