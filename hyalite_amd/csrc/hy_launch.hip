@@ -19,6 +19,15 @@ __global__ void k_fill_i32(int32_t* p, uint64_t n, int32_t v) {
     if (i < n) p[i] = v;
 }
 
+// Four consecutive bytes p[c .. c + 3] as one little-endian word from two aligned dword loads.
+// The caller guarantees c + 4 <= the sequence length; the second load may read up to 7 bytes
+// past p[c], which the pair buffers' 8 bytes of slack keep inside the allocation.
+__device__ __forceinline__ uint32_t load4_unaligned(const uint8_t* p, uint64_t c) {
+    const uintptr_t addr = reinterpret_cast<uintptr_t>(p + c);
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(addr & ~(uintptr_t)3);
+    return __builtin_amdgcn_alignbyte(w[1], w[0], (uint32_t)(addr & 3));
+}
+
 // One workgroup per pair group: lane l gathers four symbols of its pair per word (the padding
 // symbol past the pair's length, and for lanes past the group's count) and the group's words
 // are written lane-interleaved, so the pair kernel's per-column load is one coalesced line.
@@ -35,10 +44,14 @@ __global__ __launch_bounds__(64) void k_pair_pack(PairPackArgs a) {
     const uint32_t tw = (g.ncols + 3) / 4;
     for (uint32_t w = 0; w < tw; ++w) {
         uint32_t v = 0;
+        if ((uint64_t)w * 4 + 4 <= tn) {
+            v = load4_unaligned(t, (uint64_t)w * 4);
+        } else {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint64_t c = (uint64_t)w * 4 + k;
-            v |= (c < tn ? (uint32_t)t[c] : pad) << (8 * k);
+            for (int k = 0; k < 4; ++k) {
+                const uint64_t c = (uint64_t)w * 4 + k;
+                v |= (c < tn ? (uint32_t)t[c] : pad) << (8 * k);
+            }
         }
         a.twords[g.tword_off + (uint64_t)w * 64 + lane] = v;
     }
@@ -76,12 +89,26 @@ __global__ __launch_bounds__(256) void k_pair_pack16(PairPackArgs a) {
     const uint8_t* qb = a.qflat + a.qoff[pb];
     const uint32_t tw = (g.ncols + 3) / 4;
     for (uint32_t w = part; w < tw; w += 4) {
+        const uint64_t c0 = (uint64_t)w * 4;
+        // Four symbols of each target: whole words where both lie inside their sequences.
+        uint32_t xa = 0, xb = 0;
+        if (c0 + 4 <= tna) {
+            xa = load4_unaligned(ta, c0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) xa |= (c0 + k < tna ? (uint32_t)ta[c0 + k] : pad) << (8 * k);
+        }
+        if (c0 + 4 <= tnb) {
+            xb = load4_unaligned(tb, c0);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) xb |= (c0 + k < tnb ? (uint32_t)tb[c0 + k] : pad) << (8 * k);
+        }
         uint32_t v = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const uint64_t c = (uint64_t)w * 4 + k;
-            const uint32_t x = c < tna ? min((uint32_t)ta[c], pad) : pad;
-            const uint32_t y = c < tnb ? min((uint32_t)tb[c], pad) : pad;
+            const uint32_t x = min((xa >> (8 * k)) & 0xffu, pad);
+            const uint32_t y = min((xb >> (8 * k)) & 0xffu, pad);
             v |= (x * S + y) << (8 * k);
         }
         a.twords[g.tword_off + (uint64_t)w * 64 + lane] = v;

@@ -73,6 +73,10 @@ constexpr size_t kSysSpillBudget = 512ull << 20;
 // Returned by the exact per-pair checks when every pair passes.
 constexpr int kChecksPassed = -1;
 
+// Every device buffer is allocated this many bytes past its capacity: the pack kernels read
+// sequences a word at a time and may touch up to 7 bytes past a sequence's last one.
+constexpr size_t kDevSlack = 16;
+
 // Grow a device buffer (contents dropped; every entry point synchronises before returning).
 int grow(DevBuf& b, size_t bytes) {
     bytes = std::max<size_t>(bytes, 16);
@@ -80,7 +84,7 @@ int grow(DevBuf& b, size_t bytes) {
     hy::dev_free(b.p);
     b.p = nullptr;
     b.bytes = 0;
-    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    if (hipMalloc(&b.p, bytes + kDevSlack) != hipSuccess) {
         b.p = nullptr;
         return HY_E_OUT_OF_MEMORY;
     }
