@@ -149,21 +149,28 @@ extern "C" int hy_db_load(const char* path, uint32_t backend, hy_db** out, int64
     // The checksum (one serial multiply chain over the file) runs beside the build, which only
     // reads the same buffers; a mismatch discards the build's result, whatever it was.
     bool sum_ok = false;
-    std::thread check([&]() {
+    auto checksum = [&]() {
         Fnv sum;
         sum.add(&h, sizeof(h));
         sum.add(mat.data(), mat_n * 4);
         sum.add(off.data(), (h.n + 1) * 8);
         sum.add(res.get(), h.total);
         sum_ok = sum.digest() == digest;
-    });
+    };
+    std::thread check;
+    try {
+        check = std::thread(checksum);
+    } catch (...) {  // no second thread: check it here, before the build
+        checksum();
+        if (!sum_ok) return HY_E_BAD_IMAGE;
+    }
     // The build validates the scheme, every symbol, the width proof and the traceback budget
     // again, exactly as for sequences given in memory.
     int64_t detail = 0;
     const int rc = hy_db_build(ptrs.data(), lens.data(), h.n, HY_HAVE_ALL, mat.data(), mat_n,
                                h.al, h.go, h.ge, h.mode, h.search_type, h.budget, h.mql, backend,
                                out, &detail);
-    check.join();
+    if (check.joinable()) check.join();
     if (!sum_ok) {
         if (*out) hy_db_free(*out);
         *out = nullptr;

@@ -80,12 +80,18 @@ int run_workers(int device, size_t pieces, F work) {
     std::vector<int> status(workers, HY_OK);
     std::vector<std::thread> pool;
     pool.reserve(workers);
-    for (int t = 0; t < workers; ++t)
-        pool.emplace_back([&, t]() {
-            // HIP's current device is per thread.
-            status[t] = hipSetDevice(device) == hipSuccess ? work(t) : HY_E_DEVICE;
-        });
+    int spawn = HY_OK;
+    try {
+        for (int t = 0; t < workers; ++t)
+            pool.emplace_back([&, t]() {
+                // HIP's current device is per thread.
+                status[t] = hipSetDevice(device) == hipSuccess ? work(t) : HY_E_DEVICE;
+            });
+    } catch (...) {  // no thread to run on: no exception crosses the C ABI
+        spawn = HY_E_OUT_OF_MEMORY;
+    }
     for (std::thread& th : pool) th.join();
+    if (spawn != HY_OK) return spawn;
     for (int st : status)
         if (st != HY_OK) return st;
     return HY_OK;
@@ -114,6 +120,7 @@ int UploadJob::start(PinnedRing& r, int device, const UpSegment* segs, int nseg,
     status_.assign(workers, HY_OK);
     wmax_.assign(workers, 0);
     pool_.reserve(workers);
+    try {
     for (int t = 0; t < workers; ++t)
         pool_.emplace_back([this, &r, device, stream, t]() {
             // HIP's current device is per thread.
@@ -146,6 +153,11 @@ int UploadJob::start(PinnedRing& r, int device, const UpSegment* segs, int nseg,
                 r.busy[slot] = true;
             }
         });
+    } catch (...) {  // no thread to run on: the started ones finish, nothing crosses the C ABI
+        for (std::thread& th : pool_) th.join();
+        pool_.clear();
+        return HY_E_OUT_OF_MEMORY;
+    }
     return HY_OK;
 }
 
