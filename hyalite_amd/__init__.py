@@ -49,8 +49,7 @@ _LIB_NAME = "libhyalite_hip.so"
 # and the mode-templated kernels compiled once per alignment mode (SW, NW, HW, OV, SHW).
 _UNITS = [("csrc/hy_host.cpp", []), ("csrc/hy_tune.cpp", []), ("csrc/hy_serve.cpp", []),
           ("csrc/hy_align.cpp", []), ("csrc/hy_pair.cpp", []), ("csrc/hy_image.cpp", []),
-    ("csrc/hy_xfer.cpp", []),
-          ("csrc/hy_launch.hip", [])] + [
+          ("csrc/hy_xfer.cpp", []), ("csrc/hy_launch.hip", [])] + [
     (src, [f"-DHY_KMODE={k}"])
     for src in ("csrc/hy_kernels.hip", "csrc/hy_trace.hip", "csrc/hy_pair.hip")
     for k in range(5)]
@@ -1038,7 +1037,8 @@ class Scratch:
         return Tuning._from(c)
 
     def profile_read(self, reset: bool = False) -> tuple[float, float, int]:
-        """(inter-kernel ms, long-target-kernel ms, scans timed), accumulated."""
+        """(inter-kernel ms, second-stream ms, scans timed), accumulated; the second stream runs
+        the long targets and, beside the packed kernel, the targets that fail its width proof."""
         a, b, n = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_uint64(0)
         load_library().hy_scratch_profile_read(self._h, ctypes.byref(a), ctypes.byref(b),
                                                ctypes.byref(n), 1 if reset else 0)

@@ -478,6 +478,7 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
         };
         add_groups(0, n_not16);
         db->groups_score32 = (uint32_t)db->groups.size();
+        db->score32_spill = spill;
         add_groups(n_not16, ns);
         db->inter_spill = spill;
         if (dev_upload(&db->d_groups, db->groups) != hipSuccess) return fail(HY_E_DEVICE);
@@ -598,8 +599,11 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     const size_t qcap = std::max<uint64_t>(db->mql, 1);
     if (hipMalloc((void**)&s->d_query, qcap) != hipSuccess) return fail(HY_E_OUT_OF_MEMORY);
     if (hipHostMalloc((void**)&s->h_query, qcap) != hipSuccess) return fail(HY_E_OUT_OF_MEMORY);
-    // One boundary buffer serves both inter kernels: they run one after the other on the stream.
-    s->spill_entries = std::max<uint64_t>(std::max(db->inter_spill, db->inter16_spill), 1);
+    // Boundary rows: the packed kernel's groups from entry 0; the 32-bit groups that run beside it
+    // (targets that fail the packed width proof, on the auxiliary stream) after them. An
+    // end-tracking scan that runs every 32-bit group alone uses the buffer from entry 0.
+    s->spill_entries = std::max<uint64_t>(
+        std::max(db->inter_spill, db->inter16_spill + db->score32_spill), 1);
     if (hipMalloc((void**)&s->d_spill, s->spill_entries * sizeof(int2)) != hipSuccess)
         return fail(HY_E_OUT_OF_MEMORY);
     // Systolic boundary rows are needed only when a query can exceed one pass (512 rows).
@@ -699,27 +703,6 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     ia.out_tend = s->d_tend;
     ia.key = ends ? nullptr : key;
 
-    // Long targets run on the auxiliary stream, forked from and joined back into the main one,
-    // so their few long wavefronts overlap the inter-sequence kernels instead of trailing them.
-    const bool tail = !db->sys_list.empty();
-    if (tail) {
-        if (hipEventRecord(s->fork, s->stream) != hipSuccess ||
-            hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess)
-            return HY_E_DEVICE;
-        hy::SysArgs sa = sys_args(db, s, m);
-        sa.out_score = ia.out_score;
-        sa.out_qend = s->d_qend;
-        sa.out_tend = s->d_tend;
-        sa.key = ia.key;
-        if (s->profile && hipEventRecord(s->ev[2], s->aux) != hipSuccess) return HY_E_DEVICE;
-        if (hy::launch_sys32((int)db->mode, ends, false, sa, (uint32_t)db->sys_list.size(),
-                             s->aux) != hipSuccess)
-            return HY_E_DEVICE;
-        if (s->profile && hipEventRecord(s->ev[3], s->aux) != hipSuccess) return HY_E_DEVICE;
-        if (hipEventRecord(s->join, s->aux) != hipSuccess) return HY_E_DEVICE;
-    }
-
-    if (s->profile && hipEventRecord(s->ev[0], s->stream) != hipSuccess) return HY_E_DEVICE;
     // Score-only scans run the targets the packed kernel accepts there and the rest at 32 bits.
     // End tracking does the same in local mode (the packed kernel's end-tracking form keeps
     // columns in 16 bits, so groups must stay within 65,535 columns); the other modes track ends
@@ -727,11 +710,50 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     const bool has16 = db->groups16.size() > db->groups16_short_first;
     const bool packed_ends = ends && db->mode == HY_MODE_SW && has16 &&
                              db->groups16[db->groups16_short_first].ncols <= 65535u;
+    const bool run16 = (!ends || packed_ends) && has16;
     const uint32_t g32 =
         (ends && !packed_ends) ? (uint32_t)db->groups.size() : db->groups_score32;
-    if (hy::launch_inter32((int)db->mode, ends, ia, g32, s->stream) != hipSuccess)
+
+    // Work beside the packed kernel goes to the auxiliary stream, forked from and joined back
+    // into the main one, so its few long wavefronts overlap the packed kernel instead of
+    // trailing or preceding it: the long targets (systolic kernel), and the 32-bit groups when
+    // the packed kernel runs too. Those groups hold targets that fail the packed width proof,
+    // the longest ones below the long-target threshold: a handful of groups, each lane running
+    // m x (its length) cells. Measured on 2M protein targets with queries up to 3,000: 9 such
+    // groups ahead of the packed kernel on one stream took 71 ms per query on average (4.4k
+    // GCUPS); beside it, the packed kernel's rate.
+    const bool tail = !db->sys_list.empty();
+    const bool side32 = run16 && g32 > 0;
+    const bool fork = tail || side32;
+    if (fork) {
+        if (hipEventRecord(s->fork, s->stream) != hipSuccess ||
+            hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess)
+            return HY_E_DEVICE;
+        if (s->profile && hipEventRecord(s->ev[2], s->aux) != hipSuccess) return HY_E_DEVICE;
+        if (side32) {
+            hy::InterArgs sa = ia;
+            sa.spill = s->d_spill + db->inter16_spill;  // a region of their own
+            if (hy::launch_inter32((int)db->mode, ends, sa, g32, s->aux) != hipSuccess)
+                return HY_E_DEVICE;
+        }
+        if (tail) {
+            hy::SysArgs sa = sys_args(db, s, m);
+            sa.out_score = ia.out_score;
+            sa.out_qend = s->d_qend;
+            sa.out_tend = s->d_tend;
+            sa.key = ia.key;
+            if (hy::launch_sys32((int)db->mode, ends, false, sa, (uint32_t)db->sys_list.size(),
+                                 s->aux) != hipSuccess)
+                return HY_E_DEVICE;
+        }
+        if (s->profile && hipEventRecord(s->ev[3], s->aux) != hipSuccess) return HY_E_DEVICE;
+        if (hipEventRecord(s->join, s->aux) != hipSuccess) return HY_E_DEVICE;
+    }
+
+    if (s->profile && hipEventRecord(s->ev[0], s->stream) != hipSuccess) return HY_E_DEVICE;
+    if (!side32 && hy::launch_inter32((int)db->mode, ends, ia, g32, s->stream) != hipSuccess)
         return HY_E_DEVICE;
-    if ((!ends || packed_ends) && has16) {
+    if (run16) {
         hy::Inter16Args pa{};
         pa.groups = db->d_groups16;
         pa.group_first = db->groups16_short_first;
@@ -760,10 +782,10 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
         }
     }
     if (s->profile && hipEventRecord(s->ev[1], s->stream) != hipSuccess) return HY_E_DEVICE;
-    if (tail && hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess) return HY_E_DEVICE;
+    if (fork && hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess) return HY_E_DEVICE;
     if (s->profile) {
         s->pending = true;
-        s->pending_tail = tail;
+        s->pending_tail = fork;
     }
     return HY_OK;
 }

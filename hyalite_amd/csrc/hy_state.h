@@ -43,6 +43,7 @@ struct hy_db {
     // Groups [0, groups_score32) hold the targets that are not eligible for the packed kernel;
     // score-only scans run the 32-bit kernel over those and the packed kernel over the rest.
     uint32_t groups_score32 = 0;
+    uint64_t score32_spill = 0;    // int2 entries of groups [0, groups_score32)
     // Packed 16-bit partition (two targets per lane).
     std::vector<hy::Group> groups16;
     hy::Group* d_groups16 = nullptr;

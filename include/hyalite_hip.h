@@ -304,8 +304,9 @@ int hy_scratch_autotune(const hy_db* db, hy_scratch* s, const uint8_t* const* qu
 /* Instrumentation (no counterpart in the reference; for benchmarks and profilers). When enabled,
  * every scan records HIP events on the scratch's own stream around the per-target kernel
  * launches, and hy_scratch_profile_read returns the accumulated device time: `inter` is the
- * inter-sequence kernel (the dominant one), `tail` the systolic kernel over long targets, and
- * `launches` the number of scans timed. reset != 0 clears the counters after reading. */
+ * inter-sequence kernel (the dominant one), `tail` the work beside it on the scratch's second
+ * stream (the systolic kernel over long targets, and the 32-bit kernel over targets that fail the
+ * packed width proof when the packed kernel runs too), and `launches` the number of scans timed. reset != 0 clears the counters after reading. */
 int hy_scratch_profile(hy_scratch* s, int enable);
 int hy_scratch_profile_read(hy_scratch* s, double* inter_ms, double* tail_ms, uint64_t* launches,
                             int reset);

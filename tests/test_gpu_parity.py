@@ -361,6 +361,30 @@ def test_batch_with_unpackable_long_targets():
 
 
 @pytest.mark.parametrize("mode", ALL_MODES)
+def test_unpackable_groups_beside_the_packed_kernel(mode):
+    """Targets that fail the packed width proof but stay below the long-target threshold run in
+    the 32-bit kernel on the scratch's second stream while the packed kernel runs on the first,
+    each with boundary rows of its own: three 32-bit groups (some targets holding a piece of the
+    query, so the winner falls there for some queries) beside 16 packed groups, queries of one to
+    many register tiles."""
+    rng = np.random.default_rng(1400 + mode)
+    scheme = (4, [40 if a == b else -3 for a in range(4) for b in range(4)], 6, 2)
+    q = rng.integers(0, 4, size=900, dtype=np.uint8)
+    seqs = [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in rng.integers(20, 120, 2000)]
+    for k in range(0, 2000, 13):  # 154 targets of 850..1000 residues: beyond the packed bound
+        t = rng.integers(0, 4, size=int(rng.integers(850, 1001)), dtype=np.uint8)
+        if k % 5 == 0:
+            a, at = int(rng.integers(0, 600)), int(rng.integers(0, 500))
+            t[at:at + 300] = q[a:a + 300]
+        seqs[k] = t
+    queries = [q, q[:300], q[600:], q[:65], q[:33], rng.integers(0, 4, size=40, dtype=np.uint8)]
+    for st in ST:
+        check(seqs, scheme, mode, st, queries, mql=900)
+    db = build(seqs, scheme, mode, hy.SearchType.Score, 900)
+    assert db.partition_sizes()[hy.ScoreWidth.I32] == 154
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
 def test_every_remainder_tile_size(mode):
     """Query lengths 1..73 take every remainder tile (8, 16 and full rows) after 0, 1 and 2 full
     tiles; per-target scores and ends equal the oracle's."""
