@@ -210,12 +210,25 @@ struct PairArgs {
 
 // Local-mode pairs two per lane (k_pairs16, alphabets up to kPairs16MaxAlphabet): a group
 // (PairGroup, count <= 128) puts slot first + 2l in the low 16-bit half of lane l and slot
-// first + 2l + 1 in the high half. With S = al + 1 (symbol al pads), a target word holds four
-// columns as bytes ta * S + tb, a query word two rows as 16-bit byte offsets
-// 4 * (qa * S + qb) * S^2 into the kernel's LDS table T[(qa*S + qb)*S^2 + ta*S + tb] =
-// (sub(qa, ta), sub(qb, tb)), padding pairs scoring -32768. Every pair's width proof gives 16
-// bits, its query is at most 4,096 and its target at most 65,535 symbols.
-constexpr uint32_t kPairs16MaxAlphabet = 7;
+// first + 2l + 1 in the high half. With S = al + 1 (symbol al pads), padding scoring -32768:
+//  * alphabets up to kPairs16TableMaxAlphabet: a target word holds four columns as bytes
+//    ta * S + tb, a query word two rows as 16-bit byte offsets 4 * (qa * S + qb) * S^2 into the
+//    kernel's LDS table T[(qa*S + qb)*S^2 + ta*S + tb] = (sub(qa, ta), sub(qb, tb));
+//  * larger ones, up to kPairs16MaxAlphabet (pairs16_wide): four columns per lane take a uint2
+//    (twords[tword_off + (col/4) * 128 + 2 * lane + {0, 1}]), 16 bits per column holding the
+//    symbols ta and tb; a query word holds one row as the 16-bit byte offsets 4 * C * S * qa and
+//    4 * C * S * qb into the kernel's S x S table of scores, one lookup per half. The table keeps
+//    each score in a 32-bit slot, kPairs16Copies = C copies interleaved (entry e, copy c at slot
+//    e * C + c) and lane l reading copy l mod C, so a half-wavefront's 32 lookups meet at most
+//    two to a bank instead of landing on random banks: at most 40,000 bytes (S <= 25).
+// Every pair's width proof gives 16 bits, its query is at most 4,096 and its target at most
+// 65,535 symbols.
+constexpr uint32_t kPairs16MaxAlphabet = 24;
+constexpr uint32_t kPairs16TableMaxAlphabet = 7;
+constexpr uint32_t kPairs16Copies = 16;
+__host__ __device__ inline bool pairs16_wide(int al) {
+    return al > (int)kPairs16TableMaxAlphabet;
+}
 struct Pair16Args {
     const PairGroup* groups;
     const uint32_t* twords;
@@ -225,6 +238,7 @@ struct Pair16Args {
     const uint32_t* pidx;     // per slot -> pair index
     const int32_t* matrix;    // al x al
     int al, go, ge;
+    uint32_t ngroups;         // set by the launcher
     uint2* spill;             // tile boundary rows (H, F') per column
     int32_t* out_score;       // per pair
     int64_t* out_qend;

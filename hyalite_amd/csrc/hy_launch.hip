@@ -73,6 +73,7 @@ __global__ __launch_bounds__(64) void k_pair_pack(PairPackArgs a) {
 // 16-bit byte offsets 4 * (qa * S + qb) * S^2 (S = al + 1, symbol al pads; symbols past the
 // alphabet become padding, as in k_pair_pack).
 // Four threads per lane (256 per group), each writing every fourth word of the lane.
+template <bool WIDE>
 __global__ __launch_bounds__(256) void k_pair_pack16(PairPackArgs a) {
     const PairGroup g = a.groups[blockIdx.x];
     const int lane = threadIdx.x & 63, part = threadIdx.x >> 6;
@@ -104,25 +105,43 @@ __global__ __launch_bounds__(256) void k_pair_pack16(PairPackArgs a) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) xb |= (c0 + k < tnb ? (uint32_t)tb[c0 + k] : pad) << (8 * k);
         }
-        uint32_t v = 0;
+        if constexpr (WIDE) {
+            // Column c: bytes ta, tb.
+            uint32_t v[2] = {0u, 0u};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const uint32_t x = min((xa >> (8 * k)) & 0xffu, pad);
-            const uint32_t y = min((xb >> (8 * k)) & 0xffu, pad);
-            v |= (x * S + y) << (8 * k);
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t x = min((xa >> (8 * k)) & 0xffu, pad);
+                const uint32_t y = min((xb >> (8 * k)) & 0xffu, pad);
+                v[k >> 1] |= (x | y << 8) << (16 * (k & 1));
+            }
+            reinterpret_cast<uint2*>(a.twords + g.tword_off)[(uint64_t)w * 64 + lane] =
+                make_uint2(v[0], v[1]);
+        } else {
+            uint32_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const uint32_t x = min((xa >> (8 * k)) & 0xffu, pad);
+                const uint32_t y = min((xb >> (8 * k)) & 0xffu, pad);
+                v |= (x * S + y) << (8 * k);
+            }
+            a.twords[g.tword_off + (uint64_t)w * 64 + lane] = v;
         }
-        a.twords[g.tword_off + (uint64_t)w * 64 + lane] = v;
     }
     const uint32_t qt = (uint32_t)a.qtile;
-    const uint32_t qw = (g.nrows + qt - 1) / qt * qt / 2;
+    const uint32_t row_bytes = 4 * kPairs16Copies * S;  // WIDE table rows
+    constexpr uint32_t rows_per_word = WIDE ? 1 : 2;
+    const uint32_t qw = (g.nrows + qt - 1) / qt * qt / rows_per_word;
     for (uint32_t w = part; w < qw; w += 4) {
         uint32_t v = 0;
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            const uint64_t r = (uint64_t)w * 2 + k;
+        for (uint32_t k = 0; k < rows_per_word; ++k) {
+            const uint64_t r = (uint64_t)w * rows_per_word + k;
             const uint32_t x = r < qna ? min((uint32_t)qa[r], pad) : pad;
             const uint32_t y = r < qnb ? min((uint32_t)qb[r], pad) : pad;
-            v |= (4 * (x * S + y) * S * S) << (16 * k);
+            // Byte offsets of the rows: of a pair-table row (two rows per word), or of each
+            // half's row of the score table (WIDE, one row per word).
+            v |= WIDE ? (row_bytes * x) | (row_bytes * y) << 16
+                      : (4 * (x * S + y) * S * S) << (16 * k);
         }
         a.qwords[g.qword_off + (uint64_t)w * 64 + lane] = v;
     }
@@ -295,8 +314,11 @@ hipError_t launch_pair_pack(const PairPackArgs& a, uint32_t ngroups, hipStream_t
 
 hipError_t launch_pair_pack16(const PairPackArgs& a, uint32_t ngroups, hipStream_t stream) {
     if (ngroups == 0) return hipSuccess;
-    if (a.pad_symbol > kPairs16MaxAlphabet) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_pair_pack16, dim3(ngroups), dim3(256), 0, stream, a);
+    if (a.pad_symbol < 1 || a.pad_symbol > kPairs16MaxAlphabet) return hipErrorInvalidValue;
+    if (pairs16_wide((int)a.pad_symbol))
+        hipLaunchKernelGGL(k_pair_pack16<true>, dim3(ngroups), dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(k_pair_pack16<false>, dim3(ngroups), dim3(256), 0, stream, a);
     return hipGetLastError();
 }
 

@@ -213,9 +213,14 @@ int kernel_choice() {
 
 bool band_eligible(uint32_t al) { return al >= 1 && al <= hy::kBandMaxAlphabet; }
 
-bool pairs16_enabled() {
+// HYALITE_HIP_PAIRS16: 0 keeps lane pairs one per lane, 2 puts every eligible pair two per lane,
+// anything else (the default) lets the batch size decide for alphabets beyond the pair table.
+enum Pairs16Choice { kPairs16Off, kPairs16Auto, kPairs16Always };
+Pairs16Choice pairs16_choice() {
     const char* v = std::getenv("HYALITE_HIP_PAIRS16");
-    return !(v && std::strcmp(v, "0") == 0);
+    if (v && std::strcmp(v, "0") == 0) return kPairs16Off;
+    if (v && std::strcmp(v, "2") == 0) return kPairs16Always;
+    return kPairs16Auto;
 }
 
 // Pairs that do not run one per lane go to the band kernel when the alphabet fits it, except
@@ -720,10 +725,12 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     }
     const bool band = use_band(pick, alphabet_len, waves.size(), tracks_end(search_type));
     // Local-mode lane pairs of alphabets up to kPairs16MaxAlphabet whose width proof gives 16
-    // bits (queries up to kLaneMaxQuery, targets below 65,536) run two per lane (k_pairs16).
+    // bits (queries up to kLaneMaxQuery, targets below 65,536) run two per lane (k_pairs16: one
+    // pair-table lookup per cell pair up to 7 symbols, two 16-bit lookups beyond).
     // HYALITE_HIP_PAIRS16=0 keeps them one per lane.
     std::vector<uint32_t> lanes16;
-    if (mode == HY_MODE_SW && alphabet_len <= hy::kPairs16MaxAlphabet && pairs16_enabled() &&
+    const Pairs16Choice choice16 = pairs16_choice();
+    if (mode == HY_MODE_SW && alphabet_len <= hy::kPairs16MaxAlphabet && choice16 != kPairs16Off &&
         !lanes.empty()) {
         auto bits16 = [&](size_t m, size_t n) {
             int64_t b = 0;
@@ -749,6 +756,22 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
             }
             lanes.swap(rest);
         }
+    }
+    // Alphabets beyond the pair table take two lookups per cell pair: two pairs per lane pay off
+    // once their groups (128 pairs) cover half the SIMDs; fewer leave SIMDs idle that one pair
+    // per lane (twice the groups) would use. 16-row tiles (two wavefronts per SIMD; the 32-row
+    // form needs 256 registers, one) beyond one group per SIMD, 32-row tiles below. Measured on protein pairs of 250-350 x 300-400 (SW,
+    // Score, 1,024 SIMDs): 50,000 pairs 1,496 GCUPS one per lane vs 1,165 two per lane (32-row
+    // tiles); 100,000 pairs 2,045 vs 2,209 (32) and 2,078 (16); 200,000 pairs 2,469 vs 2,577 (32)
+    // and 2,797 (16); 400,000 pairs 2,830 vs 2,898 and 3,278.
+    const bool wide16 = hy::pairs16_wide((int)alphabet_len);
+    const uint64_t simds = 4ull * hy::device_cus();
+    const uint64_t ngroups16 = (lanes16.size() + 127) / 128;
+    if (wide16 && choice16 == kPairs16Auto && !lanes16.empty() && 2 * ngroups16 < simds) {
+        std::vector<uint32_t> merged(lanes.size() + lanes16.size());
+        std::merge(lanes.begin(), lanes.end(), lanes16.begin(), lanes16.end(), merged.begin());
+        lanes.swap(merged);
+        lanes16.clear();
     }
     // Lanes: sorted by (query length, target length) descending, cut into groups. The sort key
     // packs both lengths (complemented, 13 + 17 bits: lane pairs are at most 4096 x 65536 unless
@@ -799,18 +822,20 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
     // (Score), 1.78k vs 1.62k (ScoreEnd). HYALITE_HIP_PAIRS_TILE=16|32 overrides it.
     int qtile = ends ? 16 : 32;
     if (const char* v = std::getenv("HYALITE_HIP_PAIRS_TILE")) qtile = std::atoi(v) >= 32 ? 32 : 16;
-    // Two pairs per lane: 32-row tiles for both search types (200,000 150 x 500 DNA pairs:
-    // 3.57k vs 3.34k GCUPS Score, 2.76k vs 2.63k ScoreEnd against 16-row tiles).
-    // HYALITE_HIP_PAIRS16_TILE=16|32 overrides it.
-    int qtile16 = 32;
+    // Two pairs per lane: 32-row tiles for both search types with the pair table (200,000
+    // 150 x 500 DNA pairs: 3.57k vs 3.34k GCUPS Score, 2.76k vs 2.63k ScoreEnd against 16-row
+    // tiles); with two lookups as measured above. HYALITE_HIP_PAIRS16_TILE=16|32 overrides it.
+    int qtile16 = (wide16 && ngroups16 > simds) ? 16 : 32;
     if (const char* v = std::getenv("HYALITE_HIP_PAIRS16_TILE"))
         qtile16 = std::atoi(v) >= 32 ? 32 : 16;
     // Groups of both lane kernels in one array (k_pairs32's first), their slots in one list
     // (lanes, then lanes16), their words and boundary rows in shared buffers.
     std::vector<hy::PairGroup> groups;
     uint64_t twords = 0, qwords = 0, spill = 0;
+    // Per group: target words (words_per_step per four columns), query words (rows_per_word rows
+    // each, whole tiles) and, past one tile, boundary rows.
     auto add_groups = [&](const std::vector<uint32_t>& list, size_t base, uint32_t per, int qt,
-                          int rows_per_word) {
+                          int rows_per_word, uint32_t words_per_step) {
         for (size_t g0 = 0; g0 < list.size(); g0 += per) {
             hy::PairGroup g{};
             g.first = (uint32_t)(base + g0);
@@ -821,7 +846,7 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
             }
             const uint64_t tiles = (g.nrows + qt - 1) / qt;
             g.tword_off = twords;
-            twords += (uint64_t)((g.ncols + 3) / 4) * 64;
+            twords += (uint64_t)((g.ncols + 3) / 4) * words_per_step;
             g.qword_off = qwords;
             qwords += tiles * (uint64_t)(qt / rows_per_word) * 64;
             g.spill_off = tiles > 1 ? spill : 0;
@@ -829,9 +854,9 @@ int align_pairs_core(hy_pair_scratch* s, const PairSource& src, size_t npairs,
             groups.push_back(g);
         }
     };
-    add_groups(lanes, 0, 64, qtile, 4);
+    add_groups(lanes, 0, 64, qtile, 4, 64);
     const size_t ng32 = groups.size();
-    add_groups(lanes16, lanes.size(), 128, qtile16, 2);
+    add_groups(lanes16, lanes.size(), 128, qtile16, wide16 ? 1 : 2, wide16 ? 128 : 64);
     const size_t ng16 = groups.size() - ng32;
 
     Stager st{s};

@@ -481,10 +481,15 @@ hipError_t launch_band_o(const BandArgs& a, uint32_t nblocks, int m, hipStream_t
 // Local-mode pairs, two per lane in packed 16-bit arithmetic.
 // ------------------------------------------------------------------------------------------------
 // Lane l aligns slot first + 2l in the low half and first + 2l + 1 in the high half of every
-// register. One LDS lookup gives both halves' substitution scores: with S = al + 1 (symbol al
-// pads) the table T[(qa*S + qb)*S^2 + ta*S + tb] holds (sub(qa, ta), sub(qb, tb)); a lane keeps
-// the byte offsets of its tile's query-row pairs in registers and reads the column's target pair
-// from its words. The recurrence is the packed local one of the scan kernels, every quantity
+// register. With S = al + 1 (symbol al pads), small alphabets (WIDE = false) take both halves'
+// substitution scores from one LDS lookup: the table T[(qa*S + qb)*S^2 + ta*S + tb] holds
+// (sub(qa, ta), sub(qb, tb)); a lane keeps the byte offsets of its tile's query-row pairs in
+// registers and reads the column's target pair from its words. Larger alphabets (WIDE = true)
+// look each half up in an S x S score table kept in C = kPairs16Copies interleaved copies (slot
+// e*C + c), lane l reading copy l mod C so that lookups spread over the LDS banks: a query word
+// holds the two rows' byte offsets 4*C*S*qa and 4*C*S*qb, a target column the symbols ta and
+// tb. WIDE workgroups hold four wavefronts (four groups) sharing one table.
+// The recurrence is the packed local one of the scan kernels, every quantity
 // floored at 0 (H exact, E' = max(E, 0), F' = max(F, 0)):
 //     H = max(H[i-1][j-1] + sub, E', F'),  E' = max(H - go, E' - ge)+,  F' = max(H - go, F' - ge)+
 // so a cell pair costs the lookup and 9 packed operations. Cells past a pair's own query or
@@ -492,25 +497,41 @@ hipError_t launch_band_o(const BandArgs& a, uint32_t nblocks, int m, hipStream_t
 // so neither the running maximum nor the strictly-greater end tracking needs a mask. With ends,
 // each column keeps its maximum and the first row reaching it; a column that strictly beats the
 // tile's best replaces it; each tile's best is folded into the lane's with the full tie-break.
-template <int Q, bool ENDS>
-__global__ __launch_bounds__(64) void k_pairs16(Pair16Args a) {
+template <int Q, bool ENDS, bool WIDE>
+__global__ __launch_bounds__(WIDE ? 256 : 64) void k_pairs16(Pair16Args a) {
     extern __shared__ uint32_t tab[];
-    const int lane = threadIdx.x;
-    const PairGroup g = a.groups[blockIdx.x];
+    const int lane = threadIdx.x & 63;
+    const uint32_t gi = WIDE ? blockIdx.x * 4 + (threadIdx.x >> 6) : blockIdx.x;
     const int al = a.al, S = al + 1, S2 = S * S, go = a.go, ge = a.ge;
-    for (int k = lane; k < S2 * S2; k += 64) {
-        const int qa = k / (S2 * S), qb = (k / S2) % S, ta = (k / S) % S, tb = k % S;
-        const int lo = (qa < al && ta < al) ? sat16(a.matrix[qa * al + ta]) : -32768;
-        const int hi = (qb < al && tb < al) ? sat16(a.matrix[qb * al + tb]) : -32768;
-        tab[k] = as_u32(pair2(lo, hi));
+    constexpr int C = kPairs16Copies;
+    if constexpr (WIDE) {
+        for (int k = threadIdx.x; k < S2 * C; k += 256) {
+            const int q = k / C / S, t = k / C % S;
+            tab[k] = (uint16_t)(q < al && t < al ? sat16(a.matrix[q * al + t]) : -32768);
+        }
+    } else {
+        for (int k = lane; k < S2 * S2; k += 64) {
+            const int qa = k / (S2 * S), qb = (k / S2) % S, ta = (k / S) % S, tb = k % S;
+            const int lo = (qa < al && ta < al) ? sat16(a.matrix[qa * al + ta]) : -32768;
+            const int hi = (qb < al && tb < al) ? sat16(a.matrix[qb * al + tb]) : -32768;
+            tab[k] = as_u32(pair2(lo, hi));
+        }
     }
     __syncthreads();
+    if (gi >= a.ngroups) return;  // no barrier below
+    const PairGroup g = a.groups[gi];
+    const uint8_t* tb8 = reinterpret_cast<const uint8_t*>(tab);
+    // WIDE: the lane's copy, added to a column's symbol offsets.
+    const uint32_t copy4 = 4u * (uint32_t)(lane % C);
 
     const s2v go2 = splat2(go), ge2 = splat2(ge), zero2 = splat2(0), one2 = splat2(1);
     const int L = (int)g.ncols;
     const int ntiles = ((int)g.nrows + Q - 1) / Q;
     uint2* sp = a.spill + g.spill_off + lane;
+    // Target words: four columns per lane and step, one word (bytes) or a uint2 (WIDE: 16 bits
+    // per column).
     const uint32_t* wp = a.twords + g.tword_off + lane;
+    const uint2* wp2 = reinterpret_cast<const uint2*>(a.twords + g.tword_off) + lane;
     const uint32_t* qp = a.qwords + g.qword_off + lane;
     // Lane bests per half (score, column, row); (0, 0, 0) is the empty alignment.
     int bs[2] = {0, 0}, bj[2] = {0, 0}, bi[2] = {0, 0};
@@ -542,11 +563,16 @@ __global__ __launch_bounds__(64) void k_pairs16(Pair16Args a) {
             tc = p_maxu(tc, p_madu(up, splat2(j), zero2));
             tr = p_madu(up, p_subw(cr, tr), tr);
         };
+        if constexpr (WIDE) {
 #pragma unroll
-        for (int r2 = 0; r2 < Q / 2; ++r2) {
-            const uint32_t qw = qp[(size_t)(c * (Q / 2) + r2) * 64];
-            qrow[2 * r2] = qw & 0xffffu;
-            qrow[2 * r2 + 1] = qw >> 16;
+            for (int r = 0; r < Q; ++r) qrow[r] = qp[(size_t)(c * Q + r) * 64];
+        } else {
+#pragma unroll
+            for (int r2 = 0; r2 < Q / 2; ++r2) {
+                const uint32_t qw = qp[(size_t)(c * (Q / 2) + r2) * 64];
+                qrow[2 * r2] = qw & 0xffffu;
+                qrow[2 * r2 + 1] = qw >> 16;
+            }
         }
 #pragma unroll
         for (int r = 0; r < Q; ++r) {
@@ -555,18 +581,20 @@ __global__ __launch_bounds__(64) void k_pairs16(Pair16Args a) {
         }
         s2v HGd = zero2;
 
-        uint32_t w = L > 0 ? wp[0] : 0u;
+        uint2 w = make_uint2(0u, 0u);
+        if (L > 0) w = WIDE ? wp2[0] : make_uint2(wp[0], 0u);
         uint2 spv[4];
 #pragma unroll
         for (int b = 0; b < 4; ++b)
             spv[b] = (c > 0 && b + 1 <= L) ? sp[(size_t)(b + 1) * 64] : make_uint2(0u, 0u);
         for (int jb = 0; jb < L; jb += 4) {
-            uint32_t wn = 0u;
+            uint2 wn = make_uint2(0u, 0u);
             uint2 spn[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) spn[b] = make_uint2(0u, 0u);
             if (jb + 4 < L) {
-                wn = wp[(size_t)((jb >> 2) + 1) * 64];
+                if constexpr (WIDE) wn = wp2[(size_t)((jb >> 2) + 1) * 64];
+                else wn.x = wp[(size_t)((jb >> 2) + 1) * 64];
                 if (c > 0) {
 #pragma unroll
                     for (int b = 0; b < 4; ++b)
@@ -580,15 +608,33 @@ __global__ __launch_bounds__(64) void k_pairs16(Pair16Args a) {
             for (int bp = 0; bp < 4; bp += 2) {
                 const int jA = jb + bp + 1, jB = jA + 1;
                 if (jA > L) break;
-                const uint32_t tpA = 4u * ((w >> (8 * bp)) & 0xffu);
-                const uint32_t tpB = 4u * ((w >> (8 * bp + 8)) & 0xffu);
                 s2v sgA[Q], sgB[Q];
+                if constexpr (WIDE) {
+                    // Column jA's symbols in w.x (bp = 0) or w.y (bp = 2), column jB's above.
+                    const uint32_t wv = bp == 0 ? w.x : w.y;
+                    const uint32_t aA = (wv & 0xffu) * (4 * C) + copy4;
+                    const uint32_t bA = ((wv >> 8) & 0xffu) * (4 * C) + copy4;
+                    const uint32_t aB = ((wv >> 16) & 0xffu) * (4 * C) + copy4;
+                    const uint32_t bB = (wv >> 24) * (4 * C) + copy4;
+                    auto look = [&](uint32_t q, uint32_t ta, uint32_t tb) HY_INLINE {
+                        s2v v;
+                        v.x = *reinterpret_cast<const short*>(tb8 + (q & 0xffffu) + ta);
+                        v.y = *reinterpret_cast<const short*>(tb8 + (q >> 16) + tb);
+                        return v;
+                    };
 #pragma unroll
-                for (int r = 0; r < Q; ++r) {
-                    sgA[r] = as_s2(*reinterpret_cast<const uint32_t*>(
-                        reinterpret_cast<const uint8_t*>(tab) + qrow[r] + tpA));
-                    sgB[r] = as_s2(*reinterpret_cast<const uint32_t*>(
-                        reinterpret_cast<const uint8_t*>(tab) + qrow[r] + tpB));
+                    for (int r = 0; r < Q; ++r) {
+                        sgA[r] = look(qrow[r], aA, bA);
+                        sgB[r] = look(qrow[r], aB, bB);
+                    }
+                } else {
+                    const uint32_t tpA = 4u * ((w.x >> (8 * bp)) & 0xffu);
+                    const uint32_t tpB = 4u * ((w.x >> (8 * bp + 8)) & 0xffu);
+#pragma unroll
+                    for (int r = 0; r < Q; ++r) {
+                        sgA[r] = as_s2(*reinterpret_cast<const uint32_t*>(tb8 + qrow[r] + tpA));
+                        sgB[r] = as_s2(*reinterpret_cast<const uint32_t*>(tb8 + qrow[r] + tpB));
+                    }
                 }
                 s2v fA = as_s2(spv[bp].y), fB = as_s2(spv[bp + 1].y);
                 s2v hdA = HGd, hdB = as_s2(spv[bp].x);
@@ -645,11 +691,14 @@ __global__ __launch_bounds__(64) void k_pairs16(Pair16Args a) {
     }
 }
 
-template <int Q, bool ENDS>
+template <int Q, bool ENDS, bool WIDE>
 hipError_t launch_pairs16_q(const Pair16Args& a, uint32_t ngroups, hipStream_t st) {
-    const int S = a.al + 1;
-    const size_t lds = (size_t)S * S * S * S * sizeof(uint32_t);
-    hipLaunchKernelGGL((k_pairs16<Q, ENDS>), dim3(ngroups), dim3(64), lds, st, a);
+    const size_t S = (size_t)a.al + 1;
+    const size_t lds = WIDE ? S * S * kPairs16Copies * 4 : S * S * S * S * 4;
+    Pair16Args b = a;
+    b.ngroups = ngroups;
+    hipLaunchKernelGGL((k_pairs16<Q, ENDS, WIDE>), dim3(WIDE ? (ngroups + 3) / 4 : ngroups),
+                       dim3(WIDE ? 256 : 64), lds, st, b);
     return hipGetLastError();
 }
 
@@ -810,10 +859,16 @@ hipError_t launch_pairs16_sw(bool ends, const Pair16Args& a, uint32_t ngroups, i
                              hipStream_t st) {
     if (ngroups == 0) return hipSuccess;
     if (a.al < 1 || a.al > (int)kPairs16MaxAlphabet) return hipErrorInvalidValue;
-    if (ends) return q <= 16 ? launch_pairs16_q<16, true>(a, ngroups, st)
-                             : launch_pairs16_q<32, true>(a, ngroups, st);
-    return q <= 16 ? launch_pairs16_q<16, false>(a, ngroups, st)
-                   : launch_pairs16_q<32, false>(a, ngroups, st);
+    if (pairs16_wide(a.al)) {
+        if (ends) return q <= 16 ? launch_pairs16_q<16, true, true>(a, ngroups, st)
+                                 : launch_pairs16_q<32, true, true>(a, ngroups, st);
+        return q <= 16 ? launch_pairs16_q<16, false, true>(a, ngroups, st)
+                       : launch_pairs16_q<32, false, true>(a, ngroups, st);
+    }
+    if (ends) return q <= 16 ? launch_pairs16_q<16, true, false>(a, ngroups, st)
+                             : launch_pairs16_q<32, true, false>(a, ngroups, st);
+    return q <= 16 ? launch_pairs16_q<16, false, false>(a, ngroups, st)
+                   : launch_pairs16_q<32, false, false>(a, ngroups, st);
 }
 
 // Rows per lane as for the long-target kernel: 2, 4, 6 or 8 (several passes beyond 512 rows).

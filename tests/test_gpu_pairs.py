@@ -522,13 +522,15 @@ def test_large_batch_uploads_through_pinned_slots():
         whole[0].tolist()
 
 
-@pytest.mark.parametrize("al", [2, 4, 7])
+@pytest.mark.parametrize("al", [2, 4, 7, 8, 20, 24, 25])
 def test_two_pairs_per_lane_equal_one_per_lane(al, monkeypatch):
-    """Local-mode lane pairs of small alphabets run two per lane in 16-bit halves (k_pairs16):
-    scores and ends equal the one-pair-per-lane kernel and the oracle, with ties (identity and
-    all-equal matrices), queries across 16/32-row tiles, an odd last group (only low halves
-    live), empty sides, and pairs whose width proof needs 32 bits mixed in (they stay one per
-    lane in the same call)."""
+    """Local-mode lane pairs of alphabets up to 24 symbols run two per lane in 16-bit halves
+    (k_pairs16: one pair-table lookup per cell pair up to 7 symbols, two lookups in a replicated
+    score table from 8 to 24; 25 stays one per lane): scores and ends equal the one-pair-per-lane
+    kernel and the oracle, with ties (identity and all-equal matrices), queries across 16- and
+    32-row tiles (both tile sizes), an odd last group (only low halves live; with 257 pairs, the
+    last four-group workgroup of the replicated table has one live group), empty sides, and
+    pairs whose width proof needs 32 bits mixed in (they stay one per lane in the same call)."""
     rng = np.random.default_rng(1200 + al)
     schemes = [(al, identity_matrix(al, 2, -1), 2, 1),
                (al, [int(v) for v in rng.integers(-3, 5, size=al * al)], 3, 1),
@@ -547,20 +549,23 @@ def test_two_pairs_per_lane_equal_one_per_lane(al, monkeypatch):
                 t[at:at + m] = q
             pairs.append((q, t))
         for st in ST:
-            monkeypatch.delenv("HYALITE_HIP_PAIRS16", raising=False)
-            two = hy.align_pairs(pairs, s, hy.Mode.Sw, st)
             monkeypatch.setenv("HYALITE_HIP_PAIRS16", "0")
             one = hy.align_pairs(pairs, s, hy.Mode.Sw, st)
-            assert two == one, (al, scheme[1][:4], st)
+            monkeypatch.setenv("HYALITE_HIP_PAIRS16", "2")  # two per lane at any batch size
+            for tile in ("32", "16"):
+                monkeypatch.setenv("HYALITE_HIP_PAIRS16_TILE", tile)
+                two = hy.align_pairs(pairs, s, hy.Mode.Sw, st)
+                assert two == one, (al, scheme[1][:4], st, tile)
+            monkeypatch.delenv("HYALITE_HIP_PAIRS16_TILE", raising=False)
             for k, (q, t) in enumerate(pairs):
-                assert two[k] == want_hit(list(q), list(t), scheme, MODE_SW, st, k), (al, k)
+                assert one[k] == want_hit(list(q), list(t), scheme, MODE_SW, st, k), (al, k)
     # 32-bit pairs beside 16-bit ones: a large match score pushes long pairs past 32,767.
     big = (al, identity_matrix(al, 300, -1), 5, 1)
     s = scoring(big)
     pairs = [(rng.integers(0, al, size=m, dtype=np.uint8), rng.integers(0, al, size=n,
                                                                         dtype=np.uint8))
              for m, n in [(20, 40), (150, 200), (10, 10), (120, 130), (3, 500)] * 30]
-    monkeypatch.delenv("HYALITE_HIP_PAIRS16", raising=False)
+    monkeypatch.setenv("HYALITE_HIP_PAIRS16", "2")
     got = hy.align_pairs(pairs, s, hy.Mode.Sw, hy.SearchType.ScoreEnd)
     for k, (q, t) in enumerate(pairs):
         assert got[k] == want_hit(list(q), list(t), big, MODE_SW, hy.SearchType.ScoreEnd, k), k
