@@ -204,10 +204,25 @@ def serving(args, sync) -> dict:
     plain_s = time.perf_counter() - t0
     del sc
     cells = sum(len(q) for q in qs) * w.residues
+    sweep = []
     with hy.Server(db, max_lanes=args.serving_lanes, max_batch=args.serving_batch) as srv:
         srv.wait(srv.submit(qs[0]))
         res = loadgen.run_poisson(srv, qs, qps=args.qps, seed=7)
         st = srv.stats()
+        # Heavier loads on the same server: arrivals outpace one query at a time, so batches
+        # grow and pair in the batch kernel (continuous batching).
+        for qps in args.qps_sweep:
+            r = loadgen.run_poisson(srv, qs, qps=qps, seed=7)
+            if [x.hit for x in r.served] != alone:
+                raise RuntimeError(f"a served hit at {qps} QPS differs from the plain scan")
+            sweep.append({"qps": qps, "p50_ms": round(r.percentile_ms(50), 2),
+                          "p99_ms": round(r.percentile_ms(99), 2),
+                          "mean_ms": round(float(r.latencies().mean() * 1e3), 2),
+                          "mean_batch": round(float(np.mean([x.batch_size for x in r.served])), 2),
+                          "sustained_gcups": round(cells / r.wall_s / 1e9, 1),
+                          "hits_equal_alone": True})
+            log(f"serving at {qps} QPS: p50 {sweep[-1]['p50_ms']} ms, p99 {sweep[-1]['p99_ms']} "
+                f"ms, mean batch {sweep[-1]['mean_batch']}, {sweep[-1]['sustained_gcups']} GCUPS")
     same = [s.hit for s in res.served] == alone
     if not same:
         raise RuntimeError("a served hit differs from the plain scan of the same query")
@@ -226,6 +241,7 @@ def serving(args, sync) -> dict:
         "plain_loop_gcups": round(cells / plain_s / 1e9, 1),
         "plain_loop_ms_per_query": round(plain_s * 1e3 / len(qs), 2),
         "hits_equal_alone": same,
+        "load_sweep": sweep,
     }
 
 
@@ -579,6 +595,8 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="pick launch shapes with the autotuner before timing")
     ap.add_argument("--qps", type=float, default=SERVING_QPS)
+    ap.add_argument("--qps-sweep", type=float, nargs="*", default=[20.0, 40.0],
+                    help="further serving loads on the same server (continuous batching)")
     ap.add_argument("--serving-targets", type=int, default=2_000_000)
     ap.add_argument("--serving-queries", type=int, default=64)
     ap.add_argument("--serving-lanes", type=int, default=3)
