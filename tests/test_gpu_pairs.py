@@ -569,3 +569,34 @@ def test_two_pairs_per_lane_equal_one_per_lane(al, monkeypatch):
     got = hy.align_pairs(pairs, s, hy.Mode.Sw, hy.SearchType.ScoreEnd)
     for k, (q, t) in enumerate(pairs):
         assert got[k] == want_hit(list(q), list(t), big, MODE_SW, hy.SearchType.ScoreEnd, k), k
+
+
+def test_two_per_lane_protein_batch(monkeypatch):
+    """A realistic protein batch (20,000 pairs of 250-350 x 300-400, 24 symbols; 157 groups of
+    128, the last workgroup of four partly live) two per lane with two lookups, at 16- and 32-row
+    tiles: every score and end equals one pair per lane, and a sample equals the oracle."""
+    from hyalite_amd import workloads as wl
+    b = wl.protein_pairs(n=20_000, seed=31)
+    s = hy.Scoring(b.al, b.matrix, b.go, b.ge)
+    scheme = (b.al, [int(v) for v in np.asarray(b.matrix).ravel()], b.go, b.ge)
+    sc = hy.PairScratch()
+    monkeypatch.setenv("HYALITE_HIP_PAIRS_KERNEL", "lanes")  # below the lane kernels' batch size
+    for st in ST:
+        monkeypatch.setenv("HYALITE_HIP_PAIRS16", "0")
+        one = hy.align_pairs_flat(b.queries, b.qoff, b.targets, b.toff, s, hy.Mode.Sw, st,
+                                  scratch=sc)
+        monkeypatch.setenv("HYALITE_HIP_PAIRS16", "2")
+        for tile in ("16", "32"):
+            monkeypatch.setenv("HYALITE_HIP_PAIRS16_TILE", tile)
+            two = hy.align_pairs_flat(b.queries, b.qoff, b.targets, b.toff, s, hy.Mode.Sw, st,
+                                      scratch=sc)
+            for x, y in zip(two, one):
+                assert np.array_equal(x, y), (st, tile)
+        monkeypatch.delenv("HYALITE_HIP_PAIRS16_TILE", raising=False)
+        for k in range(0, b.n, 997):
+            q, t = b.pair(k)
+            w = want_hit(list(q), list(t), scheme, MODE_SW, st, k)
+            got = (int(one[0][k]), None if st == hy.SearchType.Score or one[1][k] < 0
+                   else int(one[1][k]), None if st == hy.SearchType.Score or one[2][k] < 0
+                   else int(one[2][k]))
+            assert got == (w.score, w.query_end, w.target_end), k
