@@ -614,6 +614,7 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
                            multipass ? db->mtl + 1 : 1);
     if (hipMalloc((void**)&s->d_sys_spill, sys_entries * sizeof(int2)) != hipSuccess)
         return fail(HY_E_OUT_OF_MEMORY);
+    s->sys_spill_entries = sys_entries;
     const uint64_t n = db->n;
     if (hipMalloc((void**)&s->d_scores, n * sizeof(int32_t)) != hipSuccess ||
         hipMalloc((void**)&s->d_qend, n * sizeof(int64_t)) != hipSuccess ||
@@ -825,13 +826,37 @@ extern "C" int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, siz
     if ((rc = enqueue_targets(db, s, m, false, false, s->d_key)) != HY_OK) return rc;
     const bool ends = db->search_type != HY_SEARCH_SCORE;
     if (ends) {
-        // The winner's end positions: one systolic pass over the winning target only (the role
-        // of align_core on the winner in src/database.rs:183-197), chosen on the device.
-        hy::SysArgs sa = sys_args(db, s, m);
-        sa.winner_key = s->d_key;
-        sa.result = s->d_result;
-        if (hy::launch_sys32((int)db->mode, true, true, sa, 1, s->stream) != hipSuccess)
-            return HY_E_DEVICE;
+        // The winner's end positions: one pass over the winning target only (the role of
+        // align_core on the winner in src/database.rs:183-197), chosen on the device. The band
+        // kernel's wavefronts split the query rows between them (a long query's winner pass:
+        // 2,168 rows against its winning target took 13 ms in the one-wavefront systolic kernel);
+        // it takes alphabets up to 63 symbols, and boundary rows (queries beyond one of its
+        // passes) only when the scratch holds a target's worth.
+        const bool band = db->al <= hy::kBandMaxAlphabet &&
+                          (m <= hy::band_rows_per_pass(m) || s->sys_spill_entries >= db->mtl + 1);
+        if (band) {
+            hy::BandArgs ba{};
+            ba.queries = s->d_query;
+            ba.wm = m;
+            ba.targets = db->d_flat;
+            ba.toff = db->d_off;
+            ba.matrix = db->d_matrix;
+            ba.al = (int)db->al;
+            ba.go = db->go;
+            ba.ge = db->ge;
+            ba.spill = s->d_sys_spill;
+            ba.spill_stride = 0;
+            ba.winner_key = s->d_key;
+            ba.result = s->d_result;
+            if (hy::launch_band32((int)db->mode, hy::kBandEnds, ba, 1, m, s->stream) != hipSuccess)
+                return HY_E_DEVICE;
+        } else {
+            hy::SysArgs sa = sys_args(db, s, m);
+            sa.winner_key = s->d_key;
+            sa.result = s->d_result;
+            if (hy::launch_sys32((int)db->mode, true, true, sa, 1, s->stream) != hipSuccess)
+                return HY_E_DEVICE;
+        }
     } else {
         if (hy::launch_key_result(s->d_key, s->d_result, s->stream) != hipSuccess)
             return HY_E_DEVICE;

@@ -298,6 +298,13 @@ struct BandArgs {
     int64_t* out_tend;
     int32_t* colmax;          // kBandColmax: n entries of the one pair
     int2* best;               // kBandColmax: (score, target end or -1)
+    // Winner mode (a database scan's single best, one block): the target is the database entry
+    // the key names (targets/toff are the database's residues and offsets), the query is
+    // `queries` with wm rows, and the result (ends, with a self-check of the key's score) goes to
+    // *result.
+    const unsigned long long* winner_key;
+    Result* result;
+    int wm;
 };
 // Alphabets the band kernel takes (its substitution matrix and rings fit 64 KB of LDS).
 constexpr uint32_t kBandMaxAlphabet = 63;

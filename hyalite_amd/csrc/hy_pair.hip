@@ -259,11 +259,13 @@ __global__ __launch_bounds__(64 * W) void k_band32(BandArgs a) {
     __shared__ int red[3][W];
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const uint32_t pid = a.list[blockIdx.x];
-    const uint8_t* query = a.queries + a.qoff[pid];
+    const bool winner = a.winner_key != nullptr;
+    const unsigned long long wk = winner ? *a.winner_key : 0ull;
+    const uint32_t pid = winner ? (wk != 0ull ? key_index(wk) : 0u) : a.list[blockIdx.x];
+    const uint8_t* query = winner ? a.queries : a.queries + a.qoff[pid];
     const uint8_t* t = a.targets + a.toff[pid];
     const int n = (int)(a.toff[pid + 1] - a.toff[pid]);
-    const int m = (int)a.qlen[pid], go = a.go, ge = a.ge, al = a.al;
+    const int m = winner ? a.wm : (int)a.qlen[pid], go = a.go, ge = a.ge, al = a.al;
     // The matrix plus go, row al the padding row for query rows past m.
     for (int k = threadIdx.x; k < (al + 1) * al; k += 64 * W)
         smat[k] = k < al * al ? a.matrix[k] + go : kPad32;
@@ -438,6 +440,16 @@ __global__ __launch_bounds__(64 * W) void k_band32(BandArgs a) {
         for (int k = 1; k < W; ++k) {
             if (ENDS) consider(bs, bj, bi, red[0][k], red[1][k], red[2][k]);
             else bs = max(bs, red[0][k]);
+        }
+        if (winner) {
+            Result* r = a.result;
+            r->score = bs;
+            r->db_index = pid;
+            r->query_end = ENDS ? (bi == 0 ? -1 : (int64_t)bi - 1) : -1;
+            r->target_end = ENDS ? (bj == 0 ? -1 : (int64_t)bj - 1) : -1;
+            r->check = (bs == key_score(wk) && wk != 0ull) ? 0 : 1;
+            r->pad = 0;
+            return;
         }
         a.out_score[pid] = bs;
         if (ENDS) {

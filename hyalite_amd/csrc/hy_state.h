@@ -86,6 +86,7 @@ struct hy_scratch {
     uint64_t spill_entries = 0;
     int2* d_sys_spill = nullptr;
     uint64_t sys_spill_stride = 0;
+    uint64_t sys_spill_entries = 0;
     int32_t* d_scores = nullptr;
     int64_t* d_qend = nullptr;
     int64_t* d_tend = nullptr;
