@@ -78,6 +78,16 @@ __device__ __forceinline__ s2v p_minu(s2v a, s2v b) {
     return __builtin_bit_cast(s2v, __builtin_elementwise_min(__builtin_bit_cast(u2v, a),
                                                              __builtin_bit_cast(u2v, b)));
 }
+// The same as one v_pk_min_u16 the compiler cannot see through. Written as the vector builtin,
+// min(a -sat b, 1) is recognised as the comparison a > b and lowered to a compare and a select
+// per half (with wait states between them). Which form is faster depends on the kernel's register
+// pressure: measured, this one in the two-pairs-per-lane kernel (protein ScoreEnd 1,707 -> 1,793
+// GCUPS), the builtin in the one-query scan's end tracking (its spills grow with this one).
+__device__ __forceinline__ s2v p_minu_pk(s2v a, s2v b) {
+    uint32_t d;
+    asm("v_pk_min_u16 %0, %1, %2" : "=v"(d) : "v"(as_u32(a)), "v"(as_u32(b)));
+    return as_s2(d);
+}
 __device__ __forceinline__ s2v p_maxu(s2v a, s2v b) {
     return __builtin_bit_cast(s2v, __builtin_elementwise_max(__builtin_bit_cast(u2v, a),
                                                              __builtin_bit_cast(u2v, b)));

@@ -562,7 +562,7 @@ __global__ __launch_bounds__(WIDE ? 256 : 64) void k_pairs16(Pair16Args a) {
             E[r] = p_max(hg, p_subu(E[r], ge2));
             f = p_max(hg, p_subu(f, ge2));
             if constexpr (ENDS) {
-                const s2v up = p_minu(p_subu(h, cm), one2);
+                const s2v up = p_minu_pk(p_subu(h, cm), one2);
                 cm = p_max(cm, h);
                 cr = p_maxu(cr, p_madu(up, splat2(r), zero2));
             } else {
@@ -570,7 +570,7 @@ __global__ __launch_bounds__(WIDE ? 256 : 64) void k_pairs16(Pair16Args a) {
             }
         };
         auto close_column = [&](int j, s2v cm, s2v cr) HY_INLINE {
-            const s2v up = p_minu(p_subu(cm, tb), one2);
+            const s2v up = p_minu_pk(p_subu(cm, tb), one2);
             tb = p_max(tb, cm);
             tc = p_maxu(tc, p_madu(up, splat2(j), zero2));
             tr = p_madu(up, p_subw(cr, tr), tr);
