@@ -82,7 +82,8 @@ __device__ __forceinline__ s2v p_minu(s2v a, s2v b) {
 // min(a -sat b, 1) is recognised as the comparison a > b and lowered to a compare and a select
 // per half (with wait states between them). Which form is faster depends on the kernel's register
 // pressure: measured, this one in the two-pairs-per-lane kernel (protein ScoreEnd 1,707 -> 1,793
-// GCUPS), the builtin in the one-query scan's end tracking (its spills grow with this one).
+// GCUPS) and in the one-query scan's end tracking at 24-row tiles; at 32-row tiles that kernel
+// spills more with it than with the builtin.
 __device__ __forceinline__ s2v p_minu_pk(s2v a, s2v b) {
     uint32_t d;
     asm("v_pk_min_u16 %0, %1, %2" : "=v"(d) : "v"(as_u32(a)), "v"(as_u32(b)));
