@@ -281,9 +281,13 @@ __global__ __launch_bounds__(64 * W) void k_band32(BandArgs a) {
     constexpr int P = 64 * R * W;
     const int npass = n > 0 ? (m + P - 1) / P : 0;
     int2* sp = a.spill + (size_t)blockIdx.x * a.spill_stride;
-    const int steps = n + 63;
     for (int p = 0; p < npass; ++p) {
         const bool first = p == 0;
+        // Lane l takes column n at step n + l - 1. A lone wavefront's last pass ends when its
+        // last lane holding query rows has (no other wavefront shares its barriers, and no later
+        // pass needs lane 63's bottom row); otherwise every lane runs to the end.
+        int steps = n + 63;
+        if (W == 1 && p + 1 == npass) steps = n + (min(m - p * P, P) + R - 1) / R - 1;
         const int r0 = p * P + (w * 64 + lane) * R + 1;
         int qb[R], HG[R], E[R], sub[R], rb[R], rj[R];
 #pragma unroll
