@@ -543,3 +543,24 @@ def test_server_contract_errors():
         assert _tuple(got.hit) == want[0]
         with pytest.raises(KeyError):
             srv.wait(t)  # already collected
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_winner_ends_across_band_passes(mode, monkeypatch):
+    """The one-query scan's winner ends come from the band kernel in winner mode. Forced to its
+    smallest shape (64 rows per pass), a 300-row query takes five passes with boundary rows in
+    the scratch (max_query_len 600 gives it a target's worth); with max_query_len 300 the scratch
+    has none and the winner pass falls back to the one-wavefront kernel. Hits equal the oracle."""
+    monkeypatch.setenv("HYALITE_HIP_BAND_SHAPE", "1,1")
+    rng = np.random.default_rng(1500 + mode)
+    q = rng.integers(0, 20, size=300, dtype=np.uint8)
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(1, 700, 400)]
+    for k in range(0, 400, 37):  # pieces of the query, so winners sit on long alignments
+        t = seqs[k]
+        if t.size > 120:
+            a = int(rng.integers(0, 180))
+            t[10:110] = q[a:a + 100]
+    queries = [q, q[:200], q[150:], q[:64], q[:65]]
+    for mql in (600, 300):
+        check(seqs, (20, BLOSUM62, 11, 1), mode, hy.SearchType.ScoreEnd, queries, mql=mql,
+              all_targets=False)
