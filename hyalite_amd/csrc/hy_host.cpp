@@ -359,16 +359,27 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     {
         std::vector<uint64_t> off(n + 1, 0);
         for (size_t k = 0; k < n; ++k) off[k + 1] = off[k] + lens[k];
-        // Every byte is written below, so the buffer is left uninitialised.
-        std::unique_ptr<uint8_t[]> flat(new (std::nothrow) uint8_t[std::max<uint64_t>(total, 1)]);
-        if (!flat) return fail(HY_E_OUT_OF_MEMORY);
-        for (size_t k = 0; k < n; ++k)
-            if (lens[k]) std::memcpy(flat.get() + off[k], seqs[k], lens[k]);
+        // Sequences that already lie back to back in one buffer (a database image, a flat
+        // batch from the caller) upload from there; others are gathered into one first.
+        bool contiguous = total > 0;
+        for (size_t k = 0; contiguous && k + 1 < n; ++k)
+            contiguous = reinterpret_cast<uintptr_t>(seqs[k]) + lens[k] ==
+                         reinterpret_cast<uintptr_t>(seqs[k + 1]);
+        std::unique_ptr<uint8_t[]> flat;
+        const uint8_t* host = contiguous ? seqs[0] : nullptr;
+        if (!contiguous) {
+            // Every byte is written below, so the buffer is left uninitialised.
+            flat.reset(new (std::nothrow) uint8_t[std::max<uint64_t>(total, 1)]);
+            if (!flat) return fail(HY_E_OUT_OF_MEMORY);
+            for (size_t k = 0; k < n; ++k)
+                if (lens[k]) std::memcpy(flat.get() + off[k], seqs[k], lens[k]);
+            host = flat.get();
+        }
         if (dev_upload(&db->d_off, off) != hipSuccess) return fail(HY_E_DEVICE);
         if (hipMalloc(reinterpret_cast<void**>(&db->d_flat), std::max<uint64_t>(total, 1)) !=
                 hipSuccess ||
-            hipMemcpy(db->d_flat, flat.get(), std::max<uint64_t>(total, 1),
-                      hipMemcpyHostToDevice) != hipSuccess)
+            hipMemcpy(db->d_flat, host, std::max<uint64_t>(total, 1), hipMemcpyHostToDevice) !=
+                hipSuccess)
             return fail(HY_E_DEVICE);
         db->h_off = std::move(off);
     }
