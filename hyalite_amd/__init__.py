@@ -670,9 +670,25 @@ class DatabaseBuilder:
         return self
 
     def sequences_flat(self, flat: np.ndarray, offsets: np.ndarray) -> "DatabaseBuilder":
-        """Sequences given as one residue buffer plus n+1 offsets (no per-sequence objects)."""
-        self._flat = (np.ascontiguousarray(flat, dtype=np.uint8),
-                      np.ascontiguousarray(offsets, dtype=np.uint64))
+        """Sequences given as one residue buffer plus n+1 offsets (no per-sequence objects).
+        Sequence k is flat[offsets[k]:offsets[k+1]]: the offsets start at 0, never decrease and
+        end within the buffer (ValueError otherwise, before anything is built)."""
+        buf = np.asarray(flat)
+        off = np.asarray(offsets)
+        if buf.ndim != 1 or off.ndim != 1:
+            raise ValueError("sequences_flat takes a 1-D residue buffer and 1-D offsets")
+        if off.size == 0:
+            raise ValueError("sequences_flat needs n + 1 offsets (at least one)")
+        if off.dtype.kind not in "ui" or (off.dtype.kind == "i" and (off < 0).any()):
+            raise ValueError("offsets must be non-negative integers")
+        off = off.astype(np.uint64)
+        if int(off[0]) != 0:
+            raise ValueError("offsets must start at 0")
+        if off.size > 1 and (off[1:] < off[:-1]).any():
+            raise ValueError("offsets must not decrease")
+        if int(off[-1]) > buf.size:
+            raise ValueError(f"offsets end at {int(off[-1])}, past the buffer's {buf.size} bytes")
+        self._flat = (np.ascontiguousarray(buf, dtype=np.uint8), np.ascontiguousarray(off))
         self._seqs = []
         return self
 

@@ -121,8 +121,12 @@ extern "C" int hy_db_load(const char* path, uint32_t backend, hy_db** out, int64
     if (std::fread(&h, 1, sizeof(h), in.f) != sizeof(h)) return HY_E_BAD_IMAGE;
     if (std::memcmp(h.magic, kMagic, 8) != 0 || h.version != kVersion) return HY_E_BAD_IMAGE;
     // Every section's size follows from the header; the file must hold exactly them.
+    // Any alphabet the build accepts can be saved (Scoring::new takes alphabets above 256, whose
+    // larger symbols simply never occur in a byte sequence), so the load takes it too; the matrix
+    // section is bounded by the file size before the exact-size check.
     const uint64_t mat_n = (uint64_t)h.al * h.al;
-    if (h.al == 0 || h.al > 256 || h.n == 0 || h.n > 0xFFFFFFF0ull) return HY_E_BAD_IMAGE;
+    if (h.al == 0 || mat_n > (uint64_t)size / 4 || h.n == 0 || h.n > 0xFFFFFFF0ull)
+        return HY_E_BAD_IMAGE;
     const uint64_t expect = sizeof(Header) + mat_n * 4 + (h.n + 1) * 8 + h.total + 8;
     if (h.total > (uint64_t)size || (uint64_t)size != expect) return HY_E_BAD_IMAGE;
     std::vector<int32_t> mat(mat_n);

@@ -74,3 +74,28 @@ def test_loaded_alignment_database_traces_back(tmp_path):
     assert a.db_index == b.db_index and a.alignment == b.alignment
     t = seqs[a.db_index]
     assert a.alignment.as_tuple() == orc.traceback(q, t, *dna_scheme(), 0)
+
+
+def test_alphabet_above_256_round_trips(tmp_path):
+    """Scoring::new takes alphabets above 256 (symbols past 255 never occur in a byte
+    sequence); such a database saves, loads back and scans like the original."""
+    al = 300
+    mat = [-2] * (al * al)
+    for i in range(al):
+        mat[i * al + i] = 3 + (i % 4)
+    rng = np.random.default_rng(303)
+    seqs = [rng.integers(0, 256, size=int(L), dtype=np.uint8) for L in rng.integers(0, 90, 200)]
+    db = build(seqs, (al, mat, 5, 2), 0, hy.SearchType.ScoreEnd, 60)
+    p = tmp_path / "big.img"
+    db.save(str(p))
+    ld = hy.Database.load(str(p), hy.BackendChoice.force(hy.Backend.Hip))
+    assert ld.scoring().alphabet_len() == al
+    assert np.array_equal(ld.scoring().matrix(), db.scoring().matrix())
+    flat, off = orc.flatten(seqs)
+    s1, s2 = hy.Scratch(db), hy.Scratch(ld)
+    for qlen in (0, 7, 60):
+        q = rng.integers(0, 256, size=qlen, dtype=np.uint8)
+        want = orc.scan_flat(flat, off, q, al, mat, 5, 2, 0, True, threads=4)
+        for d, s in ((db, s1), (ld, s2)):
+            h = d.scan(s, q)
+            assert (h.score, h.db_index, h.query_end, h.target_end) == tuple(want)

@@ -171,3 +171,25 @@ def test_strerror_covers_every_code():
     lib = hy.load_library()
     for code in list(range(0, 12)) + list(range(100, 109)):
         assert lib.hy_strerror(code) not in (None, b"unknown status code")
+
+
+@pytest.mark.parametrize("flat, offsets, msg", [
+    (np.zeros(8, np.uint8), np.array([1, 4, 8]), "start at 0"),
+    (np.zeros(8, np.uint8), np.array([0, 5, 3, 8]), "not decrease"),
+    (np.zeros(8, np.uint8), np.array([0, 4, 9]), "past the buffer"),
+    (np.zeros(8, np.uint8), np.array([], dtype=np.uint64), "at least one"),
+    (np.zeros(8, np.uint8), np.array([0, -1, 8]), "non-negative"),
+    (np.zeros((2, 4), np.uint8), np.array([0, 4, 8]), "1-D"),
+    (np.zeros(8, np.uint8), np.array([[0, 4, 8]]), "1-D"),
+])
+def test_sequences_flat_rejects_bad_offsets(flat, offsets, msg):
+    """Offsets that would make the build read outside the caller's buffer (or wrap a length to
+    nearly 2^64) are refused before anything reaches the library."""
+    with pytest.raises(ValueError, match=msg):
+        hy.Database.builder().sequences_flat(flat, offsets)
+
+
+def test_sequences_flat_accepts_good_offsets():
+    b = hy.Database.builder().sequences_flat(np.arange(8, dtype=np.uint8) % 4,
+                                             np.array([0, 0, 3, 8], dtype=np.int64))
+    assert b._flat[1].dtype == np.uint64 and list(b._flat[1]) == [0, 0, 3, 8]
