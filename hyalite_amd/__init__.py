@@ -38,6 +38,7 @@ __all__ = [
     "ServerTimeout", "AlignOp", "Alignment", "AlignedHit", "TracebackBudgetExceeded",
     "PairScratch", "align_pair", "align_pair_with", "align_pairs", "align_pair_position_max",
     "align_pair_position_max_with", "score2", "align_pairs_flat", "DatabaseImageError",
+    "StaticScratchLimit",
 ]
 
 BACKEND_ENV_VAR = "HYALITE_BACKEND"
@@ -49,7 +50,7 @@ _LIB_NAME = "libhyalite_hip.so"
 # and the mode-templated kernels compiled once per alignment mode (SW, NW, HW, OV, SHW).
 _UNITS = [("csrc/hy_host.cpp", []), ("csrc/hy_tune.cpp", []), ("csrc/hy_serve.cpp", []),
           ("csrc/hy_align.cpp", []), ("csrc/hy_pair.cpp", []), ("csrc/hy_image.cpp", []),
-          ("csrc/hy_xfer.cpp", []), ("csrc/hy_launch.hip", [])] + [
+          ("csrc/hy_xfer.cpp", []), ("csrc/hy_launch.hip", []), ("csrc/hy_static.hip", [])] + [
     (src, [f"-DHY_KMODE={k}"])
     for src in ("csrc/hy_kernels.hip", "csrc/hy_trace.hip", "csrc/hy_pair.hip")
     for k in range(5)]
@@ -181,6 +182,11 @@ class DatabaseImageError(ValueError):
 
 class DeviceError(RuntimeError):
     """A HIP runtime failure or a failed device-side self-check."""
+
+
+class StaticScratchLimit(RuntimeError):
+    """Database.scan_scores_static: the problem needs more than the library's static device
+    array holds (about 4 bytes per database residue plus 4 per target, 64 MiB in all)."""
 
 
 class HipLibraryMissing(RuntimeError):
@@ -532,6 +538,8 @@ def load_library():
     lib.hy_scan_all.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _i32p, _i64p, _i64p]
     lib.hy_scan_scores.restype = ctypes.c_int
     lib.hy_scan_scores.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _i32p]
+    lib.hy_scan_scores_static.restype = ctypes.c_int
+    lib.hy_scan_scores_static.argtypes = [vp, _u8p, ctypes.c_size_t, _i32p]
     lib.hy_scan_aligned.restype = ctypes.c_int
     lib.hy_scan_aligned.argtypes = [vp, vp, _u8p, ctypes.c_size_t, _u64p,
                                     ctypes.POINTER(_HyAlignment), _u8p, ctypes.c_size_t]
@@ -645,6 +653,8 @@ def _check_scan(rc: int):
     msg = load_library().hy_strerror(rc).decode()
     if rc in (100, 101, 108):
         raise ScanContractViolation(msg)
+    if rc == 110:
+        raise StaticScratchLimit(msg)
     raise DeviceError(f"{msg} (status {rc})")
 
 
@@ -913,6 +923,17 @@ class Database:
         scores = np.empty(self._n, dtype=np.int32)
         _check_scan(load_library().hy_scan_scores(self._h, scratch._h, qa.ctypes.data_as(_u8p),
                                                   q.size, scores.ctypes.data_as(_i32p)))
+        return scores
+
+    def scan_scores_static(self, query) -> np.ndarray:
+        """scan_scores without a Scratch and without any device allocation (hy_scan_scores_static:
+        a static device array per GPU, the null stream); StaticScratchLimit when the database and
+        query need more than it holds. Calls on one GPU must not overlap."""
+        q = np.asarray(query, dtype=np.uint8)
+        qa = _as_u8(q)
+        scores = np.empty(self._n, dtype=np.int32)
+        _check_scan(load_library().hy_scan_scores_static(
+            self._h, qa.ctypes.data_as(_u8p), q.size, scores.ctypes.data_as(_i32p)))
         return scores
 
     def scan_scores(self, scratch: "Scratch", query, out: Optional[list] = None) -> list:

@@ -1329,6 +1329,8 @@ extern "C" const char* hy_strerror(int code) {
     case HY_E_INTERNAL: return "device self-check failed";
     case HY_E_TIMEOUT: return "timed out waiting for a served query";
     case HY_E_BAD_IMAGE: return "not a database image, or a damaged one";
+    case HY_E_STATIC_LIMIT:
+        return "the problem needs more device memory than the static scratch array holds";
     case HY_E_NOT_ALIGNMENT:
         return "scan_aligned/scan_all_aligned require a database built with SearchType::Alignment";
     default: return "unknown status code";

@@ -87,7 +87,9 @@ enum {
     HY_E_INTERNAL = 106,             /* a device-side self-check failed */
     HY_E_TIMEOUT = 107,              /* hy_server_wait: not completed within the timeout */
     HY_E_NOT_ALIGNMENT = 108,        /* an aligned scan on a database built without ALIGNMENT */
-    HY_E_BAD_IMAGE = 109             /* hy_db_load: not a database image, or a damaged one */
+    HY_E_BAD_IMAGE = 109,            /* hy_db_load: not a database image, or a damaged one */
+    HY_E_STATIC_LIMIT = 110          /* hy_scan_scores_static: the problem needs more than its
+                                        static device array holds */
 };
 
 /* Field flags for hy_db_build's `have` mask: which builder fields were set. A field left out
@@ -163,6 +165,16 @@ int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qle
 /* Per-target scores in database order (Database::scan_scores; SearchType-independent). */
 int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
                    int32_t* scores);
+
+/* hy_scan_scores without a scratch and without allocating: no counterpart in the reference. For
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same inputs, outputs and results as hy_scan_scores; the
+ * working set (query copy, tile-boundary rows, systolic boundary rows of queries beyond 512
+ * rows, the score array: about 4 bytes per database residue plus 4 per target) lives in one
+ * static 64 MiB device array per GPU, held until the process ends, and the kernels run on the
+ * null stream. A problem needing more returns HY_E_STATIC_LIMIT. Calls that overlap on one GPU
+ * share that array and race: serialise them. */
+int hy_scan_scores_static(const hy_db* db, const uint8_t* query, size_t qlen, int32_t* scores);
 
 /* Alignment (src/align.rs:60-95): score, half-open spans [start, end) of query and target, and
  * n_ops op bytes (HY_OP_*, forward order) at ops + ops_offset of the buffer given to the scan. */
