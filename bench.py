@@ -58,13 +58,43 @@ METRIC = "GCUPS on 1 MI355X (batched vs plain kernel) + p50/p99 query latency at
 VALU_PEAK_TOPS_I32 = 256 * 64 * 2.4e9 / 1e12
 VALU_PEAK_TOPS_PK16 = 2 * VALU_PEAK_TOPS_I32
 
-# HBM bytes per launch of the dominant kernel, from rocprofv3 --pmc FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0 (the factors a 256 MiB device copy gives on this GPU); the commands and
-# counter values are in profiles/.
-MEASURED_TRAFFIC_BYTES = {
-    ("protein_sw_scoreend_batch256x1000000", "k_inter16_batch<SW,32>"): 1.355e13,
-    ("protein_sw_scoreend_batch256x1000000", "k_inter16q_batch<32>"): 1.390e13,
-}
+# HBM bytes per launch of the dominant kernel come from rocprofv3 --pmc runs (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, the factors a 256 MiB device copy gives on this GPU). Each measurement is
+# recorded in its profile summary as a line "traffic record: {json}" naming the workload, the
+# kernel, the bytes and the SHA-256 of the kernel sources it was measured on; a record whose
+# sources have changed since is reported as stale instead of used.
+TRAFFIC_SOURCES = ("hyalite_amd/csrc/hy_kernels.hip", "hyalite_amd/csrc/hy_device.h",
+                   "hyalite_amd/csrc/hy_internal.h")
+
+
+def kernel_sources_sha() -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for rel in TRAFFIC_SOURCES:
+        with open(os.path.join(ROOT, rel), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def traffic_record(workload: str, kernel: str) -> tuple:
+    """(bytes or None, note) from the traffic records in profiles/*.md."""
+    import glob
+    sha = kernel_sources_sha()
+    found = None
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic record: "):
+                    rec = json.loads(line[len("traffic record: "):].strip().strip("`"))
+                    if rec.get("workload") == workload and rec.get("kernel") == kernel:
+                        found = (rec, os.path.basename(path))
+    if found is None:
+        return None, "no traffic record for this kernel and workload"
+    rec, where = found
+    if rec.get("sources_sha256") != sha:
+        return None, (f"stale: {where} measured kernel sources {rec.get('sources_sha256')}, "
+                      f"these are {sha}")
+    return float(rec["bytes"]), f"{where}, kernel sources {sha}"
 
 SERVING_QPS = 10.0
 
@@ -95,6 +125,24 @@ def host_threads() -> int:
         return max(1, os.cpu_count() or 1)
 
 
+def physical_cores() -> tuple:
+    """(physical cores, logical CPUs) of this process's CPU set, from the sysfs topology (a
+    physical core = a distinct (package, core id) pair); (logical, logical) when it is absent."""
+    try:
+        cpus = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cpus = list(range(os.cpu_count() or 1))
+    cores = set()
+    for c in cpus:
+        base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+        try:
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            return len(cpus), len(cpus)
+    return max(1, len(cores)), len(cpus)
+
+
 def spread(queries, k):
     """k queries spread evenly over the length-sorted list."""
     order = sorted(range(len(queries)), key=lambda i: len(queries[i]))
@@ -117,7 +165,7 @@ def cpu_baseline(w: wl.Workload, queries, budget_s: float = 12.0, max_targets: i
     host threads; plus a one-thread and a length-sorted run. Every timed result is checked
     against the scalar oracle on the same sample first."""
     import oracle
-    threads = host_threads()
+    threads, logical = physical_cores()
     qs = spread(queries, 6)
     qres = sum(len(q) for q in qs)
 
@@ -158,8 +206,10 @@ def cpu_baseline(w: wl.Workload, queries, budget_s: float = 12.0, max_targets: i
         "sample": f"{len(qs)} batch queries spread over lengths {len(qs[0])}..{len(qs[-1])} "
                   f"against {sub.n} random targets ({sub.residues} residues) of the same "
                   f"database; AVX2 16 x i16 restatement of the reference avx2 backend, "
-                  f"database-order batches, {threads} threads on {cpu_model()}, scan "
-                  f"{secs:.2f} s; checked against the scalar oracle",
+                  f"database-order batches, one thread per physical core ({threads} cores "
+                  f"of {logical} logical CPUs in this process's CPU set) on {cpu_model()}, "
+                  f"scan {secs:.2f} s; checked against the scalar oracle",
+        "logical_cpus": logical,
         "one_thread_gcups": round(qres * one.residues / one_secs / 1e9, 3),
         "length_sorted_gcups": round(cells / sorted_secs / 1e9, 3),
     }
@@ -270,8 +320,9 @@ def other_configs(sync) -> dict:
     del sc, db
     base = wl.c3_dna_modes()
     for mode, name in ((1, "NW"), (2, "HW"), (4, "SHW"), (3, "OV")):
+        # Built for ScoreEnd: scan_scores ignores the search type, scan_all then tracks ends.
         w = wl.Workload(base.name.replace("mode1", name), base.al, base.matrix, base.go, base.ge,
-                        mode, 0, base.mql, base.flat, base.off, base.queries)
+                        mode, 1, base.mql, base.flat, base.off, base.queries)
         db = build_db(w)
         sc = hy.Scratch(db)
         q = w.queries[0]
@@ -283,12 +334,25 @@ def other_configs(sync) -> dict:
             scores = db.scan_scores_array(sc, q)
         sync()
         dt = (time.perf_counter() - t0) / 5
+        # Per-target hits with their ends (scan_all, ScoreEnd): 20 bytes per target come back.
+        db.scan_all_arrays(sc, q)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            s_all, _, _ = db.scan_all_arrays(sc, q)
+        sync()
+        dta = (time.perf_counter() - t0) / 5
+        if not np.array_equal(s_all, scores):
+            raise RuntimeError(f"{w.name}: scan_all scores differ from scan_scores")
         out[w.name] = {"op": "scan_scores", "ms": round(dt * 1e3, 3),
                        "gcups": round(len(q) * w.residues / dt / 1e9, 1),
+                       "scan_all_ends_ms": round(dta * 1e3, 3),
+                       "scan_all_ends_gcups": round(len(q) * w.residues / dta / 1e9, 1),
                        "partitions": {str(k): v for k, v in db.partition_sizes().items()},
                        "best_score": int(scores.max()),
                        "targets_at_best": int((scores == scores.max()).sum())}
-        log(f"{w.name}: {out[w.name]['gcups']} GCUPS")
+        log(f"{w.name}: {out[w.name]['gcups']} GCUPS scan_scores, "
+            f"{out[w.name]['scan_all_ends_gcups']} GCUPS scan_all with ends")
         del sc, db
     return out
 
@@ -601,6 +665,8 @@ def main() -> int:
     ap.add_argument("--serving-queries", type=int, default=64)
     ap.add_argument("--serving-lanes", type=int, default=3)
     ap.add_argument("--serving-batch", type=int, default=16)
+    ap.add_argument("--only-configs", action="store_true",
+                    help="only run the DNA configurations and print their JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB and exit")
     args = ap.parse_args()
@@ -619,6 +685,10 @@ def main() -> int:
     def sync():
         if _hip.hipDeviceSynchronize() != 0:
             raise RuntimeError("hipDeviceSynchronize failed")
+
+    if args.only_configs:
+        print(json.dumps(other_configs(sync)), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     w = wl.c4_batch(n_targets=args.targets, nq=args.queries)
@@ -716,13 +786,15 @@ def main() -> int:
         # One launch per step; every target is in the packed partition here.
         k_cells = cells * args.steps / launches
         achieved = k_cells * ops_per_cell(w.mode) / (k_ms / 1e3) / 1e12
+        traffic, traffic_note = traffic_record(w.name, kernel)
         roof = {
             "bound": "valu",
             "achieved": round(achieved, 3),
             "peak": round(VALU_PEAK_TOPS_PK16, 2),
             "unit": "TOP/s",
             "frac": round(achieved / VALU_PEAK_TOPS_PK16, 4),
-            "traffic": MEASURED_TRAFFIC_BYTES.get((w.name, kernel)),
+            "traffic": traffic,
+            "traffic_source": traffic_note,
             "kernel": kernel,
             "peak_note": "packed 16-bit VALU element-ops (2 per lane per instruction)",
             "kernel_ms": round(k_ms, 3),

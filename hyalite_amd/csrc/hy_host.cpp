@@ -715,13 +715,14 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     ia.out_tend = s->d_tend;
     ia.key = ends ? nullptr : key;
 
-    // Score-only scans run the targets the packed kernel accepts there and the rest at 32 bits.
-    // End tracking does the same in local mode (the packed kernel's end-tracking form keeps
-    // columns in 16 bits, so groups must stay within 65,535 columns); the other modes track ends
-    // for every non-long target at 32 bits.
+    // Scans run the targets the packed kernel accepts there and the rest at 32 bits, with or
+    // without end tracking. The local-mode end tracking keeps a tile's best column in 16 bits, so
+    // its groups must stay within 65,535 columns (else every non-long target runs at 32 bits);
+    // the other modes keep their ends in 32 bits.
     const bool has16 = db->groups16.size() > db->groups16_short_first;
-    const bool packed_ends = ends && db->mode == HY_MODE_SW && has16 &&
-                             db->groups16[db->groups16_short_first].ncols <= 65535u;
+    const bool packed_ends =
+        ends && has16 &&
+        (db->mode != HY_MODE_SW || db->groups16[db->groups16_short_first].ncols <= 65535u);
     const bool run16 = (!ends || packed_ends) && has16;
     const uint32_t g32 =
         (ends && !packed_ends) ? (uint32_t)db->groups.size() : db->groups_score32;
@@ -787,7 +788,8 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
             pa.key = nullptr;
             pa.out_qend = s->d_qend;
             pa.out_tend = s->d_tend;
-            if (hy::launch_inter16_ends_sw(pa, s->stream) != hipSuccess) return HY_E_DEVICE;
+            if (hy::launch_inter16_ends((int)db->mode, pa, s->stream) != hipSuccess)
+                return HY_E_DEVICE;
         } else if (hy::launch_inter16((int)db->mode, pa, s->q16, s->waves16, s->stream) !=
                    hipSuccess) {
             return HY_E_DEVICE;

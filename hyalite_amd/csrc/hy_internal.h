@@ -476,8 +476,10 @@ hipError_t launch_pairs32_m(bool ends, const PairArgs& a, uint32_t ngroups, int 
 hipError_t launch_posmax_sw(const PosMaxArgs& a, hipStream_t st);
 template <int MODE>
 hipError_t launch_band32_m(int out, const BandArgs& a, uint32_t nblocks, int m, hipStream_t st);
-// The packed one-query kernel with end tracking (local mode; hy_kernels.hip compiled for SW):
-// score and ends of every target of the launch's groups into out_score / out_qend / out_tend.
-hipError_t launch_inter16_ends_sw(const Inter16Args& a, hipStream_t st);
+// The packed one-query kernel with end tracking: score and ends of every target of the launch's
+// groups into out_score / out_qend / out_tend.
+template <int MODE>
+hipError_t launch_inter16_ends_m(const Inter16Args& a, hipStream_t st);
+hipError_t launch_inter16_ends(int mode, const Inter16Args& a, hipStream_t stream);
 
 }  // namespace hy

@@ -223,6 +223,10 @@ __device__ __forceinline__ int border16(int i, int j, int go, int ge) {
 #ifndef HY_PACKED_WAVES_PER_EU
 #define HY_PACKED_WAVES_PER_EU 2
 #endif
+// Query rows per tile of the packed kernel's end-tracking form in the non-local modes.
+#ifndef HY_ENDS_TILE_NONLOCAL
+#define HY_ENDS_TILE_NONLOCAL 32
+#endif
 // Running state of one wavefront's two-targets-per-lane pass over one group, carried across the
 // query tiles.
 struct PairRun {
@@ -244,7 +248,6 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                                              int nA, int nB, uint2* sp, const uint2* wp,
                                              uint32_t* pp, PairRun& st) {
     using F = MF<MODE>;
-    static_assert(!ENDS || F::local, "packed end tracking is the local-mode form");
     constexpr int S = QT + 4;
     // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
     // different symbols start in different 4-bank groups.
@@ -448,16 +451,33 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                     if (lastc && r == rm) v = hgup;
                 }
                 outb[b] = make_uint2(as_u32(HG[QT - 1]), as_u32(f));
+                // End tracking: each answer cell of a target joins its best with the full
+                // (score, column, row) tie-break, in 32 bits per target; the border cells are
+                // already there (border_best), and one column can offer row m twice (last row and
+                // last column), which the tie-break absorbs.
                 if (lastc) {
                     const bool la = F::lrow ? j <= nA : j == nA;
                     const bool lb = F::lrow ? j <= nB : j == nB;
-                    run = p_max(run, keep2(p_add(v, go2), la, lb));
+                    if constexpr (ENDS) {
+                        const s2v h = p_add(v, go2);
+                        if (la) consider(st.bsA, st.bjA, st.biA, (int)h.x, j, m);
+                        if (lb) consider(st.bsB, st.bjB, st.biB, (int)h.y, j, m);
+                    } else {
+                        run = p_max(run, keep2(p_add(v, go2), la, lb));
+                    }
                 }
                 if (F::lcol && (j == nA || j == nB)) {
 #pragma unroll
                     for (int r = 0; r < QT; ++r)
-                        if (i0 + r <= m)
-                            run = p_max(run, keep2(p_add(HG[r], go2), j == nA, j == nB));
+                        if (i0 + r <= m) {
+                            if constexpr (ENDS) {
+                                const s2v h = p_add(HG[r], go2);
+                                if (j == nA) consider(st.bsA, st.bjA, st.biA, (int)h.x, j, i0 + r);
+                                if (j == nB) consider(st.bsB, st.bjB, st.biB, (int)h.y, j, i0 + r);
+                            } else {
+                                run = p_max(run, keep2(p_add(HG[r], go2), j == nA, j == nB));
+                            }
+                        }
                 }
             };
             // Columns without answer cells run in skewed pairs (row r of column j with row r-1
@@ -520,7 +540,7 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     st.run = run;
 #pragma unroll
     for (int k = 0; k < 4; ++k) st.runq[k] = runq[k];
-    if constexpr (ENDS) {
+    if constexpr (ENDS && F::local) {
         // The tile's best of each target joins the target's best with the full tie-break: a
         // later tile can hold an equal score in an earlier column.
         const uint32_t b = as_u32(tb), c = as_u32(tc), r = as_u32(tr);
@@ -1147,20 +1167,24 @@ uint32_t inter16_batch_occupancy_m(int q, int al_eff) {
     return (uint32_t)per_cu;
 }
 
-#if HY_KMODE == 0
-// End tracking in the packed one-query kernel: 32-row tiles, one wavefront per workgroup with
-// the per-symbol profile (the one-query default shape).
-hipError_t launch_inter16_ends_sw(const Inter16Args& a, hipStream_t st) {
+// End tracking in the packed one-query kernel (scan_all with ScoreEnd): one wavefront per
+// workgroup with the per-symbol profile (the one-query default shape).
+template <int MODE>
+hipError_t launch_inter16_ends_m(const Inter16Args& a, hipStream_t st) {
     if (a.ngroups == 0) return hipSuccess;
-    // 24-row tiles fit the end tracking without register spills (32-row tiles spill 160 bytes
-    // per lane). Measured, scan_all with ends, 360 aa x 1M protein targets: 30.9 ms against
-    // 31.3 ms at 32 rows and 32.3 ms at 24 rows with the compiler's compare-and-select minimum.
-    constexpr int Q = 24;
+    // Local mode: 24-row tiles fit the end tracking without register spills (32-row tiles spill
+    // 160 bytes per lane). Measured, scan_all with ends, 360 aa x 1M protein targets: 30.9 ms
+    // against 31.3 ms at 32 rows and 32.3 ms at 24 rows with the compiler's compare-and-select
+    // minimum. The other modes track ends at their answer cells only (the last row of the last
+    // tile, the end column), so their tile keeps the score-only shape.
+    constexpr int Q = MODE == kSW ? 24 : HY_ENDS_TILE_NONLOCAL;
     const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
-    hipLaunchKernelGGL((k_inter16<kSW, Q, 1, false, true>), dim3(a.ngroups), dim3(64), lds, st,
+    hipLaunchKernelGGL((k_inter16<MODE, Q, 1, false, true>), dim3(a.ngroups), dim3(64), lds, st,
                        a);
     return hipGetLastError();
 }
+
+#if HY_KMODE == 0
 
 uint32_t inter16q_lds_bytes(int al_eff, int q) {
     return ((uint32_t)al_eff + 1) * ((uint32_t)inter16_tile_rows(q) + 4) * 4u;
@@ -1203,5 +1227,6 @@ template hipError_t launch_inter16_m<HY_KMODE>(const Inter16Args&, int, int, hip
 template hipError_t launch_inter16_batch_m<HY_KMODE>(const Inter16Args&, const BatchArgs&, int,
                                                      uint32_t, hipStream_t);
 template uint32_t inter16_batch_occupancy_m<HY_KMODE>(int, int);
+template hipError_t launch_inter16_ends_m<HY_KMODE>(const Inter16Args&, hipStream_t);
 
 }  // namespace hy

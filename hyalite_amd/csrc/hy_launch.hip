@@ -260,6 +260,18 @@ uint32_t inter16_lds_bytes(int al_eff, int q) {
     return A * A * (uint32_t)(inter16_tile_rows(q) + 4) * 4u;
 }
 
+hipError_t launch_inter16_ends(int mode, const Inter16Args& a, hipStream_t stream) {
+    if (a.ngroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_inter16_ends_m<kSW>(a, stream);
+    case kNW: return launch_inter16_ends_m<kNW>(a, stream);
+    case kHW: return launch_inter16_ends_m<kHW>(a, stream);
+    case kOV: return launch_inter16_ends_m<kOV>(a, stream);
+    case kSHW: return launch_inter16_ends_m<kSHW>(a, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_inter16(int mode, const Inter16Args& a, int q, int waves, hipStream_t stream) {
     if (a.ngroups == 0) return hipSuccess;
     switch (mode) {
