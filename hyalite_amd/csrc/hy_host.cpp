@@ -705,6 +705,30 @@ hy::SysArgs sys_args(const hy_db* db, hy_scratch* s, int m) {
     return a;
 }
 
+// Tile rows of the pipelined packed kernel for a score-only scan of `ngroups` packed groups with
+// a query of m rows, or 0 for the one-wavefront-per-group kernel. One wavefront per group leaves
+// most SIMDs idle when the groups number fewer than the CUs (a 10,000-target database has 79);
+// the pipelined kernel gives each group a workgroup whose wavefronts take one query tile each.
+// It needs 2 to 8 tiles and its LDS within 64 KB. HYALITE_HIP_PIPE=0 turns it off, =1 uses it
+// whenever the query and LDS allow (for tests).
+int pipe_rows(const hy_db* db, int m, uint32_t ngroups) {
+    const char* e = std::getenv("HYALITE_HIP_PIPE");
+    const int mode_env = e ? std::atoi(e) : -1;
+    int q = hy::inter16_pipe_rows(m);
+    // HYALITE_HIP_PIPE_ROWS=16|24|32 asks for that tile height where the query fits its tiles.
+    if (const char* r = std::getenv("HYALITE_HIP_PIPE_ROWS")) {
+        const int want = std::atoi(r);
+        if ((want == 16 || want == 24 || want == 32) && q != 0 && q <= want &&
+            (m + want - 1) / want >= 2)
+            q = want;
+    }
+    if (mode_env == 0 || q == 0 || ngroups == 0) return 0;
+    const int W = (m + q - 1) / q;
+    if (hy::inter16_pipe_lds_bytes((int)db->al_eff, q, W) > 64u * 1024u) return 0;
+    if (mode_env != 1 && ngroups > hy::device_cus()) return 0;
+    return q;
+}
+
 // Enqueue the per-target pass. ends: write (score, qend, tend) per target; otherwise write scores
 // (when `scores` is set) and/or fold the argmax key (when `key` is set).
 int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write_scores,
@@ -789,6 +813,9 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
             pa.out_qend = s->d_qend;
             pa.out_tend = s->d_tend;
             if (hy::launch_inter16_ends((int)db->mode, pa, s->stream) != hipSuccess)
+                return HY_E_DEVICE;
+        } else if (const int pq = pipe_rows(db, m, pa.ngroups)) {
+            if (hy::launch_inter16_pipe((int)db->mode, pa, pq, s->stream) != hipSuccess)
                 return HY_E_DEVICE;
         } else if (hy::launch_inter16((int)db->mode, pa, s->q16, s->waves16, s->stream) !=
                    hipSuccess) {

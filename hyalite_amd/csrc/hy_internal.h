@@ -476,6 +476,28 @@ hipError_t launch_pairs32_m(bool ends, const PairArgs& a, uint32_t ngroups, int 
 hipError_t launch_posmax_sw(const PosMaxArgs& a, hipStream_t st);
 template <int MODE>
 hipError_t launch_band32_m(int out, const BandArgs& a, uint32_t nblocks, int m, hipStream_t st);
+// Pipelined packed one-query kernel (scores and key, no ends): one workgroup per group, one
+// query tile per wavefront, at most kPipeMaxWaves tiles. inter16_pipe_rows gives the tile rows
+// for a query of m rows (0 when it needs more than kPipeMaxWaves tiles or only one).
+constexpr int kPipeMaxWaves = 8;
+inline int inter16_pipe_rows(int m) {
+    if (m <= 16) return 0;
+    if (m <= 16 * kPipeMaxWaves) return 16;
+    if (m <= 24 * kPipeMaxWaves) return 24;
+    if (m <= 32 * kPipeMaxWaves) return 32;
+    return 0;
+}
+// LDS bytes of a pipelined workgroup of W wavefronts with q-row tiles (profiles, double buffers,
+// the fold of the running maxima); the host uses the kernel only within 64 KB.
+inline uint32_t inter16_pipe_lds_bytes(int al_eff, int q, int W) {
+    const uint32_t rows = (uint32_t)q / 2;
+    const uint32_t rd = rows % 8 == 4 ? rows : rows + 4;
+    return (uint32_t)W * ((uint32_t)al_eff + 1) * rd * 4u + (uint32_t)(W - 1) * 4096u +
+           (uint32_t)W * 256u;
+}
+template <int MODE>
+hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, hipStream_t st);
+hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, hipStream_t stream);
 // The packed one-query kernel with end tracking: score and ends of every target of the launch's
 // groups into out_score / out_qend / out_tend.
 template <int MODE>

@@ -239,15 +239,27 @@ struct PairRun {
     int bsA, bjA, biA, bsB, bjB, biB;
 };
 
+// Pipelined tiles (k_inter16_pipe): the wavefronts of a workgroup hold consecutive query tiles
+// of one group, wavefront `wave` running one four-column block behind wavefront wave - 1. A tile's
+// bottom row goes to the next wavefront through an LDS double buffer (block b in half b & 1,
+// entry (half * 4 + column in block) * 64 + lane) instead of HBM, and every wavefront takes the
+// same nsteps workgroup barriers, one per block step, active or not.
+struct Pipe {
+    int wave = 0, nsteps = 0;
+    const uint2* rin = nullptr;  // the previous wavefront's bottom rows (not the first tile)
+    uint2* rout = nullptr;       // this wavefront's bottom rows (not the last tile)
+};
+
 // One tile of QT query rows starting at row i0 over every column of the group: the profile of
 // those rows is staged in LDS, the rows' H/E columns live in registers, and the tile's bottom row
 // goes to `sp` (column j of this wavefront at sp[j * 64]) for the next tile unless this is the last.
-template <int MODE, int QT, int NW, bool PAIR, bool FIRST, bool ENDS = false>
+template <int MODE, int QT, int NW, bool PAIR, bool FIRST, bool ENDS = false, bool PIPE = false>
 __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& g, bool wlive,
                                              const uint8_t* query, int m, int i0, bool lastc,
                                              int nA, int nB, uint2* sp, const uint2* wp,
-                                             uint32_t* pp, PairRun& st) {
+                                             uint32_t* pp, PairRun& st, Pipe pipe = Pipe()) {
     using F = MF<MODE>;
+    static_assert(!PIPE || (!PAIR && !ENDS), "pipelined tiles: per-symbol profile, scores only");
     constexpr int S = QT + 4;
     // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
     // different symbols start in different 4-bank groups.
@@ -260,9 +272,12 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     // Local mode keeps H itself per row and the profile holds sub; the other modes keep
     // HG = H - go and the profile holds sub + go.
     const int gadd = F::local ? 0 : go;
+    // Pipelined tiles: each wavefront stages its own tile's profile (its own LDS region).
+    const int tid = PIPE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+    const int tstride = PIPE ? 64 : 64 * NW;
     __syncthreads();
     if (PAIR) {
-        for (int k = threadIdx.x; k < A * A * QT; k += 64 * NW) {
+        for (int k = tid; k < A * A * QT; k += tstride) {
             const int pr = k / QT, r = k - pr * QT, i = i0 + r;
             const int ca = pr / A, cb = pr - ca * A;
             int va = -32768, vb = -32768;
@@ -275,7 +290,7 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                 (uint32_t)(uint16_t)(short)va | ((uint32_t)(uint16_t)(short)vb << 16);
         }
     } else {
-        for (int k = threadIdx.x; k < A * (QT / 2); k += 64 * NW) {
+        for (int k = tid; k < A * (QT / 2); k += tstride) {
             const int sym = k / (QT / 2), rr = k - sym * (QT / 2);
             const int i = i0 + 2 * rr;  // rows i (low half) and i + 1 (high half)
             int v0 = -32768, v1 = -32768;
@@ -318,7 +333,7 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     // it; the prefetch after the last block re-reads that block's inputs (in bounds).
     auto fetch = [&](int jb, uint2& wq, uint2* top) HY_INLINE {
         wq = wp[(size_t)(jb >> 2) * 64];
-        if (!FIRST) {
+        if (!FIRST && !PIPE) {
 #pragma unroll
             for (int b = 0; b < 4; ++b) top[b] = sp[(size_t)(jb + b + 1) * 64];
         }
@@ -520,22 +535,51 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
             }
         }
         if (!lastc) {
+            if constexpr (PIPE) {
+                uint2* ro = pipe.rout + (size_t)((jb >> 2) & 1) * 256 + (threadIdx.x & 63);
 #pragma unroll
-            for (int b = 0; b < 4; ++b) sp[(size_t)(jb + b + 1) * 64] = outb[b];
+                for (int b = 0; b < 4; ++b) ro[b * 64] = outb[b];
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) sp[(size_t)(jb + b + 1) * 64] = outb[b];
+            }
         }
     };
     uint2 w = make_uint2(0, 0), wn = make_uint2(0, 0);
     uint2 top[4], topn[4];
 #pragma unroll
     for (int b = 0; b < 4; ++b) top[b] = topn[b] = make_uint2(0, 0);
-    if (L > 0) fetch(0, w, top);
-    for (int jb = 0; jb < L; jb += 4) {
-        fetch(jb + 4 < L ? jb + 4 : jb, wn, topn);
-        __builtin_amdgcn_sched_barrier(0);
-        block(jb, w, top);
-        w = wn;
+    if constexpr (PIPE) {
+        // Block bi at step wave + bi; its top rows were written by the previous wavefront at the
+        // step before, with a workgroup barrier between. Residue words stream one block ahead.
+        const int nblk = L >> 2;
+        if (L > 0) w = wp[0];
+        for (int step = 0; step < pipe.nsteps; ++step) {
+            const int bi = step - pipe.wave;
+            if (bi >= 0 && bi < nblk) {
+                const int jb = bi * 4;
+                wn = wp[(size_t)((jb + 4 < L ? jb + 4 : jb) >> 2) * 64];
+                if (!FIRST) {
+                    const uint2* ri = pipe.rin + (size_t)(bi & 1) * 256 + (threadIdx.x & 63);
 #pragma unroll
-        for (int b = 0; b < 4; ++b) top[b] = topn[b];
+                    for (int b = 0; b < 4; ++b) top[b] = ri[b * 64];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+                block(jb, w, top);
+                w = wn;
+            }
+            __syncthreads();
+        }
+    } else {
+        if (L > 0) fetch(0, w, top);
+        for (int jb = 0; jb < L; jb += 4) {
+            fetch(jb + 4 < L ? jb + 4 : jb, wn, topn);
+            __builtin_amdgcn_sched_barrier(0);
+            block(jb, w, top);
+            w = wn;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) top[b] = topn[b];
+        }
     }
     st.run = run;
 #pragma unroll
@@ -552,17 +596,18 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
 
 // The tile above with FIRST chosen at run time (the first tile of a group starts from the DP
 // border instead of a boundary row).
-template <int MODE, int QT, int NW, bool PAIR, bool ENDS = false>
+template <int MODE, int QT, int NW, bool PAIR, bool ENDS = false, bool PIPE = false>
 __device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Group& g, bool wlive,
                                                 const uint8_t* query, int m, int i0, bool first,
                                                 bool lastc, int nA, int nB, uint2* sp,
-                                                const uint2* wp, uint32_t* pp, PairRun& st) {
+                                                const uint2* wp, uint32_t* pp, PairRun& st,
+                                                Pipe pipe = Pipe()) {
     if (first)
-        inter16_tile<MODE, QT, NW, PAIR, true, ENDS>(a, g, wlive, query, m, i0, lastc, nA, nB,
-                                                     sp, wp, pp, st);
+        inter16_tile<MODE, QT, NW, PAIR, true, ENDS, PIPE>(a, g, wlive, query, m, i0, lastc, nA,
+                                                           nB, sp, wp, pp, st, pipe);
     else
-        inter16_tile<MODE, QT, NW, PAIR, false, ENDS>(a, g, wlive, query, m, i0, lastc, nA, nB,
-                                                      sp, wp, pp, st);
+        inter16_tile<MODE, QT, NW, PAIR, false, ENDS, PIPE>(a, g, wlive, query, m, i0, lastc,
+                                                            nA, nB, sp, wp, pp, st, pipe);
 }
 
 // One group of 128 targets against one query: full tiles of Q rows, then one tile of the fewest
@@ -662,6 +707,91 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inte
     if (wlive) g = a.groups[gix];
     inter16_group<MODE, Q, NW, PAIR, ENDS>(a, g, wlive, a.query, a.m, a.spill + g.spill_off,
                                            a.key, pp);
+}
+
+// Pipelined one-query kernel for databases of few groups (short queries against small
+// databases, where one wavefront per group would leave most SIMDs idle): one workgroup per group,
+// its W wavefronts holding the query's W tiles of Q rows (the last the fewest of 8/16/24/Q rows
+// that cover the remainder), wavefront w one four-column block behind wavefront w - 1 and reading
+// its bottom row from LDS (Pipe). Every wavefront keeps the running maxima of its own tile; they
+// are folded through LDS and wavefront 0 writes the scores and the argmax key. LDS: W profiles
+// of pp_words words, then (W - 1) double buffers of 512 uint2, then W x 64 words for the fold.
+template <int MODE, int Q>
+__global__ __launch_bounds__(512)
+__attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_pipe(
+    Inter16Args a, uint32_t pp_words) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int W = (int)(blockDim.x >> 6);
+    const Group g = a.groups[a.group_first + blockIdx.x];
+    uint32_t* pp = lds + (size_t)wv * pp_words;
+    uint2* ring = reinterpret_cast<uint2*>(lds + (size_t)W * pp_words);
+    uint32_t* fold = lds + (size_t)W * pp_words + (size_t)(W - 1) * 1024;
+    Pipe pipe;
+    pipe.wave = wv;
+    pipe.nsteps = (int)(g.ncols >> 2) + W - 1;
+    pipe.rin = wv > 0 ? ring + (size_t)(wv - 1) * 512 : nullptr;
+    pipe.rout = wv + 1 < W ? ring + (size_t)wv * 512 : nullptr;
+
+    const int m = a.m, go = a.go, ge = a.ge;
+    const bool liveA = 2 * lane < (int)g.count, liveB = 2 * lane + 1 < (int)g.count;
+    const uint32_t slotA = g.first + 2 * lane, slotB = slotA + 1;
+    const int nA = liveA ? (int)a.lens[slotA] : 0;
+    const int nB = liveB ? (int)a.lens[slotB] : 0;
+    PairRun st;
+    st.run = splat2(-32768);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) st.runq[k] = st.run;
+    const uint2* wp = a.words + g.word_off + lane;
+    const int nfull = m / Q, rem = m - nfull * Q;
+    if (wv < nfull) {
+        inter16_tile_at<MODE, Q, 1, false, false, true>(a, g, true, a.query, m, wv * Q + 1, wv == 0,
+                                                        rem == 0 && wv == nfull - 1, nA, nB,
+                                                        nullptr, wp, pp, st, pipe);
+    } else {
+        const int i0 = nfull * Q + 1;
+        const bool first = nfull == 0;
+        if (rem <= 8)
+            inter16_tile_at<MODE, 8, 1, false, false, true>(a, g, true, a.query, m, i0, first,
+                                                            true, nA, nB, nullptr, wp, pp, st,
+                                                            pipe);
+        else if (Q > 16 && rem <= 16)
+            inter16_tile_at<MODE, 16, 1, false, false, true>(a, g, true, a.query, m, i0, first,
+                                                             true, nA, nB, nullptr, wp, pp, st,
+                                                             pipe);
+        else if (Q > 24 && rem <= 24)
+            inter16_tile_at<MODE, 24, 1, false, false, true>(a, g, true, a.query, m, i0, first,
+                                                             true, nA, nB, nullptr, wp, pp, st,
+                                                             pipe);
+        else
+            inter16_tile_at<MODE, Q, 1, false, false, true>(a, g, true, a.query, m, i0, first,
+                                                            true, nA, nB, nullptr, wp, pp, st,
+                                                            pipe);
+    }
+    const s2v mine =
+        p_max(st.run, p_max(p_max(st.runq[0], st.runq[1]), p_max(st.runq[2], st.runq[3])));
+    fold[wv * 64 + lane] = as_u32(mine);
+    __syncthreads();
+    if (wv != 0) return;
+    s2v run = mine;
+    for (int k = 1; k < W; ++k) run = p_max(run, as_s2(fold[k * 64 + lane]));
+    int bsA, bsB, dj, di;
+    border_best<MODE>(m, nA, go, ge, bsA, dj, di);
+    border_best<MODE>(m, nB, go, ge, bsB, dj, di);
+    const int sA = max(bsA, (int)run.x), sB = max(bsB, (int)run.y);
+    unsigned long long k = 0ull;
+    if (liveA) {
+        const uint32_t gi = a.gidx[slotA];
+        if (a.out_score) a.out_score[gi] = sA;
+        k = make_key(sA, gi);
+    }
+    if (liveB) {
+        const uint32_t gi = a.gidx[slotB];
+        if (a.out_score) a.out_score[gi] = sB;
+        const unsigned long long kb = make_key(sB, gi);
+        k = kb > k ? kb : k;
+    }
+    if (a.key) wave_argmax(k, a.key);
 }
 
 // Persistent batch kernel: each one-wavefront workgroup takes (group, query) items from a device
@@ -1092,6 +1222,17 @@ hipError_t launch_inter16_qw(const Inter16Args& a, hipStream_t st) {
 }
 
 template <int MODE, int Q>
+hipError_t launch_inter16_pipe_q(const Inter16Args& a, hipStream_t st) {
+    const int W = (a.m + Q - 1) / Q;
+    if (W < 2 || W > kPipeMaxWaves) return hipErrorInvalidValue;
+    const uint32_t pp_words = inter16p_lds_bytes(a.al_eff, Q) / 4;
+    const size_t lds = ((size_t)W * pp_words + (size_t)(W - 1) * 1024 + (size_t)W * 64) * 4;
+    hipLaunchKernelGGL((k_inter16_pipe<MODE, Q>), dim3(a.ngroups), dim3(64 * W), lds, st, a,
+                       pp_words);
+    return hipGetLastError();
+}
+
+template <int MODE, int Q>
 hipError_t launch_inter16_batch_q(const Inter16Args& a, const BatchArgs& b, uint32_t wgs,
                                   hipStream_t st) {
     const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
@@ -1140,6 +1281,17 @@ hipError_t launch_inter16_m(const Inter16Args& a, int q, int waves, hipStream_t 
     if (q <= 16) return launch_inter16_qw<MODE, 16, 4, true>(a, st);
     if (q <= 24) return launch_inter16_qw<MODE, 24, 4, true>(a, st);
     return launch_inter16_qw<MODE, 32, 4, true>(a, st);
+}
+
+// Tile rows of the pipelined kernel: the smallest of 16, 24, 32 whose tiles fit kPipeMaxWaves
+// wavefronts; q is that choice (inter16_pipe_rows).
+template <int MODE>
+hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, hipStream_t st) {
+    if (a.ngroups == 0) return hipSuccess;
+    if (q == 16) return launch_inter16_pipe_q<MODE, 16>(a, st);
+    if (q == 24) return launch_inter16_pipe_q<MODE, 24>(a, st);
+    if (q == 32) return launch_inter16_pipe_q<MODE, 32>(a, st);
+    return hipErrorInvalidValue;
 }
 
 template <int MODE>
@@ -1228,5 +1380,6 @@ template hipError_t launch_inter16_batch_m<HY_KMODE>(const Inter16Args&, const B
                                                      uint32_t, hipStream_t);
 template uint32_t inter16_batch_occupancy_m<HY_KMODE>(int, int);
 template hipError_t launch_inter16_ends_m<HY_KMODE>(const Inter16Args&, hipStream_t);
+template hipError_t launch_inter16_pipe_m<HY_KMODE>(const Inter16Args&, int, hipStream_t);
 
 }  // namespace hy

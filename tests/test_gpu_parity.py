@@ -564,3 +564,27 @@ def test_winner_ends_across_band_passes(mode, monkeypatch):
     for mql in (600, 300):
         check(seqs, (20, BLOSUM62, 11, 1), mode, hy.SearchType.ScoreEnd, queries, mql=mql,
               all_targets=False)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_pipelined_kernel_every_tile_count(mode, monkeypatch):
+    """The pipelined packed kernel (one query tile per wavefront of a workgroup, bottom rows
+    through LDS) for every tile count 2..8 at each tile height (16, 24, 32 rows) and every
+    remainder size, against the oracle and against the one-wavefront-per-group kernel; two
+    alphabets, targets of every padding class."""
+    rng = np.random.default_rng(1200 + mode)
+    for al, mat, go, ge in ((4, identity_matrix(4, 2, -1), 2, 1), (20, BLOSUM62, 11, 1)):
+        seqs = [rng.integers(0, al, size=int(n), dtype=np.uint8)
+                for n in rng.integers(0, 300, 700)]
+        flat, off = orc.flatten(seqs)
+        db = build(seqs, (al, mat, go, ge), mode, hy.SearchType.ScoreEnd, 256)
+        sc = hy.Scratch(db)
+        for m in (17, 24, 31, 40, 56, 97, 100, 128, 129, 150, 177, 192, 193, 230, 256):
+            q = rng.integers(0, al, size=m, dtype=np.uint8)
+            ws = orc.scan_all_flat(flat, off, q, al, mat, go, ge, mode, ends=False, threads=8)
+            want = orc.scan_flat(flat, off, q, al, mat, go, ge, mode, True, threads=8)
+            for pipe in ("1", "0"):
+                monkeypatch.setenv("HYALITE_HIP_PIPE", pipe)
+                assert np.array_equal(db.scan_scores_array(sc, q), ws), (al, m, pipe)
+                h = db.scan(sc, q)
+                assert (h.score, h.db_index, h.query_end, h.target_end) == want, (al, m, pipe)
