@@ -254,44 +254,55 @@ def serving(args, sync) -> dict:
     plain_s = time.perf_counter() - t0
     del sc
     cells = sum(len(q) for q in qs) * w.residues
-    sweep = []
-    with hy.Server(db, max_lanes=args.serving_lanes, max_batch=args.serving_batch) as srv:
-        srv.wait(srv.submit(qs[0]))
-        res = loadgen.run_poisson(srv, qs, qps=args.qps, seed=7)
-        st = srv.stats()
-        # Heavier loads on the same server: arrivals outpace one query at a time, so batches
-        # grow and pair in the batch kernel (continuous batching).
-        for qps in args.qps_sweep:
-            r = loadgen.run_poisson(srv, qs, qps=qps, seed=7)
-            if [x.hit for x in r.served] != alone:
-                raise RuntimeError(f"a served hit at {qps} QPS differs from the plain scan")
-            sweep.append({"qps": qps, "p50_ms": round(r.percentile_ms(50), 2),
-                          "p99_ms": round(r.percentile_ms(99), 2),
-                          "mean_ms": round(float(r.latencies().mean() * 1e3), 2),
-                          "mean_batch": round(float(np.mean([x.batch_size for x in r.served])), 2),
-                          "sustained_gcups": round(cells / r.wall_s / 1e9, 1),
-                          "hits_equal_alone": True})
-            log(f"serving at {qps} QPS: p50 {sweep[-1]['p50_ms']} ms, p99 {sweep[-1]['p99_ms']} "
-                f"ms, mean batch {sweep[-1]['mean_batch']}, {sweep[-1]['sustained_gcups']} GCUPS")
-    same = [s.hit for s in res.served] == alone
-    if not same:
-        raise RuntimeError("a served hit differs from the plain scan of the same query")
-    log(f"serving at {args.qps} QPS: p50 {res.percentile_ms(50):.1f} ms, "
-        f"p99 {res.percentile_ms(99):.1f} ms, {cells / res.wall_s / 1e9:.0f} GCUPS sustained")
+
+    def run(prealloc: bool) -> dict:
+        """The Poisson loads on one server: lanes built on demand within the byte budget, or
+        (prealloc) every lane built at construction."""
+        loads = []
+        with hy.Server(db, max_lanes=args.serving_lanes, max_batch=args.serving_batch,
+                       prealloc=prealloc) as srv:
+            srv.wait(srv.submit(qs[0]))
+            for qps in [args.qps] + list(args.qps_sweep):
+                before = srv.stats()
+                r = loadgen.run_poisson(srv, qs, qps=qps, seed=7)
+                st = srv.stats()
+                if [x.hit for x in r.served] != alone:
+                    raise RuntimeError(f"a served hit at {qps} QPS differs from the plain scan")
+                loads.append({
+                    "qps": qps, "p50_ms": round(r.percentile_ms(50), 2),
+                    "p99_ms": round(r.percentile_ms(99), 2),
+                    "mean_ms": round(float(r.latencies().mean() * 1e3), 2),
+                    "mean_batch": round(float(np.mean([x.batch_size for x in r.served])), 2),
+                    "sustained_gcups": round(cells / r.wall_s / 1e9, 1),
+                    "wall_s": round(r.wall_s, 3),
+                    "lanes_built": st.lanes, "lanes_before": before.lanes,
+                    "peak_queries_in_flight": st.peak_queries,
+                    "peak_gb": round(st.peak_bytes / 1e9, 2),
+                    "hits_equal_alone": True})
+                log(f"serving ({'prealloc' if prealloc else 'on demand'}) at {qps} QPS: p50 "
+                    f"{loads[-1]['p50_ms']} ms, p99 {loads[-1]['p99_ms']} ms, mean batch "
+                    f"{loads[-1]['mean_batch']}, lanes {st.lanes}, {loads[-1]['sustained_gcups']}"
+                    f" GCUPS")
+            return {"lanes_max": args.serving_lanes, "lane_gb": round(st.lane_bytes / 1e9, 2),
+                    "max_batch": st.max_batch, "loads": loads}
+
+    on_demand = run(False)
+    fixed = run(True)
+    first = on_demand["loads"][0]
     return {
         "workload": w.name, "targets": w.n, "residues": w.residues, "queries": len(qs),
         "query_lengths": "lognormal median 350 sigma 0.75 clip 20..3000",
         "partitions": parts, "qps": args.qps, "arrivals": "Poisson (seed 7)",
-        "p50_ms": round(res.percentile_ms(50), 2), "p99_ms": round(res.percentile_ms(99), 2),
-        "mean_ms": round(float(res.latencies().mean() * 1e3), 2),
-        "sustained_gcups": round(cells / res.wall_s / 1e9, 1),
-        "wall_s": round(res.wall_s, 3),
-        "lanes": st.lanes, "max_batch": st.max_batch, "lane_gb": round(st.lane_bytes / 1e9, 2),
-        "mean_batch": round(float(np.mean([s.batch_size for s in res.served])), 2),
+        "p50_ms": first["p50_ms"], "p99_ms": first["p99_ms"], "mean_ms": first["mean_ms"],
+        "sustained_gcups": first["sustained_gcups"], "wall_s": first["wall_s"],
+        "lanes": first["lanes_built"], "max_batch": on_demand["max_batch"],
+        "lane_gb": on_demand["lane_gb"], "mean_batch": first["mean_batch"],
         "plain_loop_gcups": round(cells / plain_s / 1e9, 1),
         "plain_loop_ms_per_query": round(plain_s * 1e3 / len(qs), 2),
-        "hits_equal_alone": same,
-        "load_sweep": sweep,
+        "hits_equal_alone": True,
+        "admission": "by device bytes, lanes on demand (up to lanes_max)",
+        "on_demand": on_demand,
+        "prealloc_lanes": fixed,
     }
 
 
@@ -353,6 +364,131 @@ def other_configs(sync) -> dict:
                        "targets_at_best": int((scores == scores.max()).sum())}
         log(f"{w.name}: {out[w.name]['gcups']} GCUPS scan_scores, "
             f"{out[w.name]['scan_all_ends_gcups']} GCUPS scan_all with ends")
+        del sc, db
+    # The same database with the scheme scaled by 25: the longer half of the targets prove i32
+    # and run in the 32-bit kernel beside the packed one.
+    for mode in (1, 2, 4):
+        w = wl.c3_scaled(base, mode)
+        db = build_db(w)
+        sc = hy.Scratch(db)
+        sc.profile(True)
+        q = w.queries[0]
+        db.scan_scores_array(sc, q)
+        sc.profile_read(reset=True)
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            scores = db.scan_scores_array(sc, q)
+        sync()
+        dt = (time.perf_counter() - t0) / 5
+        inter_ms, tail_ms, n = sc.profile_read(reset=True)
+        sc.profile(False)
+        parts = db.partition_sizes()
+        lens = w.lens()
+        i32_cells = int(lens[lens > 504].sum()) * len(q)
+        out[w.name] = {"op": "scan_scores", "ms": round(dt * 1e3, 3),
+                       "gcups": round(len(q) * w.residues / dt / 1e9, 1),
+                       "scheme": f"+{int(w.matrix[0])}/{int(w.matrix[1])}, gaps {w.go}/{w.ge}",
+                       "partitions": {str(k): v for k, v in parts.items()},
+                       "i32_cell_share": round(i32_cells / (len(q) * w.residues), 3),
+                       "packed_kernel_ms": round(inter_ms / max(n, 1), 3),
+                       "side_kernel_ms": round(tail_ms / max(n, 1), 3),
+                       "best_score": int(scores.max())}
+        log(f"{w.name}: {out[w.name]['gcups']} GCUPS, partitions {out[w.name]['partitions']}")
+        del sc, db
+    return out
+
+
+def reference_shapes(sync) -> dict:
+    """The reference's own scan benchmark shapes: a CR4-style database of 64 adapters (~30 nt)
+    against 91-nt reads (benches/cr4_scan.rs), as the CHANGELOG quotes them (CHANGELOG.md:112-121:
+    a 64-target x 2000-read i16 SW Score scan and an i16 HW ScoreEnd scan_all, their AVX2 and
+    SSE4.1 speed-ups over scalar). The i16 width comes from the CR4 matrix and gaps scaled by 4 (the
+    CHANGELOG leaves the scheme open; unscaled, this database proves i8). On the device: one scan
+    (or scan_all) call per read, the batched form over all reads, and for scan_all the cross
+    product of reads and adapters as one align_pairs batch (the same per-target hits). On one host
+    thread beside it: the scalar oracle and the AVX2 restatement (scores only). Every device
+    result is checked against the oracle."""
+    import oracle
+    adapters = wl.cr4_adapters(64)
+    flat, off = oracle.flatten(adapters)
+    residues = int(off[-1])
+    out = {}
+
+    def timed(f, reps=1):
+        f()
+        sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = f()
+        sync()
+        return r, (time.perf_counter() - t0) / reps
+
+    shapes = (("cr4_ov_scoreend_scan_64x256", 3, 1, 1, 256, "scan"),
+              ("sw_score_i16_scan_64x2000", 0, 0, 4, 2000, "scan"),
+              ("hw_scoreend_i16_scan_all_64x2000", 2, 1, 4, 2000, "scan_all"))
+    for name, mode, st, scale, nreads, op in shapes:
+        reads = wl.cr4_reads(nreads)
+        mat = wl.cr4_matrix(scale)
+        go = ge = 2 * scale
+        cells = sum(len(r) for r in reads) * residues
+        db = (hy.Database.builder().sequences(adapters).scoring(hy.Scoring(5, mat, go, ge))
+              .mode(hy.Mode(mode)).search_type(hy.SearchType(st)).max_query_len(91)
+              .backend(hy.BackendChoice.force(hy.Backend.Hip)).build())
+        sc = hy.Scratch(db)
+        width = str(db.score_width())
+        row = {"mode": hy.Mode(mode).code(), "search_type": "ScoreEnd" if st else "Score",
+               "targets": 64, "reads": nreads, "width": width, "cells": cells}
+        t_s = time.perf_counter()
+        if op == "scan":
+            want = [oracle.scan_flat(flat, off, r, 5, mat, go, ge, mode, bool(st)) for r in reads]
+            scalar_s = time.perf_counter() - t_s
+            loop, loop_s = timed(lambda: [db.scan(sc, r) for r in reads])
+            rflat, roff = oracle.flatten(reads)
+            batch, batch_s = timed(lambda: db.scan_batch_arrays(sc, rflat, roff), 3)
+            got = [(h.score, h.db_index, h.query_end, h.target_end) for h in loop]
+            bat = [(int(a), int(b), None if c < 0 else int(c), None if d < 0 else int(d))
+                   for a, b, c, d in zip(*batch)]
+            if got != want or bat != want:
+                raise RuntimeError(f"{name}: device hits differ from the oracle")
+            row["batch_chunk"] = sc.tuning().batch_chunk
+            row.update({"device_scan_loop_ms": round(loop_s * 1e3, 3),
+                        "device_scan_batch_ms": round(batch_s * 1e3, 3)})
+            best_s = batch_s
+        else:
+            want = [oracle.scan_all_flat(flat, off, r, 5, mat, go, ge, mode) for r in reads]
+            scalar_s = time.perf_counter() - t_s
+            loop, loop_s = timed(lambda: [db.scan_all_arrays(sc, r) for r in reads])
+            qf = np.concatenate([np.repeat(r[None, :], 64, axis=0).ravel() for r in reads])
+            qo = np.arange(0, 64 * nreads + 1, dtype=np.uint64) * 91
+            tf = np.tile(flat, nreads)
+            to = np.concatenate([off[:-1] + k * residues for k in range(nreads)] +
+                                [np.array([nreads * residues], dtype=np.uint64)])
+            psc = hy.PairScratch()
+            pairs, pairs_s = timed(lambda: hy.align_pairs_flat(
+                qf, qo, tf, to, hy.Scoring(5, mat, go, ge), hy.Mode(mode), hy.SearchType(st),
+                psc), 3)
+            for k, w3 in enumerate(want):
+                for got in (loop[k], tuple(a[64 * k:64 * (k + 1)] for a in pairs)):
+                    if not all(np.array_equal(a, b) for a, b in zip(got, w3)):
+                        raise RuntimeError(f"{name}: device hits differ from the oracle")
+            row.update({"device_scan_all_loop_ms": round(loop_s * 1e3, 3),
+                        "device_align_pairs_cross_product_ms": round(pairs_s * 1e3, 3)})
+            best_s = pairs_s
+        avx2_s = 0.0
+        for r in reads:
+            avx2_s += oracle.avx2_scan_scores_flat(flat, off, r, 5, mat, go, ge, mode,
+                                                  threads=1)[1]
+        row.update({"cpu_scalar_ms": round(scalar_s * 1e3, 2),
+                    "cpu_avx2_scores_ms": round(avx2_s * 1e3, 2),
+                    "avx2_over_scalar": round(scalar_s / avx2_s, 1),
+                    "device_gcups": round(cells / best_s / 1e9, 1),
+                    "device_over_avx2": round(avx2_s / best_s, 1),
+                    "note": "one host thread each; the AVX2 restatement computes scores only "
+                            "(no argmax or ends), the scalar oracle the full result"})
+        out[name] = row
+        log(f"{name}: device {row['device_gcups']} GCUPS, scalar {row['cpu_scalar_ms']} ms, "
+            f"avx2 {row['cpu_avx2_scores_ms']} ms")
         del sc, db
     return out
 
@@ -665,8 +801,9 @@ def main() -> int:
     ap.add_argument("--serving-queries", type=int, default=64)
     ap.add_argument("--serving-lanes", type=int, default=3)
     ap.add_argument("--serving-batch", type=int, default=16)
-    ap.add_argument("--only-configs", action="store_true",
-                    help="only run the DNA configurations and print their JSON (for profiling)")
+    ap.add_argument("--only", choices=("configs", "shapes", "serving"),
+                    help="only run the DNA configurations, the reference's scan shapes or the "
+                         "serving form and print their JSON (for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB and exit")
     args = ap.parse_args()
@@ -686,8 +823,14 @@ def main() -> int:
         if _hip.hipDeviceSynchronize() != 0:
             raise RuntimeError("hipDeviceSynchronize failed")
 
-    if args.only_configs:
-        print(json.dumps(other_configs(sync)), flush=True)
+    if args.only == "configs":
+        print(json.dumps({"other_configs": other_configs(sync)}), flush=True)
+        return 0
+    if args.only == "shapes":
+        print(json.dumps({"reference_shapes": reference_shapes(sync)}), flush=True)
+        return 0
+    if args.only == "serving":
+        print(json.dumps({"serving": serving(args, sync)}), flush=True)
         return 0
 
     t0 = time.perf_counter()
@@ -808,6 +951,7 @@ def main() -> int:
     pr, pr_keep = (None, None) if args.no_pairs else pairs(sync)
     serve = None if args.no_serving else serving(args, sync)
     others = None if args.no_configs else other_configs(sync)
+    ref_shapes = None if args.no_configs else reference_shapes(sync)
 
     out = {
         "metric": METRIC,
@@ -833,6 +977,7 @@ def main() -> int:
         "single_query": single,
         "serving": serve,
         "other_configs": others,
+        "reference_shapes": ref_shapes,
         "traceback": tb,
         "pairs": pr,
         "db_image": image,

@@ -448,7 +448,8 @@ _ALIGNMENT_DTYPE = np.dtype([("score", "<i4"), ("reserved", "<u4"), ("query_star
 
 class _HyServerConfig(ctypes.Structure):
     _fields_ = [("scratch_budget_bytes", ctypes.c_uint64), ("max_lanes", ctypes.c_uint32),
-                ("max_batch", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+                ("max_batch", ctypes.c_uint32), ("prealloc", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
 
 
 class _HyServerResult(ctypes.Structure):
@@ -463,7 +464,8 @@ class _HyServerStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
                 ("batches", ctypes.c_uint64), ("query_residues", ctypes.c_uint64),
                 ("lanes", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
-                ("lane_bytes", ctypes.c_uint64)]
+                ("lane_bytes", ctypes.c_uint64), ("peak_bytes", ctypes.c_uint64),
+                ("peak_queries", ctypes.c_uint64), ("inflight_bytes", ctypes.c_uint64)]
 
 
 class _HyTuning(ctypes.Structure):
@@ -987,6 +989,29 @@ class Database:
             return out
         return res
 
+    def scan_batch_arrays(self, scratch: "Scratch", flat, offsets):
+        """Batched form over queries given as one residue buffer plus n+1 offsets (query k =
+        flat[offsets[k]:offsets[k+1]]), with no per-query Python objects: (scores int32[n],
+        db_indices uint64[n], query_ends int64[n], target_ends int64[n]), -1 meaning None."""
+        buf = np.ascontiguousarray(np.asarray(flat, dtype=np.uint8).ravel())
+        off = np.ascontiguousarray(np.asarray(offsets, dtype=np.uint64))
+        if off.ndim != 1 or off.size == 0 or int(off[0]) != 0 or \
+                (off.size > 1 and (off[1:] < off[:-1]).any()) or int(off[-1]) > buf.size:
+            raise ValueError("offsets must start at 0, never decrease and end within the buffer")
+        nq = off.size - 1
+        keep = buf if buf.size else np.zeros(1, dtype=np.uint8)
+        ptrs = (np.uint64(keep.ctypes.data) + off[:-1]).astype(np.uint64)
+        lens = np.diff(off).astype(np.uint64)
+        s = np.empty(max(nq, 1), dtype=np.int32)
+        idx = np.empty(max(nq, 1), dtype=np.uint64)
+        qe = np.empty(max(nq, 1), dtype=np.int64)
+        te = np.empty(max(nq, 1), dtype=np.int64)
+        _check_scan(load_library().hy_scan_batch(
+            self._h, scratch._h, ptrs.ctypes.data_as(ctypes.POINTER(_u8p)),
+            lens.ctypes.data_as(_sizep), nq, s.ctypes.data_as(_i32p),
+            idx.ctypes.data_as(_u64p), qe.ctypes.data_as(_i64p), te.ctypes.data_as(_i64p)))
+        return s[:nq], idx[:nq], qe[:nq], te[:nq]
+
     def scan_batch(self, scratch: "Scratch", queries) -> list:
         """Batched form: one hit per query, each equal to scan() of that query."""
         qs = [np.ascontiguousarray(np.asarray(q, dtype=np.uint8)) for q in queries]
@@ -1111,6 +1136,9 @@ class ServerStats:
     lanes: int
     max_batch: int
     lane_bytes: int
+    peak_bytes: int = 0
+    peak_queries: int = 0
+    inflight_bytes: int = 0
 
 
 class ServerTimeout(TimeoutError):
@@ -1285,12 +1313,14 @@ def score2(colmax, best_score: int, best_target_end: int, matrix_max: int,
 
 class Server:
     """Serving form over a resident database (hy_server_*): queries are queued, admitted by free
-    lanes in batches of up to max_batch and scanned while other batches run; the number of lanes
-    is the most that fit scratch_budget_bytes (0 = no limit), at most max_lanes."""
+    lanes in batches of up to max_batch and scanned while other batches run. Admission is by
+    device bytes: scratch_budget_bytes (0 = no limit) covers the lanes and each request in flight
+    (its query copy, key and result, and beyond 512 rows its winner boundary row); lanes are
+    added on demand up to max_lanes (prealloc=True builds every lane the budget holds at once)."""
 
     def __init__(self, db: Database, scratch_budget_bytes: int = 0, max_lanes: int = 2,
-                 max_batch: int = 64):
-        cfg = _HyServerConfig(scratch_budget_bytes, max_lanes, max_batch)
+                 max_batch: int = 64, prealloc: bool = False):
+        cfg = _HyServerConfig(scratch_budget_bytes, max_lanes, max_batch, int(bool(prealloc)))
         out = ctypes.c_void_p(None)
         rc = load_library().hy_server_new(db._h, ctypes.byref(cfg), ctypes.byref(out))
         if rc == 103:
@@ -1328,7 +1358,8 @@ class Server:
         st = _HyServerStats()
         load_library().hy_server_stats_read(self._h, ctypes.byref(st))
         return ServerStats(st.submitted, st.completed, st.batches, st.query_residues, st.lanes,
-                           st.max_batch, st.lane_bytes)
+                           st.max_batch, st.lane_bytes, st.peak_bytes, st.peak_queries,
+                           st.inflight_bytes)
 
     def close(self) -> None:
         """Finish every queued query, then stop the lanes."""

@@ -457,14 +457,24 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     }
     const size_t n_not16 = inter_idx.size();
     const size_t n16_long = idx16.size();
+    // End-tracking scans run the packed targets in the packed kernel too, except in local mode
+    // when a packed group is wider than the 16-bit column its end tracking keeps (65,535): then
+    // every non-long target runs at 32 bits, and only then does the 32-bit layout hold the
+    // packed targets as well. The longest short packed target comes first in length order.
     for (uint32_t k : order_idx)
         if (!(all_systolic || lens[k] > long_threshold) && elig16[k]) {
-            inter_idx.push_back(k);
+            db->ends32_all = mode == HY_MODE_SW && ((lens[k] + 3) & ~(size_t)3) > 65535;
+            break;
+        }
+    for (uint32_t k : order_idx)
+        if (!(all_systolic || lens[k] > long_threshold) && elig16[k]) {
+            if (db->ends32_all) inter_idx.push_back(k);
             idx16.push_back(k);
         }
 
     // Inter layout: groups of 64 consecutive (length-sorted) targets; residues interleaved so a
-    // wavefront's load of column word w reads 64 consecutive 32-bit words.
+    // wavefront's load of column word w reads 64 consecutive 32-bit words. It holds the targets
+    // the packed kernel cannot take (and the packed ones as well when ends32_all is set).
     {
         const uint32_t pad = db->al_eff;  // padding symbol
         const size_t ns = inter_idx.size();
@@ -601,6 +611,20 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     s->q16 = db->q16;
     s->waves16 = db->waves16;
     s->bq16 = db->q16;
+    // Batched submissions: each query (pair) adds one item per packed group to the persistent
+    // queue, so a database of few groups needs more queries per submission to give the resident
+    // workgroups (about 8 per CU) several items each; 256 otherwise, at most 4,096, and no more
+    // than keeps the per-query winner rows of queries beyond 512 rows within 256 MB.
+    if (!db->groups16.empty()) {
+        const uint64_t slots = 8ull * std::max<uint32_t>(hy::device_cus(), 1);
+        const uint64_t groups = db->groups16.size();
+        const uint64_t row_bytes = db->mql > 512 ? (db->mtl + 1) * sizeof(int2) : 0;
+        uint64_t chunk = 256;
+        while (chunk < 4096 && groups * chunk < 4 * slots &&
+               2 * chunk * row_bytes <= (256ull << 20))
+            chunk *= 2;
+        s->bchunk = (uint32_t)chunk;
+    }
     auto fail = [&](int code) { delete s; return code; };
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&s->aux, hipStreamNonBlocking) != hipSuccess ||
@@ -718,8 +742,8 @@ int pipe_rows(const hy_db* db, int m, uint32_t ngroups) {
     // HYALITE_HIP_PIPE_ROWS=16|24|32 asks for that tile height where the query fits its tiles.
     if (const char* r = std::getenv("HYALITE_HIP_PIPE_ROWS")) {
         const int want = std::atoi(r);
-        if ((want == 16 || want == 24 || want == 32) && q != 0 && q <= want &&
-            (m + want - 1) / want >= 2)
+        if ((want == 16 || want == 24 || want == 32) && q != 0 &&
+            (m + want - 1) / want >= 2 && (m + want - 1) / want <= hy::kPipeMaxWaves)
             q = want;
     }
     if (mode_env == 0 || q == 0 || ngroups == 0) return 0;
@@ -744,9 +768,7 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     // its groups must stay within 65,535 columns (else every non-long target runs at 32 bits);
     // the other modes keep their ends in 32 bits.
     const bool has16 = db->groups16.size() > db->groups16_short_first;
-    const bool packed_ends =
-        ends && has16 &&
-        (db->mode != HY_MODE_SW || db->groups16[db->groups16_short_first].ncols <= 65535u);
+    const bool packed_ends = ends && has16 && !db->ends32_all;
     const bool run16 = (!ends || packed_ends) && has16;
     const uint32_t g32 =
         (ends && !packed_ends) ? (uint32_t)db->groups.size() : db->groups_score32;

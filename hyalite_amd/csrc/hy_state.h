@@ -41,8 +41,11 @@ struct hy_db {
     uint32_t* d_gidx = nullptr;
     uint64_t inter_spill = 0;      // int2 entries
     // Groups [0, groups_score32) hold the targets that are not eligible for the packed kernel;
-    // score-only scans run the 32-bit kernel over those and the packed kernel over the rest.
+    // scans run the 32-bit kernel over those and the packed kernel over the rest. ends32_all: a
+    // local-mode database with a packed group wider than 65,535 columns, whose end-tracking scans
+    // run every non-long target at 32 bits (the groups after groups_score32 hold them again).
     uint32_t groups_score32 = 0;
+    bool ends32_all = false;
     uint64_t score32_spill = 0;    // int2 entries of groups [0, groups_score32)
     // Packed 16-bit partition (two targets per lane).
     std::vector<hy::Group> groups16;

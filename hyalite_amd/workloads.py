@@ -151,6 +151,16 @@ def c3_dna_modes(seed: int = 3, n_targets: int = 2_000_000, mode: int = 1) -> Wo
                     150, flat, off, [q])
 
 
+def c3_scaled(base: Workload, mode: int, scale: int = 25) -> Workload:
+    """The DNA short-read database of c3_dna_modes with the scheme scaled by `scale` (+2s / -s,
+    gap_open 2s, gap_ext s): at scale 25 and a 150-nt query, the NW/HW/SHW width proof gives i16
+    up to 504-nt targets and i32 beyond, so one database splits across both widths."""
+    name = base.name.replace("mode1", {1: "NW", 2: "HW", 3: "OV", 4: "SHW"}.get(mode, str(mode)))
+    return Workload(name.replace("_scores_", f"_scaled{scale}_scores_"), base.al,
+                    (base.matrix * scale).astype(np.int32), base.go * scale, base.ge * scale,
+                    mode, 1, base.mql, base.flat, base.off, base.queries)
+
+
 def c4_batch_queries(seed: int = 4, nq: int = 256) -> list:
     rng = np.random.default_rng(seed)
     p = protein_freqs()
@@ -248,3 +258,53 @@ def protein_pairs(n: int = 100_000, seed: int = 7) -> PairBatch:
     np.cumsum(lt, out=toff[1:])
     return PairBatch(f"protein_pairs_250-350x300-400x{n}", 24, protein_matrix(2), 11, 1, q, qoff,
                      t, toff)
+
+
+# ---- The reference's own scan benchmark shapes (tiny databases, many reads) -------------------
+
+CR4_TSO = "AAGCAGTGGTATCAACGCAGAGTACATGGG"  # the 10x template-switch oligo
+CR4_READ_LEN = 91
+
+
+def _lcg(state: list) -> int:
+    """The corpus PRNG of benches/cr4_scan.rs:54-60 (state in a one-element list)."""
+    state[0] = (state[0] * 6364136223846793005 + 1442695040888963407) & ((1 << 64) - 1)
+    return state[0] >> 33
+
+
+def _encode_acgtn(s: str) -> np.ndarray:
+    return np.array(["ACGT".find(c) if c in "ACGT" else 4 for c in s], dtype=np.uint8)
+
+
+def cr4_adapters(count: int = 64) -> list:
+    """benches/cr4_scan.rs:66-74: the TSO, poly-A/T/G 30-mers, then random 28-33 nt fillers."""
+    v = [_encode_acgtn(CR4_TSO), np.zeros(30, np.uint8), np.full(30, 3, np.uint8),
+         np.full(30, 2, np.uint8)]
+    st = [0x00C0FFEE]
+    while len(v) < count:
+        n = 28 + _lcg(st) % 6
+        v.append(np.array([_lcg(st) % 4 for _ in range(n)], dtype=np.uint8))
+    return v
+
+
+def cr4_reads(count: int) -> list:
+    """benches/cr4_scan.rs:78-93: 91-nt reads, every third TSO-led, every third ending in 25 A."""
+    tso = _encode_acgtn(CR4_TSO)
+    st = [0x0000BEEF]
+    out = []
+    for i in range(count):
+        r = np.array([_lcg(st) % 4 for _ in range(CR4_READ_LEN)], dtype=np.uint8)
+        if i % 3 == 0:
+            r[:len(tso)] = tso
+        elif i % 3 == 1:
+            r[CR4_READ_LEN - 25:] = 0
+        out.append(r)
+    return out
+
+
+def cr4_matrix(scale: int = 1) -> np.ndarray:
+    """STAR's CellRanger4 scoring (benches/cr4_scan.rs:26-36): +1 match, -2 mismatch or N, N-N 0."""
+    m = np.full((5, 5), -2, dtype=np.int32)
+    np.fill_diagonal(m, 1)
+    m[4, 4] = 0
+    return (m * scale).reshape(-1)

@@ -284,8 +284,8 @@ int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
 int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries);
 
 /* Launch shapes of the packed 16-bit kernels, held per scratch (defaults come from the database:
- * 24-row tiles, one wavefront per workgroup, the occupancy limit of resident workgroups and
- * 256 queries per batched submission). Shapes change speed only, never results. No reference
+ * 32-row tiles, one wavefront per workgroup, the occupancy limit of resident workgroups and
+ * 256 queries per batched submission, up to 4,096 for databases of few packed groups). Shapes change speed only, never results. No reference
  * counterpart (the reference picks its SIMD layout in src/inter.rs:277-386). */
 typedef struct hy_tuning {
     int32_t tile_rows;        /* one-query scans: query rows per register tile (16, 24 or 32) */
@@ -326,21 +326,27 @@ int hy_scratch_profile_read(hy_scratch* s, double* inter_ms, double* tail_ms, ui
 int hy_scratch_batch_profile_read(hy_scratch* s, double* kernel_ms, uint64_t* launches, int reset);
 
 /* Serving form (no reference counterpart: the reference is a library without a request loop).
- * A server keeps the database resident and owns `lanes` scratches, each with its own stream and
+ * A server keeps the database resident and runs lanes, each a scratch with its own stream and
  * worker thread. Submitted queries wait in one FIFO; a free lane admits the queries at its head,
  * up to max_batch, and runs them as one batched scan (hy_scan_batch), so requests that arrive
  * while other batches are on the GPU are admitted as soon as a lane is free and their work
- * overlaps on the device. Admission is bounded by device memory: the number of lanes is the
- * largest that fits scratch_budget_bytes (at most max_lanes; HY_E_OUT_OF_MEMORY if not even one
- * fits), each lane's batched-form reservation sized for max_batch queries. Every hit equals
- * Database::scan of that query alone (src/database.rs:83). */
+ * overlaps on the device. Admission is by device bytes: scratch_budget_bytes covers the lanes'
+ * buffers (lane_bytes each) plus the requests in flight, each charged its own bytes (its query
+ * copy, key and result slots, and for a query beyond 512 rows its winner pass's boundary row of
+ * max_target_len + 1 entries); a request is admitted when its bytes fit what is left. The server
+ * starts with one lane and adds one, up to max_lanes, when a request waits, every lane is busy
+ * and the budget holds another lane plus that request (prealloc = 1 builds every lane the budget
+ * holds at once instead). HY_E_OUT_OF_MEMORY if the budget does not hold one lane and one
+ * request of max_query_len. Every hit equals Database::scan of that query alone
+ * (src/database.rs:83). */
 typedef struct hy_server hy_server;
 
 typedef struct hy_server_config {
-    uint64_t scratch_budget_bytes; /* device bytes the lanes may hold; 0 = no limit */
+    uint64_t scratch_budget_bytes; /* device bytes the lanes and requests may hold; 0 = no limit */
     uint32_t max_lanes;            /* concurrent batches (1..64) */
     uint32_t max_batch;            /* queries per batch (1..65536) */
-    uint32_t reserved[4];          /* must be 0 */
+    uint32_t prealloc;             /* 1: build every lane at construction; 0: on demand */
+    uint32_t reserved[3];          /* must be 0 */
 } hy_server_config;
 
 typedef struct hy_server_result {
@@ -357,8 +363,11 @@ typedef struct hy_server_result {
 typedef struct hy_server_stats {
     uint64_t submitted, completed, batches;
     uint64_t query_residues;       /* sum of completed query lengths */
-    uint32_t lanes, max_batch;
+    uint32_t lanes, max_batch;     /* lanes built so far */
     uint64_t lane_bytes;           /* device bytes one lane holds */
+    uint64_t peak_bytes;           /* most bytes held at once (lanes + requests in flight) */
+    uint64_t peak_queries;         /* most requests in flight at once */
+    uint64_t inflight_bytes;       /* bytes of the requests in flight now */
 } hy_server_stats;
 
 int hy_server_new(const hy_db* db, const hy_server_config* cfg, hy_server** out);

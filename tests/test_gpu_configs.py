@@ -97,3 +97,34 @@ def test_protein_sw_360x1M_properties():
     qs = [x[:w.mql] for x in wl.c4_batch_queries(nq=6)] + [q]
     sc.set_tuning(hy.Tuning(batch_chunk=4))
     assert db.scan_batch(sc, qs) == [db.scan(sc, x) for x in qs]
+
+
+@pytest.mark.parametrize("mode", [1, 2, 4])
+def test_dna_modes_scaled_150x2M_two_widths(dna_modes, mode):
+    """The mode-sweep database with the scheme scaled by 25: targets up to 504 nt prove i16
+    (packed kernel), longer ones i32 (32-bit kernel beside it); scan_scores, scan and scan_all
+    with ends over all 2,000,000 targets."""
+    w = wl.c3_scaled(dna_modes, mode)
+    q = w.queries[0]
+    db = build(w)
+    parts = db.partition_sizes()
+    lens = w.lens()
+    assert parts[hy.ScoreWidth.I32] == int((lens > 504).sum()) > 500_000
+    assert parts[hy.ScoreWidth.I16] == int((lens <= 504).sum())
+    sc = hy.Scratch(db)
+    scores = db.scan_scores_array(sc, q)
+    hit = db.scan(sc, q)
+    best = int(scores.max())
+    assert hit.score == best and hit.db_index == int(np.flatnonzero(scores == best)[0])
+    winner_check(w, q, hit, mode)
+    sample_check(w, q, scores, mode, seed=10 + mode)
+    s_all, qe, te = db.scan_all_arrays(sc, q)
+    assert np.array_equal(s_all, scores)
+    # Ends of a sample against the oracle's align_core, both widths represented.
+    rng = np.random.default_rng(20 + mode)
+    for k in list(rng.choice(np.flatnonzero(lens <= 504), 40)) + \
+            list(rng.choice(np.flatnonzero(lens > 504), 40)):
+        t = w.flat[int(w.off[k]):int(w.off[k + 1])]
+        s, e1, e2 = orc.align_core(q, t, w.al, w.matrix, w.go, w.ge, mode)
+        assert (int(s_all[k]), int(qe[k]), int(te[k])) == (s, -1 if e1 is None else e1,
+                                                            -1 if e2 is None else e2)

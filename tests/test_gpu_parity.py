@@ -5,6 +5,7 @@ tests/properties.rs, tests/opal_regressions.rs, tests/real_data.rs): the Hip bac
 its (score, db_index, query_end, target_end) compared with the oracle's scalar scan, per target
 (scan_all, scan_scores) and single best (scan), on the same seeded inputs.
 """
+import dataclasses
 import threading
 
 import numpy as np
@@ -406,6 +407,14 @@ def _tuning_db(mode, st):
     return seqs, qs, build(seqs, (20, BLOSUM62, 11, 1), mode, st, 150)
 
 
+def default_shapes(sc, db):
+    """The scratch starts at the default shapes; its batched submissions hold 256 queries, or
+    more (a power of two up to 4,096) for a database of few packed groups."""
+    t = sc.tuning()
+    return (dataclasses.replace(t, batch_chunk=256) == hy.Tuning() and
+            t.batch_chunk in (256, 512, 1024, 2048, 4096))
+
+
 @pytest.mark.parametrize("mode", [MODE_SW, MODE_NW, MODE_OV])
 def test_every_launch_shape_gives_the_same_hits(mode):
     """Tile rows x workgroup shape of the one-query kernel and tile rows x resident workgroups x
@@ -417,7 +426,7 @@ def test_every_launch_shape_gives_the_same_hits(mode):
     want_scores = [orc.scan_all_flat(flat, off, q, 20, BLOSUM62, 11, 1, mode, threads=4)[0]
                    for q in qs[:2]]
     sc = hy.Scratch(db)
-    assert sc.tuning() == hy.Tuning()
+    assert default_shapes(sc, db)
     for rows in (16, 24, 32):
         for waves in (1, 4, 8):
             sc.set_tuning(hy.Tuning(tile_rows=rows, waves=waves))
@@ -445,7 +454,7 @@ def test_invalid_launch_shapes_are_rejected():
                 hy.Tuning(batch_wgs_per_cu=-1), hy.Tuning(batch_pairs=2)):
         with pytest.raises(ValueError):
             sc.set_tuning(bad)
-    assert sc.tuning() == hy.Tuning()
+    assert default_shapes(sc, db)
 
 
 def test_autotune_keeps_results_and_uses_its_cache(tmp_path, monkeypatch):
@@ -508,15 +517,26 @@ def test_served_hit_is_the_same_alone_and_in_the_mix(st):
         assert all(s.latency_s >= s.queued_s >= 0 for s in res.served)
         stt = srv.stats()
         assert stt.completed == stt.submitted == 6 + 2 * len(qs)
-        assert stt.lanes == 2
+        assert 1 <= stt.lanes <= 2 and stt.peak_queries > 1 and stt.inflight_bytes == 0
 
 
 def test_server_memory_budget_sets_the_lanes():
+    """Lanes are built on demand within the byte budget; prealloc builds them all at once."""
     db, qs, want = _serving_db()
-    with hy.Server(db, max_lanes=4, max_batch=8) as srv:
+    with hy.Server(db, max_lanes=4, max_batch=8, prealloc=True) as srv:
         lane = srv.stats().lane_bytes
         assert srv.stats().lanes == 4 and lane > 0
-    with hy.Server(db, scratch_budget_bytes=int(lane * 2.5), max_lanes=4, max_batch=8) as srv:
+    with hy.Server(db, max_lanes=4, max_batch=8) as srv:
+        assert srv.stats().lanes == 1  # on demand: one lane until requests wait
+    budget = int(lane * 2.5)
+    with hy.Server(db, scratch_budget_bytes=budget, max_lanes=4, max_batch=2) as srv:
+        assert srv.stats().lanes == 1
+        tickets = [srv.submit(q) for q in qs]  # a burst: every lane busy, requests waiting
+        assert [_tuple(srv.wait(t).hit) for t in tickets] == want
+        st = srv.stats()
+        assert 1 <= st.lanes <= 2 and st.peak_bytes <= budget, st
+    with hy.Server(db, scratch_budget_bytes=budget, max_lanes=4, max_batch=8,
+                   prealloc=True) as srv:
         assert srv.stats().lanes == 2
         assert [_tuple(srv.wait(srv.submit(q)).hit) for q in qs[:3]] == want[:3]
     with pytest.raises(hy.DeviceError):
@@ -524,6 +544,31 @@ def test_server_memory_budget_sets_the_lanes():
     for bad in (dict(max_lanes=0), dict(max_batch=0), dict(max_lanes=65)):
         with pytest.raises(ValueError):
             hy.Server(db, **bad)
+
+
+def test_server_admits_requests_by_their_bytes():
+    """A budget that holds one lane plus about two long requests: requests beyond 512 rows carry
+    a winner boundary row each, so at most that many are in flight at once; every hit still
+    equals the oracle."""
+    rng = np.random.default_rng(901)
+    seqs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(1, 400, 500)]
+    seqs[3] = rng.integers(0, 20, size=60_000, dtype=np.uint8)  # max_target_len 60,000
+    db = build(seqs, (20, BLOSUM62, 11, 1), MODE_SW, hy.SearchType.ScoreEnd, 700)
+    qs = [rng.integers(0, 20, size=int(n), dtype=np.uint8) for n in rng.integers(520, 700, 8)]
+    flat, off = orc.flatten(seqs)
+    want = [orc.scan_flat(flat, off, q, 20, BLOSUM62, 11, 1, MODE_SW, True, threads=4)
+            for q in qs]
+    with hy.Server(db, max_lanes=1, max_batch=8) as probe:
+        lane = probe.stats().lane_bytes
+    row = (60_000 + 1) * 8  # one winner boundary row
+    budget = lane + 2 * (row + 1024) + row // 2
+    with hy.Server(db, scratch_budget_bytes=budget, max_lanes=1, max_batch=8) as srv:
+        tickets = [srv.submit(q) for q in qs]
+        served = [srv.wait(t) for t in tickets]
+        assert [_tuple(s.hit) for s in served] == want
+        st = srv.stats()
+        assert st.peak_queries == 2 and max(s.batch_size for s in served) <= 2, st
+        assert st.peak_bytes <= budget and st.inflight_bytes == 0
 
 
 def test_server_contract_errors():
@@ -588,3 +633,19 @@ def test_pipelined_kernel_every_tile_count(mode, monkeypatch):
                 assert np.array_equal(db.scan_scores_array(sc, q), ws), (al, m, pipe)
                 h = db.scan(sc, q)
                 assert (h.score, h.db_index, h.query_end, h.target_end) == want, (al, m, pipe)
+
+
+@pytest.mark.parametrize("mode", [MODE_SW, MODE_NW, MODE_OV])
+def test_packed_groups_wider_than_16bit_columns(mode, monkeypatch):
+    """Targets of 66,000-70,000 residues kept out of the systolic kernel (the long-target
+    threshold raised) sit in packed groups wider than 65,535 columns: local-mode end tracking
+    keeps columns in 16 bits, so its scan_all then runs every non-long target at 32 bits; the
+    other modes keep their ends in 32 bits inside the packed kernel. Both against the oracle."""
+    monkeypatch.setenv("HYALITE_HIP_LONG_THRESHOLD", "200000")
+    rng = np.random.default_rng(1300 + mode)
+    seqs = [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in (66_000, 70_000, 67_500)]
+    seqs += [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in rng.integers(0, 300, 200)]
+    q = rng.integers(0, 4, size=40, dtype=np.uint8)
+    seqs[1][30_000:30_040] = q  # a full-length copy far beyond column 65,535 in one case
+    seqs[2][66_000:66_040] = q
+    check(seqs, dna_scheme(), mode, hy.SearchType.ScoreEnd, [q, q[:20]], mql=40)
