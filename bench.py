@@ -792,8 +792,9 @@ def main() -> int:
     ap.add_argument("--no-image", action="store_true", help="skip the database image round trip")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the DNA configurations (10k-target scan, 2M-target mode sweep)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="pick launch shapes with the autotuner before timing")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of running the autotuner "
+                         "(it picks them before timing, from the batch's own queries)")
     ap.add_argument("--qps", type=float, default=SERVING_QPS)
     ap.add_argument("--qps-sweep", type=float, nargs="*", default=[20.0, 40.0],
                     help="further serving loads on the same server (continuous batching)")
@@ -847,12 +848,13 @@ def main() -> int:
         f"({sum(len(q) for q in qs)} residues), gen {gen_s:.1f}s, build+upload {build_s:.1f}s, "
         f"partitions {parts}")
     tune_s = None
-    if args.autotune:
+    default_shapes = sc.tuning()
+    if not args.no_autotune:
         t0 = time.perf_counter()
         sc.autotune(qs)
         tune_s = time.perf_counter() - t0
     tuning = sc.tuning()
-    log(f"launch shapes {tuning}")
+    log(f"launch shapes {tuning} (defaults {default_shapes}), autotune {tune_s} s")
 
     # Warm-up submissions are graph-captured and replayed; the timed steps record HIP events
     # around the persistent kernel (profiling submits the same sequence without the graph).
@@ -972,6 +974,7 @@ def main() -> int:
                    "search_type": "ScoreEnd", "max_query_len": w.mql,
                    "target_lengths": "lognormal median 300 sigma 0.75 clip 30..5000",
                    "partitions": parts, "launch_shapes": tuning.__dict__,
+                   "default_shapes": default_shapes.__dict__,
                    "autotune_s": None if tune_s is None else round(tune_s, 1)},
         "plain": plain,
         "single_query": single,

@@ -40,6 +40,41 @@ Sample spread(const uint8_t* const* qs, const size_t* ls, size_t nq, size_t k) {
     return out;
 }
 
+// Up to 2 * npairs queries as npairs adjacent pairs of the length-sorted input, the pairs placed
+// at evenly spaced fractions of the input's residues (so of its cells: long queries, which carry
+// most of a batch's work, are sampled as often as they weigh). The batch kernel pairs queries of
+// neighbouring lengths in its lanes, so its efficiency on a batch is that of such pairs: queries
+// spread over the whole range pair a 2,000-residue query with a 1,000-residue one, and queries
+// spread by count over-weight the short ones, whose remainder tiles favour smaller tiles.
+Sample adjacent_pairs(const uint8_t* const* qs, const size_t* ls, size_t nq, size_t npairs) {
+    std::vector<size_t> idx(nq);
+    std::iota(idx.begin(), idx.end(), size_t{0});
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t a, size_t b) { return ls[a] < ls[b]; });
+    Sample out;
+    if (nq <= 2 * npairs) {
+        for (size_t i : idx) {
+            out.q.push_back(qs[i]);
+            out.len.push_back(ls[i]);
+        }
+        return out;
+    }
+    std::vector<double> cum(nq + 1, 0.0);
+    for (size_t i = 0; i < nq; ++i) cum[i + 1] = cum[i] + (double)ls[idx[i]];
+    size_t prev_end = 0;
+    for (size_t p = 0; p < npairs; ++p) {
+        const double at = (p + 0.5) / (double)npairs * cum[nq];
+        size_t pos = (size_t)(std::lower_bound(cum.begin() + 1, cum.end(), at) - cum.begin()) - 1;
+        // Distinct pairs: after the previous one, and within the input.
+        pos = std::min(std::max(pos, prev_end), nq - 2 * (npairs - p));
+        for (size_t i = pos; i < pos + 2; ++i) {
+            out.q.push_back(qs[idx[i]]);
+            out.len.push_back(ls[idx[i]]);
+        }
+        prev_end = pos + 2;
+    }
+    return out;
+}
+
 // Device time of the one-query kernels over the sample: per query the best of `reps` timed
 // scans after one untimed scan, summed over the sample.
 int time_plain(const hy_db* db, hy_scratch* s, const Sample& smp, int reps, double* ms) {
@@ -203,9 +238,8 @@ extern "C" int hy_scratch_autotune(const hy_db* db, hy_scratch* s, const uint8_t
     }
 
     if (forms & 2u) {
-        const Sample smp = spread(queries, qlens, nq, 8);
-        double best_ms = std::numeric_limits<double>::infinity(), best_wall = 0.0;
-        uint32_t best_slots = 0;
+        const Sample smp = adjacent_pairs(queries, qlens, nq, 8);
+        double best_ms = std::numeric_limits<double>::infinity();
         hy_tuning pick = best;
         std::vector<int> pair_opts{best.batch_pairs};
         if (db->mode == HY_MODE_SW) pair_opts = {1, 0};
@@ -226,37 +260,21 @@ extern "C" int hy_scratch_autotune(const hy_db* db, hy_scratch* s, const uint8_t
                 if ((rc = hy_scratch_set_tuning(s, &t)) != HY_OK) return finish(rc);
                 double kms = 0.0, wall = 0.0;
                 if ((rc = time_batch(db, s, smp, &kms, &wall)) != HY_OK) return finish(rc);
+                (void)wall;
                 if (kms < best_ms) {
                     best_ms = kms;
-                    best_wall = wall;
-                    best_slots = s->nslots;
                     pick = t;
                 }
             }
         }
-        // Submission size: the makespan of a submission of c queries is about
-        //   overhead + max(c * mean_cells / rate, largest_item_cells / per_workgroup_rate),
-        // the largest item being the longest query against the group of the longest targets.
-        // Take the smallest c whose ideal time is within 3% of that.
-        double sample_len = 0.0;
-        for (size_t v : smp.len) sample_len += (double)v;
-        double mean_len = 0.0;
-        for (size_t k = 0; k < nq; ++k) mean_len += (double)qlens[k];
-        mean_len /= (double)nq;
-        const double res16 = (double)std::max<uint64_t>(db->residues16, 1);
-        const double rate = sample_len * res16 / std::max(best_ms, 1e-3);  // cells per ms
-        const double per_wg = rate / std::max<uint32_t>(best_slots, 1);
-        const double item_max = (double)longest * (double)db->max_ncols16 * 128.0;
-        const double overhead = std::max(best_wall - best_ms, 0.0);
-        uint32_t chunk = 1024;
-        for (uint32_t c = 8; c <= 1024; c *= 2) {
-            const double ideal = (double)c * mean_len * res16 / rate;
-            const double span = overhead + std::max(ideal, item_max / per_wg);
-            if (ideal >= 0.97 * span) {
-                chunk = c;
-                break;
-            }
-        }
+        // Submission size: the whole call in one persistent launch, up to 1,024 queries (and no
+        // fewer than the scratch's default). Within a launch the queue keeps every workgroup
+        // busy to the end, longest items first; each further submission pays the tail of its
+        // longest item again (256 protein queries x 1M targets: 16-query submissions ran 13%
+        // slower than one).
+        uint32_t chunk = 1;
+        while (chunk < nq && chunk < 1024) chunk *= 2;
+        chunk = std::max<uint32_t>(chunk, (uint32_t)best.batch_chunk);
         best.batch_tile_rows = pick.batch_tile_rows;
         best.batch_wgs_per_cu = pick.batch_wgs_per_cu;
         best.batch_pairs = pick.batch_pairs;

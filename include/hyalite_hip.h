@@ -303,11 +303,12 @@ int hy_scratch_get_tuning(const hy_scratch* s, hy_tuning* out);
  * shapes releases its reservation, which the next batched scan makes again. */
 int hy_scratch_set_tuning(hy_scratch* s, const hy_tuning* t);
 
-/* Autotuner: times candidate shapes on the given sample queries (at most 8 are used, spread over
- * their length range) against the whole database and keeps the fastest in the scratch. forms:
- * bit 0 = one-query scans (tile rows x waves), bit 1 = batched form (tile rows x resident
- * workgroups x, in local mode, query pairs or target pairs per lane; the submission size is the smallest whose predicted makespan, from the measured
- * rates, is within 3% of the ideal). With HYALITE_HIP_TUNE_CACHE naming a file, results are
+/* Autotuner: times candidate shapes on the given queries against the whole database and keeps
+ * the fastest in the scratch. forms: bit 0 = one-query scans (tile rows x waves, timed on 3
+ * queries spread over the length range), bit 1 = batched form (tile rows x resident workgroups
+ * x, in local mode, query pairs or target pairs per lane, timed on 8 adjacent pairs of the
+ * length-sorted queries, which pair in the lanes as the whole batch does; the submission size
+ * becomes the whole call, up to 1,024 queries, and no less than the scratch's default). With HYALITE_HIP_TUNE_CACHE naming a file, results are
  * looked up there first (key: device, mode, alphabet, database shape, sample lengths) and
  * appended after tuning. `out` (optional) receives the chosen shapes. */
 int hy_scratch_autotune(const hy_db* db, hy_scratch* s, const uint8_t* const* queries,
