@@ -436,7 +436,7 @@ hipError_t launch_inter16_batch(int mode, const Inter16Args& a, const BatchArgs&
 int inter16_tile_rows(int q);
 // LDS bytes of the pair profile for an alphabet of al_eff symbols (plus the padding symbol).
 uint32_t inter16_lds_bytes(int al_eff, int q);
-// LDS bytes of the per-symbol profile used by one-wavefront workgroups.
+// LDS bytes of the per-symbol profile (one-wavefront workgroups of larger alphabets).
 uint32_t inter16p_lds_bytes(int al_eff, int q);
 hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream);
 // Batched form: zero n keys and the queue counter; write a Result per query from its key.
@@ -489,12 +489,23 @@ inline int inter16_pipe_rows(int m) {
     if (m <= 32 * kPipeMaxWaves) return 32;
     return 0;
 }
+// One-wavefront workgroups of the packed kernel keep a residue-pair profile of their tile (the
+// two targets' scores of a query row in one word, read with no per-lane pairing) when it takes at
+// most kPairProfileSmallBytes of LDS, which holds for alphabets of up to 9 symbols (DNA, with
+// its padding symbol: 5 x 5 pairs); larger alphabets keep the per-symbol profile and pair per lane.
+constexpr uint32_t kPairProfileSmallBytes = 16u * 1024u;
+inline bool inter16_pair_small(int al_eff, int q) {
+    return inter16_lds_bytes(al_eff, q) <= kPairProfileSmallBytes;
+}
+// Words of one wavefront's tile profile for q-row tiles.
+inline uint32_t inter16_profile_words(int al_eff, int q) {
+    return (inter16_pair_small(al_eff, q) ? inter16_lds_bytes(al_eff, q)
+                                          : inter16p_lds_bytes(al_eff, q)) / 4;
+}
 // LDS bytes of a pipelined workgroup of W wavefronts with q-row tiles (profiles, double buffers,
 // the fold of the running maxima); the host uses the kernel only within 64 KB.
 inline uint32_t inter16_pipe_lds_bytes(int al_eff, int q, int W) {
-    const uint32_t rows = (uint32_t)q / 2;
-    const uint32_t rd = rows % 8 == 4 ? rows : rows + 4;
-    return (uint32_t)W * ((uint32_t)al_eff + 1) * rd * 4u + (uint32_t)(W - 1) * 4096u +
+    return (uint32_t)W * inter16_profile_words(al_eff, q) * 4u + (uint32_t)(W - 1) * 4096u +
            (uint32_t)W * 256u;
 }
 template <int MODE>

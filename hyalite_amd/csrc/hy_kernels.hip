@@ -259,7 +259,7 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                                              int nA, int nB, uint2* sp, const uint2* wp,
                                              uint32_t* pp, PairRun& st, Pipe pipe = Pipe()) {
     using F = MF<MODE>;
-    static_assert(!PIPE || (!PAIR && !ENDS), "pipelined tiles: per-symbol profile, scores only");
+    static_assert(!PIPE || !ENDS, "pipelined tiles: scores only");
     constexpr int S = QT + 4;
     // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
     // different symbols start in different 4-bank groups.
@@ -716,7 +716,7 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inte
 // its bottom row from LDS (Pipe). Every wavefront keeps the running maxima of its own tile; they
 // are folded through LDS and wavefront 0 writes the scores and the argmax key. LDS: W profiles
 // of pp_words words, then (W - 1) double buffers of 512 uint2, then W x 64 words for the fold.
-template <int MODE, int Q>
+template <int MODE, int Q, bool PAIR>
 __global__ __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_pipe(
     Inter16Args a, uint32_t pp_words) {
@@ -745,26 +745,26 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_pipe
     const uint2* wp = a.words + g.word_off + lane;
     const int nfull = m / Q, rem = m - nfull * Q;
     if (wv < nfull) {
-        inter16_tile_at<MODE, Q, 1, false, false, true>(a, g, true, a.query, m, wv * Q + 1, wv == 0,
+        inter16_tile_at<MODE, Q, 1, PAIR, false, true>(a, g, true, a.query, m, wv * Q + 1, wv == 0,
                                                         rem == 0 && wv == nfull - 1, nA, nB,
                                                         nullptr, wp, pp, st, pipe);
     } else {
         const int i0 = nfull * Q + 1;
         const bool first = nfull == 0;
         if (rem <= 8)
-            inter16_tile_at<MODE, 8, 1, false, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, 8, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
                                                             true, nA, nB, nullptr, wp, pp, st,
                                                             pipe);
         else if (Q > 16 && rem <= 16)
-            inter16_tile_at<MODE, 16, 1, false, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, 16, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
                                                              true, nA, nB, nullptr, wp, pp, st,
                                                              pipe);
         else if (Q > 24 && rem <= 24)
-            inter16_tile_at<MODE, 24, 1, false, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, 24, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
                                                              true, nA, nB, nullptr, wp, pp, st,
                                                              pipe);
         else
-            inter16_tile_at<MODE, Q, 1, false, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, Q, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
                                                             true, nA, nB, nullptr, wp, pp, st,
                                                             pipe);
     }
@@ -797,8 +797,9 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_pipe
 // Persistent batch kernel: each one-wavefront workgroup takes (group, query) items from a device
 // counter until the queue is empty. Items run group-major over groups in descending column count
 // and, within a group, queries in descending length, so the longest work starts first. Each
-// workgroup owns one boundary-row slot of bt.slot_stride entries.
-template <int MODE, int Q>
+// workgroup owns one boundary-row slot of bt.slot_stride entries. PAIR: the residue-pair
+// profile (small alphabets), else the per-symbol one.
+template <int MODE, int Q, bool PAIR>
 __global__ __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_batch(Inter16Args a,
                                                                                    BatchArgs bt) {
@@ -816,8 +817,8 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_batc
         const uint32_t gix = a.group_first + item / nq;
         const uint32_t qi = bt.qorder[item % nq];
         const Group g = a.groups[gix];
-        inter16_group<MODE, Q, 1, false>(a, g, true, bt.queries + bt.qoff[qi], (int)bt.qlen[qi],
-                                         slot, bt.keys + qi, pp);
+        inter16_group<MODE, Q, 1, PAIR>(a, g, true, bt.queries + bt.qoff[qi], (int)bt.qlen[qi],
+                                        slot, bt.keys + qi, pp);
     }
 }
 
@@ -1225,18 +1226,25 @@ template <int MODE, int Q>
 hipError_t launch_inter16_pipe_q(const Inter16Args& a, hipStream_t st) {
     const int W = (a.m + Q - 1) / Q;
     if (W < 2 || W > kPipeMaxWaves) return hipErrorInvalidValue;
-    const uint32_t pp_words = inter16p_lds_bytes(a.al_eff, Q) / 4;
+    const uint32_t pp_words = inter16_profile_words(a.al_eff, Q);
     const size_t lds = ((size_t)W * pp_words + (size_t)(W - 1) * 1024 + (size_t)W * 64) * 4;
-    hipLaunchKernelGGL((k_inter16_pipe<MODE, Q>), dim3(a.ngroups), dim3(64 * W), lds, st, a,
-                       pp_words);
+    if (inter16_pair_small(a.al_eff, Q))
+        hipLaunchKernelGGL((k_inter16_pipe<MODE, Q, true>), dim3(a.ngroups), dim3(64 * W), lds,
+                           st, a, pp_words);
+    else
+        hipLaunchKernelGGL((k_inter16_pipe<MODE, Q, false>), dim3(a.ngroups), dim3(64 * W), lds,
+                           st, a, pp_words);
     return hipGetLastError();
 }
 
 template <int MODE, int Q>
 hipError_t launch_inter16_batch_q(const Inter16Args& a, const BatchArgs& b, uint32_t wgs,
                                   hipStream_t st) {
-    const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
-    hipLaunchKernelGGL((k_inter16_batch<MODE, Q>), dim3(wgs), dim3(64), lds, st, a, b);
+    const uint32_t lds = inter16_profile_words(a.al_eff, Q) * 4u;
+    if (inter16_pair_small(a.al_eff, Q))
+        hipLaunchKernelGGL((k_inter16_batch<MODE, Q, true>), dim3(wgs), dim3(64), lds, st, a, b);
+    else
+        hipLaunchKernelGGL((k_inter16_batch<MODE, Q, false>), dim3(wgs), dim3(64), lds, st, a, b);
     return hipGetLastError();
 }
 
@@ -1264,10 +1272,17 @@ hipError_t launch_sys32_m(bool ends, bool winner, const SysArgs& a, uint32_t nbl
     return launch_sys32_mr<MODE, 8>(ends, winner, a, nblocks, st, nq);
 }
 
-// waves: 1 = one-wavefront workgroups with the per-symbol profile; 4 or 8 = workgroups sharing
-// the residue-pair profile.
+// waves: 1 = one-wavefront workgroups, each with its own profile (the residue-pair profile for
+// small alphabets, the per-symbol one otherwise); 4 or 8 = workgroups sharing the residue-pair
+// profile.
 template <int MODE>
 hipError_t launch_inter16_m(const Inter16Args& a, int q, int waves, hipStream_t st) {
+    if (waves <= 1 && inter16_pair_small(a.al_eff, q)) {
+        // Small alphabets: the tile's residue-pair profile fits a few KB of the wavefront's LDS.
+        if (q <= 16) return launch_inter16_qw<MODE, 16, 1, true>(a, st);
+        if (q <= 24) return launch_inter16_qw<MODE, 24, 1, true>(a, st);
+        return launch_inter16_qw<MODE, 32, 1, true>(a, st);
+    }
     if (waves <= 1) {
         if (q <= 16) return launch_inter16_qw<MODE, 16, 1, false>(a, st);
         if (q <= 24) return launch_inter16_qw<MODE, 24, 1, false>(a, st);
@@ -1307,20 +1322,26 @@ template <int MODE>
 uint32_t inter16_batch_occupancy_m(int q, int al_eff) {
     int per_cu = 0;
     const int rows = inter16_tile_rows(q);
-    const size_t lds = inter16p_lds_bytes(al_eff, rows) + sizeof(uint32_t);
+    const size_t lds = inter16_profile_words(al_eff, rows) * 4u + sizeof(uint32_t);
     hipError_t e = hipErrorInvalidValue;
     auto occ = [&](const void* f) {
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, 64, lds);
     };
-    if (rows == 16) occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 16>));
-    else if (rows == 24) occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 24>));
-    else occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 32>));
+    if (inter16_pair_small(al_eff, rows)) {
+        if (rows == 16) occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 16, true>));
+        else if (rows == 24) occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 24, true>));
+        else occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 32, true>));
+    } else {
+        if (rows == 16) occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 16, false>));
+        else if (rows == 24) occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 24, false>));
+        else occ(reinterpret_cast<const void*>(k_inter16_batch<MODE, 32, false>));
+    }
     if (e != hipSuccess || per_cu <= 0) per_cu = 2;
     return (uint32_t)per_cu;
 }
 
 // End tracking in the packed one-query kernel (scan_all with ScoreEnd): one wavefront per
-// workgroup with the per-symbol profile (the one-query default shape).
+// workgroup with its own profile (the one-query default shape).
 template <int MODE>
 hipError_t launch_inter16_ends_m(const Inter16Args& a, hipStream_t st) {
     if (a.ngroups == 0) return hipSuccess;
@@ -1330,9 +1351,15 @@ hipError_t launch_inter16_ends_m(const Inter16Args& a, hipStream_t st) {
     // minimum. The other modes track ends at their answer cells only (the last row of the last
     // tile, the end column), so their tile keeps the score-only shape.
     constexpr int Q = MODE == kSW ? 24 : HY_ENDS_TILE_NONLOCAL;
-    const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
-    hipLaunchKernelGGL((k_inter16<MODE, Q, 1, false, true>), dim3(a.ngroups), dim3(64), lds, st,
-                       a);
+    if (inter16_pair_small(a.al_eff, Q)) {
+        const uint32_t lds = inter16_lds_bytes(a.al_eff, Q);
+        hipLaunchKernelGGL((k_inter16<MODE, Q, 1, true, true>), dim3(a.ngroups), dim3(64), lds,
+                           st, a);
+    } else {
+        const uint32_t lds = inter16p_lds_bytes(a.al_eff, Q);
+        hipLaunchKernelGGL((k_inter16<MODE, Q, 1, false, true>), dim3(a.ngroups), dim3(64), lds,
+                           st, a);
+    }
     return hipGetLastError();
 }
 

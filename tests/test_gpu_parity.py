@@ -132,7 +132,7 @@ def test_scoring_parameter_grid():
                 check(seqs, scheme, mode, hy.SearchType.ScoreEnd, queries, mql=9)
 
 
-@pytest.mark.parametrize("al", [16, 17, 24, 64, 255, 256])
+@pytest.mark.parametrize("al", [8, 16, 17, 24, 64, 255, 256])
 def test_alphabet_sizes(al):
     """tests/sweeps.rs:238-262 plus protein-sized, large and byte-exhausting alphabets (256 has
     no free padding symbol and runs entirely in the systolic kernel)."""
@@ -144,6 +144,43 @@ def test_alphabet_sizes(al):
     queries = [lcg_seq(state, 6 + lcg_next(state) % 30, al) for _ in range(3)]
     for mode in ALL_MODES:
         check(seqs, (al, mat, 2, 1), mode, hy.SearchType.ScoreEnd, queries)
+
+
+@pytest.mark.parametrize("mode", [MODE_SW, MODE_NW, MODE_OV])
+def test_small_alphabet_profiles_every_tile_height(mode, monkeypatch):
+    """One-wavefront tiles hold a residue-pair profile when it fits 16 KB of LDS
+    ((al + 1)^2 x (rows + 4) words) and the per-symbol profile otherwise. Alphabets 4, 9, 10 and
+    12 put 32-, 24- and 16-row tiles on either side of that limit (10: pair profile at 16 and 24
+    rows only; 12: at 16 rows only); every tile height of the one-query kernel (one wavefront
+    per group, and pipelined), the batch kernel and the end-tracking scan equals the oracle."""
+    rng = np.random.default_rng(1500 + mode)
+    for al in (4, 9, 10, 12):
+        mat = [int(rng.integers(-3, 4)) for _ in range(al * al)]
+        for i in range(al):
+            mat[i * al + i] = 2 + i % 3
+        seqs = [rng.integers(0, al, size=int(n), dtype=np.uint8)
+                for n in rng.integers(0, 200, 400)]
+        qs = [rng.integers(0, al, size=int(m), dtype=np.uint8) for m in (7, 33, 70)]
+        flat, off = orc.flatten(seqs)
+        db = build(seqs, (al, mat, 3, 1), mode, hy.SearchType.ScoreEnd, 70)
+        sc = hy.Scratch(db)
+        want = [orc.scan_flat(flat, off, q, al, mat, 3, 1, mode, True, threads=4) for q in qs]
+        ends = [orc.scan_all_flat(flat, off, q, al, mat, 3, 1, mode, threads=4) for q in qs]
+        for rows in (16, 24, 32):
+            sc.set_tuning(hy.Tuning(tile_rows=rows, batch_tile_rows=rows, batch_pairs=0))
+            monkeypatch.setenv("HYALITE_HIP_PIPE_ROWS", str(rows))
+            for pipe in ("0", "1"):
+                monkeypatch.setenv("HYALITE_HIP_PIPE", pipe)
+                for q, w, (ws, wqe, wte) in zip(qs, want, ends):
+                    h = db.scan(sc, q)
+                    assert (h.score, h.db_index, h.query_end, h.target_end) == w, (al, rows, pipe)
+                    assert np.array_equal(db.scan_scores_array(sc, q), ws), (al, rows, pipe)
+                    s, qe, te = db.scan_all_arrays(sc, q)
+                    assert (np.array_equal(s, ws) and np.array_equal(qe, wqe) and
+                            np.array_equal(te, wte)), (al, rows, pipe)
+            got = db.scan_batch(sc, qs)
+            assert [(h.score, h.db_index, h.query_end, h.target_end) for h in got] == want, \
+                (al, rows)
 
 
 def test_score_width_boundaries():
