@@ -245,6 +245,138 @@ __device__ __forceinline__ int2 pass_top(const int2* sp, bool first, int col, in
     return col <= n ? sp[col] : make_int2(0, kNeg32);
 }
 
+// Lean single-pass sweep of a local-mode pair by one wavefront (the query's rows fit one pass of
+// 64 R rows; a winner pass of a short query is its usual case). Lane l holds rows lR+1 .. lR+R and
+// at step s works column j = s - l + 1. Against the general sweep below it drops every per-step
+// test and hand-off it can:
+//  * no column-range mask: columns j < 1 read the padding symbol (sub = kPad32), for which the
+//    local recurrence recomputes the left border (H = 0, H - go = E = F = -go) from the border
+//    tops every lane starts with, and columns j > n read it too, so their cells come from real
+//    cells through gaps only, never above them (strict > keeps the first real column on ties);
+//  * lane 0's top input is the DP border (H = 0, F = -inf), held in the lane-0 slot of the
+//    top registers, which the lane shift never writes; no ring, no barriers;
+//  * the residue of column j comes from a 256-entry ring in LDS (slot x & 255 holds residue x,
+//    the padding symbol outside 0..n-1), refilled three epochs of 64 steps ahead, read two
+//    steps ahead; the substitution scores of step s + 1 are read during step s.
+// smat: (al + 1) x (al + 1) entries, the matrix plus go with row and column al the padding.
+// Rows past m record nothing (their row best starts at INT_MAX), and a row best starts at 0:
+// score-0 ends are the border's (0, 0, 0), which wins every tie.
+//
+// KNOWN (a winner pass, whose best score S > 0 the scan has already found): a row records the
+// first column where H reaches S, which needs no running maximum, and the sweep stops early.
+// A first hit in lane l at step s is at column s - l + 1; the answer's column is at most that
+// of any hit, so once the last lane holding query rows (L) has passed the smallest column hit
+// so far, every cell that can be the answer has been swept. The hitting lane with the largest
+// index holds that column (the ballot's last set bit), so the bound needs no reduction.
+template <int MODE, int R, bool ENDS, bool KNOWN>
+__device__ __forceinline__ void band_lean(const uint8_t* query, const uint8_t* t, int m, int n,
+                                          int al, int go, int ge, const int* smat, uint8_t* tres,
+                                          int S, int& bs, int& bj, int& bi) {
+    static_assert(MF<MODE>::local, "the lean sweep relies on the local clamp at 0");
+    static_assert(!KNOWN || ENDS, "a known score only shortens an end search");
+    const int lane = threadIdx.x & 63;
+    const int A1 = al + 1;
+    const int r0 = lane * R + 1;
+    const int ngo = wsub(0, go);
+    int qb[R], HG[R], E[R], rb[R], rj[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = r0 + r;
+        qb[r] = (i <= m ? (int)query[i - 1] : al) * A1;
+        HG[r] = ngo;
+        E[r] = kNeg32;
+        // KNOWN: the value a row looks for (rows past m look for none: H is never negative).
+        rb[r] = KNOWN ? (i <= m ? S : -1) : (i <= m ? 0 : INT_MAX);
+        rj[r] = INT_MAX;
+    }
+    auto res = [&](int x) HY_INLINE { return (x >= 0 && x < n) ? (int)t[x] : al; };
+    // Epochs 0-2 and the padding before column 1 (slots 192-255 stand for x = -64 .. -1).
+    tres[lane] = (uint8_t)res(lane);
+    tres[64 + lane] = (uint8_t)res(64 + lane);
+    tres[128 + lane] = (uint8_t)res(128 + lane);
+    tres[192 + lane] = (uint8_t)al;
+    int rf = res(192 + lane);  // epoch 3, stored at the end of epoch 0
+    const int last = (m + R - 1) / R - 1;  // the last lane holding query rows
+    const int steps = n + last;
+    int stop = steps;
+    // Registers alternate between even and odd steps so that nothing is copied between steps
+    // (a copy of a register with an LDS read in flight would wait for the read): the tops of
+    // step s sit in T[s & 1] and its diagonal input in T[(s + 1) & 1], the lane shift writes
+    // step s + 1's tops over the latter; substitution scores and residues ping-pong the same way.
+    int TA = ngo, TB = ngo, topF = kNeg32;
+    int jv = 1 - lane;
+    int subA[R], subB[R];
+    {
+        const int c0 = tres[(0 - lane) & 255];
+#pragma unroll
+        for (int r = 0; r < R; ++r) subA[r] = smat[qb[r] + c0];
+    }
+    int cA = 0, cB = tres[(1 - lane) & 255];
+    // One step s: sub holds its scores, cuse the residue of step s + 1 (its scores go to subn),
+    // cload receives the residue of step s + 2; top and diag as above.
+    auto step = [&](int s, const int* sub, int* subn, int cuse, int& cload, int& top,
+                    int& diagin) HY_INLINE {
+        cload = tres[(s + 2 - lane) & 255];
+#pragma unroll
+        for (int r = 0; r < R; ++r) subn[r] = smat[qb[r] + cuse];
+        int f = topF, hgup = top, dg = diagin;
+        int hv[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int diag = wadd(dg, sub[r]);
+            const int e = max(HG[r], wsub(E[r], ge));
+            f = max(hgup, wsub(f, ge));
+            const int h = max(max(max(diag, e), f), 0);
+            dg = HG[r];
+            E[r] = e;
+            HG[r] = wsub(h, go);
+            hgup = HG[r];
+            if (KNOWN) {
+                hv[r] = h;
+            } else if (ENDS) {
+                const bool up = h > rb[r];
+                rb[r] = up ? h : rb[r];
+                rj[r] = up ? jv : rj[r];
+            } else {
+                bs = max(bs, h);
+            }
+        }
+        if (KNOWN) {
+            // Lanes with a row at S: one wavefront-wide compare mask per row (v_cmp into SGPRs).
+            unsigned long long hits = 0ull;
+#pragma unroll
+            for (int r = 0; r < R; ++r) hits |= __builtin_amdgcn_sicmp(hv[r], rb[r], 32);
+            if (hits != 0ull) {
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    rj[r] = (hv[r] == rb[r] && rj[r] == INT_MAX) ? jv : rj[r];
+                const int lmax = 63 - __builtin_clzll(hits);
+                stop = min(stop, s + 1 - lmax + last);
+            }
+        }
+        // Step s + 1's tops: lane l takes lane l-1's bottom row; lane 0 keeps the border.
+        diagin = __builtin_amdgcn_update_dpp(diagin, HG[R - 1], 0x138, 0xf, 0xf, false);
+        topF = __builtin_amdgcn_update_dpp(topF, f, 0x138, 0xf, 0xf, false);
+        ++jv;
+        if ((s & 63) == 63) {
+            // Epoch e = s >> 6 ends: epoch e + 3 replaces epoch e - 1, the next is loaded.
+            tres[(s + 129 + lane) & 255] = (uint8_t)rf;
+            rf = res(s + 193 + lane);
+        }
+    };
+    int s = 0;
+    for (; s + 1 < stop; s += 2) {
+        step(s, subA, subB, cB, cA, TA, TB);
+        step(s + 1, subB, subA, cA, cB, TB, TA);
+    }
+    if (s < stop) step(s, subA, subB, cB, cA, TA, TB);
+    if (ENDS) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            if (rj[r] != INT_MAX) consider(bs, bj, bi, rb[r], rj[r], r0 + r);
+    }
+}
+
 template <int MODE, int R, int W, int OUT>
 __global__ __launch_bounds__(64 * W) void k_band32(BandArgs a) {
     using F = MF<MODE>;
@@ -266,9 +398,22 @@ __global__ __launch_bounds__(64 * W) void k_band32(BandArgs a) {
     const uint8_t* t = a.targets + a.toff[pid];
     const int n = (int)(a.toff[pid + 1] - a.toff[pid]);
     const int m = winner ? a.wm : (int)a.qlen[pid], go = a.go, ge = a.ge, al = a.al;
-    // The matrix plus go, row al the padding row for query rows past m.
-    for (int k = threadIdx.x; k < (al + 1) * al; k += 64 * W)
-        smat[k] = k < al * al ? a.matrix[k] + go : kPad32;
+    constexpr int P = 64 * R * W;
+    const int npass = n > 0 ? (m + P - 1) / P : 0;
+    // One wavefront with the whole query in one pass, local mode: the lean sweep (band_lean).
+    constexpr bool LEAN_OK = W == 1 && F::local && !COLMAX;
+    const bool lean = LEAN_OK && npass == 1;
+    if (lean) {
+        // The matrix plus go, row and column al the padding symbol.
+        for (int k = threadIdx.x; k < (al + 1) * (al + 1); k += 64 * W) {
+            const int r = k / (al + 1), c = k - r * (al + 1);
+            smat[k] = r < al && c < al ? a.matrix[r * al + c] + go : kPad32;
+        }
+    } else {
+        // The matrix plus go, row al the padding row for query rows past m.
+        for (int k = threadIdx.x; k < (al + 1) * al; k += 64 * W)
+            smat[k] = k < al * al ? a.matrix[k] + go : kPad32;
+    }
     if (COLMAX) {
         for (int k = threadIdx.x; k < n; k += 64 * W) a.colmax[k] = 0;
         __threadfence();  // the zeros reach the device-scope atomics below
@@ -278,10 +423,20 @@ __global__ __launch_bounds__(64 * W) void k_band32(BandArgs a) {
     int bs = INT_MIN, bj = INT_MAX, bi = INT_MAX;
     if (!COLMAX && threadIdx.x == 0) border_best<MODE>(m, n, go, ge, bs, bj, bi);
 
-    constexpr int P = 64 * R * W;
-    const int npass = n > 0 ? (m + P - 1) / P : 0;
+    if constexpr (LEAN_OK) {
+        uint8_t* tres = reinterpret_cast<uint8_t*>(&ring[0][0]);
+        const int S = (ENDS && winner && wk != 0ull) ? key_score(wk) : 0;
+        if (lean && S > 0) {
+            if constexpr (ENDS)
+                band_lean<MODE, R, true, true>(query, t, m, n, al, go, ge, smat, tres, S, bs, bj,
+                                               bi);
+        } else if (lean) {
+            band_lean<MODE, R, ENDS, false>(query, t, m, n, al, go, ge, smat, tres, 0, bs, bj,
+                                            bi);
+        }
+    }
     int2* sp = a.spill + (size_t)blockIdx.x * a.spill_stride;
-    for (int p = 0; p < npass; ++p) {
+    for (int p = 0; p < (lean ? 0 : npass); ++p) {
         const bool first = p == 0;
         // Lane l takes column n at step n + l - 1. A lone wavefront's last pass ends when its
         // last lane holding query rows has (no other wavefront shares its barriers, and no later
@@ -465,7 +620,7 @@ __global__ __launch_bounds__(64 * W) void k_band32(BandArgs a) {
 
 template <int R, int W, int OUT>
 hipError_t launch_band_rwo(const BandArgs& a, uint32_t nblocks, hipStream_t st) {
-    const size_t lds = (size_t)(a.al + 1) * a.al * sizeof(int);
+    const size_t lds = (size_t)(a.al + 1) * (a.al + 1) * sizeof(int);
     hipLaunchKernelGGL((k_band32<HY_KMODE, R, W, OUT>), dim3(nblocks), dim3(64 * W), lds, st, a);
     return hipGetLastError();
 }
