@@ -593,6 +593,8 @@ def load_library():
     lib.hy_scratch_batch_profile_read.restype = ctypes.c_int
     lib.hy_scratch_batch_profile_read.argtypes = [vp, ctypes.POINTER(ctypes.c_double), _u64p,
                                                   ctypes.c_int]
+    lib.hy_scratch_last_kernel.restype = ctypes.c_int
+    lib.hy_scratch_last_kernel.argtypes = [vp, _i32p]
     lib.hy_scratch_get_tuning.restype = ctypes.c_int
     lib.hy_scratch_get_tuning.argtypes = [vp, ctypes.POINTER(_HyTuning)]
     lib.hy_scratch_set_tuning.restype = ctypes.c_int
@@ -1065,6 +1067,15 @@ class Scratch:
         load_library().hy_scratch_batch_profile_read(self._h, ctypes.byref(a), ctypes.byref(n),
                                                      1 if reset else 0)
         return a.value, n.value
+
+    def last_kernel(self) -> str:
+        """The packed kernel of this scratch's last one-query scan: "none", "packed",
+        "pipelined", "chained" or "ends" (hy_scratch_last_kernel)."""
+        k = ctypes.c_int32(-1)
+        rc = load_library().hy_scratch_last_kernel(self._h, ctypes.byref(k))
+        if rc != 0:
+            raise DeviceError(load_library().hy_strerror(rc).decode())
+        return {-1: "none", 0: "packed", 1: "pipelined", 2: "chained", 3: "ends"}[k.value]
 
     def tuning(self) -> Tuning:
         c = _HyTuning()

@@ -126,7 +126,7 @@ extern "C" int hy_scan_aligned(const hy_db* db, hy_scratch* s, const uint8_t* qu
                        s->stream) != hipSuccess)
         return HY_E_DEVICE;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
-    hy::collect_profile(s);
+    if ((rc = hy::scan_done(s)) != HY_OK) return rc;
     const hy::TraceOut o = *s->h_tout;
     if (o.check != 0 || o.ops_offset + o.n_ops > (uint64_t)qlen + db->mtl) return HY_E_INTERNAL;
     std::memcpy(ops, s->h_ops + o.ops_offset, (size_t)o.n_ops);

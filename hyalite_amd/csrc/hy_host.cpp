@@ -659,9 +659,21 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
         hipMalloc((void**)&s->d_result, sizeof(Result)) != hipSuccess ||
         hipHostMalloc((void**)&s->h_result, sizeof(Result)) != hipSuccess)
         return fail(HY_E_OUT_OF_MEMORY);
+    // Progress words and the ticket counter of the chained pipelined kernel (databases with few
+    // packed groups), zeroed once: every launch counts in an epoch of its own and knows how many
+    // tickets the launches before it took.
+    const uint64_t chain_words =
+        (uint64_t)std::max<size_t>(db->groups16.size(), 1) * hy::kChainMaxLinks + 1;
+    if (hipMalloc((void**)&s->d_chain, chain_words * sizeof(unsigned long long)) != hipSuccess)
+        return fail(HY_E_OUT_OF_MEMORY);
+    if (hipMemset(s->d_chain, 0, chain_words * sizeof(unsigned long long)) != hipSuccess)
+        return fail(HY_E_DEVICE);
+    if (hipHostMalloc((void**)&s->h_chain_err, sizeof(unsigned int)) != hipSuccess)
+        return fail(HY_E_OUT_OF_MEMORY);
+    *s->h_chain_err = 0u;
     s->dev_bytes = qcap + s->spill_entries * sizeof(int2) + sys_entries * sizeof(int2) +
                    n * (sizeof(int32_t) + 2 * sizeof(int64_t)) + sizeof(unsigned long long) +
-                   sizeof(Result);
+                   sizeof(Result) + chain_words * sizeof(unsigned long long);
     *out = s;
     return HY_OK;
 }
@@ -753,6 +765,98 @@ int pipe_rows(const hy_db* db, int m, uint32_t ngroups) {
     return q;
 }
 
+// Shape of the chained pipelined kernel for a score-only scan of `ngroups` packed groups with a
+// query of m rows, or rows = 0 for none. The pipelined kernel keeps a group on one CU, and the
+// one-wavefront kernel a group on one SIMD, so a database of few groups leaves most of the chip
+// idle and its longest group sets the time; chaining spreads a group's query tiles over `chunks`
+// workgroups (CUs) of `waves` wavefronts. The shape is the first of (tile rows x waves) 8x4, 8x8,
+// 16x8, 16x4, 24x4, 24x8, 32x4, 32x8 that gives 2-8 chunks within 3,200 wavefronts (12.5 per
+// CU); a query too long for the pipelined kernel takes the valid shape of fewest wavefronts when
+// none fits that budget. Measured on DNA databases of 1,280-20,000 targets (10-157 groups) with
+// queries of 40-400 rows, that rule picks the fastest shape or one within 5% of it; for 10,000
+// targets and a 100-row query (8x4, 4 chunks) 0.169 ms per scan against 0.218 ms pipelined, for
+// 400 rows (16x8, 4 chunks) 0.46 ms against 2.57 ms with one wavefront per group. Databases of
+// more than 2 x CUs groups keep the other kernels. HYALITE_HIP_CHAIN=0 turns it off, =1 takes a
+// valid shape whatever its size; HYALITE_HIP_CHAIN_ROWS (8, 16, 24, 32) and
+// HYALITE_HIP_CHAIN_WAVES (1-8) fix the shape, HYALITE_HIP_CHAIN_SLACK (1-64) and
+// HYALITE_HIP_CHAIN_EVERY (1-16) the link's reader lead and publication interval (tests and
+// tuning).
+struct ChainShape {
+    int rows = 0, waves = 0, tiles = 0, chunks = 0, slack = 4, every = 1;
+};
+ChainShape chain_shape(const hy_db* db, int m, uint32_t ngroups) {
+    ChainShape c;
+    const char* e = std::getenv("HYALITE_HIP_CHAIN");
+    const int mode_env = e ? std::atoi(e) : -1;
+    if (mode_env == 0 || ngroups == 0 || m <= 8) return c;
+    const uint32_t cus = hy::device_cus();
+    if (mode_env != 1 && ngroups > 2 * cus) return c;
+    int want_rows = 0, want_waves = 0;
+    if (const char* r = std::getenv("HYALITE_HIP_CHAIN_ROWS")) {
+        const int v = std::atoi(r);
+        if (v == 8 || v == 16 || v == 24 || v == 32) want_rows = v;
+    }
+    if (const char* w = std::getenv("HYALITE_HIP_CHAIN_WAVES")) {
+        const int v = std::atoi(w);
+        if (v >= 1 && v <= 8) want_waves = v;
+    }
+    if (const char* sl = std::getenv("HYALITE_HIP_CHAIN_SLACK")) {
+        const int v = std::atoi(sl);
+        if (v >= 1 && v <= 64) c.slack = v;
+    }
+    if (const char* ev = std::getenv("HYALITE_HIP_CHAIN_EVERY")) {
+        const int v = std::atoi(ev);
+        if (v >= 1 && v <= 16) c.every = v;
+    }
+    static const int kShapes[][2] = {{8, 4},  {8, 8},  {16, 8}, {16, 4},
+                                     {24, 4}, {24, 8}, {32, 4}, {32, 8}};
+    const uint64_t g8 = (uint64_t)(ngroups + 7) / 8 * 8;
+    const uint64_t budget = 3200;  // wavefronts
+    const bool no_pipe = hy::inter16_pipe_rows(m) == 0;
+    struct Cand {
+        int rows = 0, waves = 0, tiles = 0, chunks = 0;
+        uint64_t total = 0;  // wavefronts of the launch
+    };
+    // A shape is valid when it gives 2-8 chunks and its workgroup's LDS fits 64 KB.
+    auto make = [&](int rows, int waves, Cand& out) {
+        const int tiles = (m + rows - 1) / rows;
+        waves = std::min(waves, tiles);
+        const int chunks = (tiles + waves - 1) / waves;
+        if (chunks < 2 || chunks > hy::kChainMaxChunks) return false;
+        const uint32_t pp_words = hy::inter16_profile_words((int)db->al_eff, rows);
+        if (((uint64_t)waves * pp_words + (uint64_t)(waves - 1) * 1024 + (uint64_t)waves * 64) *
+                4 > 64u * 1024u)
+            return false;
+        out.rows = rows;
+        out.waves = waves;
+        out.tiles = tiles;
+        out.chunks = chunks;
+        out.total = g8 * (uint64_t)chunks * (uint64_t)waves;
+        return true;
+    };
+    Cand pick, cd;
+    if (want_rows || want_waves) {
+        if (make(want_rows ? want_rows : 16, want_waves ? want_waves : 4, cd) &&
+            (mode_env == 1 || cd.total <= budget || no_pipe))
+            pick = cd;
+    } else {
+        for (const auto& sh : kShapes)
+            if (make(sh[0], sh[1], cd) && (mode_env == 1 || cd.total <= budget)) {
+                pick = cd;
+                break;
+            }
+        if (pick.rows == 0 && no_pipe)
+            for (const auto& sh : kShapes)
+                if (make(sh[0], sh[1], cd) && (pick.rows == 0 || cd.total < pick.total)) pick = cd;
+    }
+    if (pick.rows == 0) return c;
+    c.rows = pick.rows;
+    c.waves = pick.waves;
+    c.tiles = pick.tiles;
+    c.chunks = pick.chunks;
+    return c;
+}
+
 // Enqueue the per-target pass. ends: write (score, qend, tend) per target; otherwise write scores
 // (when `scores` is set) and/or fold the argmax key (when `key` is set).
 int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write_scores,
@@ -812,6 +916,7 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
     if (s->profile && hipEventRecord(s->ev[0], s->stream) != hipSuccess) return HY_E_DEVICE;
     if (!side32 && hy::launch_inter32((int)db->mode, ends, ia, g32, s->stream) != hipSuccess)
         return HY_E_DEVICE;
+    s->last_kernel = HY_KERNEL_NONE;
     if (run16) {
         hy::Inter16Args pa{};
         pa.groups = db->d_groups16;
@@ -836,12 +941,33 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
             pa.out_tend = s->d_tend;
             if (hy::launch_inter16_ends((int)db->mode, pa, s->stream) != hipSuccess)
                 return HY_E_DEVICE;
+            s->last_kernel = HY_KERNEL_ENDS;
+        } else if (const ChainShape cs = chain_shape(db, m, pa.ngroups); cs.rows != 0) {
+            hy::ChainArgs ca{};
+            ca.progress = s->d_chain;
+            ca.epoch = (unsigned long long)(++s->chain_epoch) << 32;
+            ca.err = s->h_chain_err;
+            ca.ticket =
+                s->d_chain + std::max<size_t>(db->groups16.size(), 1) * hy::kChainMaxLinks;
+            ca.ticket_base = s->chain_tickets;
+            ca.tiles = cs.tiles;
+            ca.chunks = cs.chunks;
+            ca.slack = cs.slack;
+            ca.every = cs.every;
+            if (hy::launch_inter16_chain((int)db->mode, pa, ca, cs.rows, cs.waves, s->stream) !=
+                hipSuccess)
+                return HY_E_DEVICE;
+            s->chain_tickets += (uint64_t)(pa.ngroups + 7) / 8 * 8 * (uint64_t)cs.chunks;
+            s->last_kernel = HY_KERNEL_CHAINED;
         } else if (const int pq = pipe_rows(db, m, pa.ngroups)) {
             if (hy::launch_inter16_pipe((int)db->mode, pa, pq, s->stream) != hipSuccess)
                 return HY_E_DEVICE;
-        } else if (hy::launch_inter16((int)db->mode, pa, s->q16, s->waves16, s->stream) !=
-                   hipSuccess) {
-            return HY_E_DEVICE;
+            s->last_kernel = HY_KERNEL_PIPELINED;
+        } else {
+            if (hy::launch_inter16((int)db->mode, pa, s->q16, s->waves16, s->stream) !=
+                hipSuccess)
+                return HY_E_DEVICE;
+            s->last_kernel = HY_KERNEL_PACKED;
         }
     }
     if (s->profile && hipEventRecord(s->ev[1], s->stream) != hipSuccess) return HY_E_DEVICE;
@@ -866,10 +992,20 @@ void collect_profile(hy_scratch* s) {
     s->pending = false;
 }
 
+int scan_done(hy_scratch* s) {
+    collect_profile(s);
+    if (s->h_chain_err && __atomic_load_n(s->h_chain_err, __ATOMIC_ACQUIRE) != 0u) {
+        *s->h_chain_err = 0u;
+        return HY_E_INTERNAL;
+    }
+    return HY_OK;
+}
+
 }  // namespace hy
 
 using hy::check_query;
 using hy::collect_profile;
+using hy::scan_done;
 using hy::inter_args;
 using hy::sys_args;
 using hy::enqueue_targets;
@@ -927,7 +1063,7 @@ extern "C" int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, siz
                        s->stream) != hipSuccess)
         return HY_E_DEVICE;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
-    collect_profile(s);
+    if ((rc = scan_done(s)) != HY_OK) return rc;
     const Result r = *s->h_result;
     if (r.check != 0) return HY_E_INTERNAL;
     *score = r.score;
@@ -954,7 +1090,7 @@ extern "C" int hy_scan_all(const hy_db* db, hy_scratch* s, const uint8_t* query,
     if (ends && target_ends) segs[nseg++] = {target_ends, s->d_tend, n * sizeof(int64_t)};
     if ((rc = hy::download(s->ring, s->device, segs, nseg, s->stream)) != HY_OK) return rc;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
-    collect_profile(s);
+    if ((rc = scan_done(s)) != HY_OK) return rc;
     if (!ends) {
         for (size_t k = 0; k < n; ++k) {
             if (query_ends) query_ends[k] = -1;
@@ -976,7 +1112,7 @@ extern "C" int hy_scan_scores(const hy_db* db, hy_scratch* s, const uint8_t* que
                        hipMemcpyDeviceToHost, s->stream) != hipSuccess)
         return HY_E_DEVICE;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
-    collect_profile(s);
+    if ((rc = scan_done(s)) != HY_OK) return rc;
     return HY_OK;
 }
 
@@ -1351,6 +1487,12 @@ extern "C" int hy_scratch_profile_read(hy_scratch* s, double* inter_ms, double* 
         s->inter_ms = s->tail_ms = 0.0;
         s->launches = 0;
     }
+    return HY_OK;
+}
+
+extern "C" int hy_scratch_last_kernel(const hy_scratch* s, int32_t* kernel) {
+    if (!s || !kernel) return HY_E_NULL_ARGUMENT;
+    *kernel = s->last_kernel;
     return HY_OK;
 }
 

@@ -511,6 +511,28 @@ inline uint32_t inter16_pipe_lds_bytes(int al_eff, int q, int W) {
 template <int MODE>
 hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, hipStream_t st);
 hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, hipStream_t stream);
+// Chained pipelined kernel (k_inter16_chain): the query tiles of a group spread over `chunks`
+// workgroups of W wavefronts, consecutive chunks linked through the group's boundary buffer and
+// one progress word per link. At most kChainMaxChunks chunks; tile rows 8, 16, 24 or 32. The
+// launch takes (ngroups rounded up to 8) x chunks workgroup tickets.
+constexpr int kChainMaxChunks = 8;
+constexpr int kChainMaxLinks = kChainMaxChunks - 1;
+struct ChainArgs {
+    unsigned long long* progress;  // [group * kChainMaxLinks + link], zeroed once at allocation
+    unsigned long long epoch;      // launch number << 32 (every launch takes a new one)
+    unsigned int* err;             // host-visible (pinned) flag, set when a wait gave up
+    unsigned long long* ticket;    // workgroup start counter, never reset
+    unsigned long long ticket_base;  // its value when this launch starts (tickets taken before)
+    int tiles;                     // query tiles of q rows, the last one possibly shorter
+    int chunks;                    // workgroups per group: ceil(tiles / W)
+    int slack;                     // blocks a reader lets its writer publish before it starts
+    int every;                     // a writer publishes its progress every `every` blocks
+};
+template <int MODE>
+hipError_t launch_inter16_chain_m(const Inter16Args& a, const ChainArgs& c, int q, int waves,
+                                  hipStream_t st);
+hipError_t launch_inter16_chain(int mode, const Inter16Args& a, const ChainArgs& c, int q,
+                                int waves, hipStream_t stream);
 // The packed one-query kernel with end tracking: score and ends of every target of the launch's
 // groups into out_score / out_qend / out_tend.
 template <int MODE>

@@ -244,22 +244,104 @@ struct PairRun {
 // bottom row goes to the next wavefront through an LDS double buffer (block b in half b & 1,
 // entry (half * 4 + column in block) * 64 + lane) instead of HBM, and every wavefront takes the
 // same nsteps workgroup barriers, one per block step, active or not.
+//
+// Chained workgroups (k_inter16_chain) extend the pipeline across CUs: the first wavefront of a
+// workgroup after the first of its group reads its top rows from the group's boundary buffer
+// (column j at gin[j * 64]), which the previous workgroup's last wavefront writes (gout). The
+// writer publishes a count of completed four-column blocks in a 64-bit progress word,
+// (launch epoch << 32) + count; the reader polls it, one poll per block issued before the block
+// and consumed after it, and blocks only when the rows of the block after next are not yet
+// published. Counts of earlier launches carry smaller epochs, so the words need no reset between
+// launches. Every access to the rows and the word is agent-coherent (sc1: it bypasses the
+// per-XCD L2, whose agent-scope release would write back every dirty line of the L2 at each
+// publication); the writer waits for its row stores to complete (vmcnt(0)) before it stores
+// the count, and the reader issues its row loads only after a poll returned that count.
+// Workgroup-scope fences keep the compiler from moving those accesses across the protocol.
 struct Pipe {
     int wave = 0, nsteps = 0;
     const uint2* rin = nullptr;  // the previous wavefront's bottom rows (not the first tile)
     uint2* rout = nullptr;       // this wavefront's bottom rows (not the last tile)
+    const uint2* gin = nullptr;  // chained: the previous workgroup's bottom rows
+    uint2* gout = nullptr;       // chained: this workgroup's bottom rows for the next one
+    const unsigned long long* pin = nullptr;  // progress of gin
+    unsigned long long* pout = nullptr;       // progress of gout
+    unsigned long long epoch = 0;             // launch epoch << 32
+    int slack = 2;                            // blocks published before the reader starts
+    int every = 1;                            // the writer publishes every `every` blocks
+    unsigned int* err = nullptr;              // host-visible flag: a wait gave up
 };
+
+// Spin limit of a chained wait (polls of 0.5-1 us each: an L2 round trip and a short sleep):
+// seconds, far beyond any launch's wait, so that a reader always ends even if its writer never
+// published.
+constexpr uint32_t kChainSpinLimit = 1u << 22;
+
+// Agent-coherent row accesses of the chained link.
+__device__ __forceinline__ uint2 chain_load(const uint2* p) {
+    const unsigned long long v =
+        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+    return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+}
+__device__ __forceinline__ void chain_store(uint2* p, uint2 v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                       (unsigned long long)v.x | ((unsigned long long)v.y << 32),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The count of this launch in a progress word value (0 for a value of an earlier launch).
+__device__ __forceinline__ uint32_t chain_count(const Pipe& p, unsigned long long v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    const unsigned long long u = ((unsigned long long)hi << 32) | lo;
+    return u >= p.epoch ? (uint32_t)(u - p.epoch) : 0u;
+}
+
+// Wait until the progress word shows at least `need` blocks of this launch; returns the count
+// seen. On giving up, raises the host-visible flag (the launch's results are then reported as an
+// internal error) and returns `need`.
+__device__ __forceinline__ uint32_t chain_wait(const Pipe& p, const unsigned long long* word,
+                                               uint32_t need) {
+    for (uint32_t spins = 0;; ++spins) {
+        const uint32_t n = chain_count(
+            p, __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        if (n >= need) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            return n;
+        }
+        if (spins >= kChainSpinLimit) {
+            if ((threadIdx.x & 63) == 0)
+                __hip_atomic_store(p.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            return need;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Publish `count` completed blocks: the wavefront's earlier row stores complete first.
+__device__ __forceinline__ void chain_publish(const Pipe& p, unsigned long long* word,
+                                              uint32_t count) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): every vector memory access of the wave
+    if ((threadIdx.x & 63) == 0)
+        __hip_atomic_store(word, p.epoch + count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // One tile of QT query rows starting at row i0 over every column of the group: the profile of
 // those rows is staged in LDS, the rows' H/E columns live in registers, and the tile's bottom row
 // goes to `sp` (column j of this wavefront at sp[j * 64]) for the next tile unless this is the last.
-template <int MODE, int QT, int NW, bool PAIR, bool FIRST, bool ENDS = false, bool PIPE = false>
+// LINKIN: the chained reader (pipelined tiles only): the tile's top rows come from the previous
+// workgroup through pipe.gin. A role fixed at compile time, so that the block code of the other
+// wavefronts has no global-memory top rows to wait for.
+template <int MODE, int QT, int NW, bool PAIR, bool FIRST, bool ENDS = false, bool PIPE = false,
+          bool LINKIN = false>
 __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& g, bool wlive,
                                              const uint8_t* query, int m, int i0, bool lastc,
                                              int nA, int nB, uint2* sp, const uint2* wp,
                                              uint32_t* pp, PairRun& st, Pipe pipe = Pipe()) {
     using F = MF<MODE>;
     static_assert(!PIPE || !ENDS, "pipelined tiles: scores only");
+    static_assert(!LINKIN || (PIPE && !FIRST), "a chained reader is a pipelined later tile");
     constexpr int S = QT + 4;
     // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
     // different symbols start in different 4-bank groups.
@@ -536,9 +618,20 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
         }
         if (!lastc) {
             if constexpr (PIPE) {
-                uint2* ro = pipe.rout + (size_t)((jb >> 2) & 1) * 256 + (threadIdx.x & 63);
+                if (pipe.gout) {
+                    // The blocks before this one were stored a block ago: publishing them here,
+                    // before this block's stores, keeps the wait for completion off those. Every
+                    // pipe.every blocks: each publication waits for the row stores in flight.
+                    if (jb > 0 && (jb >> 2) % pipe.every == 0)
+                        chain_publish(pipe, pipe.pout, (uint32_t)(jb >> 2));
 #pragma unroll
-                for (int b = 0; b < 4; ++b) ro[b * 64] = outb[b];
+                    for (int b = 0; b < 4; ++b)
+                        chain_store(pipe.gout + (size_t)(jb + b + 1) * 64, outb[b]);
+                } else {
+                    uint2* ro = pipe.rout + (size_t)((jb >> 2) & 1) * 256 + (threadIdx.x & 63);
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) ro[b * 64] = outb[b];
+                }
             } else {
 #pragma unroll
                 for (int b = 0; b < 4; ++b) sp[(size_t)(jb + b + 1) * 64] = outb[b];
@@ -551,31 +644,84 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
     for (int b = 0; b < 4; ++b) top[b] = topn[b] = make_uint2(0, 0);
     if constexpr (PIPE) {
         // Block bi at step wave + bi; its top rows were written by the previous wavefront at the
-        // step before, with a workgroup barrier between. Residue words stream one block ahead.
+        // step before, with a workgroup barrier between. Residue words stream three blocks ahead:
+        // a short tile's block computes in about the latency of a load. The steps before and
+        // after this wavefront's blocks take the barriers only, so the block loop issues the same
+        // loads every step (clamped at the last block) and the compiler can keep them in flight
+        // across steps instead of waiting for every load at the top of a block.
         const int nblk = L >> 2;
-        if (L > 0) w = wp[0];
-        for (int step = 0; step < pipe.nsteps; ++step) {
-            const int bi = step - pipe.wave;
-            if (bi >= 0 && bi < nblk) {
+        const int lastb = max(nblk - 1, 0);
+        auto word = [&](int b) HY_INLINE { return wp[(size_t)min(b, lastb) * 64]; };
+        for (int s = 0; s < pipe.wave; ++s) __syncthreads();
+        if (nblk > 0) {
+            uint2 wn2 = word(2), wn3 = make_uint2(0, 0);
+            w = word(0);
+            wn = word(1);
+            // Chained reader: block bi's top rows were loaded two blocks ahead, once a poll had
+            // shown them published, and each poll is read a block after the one it was issued
+            // before, so the latency of these accesses overlaps two blocks of compute.
+            uint32_t avail = 0;
+            uint2 topn2[4];
+            unsigned long long poll_prev = 0;
+            if constexpr (LINKIN) {
+                avail = chain_wait(pipe, pipe.pin, (uint32_t)min(nblk, max(pipe.slack, 2)));
+                const int c1 = min(1, lastb) * 4;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    topn[b] = chain_load(pipe.gin + (size_t)(b + 1) * 64);
+                    topn2[b] = chain_load(pipe.gin + (size_t)(c1 + b + 1) * 64);
+                }
+            }
+            for (int bi = 0; bi < nblk; ++bi) {
                 const int jb = bi * 4;
-                wn = wp[(size_t)((jb + 4 < L ? jb + 4 : jb) >> 2) * 64];
-                if (!FIRST) {
+                wn3 = word(bi + 3);
+                unsigned long long poll = 0;
+                if constexpr (LINKIN) {
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        top[b] = topn[b];
+                        topn[b] = topn2[b];
+                    }
+                    if (bi + 2 < nblk && avail < (uint32_t)bi + 3)
+                        avail = chain_wait(pipe, pipe.pin, (uint32_t)bi + 3);
+                    // Block bi + 2's rows (past the last block: the last block's again, unused).
+                    const int cb = min(bi + 2, lastb) * 4;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        topn2[b] = chain_load(pipe.gin + (size_t)(cb + b + 1) * 64);
+                    poll = __hip_atomic_load(pipe.pin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                } else if constexpr (!FIRST) {
                     const uint2* ri = pipe.rin + (size_t)(bi & 1) * 256 + (threadIdx.x & 63);
 #pragma unroll
                     for (int b = 0; b < 4; ++b) top[b] = ri[b * 64];
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 block(jb, w, top);
+                // The register rotation below copies this block's loads: kept after the block,
+                // where they have arrived, instead of hoisted above it to wait for them.
+                __builtin_amdgcn_sched_barrier(0);
                 w = wn;
+                wn = wn2;
+                wn2 = wn3;
+                if constexpr (LINKIN) {
+                    // The poll issued before the previous block: rows up to its count may now be
+                    // read.
+                    avail = max(avail, chain_count(pipe, poll_prev));
+                    poll_prev = poll;
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
+                __syncthreads();
             }
-            __syncthreads();
         }
+        for (int s = pipe.wave + nblk; s < pipe.nsteps; ++s) __syncthreads();
+        if (pipe.gout) chain_publish(pipe, pipe.pout, (uint32_t)nblk);
     } else {
         if (L > 0) fetch(0, w, top);
         for (int jb = 0; jb < L; jb += 4) {
             fetch(jb + 4 < L ? jb + 4 : jb, wn, topn);
             __builtin_amdgcn_sched_barrier(0);
             block(jb, w, top);
+            __builtin_amdgcn_sched_barrier(0);  // the copies below wait for the prefetch
             w = wn;
 #pragma unroll
             for (int b = 0; b < 4; ++b) top[b] = topn[b];
@@ -595,7 +741,8 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
 }
 
 // The tile above with FIRST chosen at run time (the first tile of a group starts from the DP
-// border instead of a boundary row).
+// border instead of a boundary row), and the chained reader role for a pipelined tile whose
+// Pipe has a link in.
 template <int MODE, int QT, int NW, bool PAIR, bool ENDS = false, bool PIPE = false>
 __device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Group& g, bool wlive,
                                                 const uint8_t* query, int m, int i0, bool first,
@@ -605,6 +752,10 @@ __device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Grou
     if (first)
         inter16_tile<MODE, QT, NW, PAIR, true, ENDS, PIPE>(a, g, wlive, query, m, i0, lastc, nA,
                                                            nB, sp, wp, pp, st, pipe);
+    else if (PIPE && pipe.gin != nullptr)
+        inter16_tile<MODE, QT, NW, PAIR, false, ENDS, PIPE, PIPE>(a, g, wlive, query, m, i0,
+                                                                  lastc, nA, nB, sp, wp, pp, st,
+                                                                  pipe);
     else
         inter16_tile<MODE, QT, NW, PAIR, false, ENDS, PIPE>(a, g, wlive, query, m, i0, lastc,
                                                             nA, nB, sp, wp, pp, st, pipe);
@@ -792,6 +943,136 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_pipe
         k = kb > k ? kb : k;
     }
     if (a.key) wave_argmax(k, a.key);
+}
+
+// Chained pipelined kernel: the pipelined kernel's query tiles spread over c.chunks workgroups per
+// group (chunk k holding tiles [k * W, k * W + W) of the c.tiles tiles, W = wavefronts per
+// workgroup), each chunk reading the previous one's bottom rows through the group's boundary
+// buffer (Pipe, chained). A database of few groups then keeps several CUs per group busy instead
+// of one. Each workgroup takes a ticket v (the order in which the launch's workgroups start) and
+// serves group (v / 8 / chunks) * 8 + v % 8 as its chunk (v / 8) % chunks. A chunk waits only on
+// the chunk before it, whose ticket is smaller, so that workgroup has already started: the
+// launch progresses whatever order the workgroups are dispatched in. The chunks of a group share
+// v mod 8, which keeps them on one XCD as long as workgroups start in the order the dispatcher
+// hands them to the XCDs in turn. The chunks fold their running maxima in order: chunk k waits
+// for chunk k - 1 to publish nblk + 1 blocks (its maxima, already folded with the chunks before
+// it, in column 0 of the buffer), folds its own, and passes them on; the last chunk writes the
+// scores and the argmax key. Wavefronts of a short last chunk without a tile take the
+// workgroup barriers only.
+template <int MODE, int Q, bool PAIR>
+__global__ __launch_bounds__(512)
+__attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_chain(
+    Inter16Args a, ChainArgs c, uint32_t pp_words) {
+    extern __shared__ __attribute__((aligned(16))) uint32_t lds[];
+    const int wv = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int W = (int)(blockDim.x >> 6);
+    __shared__ uint32_t ticket_s;
+    if (threadIdx.x == 0) ticket_s = (uint32_t)(atomicAdd(c.ticket, 1ull) - c.ticket_base);
+    __syncthreads();
+    const uint32_t v = ticket_s;
+    const uint32_t rest = v >> 3;
+    const int k = (int)(rest % (uint32_t)c.chunks);
+    const uint32_t gi = (rest / (uint32_t)c.chunks) * 8u + (v & 7u);
+    if (gi >= a.ngroups) return;  // the whole workgroup: a padding group of the grid
+    const Group g = a.groups[a.group_first + gi];
+    const int t0 = k * W;
+    const int tn = min(c.tiles - t0, W);  // tiles of this chunk
+    const int nblk = (int)(g.ncols >> 2);
+    uint32_t* pp = lds + (size_t)wv * pp_words;
+    uint2* ring = reinterpret_cast<uint2*>(lds + (size_t)W * pp_words);
+    uint32_t* fold = lds + (size_t)W * pp_words + (size_t)(W - 1) * 1024;
+    uint2* link = a.spill + g.spill_off + lane;
+    unsigned long long* prog = c.progress + (size_t)gi * kChainMaxLinks;
+    Pipe pipe;
+    pipe.wave = wv;
+    pipe.nsteps = nblk + W - 1;
+    pipe.rin = wv > 0 ? ring + (size_t)(wv - 1) * 512 : nullptr;
+    pipe.rout = wv + 1 < tn ? ring + (size_t)wv * 512 : nullptr;
+    pipe.epoch = c.epoch;
+    pipe.slack = c.slack;
+    pipe.every = c.every;
+    pipe.err = c.err;
+    if (wv == 0 && k > 0) {
+        pipe.gin = link;
+        pipe.pin = prog + (k - 1);
+    }
+    if (wv == tn - 1 && k + 1 < c.chunks) {
+        pipe.gout = link;
+        pipe.pout = prog + k;
+    }
+
+    const int m = a.m, go = a.go, ge = a.ge;
+    const bool liveA = 2 * lane < (int)g.count, liveB = 2 * lane + 1 < (int)g.count;
+    const uint32_t slotA = g.first + 2 * lane, slotB = slotA + 1;
+    const int nA = liveA ? (int)a.lens[slotA] : 0;
+    const int nB = liveB ? (int)a.lens[slotB] : 0;
+    PairRun st;
+    st.run = splat2(-32768);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) st.runq[q] = st.run;
+    const uint2* wp = a.words + g.word_off + lane;
+    const int nfull = m / Q, rem = m - nfull * Q;
+    const int t = t0 + wv;  // this wavefront's tile
+    if (wv >= tn) {
+        // No tile: the profile staging's two barriers and one per block step.
+        for (int s = 0; s < 2 + pipe.nsteps; ++s) __syncthreads();
+    } else if (t < nfull) {
+        inter16_tile_at<MODE, Q, 1, PAIR, false, true>(a, g, true, a.query, m, t * Q + 1, t == 0,
+                                                        rem == 0 && t == nfull - 1, nA, nB,
+                                                        nullptr, wp, pp, st, pipe);
+    } else {
+        const int i0 = nfull * Q + 1;
+        const bool first = nfull == 0;
+        if (rem <= 8)
+            inter16_tile_at<MODE, 8, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+                                                            true, nA, nB, nullptr, wp, pp, st,
+                                                            pipe);
+        else if (Q > 16 && rem <= 16)
+            inter16_tile_at<MODE, 16, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+                                                             true, nA, nB, nullptr, wp, pp, st,
+                                                             pipe);
+        else if (Q > 24 && rem <= 24)
+            inter16_tile_at<MODE, 24, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+                                                             true, nA, nB, nullptr, wp, pp, st,
+                                                             pipe);
+        else
+            inter16_tile_at<MODE, Q, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+                                                            true, nA, nB, nullptr, wp, pp, st,
+                                                            pipe);
+    }
+    const s2v mine =
+        p_max(st.run, p_max(p_max(st.runq[0], st.runq[1]), p_max(st.runq[2], st.runq[3])));
+    fold[wv * 64 + lane] = as_u32(mine);
+    __syncthreads();
+    if (wv != 0) return;
+    s2v run = mine;
+    for (int q = 1; q < W; ++q) run = p_max(run, as_s2(fold[q * 64 + lane]));
+    if (k > 0) {
+        (void)chain_wait(pipe, prog + (k - 1), (uint32_t)nblk + 1);
+        run = p_max(run, as_s2(chain_load(link).x));
+    }
+    if (k + 1 < c.chunks) {
+        chain_store(link, make_uint2(as_u32(run), 0u));
+        chain_publish(pipe, prog + k, (uint32_t)nblk + 1);
+        return;
+    }
+    int bsA, bsB, dj, di;
+    border_best<MODE>(m, nA, go, ge, bsA, dj, di);
+    border_best<MODE>(m, nB, go, ge, bsB, dj, di);
+    const int sA = max(bsA, (int)run.x), sB = max(bsB, (int)run.y);
+    unsigned long long key = 0ull;
+    if (liveA) {
+        const uint32_t gx = a.gidx[slotA];
+        if (a.out_score) a.out_score[gx] = sA;
+        key = make_key(sA, gx);
+    }
+    if (liveB) {
+        const uint32_t gx = a.gidx[slotB];
+        if (a.out_score) a.out_score[gx] = sB;
+        const unsigned long long kb = make_key(sB, gx);
+        key = kb > key ? kb : key;
+    }
+    if (a.key) wave_argmax(key, a.key);
 }
 
 // Persistent batch kernel: each one-wavefront workgroup takes (group, query) items from a device
@@ -1238,6 +1519,22 @@ hipError_t launch_inter16_pipe_q(const Inter16Args& a, hipStream_t st) {
 }
 
 template <int MODE, int Q>
+hipError_t launch_inter16_chain_q(const Inter16Args& a, const ChainArgs& c, int W,
+                                  hipStream_t st) {
+    const uint32_t pp_words = inter16_profile_words(a.al_eff, Q);
+    const size_t lds = ((size_t)W * pp_words + (size_t)(W - 1) * 1024 + (size_t)W * 64) * 4;
+    if (lds > 64u * 1024u) return hipErrorInvalidValue;
+    const uint32_t blocks = (a.ngroups + 7u) / 8u * 8u * (uint32_t)c.chunks;
+    if (inter16_pair_small(a.al_eff, Q))
+        hipLaunchKernelGGL((k_inter16_chain<MODE, Q, true>), dim3(blocks), dim3(64 * W), lds, st,
+                           a, c, pp_words);
+    else
+        hipLaunchKernelGGL((k_inter16_chain<MODE, Q, false>), dim3(blocks), dim3(64 * W), lds,
+                           st, a, c, pp_words);
+    return hipGetLastError();
+}
+
+template <int MODE, int Q>
 hipError_t launch_inter16_batch_q(const Inter16Args& a, const BatchArgs& b, uint32_t wgs,
                                   hipStream_t st) {
     const uint32_t lds = inter16_profile_words(a.al_eff, Q) * 4u;
@@ -1306,6 +1603,24 @@ hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, hipStream_t st) {
     if (q == 16) return launch_inter16_pipe_q<MODE, 16>(a, st);
     if (q == 24) return launch_inter16_pipe_q<MODE, 24>(a, st);
     if (q == 32) return launch_inter16_pipe_q<MODE, 32>(a, st);
+    return hipErrorInvalidValue;
+}
+
+// Chained pipelined kernel: q-row tiles, `waves` wavefronts (tiles) per workgroup. The shape
+// must cover the query exactly: tiles = ceil(m / q), chunks = ceil(tiles / waves), so every
+// chunk holds at least one tile and its last wavefront holding one is the chunk's writer.
+template <int MODE>
+hipError_t launch_inter16_chain_m(const Inter16Args& a, const ChainArgs& c, int q, int waves,
+                                  hipStream_t st) {
+    if (a.ngroups == 0) return hipSuccess;
+    if (q <= 0 || waves < 1 || waves > 8 || c.tiles != (a.m + q - 1) / q ||
+        c.chunks != (c.tiles + waves - 1) / waves || c.chunks < 1 ||
+        c.chunks > kChainMaxChunks || c.every < 1 || !c.progress || !c.err || !c.ticket)
+        return hipErrorInvalidValue;
+    if (q == 8) return launch_inter16_chain_q<MODE, 8>(a, c, waves, st);
+    if (q == 16) return launch_inter16_chain_q<MODE, 16>(a, c, waves, st);
+    if (q == 24) return launch_inter16_chain_q<MODE, 24>(a, c, waves, st);
+    if (q == 32) return launch_inter16_chain_q<MODE, 32>(a, c, waves, st);
     return hipErrorInvalidValue;
 }
 
@@ -1408,5 +1723,7 @@ template hipError_t launch_inter16_batch_m<HY_KMODE>(const Inter16Args&, const B
 template uint32_t inter16_batch_occupancy_m<HY_KMODE>(int, int);
 template hipError_t launch_inter16_ends_m<HY_KMODE>(const Inter16Args&, hipStream_t);
 template hipError_t launch_inter16_pipe_m<HY_KMODE>(const Inter16Args&, int, hipStream_t);
+template hipError_t launch_inter16_chain_m<HY_KMODE>(const Inter16Args&, const ChainArgs&, int,
+                                                     int, hipStream_t);
 
 }  // namespace hy

@@ -272,6 +272,19 @@ hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, hipStream_
     }
 }
 
+hipError_t launch_inter16_chain(int mode, const Inter16Args& a, const ChainArgs& c, int q,
+                                int waves, hipStream_t stream) {
+    if (a.ngroups == 0) return hipSuccess;
+    switch (mode) {
+    case kSW: return launch_inter16_chain_m<kSW>(a, c, q, waves, stream);
+    case kNW: return launch_inter16_chain_m<kNW>(a, c, q, waves, stream);
+    case kHW: return launch_inter16_chain_m<kHW>(a, c, q, waves, stream);
+    case kOV: return launch_inter16_chain_m<kOV>(a, c, q, waves, stream);
+    case kSHW: return launch_inter16_chain_m<kSHW>(a, c, q, waves, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_inter16_ends(int mode, const Inter16Args& a, hipStream_t stream) {
     if (a.ngroups == 0) return hipSuccess;
     switch (mode) {

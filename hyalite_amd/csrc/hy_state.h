@@ -106,6 +106,14 @@ struct hy_scratch {
     hipEvent_t fork = nullptr, join = nullptr;
     double inter_ms = 0.0, tail_ms = 0.0;
     uint64_t launches = 0;
+    // Chained pipelined kernel: progress words (one per link of each packed group) and the
+    // workgroup ticket counter, zeroed at allocation; the launch epoch; the tickets earlier
+    // launches took; and a pinned flag the kernel raises if a wait gave up.
+    unsigned long long* d_chain = nullptr;  // progress words, then the ticket counter
+    unsigned int* h_chain_err = nullptr;
+    uint32_t chain_epoch = 0;
+    uint64_t chain_tickets = 0;             // tickets taken by earlier launches
+    int32_t last_kernel = -1;               // HY_KERNEL_* of the last one-query scan
     // Batched form (reserved on first use or by hy_scratch_reserve_batch).
     uint32_t bcap = 0;                   // queries per submission
     uint32_t bq_stride = 0;              // bytes reserved per query (max_query_len, >= 1)
@@ -161,7 +169,8 @@ struct hy_scratch {
         hy::dev_free(d_bq); hy::dev_free(d_bmeta); hy::dev_free(d_bkeys); hy::dev_free(d_bres);
         hy::dev_free(d_slots); hy::dev_free(d_bwin_spill); hy::dev_free(d_btail_spill);
         hy::dev_free(d_codes); hy::dev_free(d_tends); hy::dev_free(d_tout); hy::dev_free(d_ops);
-        hy::dev_free(d_slab_off);
+        hy::dev_free(d_slab_off); hy::dev_free(d_chain);
+        if (h_chain_err) (void)hipHostFree(h_chain_err);
         if (h_tout) (void)hipHostFree(h_tout);
         if (h_ops) (void)hipHostFree(h_ops);
         if (h_key) (void)hipHostFree(h_key);
@@ -194,6 +203,9 @@ int stage_query(const hy_db* db, hy_scratch* s, const uint8_t* q, size_t qlen);
 int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write_scores,
                     unsigned long long* key);
 void collect_profile(hy_scratch* s);
+// After a scan's stream synchronisation: collect_profile, then HY_E_INTERNAL if a chained
+// kernel's wait gave up (its results are not trusted), else HY_OK.
+int scan_done(hy_scratch* s);
 // The width proof (src/width.rs:107-169): magnitude bound of a (mode, scheme, lengths) problem
 // and the narrowest width holding it (8, 16, 32; 0 when not even i32 suffices).
 __int128 magnitude_bound(uint32_t mode, int32_t min_e, int32_t max_e, int32_t go, int32_t ge,

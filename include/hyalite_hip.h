@@ -325,6 +325,19 @@ int hy_scratch_profile_read(hy_scratch* s, double* inter_ms, double* tail_ms, ui
                             int reset);
 /* Device time of the batched form's persistent kernel (profiling runs submit without a graph). */
 int hy_scratch_batch_profile_read(hy_scratch* s, double* kernel_ms, uint64_t* launches, int reset);
+/* Which packed 16-bit kernel the scratch's last one-query scan ran over the database's short packed
+ * groups: HY_KERNEL_NONE (none: no such groups, or no scan yet), HY_KERNEL_PACKED (one group per
+ * wavefront), HY_KERNEL_PIPELINED (one group per workgroup, a query tile per wavefront),
+ * HY_KERNEL_CHAINED (a group's query tiles over several workgroups) or HY_KERNEL_ENDS (end
+ * tracking). Instrumentation for tests and benchmarks; no counterpart in the reference. */
+enum {
+    HY_KERNEL_NONE = -1,
+    HY_KERNEL_PACKED = 0,
+    HY_KERNEL_PIPELINED = 1,
+    HY_KERNEL_CHAINED = 2,
+    HY_KERNEL_ENDS = 3
+};
+int hy_scratch_last_kernel(const hy_scratch* s, int32_t* kernel);
 
 /* Serving form (no reference counterpart: the reference is a library without a request loop).
  * A server keeps the database resident and runs lanes, each a scratch with its own stream and

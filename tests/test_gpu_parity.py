@@ -166,6 +166,7 @@ def test_small_alphabet_profiles_every_tile_height(mode, monkeypatch):
         sc = hy.Scratch(db)
         want = [orc.scan_flat(flat, off, q, al, mat, 3, 1, mode, True, threads=4) for q in qs]
         ends = [orc.scan_all_flat(flat, off, q, al, mat, 3, 1, mode, threads=4) for q in qs]
+        monkeypatch.setenv("HYALITE_HIP_CHAIN", "0")
         for rows in (16, 24, 32):
             sc.set_tuning(hy.Tuning(tile_rows=rows, batch_tile_rows=rows, batch_pairs=0))
             monkeypatch.setenv("HYALITE_HIP_PIPE_ROWS", str(rows))
@@ -655,6 +656,7 @@ def test_pipelined_kernel_every_tile_count(mode, monkeypatch):
     remainder size, against the oracle and against the one-wavefront-per-group kernel; two
     alphabets, targets of every padding class."""
     rng = np.random.default_rng(1200 + mode)
+    monkeypatch.setenv("HYALITE_HIP_CHAIN", "0")
     for al, mat, go, ge in ((4, identity_matrix(4, 2, -1), 2, 1), (20, BLOSUM62, 11, 1)):
         seqs = [rng.integers(0, al, size=int(n), dtype=np.uint8)
                 for n in rng.integers(0, 300, 700)]
@@ -665,11 +667,76 @@ def test_pipelined_kernel_every_tile_count(mode, monkeypatch):
             q = rng.integers(0, al, size=m, dtype=np.uint8)
             ws = orc.scan_all_flat(flat, off, q, al, mat, go, ge, mode, ends=False, threads=8)
             want = orc.scan_flat(flat, off, q, al, mat, go, ge, mode, True, threads=8)
-            for pipe in ("1", "0"):
+            for pipe, kernel in (("1", "pipelined"), ("0", "packed")):
                 monkeypatch.setenv("HYALITE_HIP_PIPE", pipe)
                 assert np.array_equal(db.scan_scores_array(sc, q), ws), (al, m, pipe)
+                assert sc.last_kernel() == kernel, (al, m, pipe)
                 h = db.scan(sc, q)
                 assert (h.score, h.db_index, h.query_end, h.target_end) == want, (al, m, pipe)
+
+
+def chain_chunks(m, rows, waves):
+    tiles = -(-m // rows)
+    return -(-tiles // min(waves, tiles))
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_chained_kernel_every_shape(mode, monkeypatch):
+    """The chained pipelined kernel (a group's query tiles spread over 2-8 workgroups, each
+    reading the previous one's bottom rows and folded maxima through the group's boundary buffer
+    under published progress counts) at every tile height (8, 16, 24, 32 rows), 1-4 and 8
+    wavefronts per workgroup (short last chunks included), reader slack 1 and 4 and writers
+    publishing every block and every third block, for query
+    lengths giving every chunk count and remainder tile, against the oracle: scan_scores and
+    scan, on 23 packed groups with targets of every padding class, two alphabets. Repeated
+    scans check that every launch's epoch and tickets start afresh."""
+    rng = np.random.default_rng(1700 + mode)
+    monkeypatch.setenv("HYALITE_HIP_CHAIN", "1")
+    ran = 0
+    for al, mat, go, ge in ((4, identity_matrix(4, 2, -1), 2, 1), (20, BLOSUM62, 11, 1)):
+        seqs = [rng.integers(0, al, size=int(n), dtype=np.uint8)
+                for n in rng.integers(0, 300, 2900)]
+        flat, off = orc.flatten(seqs)
+        db = build(seqs, (al, mat, go, ge), mode, hy.SearchType.ScoreEnd, 256)
+        sc = hy.Scratch(db)
+        for m in (9, 17, 40, 65, 100, 129, 200, 256):
+            q = rng.integers(0, al, size=m, dtype=np.uint8)
+            ws = orc.scan_all_flat(flat, off, q, al, mat, go, ge, mode, ends=False, threads=8)
+            want = orc.scan_flat(flat, off, q, al, mat, go, ge, mode, True, threads=8)
+            for rows in (8, 16, 24, 32):
+                for waves in (1, 2, 3, 4, 8):
+                    if not 2 <= chain_chunks(m, rows, waves) <= 8:
+                        continue
+                    monkeypatch.setenv("HYALITE_HIP_CHAIN_ROWS", str(rows))
+                    monkeypatch.setenv("HYALITE_HIP_CHAIN_WAVES", str(waves))
+                    for slack, every in ((1, 1), (4, 3)):
+                        monkeypatch.setenv("HYALITE_HIP_CHAIN_SLACK", str(slack))
+                        monkeypatch.setenv("HYALITE_HIP_CHAIN_EVERY", str(every))
+                        case = (al, m, rows, waves, slack, every)
+                        assert np.array_equal(db.scan_scores_array(sc, q), ws), case
+                        assert sc.last_kernel() == "chained", case
+                        h = db.scan(sc, q)
+                        assert (h.score, h.db_index, h.query_end, h.target_end) == want, case
+                        ran += 1
+            # The default rule on a database this small: the chained kernel once 8-row tiles make
+            # two chunks of four (33 rows on), the pipelined kernel below, the one-wavefront
+            # kernel for one tile; at 400 rows (below) no pipelined shape exists.
+            for var in ("HYALITE_HIP_CHAIN", "HYALITE_HIP_CHAIN_ROWS", "HYALITE_HIP_CHAIN_WAVES",
+                        "HYALITE_HIP_CHAIN_SLACK", "HYALITE_HIP_CHAIN_EVERY"):
+                monkeypatch.delenv(var, raising=False)
+            assert np.array_equal(db.scan_scores_array(sc, q), ws), (al, m)
+            assert sc.last_kernel() == ("chained" if m > 32 else
+                                        "pipelined" if m > 16 else "packed"), (al, m)
+            monkeypatch.setenv("HYALITE_HIP_CHAIN", "1")
+        q = rng.integers(0, al, size=400, dtype=np.uint8)
+        db = build(seqs, (al, mat, go, ge), mode, hy.SearchType.ScoreEnd, 400)
+        sc = hy.Scratch(db)
+        monkeypatch.delenv("HYALITE_HIP_CHAIN")
+        ws = orc.scan_all_flat(flat, off, q, al, mat, go, ge, mode, ends=False, threads=8)
+        assert np.array_equal(db.scan_scores_array(sc, q), ws), al
+        assert sc.last_kernel() == "chained", al
+        monkeypatch.setenv("HYALITE_HIP_CHAIN", "1")
+    assert ran > 100
 
 
 @pytest.mark.parametrize("mode", [MODE_SW, MODE_NW, MODE_OV])
