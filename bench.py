@@ -306,11 +306,8 @@ def serving(args, sync) -> dict:
     }
 
 
-def other_configs(sync) -> dict:
-    """The DNA configurations on the same GPU: the 100-nt SW ScoreEnd scan of 10,000 targets
-    (single best with ends) and the 150-nt mode sweep over 2,000,000 targets (scan_scores per
-    mode, the per-target score vector copied to the host)."""
-    out = {}
+def c1_config(sync) -> dict:
+    """The 100-nt DNA SW ScoreEnd scan of 10,000 targets (single best with ends)."""
     w = wl.c1_dna()
     db = build_db(w)
     sc = hy.Scratch(db)
@@ -323,12 +320,19 @@ def other_configs(sync) -> dict:
         hit = db.scan(sc, q)
     sync()
     dt = (time.perf_counter() - t0) / 50
-    out[w.name] = {"op": "scan (ScoreEnd)", "ms": round(dt * 1e3, 3),
-                   "gcups": round(len(q) * w.residues / dt / 1e9, 1),
-                   "partitions": {str(k): v for k, v in db.partition_sizes().items()},
-                   "hit": {"score": hit.score, "db_index": hit.db_index,
-                           "query_end": hit.query_end, "target_end": hit.target_end}}
-    del sc, db
+    return {w.name: {"op": "scan (ScoreEnd)", "ms": round(dt * 1e3, 3),
+                     "gcups": round(len(q) * w.residues / dt / 1e9, 1),
+                     "kernel": sc.last_kernel(),
+                     "partitions": {str(k): v for k, v in db.partition_sizes().items()},
+                     "hit": {"score": hit.score, "db_index": hit.db_index,
+                             "query_end": hit.query_end, "target_end": hit.target_end}}}
+
+
+def other_configs(sync) -> dict:
+    """The DNA configurations on the same GPU: the 100-nt SW ScoreEnd scan of 10,000 targets
+    (single best with ends) and the 150-nt mode sweep over 2,000,000 targets (scan_scores per
+    mode, the per-target score vector copied to the host)."""
+    out = c1_config(sync)
     base = wl.c3_dna_modes()
     for mode, name in ((1, "NW"), (2, "HW"), (4, "SHW"), (3, "OV")):
         # Built for ScoreEnd: scan_scores ignores the search type, scan_all then tracks ends.
@@ -802,9 +806,10 @@ def main() -> int:
     ap.add_argument("--serving-queries", type=int, default=64)
     ap.add_argument("--serving-lanes", type=int, default=3)
     ap.add_argument("--serving-batch", type=int, default=16)
-    ap.add_argument("--only", choices=("configs", "shapes", "serving"),
-                    help="only run the DNA configurations, the reference's scan shapes or the "
-                         "serving form and print their JSON (for profiling)")
+    ap.add_argument("--only", choices=("configs", "c1", "shapes", "serving"),
+                    help="only run the DNA configurations (c1: the 10,000-target scan alone), "
+                         "the reference's scan shapes or the serving form and print their JSON "
+                         "(for profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB and exit")
     args = ap.parse_args()
@@ -826,6 +831,9 @@ def main() -> int:
 
     if args.only == "configs":
         print(json.dumps({"other_configs": other_configs(sync)}), flush=True)
+        return 0
+    if args.only == "c1":
+        print(json.dumps({"other_configs": c1_config(sync)}), flush=True)
         return 0
     if args.only == "shapes":
         print(json.dumps({"reference_shapes": reference_shapes(sync)}), flush=True)
