@@ -547,6 +547,19 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
         add_groups16(n16_long, ns);
         db->max_ncols16 = db->groups16.empty() ? 0 : db->groups16.front().ncols;
         db->inter16_spill = spill;
+        // Groups wider than a batched-form slot run in the batched form's systolic list.
+        uint32_t bf = 0;
+        while (bf < db->groups16.size() && db->groups16[bf].ncols > hy::kBatchSlotMaxCols) {
+            const Group& g = db->groups16[bf];
+            for (uint32_t k = 0; k < g.count; ++k) {
+                const uint32_t t = slot_gidx[g.first + k];
+                db->sys_list_batch.push_back(t);
+                db->sys_batch_max_len = std::max<uint64_t>(db->sys_batch_max_len, lens[t]);
+            }
+            ++bf;
+        }
+        db->groups16_batch_first = bf;
+        db->batch_max_ncols16 = bf < db->groups16.size() ? db->groups16[bf].ncols : 0;
 
         if (dev_upload(&db->d_groups16, db->groups16) != hipSuccess) return fail(HY_E_DEVICE);
         if (dev_upload(&db->d_lens16, slot_len) != hipSuccess) return fail(HY_E_DEVICE);
@@ -1171,14 +1184,14 @@ extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
         return HY_E_OUT_OF_MEMORY;
     std::memset(s->h_bq, 0, qbytes);
     std::memset(s->h_bmeta, 0, meta * sizeof(uint32_t));
-    if (!db->groups16.empty()) {
+    if (db->groups16.size() > db->groups16_batch_first) {
         uint32_t per_cu =
             batch_query_pairs(db, s)
                 ? hy::inter16q_batch_occupancy(s->bq16, (int)db->al_eff)
                 : hy::inter16_batch_occupancy((int)db->mode, s->bq16, (int)db->al_eff);
         if (s->bwpc > 0) per_cu = std::min<uint32_t>(per_cu, (uint32_t)s->bwpc);
         s->nslots = per_cu * hy::device_cus();
-        s->slot_stride = (uint64_t)(db->max_ncols16 + 1) * 64;
+        s->slot_stride = (uint64_t)(db->batch_max_ncols16 + 1) * 64;
         if (s->nslots == 0 ||
             hipMalloc((void**)&s->d_slots, (size_t)s->nslots * s->slot_stride * sizeof(uint2)) !=
                 hipSuccess)
@@ -1261,11 +1274,11 @@ int enqueue_batch(const hy_db* db, hy_scratch* s, bool timed) {
         if (hipEventRecord(s->join, s->aux) != hipSuccess) return HY_E_DEVICE;
     }
 
-    if (!db->groups16.empty()) {
+    if (db->groups16.size() > db->groups16_batch_first) {
         hy::Inter16Args pa{};
         pa.groups = db->d_groups16;
-        pa.group_first = 0;
-        pa.ngroups = (uint32_t)db->groups16.size();
+        pa.group_first = db->groups16_batch_first;
+        pa.ngroups = (uint32_t)db->groups16.size() - db->groups16_batch_first;
         pa.words = db->d_words16;
         pa.lens = db->d_lens16;
         pa.gidx = db->d_gidx16;
@@ -1395,7 +1408,7 @@ extern "C" int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* cons
             if (hipGraphLaunch(s->gexec, s->stream) != hipSuccess) return HY_E_DEVICE;
         }
         if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
-        if (s->profile && !db->groups16.empty()) {
+        if (s->profile && db->groups16.size() > db->groups16_batch_first) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, s->bev[0], s->bev[1]) == hipSuccess) {
                 s->batch_ms += ms;

@@ -482,6 +482,11 @@ hipError_t launch_band32_m(int out, const BandArgs& a, uint32_t nblocks, int m, 
 // to 64, 24 up to 192, 32 beyond. Measured on a 100-row query (10,000 DNA targets): 16-row tiles
 // (7 wavefronts) 0.205 ms, 24 (5) 0.191 ms, 32 (4) 0.228 ms per launch.
 constexpr int kPipeMaxWaves = 8;
+// The batched form gives each resident workgroup a boundary slot of (widest group + 1) x 64
+// entries of 8 bytes; packed groups wider than this many columns (long targets) run in its
+// systolic list instead, which bounds a slot at 4 MB (8.6 GB over 2,048 resident workgroups)
+// whatever the database's longest packable target.
+constexpr uint32_t kBatchSlotMaxCols = 8192;
 inline int inter16_pipe_rows(int m) {
     if (m <= 16) return 0;
     if (m <= 64) return 16;

@@ -58,6 +58,10 @@ struct hy_db {
     int waves16 = 4;     // wavefronts per workgroup of the packed kernel
     // Groups [0, groups16_short_first) hold long targets; one-query scans start after them.
     uint32_t groups16_short_first = 0;
+    // The batched form's persistent kernel runs groups [groups16_batch_first, end); the groups
+    // before are wider than kBatchSlotMaxCols and their targets join sys_list_batch.
+    uint32_t groups16_batch_first = 0;
+    uint32_t batch_max_ncols16 = 0;  // widest group the persistent kernel runs
     uint32_t max_ncols16 = 0;
     uint64_t residues16 = 0;       // residues of the packed targets
     // Systolic list (long targets, or everything when the alphabet leaves no padding symbol).

@@ -400,6 +400,38 @@ def test_batch_with_unpackable_long_targets():
 
 
 @pytest.mark.parametrize("mode", ALL_MODES)
+def test_batch_slots_skip_packed_groups_wider_than_a_slot(mode):
+    """A packable target wider than a batched-form slot (8,192 columns) runs in the batched
+    form's systolic list, so one long target does not widen every resident workgroup's boundary
+    slot: batched hits (one of them inside that target) equal the oracle and the one-query scans,
+    and a serving lane stays far below the 2,048 x 9,000-column x 512-byte slots it would need."""
+    rng = np.random.default_rng(1900 + mode)
+    scheme = dna_scheme()
+    seqs = [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in rng.integers(0, 300, 1500)]
+    q = rng.integers(0, 4, size=150, dtype=np.uint8)
+    wide = rng.integers(0, 4, size=9_000, dtype=np.uint8)
+    wide[8_700:8_850] = q  # the best hit of q sits past column 8,192 of the wide target
+    seqs[700] = wide
+    seqs[701] = rng.integers(0, 4, size=8_300, dtype=np.uint8)
+    db = build(seqs, scheme, mode, hy.SearchType.ScoreEnd, 200)
+    assert db.partition_sizes()[hy.ScoreWidth.I32] == 0  # every target packable
+    sc = hy.Scratch(db)
+    qs = [q, q[:60], rng.integers(0, 4, size=200, dtype=np.uint8), q[20:], q[:9]]
+    flat, off = orc.flatten(seqs)
+    want = [orc.scan_flat(flat, off, qq, *scheme, mode, True, threads=8) for qq in qs]
+    assert want[0][1] == 700 or mode != MODE_SW
+    got = db.scan_batch(sc, qs)
+    assert [(h.score, h.db_index, h.query_end, h.target_end) for h in got] == want
+    for qq, w in zip(qs, want):
+        h = db.scan(sc, qq)
+        assert (h.score, h.db_index, h.query_end, h.target_end) == w
+    with hy.Server(db, max_lanes=1, max_batch=8, prealloc=True) as srv:
+        assert srv.stats().lane_bytes < 2 * 2**30, srv.stats().lane_bytes
+        tickets = [srv.submit(qq) for qq in qs]
+        assert [_tuple(srv.wait(t).hit) for t in tickets] == want
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
 def test_unpackable_groups_beside_the_packed_kernel(mode):
     """Targets that fail the packed width proof but stay below the long-target threshold run in
     the 32-bit kernel on the scratch's second stream while the packed kernel runs on the first,
