@@ -1205,6 +1205,7 @@ __device__ __forceinline__ void inter16q_tile(const Inter16Args& a, const Group&
         fetch(jb + 4 < L ? jb + 4 : jb, wn, topn);
         __builtin_amdgcn_sched_barrier(0);
         block(jb, w, top);
+        __builtin_amdgcn_sched_barrier(0);  // the copies below wait for the prefetch
         w = wn;
 #pragma unroll
         for (int b = 0; b < 4; ++b) top[b] = topn[b];
