@@ -520,6 +520,9 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                     close_column(jb + bp + 1, cmA, crA);
                     close_column(jb + bp + 2, cmB, crB);
                 }
+                // One scheduling region per column pair: hoisting the next pair's E - ge for
+                // every row above this one spilled the 32-row first tile (a wait per block).
+                if (QT >= 32) __builtin_amdgcn_sched_barrier(0);
             }
         } else {
             // One column with its answer cells: the last row in the last tile (every column up
