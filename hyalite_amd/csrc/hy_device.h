@@ -89,6 +89,13 @@ __device__ __forceinline__ s2v p_minu_pk(s2v a, s2v b) {
     asm("v_pk_min_u16 %0, %1, %2" : "=v"(d) : "v"(as_u32(a)), "v"(as_u32(b)));
     return as_s2(d);
 }
+// 0xFFFF in each half where d > 0, else 0, in one instruction: d x 0xFFFF with the unsigned
+// clamp (v_pk_mad_u16 ... clamp), for a select by v_bfi_b32.
+__device__ __forceinline__ uint32_t p_nonzero_mask(s2v d) {
+    uint32_t m;
+    asm("v_pk_mad_u16 %0, %1, %2, 0 clamp" : "=v"(m) : "v"(as_u32(d)), "v"(0xFFFFFFFFu));
+    return m;
+}
 __device__ __forceinline__ s2v p_maxu(s2v a, s2v b) {
     return __builtin_bit_cast(s2v, __builtin_elementwise_max(__builtin_bit_cast(u2v, a),
                                                              __builtin_bit_cast(u2v, b)));

@@ -476,8 +476,10 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
         // tile is (H of its last row, F' of the row below).
         if (F::local) {
             // With ENDS the running maximum becomes a per-column one that also keeps the first
-            // row reaching it: cm = max(cm, h); cr = max(cr, [h > cm] * r) (rows only grow within
-            // a column, so the latest strict improvement has the largest row).
+            // row reaching it: cm = max(cm, h); cr = r where h > cm (rows only grow within a
+            // column, so the latest strict improvement is the first row reaching the maximum).
+            // The mask is one clamped v_pk_mad_u16 and the select one v_bfi_b32: four operations
+            // per cell pair on top of the cell's eight.
             auto cell = [&](int r, s2v sgv, s2v& hd, s2v& f, s2v& cm, s2v& cr) HY_INLINE {
                 const s2v h = p_max(p_max(p_add(hd, sgv), E[r]), f);
                 hd = HG[r];
@@ -486,9 +488,9 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                 E[r] = p_max(hg, p_subu(E[r], ge2));
                 f = p_max(hg, p_subu(f, ge2));
                 if constexpr (ENDS) {
-                    const s2v up = p_minu_pk(p_subu(h, cm), one2);
+                    const uint32_t up = p_nonzero_mask(p_subu(h, cm));
                     cm = p_max(cm, h);
-                    cr = p_maxu(cr, p_madu(up, splat2(r), splat2(0)));
+                    cr = as_s2((up & as_u32(splat2(r))) | (~up & as_u32(cr)));
                 } else {
                     runq[r & 3] = p_max(runq[r & 3], h);
                 }
@@ -973,6 +975,13 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_chai
     if (threadIdx.x == 0) ticket_s = (uint32_t)(atomicAdd(c.ticket, 1ull) - c.ticket_base);
     __syncthreads();
     const uint32_t v = ticket_s;
+    if (v >= gridDim.x) {
+        // The counter disagrees with the tickets the host counted (only after a launch that did
+        // not complete): report it instead of serving a wrong chunk.
+        if (threadIdx.x == 0)
+            __hip_atomic_store(c.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        return;
+    }
     const uint32_t rest = v >> 3;
     const int k = (int)(rest % (uint32_t)c.chunks);
     const uint32_t gi = (rest / (uint32_t)c.chunks) * 8u + (v & 7u);
