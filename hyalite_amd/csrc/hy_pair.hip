@@ -721,9 +721,11 @@ __global__ __launch_bounds__(WIDE ? 256 : 64) void k_pairs16(Pair16Args a) {
             E[r] = p_max(hg, p_subu(E[r], ge2));
             f = p_max(hg, p_subu(f, ge2));
             if constexpr (ENDS) {
-                const s2v up = p_minu_pk(p_subu(h, cm), one2);
+                // cr = r where h beats the column's maximum so far: a clamped-multiply mask and
+                // a bit select (the latest strict improvement is the first row reaching it).
+                const uint32_t up = p_nonzero_mask(p_subu(h, cm));
                 cm = p_max(cm, h);
-                cr = p_maxu(cr, p_madu(up, splat2(r), zero2));
+                cr = as_s2((up & as_u32(splat2(r))) | (~up & as_u32(cr)));
             } else {
                 runq[r & 3] = p_max(runq[r & 3], h);
             }
