@@ -21,7 +21,7 @@ static_assert(offsetof(hy::TraceOut, n_ops) == offsetof(hy_alignment, n_ops),
 
 namespace {
 
-constexpr uint64_t kDefaultTracePool = 4ull << 30;  // bytes of move codes per chunk
+constexpr uint64_t kDefaultTracePool = 8ull << 30;  // bytes of move codes (two chunks' worth)
 
 uint64_t trace_pool_bytes() {
     if (const char* v = std::getenv("HYALITE_HIP_TRACE_POOL_BYTES")) {
@@ -51,8 +51,9 @@ int grow(T** p, uint64_t* cap, uint64_t need) {
 
 void account(hy_scratch* s) {
     s->trace_bytes = s->codes_words * 4 + s->tends_cap * sizeof(int4) +
-                     s->tout_cap * sizeof(hy::TraceOut) + s->ops_cap +
-                     (s->d_slab_off ? (s->db->n + 1) * sizeof(uint64_t) : 0);
+                     s->titem_cap * sizeof(uint32_t) + s->tout_cap * sizeof(hy::TraceOut) +
+                     s->ops_cap + (s->d_slab_off ? (s->db->n + 1) * sizeof(uint64_t) : 0) +
+                     (s->d_gslab ? (s->db->groups16.size() + 1) * sizeof(uint64_t) : 0);
 }
 
 hy::TraceArgs trace_args(const hy_db* db, hy_scratch* s, int m) {
@@ -142,6 +143,14 @@ extern "C" int hy_scan_aligned(const hy_db* db, hy_scratch* s, const uint8_t* qu
     return HY_OK;
 }
 
+// The packed traceback (k_trace16) takes the targets of the packed layout unless
+// HYALITE_HIP_TRACE16=0 (then every target runs in the systolic k_trace_dp, as the single-pair
+// scan_aligned does).
+static bool trace16_enabled(const hy_db* db) {
+    const char* e = std::getenv("HYALITE_HIP_TRACE16");
+    return !(e && std::atoi(e) == 0) && !db->groups16.empty();
+}
+
 extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t* query,
                                    size_t qlen, hy_alignment* out, uint8_t* ops, size_t ops_cap) {
     if (!db || !s || !out || !ops) return HY_E_NULL_ARGUMENT;
@@ -152,48 +161,140 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
     if ((uint64_t)ops_cap < ops_need) return HY_E_INVALID_ARGUMENT;
     const int m = (int)qlen;
     if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
+    const bool packed = trace16_enabled(db);
+    const std::vector<hy::Group>& g16 = db->groups16;
+    const uint64_t ng = packed ? g16.size() : 0;
+    auto traced16 = [&](uint64_t g) { return g16[g].ncols <= 65535u; };
 
-    // Slab offsets per database index for this query length (R and the pass count follow m).
-    if (s->slab_m != m || !s->d_slab_off) {
-        s->h_slab_off.assign(n + 1, 0);
-        for (uint64_t k = 0; k < n; ++k)
+    // Systolic part: the targets of trace_rest (or every target), slab offsets per list position
+    // for this query length (R and the pass count follow m).
+    const uint64_t nlist = packed ? db->trace_rest.size() : n;
+    auto list_at = [&](uint64_t k) -> uint64_t { return packed ? db->trace_rest[k] : k; };
+    if (s->slab_m != m || s->slab_list != (packed ? 1 : 0) || !s->d_slab_off) {
+        s->h_slab_off.assign(nlist + 1, 0);
+        for (uint64_t k = 0; k < nlist; ++k) {
+            const uint64_t t = list_at(k);
             s->h_slab_off[k + 1] =
-                s->h_slab_off[k] + hy::trace_slab_words(m, db->h_off[k + 1] - db->h_off[k]);
+                s->h_slab_off[k] + hy::trace_slab_words(m, db->h_off[t + 1] - db->h_off[t]);
+        }
         if (!s->d_slab_off &&
             hipMalloc((void**)&s->d_slab_off, (n + 1) * sizeof(uint64_t)) != hipSuccess) {
             s->d_slab_off = nullptr;
             return HY_E_OUT_OF_MEMORY;
         }
-        if (hipMemcpy(s->d_slab_off, s->h_slab_off.data(), (n + 1) * sizeof(uint64_t),
+        if (hipMemcpy(s->d_slab_off, s->h_slab_off.data(), (nlist + 1) * sizeof(uint64_t),
                       hipMemcpyHostToDevice) != hipSuccess) {
             s->slab_m = -1;
             return HY_E_DEVICE;
         }
         s->slab_m = m;
+        s->slab_list = packed ? 1 : 0;
+    }
+    // Packed part: code words before each group for this query length.
+    if (packed && (s->gslab_m != m || !s->d_gslab)) {
+        s->h_gslab.assign(ng + 1, 0);
+        for (uint64_t g = 0; g < ng; ++g)
+            s->h_gslab[g + 1] =
+                s->h_gslab[g] + (traced16(g) ? hy::trace16_group_words(m, g16[g].ncols) : 0);
+        if (!s->d_gslab &&
+            hipMalloc((void**)&s->d_gslab, (ng + 1) * sizeof(uint64_t)) != hipSuccess) {
+            s->d_gslab = nullptr;
+            return HY_E_OUT_OF_MEMORY;
+        }
+        if (hipMemcpy(s->d_gslab, s->h_gslab.data(), (ng + 1) * sizeof(uint64_t),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            s->gslab_m = -1;
+            return HY_E_DEVICE;
+        }
+        s->gslab_m = m;
     }
     const std::vector<uint64_t>& so = s->h_slab_off;
-    const uint64_t one = hy::trace_slab_words(m, db->mtl);
-    const uint64_t pool = std::min<uint64_t>(so[n], std::max<uint64_t>(one, trace_pool_bytes() / 4));
+    const std::vector<uint64_t>& gs = s->h_gslab;
+    // The pool holds the configured bytes, at least one target's slab or one group's codes, and
+    // at most what the whole run needs.
+    uint64_t one = nlist ? hy::trace_slab_words(m, db->mtl) : 0;
+    uint64_t total = so[nlist];
+    for (uint64_t g = 0; g < ng; ++g)
+        if (traced16(g)) one = std::max<uint64_t>(one, gs[g + 1] - gs[g]);
+    if (ng) total = std::max<uint64_t>(total, gs[ng]);
+    const uint64_t pool = std::min<uint64_t>(total, std::max<uint64_t>(one, trace_pool_bytes() / 4));
     if ((rc = grow(&s->d_codes, &s->codes_words, std::max<uint64_t>(pool, 1))) != HY_OK ||
         (rc = grow(&s->d_tends, &s->tends_cap, n)) != HY_OK ||
+        (rc = grow(&s->d_titem, &s->titem_cap, packed ? n : 1)) != HY_OK ||
         (rc = grow(&s->d_tout, &s->tout_cap, n)) != HY_OK ||
         (rc = grow(&s->d_ops, &s->ops_cap, std::max<uint64_t>(ops_need, 1))) != HY_OK)
         return rc;
     account(s);
 
     if ((rc = hy::stage_query(db, s, query, qlen)) != HY_OK) return rc;
-    hy::TraceArgs a = trace_args(db, s, m);
-    a.slab_off = s->d_slab_off;
-    // Chunks of consecutive targets whose slabs fit the pool (at least one target each); the
-    // stream orders each chunk's DP after the previous chunk's walk, which reads the same pool.
-    for (uint64_t k0 = 0; k0 < n;) {
-        uint64_t k1 = k0 + 1;
-        while (k1 < n && so[k1 + 1] - so[k0] <= s->codes_words) ++k1;
-        a.first = (uint32_t)k0;
-        a.nitems = (uint32_t)(k1 - k0);
-        a.slab_base = so[k0];
-        if (hy::launch_trace((int)db->mode, a, s->stream) != hipSuccess) return HY_E_DEVICE;
-        k0 = k1;
+    // Packed chunks alternate between the two halves of the pool and the scratch's two streams,
+    // so a chunk's DP starts while the previous chunk's last groups and walk finish; each half's
+    // next chunk follows that half's walk in its stream's order. The systolic chunks run in
+    // stream order after them, each DP after the previous chunk's walk.
+    if (ng) {
+        hy::TraceArgs a = trace_args(db, s, m);
+        // Wavefronts per group: up to eight query tiles in flight (HYALITE_HIP_TRACE16_WAVES=1..8
+        // sets it).
+        const int tiles = (m + hy::kTrace16Rows - 1) / hy::kTrace16Rows;
+        int waves = std::max(1, std::min(8, tiles));
+        if (const char* v = std::getenv("HYALITE_HIP_TRACE16_WAVES")) {
+            const int x = std::atoi(v);
+            if (x >= 1 && x <= 8) waves = x;
+        }
+        a.rows = waves;
+        a.groups16 = db->d_groups16;
+        a.words16 = db->d_words16;
+        a.lens16 = db->d_lens16;
+        a.gidx16 = db->d_gidx16;
+        a.spill16 = reinterpret_cast<uint2*>(s->d_spill);
+        a.gslab = s->d_gslab;
+        a.item_group = s->d_titem;
+        // Chunks of consecutive groups whose codes fit the pool (at least one group each); a
+        // group the packed traceback does not take ends a chunk.
+        const uint64_t half = s->codes_words / 2;
+        const bool two = half >= one;  // each half holds the largest group's codes
+        const uint64_t cap = two ? half : s->codes_words;
+        if (two && (hipEventRecord(s->fork, s->stream) != hipSuccess ||
+                    hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess))
+            return HY_E_DEVICE;
+        uint64_t chunk = 0;
+        for (uint64_t g0 = 0; g0 < ng;) {
+            if (!traced16(g0)) {
+                ++g0;
+                continue;
+            }
+            uint64_t g1 = g0 + 1;
+            while (g1 < ng && traced16(g1) && gs[g1 + 1] - gs[g0] <= cap) ++g1;
+            const bool odd = two && (chunk & 1);
+            a.codes = s->d_codes + (odd ? half : 0);
+            a.group_first = (uint32_t)g0;
+            a.ngroups = (uint32_t)(g1 - g0);
+            a.slot0 = g16[g0].first;
+            a.nitems = g16[g1 - 1].first + g16[g1 - 1].count - a.slot0;
+            a.gslab_base = gs[g0];
+            if (hy::launch_trace16((int)db->mode, a, odd ? s->aux : s->stream) != hipSuccess)
+                return HY_E_DEVICE;
+            g0 = g1;
+            ++chunk;
+        }
+        if (two && (hipEventRecord(s->join, s->aux) != hipSuccess ||
+                    hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess))
+            return HY_E_DEVICE;
+    }
+    if (nlist) {
+        hy::TraceArgs a = trace_args(db, s, m);
+        a.slab_off = s->d_slab_off;
+        a.list = packed ? db->d_trace_rest : nullptr;
+        // Chunks of consecutive list entries whose slabs fit the pool (at least one each).
+        for (uint64_t k0 = 0; k0 < nlist;) {
+            uint64_t k1 = k0 + 1;
+            while (k1 < nlist && so[k1 + 1] - so[k0] <= s->codes_words) ++k1;
+            a.first = (uint32_t)k0;
+            a.nitems = (uint32_t)(k1 - k0);
+            a.slab_base = so[k0];
+            if (hy::launch_trace((int)db->mode, a, s->stream) != hipSuccess) return HY_E_DEVICE;
+            k0 = k1;
+        }
     }
     // The caller's buffers are pageable: they come back through the pinned ring when large.
     const hy::Segment segs[2] = {{out, s->d_tout, n * sizeof(hy::TraceOut)},

@@ -578,6 +578,17 @@ extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_
     if (dev_upload(&db->d_sys_list, db->sys_list) != hipSuccess) return fail(HY_E_DEVICE);
     if (dev_upload(&db->d_sys_list_batch, db->sys_list_batch) != hipSuccess)
         return fail(HY_E_DEVICE);
+    // Targets outside the packed traceback: not in the packed layout, or in a group whose columns
+    // exceed the 16-bit end tracking.
+    {
+        std::vector<char> packed_tb(n, 0);
+        for (const Group& g : db->groups16)
+            if (g.ncols <= 65535u)
+                for (uint32_t k = 0; k < g.count; ++k) packed_tb[idx16[g.first + k]] = 1;
+        for (uint32_t k = 0; k < n; ++k)
+            if (!packed_tb[k]) db->trace_rest.push_back(k);
+        if (dev_upload(&db->d_trace_rest, db->trace_rest) != hipSuccess) return fail(HY_E_DEVICE);
+    }
     (void)rc;
     *out = db;
     return HY_OK;

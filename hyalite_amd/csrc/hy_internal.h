@@ -166,6 +166,20 @@ struct TraceArgs {
     int4* ends;               // per item: (score, end row, end column, database index)
     uint8_t* ops;             // winner: ops at [0, m + n); list: target g at g * m + off[g]
     TraceOut* out;            // winner: out[0]; list: out[database index]
+    const uint32_t* list;     // non-null: item k is database index list[first + k] (the slab
+                              // offsets then follow list positions)
+    // Packed form (k_trace16): groups [group_first, group_first + ngroups) of the packed 16-bit
+    // layout, item k = slot slot0 + k (its end cell and group at index slot0 + k of ends and
+    // item_group); codes of group g at gslab[g] - gslab_base.
+    const Group* groups16;
+    const uint2* words16;
+    const uint32_t* lens16;
+    const uint32_t* gidx16;
+    uint2* spill16;           // boundary rows between query tiles (the scan's layout)
+    uint32_t group_first, ngroups, slot0;
+    const uint64_t* gslab;
+    uint64_t gslab_base;
+    uint32_t* item_group;     // per item: its group (written by the DP, read by the walk)
 };
 
 // Rows per lane of the traceback DP for a query of m rows (2, 4, 6 or 8; 8 beyond 384 rows).
@@ -177,6 +191,18 @@ inline uint64_t trace_slab_words(int m, uint64_t n) {
     uint64_t w = npass * (n + 63) * 64 + (npass > 1 ? 2 * (n + 1) : 0);
     return (w + 63) & ~63ull;
 }
+
+// Packed traceback DP (k_trace16): query rows per tile, and move-code words of one group for a
+// query of m rows: per tile, column and 4-row block, one 32-bit word per lane (each 16-bit half
+// holds its target's four 4-bit codes).
+constexpr int kTrace16Rows = 16;
+inline uint64_t trace16_group_words(int m, uint32_t ncols) {
+    const uint64_t tiles = ((uint64_t)m + kTrace16Rows - 1) / kTrace16Rows;
+    return tiles * (uint64_t)ncols * (kTrace16Rows / 4) * 64;
+}
+template <int MODE>
+hipError_t launch_trace16_m(const TraceArgs& a, hipStream_t st);
+hipError_t launch_trace16(int mode, const TraceArgs& a, hipStream_t stream);
 
 // Independent (query, target) pairs (align_pairs, src/kernel.rs:429-452), one pair per lane.
 // Pairs are sorted by (query length, target length), descending, and cut into groups of 64;

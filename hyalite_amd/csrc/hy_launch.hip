@@ -486,6 +486,17 @@ hipError_t launch_trace(int mode, const TraceArgs& a, hipStream_t stream) {
     }
 }
 
+hipError_t launch_trace16(int mode, const TraceArgs& a, hipStream_t stream) {
+    switch (mode) {
+    case kSW: return launch_trace16_m<kSW>(a, stream);
+    case kNW: return launch_trace16_m<kNW>(a, stream);
+    case kHW: return launch_trace16_m<kHW>(a, stream);
+    case kOV: return launch_trace16_m<kOV>(a, stream);
+    case kSHW: return launch_trace16_m<kSHW>(a, stream);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_sys32(int mode, bool ends, bool winner, const SysArgs& a, uint32_t nblocks,
                         hipStream_t stream, uint32_t nq_grid) {
     if ((!winner && nblocks == 0) || nq_grid == 0) return hipSuccess;

@@ -68,6 +68,10 @@ struct hy_db {
     std::vector<uint32_t> sys_list;
     uint32_t* d_sys_list = nullptr;
     uint64_t sys_max_len = 0;
+    // Targets the packed traceback (k_trace16) does not take, in database order: those outside
+    // the packed layout and those in groups wider than 65,535 columns.
+    std::vector<uint32_t> trace_rest;
+    uint32_t* d_trace_rest = nullptr;
     // Long targets the packed kernel cannot take: the batched form's systolic list.
     std::vector<uint32_t> sys_list_batch;
     uint32_t* d_sys_list_batch = nullptr;
@@ -80,6 +84,7 @@ struct hy_db {
         hy::dev_free(d_groups16); hy::dev_free(d_words16); hy::dev_free(d_lens16); hy::dev_free(d_gidx16);
         hy::dev_free(d_sys_list);
         hy::dev_free(d_sys_list_batch);
+        hy::dev_free(d_trace_rest);
     }
 };
 
@@ -151,6 +156,12 @@ struct hy_scratch {
     uint64_t codes_words = 0;
     int4* d_tends = nullptr;
     uint64_t tends_cap = 0;
+    uint32_t* d_titem = nullptr;         // packed traceback: each item's group
+    uint64_t titem_cap = 0;
+    uint64_t* d_gslab = nullptr;         // packed traceback: code words before each group
+    std::vector<uint64_t> h_gslab;
+    int gslab_m = -1;                    // query length the group offsets were laid out for
+    int slab_list = -1;                  // slab offsets laid out over: 1 the rest list, 0 all
     hy::TraceOut* d_tout = nullptr;
     uint64_t tout_cap = 0;
     uint8_t* d_ops = nullptr;
@@ -173,7 +184,7 @@ struct hy_scratch {
         hy::dev_free(d_bq); hy::dev_free(d_bmeta); hy::dev_free(d_bkeys); hy::dev_free(d_bres);
         hy::dev_free(d_slots); hy::dev_free(d_bwin_spill); hy::dev_free(d_btail_spill);
         hy::dev_free(d_codes); hy::dev_free(d_tends); hy::dev_free(d_tout); hy::dev_free(d_ops);
-        hy::dev_free(d_slab_off); hy::dev_free(d_chain);
+        hy::dev_free(d_slab_off); hy::dev_free(d_titem); hy::dev_free(d_gslab); hy::dev_free(d_chain);
         if (h_chain_err) (void)hipHostFree(h_chain_err);
         if (h_tout) (void)hipHostFree(h_tout);
         if (h_ops) (void)hipHostFree(h_ops);

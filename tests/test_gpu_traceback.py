@@ -263,3 +263,37 @@ def test_scan_paths_track_ends_on_an_alignment_database():
     for k, t in enumerate(seqs):
         ws, wq, wt = orc.align_core(q, t, al, mat, go, ge, MODE_SW)
         assert (s[k], qe[k], te[k]) == (ws, -1 if wq is None else wq, -1 if wt is None else wt)
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_packed_traceback_beside_the_systolic_one(mode, monkeypatch):
+    """scan_all_aligned runs the targets of the packed 16-bit layout in the packed traceback
+    (k_trace16: tiles of 16 rows, codes per tile, column and 4-row block) and the others (here:
+    targets of 2,980-3,300 residues whose proof needs 32 bits at max_query_len 3,000) in the
+    systolic one, in the same call: every alignment equals the all-systolic run's and the
+    oracle's, with long packable targets (past the long-target threshold) and empty ones among
+    them, with a code pool small enough to split the run into chunks of single groups."""
+    rng = np.random.default_rng(60 + mode)
+    q = rng.integers(0, 20, size=300, dtype=np.uint8)
+    seqs = protein_pairs(rng, 300, 0, 700) + [mutate(rng, q) for _ in range(10)]
+    seqs += protein_pairs(rng, 6, 1200, 2500)           # long, packable
+    seqs += protein_pairs(rng, 3, 2980, 3300)           # 32-bit
+    seqs += [np.zeros(0, dtype=np.uint8)] * 3
+    seqs[5] = np.concatenate([rng.integers(0, 20, 2990, dtype=np.uint8), q[:200]])
+    scheme = (20, BLOSUM62, 11, 1)
+    db = build(seqs, scheme, mode, 3000)
+    assert db.partition_sizes()[hy.ScoreWidth.I32] >= 3
+    sc = hy.Scratch(db)
+    got = [a.as_tuple() for a in db.scan_all_aligned(sc, q)]
+    monkeypatch.setenv("HYALITE_HIP_TRACE16", "0")
+    systolic = [a.as_tuple() for a in db.scan_all_aligned(sc, q)]
+    assert got == systolic
+    monkeypatch.delenv("HYALITE_HIP_TRACE16")
+    idx = list(range(0, len(seqs), 7)) + list(range(len(seqs) - 20, len(seqs)))
+    al, mat, go, ge = scheme
+    for k in idx:
+        assert got[k] == orc.traceback(q, seqs[k], al, mat, go, ge, mode), k
+    monkeypatch.setenv("HYALITE_HIP_TRACE_POOL_BYTES", "1")
+    sc2 = hy.Scratch(db)
+    assert [a.as_tuple() for a in db.scan_all_aligned(sc2, q[:150])] == [
+        a.as_tuple() for a in db.scan_all_aligned(sc, q[:150])]
