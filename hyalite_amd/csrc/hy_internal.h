@@ -195,7 +195,7 @@ inline uint64_t trace_slab_words(int m, uint64_t n) {
 // Packed traceback DP (k_trace16): query rows per tile, and move-code words of one group for a
 // query of m rows: per tile, column and 4-row block, one 32-bit word per lane (each 16-bit half
 // holds its target's four 4-bit codes).
-constexpr int kTrace16Rows = 16;
+constexpr int kTrace16Rows = 32;
 inline uint64_t trace16_group_words(int m, uint32_t ncols) {
     const uint64_t tiles = ((uint64_t)m + kTrace16Rows - 1) / kTrace16Rows;
     return tiles * (uint64_t)ncols * (kTrace16Rows / 4) * 64;
