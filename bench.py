@@ -549,10 +549,14 @@ def traceback(w: wl.Workload, sync) -> tuple[dict, dict]:
         "scan_all_aligned_s": round(all_s, 4),
         "alignments_per_s": round(w.n / all_s, 1),
         "gcups_equivalent": round(cells / all_s / 1e9, 1),
-        "ops_bytes_to_host": int(ops.size),
+        # The ops come back packed in database order: their total, not the caller's buffer.
+        "ops_bytes_to_host": int(recs["n_ops"].sum()),
+        "record_bytes_to_host": int(recs.nbytes),
     }
     log(f"traceback: scan_aligned {out['scan_aligned_ms']} ms (scan {out['scan_ms']} ms), "
-        f"scan_all_aligned {all_s:.3f} s = {out['alignments_per_s']:.0f} alignments/s")
+        f"scan_all_aligned {all_s:.3f} s = {out['alignments_per_s']:.0f} alignments/s "
+        f"({out['ops_bytes_to_host'] / 1e6:.0f} MB of ops, "
+        f"{out['record_bytes_to_host'] / 1e6:.0f} MB of records)")
     del sc, db
     return out, {"query": q, "sample": sample}
 

@@ -52,6 +52,7 @@ int grow(T** p, uint64_t* cap, uint64_t need) {
 void account(hy_scratch* s) {
     s->trace_bytes = s->codes_words * 4 + s->tends_cap * sizeof(int4) +
                      s->titem_cap * sizeof(uint32_t) + s->tout_cap * sizeof(hy::TraceOut) +
+                     s->opsum_cap * sizeof(uint64_t) +
                      s->ops_cap + (s->d_slab_off ? (s->db->n + 1) * sizeof(uint64_t) : 0) +
                      (s->d_gslab ? (s->db->groups16.size() + 1) * sizeof(uint64_t) : 0);
 }
@@ -222,9 +223,12 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
         (rc = grow(&s->d_tends, &s->tends_cap, n)) != HY_OK ||
         (rc = grow(&s->d_titem, &s->titem_cap, packed ? n : 1)) != HY_OK ||
         (rc = grow(&s->d_tout, &s->tout_cap, n)) != HY_OK ||
-        (rc = grow(&s->d_ops, &s->ops_cap, std::max<uint64_t>(ops_need, 1))) != HY_OK)
+        (rc = grow(&s->d_ops, &s->ops_cap, std::max<uint64_t>(ops_need, 1))) != HY_OK ||
+        (rc = grow(&s->d_opsum, &s->opsum_cap, hy::ops_sum_words(n))) != HY_OK)
         return rc;
     account(s);
+    if (!s->h_key && hipHostMalloc((void**)&s->h_key, sizeof(unsigned long long)) != hipSuccess)
+        return HY_E_OUT_OF_MEMORY;
 
     if ((rc = hy::stage_query(db, s, query, qlen)) != HY_OK) return rc;
     // Packed chunks alternate between the two halves of the pool and the scratch's two streams,
@@ -233,15 +237,14 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
     // stream order after them, each DP after the previous chunk's walk.
     if (ng) {
         hy::TraceArgs a = trace_args(db, s, m);
-        // Wavefronts per group: up to eight query tiles in flight (HYALITE_HIP_TRACE16_WAVES=1..8
-        // sets it).
-        const int tiles = (m + hy::kTrace16Rows - 1) / hy::kTrace16Rows;
-        int waves = std::max(1, std::min(8, tiles));
+        // Wavefronts per group: one query tile each, as many as kTrace16MaxWaves, the tiles and
+        // the workgroup's LDS allow (HYALITE_HIP_TRACE16_WAVES=1..8 asks for fewer).
+        int want = hy::kTrace16MaxWaves;
         if (const char* v = std::getenv("HYALITE_HIP_TRACE16_WAVES")) {
             const int x = std::atoi(v);
-            if (x >= 1 && x <= 8) waves = x;
+            if (x >= 1 && x <= hy::kTrace16MaxWaves) want = x;
         }
-        a.rows = waves;
+        a.rows = hy::trace16_waves((int)db->al_eff, m, want);
         a.groups16 = db->d_groups16;
         a.words16 = db->d_words16;
         a.lens16 = db->d_lens16;
@@ -257,7 +260,17 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
         if (two && (hipEventRecord(s->fork, s->stream) != hipSuccess ||
                     hipStreamWaitEvent(s->aux, s->fork, 0) != hipSuccess))
             return HY_E_DEVICE;
-        uint64_t chunk = 0;
+        // Each chunk goes to the stream whose queued chunks are estimated to end first (its pool
+        // half with it). The estimate, in wavefront column steps: the longer of the chunk's
+        // longest group (passes of its columns plus the pipeline fill, times the wavefronts that
+        // share a SIMD) and its tile columns spread over the device's SIMDs, which the two
+        // streams share; plus the walk, about a fiftieth of a step per target. Groups are
+        // sorted longest first, so the first chunk is the one its longest group bounds.
+        const int W = a.rows;
+        const uint64_t tiles = (uint64_t)(m + hy::kTrace16Rows - 1) / hy::kTrace16Rows;
+        const uint64_t passes = (tiles + W - 1) / W;
+        const double simds = 4.0 * hy::device_cus();
+        double queued[2] = {0.0, 0.0};
         for (uint64_t g0 = 0; g0 < ng;) {
             if (!traced16(g0)) {
                 ++g0;
@@ -265,7 +278,14 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
             }
             uint64_t g1 = g0 + 1;
             while (g1 < ng && traced16(g1) && gs[g1 + 1] - gs[g0] <= cap) ++g1;
-            const bool odd = two && (chunk & 1);
+            double tile_cols = 0.0;
+            for (uint64_t g = g0; g < g1; ++g) tile_cols += (double)tiles * g16[g].ncols;
+            const double longest =
+                (double)passes * (g16[g0].ncols + W - 1) * std::max(1.0, W / 4.0);
+            const double items = (double)(g16[g1 - 1].first + g16[g1 - 1].count - g16[g0].first);
+            const double est = std::max(longest, 2.0 * tile_cols / simds) + items / 50.0;
+            const bool odd = two && queued[1] < queued[0];
+            queued[odd ? 1 : 0] += est;
             a.codes = s->d_codes + (odd ? half : 0);
             a.group_first = (uint32_t)g0;
             a.ngroups = (uint32_t)(g1 - g0);
@@ -275,7 +295,6 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
             if (hy::launch_trace16((int)db->mode, a, odd ? s->aux : s->stream) != hipSuccess)
                 return HY_E_DEVICE;
             g0 = g1;
-            ++chunk;
         }
         if (two && (hipEventRecord(s->join, s->aux) != hipSuccess ||
                     hipStreamWaitEvent(s->stream, s->join, 0) != hipSuccess))
@@ -296,14 +315,30 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
             k0 = k1;
         }
     }
+    // The ops are laid end to end in database order in the code pool (free once the walks have
+    // ended), so that only their bytes come back instead of every target's whole op region.
+    // The total is read first: the pool grows when it cannot hold them.
+    if (hy::launch_ops_sums(s->d_tout, n, s->d_opsum, s->stream) != hipSuccess ||
+        hipMemcpyAsync(s->h_key, s->d_opsum + hy::ops_blocks(n), sizeof(unsigned long long),
+                       hipMemcpyDeviceToHost, s->stream) != hipSuccess ||
+        hipStreamSynchronize(s->stream) != hipSuccess)
+        return HY_E_DEVICE;
+    const uint64_t packed_bytes = *s->h_key;
+    if (packed_bytes > ops_need) return HY_E_INTERNAL;
+    if ((rc = grow(&s->d_codes, &s->codes_words, (packed_bytes + 3) / 4)) != HY_OK) return rc;
+    account(s);
+    uint8_t* d_packed = reinterpret_cast<uint8_t*>(s->d_codes);
+    if (hy::launch_ops_pack(s->d_tout, n, s->d_opsum, s->d_ops, d_packed, s->stream) !=
+        hipSuccess)
+        return HY_E_DEVICE;
     // The caller's buffers are pageable: they come back through the pinned ring when large.
     const hy::Segment segs[2] = {{out, s->d_tout, n * sizeof(hy::TraceOut)},
-                                 {ops, s->d_ops, (size_t)ops_need}};
+                                 {ops, d_packed, (size_t)packed_bytes}};
     if ((rc = hy::download(s->ring, s->device, segs, 2, s->stream)) != HY_OK) return rc;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
     for (uint64_t k = 0; k < n; ++k) {
         if (out[k].reserved != 0) return HY_E_INTERNAL;
-        if (out[k].ops_offset + out[k].n_ops > ops_need) return HY_E_INTERNAL;
+        if (out[k].ops_offset + out[k].n_ops > packed_bytes) return HY_E_INTERNAL;
     }
     return HY_OK;
 }

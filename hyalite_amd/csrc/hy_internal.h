@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <thread>
 #include <vector>
 #include <stdint.h>
@@ -199,6 +200,30 @@ constexpr int kTrace16Rows = 32;
 inline uint64_t trace16_group_words(int m, uint32_t ncols) {
     const uint64_t tiles = ((uint64_t)m + kTrace16Rows - 1) / kTrace16Rows;
     return tiles * (uint64_t)ncols * (kTrace16Rows / 4) * 64;
+}
+// Its workgroup of W wavefronts (at most kTrace16MaxWaves; twelve, one pass of a 360-row query,
+// measured slower than eight) takes in LDS: per wavefront its tile profile, al_eff + 1 symbols
+// of trace16_profile_stride() words (half the tile rows, padded so that a symbol's row starts a
+// 16-byte line on alternate banks); W - 1 double-buffered row hand-offs of 64 lanes; W x 64
+// lanes x 6 ints for the fold of the bests. trace16_waves gives the most wavefronts, up to
+// `want` and the tile count, whose LDS fits a launch's 64 KB (one always does: al_eff <= 255 in
+// the packed layout).
+constexpr int kTrace16MaxWaves = 8;
+constexpr int trace16_profile_stride() {
+    return ((kTrace16Rows / 2) % 8 == 4) ? kTrace16Rows / 2 : kTrace16Rows / 2 + 4;
+}
+inline uint32_t trace16_pp_words(int al_eff) {
+    return ((uint32_t)(al_eff + 1) * (uint32_t)trace16_profile_stride() + 3u) & ~3u;
+}
+inline size_t trace16_lds_bytes(int al_eff, int waves) {
+    return (size_t)waves * trace16_pp_words(al_eff) * 4 + (size_t)(waves - 1) * 128 * 8 +
+           (size_t)waves * 64 * 6 * 4;
+}
+inline int trace16_waves(int al_eff, int m, int want) {
+    const int tiles = (m + kTrace16Rows - 1) / kTrace16Rows;
+    int w = std::max(1, std::min(std::min(want, kTrace16MaxWaves), tiles));
+    while (w > 1 && trace16_lds_bytes(al_eff, w) > 64u * 1024u) --w;
+    return w;
 }
 template <int MODE>
 hipError_t launch_trace16_m(const TraceArgs& a, hipStream_t st);
@@ -465,6 +490,17 @@ uint32_t inter16_lds_bytes(int al_eff, int q);
 // LDS bytes of the per-symbol profile (one-wavefront workgroups of larger alphabets).
 uint32_t inter16p_lds_bytes(int al_eff, int q);
 hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream);
+// Op packing (scan_all_aligned) in two launches around a host read of the total:
+// launch_ops_sums leaves in bsum (ops_sum_words(n) entries) each block of kOpsBlock targets'
+// first packed offset and in bsum[ops_blocks(n)] the total; launch_ops_pack then moves target k's
+// n_ops bytes from src + out[k].ops_offset to dst + (that packed offset) and rewrites
+// out[k].ops_offset to it. dst holds at least the total and does not overlap src.
+constexpr int kOpsBlock = 1024;
+inline uint64_t ops_blocks(uint64_t n) { return (n + kOpsBlock - 1) / kOpsBlock; }
+inline uint64_t ops_sum_words(uint64_t n) { return ops_blocks(n) + 1; }
+hipError_t launch_ops_sums(const TraceOut* out, uint64_t n, uint64_t* bsum, hipStream_t stream);
+hipError_t launch_ops_pack(TraceOut* out, uint64_t n, const uint64_t* bsum, const uint8_t* src,
+                           uint8_t* dst, hipStream_t stream);
 // Batched form: zero n keys and the queue counter; write a Result per query from its key.
 hipError_t launch_batch_reset(unsigned long long* keys, uint32_t n, unsigned int* counter,
                               hipStream_t stream);

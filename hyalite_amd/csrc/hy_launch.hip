@@ -232,6 +232,98 @@ __global__ void k_key_result(const unsigned long long* key, Result* r) {
     }
 }
 
+// Op packing (scan_all_aligned): each walk leaves target k's ops at the back of its region of
+// the op buffer; these kernels lay them end to end in database order. Block b covers targets
+// b * kOpsBlock .. + kOpsBlock - 1, four consecutive targets per thread.
+static_assert(kOpsBlock == 4 * 256, "four targets per thread, 256 threads");
+
+// Inclusive sum over the 256 threads of a block (x is this thread's value); tmp: 256 entries.
+__device__ __forceinline__ uint64_t block_scan256(uint64_t x, uint64_t* tmp) {
+    const int t = threadIdx.x;
+    tmp[t] = x;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+        const uint64_t y = t >= d ? tmp[t - d] : 0ull;
+        __syncthreads();
+        tmp[t] += y;
+        __syncthreads();
+    }
+    const uint64_t r = tmp[t];
+    __syncthreads();  // tmp is free again for the caller's next use
+    return r;
+}
+
+__global__ __launch_bounds__(256) void k_ops_blocksum(const TraceOut* out, uint64_t n,
+                                                      uint64_t* bsum) {
+    __shared__ uint64_t tmp[256];
+    const uint64_t i0 = (uint64_t)blockIdx.x * kOpsBlock + 4u * threadIdx.x;
+    uint64_t s = 0;
+    for (int k = 0; k < 4; ++k)
+        if (i0 + k < n) s += out[i0 + k].n_ops;
+    const uint64_t incl = block_scan256(s, tmp);
+    if (threadIdx.x == 255) bsum[blockIdx.x] = incl;
+}
+
+// One block: bsum[0 .. nb) becomes its exclusive prefix sums, bsum[nb] the total.
+__global__ __launch_bounds__(256) void k_ops_scan_blocks(uint64_t* bsum, uint64_t nb) {
+    __shared__ uint64_t tmp[256];
+    uint64_t carry = 0;
+    for (uint64_t c = 0; c < nb; c += 256) {
+        const uint64_t i = c + threadIdx.x;
+        const uint64_t x = i < nb ? bsum[i] : 0ull;
+        const uint64_t incl = block_scan256(x, tmp);
+        if (i < nb) bsum[i] = carry + incl - x;
+        // Thread 255's inclusive sum is the chunk's total: every thread reads it from tmp.
+        if (threadIdx.x == 255) tmp[0] = incl;
+        __syncthreads();
+        carry += tmp[0];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+// Targets' new offsets (block base bsum[b] plus the block's own prefix), written to
+// out[k].ops_offset after its source offset is read, then the block's output range
+// [noff[0], noff[cnt]) copied byte by byte: consecutive threads take consecutive output bytes,
+// so loads and stores stay coalesced within a target, and each thread walks its target index
+// forward as its byte advances.
+__global__ __launch_bounds__(256) void k_ops_compact(TraceOut* out, uint64_t n,
+                                                     const uint64_t* bsum, const uint8_t* src,
+                                                     uint8_t* dst) {
+    __shared__ uint64_t tmp[256];
+    __shared__ uint64_t noff[kOpsBlock + 1];
+    __shared__ uint64_t soff[kOpsBlock];
+    const uint64_t b0 = (uint64_t)blockIdx.x * kOpsBlock;
+    const int cnt = (int)min<uint64_t>((uint64_t)kOpsBlock, n - b0);
+    const int t = threadIdx.x;
+    uint64_t len[4], src_off[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int j = 4 * t + k;
+        len[k] = j < cnt ? out[b0 + j].n_ops : 0ull;
+        src_off[k] = j < cnt ? out[b0 + j].ops_offset : 0ull;
+        s += len[k];
+    }
+    uint64_t o = bsum[blockIdx.x] + block_scan256(s, tmp) - s;
+    for (int k = 0; k < 4; ++k) {
+        const int j = 4 * t + k;
+        if (j < cnt) {
+            noff[j] = o;
+            soff[j] = src_off[k];
+            out[b0 + j].ops_offset = o;
+        }
+        o += len[k];
+    }
+    // The thread holding the block's last target (cnt >= 1) ends the range.
+    if (t == (cnt - 1) / 4) noff[cnt] = o;
+    __syncthreads();
+    const uint64_t lo = noff[0], hi = noff[cnt];
+    int k = 0;
+    for (uint64_t x = lo + (uint64_t)t; x < hi; x += 256) {
+        while (k + 1 < cnt && noff[k + 1] <= x) ++k;
+        dst[x] = src[soff[k] + (x - noff[k])];
+    }
+}
+
 }  // namespace
 
 hipError_t launch_inter16_batch(int mode, const Inter16Args& a, const BatchArgs& b, int q,
@@ -459,6 +551,28 @@ uint32_t inter16_batch_occupancy(int mode, int q, int al_eff) {
 
 hipError_t launch_key_result(const unsigned long long* key, Result* r, hipStream_t stream) {
     hipLaunchKernelGGL(k_key_result, dim3(1), dim3(64), 0, stream, key, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_ops_sums(const TraceOut* out, uint64_t n, uint64_t* bsum, hipStream_t stream) {
+    const uint64_t nb = ops_blocks(n);
+    if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    if (nb) {
+        hipLaunchKernelGGL(k_ops_blocksum, dim3((uint32_t)nb), dim3(256), 0, stream, out, n, bsum);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_ops_scan_blocks, dim3(1), dim3(256), 0, stream, bsum, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_ops_pack(TraceOut* out, uint64_t n, const uint64_t* bsum, const uint8_t* src,
+                           uint8_t* dst, hipStream_t stream) {
+    const uint64_t nb = ops_blocks(n);
+    if (nb == 0) return hipSuccess;
+    if (nb > 0x7FFFFFFFull) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_ops_compact, dim3((uint32_t)nb), dim3(256), 0, stream, out, n, bsum, src,
+                       dst);
     return hipGetLastError();
 }
 

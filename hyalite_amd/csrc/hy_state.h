@@ -166,6 +166,8 @@ struct hy_scratch {
     uint64_t tout_cap = 0;
     uint8_t* d_ops = nullptr;
     uint64_t ops_cap = 0;
+    uint64_t* d_opsum = nullptr;         // op packing: per-block offsets and the total
+    uint64_t opsum_cap = 0;
     hy::PinnedRing ring;  // large pageable copies back (scan_all_aligned)
     uint64_t* d_slab_off = nullptr;
     int slab_m = -1;                     // query length the slab offsets were laid out for
@@ -185,6 +187,7 @@ struct hy_scratch {
         hy::dev_free(d_slots); hy::dev_free(d_bwin_spill); hy::dev_free(d_btail_spill);
         hy::dev_free(d_codes); hy::dev_free(d_tends); hy::dev_free(d_tout); hy::dev_free(d_ops);
         hy::dev_free(d_slab_off); hy::dev_free(d_titem); hy::dev_free(d_gslab); hy::dev_free(d_chain);
+        hy::dev_free(d_opsum);
         if (h_chain_err) (void)hipHostFree(h_chain_err);
         if (h_tout) (void)hipHostFree(h_tout);
         if (h_ops) (void)hipHostFree(h_ops);

@@ -301,13 +301,8 @@ __device__ __forceinline__ int tborder16(int i, int j, int go, int ge) {
     return MF<MODE>::left ? 0 : sat16(-(long long)gap32(go, ge, i));
 }
 
-// Words of one wavefront's tile profile (rounded to whole 16-byte rows).
-__host__ __device__ constexpr int trace16_profile_stride() {
-    return ((kTrace16Rows / 2) % 8 == 4) ? kTrace16Rows / 2 : kTrace16Rows / 2 + 4;
-}
-
 template <int MODE>
-__global__ __launch_bounds__(512) void k_trace16(TraceArgs a, uint32_t pp_words) {
+__global__ __launch_bounds__(64 * kTrace16MaxWaves) void k_trace16(TraceArgs a, uint32_t pp_words) {
     using F = MF<MODE>;
     constexpr int Q = kTrace16Rows;
     static_assert(Q % 8 == 0, "tile rows: whole profile words and 4-row code blocks");
@@ -575,10 +570,9 @@ hipError_t launch_trace16_m(const TraceArgs& a, hipStream_t st) {
     static_assert(MODE == HY_KMODE, "one mode per translation unit");
     if (a.ngroups == 0) return hipSuccess;
     const int W = a.rows;
-    if (W < 1 || W > 8) return hipErrorInvalidValue;
-    const uint32_t pp_words = ((uint32_t)(a.al_eff + 1) * trace16_profile_stride() + 3u) & ~3u;
-    const size_t lds = (size_t)W * pp_words * 4 + (size_t)(W - 1) * 128 * sizeof(uint2) +
-                       (size_t)W * 64 * 6 * sizeof(int);
+    if (W < 1 || W > kTrace16MaxWaves) return hipErrorInvalidValue;
+    const uint32_t pp_words = trace16_pp_words(a.al_eff);
+    const size_t lds = trace16_lds_bytes(a.al_eff, W);
     if (lds > 64u * 1024u) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_trace16<MODE>), dim3(a.ngroups), dim3(64 * W), lds, st, a, pp_words);
     const hipError_t e = hipGetLastError();

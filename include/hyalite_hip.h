@@ -194,14 +194,15 @@ int hy_scan_aligned(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t
                     uint64_t* db_index, hy_alignment* out, uint8_t* ops, size_t ops_cap);
 
 /* Bytes of the ops buffer hy_scan_all_aligned needs for a query of qlen symbols:
- * n * qlen + the database's total residues (target k's ops lie in [k * qlen + offset_k,
- * (k + 1) * qlen + offset_k + len_k), offset_k = residues of the targets before k). */
+ * n * qlen + the database's total residues, the most ops n alignments can have (an alignment
+ * of the query against target k has at most qlen + len_k ops). */
 int hy_db_aligned_ops_bytes(const hy_db* db, size_t qlen, uint64_t* bytes);
 
 /* One alignment per target in database order (Database::scan_all_aligned): out has n entries,
  * out[k].ops_offset locates target k's ops in `ops` (ops_cap >= hy_db_aligned_ops_bytes). The
- * pairs run in chunks whose move-code slabs fit the scratch's traceback pool (4 GiB, or
- * HYALITE_HIP_TRACE_POOL_BYTES when set). */
+ * ops are packed in database order: out[0].ops_offset = 0 and target k + 1's ops follow target
+ * k's, so only their total is written. The pairs run in chunks whose move codes fit the
+ * scratch's traceback pool (8 GiB, or HYALITE_HIP_TRACE_POOL_BYTES when set). */
 int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t* query, size_t qlen,
                         hy_alignment* out, uint8_t* ops, size_t ops_cap);
 

@@ -268,7 +268,7 @@ def test_scan_paths_track_ends_on_an_alignment_database():
 @pytest.mark.parametrize("mode", ALL_MODES)
 def test_packed_traceback_beside_the_systolic_one(mode, monkeypatch):
     """scan_all_aligned runs the targets of the packed 16-bit layout in the packed traceback
-    (k_trace16: tiles of 16 rows, codes per tile, column and 4-row block) and the others (here:
+    (k_trace16: tiles of 32 rows, codes per tile, column and 4-row block) and the others (here:
     targets of 2,980-3,300 residues whose proof needs 32 bits at max_query_len 3,000) in the
     systolic one, in the same call: every alignment equals the all-systolic run's and the
     oracle's, with long packable targets (past the long-target threshold) and empty ones among
@@ -285,6 +285,12 @@ def test_packed_traceback_beside_the_systolic_one(mode, monkeypatch):
     assert db.partition_sizes()[hy.ScoreWidth.I32] >= 3
     sc = hy.Scratch(db)
     got = [a.as_tuple() for a in db.scan_all_aligned(sc, q)]
+    # The ops come back packed in database order.
+    recs, ops = db.scan_all_aligned_arrays(sc, q)
+    n_ops = recs["n_ops"].astype(np.uint64)
+    assert np.array_equal(recs["ops_offset"],
+                          np.concatenate([[0], np.cumsum(n_ops)[:-1]]).astype(np.uint64))
+    assert int(n_ops.sum()) <= ops.size
     monkeypatch.setenv("HYALITE_HIP_TRACE16", "0")
     systolic = [a.as_tuple() for a in db.scan_all_aligned(sc, q)]
     assert got == systolic
@@ -297,3 +303,33 @@ def test_packed_traceback_beside_the_systolic_one(mode, monkeypatch):
     sc2 = hy.Scratch(db)
     assert [a.as_tuple() for a in db.scan_all_aligned(sc2, q[:150])] == [
         a.as_tuple() for a in db.scan_all_aligned(sc, q[:150])]
+
+
+@pytest.mark.parametrize("mode", [MODE_SW, MODE_NW, MODE_OV])
+def test_packed_traceback_large_alphabet_fits_its_lds(mode, monkeypatch):
+    """An alphabet of 120 symbols: each wavefront of the packed traceback stages 121 symbols of
+    its tile profile in LDS, so a 300-row query (10 tiles) runs with as many wavefronts as fit
+    the workgroup's 64 KB (5), not one per tile; every alignment equals the all-systolic run's
+    and the oracle's."""
+    rng = np.random.default_rng(80 + mode)
+    al = 120
+    mat = rng.integers(-4, 4, size=(al, al))
+    mat = np.triu(mat) + np.triu(mat, 1).T
+    np.fill_diagonal(mat, rng.integers(4, 12, size=al))
+    scheme = (al, [int(x) for x in mat.ravel()], 11, 1)
+    q = rng.integers(0, al, size=300, dtype=np.uint8)
+    seqs = [rng.integers(0, al, size=int(rng.integers(40, 400)), dtype=np.uint8)
+            for _ in range(250)]
+    for k in range(0, 250, 25):
+        at = int(rng.integers(0, 100))
+        seqs[k] = np.concatenate([seqs[k][:at], q[20:220], seqs[k][at:]])
+    db = build(seqs, scheme, mode, 300)
+    assert db.partition_sizes()[hy.ScoreWidth.I32] == 0  # every target in the packed layout
+    sc = hy.Scratch(db)
+    got = [a.as_tuple() for a in db.scan_all_aligned(sc, q)]
+    monkeypatch.setenv("HYALITE_HIP_TRACE16", "0")
+    assert got == [a.as_tuple() for a in db.scan_all_aligned(sc, q)]
+    monkeypatch.delenv("HYALITE_HIP_TRACE16")
+    al_, mat_, go, ge = scheme
+    for k in list(range(0, len(seqs), 9)):
+        assert got[k] == orc.traceback(q, seqs[k], al_, mat_, go, ge, mode), k
