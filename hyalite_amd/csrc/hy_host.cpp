@@ -804,9 +804,11 @@ int pipe_rows(const hy_db* db, int m, uint32_t ngroups) {
 // valid shape whatever its size; HYALITE_HIP_CHAIN_ROWS (8, 16, 24, 32) and
 // HYALITE_HIP_CHAIN_WAVES (1-8) fix the shape, HYALITE_HIP_CHAIN_SLACK (1-64) and
 // HYALITE_HIP_CHAIN_EVERY (1-16) the link's reader lead and publication interval (tests and
-// tuning).
+// tuning); HYALITE_HIP_CHAIN_SPIN_LIMIT (0 up to the default) the polls before a wait gives up
+// (tests of the error path: the scan then reports HY_E_INTERNAL).
 struct ChainShape {
     int rows = 0, waves = 0, tiles = 0, chunks = 0, slack = 4, every = 1;
+    uint32_t spin_limit = hy::kChainSpinLimit;
 };
 ChainShape chain_shape(const hy_db* db, int m, uint32_t ngroups) {
     ChainShape c;
@@ -831,6 +833,10 @@ ChainShape chain_shape(const hy_db* db, int m, uint32_t ngroups) {
     if (const char* ev = std::getenv("HYALITE_HIP_CHAIN_EVERY")) {
         const int v = std::atoi(ev);
         if (v >= 1 && v <= 16) c.every = v;
+    }
+    if (const char* sp = std::getenv("HYALITE_HIP_CHAIN_SPIN_LIMIT")) {
+        const long v = std::atol(sp);
+        if (v >= 0 && v <= (long)hy::kChainSpinLimit) c.spin_limit = (uint32_t)v;
     }
     static const int kShapes[][2] = {{8, 4},  {8, 8},  {16, 8}, {16, 4},
                                      {24, 4}, {24, 8}, {32, 4}, {32, 8}};
@@ -978,6 +984,7 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
             ca.chunks = cs.chunks;
             ca.slack = cs.slack;
             ca.every = cs.every;
+            ca.spin_limit = cs.spin_limit;
             if (hy::launch_inter16_chain((int)db->mode, pa, ca, cs.rows, cs.waves, s->stream) !=
                 hipSuccess)
                 return HY_E_DEVICE;

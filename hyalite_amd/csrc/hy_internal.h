@@ -594,7 +594,12 @@ struct ChainArgs {
     int chunks;                    // workgroups per group: ceil(tiles / W)
     int slack;                     // blocks a reader lets its writer publish before it starts
     int every;                     // a writer publishes its progress every `every` blocks
+    uint32_t spin_limit;           // polls before a wait gives up (kChainSpinLimit)
 };
+// Polls of a chained wait (0.5-1 us each: an L2 round trip and a short sleep) before it gives up:
+// seconds, far beyond any launch's wait, so that a reader always ends even if its writer never
+// published. Tests set fewer (HYALITE_HIP_CHAIN_SPIN_LIMIT) to take the give-up path.
+constexpr uint32_t kChainSpinLimit = 1u << 22;
 template <int MODE>
 hipError_t launch_inter16_chain_m(const Inter16Args& a, const ChainArgs& c, int q, int waves,
                                   hipStream_t st);

@@ -269,12 +269,8 @@ struct Pipe {
     int slack = 2;                            // blocks published before the reader starts
     int every = 1;                            // the writer publishes every `every` blocks
     unsigned int* err = nullptr;              // host-visible flag: a wait gave up
+    uint32_t spin_limit = kChainSpinLimit;    // polls before a wait gives up
 };
-
-// Spin limit of a chained wait (polls of 0.5-1 us each: an L2 round trip and a short sleep):
-// seconds, far beyond any launch's wait, so that a reader always ends even if its writer never
-// published.
-constexpr uint32_t kChainSpinLimit = 1u << 22;
 
 // Agent-coherent row accesses of the chained link.
 __device__ __forceinline__ uint2 chain_load(const uint2* p) {
@@ -309,7 +305,7 @@ __device__ __forceinline__ uint32_t chain_wait(const Pipe& p, const unsigned lon
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
             return n;
         }
-        if (spins >= kChainSpinLimit) {
+        if (spins >= p.spin_limit) {
             if ((threadIdx.x & 63) == 0)
                 __hip_atomic_store(p.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             return need;
@@ -1004,6 +1000,7 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_chai
     pipe.slack = c.slack;
     pipe.every = c.every;
     pipe.err = c.err;
+    pipe.spin_limit = c.spin_limit;
     if (wv == 0 && k > 0) {
         pipe.gin = link;
         pipe.pin = prog + (k - 1);

@@ -771,6 +771,42 @@ def test_chained_kernel_every_shape(mode, monkeypatch):
     assert ran > 100
 
 
+def test_chained_wait_that_gives_up_is_reported(monkeypatch):
+    """The chained kernel's cross-workgroup waits are bounded: with the bound forced to 0 polls
+    a reader gives up at its first unmet poll, the kernel still runs to its end, and the scan
+    reports an internal error instead of results (it may also find every wait already met and
+    return exact results). The same scratch's next scans, with the default bound, are exact."""
+    rng = np.random.default_rng(1750)
+    seqs = [rng.integers(0, 4, size=int(n), dtype=np.uint8) for n in rng.integers(50, 300, 3000)]
+    flat, off = orc.flatten(seqs)
+    al, mat, go, ge = 4, identity_matrix(4, 2, -1), 2, 1
+    db = build(seqs, (al, mat, go, ge), MODE_SW, hy.SearchType.ScoreEnd, 200)
+    sc = hy.Scratch(db)
+    q = rng.integers(0, al, size=200, dtype=np.uint8)
+    ws = orc.scan_all_flat(flat, off, q, al, mat, go, ge, MODE_SW, ends=False, threads=8)
+    want = orc.scan_flat(flat, off, q, al, mat, go, ge, MODE_SW, True, threads=8)
+    for var, v in (("HYALITE_HIP_CHAIN", "1"), ("HYALITE_HIP_CHAIN_ROWS", "8"),
+                   ("HYALITE_HIP_CHAIN_WAVES", "4"), ("HYALITE_HIP_CHAIN_SPIN_LIMIT", "0")):
+        monkeypatch.setenv(var, v)
+    errors = 0
+    for _ in range(4):
+        try:
+            got = db.scan_scores_array(sc, q)
+        except hy.DeviceError as e:
+            assert "status 106" in str(e), e
+            errors += 1
+        else:
+            assert np.array_equal(got, ws)
+        assert sc.last_kernel() == "chained"
+    assert errors >= 1
+    monkeypatch.delenv("HYALITE_HIP_CHAIN_SPIN_LIMIT")
+    for _ in range(2):
+        assert np.array_equal(db.scan_scores_array(sc, q), ws)
+        assert sc.last_kernel() == "chained"
+        h = db.scan(sc, q)
+        assert (h.score, h.db_index, h.query_end, h.target_end) == want
+
+
 @pytest.mark.parametrize("mode", [MODE_SW, MODE_NW, MODE_OV])
 def test_packed_groups_wider_than_16bit_columns(mode, monkeypatch):
     """Targets of 66,000-70,000 residues kept out of the systolic kernel (the long-target
