@@ -231,10 +231,10 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
         return HY_E_OUT_OF_MEMORY;
 
     if ((rc = hy::stage_query(db, s, query, qlen)) != HY_OK) return rc;
-    // Packed chunks alternate between the two halves of the pool and the scratch's two streams,
-    // so a chunk's DP starts while the previous chunk's last groups and walk finish; each half's
-    // next chunk follows that half's walk in its stream's order. The systolic chunks run in
-    // stream order after them, each DP after the previous chunk's walk.
+    // Packed chunks run on the scratch's two streams, each stream with its own half of the pool,
+    // so a chunk's DP starts while the other stream's last groups and walk finish; a half's next
+    // chunk follows that half's walk in its stream's order. The systolic chunks run in stream
+    // order after them, each DP after the previous chunk's walk.
     if (ng) {
         hy::TraceArgs a = trace_args(db, s, m);
         // Wavefronts per group: one query tile each, as many as kTrace16MaxWaves, the tiles and
