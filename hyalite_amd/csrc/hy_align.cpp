@@ -38,6 +38,7 @@ template <typename T>
 int grow(T** p, uint64_t* cap, uint64_t need) {
     need = std::max<uint64_t>(need, 1);
     if (*cap >= need) return HY_OK;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     hy::dev_free(*p);
     *p = nullptr;
     *cap = 0;
@@ -99,19 +100,24 @@ extern "C" int hy_scan_aligned(const hy_db* db, hy_scratch* s, const uint8_t* qu
     if ((uint64_t)ops_cap < (uint64_t)qlen + db->mtl) return HY_E_INVALID_ARGUMENT;
     const int m = (int)qlen;
     if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
-    if ((rc = grow(&s->d_codes, &s->codes_words, hy::trace_slab_words(m, db->mtl))) != HY_OK ||
-        (rc = grow(&s->d_tends, &s->tends_cap, 1)) != HY_OK ||
-        (rc = grow(&s->d_tout, &s->tout_cap, 1)) != HY_OK ||
-        (rc = grow(&s->d_ops, &s->ops_cap, db->mql + db->mtl)) != HY_OK)
-        return rc;
-    account(s);
-    if (!s->h_tout && hipHostMalloc((void**)&s->h_tout, sizeof(hy::TraceOut)) != hipSuccess)
-        return HY_E_OUT_OF_MEMORY;
-    if (!s->h_key && hipHostMalloc((void**)&s->h_key, sizeof(unsigned long long)) != hipSuccess)
-        return HY_E_OUT_OF_MEMORY;
-    if (!s->h_ops &&
-        hipHostMalloc((void**)&s->h_ops, std::max<uint64_t>(db->mql + db->mtl, 1)) != hipSuccess)
-        return HY_E_OUT_OF_MEMORY;
+    {
+        std::lock_guard<std::recursive_mutex> api(hy::api_lock());
+        if ((rc = grow(&s->d_codes, &s->codes_words, hy::trace_slab_words(m, db->mtl))) !=
+                HY_OK ||
+            (rc = grow(&s->d_tends, &s->tends_cap, 1)) != HY_OK ||
+            (rc = grow(&s->d_tout, &s->tout_cap, 1)) != HY_OK ||
+            (rc = grow(&s->d_ops, &s->ops_cap, db->mql + db->mtl)) != HY_OK)
+            return rc;
+        account(s);
+        if (!s->h_tout && hipHostMalloc((void**)&s->h_tout, sizeof(hy::TraceOut)) != hipSuccess)
+            return HY_E_OUT_OF_MEMORY;
+        if (!s->h_key &&
+            hipHostMalloc((void**)&s->h_key, sizeof(unsigned long long)) != hipSuccess)
+            return HY_E_OUT_OF_MEMORY;
+        if (!s->h_ops && hipHostMalloc((void**)&s->h_ops,
+                                       std::max<uint64_t>(db->mql + db->mtl, 1)) != hipSuccess)
+            return HY_E_OUT_OF_MEMORY;
+    }
 
     if ((rc = hy::stage_query(db, s, query, qlen)) != HY_OK) return rc;
     if (hy::launch_fill_u64(s->d_key, 1, 0ull, s->stream) != hipSuccess) return HY_E_DEVICE;
@@ -178,6 +184,7 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
             s->h_slab_off[k + 1] =
                 s->h_slab_off[k] + hy::trace_slab_words(m, db->h_off[t + 1] - db->h_off[t]);
         }
+        std::lock_guard<std::recursive_mutex> api(hy::api_lock());
         if (!s->d_slab_off &&
             hipMalloc((void**)&s->d_slab_off, (n + 1) * sizeof(uint64_t)) != hipSuccess) {
             s->d_slab_off = nullptr;
@@ -197,6 +204,7 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
         for (uint64_t g = 0; g < ng; ++g)
             s->h_gslab[g + 1] =
                 s->h_gslab[g] + (traced16(g) ? hy::trace16_group_words(m, g16[g].ncols) : 0);
+        std::lock_guard<std::recursive_mutex> api(hy::api_lock());
         if (!s->d_gslab &&
             hipMalloc((void**)&s->d_gslab, (ng + 1) * sizeof(uint64_t)) != hipSuccess) {
             s->d_gslab = nullptr;
@@ -227,8 +235,11 @@ extern "C" int hy_scan_all_aligned(const hy_db* db, hy_scratch* s, const uint8_t
         (rc = grow(&s->d_opsum, &s->opsum_cap, hy::ops_sum_words(n))) != HY_OK)
         return rc;
     account(s);
-    if (!s->h_key && hipHostMalloc((void**)&s->h_key, sizeof(unsigned long long)) != hipSuccess)
-        return HY_E_OUT_OF_MEMORY;
+    if (!s->h_key) {
+        std::lock_guard<std::recursive_mutex> api(hy::api_lock());
+        if (hipHostMalloc((void**)&s->h_key, sizeof(unsigned long long)) != hipSuccess)
+            return HY_E_OUT_OF_MEMORY;
+    }
 
     if ((rc = hy::stage_query(db, s, query, qlen)) != HY_OK) return rc;
     // Packed chunks run on the scratch's two streams, each stream with its own half of the pool,

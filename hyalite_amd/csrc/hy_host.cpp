@@ -198,12 +198,20 @@ int hy::first_gfx950_device() {
 // Build
 // ---------------------------------------------------------------------------------------------
 
+namespace hy {
+std::recursive_mutex& api_lock() {
+    static std::recursive_mutex m;
+    return m;
+}
+}  // namespace hy
+
 extern "C" int hy_db_build(const uint8_t* const* seqs, const size_t* lens, size_t n,
                            uint32_t have, const int32_t* matrix, size_t matrix_len,
                            uint32_t alphabet_len, int32_t gap_open, int32_t gap_ext, uint32_t mode,
                            uint32_t search_type, uint64_t traceback_max_bytes,
                            size_t max_query_len, uint32_t backend, hy_db** out,
                            int64_t* err_detail) {
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     if (!out) return HY_E_NULL_ARGUMENT;
     *out = nullptr;
     if (err_detail) *err_detail = 0;
@@ -601,7 +609,10 @@ extern "C" int hy_db_retain(hy_db* db) {
 }
 
 extern "C" void hy_db_free(hy_db* db) {
-    if (db && db->refs.fetch_sub(1) == 1) delete db;
+    if (db && db->refs.fetch_sub(1) == 1) {
+        std::lock_guard<std::recursive_mutex> api(hy::api_lock());
+        delete db;
+    }
 }
 
 extern "C" int hy_db_info(const hy_db* db, uint32_t* widest, uint64_t* n, uint64_t* mql,
@@ -626,6 +637,7 @@ extern "C" int hy_db_partitions(const hy_db* db, uint64_t* n8, uint64_t* n16, ui
 
 extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     if (!db || !out) return HY_E_NULL_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     *out = nullptr;
     if (hipSetDevice(db->device) != hipSuccess) return HY_E_DEVICE;
     hy_scratch* s = new (std::nothrow) hy_scratch();
@@ -702,7 +714,11 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     return HY_OK;
 }
 
-extern "C" void hy_scratch_free(hy_scratch* s) { delete s; }
+extern "C" void hy_scratch_free(hy_scratch* s) {
+    if (!s) return;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
+    delete s;
+}
 
 // ---------------------------------------------------------------------------------------------
 // Scans
@@ -1181,6 +1197,7 @@ void release_batch(hy_scratch* s) {
 
 extern "C" int hy_scratch_reserve_batch(hy_scratch* s, size_t max_queries) {
     if (!s) return HY_E_NULL_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     if (max_queries == 0 || max_queries > 65536) return HY_E_INVALID_ARGUMENT;
     if (s->bcap >= max_queries) return HY_OK;
     const hy_db* db = s->db;
@@ -1407,6 +1424,8 @@ extern "C" int hy_scan_batch(const hy_db* db, hy_scratch* s, const uint8_t* cons
             if ((rc = enqueue_batch(db, s, true)) != HY_OK) return rc;
         } else {
             if (!s->gexec) {
+                // No other thread's allocation or release may fall inside the capture.
+                std::lock_guard<std::recursive_mutex> api(hy::api_lock());
                 if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
                     return HY_E_DEVICE;
                 rc = enqueue_batch(db, s, false);
@@ -1484,6 +1503,7 @@ extern "C" int hy_scratch_set_tuning(hy_scratch* s, const hy_tuning* t) {
     if (hipSetDevice(s->device) != hipSuccess) return HY_E_DEVICE;
     if (t->batch_tile_rows != s->bq16 || t->batch_wgs_per_cu != s->bwpc ||
         t->batch_pairs != s->bpairs) {
+        std::lock_guard<std::recursive_mutex> api(hy::api_lock());
         if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
         release_batch(s);
     }

@@ -108,6 +108,7 @@ extern "C" int hy_db_save(const hy_db* db, const char* path) {
 
 extern "C" int hy_db_load(const char* path, uint32_t backend, hy_db** out, int64_t* err_detail) {
     if (!path || !out) return HY_E_NULL_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     *out = nullptr;
     if (err_detail) *err_detail = 0;
     File in;

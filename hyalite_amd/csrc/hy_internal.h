@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <thread>
 #include <vector>
 #include <stdint.h>
@@ -439,6 +440,14 @@ int download_pageable(PinnedRing& r, int device, const Segment* segs, int nseg,
 // The segments back to pageable memory: through the ring when together they hold at least
 // kPinnedRingMin bytes (complete on return), else as plain copies queued on `stream`.
 int download(PinnedRing& r, int device, const Segment* segs, int nseg, hipStream_t stream);
+
+// Process-wide lock held across a graph capture (the batched form's first submission) and around
+// the library's allocation and release work (databases, scratches and their reservations, pair
+// scratches, the traceback buffers, the pinned transfer slots), so that no allocation, release
+// or null-stream copy of one thread falls inside another thread's capture: the runtime may
+// invalidate a capture in progress for such a call, whatever the capture mode. Recursive: the
+// entry points nest (a server builds scratches, an image load builds a database).
+std::recursive_mutex& api_lock();
 
 // Launchers (hy_launch.hip). All launch on `stream` and never synchronise.
 hipError_t launch_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v,

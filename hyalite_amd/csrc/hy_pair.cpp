@@ -81,6 +81,7 @@ constexpr size_t kDevSlack = 16;
 int grow(DevBuf& b, size_t bytes) {
     bytes = std::max<size_t>(bytes, 16);
     if (b.bytes >= bytes) return HY_OK;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     hy::dev_free(b.p);
     b.p = nullptr;
     b.bytes = 0;
@@ -95,6 +96,7 @@ int grow(DevBuf& b, size_t bytes) {
 int grow_stage(hy_pair_scratch* s, size_t bytes) {
     bytes = std::max<size_t>(bytes, 64);
     if (s->stage_bytes >= bytes) return HY_OK;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     if (s->stage) (void)hipHostFree(s->stage);
     s->stage = nullptr;
     s->stage_bytes = 0;
@@ -405,6 +407,7 @@ extern "C" int hy_pair_scratch_new(hy_pair_scratch** out) {
     const int device = hy::first_gfx950_device();
     if (device < 0) return HY_E_BACKEND_UNAVAILABLE;
     if (hipSetDevice(device) != hipSuccess) return HY_E_DEVICE;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     hy_pair_scratch* s = new (std::nothrow) hy_pair_scratch();
     if (!s) return HY_E_OUT_OF_MEMORY;
     s->device = device;
@@ -420,7 +423,11 @@ extern "C" int hy_pair_scratch_new(hy_pair_scratch** out) {
     return HY_OK;
 }
 
-extern "C" void hy_pair_scratch_free(hy_pair_scratch* s) { delete s; }
+extern "C" void hy_pair_scratch_free(hy_pair_scratch* s) {
+    if (!s) return;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
+    delete s;
+}
 
 extern "C" int hy_align_pair(hy_pair_scratch* s, const uint8_t* query, size_t qlen,
                              const uint8_t* target, size_t tlen, const int32_t* matrix,

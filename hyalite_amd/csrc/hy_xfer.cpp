@@ -61,6 +61,7 @@ namespace {
 
 int ring_ready(PinnedRing& r) {
     if (r.ring) return HY_OK;
+    std::lock_guard<std::recursive_mutex> api(hy::api_lock());
     if (hipHostMalloc((void**)&r.ring, 2 * PinnedRing::kThreads * PinnedRing::kSlot) !=
         hipSuccess) {
         r.ring = nullptr;
