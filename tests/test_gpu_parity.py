@@ -641,6 +641,39 @@ def test_server_admits_requests_by_their_bytes():
         assert st.peak_bytes <= budget and st.inflight_bytes == 0
 
 
+def test_scratches_built_beside_first_batch_captures():
+    """Scratches built, reserved and freed on one thread while fresh scratches on another thread
+    capture their first batched submission as a graph: every batch equals the oracle, and no
+    scratch fails to build (the library's allocations never fall inside a capture)."""
+    db, qs, want = _serving_db()
+    stop = threading.Event()
+    errors = []
+
+    def churn():
+        while not stop.is_set():
+            try:
+                sc = hy.Scratch(db)
+                sc.reserve_batch(8)
+                del sc
+            except Exception as e:  # reported below
+                errors.append(e)
+                return
+
+    t = threading.Thread(target=churn)
+    t.start()
+    try:
+        for k in range(8):
+            sc = hy.Scratch(db)
+            part = slice(4 * k % 36, 4 * k % 36 + 4)
+            got = [_tuple(h) for h in db.scan_batch(sc, qs[part])]
+            assert got == want[part], k
+            del sc
+    finally:
+        stop.set()
+        t.join()
+    assert not errors, errors
+
+
 def test_server_contract_errors():
     db, qs, want = _serving_db()
     with hy.Server(db, max_lanes=1, max_batch=4) as srv:
