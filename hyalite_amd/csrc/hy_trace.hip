@@ -167,11 +167,11 @@ __global__ __launch_bounds__(64) void k_trace_dp(TraceArgs a) {
 // PACKED: the codes of k_trace16 (item k is slot slot0 + k of its group: lane (slot - first) / 2,
 // 16-bit half (slot - first) % 2; tile c, column j, rows 4b..4b+3 of the tile in word
 // ((c * ncols + j - 1) * Q / 4 + b) * 64 + lane). Otherwise the systolic slabs of k_trace_dp.
+// The walk of item k (one lane). k_trace_walk runs it one lane per item; k_trace16 runs it for its
+// own group's targets once their end cells are folded.
 template <int MODE, bool PACKED>
-__global__ __launch_bounds__(256) void k_trace_walk(TraceArgs a) {
+__device__ __forceinline__ void walk_item(const TraceArgs& a, uint32_t k) {
     using F = MF<MODE>;
-    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
-    if (k >= a.nitems) return;
     // Packed items are indexed by their slot (chunks may overlap in time), others by item.
     const int4 en = a.ends[PACKED ? a.slot0 + k : k];
     const uint32_t gi = (uint32_t)en.w;
@@ -270,6 +270,12 @@ __global__ __launch_bounds__(256) void k_trace_walk(TraceArgs a) {
     o.n_ops = (uint64_t)m + (uint64_t)n - w;
     o.ops_offset = obase + w;
     a.out[a.winner_key ? 0u : gi] = o;
+}
+
+template <int MODE, bool PACKED>
+__global__ __launch_bounds__(256) void k_trace_walk(TraceArgs a) {
+    const uint32_t k = blockIdx.x * 256u + threadIdx.x;
+    if (k < a.nitems) walk_item<MODE, PACKED>(a, k);
 }
 
 // Packed traceback DP (scan_all_aligned over the packed 16-bit layout): one workgroup of W
@@ -511,19 +517,30 @@ __global__ __launch_bounds__(64 * kTrace16MaxWaves) void k_trace16(TraceArgs a, 
     int* fw = fold + ((size_t)wv * 64 + lane) * 6;
     fw[0] = bsA; fw[1] = bjA; fw[2] = biA; fw[3] = bsB; fw[4] = bjB; fw[5] = biB;
     __syncthreads();
-    if (wv != 0) return;
-    for (int k = 1; k < W; ++k) {
-        const int* fk = fold + ((size_t)k * 64 + lane) * 6;
-        consider(bsA, bjA, biA, fk[0], fk[1], fk[2]);
-        consider(bsB, bjB, biB, fk[3], fk[4], fk[5]);
+    if (wv == 0) {
+        for (int k = 1; k < W; ++k) {
+            const int* fk = fold + ((size_t)k * 64 + lane) * 6;
+            consider(bsA, bjA, biA, fk[0], fk[1], fk[2]);
+            consider(bsB, bjB, biB, fk[3], fk[4], fk[5]);
+        }
+        if (liveA) {
+            a.ends[slotA] = make_int4(bsA, biA, bjA, (int)a.gidx16[slotA]);
+            a.item_group[slotA] = gix;
+        }
+        if (liveB) {
+            a.ends[slotB] = make_int4(bsB, biB, bjB, (int)a.gidx16[slotB]);
+            a.item_group[slotB] = gix;
+        }
     }
-    if (liveA) {
-        a.ends[slotA] = make_int4(bsA, biA, bjA, (int)a.gidx16[slotA]);
-        a.item_group[slotA] = gix;
-    }
-    if (liveB) {
-        a.ends[slotB] = make_int4(bsB, biB, bjB, (int)a.gidx16[slotB]);
-        a.item_group[slotB] = gix;
+    // The group's walks, while its codes are freshest in the caches: wavefront 0 walks the low
+    // halves' targets and wavefront 1 the high halves' (wavefront 0 both when it is alone); the
+    // others end here, so the CU can start another group beside the walks. The barrier makes
+    // wavefront 0's end cells visible to wavefront 1.
+    __syncthreads();
+    if (wv >= 2) return;
+    for (int h = W == 1 ? 0 : wv; h < 2; h += W == 1 ? 1 : 2) {
+        if (h == 0 ? liveA : liveB)
+            walk_item<MODE, true>(a, (h == 0 ? slotA : slotB) - a.slot0);
     }
 }
 
@@ -563,8 +580,8 @@ hipError_t launch_trace_m(const TraceArgs& a, hipStream_t st) {
 template hipError_t launch_trace_m<HY_KMODE>(const TraceArgs&, hipStream_t);
 
 // Packed traceback: the DP over groups [group_first, group_first + ngroups) of the packed layout
-// (a.rows wavefronts per workgroup), then the walk of their nitems targets (slots slot0 ..
-// slot0 + nitems - 1).
+// (a.rows wavefronts per workgroup), each workgroup then walking its group's targets (slots
+// slot0 .. slot0 + nitems - 1 over the launch).
 template <int MODE>
 hipError_t launch_trace16_m(const TraceArgs& a, hipStream_t st) {
     static_assert(MODE == HY_KMODE, "one mode per translation unit");
@@ -575,10 +592,6 @@ hipError_t launch_trace16_m(const TraceArgs& a, hipStream_t st) {
     const size_t lds = trace16_lds_bytes(a.al_eff, W);
     if (lds > 64u * 1024u) return hipErrorInvalidValue;
     hipLaunchKernelGGL((k_trace16<MODE>), dim3(a.ngroups), dim3(64 * W), lds, st, a, pp_words);
-    const hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_trace_walk<MODE, true>), dim3((a.nitems + 255) / 256), dim3(256), 0,
-                       st, a);
     return hipGetLastError();
 }
 
