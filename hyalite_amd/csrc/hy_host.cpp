@@ -781,26 +781,28 @@ hy::SysArgs sys_args(const hy_db* db, hy_scratch* s, int m) {
     return a;
 }
 
-// Tile rows of the pipelined packed kernel for a score-only scan of `ngroups` packed groups with
-// a query of m rows, or 0 for the one-wavefront-per-group kernel. One wavefront per group leaves
-// most SIMDs idle when the groups number fewer than the CUs (a 10,000-target database has 79);
-// the pipelined kernel gives each group a workgroup whose wavefronts take one query tile each.
-// It needs 2 to 8 tiles and its LDS within 64 KB. HYALITE_HIP_PIPE=0 turns it off, =1 uses it
-// whenever the query and LDS allow (for tests).
-int pipe_rows(const hy_db* db, int m, uint32_t ngroups) {
+// Tile rows of the pipelined packed kernel for a scan of `ngroups` packed groups with a query of
+// m rows (score-only, or with ends), or 0 for the one-wavefront-per-group kernel. One wavefront
+// per group leaves most SIMDs idle when the groups number fewer than the CUs (a 10,000-target
+// database has 79); the pipelined kernel gives each group a workgroup whose wavefronts take one
+// query tile each. It needs 2 to 8 tiles and its LDS within 64 KB. HYALITE_HIP_PIPE=0 turns it
+// off, =1 uses it whenever the query and LDS allow (for tests).
+int pipe_rows(const hy_db* db, int m, uint32_t ngroups, bool ends = false) {
     const char* e = std::getenv("HYALITE_HIP_PIPE");
     const int mode_env = e ? std::atoi(e) : -1;
-    int q = hy::inter16_pipe_rows(m);
-    // HYALITE_HIP_PIPE_ROWS=16|24|32 asks for that tile height where the query fits its tiles.
+    int q = ends ? hy::inter16_pipe_rows_ends((int)db->mode, m) : hy::inter16_pipe_rows(m);
+    // HYALITE_HIP_PIPE_ROWS=16|24|32 asks for that tile height where the query fits its tiles
+    // (at most 24 for local-mode ends).
     if (const char* r = std::getenv("HYALITE_HIP_PIPE_ROWS")) {
         const int want = std::atoi(r);
         if ((want == 16 || want == 24 || want == 32) && q != 0 &&
+            !(ends && db->mode == HY_MODE_SW && want > 24) &&
             (m + want - 1) / want >= 2 && (m + want - 1) / want <= hy::kPipeMaxWaves)
             q = want;
     }
     if (mode_env == 0 || q == 0 || ngroups == 0) return 0;
     const int W = (m + q - 1) / q;
-    if (hy::inter16_pipe_lds_bytes((int)db->al_eff, q, W) > 64u * 1024u) return 0;
+    if (hy::inter16_pipe_lds_bytes((int)db->al_eff, q, W, ends) > 64u * 1024u) return 0;
     if (mode_env != 1 && ngroups > hy::device_cus()) return 0;
     return q;
 }
@@ -985,9 +987,17 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
             pa.key = nullptr;
             pa.out_qend = s->d_qend;
             pa.out_tend = s->d_tend;
-            if (hy::launch_inter16_ends((int)db->mode, pa, s->stream) != hipSuccess)
-                return HY_E_DEVICE;
-            s->last_kernel = HY_KERNEL_ENDS;
+            if (const int pq = pipe_rows(db, m, pa.ngroups, true)) {
+                // Few groups: the pipelined kernel, its wavefronts' end cells folded.
+                if (hy::launch_inter16_pipe((int)db->mode, pa, pq, true, s->stream) !=
+                    hipSuccess)
+                    return HY_E_DEVICE;
+                s->last_kernel = HY_KERNEL_PIPELINED;
+            } else {
+                if (hy::launch_inter16_ends((int)db->mode, pa, s->stream) != hipSuccess)
+                    return HY_E_DEVICE;
+                s->last_kernel = HY_KERNEL_ENDS;
+            }
         } else if (const ChainShape cs = chain_shape(db, m, pa.ngroups); cs.rows != 0) {
             hy::ChainArgs ca{};
             ca.progress = s->d_chain;
@@ -1007,7 +1017,7 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
             s->chain_tickets += (uint64_t)(pa.ngroups + 7) / 8 * 8 * (uint64_t)cs.chunks;
             s->last_kernel = HY_KERNEL_CHAINED;
         } else if (const int pq = pipe_rows(db, m, pa.ngroups)) {
-            if (hy::launch_inter16_pipe((int)db->mode, pa, pq, s->stream) != hipSuccess)
+            if (hy::launch_inter16_pipe((int)db->mode, pa, pq, false, s->stream) != hipSuccess)
                 return HY_E_DEVICE;
             s->last_kernel = HY_KERNEL_PIPELINED;
         } else {

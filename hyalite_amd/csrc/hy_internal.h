@@ -547,8 +547,9 @@ hipError_t launch_pairs32_m(bool ends, const PairArgs& a, uint32_t ngroups, int 
 hipError_t launch_posmax_sw(const PosMaxArgs& a, hipStream_t st);
 template <int MODE>
 hipError_t launch_band32_m(int out, const BandArgs& a, uint32_t nblocks, int m, hipStream_t st);
-// Pipelined packed one-query kernel (scores and key, no ends): one workgroup per group, one
-// query tile per wavefront, at most kPipeMaxWaves tiles. inter16_pipe_rows gives the tile rows
+// Pipelined packed one-query kernel: one workgroup per group, one query tile per wavefront, at
+// most kPipeMaxWaves tiles; scores and the argmax key, or (ends) each target's score and end
+// cell, folded across the wavefronts with the full tie-break. inter16_pipe_rows gives the tile rows
 // for a query of m rows (0 when it needs more than kPipeMaxWaves tiles or only one): 16 rows up
 // to 64, 24 up to 192, 32 beyond. Measured on a 100-row query (10,000 DNA targets): 16-row tiles
 // (7 wavefronts) 0.205 ms, 24 (5) 0.191 ms, 32 (4) 0.228 ms per launch.
@@ -565,6 +566,13 @@ inline int inter16_pipe_rows(int m) {
     if (m <= 32 * kPipeMaxWaves) return 32;
     return 0;
 }
+// The same for an end-tracking scan: local mode keeps tiles of at most 24 rows (its end tracking
+// at 32 rows spills registers), so queries up to 24 x kPipeMaxWaves rows.
+inline int inter16_pipe_rows_ends(int mode, int m) {
+    const int q = inter16_pipe_rows(m);
+    if (mode != kSW || q <= 24) return q;
+    return m <= 24 * kPipeMaxWaves ? 24 : 0;
+}
 // One-wavefront workgroups of the packed kernel keep a residue-pair profile of their tile (the
 // two targets' scores of a query row in one word, read with no per-lane pairing) when it takes at
 // most kPairProfileSmallBytes of LDS, which holds for alphabets of up to 9 symbols (DNA, with
@@ -579,14 +587,16 @@ inline uint32_t inter16_profile_words(int al_eff, int q) {
                                           : inter16p_lds_bytes(al_eff, q)) / 4;
 }
 // LDS bytes of a pipelined workgroup of W wavefronts with q-row tiles (profiles, double buffers,
-// the fold of the running maxima); the host uses the kernel only within 64 KB.
-inline uint32_t inter16_pipe_lds_bytes(int al_eff, int q, int W) {
+// the fold of the running maxima, or of the six end words per lane); the host uses the kernel
+// only within 64 KB.
+inline uint32_t inter16_pipe_lds_bytes(int al_eff, int q, int W, bool ends = false) {
     return (uint32_t)W * inter16_profile_words(al_eff, q) * 4u + (uint32_t)(W - 1) * 4096u +
-           (uint32_t)W * 256u;
+           (uint32_t)W * 256u * (ends ? 6u : 1u);
 }
 template <int MODE>
-hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, hipStream_t st);
-hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, hipStream_t stream);
+hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, bool ends, hipStream_t st);
+hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, bool ends,
+                               hipStream_t stream);
 // Chained pipelined kernel (k_inter16_chain): the query tiles of a group spread over `chunks`
 // workgroups of W wavefronts, consecutive chunks linked through the group's boundary buffer and
 // one progress word per link. At most kChainMaxChunks chunks; tile rows 8, 16, 24 or 32. The

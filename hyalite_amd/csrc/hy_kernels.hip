@@ -336,7 +336,7 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                                              int nA, int nB, uint2* sp, const uint2* wp,
                                              uint32_t* pp, PairRun& st, Pipe pipe = Pipe()) {
     using F = MF<MODE>;
-    static_assert(!PIPE || !ENDS, "pipelined tiles: scores only");
+    static_assert(!LINKIN || !ENDS, "chained tiles: scores only");
     static_assert(!LINKIN || (PIPE && !FIRST), "a chained reader is a pipelined later tile");
     constexpr int S = QT + 4;
     // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
@@ -750,16 +750,21 @@ __device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Grou
                                                 bool lastc, int nA, int nB, uint2* sp,
                                                 const uint2* wp, uint32_t* pp, PairRun& st,
                                                 Pipe pipe = Pipe()) {
-    if (first)
+    if (first) {
         inter16_tile<MODE, QT, NW, PAIR, true, ENDS, PIPE>(a, g, wlive, query, m, i0, lastc, nA,
                                                            nB, sp, wp, pp, st, pipe);
-    else if (PIPE && pipe.gin != nullptr)
-        inter16_tile<MODE, QT, NW, PAIR, false, ENDS, PIPE, PIPE>(a, g, wlive, query, m, i0,
-                                                                  lastc, nA, nB, sp, wp, pp, st,
-                                                                  pipe);
-    else
-        inter16_tile<MODE, QT, NW, PAIR, false, ENDS, PIPE>(a, g, wlive, query, m, i0, lastc,
-                                                            nA, nB, sp, wp, pp, st, pipe);
+        return;
+    }
+    if constexpr (PIPE && !ENDS) {  // chained tiles track no ends
+        if (pipe.gin != nullptr) {
+            inter16_tile<MODE, QT, NW, PAIR, false, ENDS, PIPE, PIPE>(a, g, wlive, query, m, i0,
+                                                                      lastc, nA, nB, sp, wp, pp,
+                                                                      st, pipe);
+            return;
+        }
+    }
+    inter16_tile<MODE, QT, NW, PAIR, false, ENDS, PIPE>(a, g, wlive, query, m, i0, lastc, nA, nB,
+                                                        sp, wp, pp, st, pipe);
 }
 
 // One group of 128 targets against one query: full tiles of Q rows, then one tile of the fewest
@@ -866,9 +871,12 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16(Inte
 // its W wavefronts holding the query's W tiles of Q rows (the last the fewest of 8/16/24/Q rows
 // that cover the remainder), wavefront w one four-column block behind wavefront w - 1 and reading
 // its bottom row from LDS (Pipe). Every wavefront keeps the running maxima of its own tile; they
-// are folded through LDS and wavefront 0 writes the scores and the argmax key. LDS: W profiles
-// of pp_words words, then (W - 1) double buffers of 512 uint2, then W x 64 words for the fold.
-template <int MODE, int Q, bool PAIR>
+// are folded through LDS and wavefront 0 writes the scores and the argmax key. With ENDS every
+// wavefront instead keeps each target's best cell of its tile (the end tracking of the
+// one-wavefront kernel, started from the border's best), and wavefront 0 folds them with the
+// full tie-break and writes score and ends. LDS: W profiles of pp_words words, then (W - 1)
+// double buffers of 512 uint2, then W x 64 words for the fold (6 words per lane with ENDS).
+template <int MODE, int Q, bool PAIR, bool ENDS = false>
 __global__ __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_pipe(
     Inter16Args a, uint32_t pp_words) {
@@ -891,34 +899,70 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_pipe
     const int nA = liveA ? (int)a.lens[slotA] : 0;
     const int nB = liveB ? (int)a.lens[slotB] : 0;
     PairRun st;
+    if constexpr (ENDS) {
+        int dj, di;
+        border_best<MODE>(m, nA, go, ge, st.bsA, dj, di);
+        st.bjA = dj;
+        st.biA = di;
+        border_best<MODE>(m, nB, go, ge, st.bsB, dj, di);
+        st.bjB = dj;
+        st.biB = di;
+    }
     st.run = splat2(-32768);
 #pragma unroll
     for (int k = 0; k < 4; ++k) st.runq[k] = st.run;
     const uint2* wp = a.words + g.word_off + lane;
     const int nfull = m / Q, rem = m - nfull * Q;
     if (wv < nfull) {
-        inter16_tile_at<MODE, Q, 1, PAIR, false, true>(a, g, true, a.query, m, wv * Q + 1, wv == 0,
-                                                        rem == 0 && wv == nfull - 1, nA, nB,
-                                                        nullptr, wp, pp, st, pipe);
+        inter16_tile_at<MODE, Q, 1, PAIR, ENDS, true>(a, g, true, a.query, m, wv * Q + 1, wv == 0,
+                                                       rem == 0 && wv == nfull - 1, nA, nB,
+                                                       nullptr, wp, pp, st, pipe);
     } else {
         const int i0 = nfull * Q + 1;
         const bool first = nfull == 0;
         if (rem <= 8)
-            inter16_tile_at<MODE, 8, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
-                                                            true, nA, nB, nullptr, wp, pp, st,
-                                                            pipe);
+            inter16_tile_at<MODE, 8, 1, PAIR, ENDS, true>(a, g, true, a.query, m, i0, first,
+                                                           true, nA, nB, nullptr, wp, pp, st,
+                                                           pipe);
         else if (Q > 16 && rem <= 16)
-            inter16_tile_at<MODE, 16, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
-                                                             true, nA, nB, nullptr, wp, pp, st,
-                                                             pipe);
-        else if (Q > 24 && rem <= 24)
-            inter16_tile_at<MODE, 24, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
-                                                             true, nA, nB, nullptr, wp, pp, st,
-                                                             pipe);
-        else
-            inter16_tile_at<MODE, Q, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, 16, 1, PAIR, ENDS, true>(a, g, true, a.query, m, i0, first,
                                                             true, nA, nB, nullptr, wp, pp, st,
                                                             pipe);
+        else if (Q > 24 && rem <= 24)
+            inter16_tile_at<MODE, 24, 1, PAIR, ENDS, true>(a, g, true, a.query, m, i0, first,
+                                                            true, nA, nB, nullptr, wp, pp, st,
+                                                            pipe);
+        else
+            inter16_tile_at<MODE, Q, 1, PAIR, ENDS, true>(a, g, true, a.query, m, i0, first,
+                                                           true, nA, nB, nullptr, wp, pp, st,
+                                                           pipe);
+    }
+    if constexpr (ENDS) {
+        // Fold the wavefronts' best cells (a total order: any fold order gives the same answer)
+        // and write each target's score and 0-based inclusive ends (-1 on the border).
+        int* fw = reinterpret_cast<int*>(fold) + ((size_t)wv * 64 + lane) * 6;
+        fw[0] = st.bsA; fw[1] = st.bjA; fw[2] = st.biA;
+        fw[3] = st.bsB; fw[4] = st.bjB; fw[5] = st.biB;
+        __syncthreads();
+        if (wv != 0) return;
+        for (int k = 1; k < W; ++k) {
+            const int* fk = reinterpret_cast<const int*>(fold) + ((size_t)k * 64 + lane) * 6;
+            consider(st.bsA, st.bjA, st.biA, fk[0], fk[1], fk[2]);
+            consider(st.bsB, st.bjB, st.biB, fk[3], fk[4], fk[5]);
+        }
+        if (liveA) {
+            const uint32_t gi = a.gidx[slotA];
+            a.out_score[gi] = st.bsA;
+            a.out_qend[gi] = st.biA == 0 ? -1 : (int64_t)st.biA - 1;
+            a.out_tend[gi] = st.bjA == 0 ? -1 : (int64_t)st.bjA - 1;
+        }
+        if (liveB) {
+            const uint32_t gi = a.gidx[slotB];
+            a.out_score[gi] = st.bsB;
+            a.out_qend[gi] = st.biB == 0 ? -1 : (int64_t)st.biB - 1;
+            a.out_tend[gi] = st.bjB == 0 ? -1 : (int64_t)st.bjB - 1;
+        }
+        return;
     }
     const s2v mine =
         p_max(st.run, p_max(p_max(st.runq[0], st.runq[1]), p_max(st.runq[2], st.runq[3])));
@@ -1513,18 +1557,20 @@ hipError_t launch_inter16_qw(const Inter16Args& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int MODE, int Q>
+template <int MODE, int Q, bool ENDS>
 hipError_t launch_inter16_pipe_q(const Inter16Args& a, hipStream_t st) {
     const int W = (a.m + Q - 1) / Q;
     if (W < 2 || W > kPipeMaxWaves) return hipErrorInvalidValue;
+    if (ENDS && (!a.out_score || !a.out_qend || !a.out_tend)) return hipErrorInvalidValue;
     const uint32_t pp_words = inter16_profile_words(a.al_eff, Q);
-    const size_t lds = ((size_t)W * pp_words + (size_t)(W - 1) * 1024 + (size_t)W * 64) * 4;
+    const size_t lds = inter16_pipe_lds_bytes(a.al_eff, Q, W, ENDS);
+    if (lds > 64u * 1024u) return hipErrorInvalidValue;
     if (inter16_pair_small(a.al_eff, Q))
-        hipLaunchKernelGGL((k_inter16_pipe<MODE, Q, true>), dim3(a.ngroups), dim3(64 * W), lds,
-                           st, a, pp_words);
+        hipLaunchKernelGGL((k_inter16_pipe<MODE, Q, true, ENDS>), dim3(a.ngroups), dim3(64 * W),
+                           lds, st, a, pp_words);
     else
-        hipLaunchKernelGGL((k_inter16_pipe<MODE, Q, false>), dim3(a.ngroups), dim3(64 * W), lds,
-                           st, a, pp_words);
+        hipLaunchKernelGGL((k_inter16_pipe<MODE, Q, false, ENDS>), dim3(a.ngroups), dim3(64 * W),
+                           lds, st, a, pp_words);
     return hipGetLastError();
 }
 
@@ -1608,11 +1654,20 @@ hipError_t launch_inter16_m(const Inter16Args& a, int q, int waves, hipStream_t 
 // Tile rows of the pipelined kernel: the smallest of 16, 24, 32 whose tiles fit kPipeMaxWaves
 // wavefronts; q is that choice (inter16_pipe_rows).
 template <int MODE>
-hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, hipStream_t st) {
+hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, bool ends, hipStream_t st) {
     if (a.ngroups == 0) return hipSuccess;
-    if (q == 16) return launch_inter16_pipe_q<MODE, 16>(a, st);
-    if (q == 24) return launch_inter16_pipe_q<MODE, 24>(a, st);
-    if (q == 32) return launch_inter16_pipe_q<MODE, 32>(a, st);
+    if (ends) {
+        // Local-mode end tracking keeps tiles of at most 24 rows (inter16_pipe_rows_ends).
+        if (q == 16) return launch_inter16_pipe_q<MODE, 16, true>(a, st);
+        if (q == 24) return launch_inter16_pipe_q<MODE, 24, true>(a, st);
+        if constexpr (MODE != kSW) {
+            if (q == 32) return launch_inter16_pipe_q<MODE, 32, true>(a, st);
+        }
+        return hipErrorInvalidValue;
+    }
+    if (q == 16) return launch_inter16_pipe_q<MODE, 16, false>(a, st);
+    if (q == 24) return launch_inter16_pipe_q<MODE, 24, false>(a, st);
+    if (q == 32) return launch_inter16_pipe_q<MODE, 32, false>(a, st);
     return hipErrorInvalidValue;
 }
 
@@ -1732,7 +1787,7 @@ template hipError_t launch_inter16_batch_m<HY_KMODE>(const Inter16Args&, const B
                                                      uint32_t, hipStream_t);
 template uint32_t inter16_batch_occupancy_m<HY_KMODE>(int, int);
 template hipError_t launch_inter16_ends_m<HY_KMODE>(const Inter16Args&, hipStream_t);
-template hipError_t launch_inter16_pipe_m<HY_KMODE>(const Inter16Args&, int, hipStream_t);
+template hipError_t launch_inter16_pipe_m<HY_KMODE>(const Inter16Args&, int, bool, hipStream_t);
 template hipError_t launch_inter16_chain_m<HY_KMODE>(const Inter16Args&, const ChainArgs&, int,
                                                      int, hipStream_t);
 

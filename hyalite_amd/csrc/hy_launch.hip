@@ -352,14 +352,15 @@ uint32_t inter16_lds_bytes(int al_eff, int q) {
     return A * A * (uint32_t)(inter16_tile_rows(q) + 4) * 4u;
 }
 
-hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, hipStream_t stream) {
+hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, bool ends,
+                               hipStream_t stream) {
     if (a.ngroups == 0) return hipSuccess;
     switch (mode) {
-    case kSW: return launch_inter16_pipe_m<kSW>(a, q, stream);
-    case kNW: return launch_inter16_pipe_m<kNW>(a, q, stream);
-    case kHW: return launch_inter16_pipe_m<kHW>(a, q, stream);
-    case kOV: return launch_inter16_pipe_m<kOV>(a, q, stream);
-    case kSHW: return launch_inter16_pipe_m<kSHW>(a, q, stream);
+    case kSW: return launch_inter16_pipe_m<kSW>(a, q, ends, stream);
+    case kNW: return launch_inter16_pipe_m<kNW>(a, q, ends, stream);
+    case kHW: return launch_inter16_pipe_m<kHW>(a, q, ends, stream);
+    case kOV: return launch_inter16_pipe_m<kOV>(a, q, ends, stream);
+    case kSHW: return launch_inter16_pipe_m<kSHW>(a, q, ends, stream);
     default: return hipErrorInvalidValue;
     }
 }

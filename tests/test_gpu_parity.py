@@ -740,6 +740,38 @@ def test_pipelined_kernel_every_tile_count(mode, monkeypatch):
                 assert (h.score, h.db_index, h.query_end, h.target_end) == want, (al, m, pipe)
 
 
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_pipelined_kernel_tracks_ends(mode, monkeypatch):
+    """scan_all with ends on a database of few groups: the pipelined kernel, each wavefront
+    tracking its tile's best cells (local mode) or answer cells (the other modes) and wavefront 0
+    folding them with the full tie-break, gives every target's (score, query end, target end) as
+    the oracle and the one-wavefront end-tracking kernel do, over tile counts 2..8 at each tile
+    height the mode allows (local mode at most 24 rows), two alphabets, targets of every padding
+    class (empty ones included)."""
+    rng = np.random.default_rng(1250 + mode)
+    for al, mat, go, ge in ((4, identity_matrix(4, 2, -1), 2, 1), (20, BLOSUM62, 11, 1)):
+        seqs = [rng.integers(0, al, size=int(n), dtype=np.uint8)
+                for n in rng.integers(0, 300, 700)]
+        flat, off = orc.flatten(seqs)
+        db = build(seqs, (al, mat, go, ge), mode, hy.SearchType.ScoreEnd, 256)
+        sc = hy.Scratch(db)
+        for m in (17, 24, 40, 63, 100, 129, 177, 192, 193, 256):
+            q = rng.integers(0, al, size=m, dtype=np.uint8)
+            want = orc.scan_all_flat(flat, off, q, al, mat, go, ge, mode, ends=True, threads=8)
+            for pipe in ("1", "0"):
+                monkeypatch.setenv("HYALITE_HIP_PIPE", pipe)
+                got = db.scan_all_arrays(sc, q)
+                for g, w_ in zip(got, want):
+                    assert np.array_equal(np.asarray(g), np.asarray(w_)), (al, m, pipe)
+                kernel = sc.last_kernel()
+                if pipe == "0":
+                    assert kernel == "ends", (al, m)
+                elif m in (40, 100):  # shapes whose LDS fits for both alphabets
+                    assert kernel == "pipelined", (al, m)
+                else:
+                    assert kernel in ("pipelined", "ends"), (al, m)
+
+
 def chain_chunks(m, rows, waves):
     tiles = -(-m // rows)
     return -(-tiles // min(waves, tiles))
