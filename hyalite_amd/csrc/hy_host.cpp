@@ -109,6 +109,12 @@ namespace {
 using hy::magnitude_bound;
 using hy::width_bits_for;
 
+// HYALITE_HIP_CHAIN=1: the chained kernel whenever a valid shape exists (tests).
+bool chain_forced() {
+    const char* e = std::getenv("HYALITE_HIP_CHAIN");
+    return e && std::atoi(e) == 1;
+}
+
 // The traceback budget of a database (src/align.rs:228-268): the smallest max_bytes with which
 // the reference's traceback serves every (m, n) in [0, max_m] x [0, max_n]. A full traceback
 // holds three i32 matrices of (m+1)(n+1) cells; the checkpoint path, with strip height
@@ -707,9 +713,18 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     if (hipHostMalloc((void**)&s->h_chain_err, sizeof(unsigned int)) != hipSuccess)
         return fail(HY_E_OUT_OF_MEMORY);
     *s->h_chain_err = 0u;
+    // The end-tracking chained kernel's fold slots (10.5 KB per group), for the databases of few
+    // packed groups it serves; with more groups chain_shape never picks it.
+    uint64_t fold_entries = 0;
+    if (db->groups16.size() <= 2u * (size_t)hy::device_cus() || chain_forced()) {
+        fold_entries = hy::chain_fold_entries(db->groups16.size());
+        if (hipMalloc((void**)&s->d_chain_fold, fold_entries * sizeof(uint2)) != hipSuccess)
+            return fail(HY_E_OUT_OF_MEMORY);
+    }
     s->dev_bytes = qcap + s->spill_entries * sizeof(int2) + sys_entries * sizeof(int2) +
                    n * (sizeof(int32_t) + 2 * sizeof(int64_t)) + sizeof(unsigned long long) +
-                   sizeof(Result) + chain_words * sizeof(unsigned long long);
+                   sizeof(Result) + chain_words * sizeof(unsigned long long) +
+                   fold_entries * sizeof(uint2);
     *out = s;
     return HY_OK;
 }
@@ -807,8 +822,8 @@ int pipe_rows(const hy_db* db, int m, uint32_t ngroups, bool ends = false) {
     return q;
 }
 
-// Shape of the chained pipelined kernel for a score-only scan of `ngroups` packed groups with a
-// query of m rows, or rows = 0 for none. The pipelined kernel keeps a group on one CU, and the
+// Shape of the chained pipelined kernel for a scan of `ngroups` packed groups with a query of m
+// rows (score-only, or with ends), or rows = 0 for none. The pipelined kernel keeps a group on one CU, and the
 // one-wavefront kernel a group on one SIMD, so a database of few groups leaves most of the chip
 // idle and its longest group sets the time; chaining spreads a group's query tiles over `chunks`
 // workgroups (CUs) of `waves` wavefronts. The shape is the first of (tile rows x waves) 8x4, 8x8,
@@ -828,7 +843,7 @@ struct ChainShape {
     int rows = 0, waves = 0, tiles = 0, chunks = 0, slack = 4, every = 1;
     uint32_t spin_limit = hy::kChainSpinLimit;
 };
-ChainShape chain_shape(const hy_db* db, int m, uint32_t ngroups) {
+ChainShape chain_shape(const hy_db* db, int m, uint32_t ngroups, bool ends = false) {
     ChainShape c;
     const char* e = std::getenv("HYALITE_HIP_CHAIN");
     const int mode_env = e ? std::atoi(e) : -1;
@@ -860,20 +875,21 @@ ChainShape chain_shape(const hy_db* db, int m, uint32_t ngroups) {
                                      {24, 4}, {24, 8}, {32, 4}, {32, 8}};
     const uint64_t g8 = (uint64_t)(ngroups + 7) / 8 * 8;
     const uint64_t budget = 3200;  // wavefronts
-    const bool no_pipe = hy::inter16_pipe_rows(m) == 0;
+    const bool no_pipe =
+        (ends ? hy::inter16_pipe_rows_ends((int)db->mode, m) : hy::inter16_pipe_rows(m)) == 0;
     struct Cand {
         int rows = 0, waves = 0, tiles = 0, chunks = 0;
         uint64_t total = 0;  // wavefronts of the launch
     };
-    // A shape is valid when it gives 2-8 chunks and its workgroup's LDS fits 64 KB.
+    // A shape is valid when it gives 2-8 chunks and its workgroup's LDS fits 64 KB (and, with
+    // ends, its tile height is one the end-tracking chained kernel has).
     auto make = [&](int rows, int waves, Cand& out) {
+        if (!hy::inter16_chain_rows_ok((int)db->mode, rows, ends)) return false;
         const int tiles = (m + rows - 1) / rows;
         waves = std::min(waves, tiles);
         const int chunks = (tiles + waves - 1) / waves;
         if (chunks < 2 || chunks > hy::kChainMaxChunks) return false;
-        const uint32_t pp_words = hy::inter16_profile_words((int)db->al_eff, rows);
-        if (((uint64_t)waves * pp_words + (uint64_t)(waves - 1) * 1024 + (uint64_t)waves * 64) *
-                4 > 64u * 1024u)
+        if (hy::inter16_chain_lds_bytes((int)db->al_eff, rows, waves, ends) > 64u * 1024u)
             return false;
         out.rows = rows;
         out.waves = waves;
@@ -903,6 +919,24 @@ ChainShape chain_shape(const hy_db* db, int m, uint32_t ngroups) {
     c.tiles = pick.tiles;
     c.chunks = pick.chunks;
     return c;
+}
+
+// Arguments of one chained launch of shape cs (a new epoch; the tickets it takes are counted
+// after the launch).
+hy::ChainArgs chain_args(const hy_db* db, hy_scratch* s, const ChainShape& cs) {
+    hy::ChainArgs ca{};
+    ca.progress = s->d_chain;
+    ca.epoch = (unsigned long long)(++s->chain_epoch) << 32;
+    ca.err = s->h_chain_err;
+    ca.ticket = s->d_chain + std::max<size_t>(db->groups16.size(), 1) * hy::kChainMaxLinks;
+    ca.ticket_base = s->chain_tickets;
+    ca.tiles = cs.tiles;
+    ca.chunks = cs.chunks;
+    ca.slack = cs.slack;
+    ca.every = cs.every;
+    ca.spin_limit = cs.spin_limit;
+    ca.fold = s->d_chain_fold;
+    return ca;
 }
 
 // Enqueue the per-target pass. ends: write (score, qend, tend) per target; otherwise write scores
@@ -987,7 +1021,17 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
             pa.key = nullptr;
             pa.out_qend = s->d_qend;
             pa.out_tend = s->d_tend;
-            if (const int pq = pipe_rows(db, m, pa.ngroups, true)) {
+            const ChainShape cs =
+                s->d_chain_fold ? chain_shape(db, m, pa.ngroups, true) : ChainShape();
+            if (cs.rows != 0) {
+                // Few groups: the chained kernel, each chunk's end cells passed on to the next.
+                hy::ChainArgs ca = chain_args(db, s, cs);
+                if (hy::launch_inter16_chain((int)db->mode, pa, ca, cs.rows, cs.waves, true,
+                                             s->stream) != hipSuccess)
+                    return HY_E_DEVICE;
+                s->chain_tickets += (uint64_t)(pa.ngroups + 7) / 8 * 8 * (uint64_t)cs.chunks;
+                s->last_kernel = HY_KERNEL_CHAINED;
+            } else if (const int pq = pipe_rows(db, m, pa.ngroups, true)) {
                 // Few groups: the pipelined kernel, its wavefronts' end cells folded.
                 if (hy::launch_inter16_pipe((int)db->mode, pa, pq, true, s->stream) !=
                     hipSuccess)
@@ -999,20 +1043,9 @@ int enqueue_targets(const hy_db* db, hy_scratch* s, int m, bool ends, bool write
                 s->last_kernel = HY_KERNEL_ENDS;
             }
         } else if (const ChainShape cs = chain_shape(db, m, pa.ngroups); cs.rows != 0) {
-            hy::ChainArgs ca{};
-            ca.progress = s->d_chain;
-            ca.epoch = (unsigned long long)(++s->chain_epoch) << 32;
-            ca.err = s->h_chain_err;
-            ca.ticket =
-                s->d_chain + std::max<size_t>(db->groups16.size(), 1) * hy::kChainMaxLinks;
-            ca.ticket_base = s->chain_tickets;
-            ca.tiles = cs.tiles;
-            ca.chunks = cs.chunks;
-            ca.slack = cs.slack;
-            ca.every = cs.every;
-            ca.spin_limit = cs.spin_limit;
-            if (hy::launch_inter16_chain((int)db->mode, pa, ca, cs.rows, cs.waves, s->stream) !=
-                hipSuccess)
+            hy::ChainArgs ca = chain_args(db, s, cs);
+            if (hy::launch_inter16_chain((int)db->mode, pa, ca, cs.rows, cs.waves, false,
+                                         s->stream) != hipSuccess)
                 return HY_E_DEVICE;
             s->chain_tickets += (uint64_t)(pa.ngroups + 7) / 8 * 8 * (uint64_t)cs.chunks;
             s->last_kernel = HY_KERNEL_CHAINED;

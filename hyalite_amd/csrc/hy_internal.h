@@ -600,7 +600,9 @@ hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, bool ends,
 // Chained pipelined kernel (k_inter16_chain): the query tiles of a group spread over `chunks`
 // workgroups of W wavefronts, consecutive chunks linked through the group's boundary buffer and
 // one progress word per link. At most kChainMaxChunks chunks; tile rows 8, 16, 24 or 32. The
-// launch takes (ngroups rounded up to 8) x chunks workgroup tickets.
+// launch takes (ngroups rounded up to 8) x chunks workgroup tickets. With ends every chunk folds
+// its wavefronts' best cells, joins those of the chunks before it and passes them on through
+// `fold`; the last chunk writes each target's score and ends.
 constexpr int kChainMaxChunks = 8;
 constexpr int kChainMaxLinks = kChainMaxChunks - 1;
 struct ChainArgs {
@@ -614,16 +616,30 @@ struct ChainArgs {
     int slack;                     // blocks a reader lets its writer publish before it starts
     int every;                     // a writer publishes its progress every `every` blocks
     uint32_t spin_limit;           // polls before a wait gives up (kChainSpinLimit)
+    uint2* fold;  // ends: [(group * kChainMaxLinks + link) * 3 + k][64] best cells passed on
 };
+// uint2 entries of the ends fold buffer for `groups` packed groups.
+inline uint64_t chain_fold_entries(uint64_t groups) {
+    return (groups > 0 ? groups : 1) * (uint64_t)kChainMaxLinks * 3u * 64u;
+}
+// LDS bytes of a chained workgroup of W wavefronts with tiles of q rows.
+inline uint32_t inter16_chain_lds_bytes(int al_eff, int q, int W, bool ends) {
+    return inter16_pipe_lds_bytes(al_eff, q, W, ends);
+}
+// Local-mode end tracking keeps 16-row tiles or shorter in the chained kernel (the reader's
+// prefetched rows on top of a 24- or 32-row tile's end tracking would spill).
+inline bool inter16_chain_rows_ok(int mode, int q, bool ends) {
+    return (q == 8 || q == 16 || q == 24 || q == 32) && !(ends && mode == kSW && q > 16);
+}
 // Polls of a chained wait (0.5-1 us each: an L2 round trip and a short sleep) before it gives up:
 // seconds, far beyond any launch's wait, so that a reader always ends even if its writer never
 // published. Tests set fewer (HYALITE_HIP_CHAIN_SPIN_LIMIT) to take the give-up path.
 constexpr uint32_t kChainSpinLimit = 1u << 22;
 template <int MODE>
 hipError_t launch_inter16_chain_m(const Inter16Args& a, const ChainArgs& c, int q, int waves,
-                                  hipStream_t st);
+                                  bool ends, hipStream_t st);
 hipError_t launch_inter16_chain(int mode, const Inter16Args& a, const ChainArgs& c, int q,
-                                int waves, hipStream_t stream);
+                                int waves, bool ends, hipStream_t stream);
 // The packed one-query kernel with end tracking: score and ends of every target of the launch's
 // groups into out_score / out_qend / out_tend.
 template <int MODE>

@@ -336,7 +336,6 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
                                              int nA, int nB, uint2* sp, const uint2* wp,
                                              uint32_t* pp, PairRun& st, Pipe pipe = Pipe()) {
     using F = MF<MODE>;
-    static_assert(!LINKIN || !ENDS, "chained tiles: scores only");
     static_assert(!LINKIN || (PIPE && !FIRST), "a chained reader is a pipelined later tile");
     constexpr int S = QT + 4;
     // Single profile: row stride in 32-bit words, an odd multiple of 4 so that the rows of
@@ -743,8 +742,9 @@ __device__ __forceinline__ void inter16_tile(const Inter16Args& a, const Group& 
 
 // The tile above with FIRST chosen at run time (the first tile of a group starts from the DP
 // border instead of a boundary row), and the chained reader role for a pipelined tile whose
-// Pipe has a link in.
-template <int MODE, int QT, int NW, bool PAIR, bool ENDS = false, bool PIPE = false>
+// Pipe has a link in (compiled in for score-only pipelined tiles and for the chained kernel).
+template <int MODE, int QT, int NW, bool PAIR, bool ENDS = false, bool PIPE = false,
+          bool CHAIN = false>
 __device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Group& g, bool wlive,
                                                 const uint8_t* query, int m, int i0, bool first,
                                                 bool lastc, int nA, int nB, uint2* sp,
@@ -755,7 +755,7 @@ __device__ __forceinline__ void inter16_tile_at(const Inter16Args& a, const Grou
                                                            nB, sp, wp, pp, st, pipe);
         return;
     }
-    if constexpr (PIPE && !ENDS) {  // chained tiles track no ends
+    if constexpr (PIPE && (!ENDS || CHAIN)) {
         if (pipe.gin != nullptr) {
             inter16_tile<MODE, QT, NW, PAIR, false, ENDS, PIPE, PIPE>(a, g, wlive, query, m, i0,
                                                                       lastc, nA, nB, sp, wp, pp,
@@ -1003,8 +1003,13 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_pipe
 // for chunk k - 1 to publish nblk + 1 blocks (its maxima, already folded with the chunks before
 // it, in column 0 of the buffer), folds its own, and passes them on; the last chunk writes the
 // scores and the argmax key. Wavefronts of a short last chunk without a tile take the
-// workgroup barriers only.
-template <int MODE, int Q, bool PAIR>
+// workgroup barriers only. ENDS: every wavefront keeps each target's best cell of its tile (from
+// the border's best, as in k_inter16_pipe); wavefront 0 folds the workgroup's through LDS, joins
+// those of the chunks before it (c.fold, slot of link k - 1, read once link k - 1 shows nblk + 1
+// blocks) and passes the result on (slot of link k); the last chunk writes each target's score
+// and 0-based ends. The fold is a total order, so the chunks' order of folding does not change
+// the answer.
+template <int MODE, int Q, bool PAIR, bool ENDS = false>
 __global__ __launch_bounds__(512)
 __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_chain(
     Inter16Args a, ChainArgs c, uint32_t pp_words) {
@@ -1060,6 +1065,15 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_chai
     const int nA = liveA ? (int)a.lens[slotA] : 0;
     const int nB = liveB ? (int)a.lens[slotB] : 0;
     PairRun st;
+    if constexpr (ENDS) {
+        int dj, di;
+        border_best<MODE>(m, nA, go, ge, st.bsA, dj, di);
+        st.bjA = dj;
+        st.biA = di;
+        border_best<MODE>(m, nB, go, ge, st.bsB, dj, di);
+        st.bjB = dj;
+        st.biB = di;
+    }
     st.run = splat2(-32768);
 #pragma unroll
     for (int q = 0; q < 4; ++q) st.runq[q] = st.run;
@@ -1070,28 +1084,69 @@ __attribute__((amdgpu_waves_per_eu(HY_PACKED_WAVES_PER_EU))) void k_inter16_chai
         // No tile: the profile staging's two barriers and one per block step.
         for (int s = 0; s < 2 + pipe.nsteps; ++s) __syncthreads();
     } else if (t < nfull) {
-        inter16_tile_at<MODE, Q, 1, PAIR, false, true>(a, g, true, a.query, m, t * Q + 1, t == 0,
+        inter16_tile_at<MODE, Q, 1, PAIR, ENDS, true, true>(a, g, true, a.query, m, t * Q + 1, t == 0,
                                                         rem == 0 && t == nfull - 1, nA, nB,
                                                         nullptr, wp, pp, st, pipe);
     } else {
         const int i0 = nfull * Q + 1;
         const bool first = nfull == 0;
         if (rem <= 8)
-            inter16_tile_at<MODE, 8, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, 8, 1, PAIR, ENDS, true, true>(a, g, true, a.query, m, i0, first,
                                                             true, nA, nB, nullptr, wp, pp, st,
                                                             pipe);
         else if (Q > 16 && rem <= 16)
-            inter16_tile_at<MODE, 16, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, 16, 1, PAIR, ENDS, true, true>(a, g, true, a.query, m, i0, first,
                                                              true, nA, nB, nullptr, wp, pp, st,
                                                              pipe);
         else if (Q > 24 && rem <= 24)
-            inter16_tile_at<MODE, 24, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, 24, 1, PAIR, ENDS, true, true>(a, g, true, a.query, m, i0, first,
                                                              true, nA, nB, nullptr, wp, pp, st,
                                                              pipe);
         else
-            inter16_tile_at<MODE, Q, 1, PAIR, false, true>(a, g, true, a.query, m, i0, first,
+            inter16_tile_at<MODE, Q, 1, PAIR, ENDS, true, true>(a, g, true, a.query, m, i0, first,
                                                             true, nA, nB, nullptr, wp, pp, st,
                                                             pipe);
+    }
+    if constexpr (ENDS) {
+        int* fw = reinterpret_cast<int*>(fold) + ((size_t)wv * 64 + lane) * 6;
+        fw[0] = st.bsA; fw[1] = st.bjA; fw[2] = st.biA;
+        fw[3] = st.bsB; fw[4] = st.bjB; fw[5] = st.biB;
+        __syncthreads();
+        if (wv != 0) return;
+        for (int q = 1; q < W; ++q) {
+            const int* fq = reinterpret_cast<const int*>(fold) + ((size_t)q * 64 + lane) * 6;
+            consider(st.bsA, st.bjA, st.biA, fq[0], fq[1], fq[2]);
+            consider(st.bsB, st.bjB, st.biB, fq[3], fq[4], fq[5]);
+        }
+        uint2* slots = c.fold + (size_t)gi * kChainMaxLinks * 192 + lane;
+        if (k > 0) {
+            (void)chain_wait(pipe, prog + (k - 1), (uint32_t)nblk + 1);
+            const uint2* fin = slots + (size_t)(k - 1) * 192;
+            const uint2 x = chain_load(fin), y = chain_load(fin + 64), z = chain_load(fin + 128);
+            consider(st.bsA, st.bjA, st.biA, (int)x.x, (int)x.y, (int)y.x);
+            consider(st.bsB, st.bjB, st.biB, (int)y.y, (int)z.x, (int)z.y);
+        }
+        if (k + 1 < c.chunks) {
+            uint2* fout = slots + (size_t)k * 192;
+            chain_store(fout, make_uint2((uint32_t)st.bsA, (uint32_t)st.bjA));
+            chain_store(fout + 64, make_uint2((uint32_t)st.biA, (uint32_t)st.bsB));
+            chain_store(fout + 128, make_uint2((uint32_t)st.bjB, (uint32_t)st.biB));
+            chain_publish(pipe, prog + k, (uint32_t)nblk + 1);
+            return;
+        }
+        if (liveA) {
+            const uint32_t gx = a.gidx[slotA];
+            a.out_score[gx] = st.bsA;
+            a.out_qend[gx] = st.biA == 0 ? -1 : (int64_t)st.biA - 1;
+            a.out_tend[gx] = st.bjA == 0 ? -1 : (int64_t)st.bjA - 1;
+        }
+        if (liveB) {
+            const uint32_t gx = a.gidx[slotB];
+            a.out_score[gx] = st.bsB;
+            a.out_qend[gx] = st.biB == 0 ? -1 : (int64_t)st.biB - 1;
+            a.out_tend[gx] = st.bjB == 0 ? -1 : (int64_t)st.bjB - 1;
+        }
+        return;
     }
     const s2v mine =
         p_max(st.run, p_max(p_max(st.runq[0], st.runq[1]), p_max(st.runq[2], st.runq[3])));
@@ -1574,19 +1629,21 @@ hipError_t launch_inter16_pipe_q(const Inter16Args& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-template <int MODE, int Q>
+template <int MODE, int Q, bool ENDS>
 hipError_t launch_inter16_chain_q(const Inter16Args& a, const ChainArgs& c, int W,
                                   hipStream_t st) {
+    if (ENDS && (!a.out_score || !a.out_qend || !a.out_tend || !c.fold))
+        return hipErrorInvalidValue;
     const uint32_t pp_words = inter16_profile_words(a.al_eff, Q);
-    const size_t lds = ((size_t)W * pp_words + (size_t)(W - 1) * 1024 + (size_t)W * 64) * 4;
+    const size_t lds = inter16_chain_lds_bytes(a.al_eff, Q, W, ENDS);
     if (lds > 64u * 1024u) return hipErrorInvalidValue;
     const uint32_t blocks = (a.ngroups + 7u) / 8u * 8u * (uint32_t)c.chunks;
     if (inter16_pair_small(a.al_eff, Q))
-        hipLaunchKernelGGL((k_inter16_chain<MODE, Q, true>), dim3(blocks), dim3(64 * W), lds, st,
-                           a, c, pp_words);
+        hipLaunchKernelGGL((k_inter16_chain<MODE, Q, true, ENDS>), dim3(blocks), dim3(64 * W),
+                           lds, st, a, c, pp_words);
     else
-        hipLaunchKernelGGL((k_inter16_chain<MODE, Q, false>), dim3(blocks), dim3(64 * W), lds,
-                           st, a, c, pp_words);
+        hipLaunchKernelGGL((k_inter16_chain<MODE, Q, false, ENDS>), dim3(blocks), dim3(64 * W),
+                           lds, st, a, c, pp_words);
     return hipGetLastError();
 }
 
@@ -1676,16 +1733,26 @@ hipError_t launch_inter16_pipe_m(const Inter16Args& a, int q, bool ends, hipStre
 // chunk holds at least one tile and its last wavefront holding one is the chunk's writer.
 template <int MODE>
 hipError_t launch_inter16_chain_m(const Inter16Args& a, const ChainArgs& c, int q, int waves,
-                                  hipStream_t st) {
+                                  bool ends, hipStream_t st) {
     if (a.ngroups == 0) return hipSuccess;
     if (q <= 0 || waves < 1 || waves > 8 || c.tiles != (a.m + q - 1) / q ||
         c.chunks != (c.tiles + waves - 1) / waves || c.chunks < 1 ||
-        c.chunks > kChainMaxChunks || c.every < 1 || !c.progress || !c.err || !c.ticket)
+        c.chunks > kChainMaxChunks || c.every < 1 || !c.progress || !c.err || !c.ticket ||
+        !inter16_chain_rows_ok(MODE, q, ends))
         return hipErrorInvalidValue;
-    if (q == 8) return launch_inter16_chain_q<MODE, 8>(a, c, waves, st);
-    if (q == 16) return launch_inter16_chain_q<MODE, 16>(a, c, waves, st);
-    if (q == 24) return launch_inter16_chain_q<MODE, 24>(a, c, waves, st);
-    if (q == 32) return launch_inter16_chain_q<MODE, 32>(a, c, waves, st);
+    if (ends) {
+        if (q == 8) return launch_inter16_chain_q<MODE, 8, true>(a, c, waves, st);
+        if (q == 16) return launch_inter16_chain_q<MODE, 16, true>(a, c, waves, st);
+        if constexpr (MODE != kSW) {
+            if (q == 24) return launch_inter16_chain_q<MODE, 24, true>(a, c, waves, st);
+            if (q == 32) return launch_inter16_chain_q<MODE, 32, true>(a, c, waves, st);
+        }
+        return hipErrorInvalidValue;
+    }
+    if (q == 8) return launch_inter16_chain_q<MODE, 8, false>(a, c, waves, st);
+    if (q == 16) return launch_inter16_chain_q<MODE, 16, false>(a, c, waves, st);
+    if (q == 24) return launch_inter16_chain_q<MODE, 24, false>(a, c, waves, st);
+    if (q == 32) return launch_inter16_chain_q<MODE, 32, false>(a, c, waves, st);
     return hipErrorInvalidValue;
 }
 
@@ -1789,6 +1856,6 @@ template uint32_t inter16_batch_occupancy_m<HY_KMODE>(int, int);
 template hipError_t launch_inter16_ends_m<HY_KMODE>(const Inter16Args&, hipStream_t);
 template hipError_t launch_inter16_pipe_m<HY_KMODE>(const Inter16Args&, int, bool, hipStream_t);
 template hipError_t launch_inter16_chain_m<HY_KMODE>(const Inter16Args&, const ChainArgs&, int,
-                                                     int, hipStream_t);
+                                                     int, bool, hipStream_t);
 
 }  // namespace hy

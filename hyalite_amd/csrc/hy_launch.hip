@@ -366,14 +366,14 @@ hipError_t launch_inter16_pipe(int mode, const Inter16Args& a, int q, bool ends,
 }
 
 hipError_t launch_inter16_chain(int mode, const Inter16Args& a, const ChainArgs& c, int q,
-                                int waves, hipStream_t stream) {
+                                int waves, bool ends, hipStream_t stream) {
     if (a.ngroups == 0) return hipSuccess;
     switch (mode) {
-    case kSW: return launch_inter16_chain_m<kSW>(a, c, q, waves, stream);
-    case kNW: return launch_inter16_chain_m<kNW>(a, c, q, waves, stream);
-    case kHW: return launch_inter16_chain_m<kHW>(a, c, q, waves, stream);
-    case kOV: return launch_inter16_chain_m<kOV>(a, c, q, waves, stream);
-    case kSHW: return launch_inter16_chain_m<kSHW>(a, c, q, waves, stream);
+    case kSW: return launch_inter16_chain_m<kSW>(a, c, q, waves, ends, stream);
+    case kNW: return launch_inter16_chain_m<kNW>(a, c, q, waves, ends, stream);
+    case kHW: return launch_inter16_chain_m<kHW>(a, c, q, waves, ends, stream);
+    case kOV: return launch_inter16_chain_m<kOV>(a, c, q, waves, ends, stream);
+    case kSHW: return launch_inter16_chain_m<kSHW>(a, c, q, waves, ends, stream);
     default: return hipErrorInvalidValue;
     }
 }

@@ -119,6 +119,7 @@ struct hy_scratch {
     // workgroup ticket counter, zeroed at allocation; the launch epoch; the tickets earlier
     // launches took; and a pinned flag the kernel raises if a wait gave up.
     unsigned long long* d_chain = nullptr;  // progress words, then the ticket counter
+    uint2* d_chain_fold = nullptr;  // end-tracking chained scans: best cells passed between chunks
     unsigned int* h_chain_err = nullptr;
     uint32_t chain_epoch = 0;
     uint64_t chain_tickets = 0;             // tickets taken by earlier launches
@@ -186,7 +187,7 @@ struct hy_scratch {
         hy::dev_free(d_bq); hy::dev_free(d_bmeta); hy::dev_free(d_bkeys); hy::dev_free(d_bres);
         hy::dev_free(d_slots); hy::dev_free(d_bwin_spill); hy::dev_free(d_btail_spill);
         hy::dev_free(d_codes); hy::dev_free(d_tends); hy::dev_free(d_tout); hy::dev_free(d_ops);
-        hy::dev_free(d_slab_off); hy::dev_free(d_titem); hy::dev_free(d_gslab); hy::dev_free(d_chain);
+        hy::dev_free(d_slab_off); hy::dev_free(d_titem); hy::dev_free(d_gslab); hy::dev_free(d_chain); hy::dev_free(d_chain_fold);
         hy::dev_free(d_opsum);
         if (h_chain_err) (void)hipHostFree(h_chain_err);
         if (h_tout) (void)hipHostFree(h_tout);

@@ -329,7 +329,8 @@ int hy_scratch_batch_profile_read(hy_scratch* s, double* kernel_ms, uint64_t* la
 /* Which packed 16-bit kernel the scratch's last one-query scan ran over the database's short packed
  * groups: HY_KERNEL_NONE (none: no such groups, or no scan yet), HY_KERNEL_PACKED (one group per
  * wavefront), HY_KERNEL_PIPELINED (one group per workgroup, a query tile per wavefront; score-only
- * or end-tracking), HY_KERNEL_CHAINED (a group's query tiles over several workgroups) or
+ * or end-tracking), HY_KERNEL_CHAINED (a group's query tiles over several workgroups; score-only
+ * or end-tracking) or
  * HY_KERNEL_ENDS (end tracking, one group per wavefront). Instrumentation for tests and
  * benchmarks; no counterpart in the reference. */
 enum {

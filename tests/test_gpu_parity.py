@@ -747,7 +747,9 @@ def test_pipelined_kernel_tracks_ends(mode, monkeypatch):
     folding them with the full tie-break, gives every target's (score, query end, target end) as
     the oracle and the one-wavefront end-tracking kernel do, over tile counts 2..8 at each tile
     height the mode allows (local mode at most 24 rows), two alphabets, targets of every padding
-    class (empty ones included)."""
+    class (empty ones included). The chained kernel, which would take these databases first, is
+    turned off."""
+    monkeypatch.setenv("HYALITE_HIP_CHAIN", "0")
     rng = np.random.default_rng(1250 + mode)
     for al, mat, go, ge in ((4, identity_matrix(4, 2, -1), 2, 1), (20, BLOSUM62, 11, 1)):
         seqs = [rng.integers(0, al, size=int(n), dtype=np.uint8)
@@ -834,6 +836,51 @@ def test_chained_kernel_every_shape(mode, monkeypatch):
         assert sc.last_kernel() == "chained", al
         monkeypatch.setenv("HYALITE_HIP_CHAIN", "1")
     assert ran > 100
+
+
+@pytest.mark.parametrize("mode", ALL_MODES)
+def test_chained_kernel_tracks_ends(mode, monkeypatch):
+    """The chained kernel with end tracking: each chunk folds its wavefronts' best cells, joins
+    those the chunk before it passed on and passes the result to the next; the last chunk writes
+    every target's (score, query end, target end). scan_all with ends against the oracle at every
+    tile height the mode allows (local mode at most 16 rows) and 1, 2, 4 and 8 wavefronts per
+    workgroup, two alphabets, targets of every padding class, and the default rule's choice."""
+    rng = np.random.default_rng(1800 + mode)
+    rows_ok = (8, 16) if mode == MODE_SW else (8, 16, 24, 32)
+    ran = 0
+    for al, mat, go, ge in ((4, identity_matrix(4, 2, -1), 2, 1), (20, BLOSUM62, 11, 1)):
+        seqs = [rng.integers(0, al, size=int(n), dtype=np.uint8)
+                for n in rng.integers(0, 300, 2900)]
+        flat, off = orc.flatten(seqs)
+        db = build(seqs, (al, mat, go, ge), mode, hy.SearchType.ScoreEnd, 256)
+        sc = hy.Scratch(db)
+        for m in (17, 40, 100, 129, 256):
+            q = rng.integers(0, al, size=m, dtype=np.uint8)
+            want_all = orc.scan_all_flat(flat, off, q, al, mat, go, ge, mode, ends=True,
+                                         threads=8)
+            monkeypatch.setenv("HYALITE_HIP_CHAIN", "1")
+            for rows in rows_ok:
+                for waves in (1, 2, 4, 8):
+                    if not 2 <= chain_chunks(m, rows, waves) <= 8:
+                        continue
+                    monkeypatch.setenv("HYALITE_HIP_CHAIN_ROWS", str(rows))
+                    monkeypatch.setenv("HYALITE_HIP_CHAIN_WAVES", str(waves))
+                    case = (al, m, rows, waves)
+                    got = db.scan_all_arrays(sc, q)
+                    for g, w_ in zip(got, want_all):
+                        assert np.array_equal(np.asarray(g), np.asarray(w_)), case
+                    assert sc.last_kernel() == "chained", case
+                    ran += 1
+            for var in ("HYALITE_HIP_CHAIN", "HYALITE_HIP_CHAIN_ROWS",
+                        "HYALITE_HIP_CHAIN_WAVES"):
+                monkeypatch.delenv(var, raising=False)
+            # The default rule: the chained kernel once 8-row tiles make two chunks of four.
+            got = db.scan_all_arrays(sc, q)
+            for g, w_ in zip(got, want_all):
+                assert np.array_equal(np.asarray(g), np.asarray(w_)), (al, m)
+            if m > 32:
+                assert sc.last_kernel() == "chained", (al, m)
+    assert ran > 40
 
 
 def test_chained_wait_that_gives_up_is_reported(monkeypatch):
