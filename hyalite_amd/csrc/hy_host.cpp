@@ -698,7 +698,6 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
         hipMalloc((void**)&s->d_tend, n * sizeof(int64_t)) != hipSuccess)
         return fail(HY_E_OUT_OF_MEMORY);
     if (hipMalloc((void**)&s->d_key, sizeof(unsigned long long)) != hipSuccess ||
-        hipMalloc((void**)&s->d_result, sizeof(Result)) != hipSuccess ||
         hipHostMalloc((void**)&s->h_result, sizeof(Result)) != hipSuccess)
         return fail(HY_E_OUT_OF_MEMORY);
     // Progress words and the ticket counter of the chained pipelined kernel (databases with few
@@ -723,7 +722,7 @@ extern "C" int hy_scratch_new(const hy_db* db, hy_scratch** out) {
     }
     s->dev_bytes = qcap + s->spill_entries * sizeof(int2) + sys_entries * sizeof(int2) +
                    n * (sizeof(int32_t) + 2 * sizeof(int64_t)) + sizeof(unsigned long long) +
-                   sizeof(Result) + chain_words * sizeof(unsigned long long) +
+                   chain_words * sizeof(unsigned long long) +
                    fold_entries * sizeof(uint2);
     *out = s;
     return HY_OK;
@@ -1112,6 +1111,10 @@ extern "C" int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, siz
     const int m = (int)qlen;
     if (hy::launch_fill_u64(s->d_key, 1, 0ull, s->stream) != hipSuccess) return HY_E_DEVICE;
     if ((rc = enqueue_targets(db, s, m, false, false, s->d_key)) != HY_OK) return rc;
+    // The result record goes straight to the pinned host record (device-accessible), with no
+    // copy launch after it; marked unchecked until the last kernel writes it.
+    Result* res = s->h_result;
+    res->check = 1;
     const bool ends = db->search_type != HY_SEARCH_SCORE;
     if (ends) {
         // The winner's end positions: one pass over the winning target only (the role of
@@ -1135,26 +1138,23 @@ extern "C" int hy_scan(const hy_db* db, hy_scratch* s, const uint8_t* query, siz
             ba.spill = s->d_sys_spill;
             ba.spill_stride = 0;
             ba.winner_key = s->d_key;
-            ba.result = s->d_result;
+            ba.result = res;
             if (hy::launch_band32((int)db->mode, hy::kBandEnds, ba, 1, m, s->stream) != hipSuccess)
                 return HY_E_DEVICE;
         } else {
             hy::SysArgs sa = sys_args(db, s, m);
             sa.winner_key = s->d_key;
-            sa.result = s->d_result;
+            sa.result = res;
             if (hy::launch_sys32((int)db->mode, true, true, sa, 1, s->stream) != hipSuccess)
                 return HY_E_DEVICE;
         }
     } else {
-        if (hy::launch_key_result(s->d_key, s->d_result, s->stream) != hipSuccess)
+        if (hy::launch_key_result(s->d_key, res, s->stream) != hipSuccess)
             return HY_E_DEVICE;
     }
-    if (hipMemcpyAsync(s->h_result, s->d_result, sizeof(Result), hipMemcpyDeviceToHost,
-                       s->stream) != hipSuccess)
-        return HY_E_DEVICE;
     if (hipStreamSynchronize(s->stream) != hipSuccess) return HY_E_DEVICE;
     if ((rc = scan_done(s)) != HY_OK) return rc;
-    const Result r = *s->h_result;
+    const Result r = *res;
     if (r.check != 0) return HY_E_INTERNAL;
     *score = r.score;
     *db_index = r.db_index;

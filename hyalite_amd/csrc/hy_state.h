@@ -103,8 +103,7 @@ struct hy_scratch {
     int64_t* d_qend = nullptr;
     int64_t* d_tend = nullptr;
     unsigned long long* d_key = nullptr;
-    hy::Result* d_result = nullptr;
-    hy::Result* h_result = nullptr;          // pinned
+    hy::Result* h_result = nullptr;  // pinned; the one-query scan's last kernel writes it
     // Instrumentation: events around the per-target kernels of the last scan.
     bool profile = false;
     bool pending = false;
@@ -183,7 +182,7 @@ struct hy_scratch {
         if (stream) (void)hipStreamSynchronize(stream);
         hy::dev_free(d_query); hy::dev_free(d_spill); hy::dev_free(d_sys_spill);
         hy::dev_free(d_scores); hy::dev_free(d_qend); hy::dev_free(d_tend);
-        hy::dev_free(d_key); hy::dev_free(d_result);
+        hy::dev_free(d_key);
         hy::dev_free(d_bq); hy::dev_free(d_bmeta); hy::dev_free(d_bkeys); hy::dev_free(d_bres);
         hy::dev_free(d_slots); hy::dev_free(d_bwin_spill); hy::dev_free(d_btail_spill);
         hy::dev_free(d_codes); hy::dev_free(d_tends); hy::dev_free(d_tout); hy::dev_free(d_ops);
