@@ -976,15 +976,11 @@ def main() -> int:
         }
     del sc, db
 
-    # The small, latency-bound configurations run before the legs that move gigabytes through
-    # host memory: right after the traceback leg (about 1 GB of op buffers allocated, copied back
-    # and freed), the 200 KB of pageable copies in each 10,000-target scan_all took 0.51 ms
-    # instead of 0.20 ms for the next few hundred calls.
-    others = None if args.no_configs else other_configs(sync)
-    ref_shapes = None if args.no_configs else reference_shapes(sync)
     tb, tb_sample = (None, None) if args.no_traceback else traceback(w, sync)
     pr, pr_keep = (None, None) if args.no_pairs else pairs(sync)
     serve = None if args.no_serving else serving(args, sync)
+    others = None if args.no_configs else other_configs(sync)
+    ref_shapes = None if args.no_configs else reference_shapes(sync)
 
     out = {
         "metric": METRIC,
