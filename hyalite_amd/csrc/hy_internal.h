@@ -437,8 +437,8 @@ struct Segment {
 // Every segment (device src -> pageable dst), after the work already queued on `stream`.
 int download_pageable(PinnedRing& r, int device, const Segment* segs, int nseg,
                       hipStream_t stream);
-// The segments back to pageable memory: through the ring when together they hold at least
-// kPinnedRingMin bytes (complete on return), else as plain copies queued on `stream`.
+// The segments back to pageable memory, complete on return: through the ring's worker threads
+// when together they hold at least kPinnedRingMin bytes, else through its first two slots.
 int download(PinnedRing& r, int device, const Segment* segs, int nseg, hipStream_t stream);
 
 // Process-wide lock held across a graph capture (the batched form's first submission) and around
